@@ -1,0 +1,193 @@
+/*
+ * pentrack_gpu.h -- C ABI of the MI355X tracking engine (libpentrack_gpu.so).
+ *
+ * The reference (PENTrack) has no plugin loader or FFI: its hot path is the C++
+ * call TTracker::IntegrateParticle(p, tmax, particleconf, mc, geom, field)
+ * (include/tracking.h:46-47, called from src/main.cpp:159,166), with the world
+ * model behind TGeometry (include/geometry.h:84-155) and TFieldManager
+ * (include/fields.h:19-62).  This header is the drop-in boundary for that path:
+ * plain pointers and sizes, no exceptions, status codes instead of throws.
+ * All pointers are host pointers unless the name says otherwise; the library
+ * owns every device allocation it makes.
+ */
+#ifndef PENTRACK_GPU_H_
+#define PENTRACK_GPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PT_OK 0
+#define PT_EINVAL (-1)   /* invalid argument / inconsistent input */
+#define PT_ENOMEM (-2)   /* host or device allocation failed */
+#define PT_EDEVICE (-3)  /* HIP runtime error or no usable GPU */
+
+/* ---- per-particle stop codes (include/globals.h:13-23) ----------------- */
+#define PT_ID_UNKNOWN 0
+#define PT_ID_NOT_FINISH (-1)
+#define PT_ID_HIT_BOUNDARIES (-2)
+#define PT_ID_ODEINT_ERROR (-3)
+#define PT_ID_DECAYED (-4)
+#define PT_ID_CGAL_ERROR (-6)
+#define PT_ID_GEOMETRY_ERROR (-7)
+#define PT_ID_ABSORBED_IN_MATERIAL 1
+#define PT_ID_ABSORBED_ON_SURFACE 2
+/* extensions: a reference runtime_error (src/tracking.cpp:38,286,333,350) becomes a
+   per-particle invariant stop; a configuration outside the implemented path
+   (micro-roughness model, BMT spin integration, more than PT_MAX_HITS
+   intersections of one segment) stops the particle as unsupported */
+#define PT_ID_INVARIANT (-8)
+#define PT_ID_UNSUPPORTED (-9)
+
+#define PT_STATE 9 /* x y z vx vy vz proper_time polarisation path_length */
+#define PT_SPIN 5
+#define PT_MAX_HITS 16
+#define PT_MAX_SOLIDS 8
+
+/* ---- geometry ---------------------------------------------------------- */
+typedef struct pt_geometry pt_geometry;
+
+/*
+ * Replaces TGeometry::TGeometry + TTriangleMesh::ReadFile + the CGAL AABB tree
+ * (src/geometry.cpp:93-137, src/trianglemesh.cpp:17-130).
+ * tri_vertices   [n_tri][3][3] float, as stored in the STL files, meshes concatenated
+ *                in solid load order; coordinates with |v| < 1e-8 are snapped to 0
+ *                (src/trianglemesh.cpp:42)
+ * tri_solid      [n_tri] index into the solid table
+ * solid_id       [n_solid] solid IDs (> 1), solid_mat [n_solid] material index
+ * ign_off/ign    ignore intervals of solid s: ign[2k], ign[2k+1] for k in [ign_off[s], ign_off[s+1])
+ * default_mat    material of the default solid (ID 1)
+ * mats           [n_mat][9] FermiReal FermiImag DiffProb SpinflipProb RMSRoughness CorrelLength
+ *                InternalBField ModifiedLambertProb LossPerBounce (include/geometry.h:20-31)
+ * leaf_size      BVH leaf size (triangles per leaf), 1..32
+ */
+int pt_geometry_create(int n_tri, const float *tri_vertices, const int32_t *tri_solid,
+                       int n_solid, const uint32_t *solid_id, const int32_t *solid_mat,
+                       const int32_t *ign_off, const double *ign, int default_mat,
+                       int n_mat, const double *mats, int leaf_size, pt_geometry **out);
+void pt_geometry_destroy(pt_geometry *g);
+/* number of BVH nodes and depth of the built tree (diagnostics) */
+int pt_geometry_info(const pt_geometry *g, int *n_nodes, int *depth);
+
+/* ---- fields ------------------------------------------------------------ */
+#define PT_FIELD_TRICUBIC 1  /* TabField3 (src/field_3d.cpp) */
+#define PT_FIELD_UNIFORM_E 2 /* TEDMStaticEField (src/edmfields.cpp:175-187) */
+
+typedef struct {
+    int type;
+    double bscale, escale;      /* constant TFieldScaler factors (src/field.cpp:7-45) */
+    double box[6];              /* xmax xmin ymax ymin zmax zmin; unbounded unless min<max (src/field.cpp:57-69) */
+    double bw;                  /* boundary width (src/field.cpp:71-125) */
+    int nx, ny, nz;             /* tricubic grid (sorted unique coordinates) */
+    const double *xg, *yg, *zg;
+    int has_b, has_v;           /* coefficient blocks per cell: Bx By Bz if has_b, then V if has_v */
+    const double *coef;         /* [(nx-1)(ny-1)(nz-1)][ncomp][64], cell (ix*(ny-1)+iy)*(nz-1)+iz */
+    double e[3];                /* uniform E */
+} pt_field_desc;
+
+typedef struct pt_fields pt_fields;
+
+/*
+ * Replaces TFieldManager::TFieldManager (src/fields.cpp:19-136): descriptors are
+ * kept in the given order (the reference's lexicographic FIELDS key order) and
+ * summed in that order.  with_fp32 also uploads an FP32 copy of every
+ * coefficient table for the mixed-precision tracker.
+ */
+int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32, pt_fields **out);
+void pt_fields_destroy(pt_fields *f);
+
+/*
+ * TabField3::PreInterpol (src/field_3d.cpp:367-414): spline derivatives along each
+ * axis (ALGLIB spline1dgriddiffcubic, parabolic end conditions) and the 64
+ * Lekien-Marsden coefficients of every cell (libtricubic tricubic_get_coeff).
+ * values   [ncomp][nx][ny][nz] samples on the grid
+ * coef     [(nx-1)(ny-1)(nz-1)][ncomp][64] output
+ */
+int pt_tricubic_precompute(int nx, int ny, int nz, const double *xg, const double *yg, const double *zg,
+                           int ncomp, const double *values, double *coef, int threads);
+
+/* ---- particles --------------------------------------------------------- */
+typedef struct {
+    int kind;          /* 0 neutron, 1 proton, 2 electron */
+    double q, m, mu, gamma; /* charge [C], mass [eV/c^2], magnetic moment [J/T], gyromagnetic ratio [1/(T s)] */
+    double tau;        /* particleconf "tau": mean lifetime (<=0: none) */
+    double tmax;       /* particleconf "tmax": proper-time limit when tau<=0 */
+    double lmax;       /* particleconf "lmax" */
+    int flipspin;      /* particleconf "flipspin" */
+    double Bmax;       /* particleconf "Bmax" */
+} pt_species;
+
+typedef struct {
+    double t, y[PT_STATE], spin[PT_SPIN];
+    int32_t stopID, Nhit, Nstep, Nspinflip;
+    int64_t n_acc, n_rej;        /* accepted / rejected dopri5 trial steps */
+    double Hmax, noflipprob, tau;
+    int32_t solidend;            /* ID of the current solid at the end */
+    int32_t n_hits;              /* hits logged (may exceed the log capacity) */
+} pt_particle_out;
+
+typedef struct {
+    double t;                    /* time of the bracketing point before the wall */
+    int32_t tri;                 /* triangle index (input order) whose normal was used */
+    int32_t leaving, entering;   /* solid IDs */
+    double normal[3], v1[3], v2[3], pol1, pol2;
+} pt_hit;
+
+#define PT_VARIANT_FP64 0        /* plain FP64, one thread per particle */
+#define PT_VARIANT_MIXED 1       /* FP32 field tables and interpolation, FP64 state and control */
+
+typedef struct {
+    int variant;
+    int block;                   /* threads per block (64..1024) */
+    int steps_per_launch;        /* accepted steps per particle per launch (0 = unlimited) */
+    int persistent;              /* 1: persistent kernel refilling slots from a device work queue */
+    int grid;                    /* persistent grid size in blocks (0 = automatic) */
+} pt_launch_cfg;
+
+typedef struct pt_batch pt_batch;
+
+/*
+ * Uploads n particles (the state TTracker::IntegrateParticle starts from) and
+ * allocates their resumable tracker state in device memory.
+ * species_idx [n], t0 [n], y0 [n][9], spin0 [n][5], pid [n] (random-stream ids).
+ * hit_cap: hits logged per particle (0 = none).
+ */
+int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_species, const pt_species *species,
+                    double tmax_global, int n, const int32_t *species_idx, const double *t0,
+                    const double *y0, const double *spin0, uint64_t seed, const uint64_t *pid,
+                    int hit_cap, pt_batch **out);
+void pt_batch_destroy(pt_batch *b);
+/* Advance every unfinished particle (stream-ordered, asynchronous; stream may be NULL). */
+int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg);
+/* Number of particles not yet stopped (synchronises the stream). */
+int pt_batch_active(pt_batch *b, void *stream, int *active);
+/* Copy results to the host (synchronises the stream).  hits may be NULL. */
+int pt_batch_download(pt_batch *b, void *stream, pt_particle_out *out, pt_hit *hits);
+/* Sum of accepted / rejected steps over the batch (synchronises the stream). */
+int pt_batch_step_counts(pt_batch *b, void *stream, int64_t *accepted, int64_t *rejected);
+
+/*
+ * One-shot equivalent of calling TTracker::IntegrateParticle for every particle:
+ * upload, run to completion, download.
+ */
+int pt_track_batch(void *stream, const pt_geometry *g, const pt_fields *f, int n_species,
+                   const pt_species *species, double tmax_global, int n, const int32_t *species_idx,
+                   const double *t0, const double *y0, const double *spin0, uint64_t seed,
+                   const uint64_t *pid, pt_particle_out *out, pt_hit *hits, int hit_cap,
+                   const pt_launch_cfg *cfg);
+
+/* Cancellation flag mirroring the reference's global `quit` (include/globals.h:33):
+   running kernels stop advancing particles at their next step boundary. */
+void pt_request_quit(int on);
+
+/* Last HIP error string of this thread (diagnostics). */
+const char *pt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PENTRACK_GPU_H_ */

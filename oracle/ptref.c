@@ -1,0 +1,1637 @@
+/*
+ * ptref.c -- CPU oracle: a plain-C restatement of PENTrack's per-particle
+ * tracking hot path, TTracker::IntegrateParticle and everything it calls.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, the smoke check and the
+ * CPU-baseline leg of bench.py load this library; the product never does.
+ *
+ * The reference cannot be built here (it needs Boost and CGAL's Boost
+ * dependencies), so this file restates its algorithm from the sources, one
+ * function per reference function, keeping the reference's control flow
+ * (recursion included) and its floating-point expression order:
+ *
+ *   tracker          src/tracking.cpp:17-173 (IntegrateParticle), :176-226 (CheckHit),
+ *                    :228-265 (iterate_collision), :268-278 (DoStep), :280-365 (DoHit),
+ *                    :367-370 (GetCurrentsolid), :373-483 (IntegrateSpin, non-BMT branches)
+ *   stepper          Boost.Odeint runge_kutta_dopri5 + controlled_runge_kutta +
+ *                    dense_output_runge_kutta, make_dense_output(1e-9, 1e-9)
+ *                    (include/particle.h:26-28; Boost is not vendored: semantics of
+ *                    Boost >= 1.60 restated, see dopri_* below)
+ *   particle         src/particle.cpp:96-135 (derivs, EquationOfMotion),
+ *                    :196-217 (DoStep, DoHit), :225-234 (DoPolarize),
+ *                    :237-269 (kinetic / potential energy)
+ *   neutron          src/neutron.cpp:25-35, 65-115, 151-282; include/neutron.h:53-68
+ *   proton/electron  src/proton.cpp:25-33, src/electron.cpp:25-51
+ *   fields           src/fields.cpp:139-182, src/field.cpp:7-169,
+ *                    src/field_3d.cpp:36-121 (Horner), :487-529 (Interpolate),
+ *                    src/edmfields.cpp:181-187 (uniform E)
+ *   geometry         src/geometry.cpp:139-158, include/geometry.h:44-50,104-106,
+ *                    src/trianglemesh.cpp:134-164, include/trianglemesh.h:57-78,181-199,
+ *                    CGAL Simple_cartesian<double> predicates and constructions
+ *   RotateVector     src/globals.cpp:40-64 (CGAL Plane_3 base1/projection, Aff_transformation_3)
+ *
+ * Two arithmetic modes, chosen at compile time:
+ *   PTREF_REFSEM=0 ("fp64")   every long double of the reference is a double; the
+ *                             transcendental functions come from pt_detmath.h.
+ *                             This is the arithmetic of the GPU kernel, so the two
+ *                             must agree bit for bit.
+ *   PTREF_REFSEM=1 ("refsem") the reference's long double constants and expressions
+ *                             and glibc libm / C99 complex, as the reference build on
+ *                             x86-64 computes them.
+ * Both must be compiled with -ffp-contract=off.
+ *
+ * Random numbers: one stream per particle (pt_detmath.h), drawn at the same
+ * points and through the same libstdc++ transforms as the reference draws from
+ * its global std::mt19937_64.
+ */
+#include <math.h>
+#include <complex.h>
+#include <string.h>
+#include <stdlib.h>
+#include <stdint.h>
+#include <pthread.h>
+
+#include "ptref.h"
+#include "../pentrack_amd/csrc/pt_detmath.h"
+
+#ifndef PTREF_REFSEM
+#define PTREF_REFSEM 0
+#endif
+
+#if PTREF_REFSEM
+typedef long double ld;
+#define LDC(x) x##L
+#define SQRT_LD(x) sqrtl(x)
+#define R_POW(a, b) pow(a, b)
+#define R_LOG(a) log(a)
+#define R_SIN(a) sin(a)
+#define R_COS(a) cos(a)
+#define R_ACOS(a) acos(a)
+#define SQ(a) pow(a, 2)
+#else
+typedef double ld;
+#define LDC(x) x
+#define SQRT_LD(x) sqrt(x)
+#define R_POW(a, b) ptm_pow(a, b)
+#define R_LOG(a) ptm_log(a)
+#define R_SIN(a) ptm_sin(a)
+#define R_COS(a) ptm_cos(a)
+#define R_ACOS(a) ptm_acos(a)
+#define SQ(a) ((a) * (a))
+#endif
+
+/* constants, src/globals.cpp:13-32 */
+#define PI LDC(3.1415926535897932384626)
+#define ELE_E LDC(1.602176487E-19)
+#define GRAV LDC(9.80665)
+#define C_0 LDC(299792458.)
+#define HBAR LDC(1.05457266e-34)
+#define MASS_N (LDC(1.674927211E-27) / ELE_E)
+#define MASS_P (LDC(1.672621637E-27) / ELE_E)
+#define MASS_E (LDC(9.10938215e-31) / ELE_E)
+#define MU_N LDC(-0.96623641e-26)
+#define GAMMA_N LDC(-1.83247185e8)
+
+/* include/particle.h:20, include/trianglemesh.h:39 */
+#define MAX_TRACK_DEVIATION 0.001
+#define REFLECT_TOLERANCE 1e-8
+#define DBL_EPS 2.220446049250313e-16
+
+/* stop IDs, include/globals.h:13-23 */
+enum {
+    ID_UNKNOWN = 0, ID_NOT_FINISH = -1, ID_HIT_BOUNDARIES = -2, ID_ODEINT_ERROR = -3,
+    ID_DECAYED = -4, ID_CGAL_ERROR = -6, ID_GEOMETRY_ERROR = -7,
+    ID_ABSORBED_IN_MATERIAL = 1, ID_ABSORBED_ON_SURFACE = 2,
+    /* build extensions: reference invariant violations (runtime_error) and
+       configurations the restatement does not cover */
+    ID_INVARIANT = -8, ID_UNSUPPORTED = -9
+};
+
+/* ------------------------------------------------------------------------ */
+/* context                                                                   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const ptref_geom *g;
+    int n_fields;
+    const ptref_field *fields;
+    double bbox[64][6];         /* per mesh: xmin ymin zmin xmax ymax zmax (CGAL tree bbox) */
+    int n_mesh_bbox;
+    double (*normals)[3];       /* per triangle */
+    /* bounding-volume hierarchy used only to skip triangles whose box cannot
+       touch the query; results equal the brute-force loop over all triangles */
+    int n_node;
+    struct onode { double lo[3], hi[3]; int left, right, first, count; } *nodes;
+    int *perm;
+    double (*tbox)[6];
+} ctx_t;
+
+typedef struct {
+    ld q, m, mu, gamma;
+    int kind;
+} species_t;
+
+typedef struct { int sidx; int ignored; } solid_ref;   /* sidx -1 = default solid */
+
+#define MAX_SOLIDS_STACK 64
+typedef struct { int n; solid_ref s[MAX_SOLIDS_STACK]; } solid_stack;
+
+typedef struct {
+    double s;
+    double normal[3];
+    unsigned ID;
+    double distnormal;
+    int tri;
+    int ignored;
+} coll_t;
+
+#define MAX_COLLS 4096
+typedef struct { int n; coll_t c[MAX_COLLS]; } coll_list;
+
+/* dense-output dopri5 stepper state */
+typedef struct {
+    double xc[9], dxc[9];        /* current state and derivative */
+    double xo[9], dxo[9];        /* previous (old) state and derivative */
+    double k3[9], k4[9], k5[9], k6[9];
+    double t, t_old, dt;
+    int deriv_init;
+} dopri_t;
+
+typedef struct {
+    const ctx_t *c;
+    species_t sp;
+    const ptref_species *cfg;
+    uint64_t key, ctr;
+    solid_stack cur;             /* TTracker::currentsolids */
+    int stopID;
+    int Nhit, Nstep, Nspinflip;
+    long long n_acc, n_rej, n_rhs, n_field, n_tri, n_bisect, n_pieces;
+    double Hmax;
+    ld noflipprob;
+    ptref_hit *hits; int hit_cap; int n_hits;
+} track_t;
+
+static double rnd_uniform(track_t *p, double a, double b) {
+    uint64_t r = ptm_draw_u64(p->key, &p->ctr);
+    return ptm_canonical(r) * (b - a) + a;
+}
+static double rnd_exponential(track_t *p, double lambda) {
+    uint64_t r = ptm_draw_u64(p->key, &p->ctr);
+    return -R_LOG(1.0 - ptm_canonical(r)) / lambda;
+}
+/* sincos_distribution(0, pi/2), include/mc.h:162-175 */
+static double rnd_sincos(track_t *p, double amin, double amax) {
+    uint64_t r = ptm_draw_u64(p->key, &p->ctr);
+    double x = ptm_canonical(r);
+    double cmax = R_COS(amax), cmin = R_COS(amin);
+    return R_ACOS(sqrt(x * (cmax * cmax - cmin * cmin) + cmin * cmin));
+}
+
+/* ------------------------------------------------------------------------ */
+/* CGAL Simple_cartesian<double> predicates and constructions                */
+/* ------------------------------------------------------------------------ */
+
+/* orientationC3 + determinant (3x3 minors), CGAL predicates/kernel_ftC3.h, determinant.h */
+static int orient3(const double *p, const double *q, const double *r, const double *s) {
+    double a00 = q[0] - p[0], a01 = r[0] - p[0], a02 = s[0] - p[0];
+    double a10 = q[1] - p[1], a11 = r[1] - p[1], a12 = s[1] - p[1];
+    double a20 = q[2] - p[2], a21 = r[2] - p[2], a22 = s[2] - p[2];
+    double m01 = a00 * a11 - a10 * a01;
+    double m02 = a00 * a21 - a20 * a01;
+    double m12 = a10 * a21 - a20 * a11;
+    double det = m01 * a22 - m02 * a12 + m12 * a02;
+    return (det > 0) - (det < 0);
+}
+
+/* orientationC2 and coplanar_orientationC3 (3 points), predicates/kernel_ftC2.h, kernel_ftC3.h:204-218 */
+static int orient2(double px, double py, double qx, double qy, double rx, double ry) {
+    double d = (qx - px) * (ry - py) - (rx - px) * (qy - py);
+    return (d > 0) - (d < 0);
+}
+static int cop_orient(const double *p, const double *q, const double *r) {
+    int o = orient2(p[0], p[1], q[0], q[1], r[0], r[1]);
+    if (o != 0) return o;
+    o = orient2(p[1], p[2], q[1], q[2], r[1], r[2]);
+    if (o != 0) return o;
+    return orient2(p[0], p[2], q[0], q[2], r[0], r[2]);
+}
+/* do_intersect_coplanar(triangle, segment), Triangle_3_Segment_3_do_intersect.h:77-200 */
+static int coplanar_seg_tri(const double *A, const double *B, const double *C, const double *p, const double *q) {
+    const double *a = A, *b = B, *c = C;
+    if (cop_orient(A, B, C) <= 0) { b = C; c = B; }
+    int pqa = cop_orient(p, q, a), pqb = cop_orient(p, q, b), pqc = cop_orient(p, q, c);
+    if (pqa > 0) {
+        if (pqb > 0) {
+            if (pqc > 0) return 0;
+            return cop_orient(b, c, q) >= 0 && cop_orient(c, a, p) >= 0;
+        }
+        if (pqc > 0) return cop_orient(a, b, q) >= 0 && cop_orient(b, c, p) >= 0;
+        return cop_orient(a, b, q) >= 0 && cop_orient(c, a, p) >= 0;
+    } else if (pqa < 0) {
+        if (pqb > 0) {
+            if (pqc > 0) return cop_orient(a, b, p) >= 0 && cop_orient(c, a, q) >= 0;
+            return cop_orient(a, b, p) >= 0 && cop_orient(b, c, q) >= 0;
+        }
+        if (pqb < 0) {
+            if (pqc < 0) return 0;
+            return cop_orient(b, c, p) >= 0 && cop_orient(c, a, q) >= 0;
+        }
+        if (pqc < 0) return cop_orient(a, b, p) >= 0 && cop_orient(b, c, q) >= 0;
+        return cop_orient(a, b, p) >= 0 && cop_orient(c, a, q) >= 0;
+    } else {
+        if (pqb > 0) {
+            if (pqc > 0) return cop_orient(a, b, p) >= 0 && cop_orient(c, a, q) >= 0;
+            return cop_orient(a, b, p) >= 0 && cop_orient(b, c, q) >= 0;
+        }
+        if (pqb < 0) {
+            if (pqc < 0) return cop_orient(a, b, q) >= 0 && cop_orient(c, a, p) >= 0;
+            return cop_orient(a, b, q) >= 0 && cop_orient(b, c, p) >= 0;
+        }
+        if (pqc > 0) return cop_orient(b, c, p) >= 0 && cop_orient(c, a, q) >= 0;
+        return cop_orient(b, c, q) >= 0 && cop_orient(c, a, p) >= 0;
+    }
+}
+
+/* segment (p,q) vs triangle (a,b,c): returns 1 and the point, 0 no hit, -1 for a
+   segment lying in the triangle's plane and touching it (the reference's
+   intersection is then a segment, on which it throws; src/trianglemesh.cpp:146-147)
+   (CGAL Intersections_3/internal/Triangle_3_Segment_3_intersection.h:408-480) */
+static int seg_tri(const double *a, const double *b, const double *c,
+                   const double *p, const double *q, double pt[3]) {
+    int abcp = orient3(a, b, c, p);
+    int abcq = orient3(a, b, c, q);
+    const double *u = 0, *w = 0;
+    int construct = 0;
+    if (abcp > 0) {
+        if (abcq > 0) return 0;
+        u = p; w = q;
+        construct = abcq < 0;
+        if (abcq == 0) { if (orient3(p, q, a, b) <= 0 && orient3(p, q, b, c) <= 0 && orient3(p, q, c, a) <= 0) { memcpy(pt, q, 24); return 1; } return 0; }
+    } else if (abcp < 0) {
+        if (abcq < 0) return 0;
+        u = q; w = p;
+        construct = abcq > 0;
+        if (abcq == 0) { if (orient3(q, p, a, b) <= 0 && orient3(q, p, b, c) <= 0 && orient3(q, p, c, a) <= 0) { memcpy(pt, q, 24); return 1; } return 0; }
+    } else {
+        if (abcq > 0) { if (orient3(q, p, a, b) <= 0 && orient3(q, p, b, c) <= 0 && orient3(q, p, c, a) <= 0) { memcpy(pt, p, 24); return 1; } return 0; }
+        if (abcq < 0) { if (orient3(p, q, a, b) <= 0 && orient3(p, q, b, c) <= 0 && orient3(p, q, c, a) <= 0) { memcpy(pt, p, 24); return 1; } return 0; }
+        return coplanar_seg_tri(a, b, c, p, q) ? -1 : 0;
+    }
+    (void)construct;
+    if (orient3(u, w, a, b) > 0 || orient3(u, w, b, c) > 0 || orient3(u, w, c, a) > 0) return 0;
+    /* supporting plane (plane_from_pointsC3, constructions/kernel_ftC3.h:232-248) */
+    double rpx = a[0] - c[0], rpy = a[1] - c[1], rpz = a[2] - c[2];
+    double rqx = b[0] - c[0], rqy = b[1] - c[1], rqz = b[2] - c[2];
+    double pa = rpy * rqz - rqy * rpz;
+    double pb = rpz * rqx - rqz * rpx;
+    double pc = rpx * rqy - rqx * rpy;
+    double pd = -pa * c[0] - pb * c[1] - pc * c[2];
+    /* line p + (q-p), intersection_3_1_impl.h:93-121 */
+    double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    double num = pa * p[0] + pb * p[1] + pc * p[2] + pd;
+    double den = pa * dx + pb * dy + pc * dz;
+    if (den == 0) return 0;
+    double hx = den * p[0] - num * dx, hy = den * p[1] - num * dy, hz = den * p[2] - num * dz;
+    if (den != 1.0) { pt[0] = hx / den; pt[1] = hy / den; pt[2] = hz / den; }
+    else { pt[0] = hx; pt[1] = hy; pt[2] = hz; }
+    return 1;
+}
+
+/* +z ray from p vs triangle (Triangle_3_Ray_3_do_intersect.h:91-226); coplanar rays count as no hit */
+static int ray_tri(const double *a, const double *b, const double *c, const double *p) {
+    double q[3] = {p[0], p[1], p[2] + 1.0};
+    double ad[3] = {a[0], a[1], a[2] + 1.0};
+    int dir = orient3(a, b, c, ad);
+    if (dir == 0) return 0;
+    int abcp = orient3(a, b, c, p);
+    if (abcp > 0) {
+        if (dir > 0) return 0;
+        return !(orient3(p, q, a, b) > 0 || orient3(p, q, b, c) > 0 || orient3(p, q, c, a) > 0);
+    } else if (abcp < 0) {
+        if (dir < 0) return 0;
+        return !(orient3(q, p, a, b) > 0 || orient3(q, p, b, c) > 0 || orient3(q, p, c, a) > 0);
+    } else {
+        if (dir > 0) return orient3(q, p, a, b) <= 0 && orient3(q, p, b, c) <= 0 && orient3(q, p, c, a) <= 0;
+        return orient3(p, q, a, b) <= 0 && orient3(p, q, b, c) <= 0 && orient3(p, q, c, a) <= 0;
+    }
+}
+
+/* do_intersect(Segment_3, Bbox_3) with Simple_cartesian<double>
+   (Intersections_3/internal/Bbox_3_Segment_3_do_intersect.h:141-380, bounded both ends, no filter) */
+static int seg_bbox(const double *P, const double *Q, const double *bb) {
+    double px = P[0], py = P[1], pz = P[2], qx = Q[0], qy = Q[1], qz = Q[2];
+    double bxmin = bb[0], bymin = bb[1], bzmin = bb[2], bxmax = bb[3], bymax = bb[4], bzmax = bb[5];
+    if ((px >= bxmin && px <= bxmax && py >= bymin && py <= bymax && pz >= bzmin && pz <= bzmax) ||
+        (qx >= bxmin && qx <= bxmax && qy >= bymin && qy <= bymax && qz >= bzmin && qz <= bzmax))
+        return 1;
+    double dmin, tmin, tmax, dmax;
+    if (qx >= px) {
+        if (px > bxmax) return 0;
+        if (qx < bxmin) return 0;
+        if (bxmax > qx) { tmax = 1; dmax = 1; } else { tmax = bxmax - px; dmax = qx - px; }
+        tmin = bxmin - px; dmin = qx - px;
+    } else {
+        if (qx > bxmax) return 0;
+        if (px < bxmin) return 0;
+        if (bxmin < qx) { tmax = 1; dmax = 1; } else { tmax = px - bxmin; dmax = px - qx; }
+        tmin = px - bxmax; dmin = px - qx;
+    }
+    if (tmin < 0) tmin = 0; /* (max)(0, tmin) */
+    double dymin, tymin, tymax, dymax;
+    if (qy >= py) {
+        if (py > bymax) return 0;
+        if (qy < bymin) return 0;
+        if (bymax > qy) { tymax = 1; dymax = 1; } else { tymax = bymax - py; dymax = qy - py; }
+        tymin = bymin - py; dymin = qy - py;
+    } else {
+        if (qy > bymax) return 0;
+        if (py < bymin) return 0;
+        if (bymin < qy) { tymax = 1; dymax = 1; } else { tymax = py - bymin; dymax = py - qy; }
+        tymin = py - bymax; dymin = py - qy;
+    }
+    if (tymin < 0) tymin = 0;
+    if (py != qy && px != qx) {
+        if (dymax * tmin > dmin * tymax) return 0;
+        if (dmax * tymin > dymin * tmax) return 0;
+    }
+    if (px == qx || (py != qy && dmin * tymin > dymin * tmin)) { tmin = tymin; dmin = dymin; }
+    if (px == qx || (py != qy && dymax * tmax > dmax * tymax)) { tmax = tymax; dmax = dymax; }
+    double dzmin, tzmin, tzmax, dzmax;
+    if (qz >= pz) {
+        if (pz > bzmax) return 0;
+        if (qz < bzmin) return 0;
+        if (bzmax > qz) { tzmax = 1; dzmax = 1; } else { tzmax = bzmax - pz; dzmax = qz - pz; }
+        tzmin = bzmin - pz; dzmin = qz - pz;
+    } else {
+        if (qz > bzmax) return 0;
+        if (pz < bzmin) return 0;
+        if (bzmin < qz) { tzmax = 1; dzmax = 1; } else { tzmax = pz - bzmin; dzmax = pz - qz; }
+        tzmin = pz - bzmax; dzmin = pz - qz;
+    }
+    if (tzmin < 0) tzmin = 0;
+    if ((px != qx || py != qy) && pz != qz) {
+        if (dzmax * tmin > dmin * tzmax) return 0;
+        if (dmax * tzmin > dzmin * tmax) return 0;
+    }
+    return 1;
+}
+
+/* RotateVector, src/globals.cpp:40-64 */
+static void rotate_vector(double v[3], const double n[3], const double *x) {
+    double xv[3], zv[3] = {n[0], n[1], n[2]};
+    double pa = zv[0], pb = zv[1], pc = zv[2];
+    double pd = -zv[0] * 0.0 - zv[1] * 0.0 - zv[2] * 0.0;
+    int use_base = (x == 0);
+    if (!use_base) {
+        double num = pa * x[0] + pb * x[1] + pc * x[2] + pd;
+        double den = pa * pa + pb * pb + pc * pc;
+        double lambda = num / den;
+        xv[0] = x[0] - lambda * pa;
+        xv[1] = x[1] - lambda * pb;
+        xv[2] = x[2] - lambda * pc;
+        if (xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2] < 1e-30) use_base = 1;
+    }
+    if (use_base) { /* Plane_3::base1, Cartesian/function_objects.h:1746-1776 */
+        if (pa == 0) { xv[0] = 1; xv[1] = 0; xv[2] = 0; }
+        else if (pb == 0) { xv[0] = 0; xv[1] = 1; xv[2] = 0; }
+        else if (pc == 0) { xv[0] = 0; xv[1] = 0; xv[2] = 1; }
+        else {
+            double a = fabs(pa), b = fabs(pb), c = fabs(pc);
+            if (a <= b && a <= c) { xv[0] = 0; xv[1] = -pc; xv[2] = pb; }
+            else if (b <= a && b <= c) { xv[0] = -pc; xv[1] = 0; xv[2] = pa; }
+            else { xv[0] = -pb; xv[1] = pa; xv[2] = 0; }
+        }
+    }
+    double lx = sqrt(xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2]);
+    xv[0] = xv[0] / lx; xv[1] = xv[1] / lx; xv[2] = xv[2] / lx;
+    double lz = sqrt(zv[0] * zv[0] + zv[1] * zv[1] + zv[2] * zv[2]);
+    zv[0] = zv[0] / lz; zv[1] = zv[1] / lz; zv[2] = zv[2] / lz;
+    double yv[3] = {zv[1] * xv[2] - zv[2] * xv[1], zv[2] * xv[0] - zv[0] * xv[2], zv[0] * xv[1] - zv[1] * xv[0]};
+    double v0 = v[0], v1 = v[1], v2 = v[2];
+    v[0] = xv[0] * v0 + yv[0] * v1 + zv[0] * v2;
+    v[1] = xv[1] * v0 + yv[1] * v1 + zv[1] * v2;
+    v[2] = xv[2] * v0 + yv[2] * v1 + zv[2] * v2;
+}
+
+/* ------------------------------------------------------------------------ */
+/* geometry                                                                  */
+/* ------------------------------------------------------------------------ */
+
+static unsigned solid_ID(const ctx_t *c, int sidx) { return sidx < 0 ? 1u : c->g->solid_id[sidx]; }
+static const double *solid_mat(const ctx_t *c, int sidx) {
+    int m = sidx < 0 ? c->g->default_mat : c->g->solid_mat[sidx];
+    return c->g->mats + 9 * m;
+}
+/* solid::is_ignored, include/geometry.h:44-50 */
+static int solid_ignored(const ctx_t *c, int sidx, double t) {
+    if (sidx < 0) return 0;
+    for (int k = c->g->ign_off[sidx]; k < c->g->ign_off[sidx + 1]; ++k)
+        if (t >= c->g->ign[2 * k] && t < c->g->ign[2 * k + 1]) return 1;
+    return 0;
+}
+
+/* TGeometry::CheckSegment -> TTriangleMesh::InBoundingBox */
+static int check_segment(const ctx_t *c, const double *p1, const double *p2) {
+    for (int m = 0; m < c->n_mesh_bbox; ++m)
+        if (seg_bbox(p1, p2, c->bbox[m])) return 1;
+    return 0;
+}
+
+/* TTriangleMesh::Collision + TGeometry::GetCollisions: all hits of segment p1->p2,
+   sorted by s ascending then solid ID descending (stable for full ties).
+   Returns -1 on a coplanar (non-point) intersection, else the number of hits. */
+static int coll_before(const coll_t *a, const coll_t *b) {
+    /* order of the reference's sort: s ascending, then solid ID descending; full ties keep
+       brute-force (mesh, file) order, i.e. triangle index ascending */
+    if (a->s != b->s) return a->s < b->s;
+    if (a->ID != b->ID) return a->ID > b->ID;
+    return a->tri < b->tri;
+}
+
+static int get_collisions(track_t *tp, double x1, const double *p1, double x2, const double *p2, coll_list *out) {
+    const ctx_t *c = tp->c;
+    const ptref_geom *g = c->g;
+    out->n = 0;
+    double sv[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+    double sl = sv[0] * sv[0] + sv[1] * sv[1] + sv[2] * sv[2];
+    double qlo[3], qhi[3];
+    for (int a = 0; a < 3; ++a) {
+        double lo = p1[a] < p2[a] ? p1[a] : p2[a], hi = p1[a] < p2[a] ? p2[a] : p1[a];
+        double pad = 1e-9 + 1e-9 * (fabs(lo) + fabs(hi));
+        qlo[a] = lo - pad; qhi[a] = hi + pad;
+    }
+    int stack[128], sp = 0;
+    if (c->n_node > 0) stack[sp++] = 0;
+    while (sp > 0) {
+        const struct onode *nd = &c->nodes[stack[--sp]];
+        if (nd->lo[0] > qhi[0] || nd->hi[0] < qlo[0] || nd->lo[1] > qhi[1] || nd->hi[1] < qlo[1] ||
+            nd->lo[2] > qhi[2] || nd->hi[2] < qlo[2]) continue;
+        if (nd->count == 0) { stack[sp++] = nd->left; stack[sp++] = nd->right; continue; }
+        for (int k = nd->first; k < nd->first + nd->count; ++k) {
+            int i = c->perm[k];
+            const double *T = g->tri + 9 * i;
+            double pt[3];
+            tp->n_tri++;
+            int r = seg_tri(T, T + 3, T + 6, p1, p2, pt);
+            if (r == 0) continue;
+            if (r < 0) return -1;
+            if (out->n >= MAX_COLLS) return -1;
+            coll_t *cl = &out->c[out->n++];
+            const double *n = c->normals[i];
+            int m = g->tri_solid[i];
+            double d[3] = {pt[0] - p1[0], pt[1] - p1[1], pt[2] - p1[2]};
+            cl->s = (d[0] * sv[0] + d[1] * sv[1] + d[2] * sv[2]) / sl;
+            cl->normal[0] = n[0]; cl->normal[1] = n[1]; cl->normal[2] = n[2];
+            cl->ID = g->solid_id[m];
+            cl->distnormal = sv[0] * n[0] + sv[1] * n[1] + sv[2] * n[2];
+            cl->tri = i;
+            cl->ignored = solid_ignored(c, m, x1 + (x2 - x1) * cl->s);
+        }
+    }
+    for (int i = 1; i < out->n; ++i) {
+        coll_t key = out->c[i];
+        int j = i - 1;
+        while (j >= 0 && coll_before(&key, &out->c[j])) { out->c[j + 1] = out->c[j]; --j; }
+        out->c[j + 1] = key;
+    }
+    return out->n;
+}
+
+static int sidx_of_id(const ctx_t *c, unsigned ID) {
+    if (ID == 1) return -1;
+    for (int m = 0; m < c->g->n_solid; ++m) if (c->g->solid_id[m] == ID) return m;
+    return -2;
+}
+
+/* TGeometry::GetSolids: default solid + every mesh with odd +z ray crossing count */
+static void get_solids(const ctx_t *c, double t, const double *p, solid_stack *st) {
+    const ptref_geom *g = c->g;
+    st->n = 1; st->s[0].sidx = -1; st->s[0].ignored = 0;
+    for (int m = 0; m < g->n_solid; ++m) {
+        int cnt = 0;
+        for (int i = 0; i < g->n_tri; ++i) {
+            if (g->tri_solid[i] != m) continue;
+            const double *b = c->tbox[i];
+            if (p[0] < b[0] || p[0] > b[3] || p[1] < b[1] || p[1] > b[4] || p[2] > b[5]) continue;
+            const double *T = g->tri + 9 * i;
+            cnt += ray_tri(T, T + 3, T + 6, p);
+        }
+        if (cnt % 2 != 0 && st->n < MAX_SOLIDS_STACK) {
+            st->s[st->n].sidx = m;
+            st->s[st->n].ignored = solid_ignored(c, m, t);
+            st->n++;
+        }
+    }
+}
+
+/* TTracker::GetCurrentsolid: std::max_element with the reference's comparator */
+static int current_solid(const ctx_t *c, const solid_stack *st) {
+    int best = 0;
+    for (int i = 1; i < st->n; ++i) {
+        const solid_ref *a = &st->s[best], *b = &st->s[i];
+        if (a->ignored || (!b->ignored && solid_ID(c, a->sidx) < solid_ID(c, b->sidx))) best = i;
+    }
+    return st->s[best].sidx;
+}
+
+/* ------------------------------------------------------------------------ */
+/* fields                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* tricubic_eval_fast, src/field_3d.cpp:36-64 and :82-121 */
+static double tric_eval(const double *a, double x, double y, double z) {
+    double rx = 0.;
+    for (int i = 3; i >= 0; --i) {
+        double ry = 0.;
+        for (int j = 3; j >= 0; --j) {
+            double rz = 0.;
+            for (int k = 3; k >= 0; --k) {
+                rz += a[i + 4 * j + 16 * k];
+                if (k > 0) rz *= z;
+            }
+            ry += rz;
+            if (j > 0) ry *= y;
+        }
+        rx += ry;
+        if (i > 0) rx *= x;
+    }
+    return rx;
+}
+static const double FACT[4][4] = {{1.0, 0.0, 0.0, 0.0}, {1.0, 1.0, 0.0, 0.0}, {1.0, 2.0, 2.0, 0.0}, {1.0, 3.0, 6.0, 6.0}};
+static double tric_eval_d(const double *a, double x, double y, double z, int dx, int dy, int dz) {
+    double rx = 0.;
+    for (int i = 3; i >= dx; --i) {
+        double ry = 0.;
+        for (int j = 3; j >= dy; --j) {
+            double rz = 0.;
+            for (int k = 3; k >= dz; --k) {
+                rz += FACT[k][dz] * a[i + 4 * j + 16 * k];
+                if (k > dz) rz *= z;
+            }
+            ry += FACT[j][dy] * rz;
+            if (j > dy) ry *= y;
+        }
+        rx += FACT[i][dx] * ry;
+        if (i > dx) rx *= x;
+    }
+    return rx;
+}
+
+/* TabField3::Interpolate (src/field_3d.cpp:487-513) for every coefficient block of
+   the cell; returns 0 (fields untouched) outside the grid */
+static int tric_locate(const ptref_field *f, double x, double y, double z, long *cell, double r[3], double dist[3]) {
+    double p[3] = {x, y, z};
+    const double *g[3] = {f->xg, f->yg, f->zg};
+    int n[3] = {f->nx, f->ny, f->nz};
+    long idx[3];
+    for (int i = 0; i < 3; ++i) {
+        /* std::upper_bound: first grid value > p */
+        int lo = 0, len = n[i];
+        while (len > 0) {
+            int half = len / 2;
+            if (!(p[i] < g[i][lo + half])) { lo = lo + half + 1; len = len - half - 1; }
+            else len = half;
+        }
+        if (lo == n[i] || lo == 0) return 0;
+        idx[i] = lo - 1;
+        dist[i] = g[i][lo] - g[i][lo - 1];
+        r[i] = (p[i] - g[i][lo - 1]) / dist[i];
+    }
+    *cell = (idx[0] * (f->ny - 1) + idx[1]) * (f->nz - 1) + idx[2];
+    return 1;
+}
+
+static int field_ncomp(const ptref_field *f) { return (f->has_b ? 3 : 0) + (f->has_v ? 1 : 0); }
+
+static int in_bounds(const ptref_field *f, double x, double y, double z) {
+    const double *b = f->box; /* xmax xmin ymax ymin zmax zmin */
+    int has = b[1] < b[0] && b[3] < b[2] && b[5] < b[4];
+    if (!has) return 1;
+    return x >= b[1] && x < b[0] && y >= b[3] && y < b[2] && z >= b[5] && z < b[4];
+}
+static double smth(double x) { return ((6 * x - 15) * x + 10) * x * x * x; }
+static double smthd(double x) { return ((30 * x - 60) * x + 30) * x * x; }
+
+/* TFieldBoundaryBox::scaleScalarFieldAtBounds, src/field.cpp:71-112 */
+static void scale_at_bounds(const ptref_field *f, double x, double y, double z, double *F, double *dF) {
+    const double *b = f->box;
+    double xmax = b[0], xmin = b[1], ymax = b[2], ymin = b[3], zmax = b[4], zmin = b[5], bw = f->bw;
+    int has = xmin < xmax && ymin < ymax && zmin < zmax;
+    if (!has || (*F == 0 && dF == 0) || (*F == 0 && dF[0] == 0 && dF[1] == 0 && dF[2] == 0)) return;
+    if (!in_bounds(f, x, y, z)) {
+        *F = 0.;
+        if (dF) dF[0] = dF[1] = dF[2] = 0.;
+    } else if (bw > 0) {
+        double Fscale = 1.;
+        double dFadd[3] = {0., 0., 0.};
+        double lo[3] = {(x - xmin) / bw, (y - ymin) / bw, (z - zmin) / bw};
+        double hi[3] = {(xmax - x) / bw, (ymax - y) / bw, (zmax - z) / bw};
+        for (int i = 0; i < 3; ++i) {
+            if (0 <= lo[i] && lo[i] <= 1) {
+                Fscale *= smth(lo[i]);
+                if (dF) dFadd[i] += *F * smthd(lo[i]) / bw / smth(lo[i]);
+            } else if (0 <= hi[i] && hi[i] <= 1) {
+                Fscale *= smth(hi[i]);
+                if (dF) dFadd[i] += -*F * smthd(hi[i]) / bw / smth(hi[i]);
+            }
+        }
+        if (Fscale != 1.) {
+            *F *= Fscale;
+            if (dF) for (int i = 0; i < 3; i++) dF[i] = dF[i] * Fscale + dFadd[i] * Fscale;
+        }
+    }
+}
+
+/* TFieldContainer::BField (src/field.cpp:139-153) */
+static void container_B(track_t *tp, const ptref_field *f, double x, double y, double z, double t, double B[3], double dB[3][3]) {
+    (void)t;
+    if (!in_bounds(f, x, y, z) || f->bscale == 0.) {
+        for (int i = 0; i < 3; ++i) { B[i] = 0.; if (dB) for (int j = 0; j < 3; ++j) dB[i][j] = 0.; }
+        return;
+    }
+    if (f->type == PTREF_FIELD_TRICUBIC && f->has_b) {
+        long cell; double r[3], dist[3];
+        if (tric_locate(f, x, y, z, &cell, r, dist)) {
+            int nc = field_ncomp(f);
+            tp->n_field++;
+            for (int i = 0; i < 3; ++i) {
+                const double *a = f->coef + ((size_t)cell * nc + i) * 64;
+                B[i] = tric_eval(a, r[0], r[1], r[2]);
+                if (dB) {
+                    dB[i][0] = tric_eval_d(a, r[0], r[1], r[2], 1, 0, 0) / dist[0];
+                    dB[i][1] = tric_eval_d(a, r[0], r[1], r[2], 0, 1, 0) / dist[1];
+                    dB[i][2] = tric_eval_d(a, r[0], r[1], r[2], 0, 0, 1) / dist[2];
+                }
+            }
+        }
+    }
+    /* TFieldScaler::scaleVectorField */
+    for (int i = 0; i < 3; ++i) {
+        B[i] *= f->bscale;
+        if (dB) { dB[i][0] *= f->bscale; dB[i][1] *= f->bscale; dB[i][2] *= f->bscale; }
+    }
+    for (int i = 0; i < 3; ++i) scale_at_bounds(f, x, y, z, &B[i], dB ? dB[i] : 0);
+}
+
+/* TFieldContainer::EField (src/field.cpp:155-169) */
+static void container_E(track_t *tp, const ptref_field *f, double x, double y, double z, double t, double *V, double E[3]) {
+    (void)t;
+    if (!in_bounds(f, x, y, z) || f->escale == 0.) {
+        *V = 0.; E[0] = E[1] = E[2] = 0.;
+        return;
+    }
+    if (f->type == PTREF_FIELD_UNIFORM_E) { /* TEDMStaticEField::EField, src/edmfields.cpp:181-187 */
+        E[0] = f->e[0]; E[1] = f->e[1]; E[2] = f->e[2];
+        *V = E[2] * z;
+    } else if (f->type == PTREF_FIELD_TRICUBIC && f->has_v) {
+        long cell; double r[3], dist[3];
+        double dV[3] = {0., 0., 0.};
+        if (tric_locate(f, x, y, z, &cell, r, dist)) {
+            int nc = field_ncomp(f);
+            const double *a = f->coef + ((size_t)cell * nc + (f->has_b ? 3 : 0)) * 64;
+            tp->n_field++;
+            *V = tric_eval(a, r[0], r[1], r[2]);
+            dV[0] = tric_eval_d(a, r[0], r[1], r[2], 1, 0, 0) / dist[0];
+            dV[1] = tric_eval_d(a, r[0], r[1], r[2], 0, 1, 0) / dist[1];
+            dV[2] = tric_eval_d(a, r[0], r[1], r[2], 0, 0, 1) / dist[2];
+        }
+        for (int i = 0; i < 3; i++) E[i] = -dV[i];
+    }
+    /* TFieldScaler::scaleScalarField */
+    *V *= f->escale;
+    E[0] *= f->escale; E[1] *= f->escale; E[2] *= f->escale;
+    scale_at_bounds(f, x, y, z, V, E);
+}
+
+/* TFieldManager::BField / EField, src/fields.cpp:139-182 */
+static void field_B(track_t *tp, double x, double y, double z, double t, double B[3], double dB[3][3]) {
+    const ctx_t *c = tp->c;
+    for (int i = 0; i < 3; i++) { B[i] = 0; if (dB) for (int j = 0; j < 3; j++) dB[i][j] = 0; }
+    for (int k = 0; k < c->n_fields; ++k) {
+        double Bt[3] = {0, 0, 0};
+        double dBt[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        container_B(tp, &c->fields[k], x, y, z, t, Bt, dB ? dBt : 0);
+        for (int i = 0; i < 3; i++) {
+            B[i] += Bt[i];
+            if (dB) for (int j = 0; j < 3; j++) dB[i][j] += dBt[i][j];
+        }
+    }
+}
+static void field_E(track_t *tp, double x, double y, double z, double t, double *V, double E[3]) {
+    const ctx_t *c = tp->c;
+    *V = 0; E[0] = E[1] = E[2] = 0;
+    for (int k = 0; k < c->n_fields; ++k) {
+        double Vt = 0, Et[3] = {0, 0, 0};
+        container_E(tp, &c->fields[k], x, y, z, t, &Vt, Et);
+        *V += Vt;
+        for (int i = 0; i < 3; i++) E[i] += Et[i];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* particle                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* TParticle::EquationOfMotion, src/particle.cpp:105-135 */
+static void eom(const species_t *sp, const double *y, const double B[3], const double dB[3][3], const double E[3], double *dydx) {
+    dydx[0] = y[3]; dydx[1] = y[4]; dydx[2] = y[5];
+    double F[3] = {0, 0, 0};
+    F[2] += -GRAV * sp->m * ELE_E;
+    if (sp->q != 0) {
+        F[0] += sp->q * (E[0] + y[4] * B[2] - y[5] * B[1]);
+        F[1] += sp->q * (E[1] + y[5] * B[0] - y[3] * B[2]);
+        F[2] += sp->q * (E[2] + y[3] * B[1] - y[4] * B[0]);
+    }
+    if (sp->mu != 0 && y[7] != 0 && (B[0] != 0 || B[1] != 0 || B[2] != 0)) {
+        double Babs = sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+        double dBdxi[3] = {(B[0] * dB[0][0] + B[1] * dB[1][0] + B[2] * dB[2][0]) / Babs,
+                           (B[0] * dB[0][1] + B[1] * dB[1][1] + B[2] * dB[2][1]) / Babs,
+                           (B[0] * dB[0][2] + B[1] * dB[1][2] + B[2] * dB[2][2]) / Babs};
+        F[0] += y[7] * sp->mu * dBdxi[0];
+        F[1] += y[7] * sp->mu * dBdxi[1];
+        F[2] += y[7] * sp->mu * dBdxi[2];
+    }
+    double v2 = y[3] * y[3] + y[4] * y[4] + y[5] * y[5];
+    double inversegamma = SQRT_LD(1 - v2 / (C_0 * C_0));
+    dydx[3] = inversegamma / sp->m / ELE_E * (F[0] - (y[3] * y[3] * F[0] + y[3] * y[4] * F[1] + y[3] * y[5] * F[2]) / C_0 / C_0);
+    dydx[4] = inversegamma / sp->m / ELE_E * (F[1] - (y[4] * y[3] * F[0] + y[4] * y[4] * F[1] + y[4] * y[5] * F[2]) / C_0 / C_0);
+    dydx[5] = inversegamma / sp->m / ELE_E * (F[2] - (y[5] * y[3] * F[0] + y[5] * y[4] * F[1] + y[5] * y[5] * F[2]) / C_0 / C_0);
+    dydx[6] = inversegamma;
+    dydx[7] = 0;
+    dydx[8] = sqrt(v2);
+}
+
+/* TParticle::derivs, src/particle.cpp:96-103 */
+static void derivs(track_t *tp, const double *y, double *dydx, double t) {
+    double B[3] = {0, 0, 0}, dB[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, E[3] = {0, 0, 0}, V = 0;
+    tp->n_rhs++;
+    if (tp->sp.q != 0 || (tp->sp.mu != 0 && y[7] != 0)) field_B(tp, y[0], y[1], y[2], t, B, dB);
+    if (tp->sp.q != 0) field_E(tp, y[0], y[1], y[2], t, &V, E);
+    eom(&tp->sp, y, B, dB, E, dydx);
+}
+
+/* TParticle::GetKineticEnergy, src/particle.cpp:237-251 */
+static double kinetic_energy(const species_t *sp, const double *v) {
+    double beta2 = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / C_0 / C_0;
+    double gm1;
+    if (R_POW(beta2, 5) < DBL_EPS)
+        gm1 = (0.5 + (3.0 / 8.0 + (5.0 / 16.0 + 35.0 / 128.0 * beta2) * beta2) * beta2) * beta2;
+    else
+        gm1 = 1.0 / sqrt(1.0 - beta2) - 1.0;
+    return C_0 * C_0 * sp->m * gm1;
+}
+
+/* TParticle::GetPotentialEnergy, src/particle.cpp:254-269 */
+static double potential_energy(track_t *tp, double t, const double *y) {
+    double result = 0;
+    const species_t *sp = &tp->sp;
+    if (sp->q != 0 || sp->mu != 0) {
+        double B[3], E[3], V;
+        if (sp->mu != 0) {
+            field_B(tp, y[0], y[1], y[2], t, B, 0);
+            result += -y[7] * sp->mu / ELE_E * sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+        }
+        if (sp->q != 0) {
+            field_E(tp, y[0], y[1], y[2], t, &V, E);
+            result += sp->q / ELE_E * V;
+        }
+    }
+    result += sp->m * GRAV * y[2];
+    return result;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Boost.Odeint dopri5 with controller and dense output (Boost >= 1.60)      */
+/* ------------------------------------------------------------------------ */
+
+static void dopri_initialize(dopri_t *s, const double *x, double t, double dt) {
+    memcpy(s->xc, x, sizeof(s->xc));
+    s->t = t;
+    s->dt = dt;
+    s->deriv_init = 0;
+}
+
+/* runge_kutta_dopri5::do_step_impl + calc_error; returns 1 if accepted */
+static int dopri_try(track_t *tp, dopri_t *s, double xnew[9], double dxnew[9]) {
+    const double a2 = 1.0 / 5, a3 = 3.0 / 10, a4 = 4.0 / 5, a5 = 8.0 / 9;
+    const double b21 = 1.0 / 5;
+    const double b31 = 3.0 / 40, b32 = 9.0 / 40;
+    const double b41 = 44.0 / 45, b42 = -56.0 / 15, b43 = 32.0 / 9;
+    const double b51 = 19372.0 / 6561, b52 = -25360.0 / 2187, b53 = 64448.0 / 6561, b54 = -212.0 / 729;
+    const double b61 = 9017.0 / 3168, b62 = -355.0 / 33, b63 = 46732.0 / 5247, b64 = 49.0 / 176, b65 = -5103.0 / 18656;
+    const double c1 = 35.0 / 384, c3 = 500.0 / 1113, c4 = 125.0 / 192, c5 = -2187.0 / 6784, c6 = 11.0 / 84;
+    const double dc1 = c1 - 5179.0 / 57600, dc3 = c3 - 7571.0 / 16695, dc4 = c4 - 393.0 / 640,
+                 dc5 = c5 - (-92097.0 / 339200), dc6 = c6 - 187.0 / 2100, dc7 = -1.0 / 40;
+    const double *x = s->xc, *k1 = s->dxc;
+    double dt = s->dt, t = s->t;
+    double tmp[9], k2[9], xerr[9];
+    int i;
+    for (i = 0; i < 9; ++i) tmp[i] = 1.0 * x[i] + (dt * b21) * k1[i];
+    derivs(tp, tmp, k2, t + dt * a2);
+    for (i = 0; i < 9; ++i) tmp[i] = 1.0 * x[i] + (dt * b31) * k1[i] + (dt * b32) * k2[i];
+    derivs(tp, tmp, s->k3, t + dt * a3);
+    for (i = 0; i < 9; ++i) tmp[i] = 1.0 * x[i] + (dt * b41) * k1[i] + (dt * b42) * k2[i] + (dt * b43) * s->k3[i];
+    derivs(tp, tmp, s->k4, t + dt * a4);
+    for (i = 0; i < 9; ++i) tmp[i] = 1.0 * x[i] + (dt * b51) * k1[i] + (dt * b52) * k2[i] + (dt * b53) * s->k3[i] + (dt * b54) * s->k4[i];
+    derivs(tp, tmp, s->k5, t + dt * a5);
+    for (i = 0; i < 9; ++i) tmp[i] = 1.0 * x[i] + (dt * b61) * k1[i] + (dt * b62) * k2[i] + (dt * b63) * s->k3[i] + (dt * b64) * s->k4[i] + (dt * b65) * s->k5[i];
+    derivs(tp, tmp, s->k6, t + dt);
+    for (i = 0; i < 9; ++i) xnew[i] = 1.0 * x[i] + (dt * c1) * k1[i] + (dt * c3) * s->k3[i] + (dt * c4) * s->k4[i] + (dt * c5) * s->k5[i] + (dt * c6) * s->k6[i];
+    derivs(tp, xnew, dxnew, t + dt);
+    for (i = 0; i < 9; ++i) xerr[i] = (dt * dc1) * k1[i] + (dt * dc3) * s->k3[i] + (dt * dc4) * s->k4[i] + (dt * dc5) * s->k5[i] + (dt * dc6) * s->k6[i] + (dt * dc7) * dxnew[i];
+    /* default_error_checker::error + norm_inf */
+    double err = 0;
+    for (i = 0; i < 9; ++i) {
+        double e = fabs(xerr[i]) / (1e-9 + 1e-9 * (1.0 * fabs(x[i]) + dt * fabs(k1[i])));
+        err = (err < e) ? e : err;
+    }
+    if (err > 1.0) {
+        /* default_step_adjuster::decrease_step, error_order 4 */
+        double f = 9.0 / 10 * R_POW(err, -1.0 / 3);
+        double m = 1.0 / 5;
+        s->dt = dt * ((f < m) ? m : f);
+        return 0;
+    }
+    s->t_old = t;
+    s->t = t + dt;
+    if (err < 0.5) {
+        /* increase_step, stepper_order 5 */
+        double emin = R_POW(5.0, -5.0);
+        err = (emin < err) ? err : emin;
+        s->dt = dt * (9.0 / 10 * R_POW(err, -1.0 / 5));
+    }
+    return 1;
+}
+
+/* dense_output_runge_kutta::do_step; returns 0 on success, -1 when the
+   failed-step checker throws (500 tries) */
+static int dopri_do_step(track_t *tp, dopri_t *s) {
+    if (!s->deriv_init) {
+        derivs(tp, s->xc, s->dxc, s->t);
+        s->deriv_init = 1;
+    }
+    int tries = 0;
+    for (;;) {
+        double xnew[9], dxnew[9];
+        int ok = dopri_try(tp, s, xnew, dxnew);
+        if (ok) {
+            memcpy(s->xo, s->xc, sizeof(s->xo));
+            memcpy(s->dxo, s->dxc, sizeof(s->dxo));
+            memcpy(s->xc, xnew, sizeof(s->xc));
+            memcpy(s->dxc, dxnew, sizeof(s->dxc));
+            tp->n_acc++;
+        } else {
+            tp->n_rej++;
+        }
+        if (tries++ >= 500) return -1;
+        if (ok) return 0;
+    }
+}
+
+/* runge_kutta_dopri5::calc_state (dense output) */
+static void dopri_calc_state(track_t *tp, const dopri_t *s, double t, double *x) {
+    const double b1 = 35.0 / 384, b3 = 500.0 / 1113, b4 = 125.0 / 192, b5 = -2187.0 / 6784, b6 = 11.0 / 84;
+    const double dt = (s->t - s->t_old);
+    const double theta = (t - s->t_old) / dt;
+    const double X1 = 5.0 * (2558722523.0 - 31403016.0 * theta) / 11282082432.0;
+    const double X3 = 100.0 * (882725551.0 - 15701508.0 * theta) / 32700410799.0;
+    const double X4 = 25.0 * (443332067.0 - 31403016.0 * theta) / 1880347072.0;
+    const double X5 = 32805.0 * (23143187.0 - 3489224.0 * theta) / 199316789632.0;
+    const double X6 = 55.0 * (29972135.0 - 7076736.0 * theta) / 822651844.0;
+    const double X7 = 10.0 * (7414447.0 - 829305.0 * theta) / 29380423.0;
+    const double theta_m_1 = theta - 1.0;
+    const double theta_sq = theta * theta;
+    const double A = theta_sq * (3.0 - 2.0 * theta);
+    const double B = theta_sq * theta_m_1;
+    const double C = theta_sq * theta_m_1 * theta_m_1;
+    const double D = theta * theta_m_1 * theta_m_1;
+    const double b1t = A * b1 - C * X1 + D;
+    const double b3t = A * b3 + C * X3;
+    const double b4t = A * b4 - C * X4;
+    const double b5t = A * b5 + C * X5;
+    const double b6t = A * b6 - C * X6;
+    const double b7t = B + C * X7;
+    for (int i = 0; i < 9; ++i)
+        x[i] = 1.0 * s->xo[i] + (dt * b1t) * s->dxo[i] + (dt * b3t) * s->k3[i] + (dt * b4t) * s->k4[i] + (dt * b5t) * s->k5[i] + (dt * b6t) * s->k6[i] + (dt * b7t) * s->dxc[i];
+}
+
+/* ------------------------------------------------------------------------ */
+/* species hooks                                                             */
+/* ------------------------------------------------------------------------ */
+
+static double material_potential(track_t *tp, const double *mat, const double *y) {
+    /* TNeutron::MaterialPotential, include/neutron.h:53-55 (GetMagneticMoment returns double) */
+    double mu = (double)tp->sp.mu;
+    return mat[0] * 1e-9 - mat[6] * mu * y[7] / ELE_E;
+}
+
+static void neutron_reflect(const double *y1, double *x2, double x1, double *y2, const double *normal) {
+    double vnormal = y1[3] * normal[0] + y1[4] * normal[1] + y1[5] * normal[2];
+    *x2 = x1;
+    y2[0] = y1[0]; y2[1] = y1[1]; y2[2] = y1[2];
+    y2[3] -= 2 * vnormal * normal[0];
+    y2[4] -= 2 * vnormal * normal[1];
+    y2[5] -= 2 * vnormal * normal[2];
+    y2[6] = y1[6];
+    y2[8] = y1[8];
+}
+
+static void neutron_reflect_lambert(track_t *tp, const double *y1, double *x2, double x1, double *y2, const double *normal, const double *mat) {
+    double vnormal = y1[3] * normal[0] + y1[4] * normal[1] + y1[5] * normal[2];
+    double n[3] = {normal[0], normal[1], normal[2]};
+    if (vnormal > 0) { vnormal *= -1; n[0] *= -1; n[1] *= -1; n[2] *= -1; }
+    *x2 = x1;
+    y2[0] = y1[0]; y2[1] = y1[1]; y2[2] = y1[2];
+    double vabs = sqrt(y1[3] * y1[3] + y1[4] * y1[4] + y1[5] * y1[5]);
+    do {
+        double phi_r = rnd_uniform(tp, 0, (double)(2. * PI));
+        double theta_r = rnd_sincos(tp, 0, (double)(PI / 2.));
+        y2[3] = vabs * R_COS(phi_r) * R_SIN(theta_r);
+        y2[4] = vabs * R_SIN(phi_r) * R_SIN(theta_r);
+        y2[5] = vabs * R_COS(theta_r);
+        if (mat[2] > 0) {
+            rotate_vector(y2 + 3, n, y1 + 3);
+        } else {
+            double sr[3] = {y1[3] - 2 * vnormal * n[0], y1[4] - 2 * vnormal * n[1], y1[5] - 2 * vnormal * n[2]};
+            rotate_vector(y2 + 3, sr, y1 + 3);
+        }
+    } while (y2[3] * n[0] + y2[4] * n[1] + y2[5] * n[2] <= 0);
+    y2[6] = y1[6];
+    y2[8] = y1[8];
+}
+
+static void neutron_transmit(track_t *tp, const double *y1, double *y2, const double *normal, double Estep) {
+    double vnormal = y1[3] * normal[0] + y1[4] * normal[1] + y1[5] * normal[2];
+    double Enormal = 0.5 * MASS_N * vnormal * vnormal;
+    double k1 = sqrt(Enormal);
+    double k2 = sqrt(Enormal - Estep);
+    (void)tp;
+    for (int i = 0; i < 3; i++) y2[i + 3] += (k2 / k1 - 1) * (normal[i] * vnormal);
+}
+
+static void neutron_transmit_lambert(track_t *tp, const double *y1, double *y2, const double *normal, double Estep, const double *mat) {
+    double vnormal = y1[3] * normal[0] + y1[4] * normal[1] + y1[5] * normal[2];
+    double n[3] = {normal[0], normal[1], normal[2]};
+    if (vnormal < 0) { vnormal *= -1; n[0] *= -1; n[1] *= -1; n[2] *= -1; }
+    double Enormal = 0.5 * MASS_N * vnormal * vnormal;
+    double k1 = sqrt(Enormal);
+    double k2 = sqrt(Enormal - Estep);
+    double st[3] = {y2[3], y2[4], y2[5]};
+    /* valarray: specular_trans += (k2/k1 - 1)*n*vnormal -> ((k2/k1-1)*n[i])*vnormal */
+    for (int i = 0; i < 3; ++i) st[i] += ((k2 / k1 - 1) * n[i]) * vnormal;
+    double vabs = sqrt(st[0] * st[0] + st[1] * st[1] + st[2] * st[2]);
+    do {
+        double phi_t = rnd_uniform(tp, 0, (double)(2. * PI));
+        double theta_t = rnd_sincos(tp, 0, (double)(PI / 2.));
+        y2[3] = vabs * R_COS(phi_t) * R_SIN(theta_t);
+        y2[4] = vabs * R_SIN(phi_t) * R_SIN(theta_t);
+        y2[5] = vabs * R_COS(theta_t);
+        if (mat[2] > 0) rotate_vector(y2 + 3, n, y1 + 3);
+        else rotate_vector(y2 + 3, st, y1 + 3);
+    } while (y2[3] * n[0] + y2[4] * n[1] + y2[5] * n[2] <= 0);
+}
+
+/* complex helpers: |(k1-k2)/(k1+k2)|^2 with k = sqrt(E - iW) */
+static double refl_prob(double E1r, double E1i, double E2r, double E2i) {
+#if PTREF_REFSEM
+    double complex k1 = csqrt(CMPLX(E1r, E1i));
+    double complex k2 = csqrt(CMPLX(E2r, E2i));
+    double complex q = (k1 - k2) / (k1 + k2);
+    double a = cabs(q);
+    return a * a;
+#else
+    double k1r, k1i, k2r, k2i, qr, qi;
+    ptm_csqrt(E1r, E1i, &k1r, &k1i);
+    ptm_csqrt(E2r, E2i, &k2r, &k2i);
+    ptm_cdiv(k1r - k2r, k1i - k2i, k1r + k2r, k1i + k2i, &qr, &qi);
+    double a = ptm_hypot(qr, qi);
+    return a * a;
+#endif
+}
+
+/* TNeutron::OnHit, src/neutron.cpp:187-264 (micro-roughness model: MRValid false for zero roughness) */
+static void neutron_onhit(track_t *tp, double x1, const double *y1, double *x2, double *y2, const double *normal,
+                          int leaving, int entering) {
+    double vnormal = y1[3] * normal[0] + y1[4] * normal[1] + y1[5] * normal[2];
+    double Enormal = 0.5 * MASS_N * vnormal * vnormal;
+    const double *lmat = solid_mat(tp->c, leaving), *emat = solid_mat(tp->c, entering);
+    const double *mat = vnormal < 0 ? emat : lmat;
+    if (rnd_uniform(tp, 0, 1) < mat[3]) y2[7] *= -1;
+    double Estep = material_potential(tp, emat, y2) - material_potential(tp, lmat, y2);
+    if (mat[4] != 0 && mat[5] != 0) { tp->stopID = ID_UNSUPPORTED; return; } /* micro-roughness model */
+    double prob = rnd_uniform(tp, 0, 1);
+    double reflprob = refl_prob(Enormal, -lmat[1] * 1e-9, Enormal - Estep, -emat[1] * 1e-9);
+    if (Enormal > Estep) {
+        if (prob < 0 + 0 + reflprob * (1 - 0 - 0)) {
+            if (rnd_uniform(tp, 0, 1) < mat[2] + mat[7]) neutron_reflect_lambert(tp, y1, x2, x1, y2, normal, mat);
+            else neutron_reflect(y1, x2, x1, y2, normal);
+        } else {
+            if (rnd_uniform(tp, 0, 1) < mat[2] + mat[7]) neutron_transmit_lambert(tp, y1, y2, normal, Estep, mat);
+            else neutron_transmit(tp, y1, y2, normal, Estep);
+        }
+    } else {
+        double absprob = 1 - reflprob + mat[8];
+        if (prob < 0 + 0 + absprob * (1 - 0 - 0)) {
+            tp->stopID = ID_ABSORBED_ON_SURFACE;
+        } else {
+            if (rnd_uniform(tp, 0, 1) < mat[2] + mat[7]) neutron_reflect_lambert(tp, y1, x2, x1, y2, normal, mat);
+            else neutron_reflect(y1, x2, x1, y2, normal);
+        }
+    }
+}
+
+/* OnStep: neutron bulk absorption (src/neutron.cpp:267-282); charged species
+   absorbed inside any solid with ID > 1 (src/proton.cpp:25-33, src/electron.cpp:25-33) */
+static void on_step(track_t *tp, double x1, const double *y1, double *x2, double *y2, const dopri_t *st, int cursolid) {
+    if (tp->sp.kind == 0) {
+        const double *mat = solid_mat(tp->c, cursolid);
+        if (mat[1] > 0) {
+            double mn = (double)MASS_N, e = (double)ELE_E, hb = (double)HBAR;
+            double Er = 0.5 * mn * (y1[3] * y1[3] + y1[4] * y1[4] + y1[5] * y1[5]);
+            double Ei = mat[1] * 1e-9;
+            double kr, ki;
+#if PTREF_REFSEM
+            double complex k = csqrt(CMPLX((2 * mn) * Er, (2 * mn) * Ei));
+            kr = creal(k); ki = cimag(k);
+#else
+            ptm_csqrt((2 * mn) * Er, (2 * mn) * Ei, &kr, &ki);
+#endif
+            kr = kr * e; ki = ki * e;
+            kr = kr / hb; ki = ki / hb;
+            (void)kr;
+            double l = sqrt(SQ(y2[0] - y1[0]) + SQ(y2[1] - y1[1]) + SQ(y2[2] - y1[2]));
+            double abspath = rnd_exponential(tp, 2 * ki);
+            if (abspath < l) {
+                *x2 = x1 + abspath / l * (*x2 - x1);
+                dopri_calc_state(tp, st, *x2, y2);
+                tp->stopID = ID_ABSORBED_IN_MATERIAL;
+            }
+        }
+    } else {
+        if (solid_ID(tp->c, cursolid) > 1) {
+            *x2 = x1;
+            memcpy(y2, y1, 9 * sizeof(double));
+            tp->stopID = ID_ABSORBED_IN_MATERIAL;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* tracker                                                                   */
+/* ------------------------------------------------------------------------ */
+
+static int state_eq(const double *a, const double *b) {
+    for (int i = 0; i < 9; ++i) if (!(a[i] == b[i])) return 0;
+    return 1;
+}
+
+/* TTracker::DoStep + TParticle::DoStep */
+static int do_step_hook(track_t *tp, double x1, const double *y1, double *x2, double *y2, const dopri_t *st, int cursolid) {
+    double x2temp = *x2, y2temp[9];
+    memcpy(y2temp, y2, sizeof(y2temp));
+    on_step(tp, x1, y1, x2, y2, st, cursolid);
+    double H = kinetic_energy(&tp->sp, y2 + 3) + potential_energy(tp, *x2, y2);
+    tp->Hmax = (H < tp->Hmax) ? tp->Hmax : H;
+    if (y2temp[7] != y2[7]) tp->Nspinflip++;
+    tp->Nstep++;
+    return !(x2temp == *x2 && state_eq(y2temp, y2));
+}
+
+/* TTracker::DoHit, src/tracking.cpp:280-365 */
+static int do_hit(track_t *tp, double x1, const double *y1, double *x2, double *y2, const dopri_t *st) {
+    const ctx_t *c = tp->c;
+    static __thread coll_list colls;
+    (void)st;
+    int nc = get_collisions(tp, x1, y1, *x2, y2, &colls);
+    if (nc <= 0) { tp->stopID = nc < 0 ? ID_CGAL_ERROR : ID_INVARIANT; return 1; }
+    int trajectoryaltered = 0, traversed = 1;
+    solid_stack ns = tp->cur;
+    for (int k = 0; k < nc; ++k) {
+        const coll_t *cl = &colls.c[k];
+        int sidx = sidx_of_id(c, cl->ID);
+        int found = -1;
+        for (int i = 0; i < ns.n; ++i) if (solid_ID(c, ns.s[i].sidx) == cl->ID) { found = i; break; }
+        if (cl->distnormal < 0) {
+            if (found >= 0) {
+                if (cl->s > 0 && ns.n < MAX_SOLIDS_STACK) { ns.s[ns.n].sidx = sidx; ns.s[ns.n].ignored = ns.s[found].ignored; ns.n++; }
+            } else if (ns.n < MAX_SOLIDS_STACK) {
+                ns.s[ns.n].sidx = sidx; ns.s[ns.n].ignored = cl->ignored; ns.n++;
+            }
+        } else if (cl->distnormal > 0) {
+            if (found < 0) {
+                if (cl->s > 0) { tp->stopID = ID_GEOMETRY_ERROR; return 1; }
+            } else {
+                for (int i = found; i < ns.n - 1; ++i) ns.s[i] = ns.s[i + 1];
+                ns.n--;
+            }
+        } else {
+            tp->stopID = ID_GEOMETRY_ERROR;
+            return 1;
+        }
+    }
+    int leaving = current_solid(c, &tp->cur);
+    int entering = -1;
+    for (int i = 0; i < ns.n; ++i)
+        if (!ns.s[i].ignored && solid_ID(c, ns.s[i].sidx) > solid_ID(c, entering)) entering = ns.s[i].sidx;
+    if (solid_ID(c, leaving) != solid_ID(c, entering)) {
+        const coll_t *cl = 0;
+        for (int k = 0; k < nc; ++k) {
+            if (colls.c[k].ID == solid_ID(c, leaving) || (colls.c[k].ID == solid_ID(c, entering) && !colls.c[k].ignored)) { cl = &colls.c[k]; break; }
+        }
+        if (!cl) { tp->stopID = ID_INVARIANT; return 1; }
+        double x2temp = *x2, y2temp[9];
+        memcpy(y2temp, y2, sizeof(y2temp));
+        /* TParticle::DoHit */
+        if (tp->sp.kind == 0) neutron_onhit(tp, x1, y1, x2, y2, cl->normal, leaving, entering);
+        if (y2temp[7] != y2[7]) tp->Nspinflip++;
+        tp->Nhit++;
+        if (x2temp == *x2 && state_eq(y2temp, y2)) {
+            trajectoryaltered = 0; traversed = 1;
+        } else {
+            trajectoryaltered = 1;
+            if (*x2 == x2temp && y2[0] == y2temp[0] && y2[1] == y2temp[1] && y2[2] == y2temp[2]) traversed = 1;
+            else if (*x2 == x1 && y2[0] == y1[0] && y2[1] == y1[1] && y2[2] == y1[2]) traversed = 0;
+            else { tp->stopID = ID_INVARIANT; return 1; }
+        }
+        if (tp->hits && tp->n_hits < tp->hit_cap) {
+            ptref_hit *h = &tp->hits[tp->n_hits];
+            h->t = x1; h->tri = cl->tri;
+            h->leaving = (int32_t)solid_ID(c, leaving); h->entering = (int32_t)solid_ID(c, entering);
+            for (int i = 0; i < 3; ++i) { h->normal[i] = cl->normal[i]; h->v1[i] = y1[3 + i]; h->v2[i] = y2[3 + i]; }
+            h->pol1 = y1[7]; h->pol2 = y2[7];
+        }
+        tp->n_hits++;
+    }
+    if (traversed) tp->cur = ns;
+    if (trajectoryaltered || tp->stopID != ID_UNKNOWN) return 1;
+    return 0;
+}
+
+/* TTracker::iterate_collision, src/tracking.cpp:228-265 */
+static int iterate_collision(track_t *tp, double *x1, double *y1, double *x2, double *y2, const dopri_t *st, unsigned iteration) {
+    static __thread coll_list colls;
+    if (SQ(y2[0] - y1[0]) + SQ(y2[1] - y1[1]) + SQ(y2[2] - y1[2]) < REFLECT_TOLERANCE * REFLECT_TOLERANCE) return 1;
+    if (*x2 - *x1 < 4 * (*x1 + *x2) * DBL_EPS) return 1;
+    if (iteration >= 100) return 1;
+    tp->n_bisect++;
+    double xc = *x1 + (*x2 - *x1) * 0.5;
+    double yc[9];
+    dopri_calc_state(tp, st, xc, yc);
+    int n1 = get_collisions(tp, *x1, y1, xc, yc, &colls);
+    if (n1 < 0) return -1;
+    if (n1 > 0) {
+        int r = iterate_collision(tp, x1, y1, &xc, yc, st, iteration + 1);
+        if (r < 0) return r;
+        if (r) { *x2 = xc; memcpy(y2, yc, sizeof(yc)); return 1; }
+    }
+    int n2 = get_collisions(tp, xc, yc, *x2, y2, &colls);
+    if (n2 < 0) return -1;
+    if (n2 > 0) {
+        int r = iterate_collision(tp, &xc, yc, x2, y2, st, iteration + 1);
+        if (r < 0) return r;
+        if (r) { *x1 = xc; memcpy(y1, yc, sizeof(yc)); return 1; }
+    }
+    return 0;
+}
+
+/* TTracker::CheckHit, src/tracking.cpp:176-226 */
+static int check_hit(track_t *tp, double x1, const double *y1, double *x2, double *y2, const dopri_t *st) {
+    static __thread coll_list colls;
+    if (!check_segment(tp->c, y1, y2)) { tp->stopID = ID_HIT_BOUNDARIES; return 1; }
+    if (*x2 == x1) return 0;
+    int cursolid = current_solid(tp->c, &tp->cur);
+    int n = get_collisions(tp, x1, y1, *x2, y2, &colls);
+    if (n < 0) { tp->stopID = ID_CGAL_ERROR; return 1; }
+    if (n > 0) {
+        double xc1 = x1, xc2 = *x2, yc1[9], yc2[9];
+        memcpy(yc1, y1, sizeof(yc1));
+        memcpy(yc2, y2, sizeof(yc2));
+        int r = iterate_collision(tp, &xc1, yc1, &xc2, yc2, st, 0);
+        if (r < 0) { tp->stopID = ID_CGAL_ERROR; return 1; }
+        if (r) {
+            if (xc1 > x1 && do_step_hook(tp, x1, y1, &xc1, yc1, st, cursolid)) {
+                *x2 = xc1; memcpy(y2, yc1, sizeof(yc1));
+                return 1;
+            }
+            if (do_hit(tp, xc1, yc1, &xc2, yc2, st)) {
+                *x2 = xc2; memcpy(y2, yc2, sizeof(yc2));
+                return 1;
+            }
+            if (check_hit(tp, xc2, yc2, x2, y2, st)) return 1;
+        }
+    } else {
+        return do_step_hook(tp, x1, y1, x2, y2, st, cursolid);
+    }
+    return 0;
+}
+
+/* TParticle::DoPolarize, src/particle.cpp:225-234 */
+static void do_polarize(track_t *tp, double *y, double polarization, int flipspin) {
+    double flipprob = 0.5 * (1 - y[7] * polarization);
+    tp->noflipprob *= 1. - flipprob;
+    if (flipspin) {
+        double prevpol = y[7];
+        /* polarization_distribution: bernoulli(0.5*(p+1)) */
+        uint64_t r = ptm_draw_u64(tp->key, &tp->ctr);
+        y[7] = (ptm_canonical(r) < 0.5 * (polarization + 1)) ? 1 : -1;
+        if (y[7] != prevpol) ++tp->Nspinflip;
+    }
+}
+
+/* TTracker::IntegrateSpin without the BMT integration branch (src/tracking.cpp:373-394,473-483) */
+static double integrate_spin(track_t *tp, double *spin, const dopri_t *st, double x2, const double *y2, double Bmax) {
+    double x1 = st->t_old;
+    const double *y1 = st->xo;
+    double B1[3], B2[3];
+    field_B(tp, y1[0], y1[1], y1[2], x1, B1, 0);
+    field_B(tp, y2[0], y2[1], y2[2], x2, B2, 0);
+    double Babs1 = sqrt(B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]);
+    double Babs2 = sqrt(B2[0] * B2[0] + B2[1] * B2[1] + B2[2] * B2[2]);
+    if ((double)tp->sp.gamma == 0 || Babs2 == 0) {
+        for (int i = 0; i < 5; ++i) spin[i] = 0;
+        return y1[7];
+    } else if (Babs1 == 0 && Babs2 > 0) {
+        int sp = ptm_uniform_int01(tp->key, &tp->ctr) * 2 - 1;
+        for (int i = 0; i < 5; ++i) spin[i] = 0;
+        return sp;
+    } else if ((x2 > x1) && (Babs1 < Bmax || Babs2 < Bmax)) {
+        /* no spin-tracking time windows configured: parallel transport */
+        if (spin[0] != 0 || spin[1] != 0 || spin[2] != 0) {
+            double sp = (spin[0] * B1[0] + spin[1] * B1[1] + spin[2] * B1[2]) / Babs1 / sqrt(spin[0] * spin[0] + spin[1] * spin[1] + spin[2] * spin[2]);
+            if (sp * sp >= 1) { spin[0] = 0; spin[1] = 0; }
+            else {
+                double spinaz = rnd_uniform(tp, 0, (double)(2. * PI));
+                spin[0] = sqrt(1 - sp * sp) * R_SIN(spinaz);
+                spin[1] = sqrt(1 - sp * sp) * R_COS(spinaz);
+            }
+            spin[2] = sp;
+            rotate_vector(spin, B2, 0);
+        }
+    }
+    if (Babs2 > Bmax && (spin[0] != 0 || spin[1] != 0 || spin[2] != 0)) {
+        double sp = (spin[0] * B2[0] + spin[1] * B2[1] + spin[2] * B2[2]) / Babs2 / sqrt(spin[0] * spin[0] + spin[1] * spin[1] + spin[2] * spin[2]);
+        for (int i = 0; i < 5; ++i) spin[i] = 0;
+        return sp;
+    }
+    return y1[7];
+}
+
+/* TTracker::IntegrateParticle, src/tracking.cpp:17-173 */
+static void integrate_particle(const ctx_t *c, const ptref_species *cfg, double tmax_global,
+                               double t0, const double *y0, const double *spin0, uint64_t seed, uint64_t pid,
+                               ptref_out *out, ptref_hit *hits, int hit_cap) {
+    static __thread track_t tpv;
+    track_t *tp = &tpv;
+    memset(tp, 0, sizeof(*tp));
+    tp->c = c;
+    tp->cfg = cfg;
+    tp->sp.kind = cfg->kind;
+    if (cfg->kind == 0) { tp->sp.q = 0; tp->sp.m = MASS_N; tp->sp.mu = MU_N; tp->sp.gamma = GAMMA_N; }
+    else if (cfg->kind == 1) { tp->sp.q = (double)ELE_E; tp->sp.m = MASS_P; tp->sp.mu = 0; tp->sp.gamma = 0; }
+    else { tp->sp.q = (double)(-ELE_E); tp->sp.m = MASS_E; tp->sp.mu = 0; tp->sp.gamma = 0; }
+    tp->key = ptm_stream_key(seed, pid);
+    tp->ctr = 0;
+    tp->hits = hits; tp->hit_cap = hit_cap;
+    tp->noflipprob = 1;
+    tp->cur.n = 1; tp->cur.s[0].sidx = -1; tp->cur.s[0].ignored = 0;
+
+    double x = t0, y[9], spin[5];
+    memcpy(y, y0, sizeof(y));
+    memcpy(spin, spin0, sizeof(spin));
+    /* Hmax starts at the initial total energy (TParticle ctor, src/particle.cpp:89) */
+    tp->Hmax = kinetic_energy(&tp->sp, y + 3) + potential_energy(tp, x, y);
+
+    double tau = 0;
+    if (cfg->tau > 0) tau = rnd_exponential(tp, 1. / cfg->tau);
+    else tau = cfg->tmax;
+    double maxtraj = cfg->lmax;
+    out->tau = tau;
+    if (x > tmax_global) { tp->stopID = ID_INVARIANT; goto done_noint; }
+
+    {
+    int resetintegration = 0;
+    static __thread dopri_t st;
+    dopri_initialize(&st, y, x, 10. * MAX_TRACK_DEVIATION / sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]));
+    get_solids(c, x, y, &tp->cur);
+    tp->stopID = ID_UNKNOWN;
+    while (tp->stopID == ID_UNKNOWN) {
+        if (resetintegration) dopri_initialize(&st, y, x, st.dt);
+        double x1 = x, y1[9];
+        memcpy(y1, y, sizeof(y1));
+        if (dopri_do_step(tp, &st) < 0) {
+            tp->stopID = ID_ODEINT_ERROR;
+        }
+        x = st.t;
+        memcpy(y, st.xc, sizeof(y));
+        if (tau > 0 && y[6] > tau) {
+            x = x1 + (x - x1) * (tau - y1[6]) / (y[6] - y1[6]);
+            dopri_calc_state(tp, &st, x, y);
+        }
+        if (x > tmax_global) {
+            x = tmax_global;
+            dopri_calc_state(tp, &st, x, y);
+        }
+        while (x1 < x) {
+            double l2 = SQ(y[8] - y1[8]);
+            double d2 = SQ(y[0] - y1[0]) + SQ(y[1] - y1[1]) + SQ(y[2] - y1[2]);
+            double dev2 = 0.25 * (l2 - d2);
+            double x2 = x, y2[9];
+            memcpy(y2, y, sizeof(y2));
+            if (dev2 > MAX_TRACK_DEVIATION * MAX_TRACK_DEVIATION) {
+                x2 = x1 + (x - x1) / ceil(sqrt(dev2) / MAX_TRACK_DEVIATION);
+                dopri_calc_state(tp, &st, x2, y2);
+            }
+            tp->n_pieces++;
+            resetintegration = check_hit(tp, x1, y1, &x2, y2, &st);
+            if (resetintegration) { x = x2; memcpy(y, y2, sizeof(y)); }
+            x1 = x2;
+            memcpy(y1, y2, sizeof(y1));
+        }
+        double spinProjection = integrate_spin(tp, spin, &st, x, y, cfg->Bmax);
+        int spinHasFlipped = st.xo[7] != y[7];
+        if (spinProjection != st.xo[7]) {
+            do_polarize(tp, y, spinProjection, cfg->flipspin);
+            if (cfg->flipspin && spinHasFlipped) y[7] *= -1;
+        } else if (spinHasFlipped) {
+            for (int i = 0; i < 5; ++i) spin[i] = 0;
+        }
+        if (tp->stopID == ID_UNKNOWN && y[6] >= tau) tp->stopID = ID_DECAYED;
+        else if (tp->stopID == ID_UNKNOWN && (x >= tmax_global || y[8] >= maxtraj)) tp->stopID = ID_NOT_FINISH;
+    }
+    }
+done_noint:
+    out->t = x;
+    memcpy(out->y, y, sizeof(out->y));
+    memcpy(out->spin, spin, sizeof(out->spin));
+    out->stopID = tp->stopID;
+    out->Nhit = tp->Nhit; out->Nstep = tp->Nstep; out->Nspinflip = tp->Nspinflip;
+    out->n_acc = tp->n_acc; out->n_rej = tp->n_rej;
+    out->Hmax = tp->Hmax;
+    out->noflipprob = (double)tp->noflipprob;
+    out->solidend = (int32_t)solid_ID(c, current_solid(c, &tp->cur));
+    out->n_hits_logged = tp->n_hits;
+    out->n_rhs = tp->n_rhs; out->n_field = tp->n_field; out->n_tri_tests = tp->n_tri;
+    out->n_bisect = tp->n_bisect; out->n_pieces = tp->n_pieces;
+}
+
+/* ------------------------------------------------------------------------ */
+/* exported entry points                                                     */
+/* ------------------------------------------------------------------------ */
+
+static const ptref_geom *g_sort_geom;
+static int g_sort_axis;
+static int cmp_centroid(const void *a, const void *b) {
+    int i = *(const int *)a, j = *(const int *)b;
+    const double *A = g_sort_geom->tri + 9 * i, *B = g_sort_geom->tri + 9 * j;
+    double ca = A[g_sort_axis] + A[3 + g_sort_axis] + A[6 + g_sort_axis];
+    double cb = B[g_sort_axis] + B[3 + g_sort_axis] + B[6 + g_sort_axis];
+    if (ca != cb) return ca < cb ? -1 : 1;
+    return (i > j) - (i < j);
+}
+static pthread_mutex_t g_sort_lock = PTHREAD_MUTEX_INITIALIZER;
+
+static int build_node(ctx_t *c, int first, int count) {
+    int me = c->n_node++;
+    struct onode *nd = &c->nodes[me];
+    for (int a = 0; a < 3; ++a) { nd->lo[a] = INFINITY; nd->hi[a] = -INFINITY; }
+    for (int k = first; k < first + count; ++k) {
+        const double *b = c->tbox[c->perm[k]];
+        for (int a = 0; a < 3; ++a) {
+            if (b[a] < nd->lo[a]) nd->lo[a] = b[a];
+            if (b[3 + a] > nd->hi[a]) nd->hi[a] = b[3 + a];
+        }
+    }
+    if (count <= 4) { nd->first = first; nd->count = count; nd->left = nd->right = -1; return me; }
+    int axis = 0;
+    for (int a = 1; a < 3; ++a) if (nd->hi[a] - nd->lo[a] > nd->hi[axis] - nd->lo[axis]) axis = a;
+    g_sort_geom = c->g; g_sort_axis = axis;
+    qsort(c->perm + first, count, sizeof(int), cmp_centroid);
+    int half = count / 2;
+    int l = build_node(c, first, half);
+    int r = build_node(c, first + half, count - half);
+    nd = &c->nodes[me];
+    nd->left = l; nd->right = r; nd->first = 0; nd->count = 0;
+    return me;
+}
+
+static int build_ctx(ctx_t *c, const ptref_geom *g, int n_fields, const ptref_field *fields) {
+    memset(c, 0, sizeof(*c));
+    c->g = g;
+    c->n_fields = n_fields;
+    c->fields = fields;
+    if (g->n_solid > 64) return -1;
+    c->n_mesh_bbox = g->n_solid;
+    for (int m = 0; m < g->n_solid; ++m) {
+        double *b = c->bbox[m];
+        b[0] = b[1] = b[2] = INFINITY;
+        b[3] = b[4] = b[5] = -INFINITY;
+    }
+    c->normals = (double (*)[3])malloc(sizeof(double[3]) * (size_t)(g->n_tri > 0 ? g->n_tri : 1));
+    for (int i = 0; i < g->n_tri; ++i) {
+        const double *T = g->tri + 9 * i;
+        double *b = c->bbox[g->tri_solid[i]];
+        for (int v = 0; v < 3; ++v)
+            for (int a = 0; a < 3; ++a) {
+                double q = T[3 * v + a];
+                if (q < b[a]) b[a] = q;
+                if (q > b[3 + a]) b[3 + a] = q;
+            }
+        /* compute_face_normal on the face whose halfedge runs v2->v0:
+           0.5 * cross(v1 - v0, v2 - v0), normalised (Polygon_mesh_processing/compute_normal.h:62-72,136-160) */
+        double e1[3] = {T[3] - T[0], T[4] - T[1], T[5] - T[2]};
+        double e2[3] = {T[6] - T[0], T[7] - T[1], T[8] - T[2]};
+        double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        n[0] = n[0] * 0.5; n[1] = n[1] * 0.5; n[2] = n[2] * 0.5;
+        double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (len != 0) { n[0] = n[0] / len; n[1] = n[1] / len; n[2] = n[2] / len; }
+        c->normals[i][0] = n[0]; c->normals[i][1] = n[1]; c->normals[i][2] = n[2];
+    }
+    c->tbox = (double (*)[6])malloc(sizeof(double[6]) * (size_t)(g->n_tri > 0 ? g->n_tri : 1));
+    c->perm = (int *)malloc(sizeof(int) * (size_t)(g->n_tri > 0 ? g->n_tri : 1));
+    c->nodes = (struct onode *)malloc(sizeof(struct onode) * (size_t)(2 * g->n_tri + 1));
+    for (int i = 0; i < g->n_tri; ++i) {
+        const double *T = g->tri + 9 * i;
+        for (int a = 0; a < 3; ++a) {
+            double lo = T[a], hi = T[a];
+            if (T[3 + a] < lo) lo = T[3 + a];
+            if (T[6 + a] < lo) lo = T[6 + a];
+            if (T[3 + a] > hi) hi = T[3 + a];
+            if (T[6 + a] > hi) hi = T[6 + a];
+            double pad = 1e-9 + 1e-9 * (fabs(lo) + fabs(hi));
+            c->tbox[i][a] = lo - pad;
+            c->tbox[i][3 + a] = hi + pad;
+        }
+        c->perm[i] = i;
+    }
+    c->n_node = 0;
+    if (g->n_tri > 0) {
+        pthread_mutex_lock(&g_sort_lock);
+        build_node(c, 0, g->n_tri);
+        pthread_mutex_unlock(&g_sort_lock);
+    }
+    return 0;
+}
+
+static void free_ctx(ctx_t *c) {
+    free(c->normals); free(c->tbox); free(c->perm); free(c->nodes);
+}
+
+typedef struct {
+    const ctx_t *c;
+    const ptref_species *species; const int32_t *species_idx;
+    double tmax_global;
+    const double *t0, *y0, *spin0;
+    uint64_t seed; const uint64_t *pid;
+    ptref_out *out; ptref_hit *hits; int hit_cap;
+    int begin, end;
+} job_t;
+
+static void *worker(void *arg) {
+    job_t *j = (job_t *)arg;
+    for (int i = j->begin; i < j->end; ++i) {
+        integrate_particle(j->c, &j->species[j->species_idx[i]], j->tmax_global, j->t0[i], j->y0 + 9 * i,
+                           j->spin0 + 5 * i, j->seed, j->pid[i], &j->out[i],
+                           j->hits ? j->hits + (size_t)i * j->hit_cap : 0, j->hit_cap);
+    }
+    return 0;
+}
+
+/* Track n particles; threads > 1 splits the batch into contiguous ranges
+   (independent particles, like the reference's job arrays). */
+int ptref_track(const ptref_geom *g, int n_fields, const ptref_field *fields,
+                const ptref_species *species, const int32_t *species_idx, double tmax_global,
+                int n, const double *t0, const double *y0, const double *spin0,
+                uint64_t seed, const uint64_t *pid,
+                ptref_out *out, ptref_hit *hits, int hit_cap, int threads) {
+    ctx_t c;
+    if (build_ctx(&c, g, n_fields, fields)) return -1;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    job_t jobs[256];
+    for (int k = 0; k < threads; ++k) {
+        job_t *j = &jobs[k];
+        j->c = &c; j->species = species; j->species_idx = species_idx; j->tmax_global = tmax_global;
+        j->t0 = t0; j->y0 = y0; j->spin0 = spin0; j->seed = seed; j->pid = pid;
+        j->out = out; j->hits = hits; j->hit_cap = hit_cap;
+        j->begin = (int)((long long)n * k / threads);
+        j->end = (int)((long long)n * (k + 1) / threads);
+    }
+    if (threads == 1) worker(&jobs[0]);
+    else {
+        for (int k = 0; k < threads; ++k) pthread_create(&th[k], 0, worker, &jobs[k]);
+        for (int k = 0; k < threads; ++k) pthread_join(th[k], 0);
+    }
+    free_ctx(&c);
+    return 0;
+}
+
+/* ---- component entry points used by the oracle's own tests ------------- */
+
+int ptref_collisions(const ptref_geom *g, const double *p1, const double *p2, int cap,
+                     double *s, int32_t *tri, uint32_t *ID, double *distnormal, double *normal) {
+    ctx_t c;
+    static __thread coll_list cl;
+    static __thread track_t tp;
+    if (build_ctx(&c, g, 0, 0)) return -2;
+    memset(&tp, 0, sizeof(tp));
+    tp.c = &c;
+    int n = get_collisions(&tp, 0.0, p1, 1.0, p2, &cl);
+    for (int i = 0; i < n && i < cap; ++i) {
+        s[i] = cl.c[i].s; tri[i] = cl.c[i].tri; ID[i] = cl.c[i].ID; distnormal[i] = cl.c[i].distnormal;
+        normal[3 * i] = cl.c[i].normal[0]; normal[3 * i + 1] = cl.c[i].normal[1]; normal[3 * i + 2] = cl.c[i].normal[2];
+    }
+    free_ctx(&c);
+    return n;
+}
+
+int ptref_get_solids(const ptref_geom *g, double t, const double *p, uint32_t *ids, int32_t *ignored, int cap) {
+    ctx_t c;
+    solid_stack st;
+    if (build_ctx(&c, g, 0, 0)) return -2;
+    get_solids(&c, t, p, &st);
+    for (int i = 0; i < st.n && i < cap; ++i) { ids[i] = solid_ID(&c, st.s[i].sidx); ignored[i] = st.s[i].ignored; }
+    free_ctx(&c);
+    return st.n;
+}
+
+int ptref_segment_bbox(const double *p, const double *q, const double *bb) { return seg_bbox(p, q, bb); }
+
+void ptref_rotate_vector(double *v, const double *n, const double *x) { rotate_vector(v, n, x); }
+
+int ptref_field_eval(int n_fields, const ptref_field *fields, int n, const double *pts, double t,
+                     double *B, double *dB, double *V, double *E) {
+    ctx_t c;
+    static __thread track_t tp;
+    memset(&c, 0, sizeof(c));
+    c.n_fields = n_fields; c.fields = fields;
+    memset(&tp, 0, sizeof(tp));
+    tp.c = &c;
+    for (int i = 0; i < n; ++i) {
+        const double *p = pts + 3 * i;
+        double dBl[3][3];
+        field_B(&tp, p[0], p[1], p[2], t, B + 3 * i, dBl);
+        for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) dB[9 * i + 3 * a + b] = dBl[a][b];
+        field_E(&tp, p[0], p[1], p[2], t, V + i, E + 3 * i);
+    }
+    return 0;
+}
+
+double ptref_tricubic_eval(const double *a, double x, double y, double z, int dx, int dy, int dz) {
+    if (dx == 0 && dy == 0 && dz == 0) return tric_eval(a, x, y, z);
+    return tric_eval_d(a, x, y, z, dx, dy, dz);
+}
+
+/* one stand-alone dense-output integration with fixed rhs of a species in the
+   given fields (no geometry): n_steps accepted steps, returns states after each */
+int ptref_integrate_free(int n_fields, const ptref_field *fields, int kind, double t0, const double *y0,
+                         int n_steps, double *ts, double *ys, double *dts) {
+    ctx_t c;
+    static __thread track_t tp;
+    static __thread dopri_t st;
+    memset(&c, 0, sizeof(c));
+    c.n_fields = n_fields; c.fields = fields;
+    memset(&tp, 0, sizeof(tp));
+    tp.c = &c;
+    tp.sp.kind = kind;
+    if (kind == 0) { tp.sp.q = 0; tp.sp.m = MASS_N; tp.sp.mu = MU_N; tp.sp.gamma = GAMMA_N; }
+    else if (kind == 1) { tp.sp.q = (double)ELE_E; tp.sp.m = MASS_P; }
+    else { tp.sp.q = (double)(-ELE_E); tp.sp.m = MASS_E; }
+    dopri_initialize(&st, y0, t0, 10. * MAX_TRACK_DEVIATION / sqrt(y0[3] * y0[3] + y0[4] * y0[4] + y0[5] * y0[5]));
+    for (int k = 0; k < n_steps; ++k) {
+        if (dopri_do_step(&tp, &st) < 0) return -1;
+        ts[k] = st.t;
+        memcpy(ys + 9 * k, st.xc, 9 * sizeof(double));
+        if (dts) dts[k] = st.dt;
+    }
+    return 0;
+}
+
+/* dense output of the last step of ptref_integrate_free-like stepping, for tests */
+int ptref_dense_probe(int kind, double t0, const double *y0, double dt0, int n_theta, const double *theta, double *out_t, double *out_y) {
+    ctx_t c;
+    static __thread track_t tp;
+    static __thread dopri_t st;
+    memset(&c, 0, sizeof(c));
+    memset(&tp, 0, sizeof(tp));
+    tp.c = &c;
+    tp.sp.kind = kind;
+    if (kind == 0) { tp.sp.q = 0; tp.sp.m = MASS_N; tp.sp.mu = MU_N; tp.sp.gamma = GAMMA_N; }
+    dopri_initialize(&st, y0, t0, dt0);
+    if (dopri_do_step(&tp, &st) < 0) return -1;
+    for (int k = 0; k < n_theta; ++k) {
+        double t = st.t_old + (st.t - st.t_old) * theta[k];
+        out_t[k] = t;
+        dopri_calc_state(&tp, &st, t, out_y + 9 * k);
+    }
+    return 0;
+}
+
+/* species constants as the oracle sees them (q, m, mu, gamma), for tests */
+void ptref_species_constants(int kind, double *qmmg) {
+    if (kind == 0) { qmmg[0] = 0; qmmg[1] = (double)MASS_N; qmmg[2] = (double)MU_N; qmmg[3] = (double)GAMMA_N; }
+    else if (kind == 1) { qmmg[0] = (double)ELE_E; qmmg[1] = (double)MASS_P; qmmg[2] = 0; qmmg[3] = 0; }
+    else { qmmg[0] = (double)(-ELE_E); qmmg[1] = (double)MASS_E; qmmg[2] = 0; qmmg[3] = 0; }
+}
+
+int ptref_refsem(void) { return PTREF_REFSEM; }

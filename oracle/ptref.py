@@ -1,0 +1,158 @@
+"""ctypes front end of the CPU oracle (libptref_fp64.so / libptref_refsem.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never uses it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class Geom(C.Structure):
+    _fields_ = [
+        ("n_tri", C.c_int), ("tri", C.POINTER(C.c_double)), ("tri_solid", C.POINTER(C.c_int32)),
+        ("n_solid", C.c_int), ("solid_id", C.POINTER(C.c_uint32)), ("solid_mat", C.POINTER(C.c_int32)),
+        ("ign_off", C.POINTER(C.c_int32)), ("ign", C.POINTER(C.c_double)),
+        ("default_mat", C.c_int), ("n_mat", C.c_int), ("mats", C.POINTER(C.c_double)),
+    ]
+
+
+class Field(C.Structure):
+    _fields_ = [
+        ("type", C.c_int), ("bscale", C.c_double), ("escale", C.c_double),
+        ("box", C.c_double * 6), ("bw", C.c_double),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("xg", C.POINTER(C.c_double)), ("yg", C.POINTER(C.c_double)), ("zg", C.POINTER(C.c_double)),
+        ("has_b", C.c_int), ("has_v", C.c_int), ("coef", C.POINTER(C.c_double)),
+        ("e", C.c_double * 3),
+    ]
+
+
+class Species(C.Structure):
+    _fields_ = [("kind", C.c_int), ("tau", C.c_double), ("tmax", C.c_double), ("lmax", C.c_double),
+                ("flipspin", C.c_int), ("Bmax", C.c_double)]
+
+
+OUT_DTYPE = np.dtype([
+    ("t", "<f8"), ("y", "<f8", (9,)), ("spin", "<f8", (5,)),
+    ("stopID", "<i4"), ("Nhit", "<i4"), ("Nstep", "<i4"), ("Nspinflip", "<i4"),
+    ("n_acc", "<i8"), ("n_rej", "<i8"),
+    ("Hmax", "<f8"), ("noflipprob", "<f8"), ("tau", "<f8"),
+    ("solidend", "<i4"), ("n_hits", "<i4"),
+    ("n_rhs", "<i8"), ("n_field", "<i8"), ("n_tri_tests", "<i8"), ("n_bisect", "<i8"), ("n_pieces", "<i8"),
+], align=True)
+HIT_DTYPE = np.dtype([
+    ("t", "<f8"), ("tri", "<i4"), ("leaving", "<i4"), ("entering", "<i4"),
+    ("normal", "<f8", (3,)), ("v1", "<f8", (3,)), ("v2", "<f8", (3,)),
+    ("pol1", "<f8"), ("pol2", "<f8"),
+], align=True)
+
+_libs = {}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load(mode="fp64"):
+    if mode in _libs:
+        return _libs[mode]
+    path = os.path.join(HERE, f"libptref_{mode}.so")
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    L.ptref_track.argtypes = [C.POINTER(Geom), C.c_int, C.POINTER(Field), C.POINTER(Species), C.POINTER(C.c_int32),
+                              C.c_double, C.c_int, dp, dp, dp, C.c_uint64, C.POINTER(C.c_uint64), C.c_void_p,
+                              C.c_void_p, C.c_int, C.c_int]
+    L.ptref_collisions.argtypes = [C.POINTER(Geom), dp, dp, C.c_int, dp, C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
+                                   dp, dp]
+    L.ptref_get_solids.argtypes = [C.POINTER(Geom), C.c_double, dp, C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.c_int]
+    L.ptref_segment_bbox.argtypes = [dp, dp, dp]
+    L.ptref_rotate_vector.argtypes = [dp, dp, dp]
+    L.ptref_rotate_vector.restype = None
+    L.ptref_field_eval.argtypes = [C.c_int, C.POINTER(Field), C.c_int, dp, C.c_double, dp, dp, dp, dp]
+    L.ptref_tricubic_eval.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
+    L.ptref_tricubic_eval.restype = C.c_double
+    L.ptref_integrate_free.argtypes = [C.c_int, C.POINTER(Field), C.c_int, C.c_double, dp, C.c_int, dp, dp, dp]
+    L.ptref_dense_probe.argtypes = [C.c_int, C.c_double, dp, C.c_double, C.c_int, dp, dp, dp]
+    L.ptref_species_constants.argtypes = [C.c_int, dp]
+    L.ptref_species_constants.restype = None
+    _libs[mode] = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Problem:
+    """Geometry and fields of a workload in the oracle's layout (keeps buffers alive)."""
+
+    def __init__(self, geometry, fields):
+        a = geometry.arrays()
+        # the oracle takes the snapped double vertices
+        tri = np.ascontiguousarray(a["tri"].astype(np.float64))
+        tri[np.abs(tri) < 1e-8] = 0.0
+        self._keep = [tri, a["tri_solid"], a["solid_id"], a["solid_mat"], a["ign_off"], a["ign"], a["mats"]]
+        g = Geom()
+        g.n_tri = len(tri)
+        g.tri = _dp(tri)
+        g.tri_solid = a["tri_solid"].ctypes.data_as(C.POINTER(C.c_int32))
+        g.n_solid = len(a["solid_id"])
+        g.solid_id = a["solid_id"].ctypes.data_as(C.POINTER(C.c_uint32))
+        g.solid_mat = a["solid_mat"].ctypes.data_as(C.POINTER(C.c_int32))
+        g.ign_off = a["ign_off"].ctypes.data_as(C.POINTER(C.c_int32))
+        g.ign = _dp(a["ign"])
+        g.default_mat = a["default_mat"]
+        g.n_mat = len(a["mats"])
+        g.mats = _dp(a["mats"])
+        self.geom = g
+        n = max(1, len(fields.containers))
+        self.fields = (Field * n)()
+        self.n_fields = len(fields.containers)
+        for k, c in enumerate(fields.containers):
+            f = self.fields[k]
+            f.type, f.bscale, f.escale, f.bw = c.type, c.bscale, c.escale, c.bw
+            for i in range(6):
+                f.box[i] = c.box[i]
+            for i in range(3):
+                f.e[i] = c.e[i]
+            if c.grid is not None:
+                xg, yg, zg = (np.ascontiguousarray(g_, dtype=np.float64) for g_ in c.grid)
+                coef = np.ascontiguousarray(c.coef, dtype=np.float64)
+                self._keep.extend([xg, yg, zg, coef])
+                f.nx, f.ny, f.nz = len(xg), len(yg), len(zg)
+                f.xg, f.yg, f.zg = _dp(xg), _dp(yg), _dp(zg)
+                f.has_b, f.has_v = c.has_b, c.has_v
+                f.coef = _dp(coef)
+
+
+def track(workload, mode="fp64", hit_cap=0, threads=1, sel=None, problem=None):
+    """Run the oracle on a workload (optionally a subset `sel` of particle indices)."""
+    L = load(mode)
+    P = problem or Problem(workload.geometry, workload.fields)
+    idx = np.arange(workload.n) if sel is None else np.asarray(sel)
+    n = len(idx)
+    sp = (Species * len(workload.species))()
+    for k, s in enumerate(workload.species):
+        sp[k].kind, sp[k].tau, sp[k].tmax, sp[k].lmax, sp[k].flipspin, sp[k].Bmax = (
+            s.kind, s.tau, s.tmax, s.lmax, s.flipspin, s.Bmax)
+    si = np.ascontiguousarray(workload.species_idx[idx], dtype=np.int32)
+    t0 = np.ascontiguousarray(workload.t0[idx])
+    y0 = np.ascontiguousarray(workload.y0[idx])
+    s0 = np.ascontiguousarray(workload.spin0[idx])
+    pid = np.ascontiguousarray(workload.pid[idx], dtype=np.uint64)
+    out = np.zeros(n, dtype=OUT_DTYPE)
+    hits = np.zeros(n * hit_cap, dtype=HIT_DTYPE) if hit_cap else None
+    rc = L.ptref_track(C.byref(P.geom), P.n_fields, P.fields, sp, si.ctypes.data_as(C.POINTER(C.c_int32)),
+                       float(workload.tmax), n, _dp(t0), _dp(y0), _dp(s0), int(workload.seed),
+                       pid.ctypes.data_as(C.POINTER(C.c_uint64)), out.ctypes.data_as(C.c_void_p),
+                       hits.ctypes.data_as(C.c_void_p) if hits is not None else None, hit_cap, threads)
+    if rc != 0:
+        raise RuntimeError(f"ptref_track failed ({rc})")
+    return out, (hits.reshape(n, hit_cap) if hits is not None else None)

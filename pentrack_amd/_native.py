@@ -1,0 +1,157 @@
+"""ctypes binding of libpentrack_gpu.so (the C ABI in include/pentrack_gpu.h).
+
+The library is built in-tree by pentrack_amd.build (hipcc, gfx950).  Loading
+fails loudly when it is missing: there is no CPU fallback for the tracking path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpentrack_gpu.so")
+
+PT_OK = 0
+PT_FIELD_TRICUBIC = 1
+PT_FIELD_UNIFORM_E = 2
+PT_VARIANT_FP64 = 0
+PT_VARIANT_MIXED = 1
+
+# stop IDs, include/globals.h:13-23 (+ extensions)
+ID_UNKNOWN = 0
+ID_NOT_FINISH = -1
+ID_HIT_BOUNDARIES = -2
+ID_ODEINT_ERROR = -3
+ID_DECAYED = -4
+ID_CGAL_ERROR = -6
+ID_GEOMETRY_ERROR = -7
+ID_INVARIANT = -8
+ID_UNSUPPORTED = -9
+ID_ABSORBED_IN_MATERIAL = 1
+ID_ABSORBED_ON_SURFACE = 2
+
+
+class FieldDesc(C.Structure):
+    _fields_ = [
+        ("type", C.c_int),
+        ("bscale", C.c_double), ("escale", C.c_double),
+        ("box", C.c_double * 6), ("bw", C.c_double),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("xg", C.POINTER(C.c_double)), ("yg", C.POINTER(C.c_double)), ("zg", C.POINTER(C.c_double)),
+        ("has_b", C.c_int), ("has_v", C.c_int),
+        ("coef", C.POINTER(C.c_double)),
+        ("e", C.c_double * 3),
+    ]
+
+
+class Species(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("q", C.c_double), ("m", C.c_double), ("mu", C.c_double), ("gamma", C.c_double),
+        ("tau", C.c_double), ("tmax", C.c_double), ("lmax", C.c_double),
+        ("flipspin", C.c_int), ("Bmax", C.c_double),
+    ]
+
+
+class ParticleOut(C.Structure):
+    _fields_ = [
+        ("t", C.c_double), ("y", C.c_double * 9), ("spin", C.c_double * 5),
+        ("stopID", C.c_int32), ("Nhit", C.c_int32), ("Nstep", C.c_int32), ("Nspinflip", C.c_int32),
+        ("n_acc", C.c_int64), ("n_rej", C.c_int64),
+        ("Hmax", C.c_double), ("noflipprob", C.c_double), ("tau", C.c_double),
+        ("solidend", C.c_int32), ("n_hits", C.c_int32),
+    ]
+
+
+class HitRec(C.Structure):
+    _fields_ = [
+        ("t", C.c_double), ("tri", C.c_int32), ("leaving", C.c_int32), ("entering", C.c_int32),
+        ("normal", C.c_double * 3), ("v1", C.c_double * 3), ("v2", C.c_double * 3),
+        ("pol1", C.c_double), ("pol2", C.c_double),
+    ]
+
+
+class LaunchCfg(C.Structure):
+    _fields_ = [
+        ("variant", C.c_int), ("block", C.c_int), ("steps_per_launch", C.c_int),
+        ("persistent", C.c_int), ("grid", C.c_int),
+    ]
+
+
+# numpy views of the output records
+OUT_DTYPE = np.dtype([
+    ("t", "<f8"), ("y", "<f8", (9,)), ("spin", "<f8", (5,)),
+    ("stopID", "<i4"), ("Nhit", "<i4"), ("Nstep", "<i4"), ("Nspinflip", "<i4"),
+    ("n_acc", "<i8"), ("n_rej", "<i8"),
+    ("Hmax", "<f8"), ("noflipprob", "<f8"), ("tau", "<f8"),
+    ("solidend", "<i4"), ("n_hits", "<i4"),
+], align=True)
+HIT_DTYPE = np.dtype([
+    ("t", "<f8"), ("tri", "<i4"), ("leaving", "<i4"), ("entering", "<i4"),
+    ("normal", "<f8", (3,)), ("v1", "<f8", (3,)), ("v2", "<f8", (3,)),
+    ("pol1", "<f8"), ("pol2", "<f8"),
+], align=True)
+assert OUT_DTYPE.itemsize == C.sizeof(ParticleOut), (OUT_DTYPE.itemsize, C.sizeof(ParticleOut))
+assert HIT_DTYPE.itemsize == C.sizeof(HitRec), (HIT_DTYPE.itemsize, C.sizeof(HitRec))
+
+# every symbol include/pentrack_gpu.h declares (checked by the test suite)
+EXPORTS = [
+    "pt_geometry_create", "pt_geometry_destroy", "pt_geometry_info",
+    "pt_fields_create", "pt_fields_destroy", "pt_tricubic_precompute",
+    "pt_batch_create", "pt_batch_destroy", "pt_batch_run", "pt_batch_active",
+    "pt_batch_download", "pt_batch_step_counts", "pt_track_batch",
+    "pt_request_quit", "pt_last_error",
+]
+
+_lib = None
+
+
+def dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def lib():
+    """Load the library once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(hipcc, gfx950). The tracking path has no CPU fallback.")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    ip = C.POINTER(C.c_int32)
+    L.pt_geometry_create.argtypes = [C.c_int, C.POINTER(C.c_float), ip, C.c_int, C.POINTER(C.c_uint32), ip, ip,
+                                     C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int,
+                                     C.POINTER(vp)]
+    L.pt_geometry_destroy.argtypes = [vp]
+    L.pt_geometry_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.pt_fields_create.argtypes = [C.c_int, C.POINTER(FieldDesc), C.c_int, C.POINTER(vp)]
+    L.pt_fields_destroy.argtypes = [vp]
+    L.pt_tricubic_precompute.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), C.c_int]
+    L.pt_tricubic_matrix.argtypes = [ip]
+    L.pt_batch_create.argtypes = [vp, vp, C.c_int, C.POINTER(Species), C.c_double, C.c_int, ip,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.c_uint64, C.POINTER(C.c_uint64), C.c_int, C.POINTER(vp)]
+    L.pt_batch_destroy.argtypes = [vp]
+    L.pt_batch_run.argtypes = [vp, vp, C.POINTER(LaunchCfg)]
+    L.pt_batch_active.argtypes = [vp, vp, C.POINTER(C.c_int)]
+    L.pt_batch_download.argtypes = [vp, vp, vp, vp]
+    L.pt_batch_step_counts.argtypes = [vp, vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.pt_track_batch.argtypes = [vp, vp, vp, C.c_int, C.POINTER(Species), C.c_double, C.c_int, ip,
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.c_uint64, C.POINTER(C.c_uint64), vp, vp, C.c_int, C.POINTER(LaunchCfg)]
+    L.pt_request_quit.argtypes = [C.c_int]
+    L.pt_request_quit.restype = None
+    L.pt_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != PT_OK:
+        msg = lib().pt_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed with status {rc}: {msg}")

@@ -1,0 +1,493 @@
+/*
+ * pt_device.h -- device-side physics of the MI355X tracking engine.
+ *
+ * One GPU thread advances one particle through TTracker::IntegrateParticle's
+ * loop (src/tracking.cpp:71-148).  The reference's recursion (CheckHit,
+ * iterate_collision) becomes loops with bounded state; its CGAL AABB tree
+ * becomes a flattened BVH walked with a short per-thread stack; its
+ * Boost.Odeint dense-output stepper keeps its stage vectors in registers.
+ * Floating-point expressions keep the reference's evaluation order and the
+ * build disables FP contraction, so the FP64 path reproduces the reference
+ * arithmetic (with long double replaced by double) bit for bit.
+ */
+#ifndef PT_DEVICE_H_
+#define PT_DEVICE_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "pt_detmath.h"
+
+#define PTD __device__ __forceinline__
+#define PTD_NOINLINE __device__ __noinline__
+#define PTHD __host__ __device__ __forceinline__
+
+namespace ptd {
+
+/* ---- constants (src/globals.cpp:13-32, double-rounded) ------------------- */
+constexpr double ELE_E = 1.602176487E-19;
+constexpr double GRAV = 9.80665;
+constexpr double C_0 = 299792458.;
+constexpr double PI = 3.1415926535897932384626;
+constexpr double HBAR = 1.05457266e-34;
+constexpr double MAX_TRACK_DEVIATION = 0.001;   /* include/particle.h:20 */
+constexpr double REFLECT_TOLERANCE = 1e-8;      /* include/trianglemesh.h:39 */
+constexpr double DBL_EPS = 2.220446049250313e-16;
+
+enum {
+    ID_UNKNOWN = 0, ID_NOT_FINISH = -1, ID_HIT_BOUNDARIES = -2, ID_ODEINT_ERROR = -3,
+    ID_DECAYED = -4, ID_CGAL_ERROR = -6, ID_GEOMETRY_ERROR = -7,
+    ID_ABSORBED_IN_MATERIAL = 1, ID_ABSORBED_ON_SURFACE = 2,
+    ID_INVARIANT = -8, ID_UNSUPPORTED = -9
+};
+
+constexpr int MAX_HITS = 16;     /* intersections kept per segment query */
+constexpr int MAX_SOLIDS = 8;    /* entries of the per-particle solid stack */
+constexpr int BVH_STACK = 48;
+
+/* ---- device-side data ---------------------------------------------------- */
+
+struct BvhNode {          /* 32 B: two 16-B loads */
+    float lo[3]; int a;   /* internal: a = left child; leaf: a = -(first + 1) */
+    float hi[3]; int b;   /* internal: b = right child; leaf: b = count */
+};
+
+struct Tri {              /* 48 B: three 16-B loads, vertices exact in float */
+    float v[9];
+    int solid;            /* solid index */
+    int orig;             /* triangle index in input order */
+    int pad;
+};
+
+struct Geom {
+    int n_tri, n_node, n_solid, default_mat;
+    const Tri *tri;           /* BVH leaf order */
+    const double *nrm;        /* [n_tri][3] face normals, BVH leaf order */
+    const BvhNode *node;
+    const uint32_t *solid_id;
+    const int *solid_mat;
+    const int *ign_off;
+    const double *ign;
+    const double *mats;       /* [n_mat][9] */
+    const double *mesh_bbox;  /* [n_solid][6] xmin ymin zmin xmax ymax zmax */
+};
+
+struct Field {
+    int type;                 /* 1 tricubic, 2 uniform E */
+    int has_b, has_v, ncomp;
+    int nx, ny, nz;
+    double bscale, escale, bw;
+    double box[6];
+    double e[3];
+    double x0[3], inv_h[3];   /* first grid coordinate and 1/spacing guess for the cell search */
+    const double *g[3];       /* grid coordinates */
+    const double *coef;       /* FP64 table */
+    const float *coef32;      /* FP32 table (mixed precision) */
+};
+
+struct Species {
+    int kind;
+    double q, m, mu, gamma, tau, tmax, lmax, Bmax;
+    int flipspin;
+};
+
+struct Hit {
+    double t;
+    int tri, leaving, entering;
+    double normal[3], v1[3], v2[3], pol1, pol2;
+};
+
+/* resumable per-particle state, structure of arrays */
+struct State {
+    int n;
+    double *t, *y, *xc, *dxc, *ts, *dt, *spin, *Hmax, *noflip, *tau;  /* y,xc,dxc: [9][n]; spin: [5][n] */
+    int *stop, *flags, *Nhit, *Nstep, *Nspinflip, *nsolid, *nhitlog, *species;
+    long long *n_acc, *n_rej;
+    unsigned long long *ctr, *key, *solids;
+    Hit *hits;
+    int hit_cap;
+};
+
+struct Params {
+    Geom geom;
+    int n_fields;
+    const Field *fields;
+    const Species *species;
+    double tmax_global;
+    State st;
+    int max_steps;            /* accepted steps per particle per launch; <= 0: unlimited */
+    const int *quit;
+};
+
+/* ---- small helpers ------------------------------------------------------- */
+
+PTD double dabs(double x) { return __builtin_fabs(x); }
+
+/* ---- CGAL Simple_cartesian<double> predicates ---------------------------- */
+
+PTD int orient3(const double *p, const double *q, const double *r, const double *s) {
+    double a00 = q[0] - p[0], a01 = r[0] - p[0], a02 = s[0] - p[0];
+    double a10 = q[1] - p[1], a11 = r[1] - p[1], a12 = s[1] - p[1];
+    double a20 = q[2] - p[2], a21 = r[2] - p[2], a22 = s[2] - p[2];
+    double m01 = a00 * a11 - a10 * a01;
+    double m02 = a00 * a21 - a20 * a01;
+    double m12 = a10 * a21 - a20 * a11;
+    double det = m01 * a22 - m02 * a12 + m12 * a02;
+    return (det > 0) - (det < 0);
+}
+
+PTD int orient2(double px, double py, double qx, double qy, double rx, double ry) {
+    double d = (qx - px) * (ry - py) - (rx - px) * (qy - py);
+    return (d > 0) - (d < 0);
+}
+PTD int cop_orient(const double *p, const double *q, const double *r) {
+    int o = orient2(p[0], p[1], q[0], q[1], r[0], r[1]);
+    if (o != 0) return o;
+    o = orient2(p[1], p[2], q[1], q[2], r[1], r[2]);
+    if (o != 0) return o;
+    return orient2(p[0], p[2], q[0], q[2], r[0], r[2]);
+}
+/* CGAL do_intersect_coplanar(Triangle_3, Segment_3) */
+PTD_NOINLINE int coplanar_seg_tri(const double *A, const double *B, const double *C, const double *p, const double *q) {
+    const double *a = A, *b = B, *c = C;
+    if (cop_orient(A, B, C) <= 0) { b = C; c = B; }
+    int pqa = cop_orient(p, q, a), pqb = cop_orient(p, q, b), pqc = cop_orient(p, q, c);
+    if (pqa > 0) {
+        if (pqb > 0) {
+            if (pqc > 0) return 0;
+            return cop_orient(b, c, q) >= 0 && cop_orient(c, a, p) >= 0;
+        }
+        if (pqc > 0) return cop_orient(a, b, q) >= 0 && cop_orient(b, c, p) >= 0;
+        return cop_orient(a, b, q) >= 0 && cop_orient(c, a, p) >= 0;
+    } else if (pqa < 0) {
+        if (pqb > 0) {
+            if (pqc > 0) return cop_orient(a, b, p) >= 0 && cop_orient(c, a, q) >= 0;
+            return cop_orient(a, b, p) >= 0 && cop_orient(b, c, q) >= 0;
+        }
+        if (pqb < 0) {
+            if (pqc < 0) return 0;
+            return cop_orient(b, c, p) >= 0 && cop_orient(c, a, q) >= 0;
+        }
+        if (pqc < 0) return cop_orient(a, b, p) >= 0 && cop_orient(b, c, q) >= 0;
+        return cop_orient(a, b, p) >= 0 && cop_orient(c, a, q) >= 0;
+    } else {
+        if (pqb > 0) {
+            if (pqc > 0) return cop_orient(a, b, p) >= 0 && cop_orient(c, a, q) >= 0;
+            return cop_orient(a, b, p) >= 0 && cop_orient(b, c, q) >= 0;
+        }
+        if (pqb < 0) {
+            if (pqc < 0) return cop_orient(a, b, q) >= 0 && cop_orient(c, a, p) >= 0;
+            return cop_orient(a, b, q) >= 0 && cop_orient(b, c, p) >= 0;
+        }
+        if (pqc > 0) return cop_orient(b, c, p) >= 0 && cop_orient(c, a, q) >= 0;
+        return cop_orient(b, c, q) >= 0 && cop_orient(c, a, p) >= 0;
+    }
+}
+
+/* CGAL intersection(Triangle_3, Segment_3): 1 hit (point in pt), 0 none, -1 coplanar overlap */
+PTD int seg_tri(const double *a, const double *b, const double *c, const double *p, const double *q, double pt[3]) {
+    int abcp = orient3(a, b, c, p);
+    int abcq = orient3(a, b, c, q);
+    const double *u, *w;
+    if (abcp > 0) {
+        if (abcq > 0) return 0;
+        if (abcq == 0) {
+            if (orient3(p, q, a, b) <= 0 && orient3(p, q, b, c) <= 0 && orient3(p, q, c, a) <= 0) { pt[0] = q[0]; pt[1] = q[1]; pt[2] = q[2]; return 1; }
+            return 0;
+        }
+        u = p; w = q;
+    } else if (abcp < 0) {
+        if (abcq < 0) return 0;
+        if (abcq == 0) {
+            if (orient3(q, p, a, b) <= 0 && orient3(q, p, b, c) <= 0 && orient3(q, p, c, a) <= 0) { pt[0] = q[0]; pt[1] = q[1]; pt[2] = q[2]; return 1; }
+            return 0;
+        }
+        u = q; w = p;
+    } else {
+        if (abcq > 0) {
+            if (orient3(q, p, a, b) <= 0 && orient3(q, p, b, c) <= 0 && orient3(q, p, c, a) <= 0) { pt[0] = p[0]; pt[1] = p[1]; pt[2] = p[2]; return 1; }
+            return 0;
+        }
+        if (abcq < 0) {
+            if (orient3(p, q, a, b) <= 0 && orient3(p, q, b, c) <= 0 && orient3(p, q, c, a) <= 0) { pt[0] = p[0]; pt[1] = p[1]; pt[2] = p[2]; return 1; }
+            return 0;
+        }
+        return coplanar_seg_tri(a, b, c, p, q) ? -1 : 0;
+    }
+    if (orient3(u, w, a, b) > 0 || orient3(u, w, b, c) > 0 || orient3(u, w, c, a) > 0) return 0;
+    double rpx = a[0] - c[0], rpy = a[1] - c[1], rpz = a[2] - c[2];
+    double rqx = b[0] - c[0], rqy = b[1] - c[1], rqz = b[2] - c[2];
+    double pa = rpy * rqz - rqy * rpz;
+    double pb = rpz * rqx - rqz * rpx;
+    double pc = rpx * rqy - rqx * rpy;
+    double pd = -pa * c[0] - pb * c[1] - pc * c[2];
+    double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    double num = pa * p[0] + pb * p[1] + pc * p[2] + pd;
+    double den = pa * dx + pb * dy + pc * dz;
+    if (den == 0) return 0;
+    double hx = den * p[0] - num * dx, hy = den * p[1] - num * dy, hz = den * p[2] - num * dz;
+    if (den != 1.0) { pt[0] = hx / den; pt[1] = hy / den; pt[2] = hz / den; }
+    else { pt[0] = hx; pt[1] = hy; pt[2] = hz; }
+    return 1;
+}
+
+/* CGAL do_intersect(Triangle_3, Ray_3) for the +z ray of TTriangleMesh::GetSolids */
+PTD int ray_tri(const double *a, const double *b, const double *c, const double *p) {
+    double q[3] = {p[0], p[1], p[2] + 1.0};
+    double ad[3] = {a[0], a[1], a[2] + 1.0};
+    int dir = orient3(a, b, c, ad);
+    if (dir == 0) return 0;
+    int abcp = orient3(a, b, c, p);
+    if (abcp > 0) {
+        if (dir > 0) return 0;
+        return !(orient3(p, q, a, b) > 0 || orient3(p, q, b, c) > 0 || orient3(p, q, c, a) > 0);
+    } else if (abcp < 0) {
+        if (dir < 0) return 0;
+        return !(orient3(q, p, a, b) > 0 || orient3(q, p, b, c) > 0 || orient3(q, p, c, a) > 0);
+    }
+    if (dir > 0) return orient3(q, p, a, b) <= 0 && orient3(q, p, b, c) <= 0 && orient3(q, p, c, a) <= 0;
+    return orient3(p, q, a, b) <= 0 && orient3(p, q, b, c) <= 0 && orient3(p, q, c, a) <= 0;
+}
+
+/* CGAL do_intersect(Segment_3, Bbox_3) (Bbox_3_Segment_3_do_intersect.h, bounded, unfiltered) */
+PTD int seg_bbox(const double *P, const double *Q, const double *bb) {
+    double px = P[0], py = P[1], pz = P[2], qx = Q[0], qy = Q[1], qz = Q[2];
+    double bxmin = bb[0], bymin = bb[1], bzmin = bb[2], bxmax = bb[3], bymax = bb[4], bzmax = bb[5];
+    if ((px >= bxmin && px <= bxmax && py >= bymin && py <= bymax && pz >= bzmin && pz <= bzmax) ||
+        (qx >= bxmin && qx <= bxmax && qy >= bymin && qy <= bymax && qz >= bzmin && qz <= bzmax))
+        return 1;
+    double dmin, tmin, tmax, dmax;
+    if (qx >= px) {
+        if (px > bxmax || qx < bxmin) return 0;
+        if (bxmax > qx) { tmax = 1; dmax = 1; } else { tmax = bxmax - px; dmax = qx - px; }
+        tmin = bxmin - px; dmin = qx - px;
+    } else {
+        if (qx > bxmax || px < bxmin) return 0;
+        if (bxmin < qx) { tmax = 1; dmax = 1; } else { tmax = px - bxmin; dmax = px - qx; }
+        tmin = px - bxmax; dmin = px - qx;
+    }
+    if (tmin < 0) tmin = 0;
+    double dymin, tymin, tymax, dymax;
+    if (qy >= py) {
+        if (py > bymax || qy < bymin) return 0;
+        if (bymax > qy) { tymax = 1; dymax = 1; } else { tymax = bymax - py; dymax = qy - py; }
+        tymin = bymin - py; dymin = qy - py;
+    } else {
+        if (qy > bymax || py < bymin) return 0;
+        if (bymin < qy) { tymax = 1; dymax = 1; } else { tymax = py - bymin; dymax = py - qy; }
+        tymin = py - bymax; dymin = py - qy;
+    }
+    if (tymin < 0) tymin = 0;
+    if (py != qy && px != qx) {
+        if (dymax * tmin > dmin * tymax) return 0;
+        if (dmax * tymin > dymin * tmax) return 0;
+    }
+    if (px == qx || (py != qy && dmin * tymin > dymin * tmin)) { tmin = tymin; dmin = dymin; }
+    if (px == qx || (py != qy && dymax * tmax > dmax * tymax)) { tmax = tymax; dmax = dymax; }
+    double dzmin, tzmin, tzmax, dzmax;
+    if (qz >= pz) {
+        if (pz > bzmax || qz < bzmin) return 0;
+        if (bzmax > qz) { tzmax = 1; dzmax = 1; } else { tzmax = bzmax - pz; dzmax = qz - pz; }
+        tzmin = bzmin - pz; dzmin = qz - pz;
+    } else {
+        if (qz > bzmax || pz < bzmin) return 0;
+        if (bzmin < qz) { tzmax = 1; dzmax = 1; } else { tzmax = pz - bzmin; dzmax = pz - qz; }
+        tzmin = pz - bzmax; dzmin = pz - qz;
+    }
+    if (tzmin < 0) tzmin = 0;
+    if ((px != qx || py != qy) && pz != qz) {
+        if (dzmax * tmin > dmin * tzmax) return 0;
+        if (dmax * tzmin > dzmin * tmax) return 0;
+    }
+    return 1;
+}
+
+/* conservative segment-vs-node test for BVH culling: the box is widened so that
+   rounding never rejects a node holding a triangle the exact test would hit */
+PTD bool node_hit(const BvhNode &nd, const double *p, const double *inv, const double *d) {
+    double t0 = 0.0, t1 = 1.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double lo = (double)nd.lo[k], hi = (double)nd.hi[k];
+        double pad = 1e-9 + 1e-7 * (dabs(lo) + dabs(hi));
+        lo -= pad; hi += pad;
+        if (d[k] == 0.0) {
+            if (p[k] < lo || p[k] > hi) return false;
+        } else {
+            double ta = (lo - p[k]) * inv[k], tb = (hi - p[k]) * inv[k];
+            if (ta > tb) { double s = ta; ta = tb; tb = s; }
+            t0 = ta > t0 ? ta : t0;
+            t1 = tb < t1 ? tb : t1;
+            if (t0 > t1) return false;
+        }
+    }
+    return true;
+}
+
+PTD void load_tri(const Tri *T, double a[3], double b[3], double c[3], int &solid, int &orig) {
+    const float4 *q = reinterpret_cast<const float4 *>(T);
+    float4 w0 = q[0], w1 = q[1], w2 = q[2];
+    a[0] = w0.x; a[1] = w0.y; a[2] = w0.z;
+    b[0] = w0.w; b[1] = w1.x; b[2] = w1.y;
+    c[0] = w1.z; c[1] = w1.w; c[2] = w2.x;
+    solid = __float_as_int(w2.y);
+    orig = __float_as_int(w2.z);
+}
+
+PTD void load_node(const BvhNode *N, BvhNode &nd) {
+    const float4 *q = reinterpret_cast<const float4 *>(N);
+    float4 w0 = q[0], w1 = q[1];
+    nd.lo[0] = w0.x; nd.lo[1] = w0.y; nd.lo[2] = w0.z; nd.a = __float_as_int(w0.w);
+    nd.hi[0] = w1.x; nd.hi[1] = w1.y; nd.hi[2] = w1.z; nd.b = __float_as_int(w1.w);
+}
+
+/* sorted intersection list of one segment: (s asc, solid ID desc, input index asc) */
+struct HitList {
+    int n;
+    double s[MAX_HITS];
+    int tri[MAX_HITS];     /* leaf-order index */
+    int orig[MAX_HITS];
+    unsigned id[MAX_HITS];
+};
+
+/*
+ * TGeometry::GetCollisions (src/geometry.cpp:139-148) over the BVH.
+ * list == nullptr: count only.  Returns the number of intersections, or
+ * -1 (coplanar overlap: the reference throws) / -2 (more than MAX_HITS).
+ */
+PTD_NOINLINE int seg_query(const Geom &G, const double *p, const double *q, HitList *list, long long *tests) {
+    double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
+    double inv[3] = {1.0 / d[0], 1.0 / d[1], 1.0 / d[2]};
+    double sl = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    int stack[BVH_STACK];
+    int sp = 0;
+    int node = 0;
+    int count = 0;
+    if (G.n_node == 0) return 0;
+    for (;;) {
+        BvhNode nd;
+        load_node(G.node + node, nd);
+        if (node_hit(nd, p, inv, d)) {
+            if (nd.a < 0) {
+                int first = -nd.a - 1;
+                for (int k = 0; k < nd.b; ++k) {
+                    double a[3], b[3], c[3], pt[3];
+                    int solid, orig;
+                    load_tri(G.tri + first + k, a, b, c, solid, orig);
+                    ++*tests;
+                    int r = seg_tri(a, b, c, p, q, pt);
+                    if (r == 0) continue;
+                    if (r < 0) return -1;
+                    if (count >= MAX_HITS) return -2;
+                    ++count;
+                    if (list) {
+                        double v[3] = {pt[0] - p[0], pt[1] - p[1], pt[2] - p[2]};
+                        double s = (v[0] * d[0] + v[1] * d[1] + v[2] * d[2]) / sl;
+                        unsigned id = G.solid_id[solid];
+                        int j = list->n;
+                        while (j > 0) {
+                            double so = list->s[j - 1];
+                            unsigned io = list->id[j - 1];
+                            bool before = (s == so) ? (id > io || (id == io && orig < list->orig[j - 1])) : (s < so);
+                            if (!before) break;
+                            list->s[j] = so; list->id[j] = io; list->tri[j] = list->tri[j - 1]; list->orig[j] = list->orig[j - 1];
+                            --j;
+                        }
+                        list->s[j] = s; list->id[j] = id; list->tri[j] = first + k; list->orig[j] = orig;
+                        list->n++;
+                    }
+                }
+            } else {
+                if (sp < BVH_STACK) stack[sp++] = nd.b;
+                else return -2;
+                node = nd.a;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+    return count;
+}
+
+/* +z ray parity per solid (TTriangleMesh::GetSolids, include/trianglemesh.h:181-188):
+   bit m of the result is set if the ray from p crosses solid m an odd number of times */
+PTD_NOINLINE unsigned long long ray_parity(const Geom &G, const double *p) {
+    unsigned long long par = 0;
+    int stack[BVH_STACK];
+    int sp = 0, node = 0;
+    if (G.n_node == 0) return 0;
+    for (;;) {
+        BvhNode nd;
+        load_node(G.node + node, nd);
+        double pad0 = 1e-9 + 1e-7 * (dabs(nd.lo[0]) + dabs(nd.hi[0]));
+        double pad1 = 1e-9 + 1e-7 * (dabs(nd.lo[1]) + dabs(nd.hi[1]));
+        double pad2 = 1e-9 + 1e-7 * (dabs(nd.lo[2]) + dabs(nd.hi[2]));
+        bool hit = p[0] >= (double)nd.lo[0] - pad0 && p[0] <= (double)nd.hi[0] + pad0 &&
+                   p[1] >= (double)nd.lo[1] - pad1 && p[1] <= (double)nd.hi[1] + pad1 &&
+                   p[2] <= (double)nd.hi[2] + pad2;
+        if (hit) {
+            if (nd.a < 0) {
+                int first = -nd.a - 1;
+                for (int k = 0; k < nd.b; ++k) {
+                    double a[3], b[3], c[3];
+                    int solid, orig;
+                    load_tri(G.tri + first + k, a, b, c, solid, orig);
+                    if (ray_tri(a, b, c, p)) par ^= (1ull << solid);
+                }
+            } else {
+                if (sp < BVH_STACK) stack[sp++] = nd.b;
+                node = nd.a;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+    return par;
+}
+
+/* ---- solid stack: up to MAX_SOLIDS entries packed in one 64-bit word -------
+   entry = (solid index + 1) | ignored << 7; index 0 stands for the default solid */
+PTHD int sk_sidx(unsigned long long s, int i) { return (int)((s >> (8 * i)) & 0x7f) - 1; }
+PTHD int sk_ign(unsigned long long s, int i) { return (int)((s >> (8 * i + 7)) & 1); }
+PTHD unsigned long long sk_set(unsigned long long s, int i, int sidx, int ign) {
+    unsigned long long e = (unsigned long long)((sidx + 1) | (ign << 7));
+    return (s & ~(0xffull << (8 * i))) | (e << (8 * i));
+}
+PTHD unsigned long long sk_erase(unsigned long long s, int i) {
+    unsigned long long low = (i == 0) ? 0ull : (s & ((1ull << (8 * i)) - 1));
+    unsigned long long high = (i >= 7) ? 0ull : ((s >> (8 * (i + 1))) << (8 * i));
+    return low | high;
+}
+
+PTD unsigned solid_ID(const Geom &G, int sidx) { return sidx < 0 ? 1u : G.solid_id[sidx]; }
+PTD const double *solid_mat(const Geom &G, int sidx) {
+    int m = sidx < 0 ? G.default_mat : G.solid_mat[sidx];
+    return G.mats + 9 * m;
+}
+PTD int solid_ignored(const Geom &G, int sidx, double t) {
+    if (sidx < 0) return 0;
+    for (int k = G.ign_off[sidx]; k < G.ign_off[sidx + 1]; ++k)
+        if (t >= G.ign[2 * k] && t < G.ign[2 * k + 1]) return 1;
+    return 0;
+}
+/* TTracker::GetCurrentsolid (std::max_element with the reference comparator) */
+PTD int current_solid(const Geom &G, unsigned long long s, int n) {
+    int best = 0;
+    for (int i = 1; i < n; ++i) {
+        int ai = sk_ign(s, best), bi = sk_ign(s, i);
+        if (ai || (!bi && solid_ID(G, sk_sidx(s, best)) < solid_ID(G, sk_sidx(s, i)))) best = i;
+    }
+    return sk_sidx(s, best);
+}
+
+/* TGeometry::CheckSegment -> TTriangleMesh::InBoundingBox */
+PTD bool check_segment(const Geom &G, const double *p1, const double *p2) {
+    for (int m = 0; m < G.n_solid; ++m)
+        if (seg_bbox(p1, p2, G.mesh_bbox + 6 * m)) return true;
+    return false;
+}
+
+} // namespace ptd
+
+#endif /* PT_DEVICE_H_ */

@@ -1,0 +1,733 @@
+/*
+ * pt_host.cpp -- host side of libpentrack_gpu.so: world-model builders (BVH,
+ * field tables, tricubic pre-interpolation), device memory of a particle batch,
+ * and the C ABI declared in include/pentrack_gpu.h.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/pentrack_gpu.h"
+#include "pt_device.h"
+#include "pt_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+std::atomic<int> g_quit{0};
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define PT_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) return fail(PT_EDEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+/* device allocation list owned by a handle */
+struct DevMem {
+    std::vector<void *> ptrs;
+    ~DevMem() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    int alloc(T **out, size_t count) {
+        void *p = nullptr;
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return fail(PT_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        ptrs.push_back(p);
+        *out = static_cast<T *>(p);
+        return PT_OK;
+    }
+    template <class T>
+    int upload(T **out, const T *src, size_t count) {
+        int r = alloc(out, count);
+        if (r) return r;
+        if (count) {
+            hipError_t e = hipMemcpy(*out, src, count * sizeof(T), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return fail(PT_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+        }
+        return PT_OK;
+    }
+};
+
+} // namespace
+
+/* ======================================================================== */
+/* geometry                                                                  */
+/* ======================================================================== */
+
+struct pt_geometry {
+    DevMem mem;
+    ptd::Geom g{};
+    int n_nodes = 0, depth = 0;
+    int n_solid = 0;
+};
+
+namespace {
+
+struct BuildPrim {
+    float lo[3], hi[3];
+    float c[3];
+    int idx;
+};
+
+/* median split on the longest axis of the centroid box (the AABB_traits split of
+   the reference's CGAL tree, AABB_traits.h:296-328, applied to centroids) */
+int build_bvh(std::vector<BuildPrim> &prims, int begin, int end, int leaf_size, std::vector<ptd::BvhNode> &nodes,
+              std::vector<int> &order, int depth, int &max_depth) {
+    int me = (int)nodes.size();
+    nodes.emplace_back();
+    max_depth = std::max(max_depth, depth);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = begin; i < end; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], prims[i].lo[a]);
+            hi[a] = std::max(hi[a], prims[i].hi[a]);
+            clo[a] = std::min(clo[a], prims[i].c[a]);
+            chi[a] = std::max(chi[a], prims[i].c[a]);
+        }
+    ptd::BvhNode nd;
+    for (int a = 0; a < 3; ++a) { nd.lo[a] = lo[a]; nd.hi[a] = hi[a]; }
+    if (end - begin <= leaf_size) {
+        nd.a = -((int)order.size() + 1);
+        nd.b = end - begin;
+        for (int i = begin; i < end; ++i) order.push_back(prims[i].idx);
+        nodes[me] = nd;
+        return me;
+    }
+    int axis = 0;
+    float ext = chi[0] - clo[0];
+    for (int a = 1; a < 3; ++a)
+        if (chi[a] - clo[a] > ext) { ext = chi[a] - clo[a]; axis = a; }
+    int mid = begin + (end - begin) / 2;
+    std::nth_element(prims.begin() + begin, prims.begin() + mid, prims.begin() + end,
+                     [axis](const BuildPrim &p, const BuildPrim &q) {
+                         if (p.c[axis] != q.c[axis]) return p.c[axis] < q.c[axis];
+                         return p.idx < q.idx;
+                     });
+    int left = build_bvh(prims, begin, mid, leaf_size, nodes, order, depth + 1, max_depth);
+    int right = build_bvh(prims, mid, end, leaf_size, nodes, order, depth + 1, max_depth);
+    nd.a = left;
+    nd.b = right;
+    nodes[me] = nd;
+    return me;
+}
+
+} // namespace
+
+extern "C" int pt_geometry_create(int n_tri, const float *tri_vertices, const int32_t *tri_solid, int n_solid,
+                                  const uint32_t *solid_id, const int32_t *solid_mat, const int32_t *ign_off,
+                                  const double *ign, int default_mat, int n_mat, const double *mats, int leaf_size,
+                                  pt_geometry **out) {
+    if (!out || n_tri < 0 || n_solid < 0 || n_solid > 64 || n_mat <= 0 || !mats) return fail(PT_EINVAL, "pt_geometry_create: bad sizes");
+    if (n_tri > 0 && (!tri_vertices || !tri_solid)) return fail(PT_EINVAL, "pt_geometry_create: null triangle arrays");
+    if (n_solid > 0 && (!solid_id || !solid_mat || !ign_off)) return fail(PT_EINVAL, "pt_geometry_create: null solid arrays");
+    if (default_mat < 0 || default_mat >= n_mat) return fail(PT_EINVAL, "pt_geometry_create: default material out of range");
+    if (leaf_size < 1 || leaf_size > 32) return fail(PT_EINVAL, "pt_geometry_create: leaf_size must be 1..32");
+    for (int s = 0; s < n_solid; ++s) {
+        if (solid_id[s] <= 1) return fail(PT_EINVAL, "pt_geometry_create: solid IDs must be > 1");
+        if (solid_mat[s] < 0 || solid_mat[s] >= n_mat) return fail(PT_EINVAL, "pt_geometry_create: solid material out of range");
+        for (int t = 0; t < s; ++t)
+            if (solid_id[t] == solid_id[s]) return fail(PT_EINVAL, "pt_geometry_create: duplicate solid ID");
+        if (ign_off[s + 1] < ign_off[s]) return fail(PT_EINVAL, "pt_geometry_create: bad ignore offsets");
+    }
+    for (int i = 0; i < n_tri; ++i)
+        if (tri_solid[i] < 0 || tri_solid[i] >= n_solid) return fail(PT_EINVAL, "pt_geometry_create: triangle solid index out of range");
+
+    std::unique_ptr<pt_geometry> G(new pt_geometry());
+    /* vertex snap |v| < 1e-8 -> 0 (src/trianglemesh.cpp:42) */
+    std::vector<float> v(tri_vertices, tri_vertices + (size_t)9 * n_tri);
+    for (float &c : v)
+        if (std::fabs(c) < 1e-8) c = 0.f;
+    std::vector<BuildPrim> prims(n_tri);
+    std::vector<double> bbox((size_t)6 * std::max(n_solid, 1));
+    for (int s = 0; s < n_solid; ++s) {
+        for (int a = 0; a < 3; ++a) { bbox[6 * s + a] = INFINITY; bbox[6 * s + 3 + a] = -INFINITY; }
+    }
+    std::vector<double> nrm((size_t)3 * n_tri);
+    for (int i = 0; i < n_tri; ++i) {
+        const float *T = &v[(size_t)9 * i];
+        BuildPrim &p = prims[i];
+        p.idx = i;
+        for (int a = 0; a < 3; ++a) {
+            p.lo[a] = std::min(T[a], std::min(T[3 + a], T[6 + a]));
+            p.hi[a] = std::max(T[a], std::max(T[3 + a], T[6 + a]));
+            p.c[a] = (p.lo[a] + p.hi[a]) * 0.5f;
+            double *b = &bbox[6 * tri_solid[i]];
+            b[a] = std::min(b[a], (double)p.lo[a]);
+            b[3 + a] = std::max(b[3 + a], (double)p.hi[a]);
+        }
+        /* CGAL compute_face_normal: 0.5*cross(v1-v0, v2-v0), normalised */
+        double v0[3] = {T[0], T[1], T[2]}, v1[3] = {T[3], T[4], T[5]}, v2[3] = {T[6], T[7], T[8]};
+        double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+        double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+        double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        n[0] = n[0] * 0.5; n[1] = n[1] * 0.5; n[2] = n[2] * 0.5;
+        double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (len != 0) { n[0] = n[0] / len; n[1] = n[1] / len; n[2] = n[2] / len; }
+        nrm[3 * i] = n[0]; nrm[3 * i + 1] = n[1]; nrm[3 * i + 2] = n[2];
+    }
+    std::vector<ptd::BvhNode> nodes;
+    std::vector<int> order;
+    int max_depth = 0;
+    if (n_tri > 0) build_bvh(prims, 0, n_tri, leaf_size, nodes, order, 0, max_depth);
+    if (max_depth + 2 >= ptd::BVH_STACK) return fail(PT_EINVAL, "pt_geometry_create: BVH too deep");
+    std::vector<ptd::Tri> tris(n_tri);
+    std::vector<double> nrm_ord((size_t)3 * n_tri);
+    for (int k = 0; k < n_tri; ++k) {
+        int i = order[k];
+        std::memcpy(tris[k].v, &v[(size_t)9 * i], 9 * sizeof(float));
+        tris[k].solid = tri_solid[i];
+        tris[k].orig = i;
+        tris[k].pad = 0;
+        for (int a = 0; a < 3; ++a) nrm_ord[3 * k + a] = nrm[3 * i + a];
+    }
+    ptd::Geom &g = G->g;
+    g.n_tri = n_tri;
+    g.n_node = (int)nodes.size();
+    g.n_solid = n_solid;
+    g.default_mat = default_mat;
+    int r;
+    ptd::Tri *dtri; double *dnrm; ptd::BvhNode *dnode; uint32_t *dsid; int *dsmat, *dign_off; double *dign, *dmats, *dbbox;
+    std::vector<int> ioff(n_solid + 1, 0);
+    for (int s = 0; s <= n_solid; ++s) ioff[s] = n_solid ? ign_off[s] - ign_off[0] : 0;
+    int n_ign = n_solid ? ign_off[n_solid] - ign_off[0] : 0;
+    std::vector<int> smat(solid_mat, solid_mat + n_solid);
+    if ((r = G->mem.upload(&dtri, tris.data(), tris.size()))) return r;
+    if ((r = G->mem.upload(&dnrm, nrm_ord.data(), nrm_ord.size()))) return r;
+    if ((r = G->mem.upload(&dnode, nodes.data(), nodes.size()))) return r;
+    if ((r = G->mem.upload(&dsid, solid_id, (size_t)n_solid))) return r;
+    if ((r = G->mem.upload(&dsmat, smat.data(), smat.size()))) return r;
+    if ((r = G->mem.upload(&dign_off, ioff.data(), ioff.size()))) return r;
+    if ((r = G->mem.upload(&dign, n_ign ? ign + 2 * ign_off[0] : nullptr, (size_t)2 * n_ign))) return r;
+    if ((r = G->mem.upload(&dmats, mats, (size_t)9 * n_mat))) return r;
+    if ((r = G->mem.upload(&dbbox, bbox.data(), bbox.size()))) return r;
+    g.tri = dtri; g.nrm = dnrm; g.node = dnode; g.solid_id = dsid; g.solid_mat = dsmat;
+    g.ign_off = dign_off; g.ign = dign; g.mats = dmats; g.mesh_bbox = dbbox;
+    G->n_nodes = (int)nodes.size();
+    G->depth = max_depth;
+    G->n_solid = n_solid;
+    *out = G.release();
+    return PT_OK;
+}
+
+extern "C" void pt_geometry_destroy(pt_geometry *g) { delete g; }
+
+extern "C" int pt_geometry_info(const pt_geometry *g, int *n_nodes, int *depth) {
+    if (!g) return fail(PT_EINVAL, "null geometry");
+    if (n_nodes) *n_nodes = g->n_nodes;
+    if (depth) *depth = g->depth;
+    return PT_OK;
+}
+
+/* ======================================================================== */
+/* fields                                                                    */
+/* ======================================================================== */
+
+struct pt_fields {
+    DevMem mem;
+    std::vector<ptd::Field> host;
+    ptd::Field *dev = nullptr;
+    int n = 0;
+    bool has_fp32 = false;
+};
+
+extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32, pt_fields **out) {
+    if (!out || n < 0 || (n > 0 && !descs)) return fail(PT_EINVAL, "pt_fields_create: bad arguments");
+    std::unique_ptr<pt_fields> F(new pt_fields());
+    F->n = n;
+    F->has_fp32 = with_fp32 != 0;
+    F->host.resize(std::max(n, 1));
+    for (int k = 0; k < n; ++k) {
+        const pt_field_desc &d = descs[k];
+        ptd::Field f{};
+        f.type = d.type;
+        f.bscale = d.bscale; f.escale = d.escale; f.bw = d.bw;
+        for (int a = 0; a < 6; ++a) f.box[a] = d.box[a];
+        for (int a = 0; a < 3; ++a) f.e[a] = d.e[a];
+        if (d.type == PT_FIELD_TRICUBIC) {
+            if (d.nx < 2 || d.ny < 2 || d.nz < 2 || !d.xg || !d.yg || !d.zg || !d.coef || (!d.has_b && !d.has_v))
+                return fail(PT_EINVAL, "pt_fields_create: bad tricubic descriptor");
+            const double *gs[3] = {d.xg, d.yg, d.zg};
+            int ns[3] = {d.nx, d.ny, d.nz};
+            for (int a = 0; a < 3; ++a) {
+                for (int i = 1; i < ns[a]; ++i)
+                    if (!(gs[a][i] > gs[a][i - 1])) return fail(PT_EINVAL, "pt_fields_create: grid not strictly increasing");
+                double *dg;
+                int r = F->mem.upload(&dg, gs[a], (size_t)ns[a]);
+                if (r) return r;
+                f.g[a] = dg;
+                f.x0[a] = gs[a][0];
+                f.inv_h[a] = (double)(ns[a] - 1) / (gs[a][ns[a] - 1] - gs[a][0]);
+            }
+            f.nx = d.nx; f.ny = d.ny; f.nz = d.nz;
+            f.has_b = d.has_b ? 1 : 0;
+            f.has_v = d.has_v ? 1 : 0;
+            f.ncomp = 3 * f.has_b + f.has_v;
+            size_t ncoef = (size_t)(d.nx - 1) * (d.ny - 1) * (d.nz - 1) * f.ncomp * 64;
+            double *dc;
+            int r = F->mem.upload(&dc, d.coef, ncoef);
+            if (r) return r;
+            f.coef = dc;
+            if (with_fp32) {
+                std::vector<float> c32(ncoef);
+                for (size_t i = 0; i < ncoef; ++i) c32[i] = (float)d.coef[i];
+                float *dc32;
+                r = F->mem.upload(&dc32, c32.data(), ncoef);
+                if (r) return r;
+                f.coef32 = dc32;
+            }
+        } else if (d.type != PT_FIELD_UNIFORM_E) {
+            return fail(PT_EINVAL, "pt_fields_create: unknown field type");
+        }
+        F->host[k] = f;
+    }
+    int r = F->mem.upload(&F->dev, F->host.data(), F->host.size());
+    if (r) return r;
+    *out = F.release();
+    return PT_OK;
+}
+
+extern "C" void pt_fields_destroy(pt_fields *f) { delete f; }
+
+/* ======================================================================== */
+/* tricubic pre-interpolation (TabField3::PreInterpol, src/field_3d.cpp:336-414) */
+/* ======================================================================== */
+
+namespace {
+
+/* spline1dgriddiffcubic with parabolically terminated ends (boundltype = boundrtype = 0),
+   ALGLIB interpolation.cpp:26709-26882 with the tridiagonal solver at :26970-27010 */
+void spline_grid_diff(const double *x, const double *y, int n, double *d, std::vector<double> &buf) {
+    if (n == 2) {
+        d[0] = (y[1] - y[0]) / (x[1] - x[0]);
+        d[1] = d[0];
+        return;
+    }
+    buf.resize((size_t)4 * n);
+    double *a1 = buf.data(), *a2 = a1 + n, *a3 = a2 + n, *b = a3 + n;
+    a1[0] = 0; a2[0] = 1; a3[0] = 1;
+    b[0] = 2 * (y[1] - y[0]) / (x[1] - x[0]);
+    for (int i = 1; i <= n - 2; i++) {
+        a1[i] = x[i + 1] - x[i];
+        a2[i] = 2 * (x[i + 1] - x[i - 1]);
+        a3[i] = x[i] - x[i - 1];
+        b[i] = 3 * (y[i] - y[i - 1]) / (x[i] - x[i - 1]) * (x[i + 1] - x[i]) + 3 * (y[i + 1] - y[i]) / (x[i + 1] - x[i]) * (x[i] - x[i - 1]);
+    }
+    a1[n - 1] = 1; a2[n - 1] = 1; a3[n - 1] = 0;
+    b[n - 1] = 2 * (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
+    /* Thomas algorithm on (sub a1, diag a2, super a3, rhs b) */
+    for (int k = 1; k <= n - 1; k++) {
+        double t = a1[k] / a2[k - 1];
+        a2[k] = a2[k] - t * a3[k - 1];
+        b[k] = b[k] - t * b[k - 1];
+    }
+    d[n - 1] = b[n - 1] / a2[n - 1];
+    for (int k = n - 2; k >= 0; k--) d[k] = (b[k] - a3[k] * d[k + 1]) / a2[k];
+}
+
+/* d/d(axis) of a [nx][ny][nz] table, one spline per grid line (TabField3::CalcDerivs) */
+void calc_derivs(const std::vector<double> *xyz, int nx, int ny, int nz, const double *tab, int axis, double *out) {
+    int n[3] = {nx, ny, nz};
+    int len = n[axis];
+    int d1 = (axis + 1) % 3, d2 = (axis + 2) % 3;
+    std::vector<double> yv(len), dv(len), buf;
+    size_t stride[3] = {(size_t)ny * nz, (size_t)nz, 1};
+    for (int i1 = 0; i1 < n[d1]; ++i1)
+        for (int i2 = 0; i2 < n[d2]; ++i2) {
+            size_t base = (size_t)i1 * stride[d1] + (size_t)i2 * stride[d2];
+            for (int k = 0; k < len; ++k) yv[k] = tab[base + (size_t)k * stride[axis]];
+            spline_grid_diff(xyz[axis].data(), yv.data(), len, dv.data(), buf);
+            for (int k = 0; k < len; ++k) out[base + (size_t)k * stride[axis]] = dv[k];
+        }
+}
+
+/*
+ * The 64x64 matrix mapping (f, fx, fy, fz, fxy, fxz, fyz, fxyz at the 8 cell
+ * corners) to the coefficients a[i+4j+16k] of sum a x^i y^j z^k on the unit cube
+ * (Lekien & Marsden, Int. J. Numer. Meth. Engng 63 (2005) 455).  It is the inverse
+ * of the constraint matrix, computed here by exact Gauss-Jordan elimination over
+ * the rationals (the inverse is an integer matrix).
+ */
+struct TricubicMatrix {
+    int A[64][64];
+    TricubicMatrix() {
+        /* constraint matrix M[8*d + c][i + 4j + 16k] */
+        static const int der[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+        auto mono_d = [](int p, int m, int x) -> long long {
+            /* d^m/dx^m x^p at x in {0,1} */
+            if (p < m) return 0;
+            long long c = 1;
+            for (int q = 0; q < m; ++q) c *= (p - q);
+            if (p - m == 0) return c;
+            return x ? c : 0;
+        };
+        /* rational Gauss-Jordan with numerators/denominators in __int128 */
+        typedef __int128 I;
+        static I num[64][128], den[64][128];
+        auto gcd = [](I a, I b) { if (a < 0) a = -a; if (b < 0) b = -b; while (b) { I t = a % b; a = b; b = t; } return a; };
+        auto norm = [&](I &n, I &d) { if (d < 0) { n = -n; d = -d; } I g = gcd(n, d); if (g > 1) { n /= g; d /= g; } if (n == 0) d = 1; };
+        for (int d = 0; d < 8; ++d)
+            for (int c = 0; c < 8; ++c) {
+                int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+                int row = 8 * d + c;
+                for (int k = 0; k < 4; ++k)
+                    for (int j = 0; j < 4; ++j)
+                        for (int i = 0; i < 4; ++i) {
+                            long long v = mono_d(i, der[d][0], cx) * mono_d(j, der[d][1], cy) * mono_d(k, der[d][2], cz);
+                            num[row][i + 4 * j + 16 * k] = v;
+                            den[row][i + 4 * j + 16 * k] = 1;
+                        }
+                for (int q = 0; q < 64; ++q) { num[row][64 + q] = (q == row); den[row][64 + q] = 1; }
+            }
+        for (int col = 0; col < 64; ++col) {
+            int piv = col;
+            while (num[piv][col] == 0) ++piv;
+            if (piv != col)
+                for (int q = 0; q < 128; ++q) { std::swap(num[piv][q], num[col][q]); std::swap(den[piv][q], den[col][q]); }
+            I pn = num[col][col], pd = den[col][col];
+            for (int q = 0; q < 128; ++q) { num[col][q] *= pd; den[col][q] *= pn; norm(num[col][q], den[col][q]); }
+            for (int r = 0; r < 64; ++r) {
+                if (r == col || num[r][col] == 0) continue;
+                I fn = num[r][col], fd = den[r][col];
+                for (int q = 0; q < 128; ++q) {
+                    if (num[col][q] == 0) continue;
+                    /* row_r -= f * row_col */
+                    I n2 = fn * num[col][q], d2 = fd * den[col][q];
+                    I nn = num[r][q] * d2 - n2 * den[r][q], dd = den[r][q] * d2;
+                    norm(nn, dd);
+                    num[r][q] = nn; den[r][q] = dd;
+                }
+            }
+        }
+        for (int r = 0; r < 64; ++r)
+            for (int q = 0; q < 64; ++q) A[r][q] = (int)(num[r][64 + q] / den[r][64 + q]);
+    }
+};
+
+const TricubicMatrix &tricubic_matrix() {
+    static TricubicMatrix m;
+    return m;
+}
+
+} // namespace
+
+extern "C" int pt_tricubic_precompute(int nx, int ny, int nz, const double *xg, const double *yg, const double *zg,
+                                      int ncomp, const double *values, double *coef, int threads) {
+    if (nx < 2 || ny < 2 || nz < 2 || ncomp < 1 || !xg || !yg || !zg || !values || !coef)
+        return fail(PT_EINVAL, "pt_tricubic_precompute: bad arguments");
+    std::vector<double> xyz[3] = {std::vector<double>(xg, xg + nx), std::vector<double>(yg, yg + ny), std::vector<double>(zg, zg + nz)};
+    for (int a = 0; a < 3; ++a)
+        for (size_t i = 1; i < xyz[a].size(); ++i)
+            if (!(xyz[a][i] > xyz[a][i - 1])) return fail(PT_EINVAL, "pt_tricubic_precompute: grid not strictly increasing");
+    const TricubicMatrix &M = tricubic_matrix();
+    size_t np = (size_t)nx * ny * nz;
+    size_t ncell = (size_t)(nx - 1) * (ny - 1) * (nz - 1);
+    if (threads < 1) threads = 1;
+    for (int comp = 0; comp < ncomp; ++comp) {
+        const double *tab = values + comp * np;
+        std::vector<double> dx(np), dy(np), dz(np), dxy(np), dxz(np), dyz(np), dxyz(np);
+        calc_derivs(xyz, nx, ny, nz, tab, 0, dx.data());
+        calc_derivs(xyz, nx, ny, nz, tab, 1, dy.data());
+        calc_derivs(xyz, nx, ny, nz, tab, 2, dz.data());
+        calc_derivs(xyz, nx, ny, nz, dx.data(), 1, dxy.data());
+        calc_derivs(xyz, nx, ny, nz, dx.data(), 2, dxz.data());
+        calc_derivs(xyz, nx, ny, nz, dy.data(), 2, dyz.data());
+        calc_derivs(xyz, nx, ny, nz, dxy.data(), 2, dxyz.data());
+        auto work = [&](int t) {
+            for (size_t cell = (size_t)t; cell < ncell; cell += (size_t)threads) {
+                int iz = (int)(cell % (nz - 1));
+                int iy = (int)((cell / (nz - 1)) % (ny - 1));
+                int ix = (int)(cell / ((size_t)(nz - 1) * (ny - 1)));
+                double cellx = xyz[0][ix + 1] - xyz[0][ix];
+                double celly = xyz[1][iy + 1] - xyz[1][iy];
+                double cellz = xyz[2][iz + 1] - xyz[2][iz];
+                double xv[64];
+                for (int c = 0; c < 8; ++c) {
+                    size_t p = ((size_t)(ix + (c & 1)) * ny + (iy + ((c >> 1) & 1))) * nz + (iz + ((c >> 2) & 1));
+                    xv[c] = tab[p];
+                    xv[8 + c] = dx[p] * cellx;
+                    xv[16 + c] = dy[p] * celly;
+                    xv[24 + c] = dz[p] * cellz;
+                    xv[32 + c] = dxy[p] * cellx * celly;
+                    xv[40 + c] = dxz[p] * cellx * cellz;
+                    xv[48 + c] = dyz[p] * celly * cellz;
+                    xv[56 + c] = dxyz[p] * cellx * celly * cellz;
+                }
+                double *a = coef + (cell * ncomp + comp) * 64;
+                /* tricubic_get_coeff_stacked: a[i] = sum_j A[i][j] x[j] in order */
+                for (int i = 0; i < 64; ++i) {
+                    double s = 0.0;
+                    for (int j = 0; j < 64; ++j) s += (double)M.A[i][j] * xv[j];
+                    a[i] = s;
+                }
+            }
+        };
+        if (threads == 1) work(0);
+        else {
+            std::vector<std::thread> th;
+            for (int t = 0; t < threads; ++t) th.emplace_back(work, t);
+            for (auto &t : th) t.join();
+        }
+    }
+    return PT_OK;
+}
+
+/* exported for tests: the 64x64 integer matrix */
+extern "C" int pt_tricubic_matrix(int32_t *out) {
+    const TricubicMatrix &M = tricubic_matrix();
+    for (int i = 0; i < 64; ++i)
+        for (int j = 0; j < 64; ++j) out[64 * i + j] = M.A[i][j];
+    return PT_OK;
+}
+
+/* ======================================================================== */
+/* particle batches                                                          */
+/* ======================================================================== */
+
+struct pt_batch {
+    DevMem mem;
+    ptd::Params P{};
+    int n = 0;
+    int hit_cap = 0;
+    int *d_quit = nullptr;
+    int *d_count = nullptr;
+    const pt_geometry *geom = nullptr;
+    const pt_fields *fields = nullptr;
+    ptd::Species *d_species = nullptr;
+};
+
+extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_species, const pt_species *species,
+                               double tmax_global, int n, const int32_t *species_idx, const double *t0,
+                               const double *y0, const double *spin0, uint64_t seed, const uint64_t *pid,
+                               int hit_cap, pt_batch **out) {
+    if (!g || !f || !out || n < 0 || n_species <= 0 || !species || hit_cap < 0) return fail(PT_EINVAL, "pt_batch_create: bad arguments");
+    if (n > 0 && (!species_idx || !t0 || !y0 || !spin0 || !pid)) return fail(PT_EINVAL, "pt_batch_create: null particle arrays");
+    for (int i = 0; i < n; ++i)
+        if (species_idx[i] < 0 || species_idx[i] >= n_species) return fail(PT_EINVAL, "pt_batch_create: species index out of range");
+    for (int s = 0; s < n_species; ++s)
+        if (species[s].kind < 0 || species[s].kind > 2) return fail(PT_EINVAL, "pt_batch_create: unknown species kind");
+    std::unique_ptr<pt_batch> B(new pt_batch());
+    B->n = n;
+    B->hit_cap = hit_cap;
+    B->geom = g;
+    B->fields = f;
+    std::vector<ptd::Species> sp(n_species);
+    for (int s = 0; s < n_species; ++s) {
+        const pt_species &q = species[s];
+        sp[s].kind = q.kind; sp[s].q = q.q; sp[s].m = q.m; sp[s].mu = q.mu; sp[s].gamma = q.gamma;
+        sp[s].tau = q.tau; sp[s].tmax = q.tmax; sp[s].lmax = q.lmax; sp[s].Bmax = q.Bmax; sp[s].flipspin = q.flipspin;
+    }
+    int r;
+    if ((r = B->mem.upload(&B->d_species, sp.data(), sp.size()))) return r;
+    ptd::State &S = B->P.st;
+    S.n = n;
+    size_t N = (size_t)std::max(n, 1);
+    std::vector<double> y(9 * N), sp5(5 * N), ones(N, 1.0), zeros(9 * N, 0.0);
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 9; ++k) y[(size_t)k * n + i] = y0[9 * (size_t)i + k];
+        for (int k = 0; k < 5; ++k) sp5[(size_t)k * n + i] = spin0[5 * (size_t)i + k];
+    }
+    std::vector<unsigned long long> key(N), zero64(N, 0ull);
+    for (int i = 0; i < n; ++i) key[i] = ptm_stream_key(seed, pid[i]);
+    std::vector<int> izero(N, 0), nsol(N, 1), spidx(species_idx, species_idx + n);
+    std::vector<long long> lzero(N, 0);
+    if ((r = B->mem.upload(&S.t, t0, (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.y, y.data(), 9 * (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.xc, y.data(), 9 * (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.dxc, zeros.data(), 9 * (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.ts, t0, (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.dt, zeros.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.spin, sp5.data(), 5 * (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.Hmax, zeros.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.noflip, ones.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.tau, zeros.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.stop, izero.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.flags, izero.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.Nhit, izero.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.Nstep, izero.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.Nspinflip, izero.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.nsolid, nsol.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.nhitlog, izero.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.species, spidx.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.n_acc, lzero.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.n_rej, lzero.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.ctr, zero64.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.key, key.data(), (size_t)n))) return r;
+    if ((r = B->mem.upload(&S.solids, zero64.data(), (size_t)n))) return r;
+    if ((r = B->mem.alloc(&S.hits, (size_t)n * std::max(hit_cap, 0)))) return r;
+    S.hit_cap = hit_cap;
+    int zero = 0;
+    if ((r = B->mem.upload(&B->d_quit, &zero, 1))) return r;
+    if ((r = B->mem.upload(&B->d_count, &zero, 1))) return r;
+    ptd::Params &P = B->P;
+    P.geom = g->g;
+    P.n_fields = f->n;
+    P.fields = f->dev;
+    P.species = B->d_species;
+    P.tmax_global = tmax_global;
+    P.quit = B->d_quit;
+    *out = B.release();
+    return PT_OK;
+}
+
+extern "C" void pt_batch_destroy(pt_batch *b) { delete b; }
+
+extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg) {
+    if (!b || !cfg) return fail(PT_EINVAL, "pt_batch_run: bad arguments");
+    if (cfg->variant != PT_VARIANT_FP64 && cfg->variant != PT_VARIANT_MIXED) return fail(PT_EINVAL, "pt_batch_run: unknown variant");
+    if (cfg->variant == PT_VARIANT_MIXED && !b->fields->has_fp32) {
+        bool any_map = false;
+        for (int k = 0; k < b->fields->n; ++k) any_map |= b->fields->host[k].type == PT_FIELD_TRICUBIC;
+        if (any_map) return fail(PT_EINVAL, "pt_batch_run: mixed precision needs fields created with FP32 tables");
+    }
+    int block = cfg->block > 0 ? cfg->block : 256;
+    if (block % 64 != 0 || block > 256) return fail(PT_EINVAL, "pt_batch_run: block must be a multiple of 64 and <= 256");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int q = g_quit.load();
+    PT_HIP(hipMemcpyAsync(b->d_quit, &q, sizeof(int), hipMemcpyHostToDevice, s));
+    b->P.max_steps = cfg->steps_per_launch;
+    PT_HIP(pt_launch_track(b->P, cfg->variant, block, s));
+    return PT_OK;
+}
+
+namespace {
+__global__ void count_active_kernel(const int *stop, const int *flags, int n, int *count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = i < n && (stop[i] == 0 || !(flags[i] & 1));
+    unsigned long long ballot = __ballot(active);
+    if ((threadIdx.x & 63) == 0 && ballot) atomicAdd(count, __popcll(ballot));
+}
+} // namespace
+
+extern "C" int pt_batch_active(pt_batch *b, void *stream, int *active) {
+    if (!b || !active) return fail(PT_EINVAL, "pt_batch_active: bad arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int zero = 0;
+    PT_HIP(hipMemcpyAsync(b->d_count, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    if (b->n > 0) {
+        int grid = (b->n + 255) / 256;
+        hipLaunchKernelGGL(count_active_kernel, dim3(grid), dim3(256), 0, s, b->P.st.stop, b->P.st.flags, b->n, b->d_count);
+        PT_HIP(hipGetLastError());
+    }
+    PT_HIP(hipMemcpyAsync(active, b->d_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    PT_HIP(hipStreamSynchronize(s));
+    return PT_OK;
+}
+
+extern "C" int pt_batch_download(pt_batch *b, void *stream, pt_particle_out *out, pt_hit *hits) {
+    if (!b || !out) return fail(PT_EINVAL, "pt_batch_download: bad arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    PT_HIP(hipStreamSynchronize(s));
+    int n = b->n;
+    if (n == 0) return PT_OK;
+    const ptd::State &S = b->P.st;
+    std::vector<double> t(n), y(9 * (size_t)n), spin(5 * (size_t)n), Hmax(n), noflip(n), tau(n);
+    std::vector<int> stop(n), Nhit(n), Nstep(n), Nsf(n), nsolid(n), nhit(n);
+    std::vector<long long> acc(n), rej(n);
+    std::vector<unsigned long long> solids(n);
+    PT_HIP(hipMemcpy(t.data(), S.t, n * sizeof(double), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(y.data(), S.y, 9 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(spin.data(), S.spin, 5 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(Hmax.data(), S.Hmax, n * sizeof(double), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(noflip.data(), S.noflip, n * sizeof(double), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(tau.data(), S.tau, n * sizeof(double), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(stop.data(), S.stop, n * sizeof(int), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(Nhit.data(), S.Nhit, n * sizeof(int), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(Nstep.data(), S.Nstep, n * sizeof(int), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(Nsf.data(), S.Nspinflip, n * sizeof(int), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(nsolid.data(), S.nsolid, n * sizeof(int), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(nhit.data(), S.nhitlog, n * sizeof(int), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(acc.data(), S.n_acc, n * sizeof(long long), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(rej.data(), S.n_rej, n * sizeof(long long), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(solids.data(), S.solids, n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    /* solid IDs for the end solid */
+    const pt_geometry *G = b->geom;
+    std::vector<uint32_t> sid(G->n_solid);
+    if (G->n_solid) PT_HIP(hipMemcpy(sid.data(), G->g.solid_id, G->n_solid * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+        pt_particle_out &o = out[i];
+        o.t = t[i];
+        for (int k = 0; k < 9; ++k) o.y[k] = y[(size_t)k * n + i];
+        for (int k = 0; k < 5; ++k) o.spin[k] = spin[(size_t)k * n + i];
+        o.stopID = stop[i]; o.Nhit = Nhit[i]; o.Nstep = Nstep[i]; o.Nspinflip = Nsf[i];
+        o.n_acc = acc[i]; o.n_rej = rej[i];
+        o.Hmax = Hmax[i]; o.noflipprob = noflip[i]; o.tau = tau[i];
+        o.n_hits = nhit[i];
+        /* TTracker::GetCurrentsolid on the final stack */
+        unsigned long long sk = solids[i];
+        int best = 0;
+        auto id_of = [&](int e) -> uint32_t { int sidx = ptd::sk_sidx(sk, e); return sidx < 0 ? 1u : sid[sidx]; };
+        for (int e = 1; e < nsolid[i]; ++e) {
+            int ai = ptd::sk_ign(sk, best), bi = ptd::sk_ign(sk, e);
+            if (ai || (!bi && id_of(best) < id_of(e))) best = e;
+        }
+        o.solidend = (int32_t)id_of(best);
+    }
+    if (hits && b->hit_cap > 0) {
+        std::vector<ptd::Hit> h((size_t)n * b->hit_cap);
+        PT_HIP(hipMemcpy(h.data(), S.hits, h.size() * sizeof(ptd::Hit), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < h.size(); ++k) {
+            pt_hit &o = hits[k];
+            o.t = h[k].t; o.tri = h[k].tri; o.leaving = h[k].leaving; o.entering = h[k].entering;
+            for (int a = 0; a < 3; ++a) { o.normal[a] = h[k].normal[a]; o.v1[a] = h[k].v1[a]; o.v2[a] = h[k].v2[a]; }
+            o.pol1 = h[k].pol1; o.pol2 = h[k].pol2;
+        }
+    }
+    return PT_OK;
+}
+
+extern "C" int pt_batch_step_counts(pt_batch *b, void *stream, int64_t *accepted, int64_t *rejected) {
+    if (!b) return fail(PT_EINVAL, "pt_batch_step_counts: bad arguments");
+    PT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    int n = b->n;
+    std::vector<long long> acc(std::max(n, 1)), rej(std::max(n, 1));
+    if (n) {
+        PT_HIP(hipMemcpy(acc.data(), b->P.st.n_acc, n * sizeof(long long), hipMemcpyDeviceToHost));
+        PT_HIP(hipMemcpy(rej.data(), b->P.st.n_rej, n * sizeof(long long), hipMemcpyDeviceToHost));
+    }
+    long long sa = 0, sr = 0;
+    for (int i = 0; i < n; ++i) { sa += acc[i]; sr += rej[i]; }
+    if (accepted) *accepted = sa;
+    if (rejected) *rejected = sr;
+    return PT_OK;
+}
+
+extern "C" int pt_track_batch(void *stream, const pt_geometry *g, const pt_fields *f, int n_species,
+                              const pt_species *species, double tmax_global, int n, const int32_t *species_idx,
+                              const double *t0, const double *y0, const double *spin0, uint64_t seed,
+                              const uint64_t *pid, pt_particle_out *out, pt_hit *hits, int hit_cap,
+                              const pt_launch_cfg *cfg) {
+    pt_batch *b = nullptr;
+    int r = pt_batch_create(g, f, n_species, species, tmax_global, n, species_idx, t0, y0, spin0, seed, pid, hit_cap, &b);
+    if (r) return r;
+    std::unique_ptr<pt_batch> guard(b);
+    pt_launch_cfg c = cfg ? *cfg : pt_launch_cfg{PT_VARIANT_FP64, 256, 0, 0, 0};
+    for (;;) {
+        r = pt_batch_run(b, stream, &c);
+        if (r) return r;
+        int active = 0;
+        r = pt_batch_active(b, stream, &active);
+        if (r) return r;
+        if (active == 0 || g_quit.load()) break;
+    }
+    return pt_batch_download(b, stream, out, hits);
+}
+
+extern "C" void pt_request_quit(int on) { g_quit.store(on ? 1 : 0); }
+
+extern "C" const char *pt_last_error(void) { return g_last_error.c_str(); }

@@ -1,0 +1,177 @@
+"""TFieldManager for the MI355X engine.
+
+Field containers mirror TFieldContainer (include/field.h, src/field.cpp):
+a field, constant B and E scale factors (the reference's exprtk time formulas
+reduced to constants) and an optional boundary box with a smooth taper.
+Supported fields on the tracking path:
+  * TabField3 -- tricubic interpolation of a 3D table (src/field_3d.cpp);
+    the 64 coefficients per cell come from pt_tricubic_precompute (host C++)
+  * TEDMStaticEField -- uniform E with V = Ez*z (src/edmfields.cpp:175-187)
+Coil fields used to fill the synthetic tables are computed here on the host.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native
+from .constants import MU0
+
+
+@dataclass
+class FieldContainer:
+    type: int
+    bscale: float = 1.0
+    escale: float = 1.0
+    box: Sequence[float] = (0, 0, 0, 0, 0, 0)   # xmax xmin ymax ymin zmax zmin (no bounds by default)
+    bw: float = 0.0
+    # tricubic
+    grid: Optional[tuple] = None                 # (xg, yg, zg)
+    has_b: int = 0
+    has_v: int = 0
+    coef: Optional[np.ndarray] = None            # [cells][ncomp][64]
+    # uniform E
+    e: Sequence[float] = (0.0, 0.0, 0.0)
+
+    def __post_init__(self):
+        b = self.box
+        if b[1] < b[0] and b[3] < b[2] and b[5] < b[4]:
+            if self.bw < 0 or self.bw > (b[0] - b[1]) / 2 or self.bw > (b[2] - b[3]) / 2 or self.bw > (b[4] - b[5]) / 2:
+                raise ValueError("You defined a field with an invalid boundary. Check field parameters.")
+
+
+def tabfield3(xg, yg, zg, B=None, V=None, bscale=1.0, escale=0.0, box=(0, 0, 0, 0, 0, 0), bw=0.0, threads=None):
+    """TabField3 container from samples on a grid.
+
+    B: [3][nx][ny][nz] (T) or None; V: [nx][ny][nz] (V) or None.
+    Grid coordinates are made sorted and unique as in TabField3::TabField3
+    (src/field_3d.cpp:417-426).
+    """
+    xg, yg, zg = (np.unique(np.asarray(g, dtype=np.float64)) for g in (xg, yg, zg))
+    comps = []
+    if B is not None:
+        B = np.asarray(B, dtype=np.float64).reshape(3, len(xg), len(yg), len(zg))
+        comps.extend([B[0], B[1], B[2]])
+    if V is not None:
+        comps.append(np.asarray(V, dtype=np.float64).reshape(len(xg), len(yg), len(zg)))
+    if not comps:
+        raise ValueError("tabfield3 needs B or V samples")
+    values = np.ascontiguousarray(np.stack(comps))
+    coef = tricubic_coefficients(xg, yg, zg, values, threads=threads)
+    return FieldContainer(type=_native.PT_FIELD_TRICUBIC, bscale=bscale, escale=escale, box=tuple(box), bw=bw,
+                          grid=(xg, yg, zg), has_b=int(B is not None), has_v=int(V is not None), coef=coef)
+
+
+def tricubic_coefficients(xg, yg, zg, values, threads=None):
+    """TabField3::PreInterpol via the native host routine: [cells][ncomp][64]."""
+    L = _native.lib()
+    nx, ny, nz = len(xg), len(yg), len(zg)
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    ncomp = values.shape[0]
+    coef = np.empty(((nx - 1) * (ny - 1) * (nz - 1), ncomp, 64), dtype=np.float64)
+    xg, yg, zg = (np.ascontiguousarray(g, dtype=np.float64) for g in (xg, yg, zg))
+    if threads is None:
+        threads = min(16, os.cpu_count() or 1)
+    rc = L.pt_tricubic_precompute(nx, ny, nz, _native.dptr(xg), _native.dptr(yg), _native.dptr(zg), ncomp,
+                                  _native.dptr(values), _native.dptr(coef), threads)
+    _native.check(rc, "pt_tricubic_precompute")
+    return coef
+
+
+def uniform_efield(ex, ey, ez, escale=1.0):
+    """TEDMStaticEField in a container with B scale "0" (src/fields.cpp:100-104)."""
+    return FieldContainer(type=_native.PT_FIELD_UNIFORM_E, bscale=0.0, escale=escale, e=(ex, ey, ez))
+
+
+def loop_field(points, radius, z0, current):
+    """B of a circular current loop coaxial with z (Biot-Savart, elliptic integrals).
+
+    points [n][3] -> B [n][3] in T.
+    """
+    from scipy.special import ellipe, ellipk
+    p = np.asarray(points, dtype=np.float64)
+    x, y, z = p[:, 0], p[:, 1], p[:, 2] - z0
+    rho = np.hypot(x, y)
+    a = radius
+    q = (a + rho) ** 2 + z ** 2
+    m = 4 * a * rho / q
+    K = ellipk(m)
+    E = ellipe(m)
+    d = (a - rho) ** 2 + z ** 2
+    pref = MU0 * current / (2 * np.pi)
+    Bz = pref / np.sqrt(q) * (K + (a * a - rho * rho - z * z) / d * E)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        Brho = np.where(rho > 1e-12, pref * z / (rho * np.sqrt(q)) * (-K + (a * a + rho * rho + z * z) / d * E), 0.0)
+        cx = np.where(rho > 1e-12, x / np.where(rho > 0, rho, 1), 0.0)
+        cy = np.where(rho > 1e-12, y / np.where(rho > 0, rho, 1), 0.0)
+    return np.stack([Brho * cx, Brho * cy, Bz], axis=1)
+
+
+def coil_set(seed, n=8, radius=0.30, zmin=-0.05, zmax=0.55):
+    """8 coaxial loops with seed-derived currents of alternating sign."""
+    rng = np.random.default_rng(seed)
+    zs = np.linspace(zmin, zmax, n)
+    currents = rng.uniform(0.5, 1.5, n) * np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    return [(radius, float(z), float(I)) for z, I in zip(zs, currents)]
+
+
+def sample_coils(coils, xg, yg, zg, scale=1.0):
+    X, Y, Z = np.meshgrid(xg, yg, zg, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    B = np.zeros_like(pts)
+    for a, z0, I in coils:
+        B += loop_field(pts, a, z0, I * scale)
+    return B.T.reshape(3, len(xg), len(yg), len(zg))
+
+
+class TFieldManager:
+    """Ordered list of field containers, summed in order (src/fields.cpp:139-182)."""
+
+    def __init__(self, containers=()):
+        self.containers = list(containers)
+        self._handle = None
+
+    def descs(self):
+        ds = (_native.FieldDesc * max(1, len(self.containers)))()
+        keep = []
+        for k, c in enumerate(self.containers):
+            d = ds[k]
+            d.type = c.type
+            d.bscale = c.bscale
+            d.escale = c.escale
+            for i in range(6):
+                d.box[i] = c.box[i]
+            d.bw = c.bw
+            for i in range(3):
+                d.e[i] = c.e[i]
+            if c.type == _native.PT_FIELD_TRICUBIC:
+                xg, yg, zg = (np.ascontiguousarray(g, dtype=np.float64) for g in c.grid)
+                coef = np.ascontiguousarray(c.coef, dtype=np.float64)
+                keep.extend([xg, yg, zg, coef])
+                d.nx, d.ny, d.nz = len(xg), len(yg), len(zg)
+                d.xg, d.yg, d.zg = _native.dptr(xg), _native.dptr(yg), _native.dptr(zg)
+                d.has_b, d.has_v = c.has_b, c.has_v
+                d.coef = _native.dptr(coef)
+        return ds, keep
+
+    def device(self, with_fp32=False):
+        if self._handle is not None and (self._handle[1] or not with_fp32):
+            return self._handle[0]
+        L = _native.lib()
+        ds, keep = self.descs()
+        h = C.c_void_p()
+        rc = L.pt_fields_create(len(self.containers), ds, int(bool(with_fp32)), C.byref(h))
+        _native.check(rc, "pt_fields_create")
+        if self._handle is not None:
+            L.pt_fields_destroy(self._handle[0])
+        self._handle = (h, bool(with_fp32))
+        return h
+
+    def __del__(self):
+        try:
+            if self._handle is not None and _native._lib is not None:
+                _native._lib.pt_fields_destroy(self._handle[0])
+        except Exception:
+            pass

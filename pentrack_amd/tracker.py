@@ -1,0 +1,118 @@
+"""TTracker for the MI355X engine.
+
+`TTracker.IntegrateParticle` keeps the reference signature's meaning
+(include/tracking.h:46-47): integrate particles from their current state to a
+stop condition, at most until global time tmax.  `IntegrateBatch` is the batch
+form the GPU runs; `Batch` exposes the resumable device state for callers that
+advance a batch in slices (the benchmark, the persistent variant).
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native
+from .particle import SpeciesConfig
+
+
+@dataclass
+class LaunchConfig:
+    variant: int = _native.PT_VARIANT_FP64
+    block: int = 256
+    steps_per_launch: int = 0
+    persistent: int = 0
+    grid: int = 0
+    leaf_size: int = 4
+
+    def native(self):
+        c = _native.LaunchCfg()
+        c.variant, c.block, c.steps_per_launch, c.persistent, c.grid = (
+            self.variant, self.block, self.steps_per_launch, self.persistent, self.grid)
+        return c
+
+
+class Batch:
+    """Particles resident in device memory with their resumable tracker state."""
+
+    def __init__(self, geometry, fields, species, tmax, species_idx, t0, y0, spin0, seed, pid,
+                 hit_cap=0, with_fp32=False, leaf_size=4):
+        L = _native.lib()
+        self.n = len(t0)
+        self.hit_cap = hit_cap
+        sp = (_native.Species * len(species))(*[s.native() for s in species])
+        self._keep = [np.ascontiguousarray(species_idx, dtype=np.int32), np.ascontiguousarray(t0, dtype=np.float64),
+                      np.ascontiguousarray(y0, dtype=np.float64).reshape(-1, 9),
+                      np.ascontiguousarray(spin0, dtype=np.float64).reshape(-1, 5),
+                      np.ascontiguousarray(pid, dtype=np.uint64), sp]
+        si, t, y, s, p, _ = self._keep
+        self.geometry, self.fields = geometry, fields
+        g = geometry.device(leaf_size)
+        f = fields.device(with_fp32)
+        h = C.c_void_p()
+        rc = L.pt_batch_create(g, f, len(species), sp, float(tmax), self.n, si.ctypes.data_as(C.POINTER(C.c_int32)),
+                               _native.dptr(t), _native.dptr(y), _native.dptr(s), int(seed) & (2 ** 64 - 1),
+                               p.ctypes.data_as(C.POINTER(C.c_uint64)), hit_cap, C.byref(h))
+        _native.check(rc, "pt_batch_create")
+        self.h = h
+
+    def run(self, cfg=None, stream=None):
+        cfg = cfg or LaunchConfig()
+        c = cfg.native()
+        _native.check(_native.lib().pt_batch_run(self.h, stream, C.byref(c)), "pt_batch_run")
+
+    def active(self, stream=None):
+        a = C.c_int()
+        _native.check(_native.lib().pt_batch_active(self.h, stream, C.byref(a)), "pt_batch_active")
+        return a.value
+
+    def run_to_completion(self, cfg=None, stream=None, max_launches=1 << 30):
+        for _ in range(max_launches):
+            self.run(cfg, stream)
+            if self.active(stream) == 0:
+                return True
+        return False
+
+    def step_counts(self, stream=None):
+        a, r = C.c_int64(), C.c_int64()
+        _native.check(_native.lib().pt_batch_step_counts(self.h, stream, C.byref(a), C.byref(r)), "pt_batch_step_counts")
+        return a.value, r.value
+
+    def download(self, stream=None):
+        out = np.zeros(self.n, dtype=_native.OUT_DTYPE)
+        hits = np.zeros(self.n * self.hit_cap, dtype=_native.HIT_DTYPE) if self.hit_cap else None
+        rc = _native.lib().pt_batch_download(self.h, stream, out.ctypes.data_as(C.c_void_p),
+                                             hits.ctypes.data_as(C.c_void_p) if hits is not None else None)
+        _native.check(rc, "pt_batch_download")
+        if hits is not None:
+            hits = hits.reshape(self.n, self.hit_cap)
+        return out, hits
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None and _native._lib is not None:
+                _native._lib.pt_batch_destroy(self.h)
+        except Exception:
+            pass
+
+
+class TTracker:
+    """Drop-in for TTracker::IntegrateParticle over batches of particles."""
+
+    def __init__(self, geometry, fields, launch=None):
+        self.geometry = geometry
+        self.fields = fields
+        self.launch = launch or LaunchConfig()
+
+    def IntegrateBatch(self, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap=0):
+        cfg = self.launch
+        b = Batch(self.geometry, self.fields, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap,
+                  with_fp32=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
+        b.run_to_completion(cfg)
+        return b.download()
+
+    def IntegrateParticle(self, species: SpeciesConfig, tmax, t0, y0, spin0=None, seed=0, pid=0, hit_cap=0):
+        out, hits = self.IntegrateBatch([species], tmax, np.zeros(1, np.int32), np.array([t0], np.float64),
+                                        np.asarray(y0, np.float64).reshape(1, 9),
+                                        np.zeros((1, 5)) if spin0 is None else np.asarray(spin0).reshape(1, 5),
+                                        seed, np.array([pid], np.uint64), hit_cap)
+        return out[0], (hits[0] if hits is not None else None)

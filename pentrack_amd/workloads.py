@@ -1,0 +1,114 @@
+"""Synthetic workloads of the tracking path (inputs shared by tests, smoke and bench).
+
+cube        1 m cube (12 triangles) as solid 2 of material "vacuum" inside a default
+            solid of Fermi potential 335 neV; specular walls; gravity only.
+cylinder    closed cylinder R = 0.25 m, H = 0.5 m (about 20k triangles); 2% Lambert,
+            loss 1e-4 per bounce; B from a tricubic map of 8 coaxial loops.
+Particles are created like a volume source: uniform positions inside, kinetic
+energy uniform in [0, Emax], isotropic directions (src/source.cpp:157-202 draws
+the same quantities).  Host-side sampling uses numpy's PCG64 seeded per
+workload; the tracker's own random draws use the per-particle streams.
+"""
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+
+from .constants import NEUTRON
+from .fields import TFieldManager, coil_set, sample_coils, tabfield3
+from .geometry import Material, Solid, TGeometry, box_mesh, cylinder_mesh
+from .particle import SpeciesConfig, make_states
+
+
+@dataclass
+class Workload:
+    name: str
+    geometry: TGeometry
+    fields: TFieldManager
+    species: List[SpeciesConfig]
+    tmax: float
+    species_idx: np.ndarray
+    t0: np.ndarray
+    y0: np.ndarray
+    spin0: np.ndarray
+    pid: np.ndarray
+    seed: int
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return len(self.t0)
+
+
+def cube_geometry(fermi=335.0, diffprob=0.0, loss=0.0):
+    mats = [Material("wall", FermiReal=fermi), Material("vacuum", DiffProb=diffprob, LossPerBounce=loss)]
+    return TGeometry(mats, "wall", [Solid(2, "vacuum", box_mesh((0, 0, 0), (1, 1, 1)))])
+
+
+def cylinder_geometry(radius=0.25, height=0.5, n_phi=200, n_z=40, n_r=5, fermi=335.0, diffprob=0.02, loss=1e-4):
+    mats = [Material("wall", FermiReal=fermi), Material("storage", DiffProb=diffprob, LossPerBounce=loss)]
+    return TGeometry(mats, "wall", [Solid(2, "storage", cylinder_mesh(radius, height, n_phi, n_z, n_r))])
+
+
+def neutron_source(rng, n, inside, lo, hi, Emax, pol=None):
+    pos = np.empty((n, 3))
+    k = 0
+    while k < n:
+        p = rng.uniform(lo, hi, size=(max(2 * (n - k), 16), 3))
+        p = p[inside(p)]
+        take = min(len(p), n - k)
+        pos[k:k + take] = p[:take]
+        k += take
+    E = rng.uniform(0.0, Emax, n)
+    phi = rng.uniform(0.0, 2 * np.pi, n)
+    theta = np.arccos(rng.uniform(-1.0, 1.0, n))
+    if pol is None:
+        pol = np.where(rng.uniform(size=n) < 0.5, 1.0, -1.0)
+    else:
+        pol = np.full(n, float(pol))
+    return make_states(NEUTRON, pos, E, phi, theta, pol)
+
+
+def coil_map(seed, n_grid, lo, hi, Bmax=1.0, threads=None):
+    """n_grid^3 tricubic B map of the 8-loop coil set over [lo, hi] plus one cell of margin,
+    scaled so that max |B| on the grid is Bmax."""
+    lo = np.asarray(lo, float)
+    hi = np.asarray(hi, float)
+    h = (hi - lo) / (n_grid - 3)
+    axes = [lo[a] - h[a] + h[a] * np.arange(n_grid) for a in range(3)]
+    coils = coil_set(seed)
+    B = sample_coils(coils, *axes)
+    scale = Bmax / np.sqrt((B ** 2).sum(axis=0)).max()
+    B *= scale
+    return tabfield3(*axes, B=B, threads=threads), dict(coils=coils, scale=scale, axes=axes)
+
+
+def cube(n=10000, seed=1, tmax=10.0, Emax=300e-9, with_field=False, n_grid=32, Bmax=0.1):
+    """Cube workload (SURVEY config 1; with a field map: the config-3 geometry and source)."""
+    rng = np.random.default_rng(seed)
+    geom = cube_geometry()
+    eps = 1e-6
+    t0, y0, spin0 = neutron_source(rng, n, lambda p: np.ones(len(p), bool), eps, 1 - eps, Emax)
+    containers = []
+    meta = {}
+    if with_field:
+        fmap, meta = coil_map(seed, n_grid, (0, 0, 0), (1, 1, 1), Bmax=Bmax)
+        containers.append(fmap)
+    sp = [SpeciesConfig(kind=NEUTRON, tau=0.0, tmax=9e99, lmax=9e99)]
+    return Workload("cube", geom, TFieldManager(containers), sp, tmax, np.zeros(n, np.int32), t0, y0, spin0,
+                    np.arange(n, dtype=np.uint64), seed, meta)
+
+
+def cylinder(n=1000000, seed=2, tmax=200.0, Emax=250e-9, n_grid=64, Bmax=1.0, n_phi=200, n_z=40, n_r=5):
+    """Cylinder storage volume with a coil-field map (SURVEY config 2)."""
+    rng = np.random.default_rng(seed)
+    R, H = 0.25, 0.5
+    geom = cylinder_geometry(R, H, n_phi, n_z, n_r)
+    # inscribed polygon radius, so sampled points are inside the faceted mesh
+    r_in = R * np.cos(np.pi / n_phi) * (1 - 1e-6)
+    inside = lambda p: np.hypot(p[:, 0], p[:, 1]) < r_in
+    t0, y0, spin0 = neutron_source(rng, n, inside, (-R, -R, 1e-6), (R, R, H - 1e-6), Emax, pol=1.0)
+    fmap, meta = coil_map(seed, n_grid, (-R, -R, 0.0), (R, R, H), Bmax=Bmax)
+    sp = [SpeciesConfig(kind=NEUTRON, tau=0.0, tmax=9e99, lmax=9e99)]
+    return Workload("cylinder", geom, TFieldManager([fmap]), sp, tmax, np.zeros(n, np.int32), t0, y0, spin0,
+                    np.arange(n, dtype=np.uint64), seed, meta)
