@@ -1635,3 +1635,33 @@ void ptref_species_constants(int kind, double *qmmg) {
 }
 
 int ptref_refsem(void) { return PTREF_REFSEM; }
+
+/* deterministic math and random-stream primitives, exported for their tests */
+int ptref_detmath(int which, int n, const double *x, const double *y, double *out) {
+    for (int i = 0; i < n; ++i) {
+        double r = 0, im = 0;
+        switch (which) {
+            case 0: r = ptm_log(x[i]); break;
+            case 1: r = ptm_exp(x[i]); break;
+            case 2: r = ptm_pow(x[i], y[i]); break;
+            case 3: r = ptm_sin(x[i]); break;
+            case 4: r = ptm_cos(x[i]); break;
+            case 5: r = ptm_acos(x[i]); break;
+            case 6: r = ptm_hypot(x[i], y[i]); break;
+            case 7: ptm_csqrt(x[i], y[i], &r, &im); break;
+            case 8: ptm_csqrt(x[i], y[i], &im, &r); break;
+            default: return -1;
+        }
+        out[i] = r;
+    }
+    return 0;
+}
+
+int ptref_rng(uint64_t seed, uint64_t id, int n, uint64_t *raw, double *canonical) {
+    uint64_t key = ptm_stream_key(seed, id), ctr = 0;
+    for (int i = 0; i < n; ++i) {
+        raw[i] = ptm_draw_u64(key, &ctr);
+        canonical[i] = ptm_canonical(raw[i]);
+    }
+    return 0;
+}

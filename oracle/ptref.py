@@ -82,6 +82,8 @@ def load(mode="fp64"):
     L.ptref_dense_probe.argtypes = [C.c_int, C.c_double, dp, C.c_double, C.c_int, dp, dp, dp]
     L.ptref_species_constants.argtypes = [C.c_int, dp]
     L.ptref_species_constants.restype = None
+    L.ptref_detmath.argtypes = [C.c_int, C.c_int, dp, dp, dp]
+    L.ptref_rng.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.POINTER(C.c_uint64), dp]
     _libs[mode] = L
     return L
 
@@ -156,3 +158,80 @@ def track(workload, mode="fp64", hit_cap=0, threads=1, sel=None, problem=None):
     if rc != 0:
         raise RuntimeError(f"ptref_track failed ({rc})")
     return out, (hits.reshape(n, hit_cap) if hits is not None else None)
+
+
+def detmath(which, x, y=None, mode="fp64"):
+    """Deterministic elementary functions of pt_detmath.h (log exp pow sin cos acos hypot csqrt)."""
+    names = {"log": 0, "exp": 1, "pow": 2, "sin": 3, "cos": 4, "acos": 5, "hypot": 6, "csqrt_re": 7, "csqrt_im": 8}
+    L = load(mode)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, dtype=np.float64)
+    out = np.empty_like(x)
+    assert L.ptref_detmath(names[which], len(x), _dp(x), _dp(y), _dp(out)) == 0
+    return out
+
+
+def rng_stream(seed, pid, n, mode="fp64"):
+    L = load(mode)
+    raw = np.empty(n, dtype=np.uint64)
+    can = np.empty(n, dtype=np.float64)
+    L.ptref_rng(int(seed), int(pid), n, raw.ctypes.data_as(C.POINTER(C.c_uint64)), _dp(can))
+    return raw, can
+
+
+def single_geometry(tris, solid_ids, mats, default_mat=0, ign=None):
+    """Oracle geometry from raw arrays (tests of the collision primitives)."""
+    class _G:
+        pass
+    tris = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+    g = Geom()
+    keep = [tris]
+    ts = np.ascontiguousarray(solid_ids[0], dtype=np.int32)
+    sid = np.ascontiguousarray(solid_ids[1], dtype=np.uint32)
+    smat = np.ascontiguousarray(solid_ids[2], dtype=np.int32)
+    ign_off = np.zeros(len(sid) + 1, dtype=np.int32)
+    ign_arr = np.zeros((0, 2))
+    if ign is not None:
+        ign_off = np.ascontiguousarray(ign[0], dtype=np.int32)
+        ign_arr = np.ascontiguousarray(ign[1], dtype=np.float64).reshape(-1, 2)
+    mats = np.ascontiguousarray(mats, dtype=np.float64).reshape(-1, 9)
+    keep += [ts, sid, smat, ign_off, ign_arr, mats]
+    g.n_tri = len(tris); g.tri = _dp(tris); g.tri_solid = ts.ctypes.data_as(C.POINTER(C.c_int32))
+    g.n_solid = len(sid); g.solid_id = sid.ctypes.data_as(C.POINTER(C.c_uint32))
+    g.solid_mat = smat.ctypes.data_as(C.POINTER(C.c_int32)); g.ign_off = ign_off.ctypes.data_as(C.POINTER(C.c_int32))
+    g.ign = _dp(ign_arr); g.default_mat = default_mat; g.n_mat = len(mats); g.mats = _dp(mats)
+    o = _G()
+    o.geom, o._keep = g, keep
+    return o
+
+
+def collisions(geom, p1, p2, mode="fp64", cap=64):
+    L = load(mode)
+    p1 = np.ascontiguousarray(p1, dtype=np.float64)
+    p2 = np.ascontiguousarray(p2, dtype=np.float64)
+    s = np.zeros(cap); tri = np.zeros(cap, np.int32); ID = np.zeros(cap, np.uint32); dn = np.zeros(cap)
+    nrm = np.zeros((cap, 3))
+    n = L.ptref_collisions(C.byref(geom.geom), _dp(p1), _dp(p2), cap, _dp(s), tri.ctypes.data_as(C.POINTER(C.c_int32)),
+                           ID.ctypes.data_as(C.POINTER(C.c_uint32)), _dp(dn), _dp(nrm))
+    if n < 0:
+        return n, None
+    n = min(n, cap)
+    return n, dict(s=s[:n], tri=tri[:n], ID=ID[:n], distnormal=dn[:n], normal=nrm[:n])
+
+
+def get_solids(geom, t, p, mode="fp64"):
+    L = load(mode)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    ids = np.zeros(16, np.uint32); ign = np.zeros(16, np.int32)
+    n = L.ptref_get_solids(C.byref(geom.geom), float(t), _dp(p), ids.ctypes.data_as(C.POINTER(C.c_uint32)),
+                           ign.ctypes.data_as(C.POINTER(C.c_int32)), 16)
+    return list(ids[:n]), list(ign[:n])
+
+
+def field_eval(problem, pts, t=0.0, mode="fp64"):
+    L = load(mode)
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    n = len(pts)
+    B = np.zeros((n, 3)); dB = np.zeros((n, 3, 3)); V = np.zeros(n); E = np.zeros((n, 3))
+    L.ptref_field_eval(problem.n_fields, problem.fields, n, _dp(pts), float(t), _dp(B), _dp(dB), _dp(V), _dp(E))
+    return B, dB, V, E

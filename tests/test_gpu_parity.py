@@ -1,0 +1,125 @@
+"""GPU parity: the HIP tracker against the FP64 oracle on the same inputs.
+
+Integer/index results (stop reasons, hit counts, hit-triangle sequences, step
+counts) and the FP64 state must agree bit for bit; the mixed-precision variant
+is checked against the FP64 oracle within its stated tolerance.
+"""
+import numpy as np
+import pytest
+
+from pentrack_amd import _native, workloads
+from pentrack_amd.constants import NEUTRON, PROTON, ELECTRON
+from pentrack_amd.fields import TFieldManager, uniform_efield
+from pentrack_amd.particle import SpeciesConfig, make_states
+from pentrack_amd.tracker import Batch, LaunchConfig
+from pentrack_amd.workloads import Workload
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("t", "y", "spin", "stopID", "Nhit", "Nstep", "Nspinflip", "n_acc", "n_rej", "Hmax", "noflipprob", "tau",
+          "solidend", "n_hits")
+
+
+def run_gpu(w, hit_cap=16, cfg=None, with_fp32=False):
+    b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
+              hit_cap=hit_cap, with_fp32=with_fp32)
+    assert b.run_to_completion(cfg or LaunchConfig(), max_launches=100000)
+    return b.download()
+
+
+def assert_bit_exact(g, gh, o, oh):
+    for f in FIELDS:
+        if not np.array_equal(g[f], o[f]):
+            bad = np.where(~np.all((g[f] == o[f]).reshape(len(g), -1), axis=1))[0]
+            raise AssertionError(f"{f} differs for {len(bad)} particles, first {bad[:5]}")
+    if gh is not None:
+        n = np.minimum(g["n_hits"], gh.shape[1])
+        for k in np.where(n > 0)[0][:2000]:
+            for f in ("t", "tri", "leaving", "entering", "normal", "v1", "v2", "pol1", "pol2"):
+                assert np.array_equal(gh[k, :n[k]][f], oh[k, :n[k]][f]), (k, f)
+
+
+def test_cube_specular_bit_exact(gpu, oracle_lib):
+    w = workloads.cube(n=3000, seed=1, tmax=10.0)
+    print("case: cube n=3000 tmax=10 seed=1", flush=True)
+    g, gh = run_gpu(w)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
+    assert_bit_exact(g, gh, o, oh)
+    assert (g["stopID"] == -1).mean() > 0.99 and g["Nhit"].mean() > 50
+
+
+def test_sliced_launches_equal_single_launch(gpu):
+    w = workloads.cube(n=1000, seed=3, tmax=5.0)
+    print("case: cube sliced n=1000", flush=True)
+    a, ah = run_gpu(w)
+    b, bh = run_gpu(w, cfg=LaunchConfig(steps_per_launch=7))
+    assert_bit_exact(a, ah, b, bh)
+
+
+def test_cylinder_field_map_lambert_bit_exact(gpu, oracle_lib):
+    w = workloads.cylinder(n=2000, seed=2, tmax=3.0, n_grid=32)
+    print("case: cylinder n=2000 tmax=3 grid=32", flush=True)
+    g, gh = run_gpu(w)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
+    assert_bit_exact(g, gh, o, oh)
+    assert g["Nhit"].sum() > 1000
+
+
+def test_charged_species_uniform_E_and_map(gpu, oracle_lib):
+    """Protons and electrons (Lorentz force, relativistic EoM, absorption inside solids)
+    in a uniform E plus the coil map, mixed with neutrons with a finite lifetime."""
+    base = workloads.cylinder(n=600, seed=4, tmax=0.2, n_grid=24, Bmax=0.5)
+    rng = np.random.default_rng(4)
+    n = base.n
+    kinds = np.where(np.arange(n) % 3 == 0, 1, np.where(np.arange(n) % 3 == 1, 2, 0)).astype(np.int32)
+    E = np.where(kinds == 1, rng.uniform(1, 800, n), np.where(kinds == 2, rng.uniform(1e3, 7e5, n), rng.uniform(0, 250e-9, n)))
+    phi = rng.uniform(0, 2 * np.pi, n)
+    theta = np.arccos(rng.uniform(-1, 1, n))
+    y0 = base.y0.copy()
+    for k in (0, 1, 2):
+        sel = kinds == k
+        _, yk, _ = make_states(k, base.y0[sel, :3], E[sel], phi[sel], theta[sel], 1.0)
+        y0[sel] = yk
+    species = [SpeciesConfig(NEUTRON, tau=0.5, tmax=9e99), SpeciesConfig(PROTON, tmax=3e-3),
+               SpeciesConfig(ELECTRON, tmax=1e-5)]
+    fm = TFieldManager(base.fields.containers + [uniform_efield(0.0, 0.0, 1e5)])
+    w = Workload("mixed-species", base.geometry, fm, species, base.tmax, kinds, base.t0, y0, base.spin0, base.pid, 4)
+    print("case: charged species n=600", flush=True)
+    g, gh = run_gpu(w)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
+    assert_bit_exact(g, gh, o, oh)
+    assert set(np.unique(g["stopID"])) >= {-4, 1}
+
+
+def test_edge_cases(gpu, oracle_lib):
+    w = workloads.cube(n=8, seed=5, tmax=1.0)
+    w.t0[0] = 2.0                  # starts past tmax: the reference throws (invariant stop)
+    w.y0[1, :3] = [3.0, 3.0, 3.0]  # outside every mesh bounding box: ID_HIT_BOUNDARIES
+    w.y0[2, 2] = 1.0 - 1e-9        # just below the ceiling
+    print("case: edge cases", flush=True)
+    g, gh = run_gpu(w)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=16)
+    assert_bit_exact(g, gh, o, oh)
+    assert g["stopID"][0] == _native.ID_INVARIANT and g["stopID"][1] == _native.ID_HIT_BOUNDARIES
+    # an empty batch is valid
+    e = workloads.cube(n=1, seed=5, tmax=1.0)
+    b = Batch(e.geometry, e.fields, e.species, e.tmax, e.species_idx[:0], e.t0[:0], e.y0[:0], e.spin0[:0], e.seed,
+              e.pid[:0])
+    b.run()
+    assert b.active() == 0
+
+
+def test_mixed_precision_error_bound(gpu, oracle_lib):
+    """FP32 field interpolation, FP64 state/control: end positions within 1e-6 m of the
+    FP64 oracle over a short horizon with identical collision sequences."""
+    w = workloads.cylinder(n=1000, seed=6, tmax=0.5, n_grid=32)
+    print("case: mixed precision n=1000", flush=True)
+    g, gh = run_gpu(w, cfg=LaunchConfig(variant=_native.PT_VARIANT_MIXED), with_fp32=True)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
+    same = (g["stopID"] == o["stopID"]) & (g["Nhit"] == o["Nhit"])
+    n = np.minimum(g["n_hits"], 16)
+    for k in np.where(same)[0]:
+        same[k] &= np.array_equal(gh[k, :n[k]]["tri"], oh[k, :n[k]]["tri"])
+    assert same.mean() > 0.99
+    d = np.linalg.norm(g["y"][same, :3] - o["y"][same, :3], axis=1)
+    assert np.percentile(d, 99) < 1e-6
