@@ -69,6 +69,14 @@ int pt_geometry_create(int n_tri, const float *tri_vertices, const int32_t *tri_
                        const int32_t *ign_off, const double *ign, int default_mat,
                        int n_mat, const double *mats, int leaf_size, pt_geometry **out);
 void pt_geometry_destroy(pt_geometry *g);
+/*
+ * Host-only diagnostic (no device memory): the clearance grid pt_geometry_create
+ * builds over the mesh bounding box, a lower bound per cell of the distance to the
+ * nearest triangle.  No counterpart in the reference (its AABB tree is queried for
+ * every segment); with out == NULL only gn[3] and lo[3] are filled.
+ */
+int pt_clearance_grid(int n_tri, const float *tri_vertices, int cells_long, float *out, int32_t *gn,
+                      double *lo, double *inv_h);
 /* number of BVH nodes and depth of the built tree (diagnostics) */
 int pt_geometry_info(const pt_geometry *g, int *n_nodes, int *depth);
 
@@ -108,6 +116,9 @@ void pt_fields_destroy(pt_fields *f);
  */
 int pt_tricubic_precompute(int nx, int ny, int nz, const double *xg, const double *yg, const double *zg,
                            int ncomp, const double *values, double *coef, int threads);
+/* the 64x64 integer matrix of libtricubic's tricubic_get_coeff (coefficients = A x),
+   row-major into out[64*64]; for tests */
+int pt_tricubic_matrix(int32_t *out);
 
 /* ---- particles --------------------------------------------------------- */
 typedef struct {
@@ -142,9 +153,11 @@ typedef struct {
 typedef struct {
     int variant;
     int block;                   /* threads per block (64..1024) */
-    int steps_per_launch;        /* accepted steps per particle per launch (0 = unlimited) */
+    int steps_per_launch;        /* accepted steps per particle per pt_batch_run call (0 = unlimited) */
     int persistent;              /* 1: persistent kernel refilling slots from a device work queue */
     int grid;                    /* persistent grid size in blocks (0 = automatic) */
+    int rounds;                  /* step-kernel / collision-kernel rounds per pt_batch_run call
+                                    (0 = 4); each round resolves at most one wall collision per particle */
 } pt_launch_cfg;
 
 typedef struct pt_batch pt_batch;

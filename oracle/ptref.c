@@ -1319,7 +1319,9 @@ static void integrate_particle(const ctx_t *c, const ptref_species *cfg, double 
         double x1 = x, y1[9];
         memcpy(y1, y, sizeof(y1));
         if (dopri_do_step(tp, &st) < 0) {
+            /* odeint threw: x and y keep the step-start values (src/tracking.cpp:78-85) */
             tp->stopID = ID_ODEINT_ERROR;
+            break;
         }
         x = st.t;
         memcpy(y, st.xc, sizeof(y));

@@ -74,7 +74,7 @@ class HitRec(C.Structure):
 class LaunchCfg(C.Structure):
     _fields_ = [
         ("variant", C.c_int), ("block", C.c_int), ("steps_per_launch", C.c_int),
-        ("persistent", C.c_int), ("grid", C.c_int),
+        ("persistent", C.c_int), ("grid", C.c_int), ("rounds", C.c_int),
     ]
 
 
@@ -100,7 +100,7 @@ EXPORTS = [
     "pt_fields_create", "pt_fields_destroy", "pt_tricubic_precompute",
     "pt_batch_create", "pt_batch_destroy", "pt_batch_run", "pt_batch_active",
     "pt_batch_download", "pt_batch_step_counts", "pt_track_batch",
-    "pt_request_quit", "pt_last_error",
+    "pt_request_quit", "pt_last_error", "pt_clearance_grid", "pt_tricubic_matrix",
 ]
 
 _lib = None
@@ -145,6 +145,8 @@ def lib():
                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.c_uint64, C.POINTER(C.c_uint64), vp, vp, C.c_int, C.POINTER(LaunchCfg)]
     L.pt_request_quit.argtypes = [C.c_int]
+    L.pt_clearance_grid.argtypes = [C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float),
+                                    C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pt_request_quit.restype = None
     L.pt_last_error.restype = C.c_char_p
     _lib = L

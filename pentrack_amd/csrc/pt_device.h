@@ -69,6 +69,11 @@ struct Geom {
     const double *ign;
     const double *mats;       /* [n_mat][9] */
     const double *mesh_bbox;  /* [n_solid][6] xmin ymin zmin xmax ymax zmax */
+    /* clearance grid: lower bound of the distance from any point of a cell to any
+       triangle; a segment shorter than the bound of its start cell hits nothing */
+    const float *clear;       /* [gn0][gn1][gn2] */
+    int gn[3];
+    double glo[3], ginv_h[3];
 };
 
 struct Field {
@@ -105,7 +110,15 @@ struct State {
     unsigned long long *ctr, *key, *solids;
     Hit *hits;
     int hit_cap;
+    double *Bc, *Vc;          /* field at the stepper's current state, [3][n] and [n] (valid if FL_BVALID) */
+    double *pend;             /* pending wall-collision piece: x1, y1[9], x2, y2[9] -> [20][n] */
+    double *dense;            /* dense-output state of the last step: [DENSE][n] */
+    int *budget;              /* accepted steps left in the current run call */
+    int *queue;               /* indices of particles parked on a wall collision */
+    int *qcount;              /* queue length */
 };
+
+constexpr int DENSE = 59;     /* xo[8] dxo[8] k3..k6[32] dxc[8] t t_old pol */
 
 struct Params {
     Geom geom;
@@ -114,7 +127,6 @@ struct Params {
     const Species *species;
     double tmax_global;
     State st;
-    int max_steps;            /* accepted steps per particle per launch; <= 0: unlimited */
     const int *quit;
 };
 
@@ -354,7 +366,9 @@ struct HitList {
  * list == nullptr: count only.  Returns the number of intersections, or
  * -1 (coplanar overlap: the reference throws) / -2 (more than MAX_HITS).
  */
-PTD_NOINLINE int seg_query(const Geom &G, const double *p, const double *q, HitList *list, long long *tests) {
+PTD_NOINLINE int2 seg_query_v(const Geom &G, double p0, double p1, double p2, double q0, double q1, double q2, HitList *list) {
+    const double p[3] = {p0, p1, p2}, q[3] = {q0, q1, q2};
+    int ntests = 0;
     double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
     double inv[3] = {1.0 / d[0], 1.0 / d[1], 1.0 / d[2]};
     double sl = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -362,7 +376,7 @@ PTD_NOINLINE int seg_query(const Geom &G, const double *p, const double *q, HitL
     int sp = 0;
     int node = 0;
     int count = 0;
-    if (G.n_node == 0) return 0;
+    if (G.n_node == 0) return make_int2(0, 0);
     for (;;) {
         BvhNode nd;
         load_node(G.node + node, nd);
@@ -373,11 +387,11 @@ PTD_NOINLINE int seg_query(const Geom &G, const double *p, const double *q, HitL
                     double a[3], b[3], c[3], pt[3];
                     int solid, orig;
                     load_tri(G.tri + first + k, a, b, c, solid, orig);
-                    ++*tests;
+                    ++ntests;
                     int r = seg_tri(a, b, c, p, q, pt);
                     if (r == 0) continue;
-                    if (r < 0) return -1;
-                    if (count >= MAX_HITS) return -2;
+                    if (r < 0) return make_int2(-1, ntests);
+                    if (count >= MAX_HITS) return make_int2(-2, ntests);
                     ++count;
                     if (list) {
                         double v[3] = {pt[0] - p[0], pt[1] - p[1], pt[2] - p[2]};
@@ -398,7 +412,7 @@ PTD_NOINLINE int seg_query(const Geom &G, const double *p, const double *q, HitL
                 }
             } else {
                 if (sp < BVH_STACK) stack[sp++] = nd.b;
-                else return -2;
+                else return make_int2(-2, ntests);
                 node = nd.a;
                 continue;
             }
@@ -406,12 +420,19 @@ PTD_NOINLINE int seg_query(const Geom &G, const double *p, const double *q, HitL
         if (sp == 0) break;
         node = stack[--sp];
     }
-    return count;
+    return make_int2(count, ntests);
+}
+
+PTD int seg_query(const Geom &G, const double *p, const double *q, HitList *list, long long *tests) {
+    int2 r = seg_query_v(G, p[0], p[1], p[2], q[0], q[1], q[2], list);
+    *tests += r.y;
+    return r.x;
 }
 
 /* +z ray parity per solid (TTriangleMesh::GetSolids, include/trianglemesh.h:181-188):
    bit m of the result is set if the ray from p crosses solid m an odd number of times */
-PTD_NOINLINE unsigned long long ray_parity(const Geom &G, const double *p) {
+PTD_NOINLINE unsigned long long ray_parity(const Geom &G, double p0, double p1, double p2) {
+    const double p[3] = {p0, p1, p2};
     unsigned long long par = 0;
     int stack[BVH_STACK];
     int sp = 0, node = 0;
@@ -479,6 +500,23 @@ PTD int current_solid(const Geom &G, unsigned long long s, int n) {
         if (ai || (!bi && solid_ID(G, sk_sidx(s, best)) < solid_ID(G, sk_sidx(s, i)))) best = i;
     }
     return sk_sidx(s, best);
+}
+
+/* true if segment p->q provably misses every triangle (clearance grid) */
+PTD bool clear_of_walls(const Geom &G, const double *p, const double *q) {
+    if (G.clear == nullptr) return false;
+    int c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double u = (p[a] - G.glo[a]) * G.ginv_h[a];
+        if (!(u >= 0.0) || u >= (double)G.gn[a]) return false;
+        c[a] = (int)u;
+    }
+    double bound = (double)G.clear[((size_t)c[0] * G.gn[1] + c[1]) * G.gn[2] + c[2]];
+    double d0 = q[0] - p[0], d1 = q[1] - p[1], d2 = q[2] - p[2];
+    double L2 = d0 * d0 + d1 * d1 + d2 * d2;
+    double b = 0.999 * bound - 1e-9;
+    return b > 0.0 && L2 < b * b;
 }
 
 /* TGeometry::CheckSegment -> TTriangleMesh::InBoundingBox */

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <atomic>
 #include <cmath>
 #include <cstdint>
@@ -127,6 +128,128 @@ int build_bvh(std::vector<BuildPrim> &prims, int begin, int end, int leaf_size, 
     return me;
 }
 
+/* squared distance from point p to triangle (a, b, c): closest point by the
+   Voronoi regions of the vertices, edges and face */
+double point_tri_dist2(const double *p, const double *a, const double *b, const double *c) {
+    auto sub = [](const double *u, const double *v, double *w) { w[0] = u[0] - v[0]; w[1] = u[1] - v[1]; w[2] = u[2] - v[2]; };
+    auto dot = [](const double *u, const double *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+    double ab[3], ac[3], ap[3], q[3];
+    sub(b, a, ab); sub(c, a, ac); sub(p, a, ap);
+    auto at = [&](double s, double t) {
+        for (int k = 0; k < 3; ++k) q[k] = a[k] + s * ab[k] + t * ac[k];
+        double d[3];
+        sub(p, q, d);
+        return dot(d, d);
+    };
+    double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) return at(0, 0);
+    double bp[3];
+    sub(p, b, bp);
+    double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) return at(1, 0);
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) return at(d1 / (d1 - d3), 0);
+    double cp[3];
+    sub(p, c, cp);
+    double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) return at(0, 1);
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) return at(0, d2 / (d2 - d6));
+    double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+        double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        return at(1 - w, w);
+    }
+    double den = 1.0 / (va + vb + vc);
+    if (!std::isfinite(den)) {   /* degenerate triangle: nearest of its edges */
+        double best = INFINITY;
+        const double *V[3] = {a, b, c};
+        for (int e = 0; e < 3; ++e) {
+            const double *u = V[e], *v = V[(e + 1) % 3];
+            double uv[3], up[3];
+            sub(v, u, uv); sub(p, u, up);
+            double l2 = dot(uv, uv), t = l2 > 0 ? std::min(1.0, std::max(0.0, dot(up, uv) / l2)) : 0.0;
+            double d[3] = {up[0] - t * uv[0], up[1] - t * uv[1], up[2] - t * uv[2]};
+            best = std::min(best, dot(d, d));
+        }
+        return best;
+    }
+    return at(vb * den, vc * den);
+}
+
+/* distance from p to the nearest triangle: branch and bound over the BVH */
+double nearest_tri_dist(const double *p, const std::vector<ptd::BvhNode> &nodes, const std::vector<double> &tv) {
+    double best2 = INFINITY;
+    int stack[ptd::BVH_STACK];
+    int sp = 0;
+    stack[sp++] = 0;
+    auto box_d2 = [&](const ptd::BvhNode &n) {
+        double d2 = 0;
+        for (int a = 0; a < 3; ++a) {
+            double d = std::max(std::max((double)n.lo[a] - p[a], p[a] - (double)n.hi[a]), 0.0);
+            d2 += d * d;
+        }
+        return d2;
+    };
+    while (sp > 0) {
+        const ptd::BvhNode &n = nodes[stack[--sp]];
+        if (box_d2(n) >= best2) continue;
+        if (n.a < 0) {
+            int first = -n.a - 1;
+            for (int k = first; k < first + n.b; ++k) {
+                const double *T = &tv[(size_t)9 * k];
+                best2 = std::min(best2, point_tri_dist2(p, T, T + 3, T + 6));
+            }
+        } else {
+            double dl = box_d2(nodes[n.a]), dr = box_d2(nodes[n.b]);
+            /* nearer child on top */
+            if (dl < dr) { if (dr < best2) stack[sp++] = n.b; if (dl < best2) stack[sp++] = n.a; }
+            else { if (dl < best2) stack[sp++] = n.a; if (dr < best2) stack[sp++] = n.b; }
+        }
+    }
+    return std::sqrt(best2);
+}
+
+/*
+ * Clearance grid over the mesh's bounding box: for each cell a lower bound of the
+ * distance from any point of the cell to any triangle (distance from the cell
+ * centre to the nearest triangle minus half the cell diagonal, rounded down to
+ * float).  A segment that starts in a cell and is shorter than its bound crosses
+ * no triangle, so the tracker skips the BVH query for it.
+ */
+void build_clearance(const std::vector<ptd::BvhNode> &nodes, const std::vector<double> &tv, const double lo[3],
+                     const double hi[3], int cells_long, std::vector<float> &clear, int gn[3], double ginv_h[3]) {
+    double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    double h = ext / cells_long;
+    double hs[3];
+    for (int a = 0; a < 3; ++a) {
+        gn[a] = std::max(1, (int)std::ceil((hi[a] - lo[a]) / h));
+        hs[a] = (hi[a] - lo[a]) / gn[a];
+        if (!(hs[a] > 0)) { hs[a] = h; }
+        ginv_h[a] = 1.0 / hs[a];
+    }
+    double half_diag = 0.5 * std::sqrt(hs[0] * hs[0] + hs[1] * hs[1] + hs[2] * hs[2]);
+    size_t ncell = (size_t)gn[0] * gn[1] * gn[2];
+    clear.assign(ncell, 0.f);
+    int threads = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    auto work = [&](int t) {
+        for (size_t cidx = (size_t)t; cidx < ncell; cidx += (size_t)threads) {
+            int iz = (int)(cidx % gn[2]);
+            int iy = (int)((cidx / gn[2]) % gn[1]);
+            int ix = (int)(cidx / ((size_t)gn[2] * gn[1]));
+            double p[3] = {lo[0] + (ix + 0.5) * hs[0], lo[1] + (iy + 0.5) * hs[1], lo[2] + (iz + 0.5) * hs[2]};
+            double b = nearest_tri_dist(p, nodes, tv) - half_diag;
+            b = b > 0 ? b * (1 - 1e-6) : 0.0;
+            float f = (float)b;
+            if ((double)f > b) f = std::nextafter(f, 0.f);
+            clear[cidx] = f;
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; ++t) th.emplace_back(work, t);
+    for (auto &t : th) t.join();
+}
+
 } // namespace
 
 extern "C" int pt_geometry_create(int n_tri, const float *tri_vertices, const int32_t *tri_solid, int n_solid,
@@ -218,6 +341,23 @@ extern "C" int pt_geometry_create(int n_tri, const float *tri_vertices, const in
     if ((r = G->mem.upload(&dbbox, bbox.data(), bbox.size()))) return r;
     g.tri = dtri; g.nrm = dnrm; g.node = dnode; g.solid_id = dsid; g.solid_mat = dsmat;
     g.ign_off = dign_off; g.ign = dign; g.mats = dmats; g.mesh_bbox = dbbox;
+    g.clear = nullptr;
+    if (n_tri > 0) {
+        std::vector<double> tv((size_t)9 * n_tri);
+        double glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int k = 0; k < n_tri; ++k)
+            for (int c = 0; c < 9; ++c) {
+                tv[(size_t)9 * k + c] = tris[k].v[c];
+                glo[c % 3] = std::min(glo[c % 3], (double)tris[k].v[c]);
+                ghi[c % 3] = std::max(ghi[c % 3], (double)tris[k].v[c]);
+            }
+        std::vector<float> clear;
+        build_clearance(nodes, tv, glo, ghi, 96, clear, g.gn, g.ginv_h);
+        for (int a = 0; a < 3; ++a) g.glo[a] = glo[a];
+        float *dclear;
+        if ((r = G->mem.upload(&dclear, clear.data(), clear.size()))) return r;
+        g.clear = dclear;
+    }
     G->n_nodes = (int)nodes.size();
     G->depth = max_depth;
     G->n_solid = n_solid;
@@ -226,6 +366,51 @@ extern "C" int pt_geometry_create(int n_tri, const float *tri_vertices, const in
 }
 
 extern "C" void pt_geometry_destroy(pt_geometry *g) { delete g; }
+
+/* host-only: the clearance grid pt_geometry_create builds (for tests and diagnostics);
+   with out == NULL only gn, lo and inv_h are returned */
+extern "C" int pt_clearance_grid(int n_tri, const float *tri_vertices, int cells_long, float *out, int32_t *gn,
+                                 double *lo, double *inv_h) {
+    if (n_tri <= 0 || !tri_vertices || cells_long < 1 || cells_long > 1024 || !gn || !lo || !inv_h)
+        return fail(PT_EINVAL, "pt_clearance_grid: bad arguments");
+    std::vector<BuildPrim> prims(n_tri);
+    double glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = 0; i < n_tri; ++i) {
+        const float *T = tri_vertices + (size_t)9 * i;
+        BuildPrim &p = prims[i];
+        p.idx = i;
+        for (int a = 0; a < 3; ++a) {
+            p.lo[a] = std::min(T[a], std::min(T[3 + a], T[6 + a]));
+            p.hi[a] = std::max(T[a], std::max(T[3 + a], T[6 + a]));
+            p.c[a] = (p.lo[a] + p.hi[a]) * 0.5f;
+            glo[a] = std::min(glo[a], (double)p.lo[a]);
+            ghi[a] = std::max(ghi[a], (double)p.hi[a]);
+        }
+    }
+    std::vector<ptd::BvhNode> nodes;
+    std::vector<int> order;
+    int depth = 0;
+    build_bvh(prims, 0, n_tri, 4, nodes, order, 0, depth);
+    if (depth + 2 >= ptd::BVH_STACK) return fail(PT_EINVAL, "pt_clearance_grid: BVH too deep");
+    std::vector<double> tv((size_t)9 * n_tri);
+    for (int k = 0; k < n_tri; ++k)
+        for (int c = 0; c < 9; ++c) tv[(size_t)9 * k + c] = tri_vertices[(size_t)9 * order[k] + c];
+    std::vector<float> clear;
+    int g[3];
+    double ih[3];
+    if (!out) {
+        double ext = std::max(ghi[0] - glo[0], std::max(ghi[1] - glo[1], ghi[2] - glo[2]));
+        for (int a = 0; a < 3; ++a) {
+            g[a] = std::max(1, (int)std::ceil((ghi[a] - glo[a]) / (ext / cells_long)));
+            gn[a] = g[a]; lo[a] = glo[a];
+        }
+        return PT_OK;
+    }
+    build_clearance(nodes, tv, glo, ghi, cells_long, clear, g, ih);
+    std::memcpy(out, clear.data(), clear.size() * sizeof(float));
+    for (int a = 0; a < 3; ++a) { gn[a] = g[a]; lo[a] = glo[a]; inv_h[a] = ih[a]; }
+    return PT_OK;
+}
 
 extern "C" int pt_geometry_info(const pt_geometry *g, int *n_nodes, int *depth) {
     if (!g) return fail(PT_EINVAL, "null geometry");
@@ -279,13 +464,20 @@ extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32
             f.has_v = d.has_v ? 1 : 0;
             f.ncomp = 3 * f.has_b + f.has_v;
             size_t ncoef = (size_t)(d.nx - 1) * (d.ny - 1) * (d.nz - 1) * f.ncomp * 64;
+            /* libtricubic order a[i + 4j + 16k] (coefficient of x^i y^j z^k) becomes
+               a[16i + 4j + k]: the 16 coefficients of one power of x are contiguous */
+            std::vector<double> cdev(ncoef);
+            for (size_t b = 0; b < ncoef; b += 64)
+                for (int i = 0; i < 4; ++i)
+                    for (int j = 0; j < 4; ++j)
+                        for (int k = 0; k < 4; ++k) cdev[b + 16 * i + 4 * j + k] = d.coef[b + i + 4 * j + 16 * k];
             double *dc;
-            int r = F->mem.upload(&dc, d.coef, ncoef);
+            int r = F->mem.upload(&dc, cdev.data(), ncoef);
             if (r) return r;
             f.coef = dc;
             if (with_fp32) {
                 std::vector<float> c32(ncoef);
-                for (size_t i = 0; i < ncoef; ++i) c32[i] = (float)d.coef[i];
+                for (size_t i = 0; i < ncoef; ++i) c32[i] = (float)cdev[i];
                 float *dc32;
                 r = F->mem.upload(&dc32, c32.data(), ncoef);
                 if (r) return r;
@@ -571,6 +763,13 @@ extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_s
     if ((r = B->mem.upload(&S.solids, zero64.data(), (size_t)n))) return r;
     if ((r = B->mem.alloc(&S.hits, (size_t)n * std::max(hit_cap, 0)))) return r;
     S.hit_cap = hit_cap;
+    if ((r = B->mem.alloc(&S.Bc, 3 * N))) return r;
+    if ((r = B->mem.alloc(&S.Vc, N))) return r;
+    if ((r = B->mem.alloc(&S.pend, 20 * N))) return r;
+    if ((r = B->mem.alloc(&S.dense, (size_t)ptd::DENSE * N))) return r;
+    if ((r = B->mem.upload(&S.budget, izero.data(), N))) return r;
+    if ((r = B->mem.alloc(&S.queue, N))) return r;
+    if ((r = B->mem.upload(&S.qcount, izero.data(), 1))) return r;
     int zero = 0;
     if ((r = B->mem.upload(&B->d_quit, &zero, 1))) return r;
     if ((r = B->mem.upload(&B->d_count, &zero, 1))) return r;
@@ -600,8 +799,11 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
     hipStream_t s = static_cast<hipStream_t>(stream);
     int q = g_quit.load();
     PT_HIP(hipMemcpyAsync(b->d_quit, &q, sizeof(int), hipMemcpyHostToDevice, s));
-    b->P.max_steps = cfg->steps_per_launch;
-    PT_HIP(pt_launch_track(b->P, cfg->variant, block, s));
+    int steps = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : INT_MAX;
+    int rounds = cfg->rounds > 0 ? cfg->rounds : 4;
+    int coll_grid = std::max(1, std::min((b->n + 255) / 256, 2048));
+    PT_HIP(pt_launch_init(b->P, cfg->variant, steps, s));
+    for (int r = 0; r < rounds; ++r) PT_HIP(pt_launch_round(b->P, cfg->variant, block, coll_grid, s));
     return PT_OK;
 }
 
@@ -716,7 +918,7 @@ extern "C" int pt_track_batch(void *stream, const pt_geometry *g, const pt_field
     int r = pt_batch_create(g, f, n_species, species, tmax_global, n, species_idx, t0, y0, spin0, seed, pid, hit_cap, &b);
     if (r) return r;
     std::unique_ptr<pt_batch> guard(b);
-    pt_launch_cfg c = cfg ? *cfg : pt_launch_cfg{PT_VARIANT_FP64, 256, 0, 0, 0};
+    pt_launch_cfg c = cfg ? *cfg : pt_launch_cfg{PT_VARIANT_FP64, 256, 0, 0, 0, 0};
     for (;;) {
         r = pt_batch_run(b, stream, &c);
         if (r) return r;

@@ -2,11 +2,18 @@
  * pt_track.hip -- the tracking kernels of the MI355X engine.
  *
  * One thread per particle runs TTracker::IntegrateParticle's loop
- * (src/tracking.cpp:17-173) on resumable state kept in HBM: each launch
- * advances every live particle by up to max_steps accepted dopri5 steps.
+ * (src/tracking.cpp:17-173) on resumable state kept in HBM.  A run call is a
+ * few rounds of two kernels:
+ *  - the step kernel keeps the dopri5 stepper in registers (the polarisation,
+ *    whose derivative is identically zero, is carried outside the stage
+ *    vectors) and finishes every step whose chord pieces touch no wall; a
+ *    particle whose chord piece meets the mesh parks its piece and dense-output
+ *    state in HBM and enters the collision queue;
+ *  - the collision kernel runs the queued particles, compacted into full waves,
+ *    through CheckHit's bisection, DoHit and OnHit and the rest of their step.
  * Built with -ffp-contract=off: the FP64 path evaluates every expression in
  * the reference's order, with the reference's long double constants rounded
- * to double.
+ * to double, so results match the FP64 CPU restatement bit for bit.
  */
 #include "pt_device.h"
 #include "pt_kernels.h"
@@ -48,18 +55,22 @@ PTD bool tric_locate(const Field &f, double x, double y, double z, long &cell, d
  * coefficients.  Multiplications by fact[..][0] = 1 are exact, so sharing the
  * inner z- and y-Horner partial sums between value and derivatives leaves every
  * result bit-identical to the four separate evaluations.
+ * Device layout of a block: a[16 i + 4 j + k] holds the coefficient of x^i y^j z^k,
+ * so the 16 coefficients of one power of x are one contiguous 128-byte chunk.
  */
 PTD void tric_eval_grad64(const double *__restrict__ a, double x, double y, double z, bool grad,
                           double &F, double &Fx, double &Fy, double &Fz) {
     double ry[4], ryd[4], ryz[4];
 #pragma unroll
     for (int i = 3; i >= 0; --i) {
+        const double2 *c2 = reinterpret_cast<const double2 *>(a + 16 * i);
+        double c[16];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { double2 v = c2[q]; c[2 * q] = v.x; c[2 * q + 1] = v.y; }
         double rY = 0., rYd = 0., rYz = 0.;
 #pragma unroll
         for (int j = 3; j >= 0; --j) {
-            const double2 *c2 = reinterpret_cast<const double2 *>(a + i + 4 * j);
-            double a0 = a[i + 4 * j], a1 = a[i + 4 * j + 16], a2 = a[i + 4 * j + 32], a3 = a[i + 4 * j + 48];
-            (void)c2;
+            double a0 = c[4 * j], a1 = c[4 * j + 1], a2 = c[4 * j + 2], a3 = c[4 * j + 3];
             double rz = 0.;
             rz += a3; rz *= z;
             rz += a2; rz *= z;
@@ -103,16 +114,20 @@ PTD void tric_eval_grad64(const double *__restrict__ a, double x, double y, doub
     }
 }
 
-/* FP32 variant for the mixed-precision tracker: coefficients and Horner in FP32 */
+/* FP32 variant for the mixed-precision tracker: coefficients and Horner in FP32 (with FMA) */
 PTD void tric_eval_grad32(const float *__restrict__ a, float x, float y, float z, bool grad,
                           float &F, float &Fx, float &Fy, float &Fz) {
     float ry[4], ryd[4], ryz[4];
 #pragma unroll
     for (int i = 3; i >= 0; --i) {
+        const float4 *c4 = reinterpret_cast<const float4 *>(a + 16 * i);
+        float c[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { float4 v = c4[q]; c[4 * q] = v.x; c[4 * q + 1] = v.y; c[4 * q + 2] = v.z; c[4 * q + 3] = v.w; }
         float rY = 0.f, rYd = 0.f, rYz = 0.f;
 #pragma unroll
         for (int j = 3; j >= 0; --j) {
-            float a0 = a[i + 4 * j], a1 = a[i + 4 * j + 16], a2 = a[i + 4 * j + 32], a3 = a[i + 4 * j + 48];
+            float a0 = c[4 * j], a1 = c[4 * j + 1], a2 = c[4 * j + 2], a3 = c[4 * j + 3];
             float rz = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(a3, z, a2), z, a1), z, a0);
             rY = __builtin_fmaf(rY, y, rz);
             if (grad) {
@@ -140,15 +155,19 @@ PTD bool in_bounds(const Field &f, double x, double y, double z) {
 PTD double smth(double x) { return ((6 * x - 15) * x + 10) * x * x * x; }
 PTD double smthd(double x) { return ((30 * x - 60) * x + 30) * x * x; }
 
-/* TFieldBoundaryBox::scaleScalarFieldAtBounds (src/field.cpp:71-112) */
-PTD_NOINLINE void scale_at_bounds(const Field &f, double x, double y, double z, double *F, double *dF) {
+/* TFieldBoundaryBox::scaleScalarFieldAtBounds (src/field.cpp:71-112); F and its gradient
+   (if grad) in, scaled values out */
+struct FG { double F, d0, d1, d2; };
+PTD_NOINLINE FG scale_at_bounds(const Field &f, double x, double y, double z, double F, double g0, double g1, double g2, bool grad) {
     const double *b = f.box;
     double xmax = b[0], xmin = b[1], ymax = b[2], ymin = b[3], zmax = b[4], zmin = b[5], bw = f.bw;
+    double dF[3] = {g0, g1, g2};
+    FG out = {F, g0, g1, g2};
     bool has = xmin < xmax && ymin < ymax && zmin < zmax;
-    if (!has || (*F == 0 && dF == nullptr) || (*F == 0 && dF[0] == 0 && dF[1] == 0 && dF[2] == 0)) return;
+    if (!has || (F == 0 && !grad) || (F == 0 && dF[0] == 0 && dF[1] == 0 && dF[2] == 0)) return out;
     if (!in_bounds(f, x, y, z)) {
-        *F = 0.;
-        if (dF) dF[0] = dF[1] = dF[2] = 0.;
+        out.F = 0.;
+        if (grad) { out.d0 = 0.; out.d1 = 0.; out.d2 = 0.; }
     } else if (bw > 0) {
         double Fscale = 1.;
         double dFadd[3] = {0., 0., 0.};
@@ -157,17 +176,22 @@ PTD_NOINLINE void scale_at_bounds(const Field &f, double x, double y, double z, 
         for (int i = 0; i < 3; ++i) {
             if (0 <= lo[i] && lo[i] <= 1) {
                 Fscale *= smth(lo[i]);
-                if (dF) dFadd[i] += *F * smthd(lo[i]) / bw / smth(lo[i]);
+                if (grad) dFadd[i] += F * smthd(lo[i]) / bw / smth(lo[i]);
             } else if (0 <= hi[i] && hi[i] <= 1) {
                 Fscale *= smth(hi[i]);
-                if (dF) dFadd[i] += -*F * smthd(hi[i]) / bw / smth(hi[i]);
+                if (grad) dFadd[i] += -F * smthd(hi[i]) / bw / smth(hi[i]);
             }
         }
         if (Fscale != 1.) {
-            *F *= Fscale;
-            if (dF) for (int i = 0; i < 3; i++) dF[i] = dF[i] * Fscale + dFadd[i] * Fscale;
+            out.F = F * Fscale;
+            if (grad) {
+                out.d0 = dF[0] * Fscale + dFadd[0] * Fscale;
+                out.d1 = dF[1] * Fscale + dFadd[1] * Fscale;
+                out.d2 = dF[2] * Fscale + dFadd[2] * Fscale;
+            }
         }
     }
+    return out;
 }
 
 PTD bool has_box(const Field &f) { return f.box[1] < f.box[0] && f.box[3] < f.box[2] && f.box[5] < f.box[4]; }
@@ -183,7 +207,7 @@ PTD void container_B(const Field &f, double x, double y, double z, double B[3], 
             long cell; double r[3], dist[3];
             if (tric_locate(f, x, y, z, cell, r, dist)) {
                 ++nfield;
-#pragma unroll
+#pragma unroll 1
                 for (int i = 0; i < 3; ++i) {
                     double Fv, Fx = 0, Fy = 0, Fz = 0;
                     if (MIXED) {
@@ -193,8 +217,11 @@ PTD void container_B(const Field &f, double x, double y, double z, double B[3], 
                     } else {
                         tric_eval_grad64(f.coef + ((size_t)cell * f.ncomp + i) * 64, r[0], r[1], r[2], grad, Fv, Fx, Fy, Fz);
                     }
-                    Bt[i] = Fv;
-                    if (grad) { dBt[i][0] = Fx / dist[0]; dBt[i][1] = Fy / dist[1]; dBt[i][2] = Fz / dist[2]; }
+                    double d0 = Fx / dist[0], d1 = Fy / dist[1], d2 = Fz / dist[2];
+                    /* write the component without a dynamically indexed array */
+                    if (i == 0) { Bt[0] = Fv; if (grad) { dBt[0][0] = d0; dBt[0][1] = d1; dBt[0][2] = d2; } }
+                    else if (i == 1) { Bt[1] = Fv; if (grad) { dBt[1][0] = d0; dBt[1][1] = d1; dBt[1][2] = d2; } }
+                    else { Bt[2] = Fv; if (grad) { dBt[2][0] = d0; dBt[2][1] = d1; dBt[2][2] = d2; } }
                 }
             }
         }
@@ -203,8 +230,14 @@ PTD void container_B(const Field &f, double x, double y, double z, double B[3], 
             Bt[i] *= f.bscale;
             if (grad) { dBt[i][0] *= f.bscale; dBt[i][1] *= f.bscale; dBt[i][2] *= f.bscale; }
         }
-        if (has_box(f))
-            for (int i = 0; i < 3; ++i) scale_at_bounds(f, x, y, z, &Bt[i], grad ? dBt[i] : nullptr);
+        if (has_box(f)) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                FG r = scale_at_bounds(f, x, y, z, Bt[i], dBt[i][0], dBt[i][1], dBt[i][2], grad);
+                Bt[i] = r.F;
+                if (grad) { dBt[i][0] = r.d0; dBt[i][1] = r.d1; dBt[i][2] = r.d2; }
+            }
+        }
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -225,7 +258,7 @@ PTD void field_B(const Params &P, double x, double y, double z, double B[3], dou
 
 /* TFieldContainer::EField / TFieldManager::EField (src/field.cpp:155-169, src/fields.cpp:167-182) */
 template <bool MIXED>
-PTD_NOINLINE void field_E(const Params &P, double x, double y, double z, double *V, double E[3], long long &nfield) {
+PTD void field_E(const Params &P, double x, double y, double z, double *V, double E[3], long long &nfield) {
     *V = 0; E[0] = E[1] = E[2] = 0;
     for (int k = 0; k < P.n_fields; ++k) {
         const Field &f = P.fields[k];
@@ -255,7 +288,10 @@ PTD_NOINLINE void field_E(const Params &P, double x, double y, double z, double 
             }
             Vt *= f.escale;
             Et[0] *= f.escale; Et[1] *= f.escale; Et[2] *= f.escale;
-            if (has_box(f)) scale_at_bounds(f, x, y, z, &Vt, Et);
+            if (has_box(f)) {
+                FG r = scale_at_bounds(f, x, y, z, Vt, Et[0], Et[1], Et[2], true);
+                Vt = r.F; Et[0] = r.d0; Et[1] = r.d1; Et[2] = r.d2;
+            }
         }
         *V += Vt;
         for (int i = 0; i < 3; i++) E[i] += Et[i];
@@ -301,17 +337,6 @@ struct Ctr {
     long long rhs, field, tri;
 };
 
-/* TParticle::derivs, src/particle.cpp:96-103 */
-template <bool MIXED>
-PTD void derivs(const Params &P, const Species &sp, const double *y, double *dydx, double t, Ctr &c) {
-    double B[3] = {0, 0, 0}, dB[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, E[3] = {0, 0, 0}, V = 0;
-    (void)t;
-    ++c.rhs;
-    if (sp.q != 0 || (sp.mu != 0 && y[7] != 0)) field_B<MIXED>(P, y[0], y[1], y[2], B, dB, true, c.field);
-    if (sp.q != 0) field_E<MIXED>(P, y[0], y[1], y[2], &V, E, c.field);
-    eom(sp, y, B, dB, E, dydx);
-}
-
 /* TParticle::GetKineticEnergy, src/particle.cpp:237-251 */
 PTD double kinetic_energy(const Species &sp, const double *v) {
     double beta2 = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / C_0 / C_0;
@@ -323,19 +348,22 @@ PTD double kinetic_energy(const Species &sp, const double *v) {
     return C_0 * C_0 * sp.m * gm1;
 }
 
-/* TParticle::GetPotentialEnergy, src/particle.cpp:254-269 */
+/* TParticle::GetPotentialEnergy, src/particle.cpp:254-269; Bc/Vc: field values already
+   known at this point (from the equation-of-motion evaluation), or nullptr */
 template <bool MIXED>
-PTD_NOINLINE double potential_energy(const Params &P, const Species &sp, double t, const double *y, Ctr &c) {
+PTD double potential_energy(const Params &P, const Species &sp, const double *y, const double *Bc, const double *Vc, Ctr &c) {
     double result = 0;
-    (void)t;
     if (sp.q != 0 || sp.mu != 0) {
-        double B[3], dB[3][3], E[3], V;
         if (sp.mu != 0) {
-            field_B<MIXED>(P, y[0], y[1], y[2], B, dB, false, c.field);
+            double B[3], dB[3][3];
+            if (Bc) { B[0] = Bc[0]; B[1] = Bc[1]; B[2] = Bc[2]; }
+            else field_B<MIXED>(P, y[0], y[1], y[2], B, dB, false, c.field);
             result += -y[7] * sp.mu / ELE_E * __builtin_sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
         }
         if (sp.q != 0) {
-            field_E<MIXED>(P, y[0], y[1], y[2], &V, E, c.field);
+            double E[3], V;
+            if (Vc) V = *Vc;
+            else field_E<MIXED>(P, y[0], y[1], y[2], &V, E, c.field);
             result += sp.q / ELE_E * V;
         }
     }
@@ -347,98 +375,14 @@ PTD_NOINLINE double potential_energy(const Params &P, const Species &sp, double 
 /* Boost.Odeint dense-output dopri5 (runge_kutta_dopri5 + controlled + dense) */
 /* ------------------------------------------------------------------------ */
 
+/* dense-output state of the collision path (all nine state components) */
 struct Dopri {
-    double xc[9], dxc[9];            /* current state / derivative (FSAL) */
-    double xo[9], dxo[9];            /* previous state / derivative */
+    double xc[9], dxc[9];
+    double xo[9], dxo[9];
     double k3[9], k4[9], k5[9], k6[9];
     double t, t_old, dt;
     bool deriv_init;
 };
-
-/* returns 1 accepted, 0 rejected (stepper state updated like try_step) */
-template <bool MIXED>
-PTD int dopri_try(const Params &P, const Species &sp, Dopri &s, double xnew[9], double dxnew[9], Ctr &c) {
-    constexpr double a2 = 1.0 / 5, a3 = 3.0 / 10, a4 = 4.0 / 5, a5 = 8.0 / 9;
-    constexpr double b21 = 1.0 / 5;
-    constexpr double b31 = 3.0 / 40, b32 = 9.0 / 40;
-    constexpr double b41 = 44.0 / 45, b42 = -56.0 / 15, b43 = 32.0 / 9;
-    constexpr double b51 = 19372.0 / 6561, b52 = -25360.0 / 2187, b53 = 64448.0 / 6561, b54 = -212.0 / 729;
-    constexpr double b61 = 9017.0 / 3168, b62 = -355.0 / 33, b63 = 46732.0 / 5247, b64 = 49.0 / 176, b65 = -5103.0 / 18656;
-    constexpr double c1 = 35.0 / 384, c3 = 500.0 / 1113, c4 = 125.0 / 192, c5 = -2187.0 / 6784, c6 = 11.0 / 84;
-    constexpr double dc1 = c1 - 5179.0 / 57600, dc3 = c3 - 7571.0 / 16695, dc4 = c4 - 393.0 / 640,
-                     dc5 = c5 - (-92097.0 / 339200), dc6 = c6 - 187.0 / 2100, dc7 = -1.0 / 40;
-    const double dt = s.dt, t = s.t;
-    double tmp[9], k2[9];
-    const double h21 = dt * b21;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) tmp[i] = s.xc[i] + h21 * s.dxc[i];
-    derivs<MIXED>(P, sp, tmp, k2, t + dt * a2, c);
-    const double h31 = dt * b31, h32 = dt * b32;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) tmp[i] = s.xc[i] + h31 * s.dxc[i] + h32 * k2[i];
-    derivs<MIXED>(P, sp, tmp, s.k3, t + dt * a3, c);
-    const double h41 = dt * b41, h42 = dt * b42, h43 = dt * b43;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) tmp[i] = s.xc[i] + h41 * s.dxc[i] + h42 * k2[i] + h43 * s.k3[i];
-    derivs<MIXED>(P, sp, tmp, s.k4, t + dt * a4, c);
-    const double h51 = dt * b51, h52 = dt * b52, h53 = dt * b53, h54 = dt * b54;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) tmp[i] = s.xc[i] + h51 * s.dxc[i] + h52 * k2[i] + h53 * s.k3[i] + h54 * s.k4[i];
-    derivs<MIXED>(P, sp, tmp, s.k5, t + dt * a5, c);
-    const double h61 = dt * b61, h62 = dt * b62, h63 = dt * b63, h64 = dt * b64, h65 = dt * b65;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) tmp[i] = s.xc[i] + h61 * s.dxc[i] + h62 * k2[i] + h63 * s.k3[i] + h64 * s.k4[i] + h65 * s.k5[i];
-    derivs<MIXED>(P, sp, tmp, s.k6, t + dt, c);
-    const double g1 = dt * c1, g3 = dt * c3, g4 = dt * c4, g5 = dt * c5, g6 = dt * c6;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) xnew[i] = s.xc[i] + g1 * s.dxc[i] + g3 * s.k3[i] + g4 * s.k4[i] + g5 * s.k5[i] + g6 * s.k6[i];
-    derivs<MIXED>(P, sp, xnew, dxnew, t + dt, c);
-    const double e1 = dt * dc1, e3 = dt * dc3, e4 = dt * dc4, e5 = dt * dc5, e6 = dt * dc6, e7 = dt * dc7;
-    double err = 0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        double xe = e1 * s.dxc[i] + e3 * s.k3[i] + e4 * s.k4[i] + e5 * s.k5[i] + e6 * s.k6[i] + e7 * dxnew[i];
-        double e = dabs(xe) / (1e-9 + 1e-9 * (dabs(s.xc[i]) + dt * dabs(s.dxc[i])));
-        err = (err < e) ? e : err;
-    }
-    if (err > 1.0) {
-        double f = 9.0 / 10 * ptm_pow(err, -1.0 / 3);
-        double m = 1.0 / 5;
-        s.dt = dt * ((f < m) ? m : f);
-        return 0;
-    }
-    s.t_old = t;
-    s.t = t + dt;
-    if (err < 0.5) {
-        double emin = ptm_pow(5.0, -5.0);
-        err = (emin < err) ? err : emin;
-        s.dt = dt * (9.0 / 10 * ptm_pow(err, -1.0 / 5));
-    }
-    return 1;
-}
-
-/* dense_output_runge_kutta::do_step; returns false when the failed-step checker throws */
-template <bool MIXED>
-PTD bool dopri_do_step(const Params &P, const Species &sp, Dopri &s, long long &nacc, long long &nrej, Ctr &c) {
-    if (!s.deriv_init) {
-        derivs<MIXED>(P, sp, s.xc, s.dxc, s.t, c);
-        s.deriv_init = true;
-    }
-    int tries = 0;
-    for (;;) {
-        double xnew[9], dxnew[9];
-        int ok = dopri_try<MIXED>(P, sp, s, xnew, dxnew, c);
-        if (ok) {
-#pragma unroll
-            for (int i = 0; i < 9; ++i) { s.xo[i] = s.xc[i]; s.dxo[i] = s.dxc[i]; s.xc[i] = xnew[i]; s.dxc[i] = dxnew[i]; }
-            ++nacc;
-        } else {
-            ++nrej;
-        }
-        if (tries++ >= 500) return false;
-        if (ok) return true;
-    }
-}
 
 /* runge_kutta_dopri5::calc_state */
 PTD void dopri_calc_state(const Dopri &s, double t, double *x) {
@@ -468,9 +412,69 @@ PTD void dopri_calc_state(const Dopri &s, double t, double *x) {
         x[i] = s.xo[i] + h1 * s.dxo[i] + h3 * s.k3[i] + h4 * s.k4[i] + h5 * s.k5[i] + h6 * s.k6[i] + h7 * s.dxc[i];
 }
 
-/* ------------------------------------------------------------------------ */
-/* tracker                                                                   */
-/* ------------------------------------------------------------------------ */
+PTD void to9(const double *s, double pol, double *y) {
+    y[0] = s[0]; y[1] = s[1]; y[2] = s[2]; y[3] = s[3]; y[4] = s[4]; y[5] = s[5];
+    y[6] = s[6]; y[7] = pol; y[8] = s[7];
+}
+
+/* TParticle::derivs (src/particle.cpp:96-103) on the eight stepped components
+   (x, y, z, vx, vy, vz, proper time, path length); the polarisation, whose
+   derivative is identically +0.0, is passed separately.  B and V return the
+   field the equation of motion used (hasB: B was evaluated). */
+template <bool MIXED>
+PTD void derivs8(const Params &P, const Species &sp, const double *s, double pol, double *d, double B[3], double &V,
+                 bool &hasB, Ctr &c) {
+    double y[9];
+    to9(s, pol, y);
+    double dB[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, E[3] = {0, 0, 0};
+    B[0] = 0; B[1] = 0; B[2] = 0; V = 0;
+    ++c.rhs;
+    hasB = (sp.q != 0 || (sp.mu != 0 && pol != 0));
+    if (hasB) field_B<MIXED>(P, y[0], y[1], y[2], B, dB, true, c.field);
+    if (sp.q != 0) field_E<MIXED>(P, y[0], y[1], y[2], &V, E, c.field);
+    double d9[9];
+    eom(sp, y, B, dB, E, d9);
+    d[0] = d9[0]; d[1] = d9[1]; d[2] = d9[2]; d[3] = d9[3]; d[4] = d9[4]; d[5] = d9[5]; d[6] = d9[6]; d[7] = d9[8];
+}
+
+/* runge_kutta_dopri5::calc_state from the dense-output data in HBM */
+/* dense-output data of particle i in HBM: [59][n] = xo dxo k3 k4 k5 k6 k7 (8 each), t, t_old, pol */
+struct DenseRef { const Params *P; int i; };
+PTD void dense_calc(const DenseRef &r, double t, double *y) {
+    const double *ds = r.P->st.dense;
+    const size_t n = (size_t)r.P->st.n;
+    const int i = r.i;
+    constexpr double b1 = 35.0 / 384, b3 = 500.0 / 1113, b4 = 125.0 / 192, b5 = -2187.0 / 6784, b6 = 11.0 / 84;
+    const double st_t = ds[56 * n + i], st_told = ds[57 * n + i], polo = ds[58 * n + i];
+    const double dt = (st_t - st_told);
+    const double theta = (t - st_told) / dt;
+    const double X1 = 5.0 * (2558722523.0 - 31403016.0 * theta) / 11282082432.0;
+    const double X3 = 100.0 * (882725551.0 - 15701508.0 * theta) / 32700410799.0;
+    const double X4 = 25.0 * (443332067.0 - 31403016.0 * theta) / 1880347072.0;
+    const double X5 = 32805.0 * (23143187.0 - 3489224.0 * theta) / 199316789632.0;
+    const double X6 = 55.0 * (29972135.0 - 7076736.0 * theta) / 822651844.0;
+    const double X7 = 10.0 * (7414447.0 - 829305.0 * theta) / 29380423.0;
+    const double theta_m_1 = theta - 1.0;
+    const double theta_sq = theta * theta;
+    const double A = theta_sq * (3.0 - 2.0 * theta);
+    const double B = theta_sq * theta_m_1;
+    const double C = theta_sq * theta_m_1 * theta_m_1;
+    const double D = theta * theta_m_1 * theta_m_1;
+    const double h1 = dt * (A * b1 - C * X1 + D);
+    const double h3 = dt * (A * b3 + C * X3);
+    const double h4 = dt * (A * b4 - C * X4);
+    const double h5 = dt * (A * b5 + C * X5);
+    const double h6 = dt * (A * b6 - C * X6);
+    const double h7 = dt * (B + C * X7);
+    double x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        x[k] = ds[k * n + i] + h1 * ds[(8 + k) * n + i] + h3 * ds[(16 + k) * n + i] + h4 * ds[(24 + k) * n + i] +
+               h5 * ds[(32 + k) * n + i] + h6 * ds[(40 + k) * n + i] + h7 * ds[(48 + k) * n + i];
+    to9(x, polo + h1 * 0.0 + h3 * 0.0 + h4 * 0.0 + h5 * 0.0 + h6 * 0.0 + h7 * 0.0, y);
+}
+
+PTD void dense_calc(const Dopri &s, double t, double *y) { dopri_calc_state(s, t, y); }
 
 struct Track {
     int i;                      /* particle index */
@@ -501,7 +505,7 @@ PTD bool state_eq(const double *a, const double *b) {
 }
 
 /* RotateVector, src/globals.cpp:40-64 */
-PTD_NOINLINE void rotate_vector(double v[3], const double n[3], const double *x) {
+PTD void rotate_vector(double v[3], const double n[3], const double *x) {
     double xv[3], zv[3] = {n[0], n[1], n[2]};
     double pa = zv[0], pb = zv[1], pc = zv[2];
     double pd = -zv[0] * 0.0 - zv[1] * 0.0 - zv[2] * 0.0;
@@ -535,6 +539,15 @@ PTD_NOINLINE void rotate_vector(double v[3], const double n[3], const double *x)
     v[0] = xv[0] * v0 + yv[0] * v1 + zv[0] * v2;
     v[1] = xv[1] * v0 + yv[1] * v1 + zv[1] * v2;
     v[2] = xv[2] * v0 + yv[2] * v1 + zv[2] * v2;
+}
+
+struct V3 { double x, y, z; };
+/* RotateVector with values in and out (no address-taken arrays on the step path) */
+PTD_NOINLINE V3 rotate_vector_v(V3 v, V3 n) {
+    double a[3] = {v.x, v.y, v.z}, nn[3] = {n.x, n.y, n.z};
+    rotate_vector(a, nn, nullptr);
+    V3 r = {a[0], a[1], a[2]};
+    return r;
 }
 
 /* neutron wall interaction, TNeutron::OnHit and helpers (src/neutron.cpp:25-264) */
@@ -647,7 +660,8 @@ PTD_NOINLINE void neutron_onhit(const Params &P, Track &T, double x1, const doub
 }
 
 /* OnStep hooks (src/neutron.cpp:267-282, src/proton.cpp:25-33, src/electron.cpp:25-33) */
-PTD_NOINLINE void on_step(const Params &P, Track &T, double x1, const double *y1, double &x2, double *y2, const Dopri &st, int cursolid) {
+template <class DS>
+PTD void on_step(const Params &P, Track &T, double x1, const double *y1, double &x2, double *y2, const DS &st, int cursolid) {
     if (T.sp.kind == 0) {
         const double *mat = solid_mat(P.geom, cursolid);
         if (mat[1] > 0) {
@@ -662,7 +676,7 @@ PTD_NOINLINE void on_step(const Params &P, Track &T, double x1, const double *y1
             double abspath = -ptm_log(1.0 - ptm_canonical(ptm_draw_u64(T.key, &T.ctr))) / (2 * ki);
             if (abspath < l) {
                 x2 = x1 + abspath / l * (x2 - x1);
-                dopri_calc_state(st, x2, y2);
+                dense_calc(st, x2, y2);
                 T.stop = ID_ABSORBED_IN_MATERIAL;
             }
         }
@@ -681,7 +695,7 @@ PTD_NOINLINE bool do_step_hook(const Params &P, Track &T, double x1, const doubl
     double x2temp = x2, y2temp[9];
     for (int k = 0; k < 9; ++k) y2temp[k] = y2[k];
     on_step(P, T, x1, y1, x2, y2, st, cursolid);
-    double H = kinetic_energy(T.sp, y2 + 3) + potential_energy<MIXED>(P, T.sp, x2, y2, T.c);
+    double H = kinetic_energy(T.sp, y2 + 3) + potential_energy<MIXED>(P, T.sp, y2, nullptr, nullptr, T.c);
     T.Hmax = (H < T.Hmax) ? T.Hmax : H;
     if (y2temp[7] != y2[7]) T.Nspinflip++;
     T.Nstep++;
@@ -773,8 +787,11 @@ PTD_NOINLINE bool do_hit(const Params &P, Track &T, double x1, const double *y1,
 }
 
 /* number of intersections of segment p1->p2 (-1 coplanar, -2 overflow) */
-PTD int count_hits(const Params &P, Track &T, const double *p1, const double *p2) {
-    return seg_query(P.geom, p1, p2, nullptr, &T.c.tri);
+PTD int count_hits(const Params &P, Ctr &c, const double *p1, const double *p2) {
+    if (clear_of_walls(P.geom, p1, p2)) return 0;
+    int2 r = seg_query_v(P.geom, p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], nullptr);
+    c.tri += r.y;
+    return r.x;
 }
 
 /*
@@ -803,7 +820,7 @@ PTD_NOINLINE int iterate_collision(const Params &P, Track &T, const Dopri &st, d
             mid = lo + (hi - lo) * 0.5;
             dopri_calc_state(st, mid, ymid);
             pmid[0] = ymid[0]; pmid[1] = ymid[1]; pmid[2] = ymid[2];
-            int n1 = count_hits(P, T, plo, pmid);
+            int n1 = count_hits(P, T.c, plo, pmid);
             if (n1 < 0) return -1;
             if (n1 > 0) {
                 ph[d >> 6] &= ~(1ull << (d & 63));
@@ -813,7 +830,7 @@ PTD_NOINLINE int iterate_collision(const Params &P, Track &T, const Dopri &st, d
             }
         }
         branch = 0;
-        int n2 = count_hits(P, T, pmid, phi);
+        int n2 = count_hits(P, T.c, pmid, phi);
         if (n2 < 0) return -1;
         if (n2 > 0) {
             ph[d >> 6] |= (1ull << (d & 63));
@@ -863,7 +880,7 @@ PTD_NOINLINE bool check_hit(const Params &P, Track &T, double x1, const double *
         if (!check_segment(P.geom, y1, y2)) { T.stop = ID_HIT_BOUNDARIES; return true; }
         if (x2 == x1) return false;
         int cursolid = current_solid(P.geom, T.solids, T.nsolid);
-        int n = count_hits(P, T, y1, y2);
+        int n = count_hits(P, T.c, y1, y2);
         if (n < 0) { T.stop = n == -1 ? ID_CGAL_ERROR : ID_UNSUPPORTED; return true; }
         if (n == 0) return do_step_hook<MIXED>(P, T, x1, y1, x2, y2, st, cursolid);
         double xc1 = x1, xc2 = x2, yc1[9], yc2[9];
@@ -936,142 +953,546 @@ PTD_NOINLINE double integrate_spin(const Params &P, Track &T, double *spin, cons
     return y1[7];
 }
 
+/* ------------------------------------------------------------------------ */
+/* resumable state, step path and collision path                             */
+/* ------------------------------------------------------------------------ */
+
 /* flags bits of the resumable state */
-constexpr int FL_STARTED = 1, FL_DERIV = 2, FL_RESET = 4;
+/* FL_DETACHED: the stepper's state (S.xc, S.ts) is not the tracker's (x, y); this
+   happens after a decay truncation that does not stop the particle, when the
+   stepper goes on from its own end of step (src/tracking.cpp:86-89) */
+constexpr int FL_STARTED = 1, FL_DERIV = 2, FL_RESET = 4, FL_BVALID = 8, FL_DETACHED = 16;
 
 /*
- * Advance particle i by up to max_steps accepted steps of
- * TTracker::IntegrateParticle's loop (src/tracking.cpp:71-148).
- * Returns true when the particle has stopped.
+ * Per-lane state of the step path, in registers.  y[0..6] and y[8] double as the
+ * stepper's current state xc unless FL_RESET is set (then the next step restarts
+ * from y, as stepper.initialize does); pol is the stepper's own polarisation.
+ * Bc/Vc: the field of the FSAL evaluation at xc.  Statistics that change rarely
+ * (spin, no-flip probability, random-stream counter) stay in HBM.
+ */
+struct Lane {
+    int sidx, stop, flags, nsolid, Nstep;
+    long long nacc, nrej;
+    unsigned long long solids;
+    double x, dt, pol, tau, Hmax;
+    double y[9], dxc[8];
+    double Bc[3], Vc;
+};
+
+PTD void load_lane(const Params &P, int i, Lane &L) {
+    const State &S = P.st;
+    const size_t n = (size_t)S.n;
+    L.sidx = S.species[i];
+    L.stop = S.stop[i]; L.flags = S.flags[i]; L.nsolid = S.nsolid[i];
+    L.Nstep = S.Nstep[i];
+    L.nacc = S.n_acc[i]; L.nrej = S.n_rej[i];
+    L.solids = S.solids[i];
+    L.x = S.t[i]; L.dt = S.dt[i]; L.tau = S.tau[i]; L.Hmax = S.Hmax[i];
+    L.pol = S.xc[7 * n + i];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) L.y[k] = S.y[k * n + i];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) L.dxc[k] = S.dxc[k * n + i];
+    L.dxc[6] = S.dxc[6 * n + i];
+    L.dxc[7] = S.dxc[8 * n + i];
+    L.Bc[0] = S.Bc[i]; L.Bc[1] = S.Bc[n + i]; L.Bc[2] = S.Bc[2 * n + i];
+    L.Vc = S.Vc[i];
+}
+
+/* write the lane back (the stepper's xc is y with the stepper's polarisation) */
+PTD void store_lane(const Params &P, int i, const Lane &L) {
+    const State &S = P.st;
+    const size_t n = (size_t)S.n;
+    S.stop[i] = L.stop; S.flags[i] = L.flags; S.nsolid[i] = L.nsolid;
+    S.Nstep[i] = L.Nstep;
+    S.n_acc[i] = L.nacc; S.n_rej[i] = L.nrej;
+    S.solids[i] = L.solids;
+    S.t[i] = L.x; S.dt[i] = L.dt; S.tau[i] = L.tau; S.Hmax[i] = L.Hmax;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) S.y[k * n + i] = L.y[k];
+    if (!(L.flags & FL_DETACHED)) {
+        S.ts[i] = L.x;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) S.xc[k * n + i] = (k == 7) ? L.pol : L.y[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.dxc[k * n + i] = L.dxc[k];
+    S.dxc[6 * n + i] = L.dxc[6];
+    S.dxc[7 * n + i] = 0.0;
+    S.dxc[8 * n + i] = L.dxc[7];
+    S.Bc[i] = L.Bc[0]; S.Bc[n + i] = L.Bc[1]; S.Bc[2 * n + i] = L.Bc[2];
+    S.Vc[i] = L.Vc;
+}
+
+/* OnStep of the step path (src/neutron.cpp:267-282, src/proton.cpp:25-33,
+   src/electron.cpp:25-33); random draws come from the particle's stream in HBM */
+PTD void on_step_A(const Params &P, const Species &sp, int i, int &stop, double x1, const double *y1, double &x2,
+                   double *y2, const DenseRef &dref, int cursolid) {
+    if (sp.kind == 0) {
+        const double *mat = solid_mat(P.geom, cursolid);
+        if (mat[1] > 0) {
+            double mn = sp.m, e = ELE_E, hb = HBAR;
+            double Er = 0.5 * mn * (y1[3] * y1[3] + y1[4] * y1[4] + y1[5] * y1[5]);
+            double Ei = mat[1] * 1e-9;
+            double kr, ki;
+            ptm_csqrt((2 * mn) * Er, (2 * mn) * Ei, &kr, &ki);
+            ki = ki * e;
+            ki = ki / hb;
+            double l = __builtin_sqrt((y2[0] - y1[0]) * (y2[0] - y1[0]) + (y2[1] - y1[1]) * (y2[1] - y1[1]) + (y2[2] - y1[2]) * (y2[2] - y1[2]));
+            uint64_t key = P.st.key[i], ctr = P.st.ctr[i];
+            double abspath = -ptm_log(1.0 - ptm_canonical(ptm_draw_u64(key, &ctr))) / (2 * ki);
+            P.st.ctr[i] = ctr;
+            if (abspath < l) {
+                x2 = x1 + abspath / l * (x2 - x1);
+                dense_calc(dref, x2, y2);
+                stop = ID_ABSORBED_IN_MATERIAL;
+            }
+        }
+    } else {
+        if (solid_ID(P.geom, cursolid) > 1) {
+            x2 = x1;
+            for (int k = 0; k < 9; ++k) y2[k] = y1[k];
+            stop = ID_ABSORBED_IN_MATERIAL;
+        }
+    }
+}
+
+/* TParticle::DoStep on the step path (src/particle.cpp:196-206); at_end: (x2, y2) is the
+   unmodified end of the step, where the last equation-of-motion evaluation left the field */
+template <bool MIXED>
+PTD bool do_step_hook_A(const Params &P, const Species &sp, Lane &L, int i, const DenseRef &dref, double x1,
+                        const double *y1, double &x2, double *y2, int cursolid, bool at_end, Ctr &c) {
+    double x2temp = x2, y2temp[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) y2temp[k] = y2[k];
+    on_step_A(P, sp, i, L.stop, x1, y1, x2, y2, dref, cursolid);
+    bool changed = !(x2temp == x2 && state_eq(y2temp, y2));
+    bool cached = at_end && !changed && (L.flags & FL_BVALID);
+    double H = kinetic_energy(sp, y2 + 3) +
+               potential_energy<MIXED>(P, sp, y2, cached ? L.Bc : nullptr, cached ? &L.Vc : nullptr, c);
+    L.Hmax = (H < L.Hmax) ? L.Hmax : H;
+    if (y2temp[7] != y2[7]) P.st.Nspinflip[i] += 1;
+    L.Nstep++;
+    return changed;
+}
+
+/* TTracker::IntegrateSpin without the BMT branch (src/tracking.cpp:373-394, 473-483) and
+   TParticle::DoPolarize (src/particle.cpp:225-234) on the step path.  B1/B2: the field at the
+   step start / end when the stepper evaluated it there (else nullptr).  Returns the new
+   tracker polarisation in y[7]. */
+template <bool MIXED>
+PTD void spin_step(const Params &P, int i, const Species &sp, double x1, double ys0, double ys1, double ys2,
+                            double polo, double x2, double *y, bool haveB1, double B10, double B11, double B12,
+                            bool haveB2, double B20, double B21, double B22, Ctr &c) {
+    const State &S = P.st;
+    const size_t n = (size_t)S.n;
+    double B1[3], B2[3], dB[3][3];
+    if (haveB1) { B1[0] = B10; B1[1] = B11; B1[2] = B12; }
+    else field_B<MIXED>(P, ys0, ys1, ys2, B1, dB, false, c.field);
+    if (haveB2) { B2[0] = B20; B2[1] = B21; B2[2] = B22; }
+    else field_B<MIXED>(P, y[0], y[1], y[2], B2, dB, false, c.field);
+    double Babs1 = __builtin_sqrt(B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]);
+    double Babs2 = __builtin_sqrt(B2[0] * B2[0] + B2[1] * B2[1] + B2[2] * B2[2]);
+    double spin[5];
+    for (int k = 0; k < 5; ++k) spin[k] = S.spin[k * n + i];
+    const double spin_in[3] = {spin[0], spin[1], spin[2]};
+    uint64_t key = S.key[i], ctr = S.ctr[i];
+    double spinProjection;
+    bool done = false;
+    if (sp.gamma == 0 || Babs2 == 0) {
+        for (int k = 0; k < 5; ++k) spin[k] = 0;
+        spinProjection = polo;
+        done = true;
+    } else if (Babs1 == 0 && Babs2 > 0) {
+        int pr = ptm_uniform_int01(key, &ctr) * 2 - 1;
+        for (int k = 0; k < 5; ++k) spin[k] = 0;
+        spinProjection = pr;
+        done = true;
+    } else if ((x2 > x1) && (Babs1 < sp.Bmax || Babs2 < sp.Bmax)) {
+        if (spin[0] != 0 || spin[1] != 0 || spin[2] != 0) {
+            double pr = (spin[0] * B1[0] + spin[1] * B1[1] + spin[2] * B1[2]) / Babs1 / __builtin_sqrt(spin[0] * spin[0] + spin[1] * spin[1] + spin[2] * spin[2]);
+            if (pr * pr >= 1) { spin[0] = 0; spin[1] = 0; }
+            else {
+                double spinaz = ptm_canonical(ptm_draw_u64(key, &ctr)) * (2. * PI - 0) + 0;
+                spin[0] = __builtin_sqrt(1 - pr * pr) * ptm_sin(spinaz);
+                spin[1] = __builtin_sqrt(1 - pr * pr) * ptm_cos(spinaz);
+            }
+            spin[2] = pr;
+            rotate_vector(spin, B2, nullptr);
+        }
+    }
+    if (!done) {
+        if (Babs2 > sp.Bmax && (spin[0] != 0 || spin[1] != 0 || spin[2] != 0)) {
+            double pr = (spin[0] * B2[0] + spin[1] * B2[1] + spin[2] * B2[2]) / Babs2 / __builtin_sqrt(spin[0] * spin[0] + spin[1] * spin[1] + spin[2] * spin[2]);
+            for (int k = 0; k < 5; ++k) spin[k] = 0;
+            spinProjection = pr;
+        } else {
+            spinProjection = polo;
+        }
+    }
+    /* src/tracking.cpp:128-138 */
+    bool spinHasFlipped = polo != y[7];
+    if (spinProjection != polo) {
+        double flipprob = 0.5 * (1 - y[7] * spinProjection);
+        S.noflip[i] *= 1. - flipprob;
+        if (sp.flipspin) {
+            double prevpol = y[7];
+            y[7] = (ptm_canonical(ptm_draw_u64(key, &ctr)) < 0.5 * (spinProjection + 1)) ? 1 : -1;
+            if (y[7] != prevpol) S.Nspinflip[i] += 1;
+            if (spinHasFlipped) y[7] *= -1;
+        }
+    } else if (spinHasFlipped) {
+        for (int k = 0; k < 5; ++k) spin[k] = 0;
+    }
+    S.ctr[i] = ctr;
+    for (int k = 0; k < 5; ++k) S.spin[k * n + i] = spin[k];
+    (void)spin_in;
+}
+
+/*
+ * One step of TTracker::IntegrateParticle (src/tracking.cpp:71-148) on the step path.
+ * The dopri5 trial keeps its stage vectors in registers; the dense output is written to
+ * HBM only when the step can need it (truncation, chord splitting, bulk absorption, or
+ * a chord that comes near a wall).  Returns pending = true, with the chord piece in HBM,
+ * when a chord piece intersects the mesh; the collision path then finishes the step.
  */
 template <bool MIXED>
-PTD bool track_particle(const Params &P, int i, int max_steps, long long *acc_out) {
+PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
+    const Geom &G = P.geom;
+    const Species &sp = P.species[L.sidx];
+    const double tmax = P.tmax_global;
+    const size_t n = (size_t)P.st.n;
+    double *ds = P.st.dense;
+    if (L.flags & FL_RESET) {
+        /* stepper.initialize(y, x, current_time_step()) (src/tracking.cpp:72-74) */
+        L.pol = L.y[7];
+        L.flags &= ~(FL_DERIV | FL_DETACHED);
+    }
+    /* the stepper starts from its own state: the tracker's unless detached */
+    double xc[8], ts;
+    if (L.flags & FL_DETACHED) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) xc[k] = P.st.xc[(k < 7 ? k : 8) * n + i];
+        ts = P.st.ts[i];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) xc[k] = L.y[k < 7 ? k : 8];
+        ts = L.x;
+    }
+    if (!(L.flags & FL_DERIV)) {
+        bool hb;
+        derivs8<MIXED>(P, sp, xc, L.pol, L.dxc, L.Bc, L.Vc, hb, c);
+        L.flags = (L.flags | FL_DERIV);
+        L.flags = hb ? (L.flags | FL_BVALID) : (L.flags & ~FL_BVALID);
+    }
+    const int cursolid = current_solid(G, L.solids, L.nsolid);
+    const bool absorb = sp.kind == 0 && solid_mat(G, cursolid)[1] > 0;
+    const double x1 = L.x;
+
+    /* ---- dense_output_runge_kutta::do_step (controlled dopri5, abs = rel = 1e-9) ---- */
+    constexpr double b21 = 1.0 / 5;
+    constexpr double b31 = 3.0 / 40, b32 = 9.0 / 40;
+    constexpr double b41 = 44.0 / 45, b42 = -56.0 / 15, b43 = 32.0 / 9;
+    constexpr double b51 = 19372.0 / 6561, b52 = -25360.0 / 2187, b53 = 64448.0 / 6561, b54 = -212.0 / 729;
+    constexpr double b61 = 9017.0 / 3168, b62 = -355.0 / 33, b63 = 46732.0 / 5247, b64 = 49.0 / 176, b65 = -5103.0 / 18656;
+    constexpr double c1 = 35.0 / 384, c3 = 500.0 / 1113, c4 = 125.0 / 192, c5 = -2187.0 / 6784, c6 = 11.0 / 84;
+    constexpr double dc1 = c1 - 5179.0 / 57600, dc3 = c3 - 7571.0 / 16695, dc4 = c4 - 393.0 / 640,
+                     dc5 = c5 - (-92097.0 / 339200), dc6 = c6 - 187.0 / 2100, dc7 = -1.0 / 40;
+    double xnew[8], k7[8], Bn[3], Vn, polnew, dt_used;
+    bool bn, need_dense;
+    int tries = 0;
+    for (;;) {
+        const double dt = L.dt;
+        double tmp[8], k2[8], k3[8], k4[8], k5[8], k6[8], Bd[3], Vd;
+        bool bd;
+        const double h21 = dt * b21;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h21 * L.dxc[k];
+        derivs8<MIXED>(P, sp, tmp, L.pol, k2, Bd, Vd, bd, c);
+        const double h31 = dt * b31, h32 = dt * b32;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h31 * L.dxc[k] + h32 * k2[k];
+        derivs8<MIXED>(P, sp, tmp, L.pol, k3, Bd, Vd, bd, c);
+        const double h41 = dt * b41, h42 = dt * b42, h43 = dt * b43;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h41 * L.dxc[k] + h42 * k2[k] + h43 * k3[k];
+        derivs8<MIXED>(P, sp, tmp, L.pol, k4, Bd, Vd, bd, c);
+        const double h51 = dt * b51, h52 = dt * b52, h53 = dt * b53, h54 = dt * b54;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h51 * L.dxc[k] + h52 * k2[k] + h53 * k3[k] + h54 * k4[k];
+        derivs8<MIXED>(P, sp, tmp, L.pol, k5, Bd, Vd, bd, c);
+        const double h61 = dt * b61, h62 = dt * b62, h63 = dt * b63, h64 = dt * b64, h65 = dt * b65;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h61 * L.dxc[k] + h62 * k2[k] + h63 * k3[k] + h64 * k4[k] + h65 * k5[k];
+        derivs8<MIXED>(P, sp, tmp, L.pol, k6, Bd, Vd, bd, c);
+        const double g1 = dt * c1, g3 = dt * c3, g4 = dt * c4, g5 = dt * c5, g6 = dt * c6;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) xnew[k] = xc[k] + g1 * L.dxc[k] + g3 * k3[k] + g4 * k4[k] + g5 * k5[k] + g6 * k6[k];
+        polnew = L.pol + g1 * 0.0 + g3 * 0.0 + g4 * 0.0 + g5 * 0.0 + g6 * 0.0;
+        const double e1 = dt * dc1, e3 = dt * dc3, e4 = dt * dc4, e5 = dt * dc5, e6 = dt * dc6, e7 = dt * dc7;
+        double errp[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) errp[k] = e1 * L.dxc[k] + e3 * k3[k] + e4 * k4[k] + e5 * k5[k] + e6 * k6[k];
+        /* can the rest of this step need the dense output? */
+        {
+            double l2 = (xnew[7] - xc[7]) * (xnew[7] - xc[7]);
+            double d2 = (xnew[0] - xc[0]) * (xnew[0] - xc[0]) + (xnew[1] - xc[1]) * (xnew[1] - xc[1]) + (xnew[2] - xc[2]) * (xnew[2] - xc[2]);
+            bool split = 0.25 * (l2 - d2) > MAX_TRACK_DEVIATION * MAX_TRACK_DEVIATION;
+            bool trunc = (L.tau > 0 && xnew[6] > L.tau) || (ts + dt > tmax);
+            /* detached: the chord starts at the tracker's point, not at xc */
+            need_dense = trunc || split || absorb || (L.flags & FL_DETACHED) || !clear_of_walls(G, xc, xnew);
+        }
+        if (need_dense) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                ds[k * n + i] = xc[k];
+                ds[(8 + k) * n + i] = L.dxc[k];
+                ds[(16 + k) * n + i] = k3[k];
+                ds[(24 + k) * n + i] = k4[k];
+                ds[(32 + k) * n + i] = k5[k];
+                ds[(40 + k) * n + i] = k6[k];
+            }
+        }
+        derivs8<MIXED>(P, sp, xnew, polnew, k7, Bn, Vn, bn, c);
+        double err = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            double xe = errp[k] + e7 * k7[k];
+            double e = dabs(xe) / (1e-9 + 1e-9 * (dabs(xc[k]) + dt * dabs(L.dxc[k])));
+            err = (err < e) ? e : err;
+        }
+        bool ok = !(err > 1.0);
+        if (!ok) {
+            double f = 9.0 / 10 * ptm_pow(err, -1.0 / 3);
+            double m = 1.0 / 5;
+            L.dt = dt * ((f < m) ? m : f);
+            ++L.nrej;
+        } else {
+            dt_used = dt;
+            if (err < 0.5) {
+                double emin = ptm_pow(5.0, -5.0);
+                err = (emin < err) ? err : emin;
+                L.dt = dt * (9.0 / 10 * ptm_pow(err, -1.0 / 5));
+            }
+            ++L.nacc;
+        }
+        if (tries++ >= 500) {
+            /* the failed-step checker throws; IntegrateParticle keeps x, y and stops (src/tracking.cpp:78-85) */
+            L.stop = ID_ODEINT_ERROR;
+            return;
+        }
+        if (ok) break;
+    }
+    const double polo = L.pol;
+    const double told = ts;
+    const double tnew = ts + dt_used;
+    if (need_dense) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ds[(48 + k) * n + i] = k7[k];
+        ds[56 * n + i] = tnew;
+        ds[57 * n + i] = told;
+        ds[58 * n + i] = polo;
+    }
+    /* accepted: the FSAL derivative and field move to the new state */
+    const bool b1v = (L.flags & FL_BVALID) != 0;
+    const double B1c[3] = {L.Bc[0], L.Bc[1], L.Bc[2]};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) L.dxc[k] = k7[k];
+    L.pol = polnew;
+    L.Bc[0] = Bn[0]; L.Bc[1] = Bn[1]; L.Bc[2] = Bn[2]; L.Vc = Vn;
+    L.flags = bn ? (L.flags | FL_BVALID) : (L.flags & ~FL_BVALID);
+
+    double x = tnew, y[9], y1[9];
+    to9(xnew, polnew, y);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) y1[k] = L.y[k];
+    const DenseRef dref = {&P, i};
+    bool at_end = true;
+    if (L.tau > 0 && y[6] > L.tau) {
+        x = x1 + (x - x1) * (L.tau - y1[6]) / (y[6] - y1[6]);
+        dense_calc(dref, x, y);
+        at_end = false;
+    }
+    if (x > tmax) {
+        x = tmax;
+        dense_calc(dref, x, y);
+        at_end = false;
+    }
+    if (!at_end) {
+        /* the tracker's end of step is interpolated; the stepper keeps its own */
+#pragma unroll
+        for (int k = 0; k < 9; ++k) P.st.xc[k * n + i] = (k == 7) ? polnew : xnew[k < 7 ? k : 7];
+        P.st.ts[i] = tnew;
+        L.flags |= FL_DETACHED;
+    } else {
+        L.flags &= ~FL_DETACHED;
+    }
+    bool reset = (L.flags & FL_RESET) != 0;
+    bool altered = false;
+    double xa = x1;
+    while (xa < x) {
+        double l2 = (y[8] - y1[8]) * (y[8] - y1[8]);
+        double d2 = (y[0] - y1[0]) * (y[0] - y1[0]) + (y[1] - y1[1]) * (y[1] - y1[1]) + (y[2] - y1[2]) * (y[2] - y1[2]);
+        double dev2 = 0.25 * (l2 - d2);
+        double x2 = x, y2[9];
+        bool piece_end = at_end;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) y2[k] = y[k];
+        if (dev2 > MAX_TRACK_DEVIATION * MAX_TRACK_DEVIATION) {
+            x2 = xa + (x - xa) / __builtin_ceil(__builtin_sqrt(dev2) / MAX_TRACK_DEVIATION);
+            dense_calc(dref, x2, y2);
+            piece_end = false;
+        }
+        bool r;
+        /* TTracker::CheckHit (src/tracking.cpp:176-226) up to the collision query */
+        if (!check_segment(G, y1, y2)) {
+            L.stop = ID_HIT_BOUNDARIES;
+            r = true;
+        } else if (x2 == xa) {
+            r = false;
+        } else {
+            int nh = count_hits(P, c, y1, y2);
+            if (nh < 0) {
+                L.stop = nh == -1 ? ID_CGAL_ERROR : ID_UNSUPPORTED;
+                r = true;
+            } else if (nh > 0) {
+                /* defer to the collision path: chord piece, step end and stepper state to HBM */
+                double *pd = P.st.pend;
+                pd[i] = xa;
+                pd[10 * n + i] = x2;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) { pd[(1 + k) * n + i] = y1[k]; pd[(11 + k) * n + i] = y2[k]; }
+                L.x = x;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) L.y[k] = y[k];
+                L.flags = reset ? (L.flags | FL_RESET) : (L.flags & ~FL_RESET);
+                pending = true;
+                return;
+            } else {
+                r = do_step_hook_A<MIXED>(P, sp, L, i, dref, xa, y1, x2, y2, cursolid, piece_end, c);
+            }
+        }
+        reset = r;
+        if (r) {
+            altered = true;
+            x = x2;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) y[k] = y2[k];
+        }
+        xa = x2;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) y1[k] = y2[k];
+    }
+    L.flags = reset ? (L.flags | FL_RESET) : (L.flags & ~FL_RESET);
+    const bool y_is_xc = at_end && !altered && (L.flags & FL_BVALID);
+    spin_step<MIXED>(P, i, sp, told, xc[0], xc[1], xc[2], polo, x, y, b1v, B1c[0], B1c[1], B1c[2],
+                     y_is_xc, L.Bc[0], L.Bc[1], L.Bc[2], c);
+    if (L.stop == ID_UNKNOWN && y[6] >= L.tau) L.stop = ID_DECAYED;
+    else if (L.stop == ID_UNKNOWN && (x >= tmax || y[8] >= sp.lmax)) L.stop = ID_NOT_FINISH;
+    L.x = x;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) L.y[k] = y[k];
+}
+
+/* stop checks at the end of a step (src/tracking.cpp:144-147) */
+PTD void finish_checks(Track &T, double x, const double *y, double tau, double tmax) {
+    if (T.stop == ID_UNKNOWN && y[6] >= tau) T.stop = ID_DECAYED;
+    else if (T.stop == ID_UNKNOWN && (x >= tmax || y[8] >= T.sp.lmax)) T.stop = ID_NOT_FINISH;
+}
+
+/*
+ * Collision path: finishes the step of particle i from the state in HBM, starting
+ * with the chord piece whose collision the step path deferred.  This is the rest of
+ * the loop body of TTracker::IntegrateParticle with CheckHit's recursion
+ * (src/tracking.cpp:96-147).
+ */
+template <bool MIXED>
+PTD void collision_path(const Params &P, int i, Ctr &c) {
     const State &S = P.st;
     const int n = S.n;
     Track T;
     T.i = i;
     T.sp = P.species[S.species[i]];
-    T.key = S.key[i];
-    T.ctr = S.ctr[i];
+    T.key = S.key[i]; T.ctr = S.ctr[i];
     T.stop = S.stop[i];
     T.Nhit = S.Nhit[i]; T.Nstep = S.Nstep[i]; T.Nspinflip = S.Nspinflip[i];
     T.nhitlog = S.nhitlog[i];
     T.nacc = S.n_acc[i]; T.nrej = S.n_rej[i];
     T.Hmax = S.Hmax[i]; T.noflip = S.noflip[i];
     T.solids = S.solids[i]; T.nsolid = S.nsolid[i];
-    T.c.rhs = 0; T.c.field = 0; T.c.tri = 0;
+    T.c = c;
     int flags = S.flags[i];
-    double x = S.t[i];
-    double y[9], spin[5];
-    Dopri st;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) { y[k] = S.y[k * n + i]; st.xc[k] = S.xc[k * n + i]; st.dxc[k] = S.dxc[k * n + i]; }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) spin[k] = S.spin[k * n + i];
-    st.t = S.ts[i];
-    st.dt = S.dt[i];
-    st.t_old = st.t;
-    st.deriv_init = (flags & FL_DERIV) != 0;
     double tau = S.tau[i];
     const double tmax = P.tmax_global;
-
-    if (!(flags & FL_STARTED)) {
-        /* TParticle ctor Hmax (src/particle.cpp:89) and IntegrateParticle set-up (src/tracking.cpp:19-68) */
-        T.Hmax = kinetic_energy(T.sp, y + 3) + potential_energy<MIXED>(P, T.sp, x, y, T.c);
-        if (T.sp.tau > 0) tau = -ptm_log(1.0 - ptm_canonical(ptm_draw_u64(T.key, &T.ctr))) / (1. / T.sp.tau);
-        else tau = T.sp.tmax;
-        if (x > tmax) {
-            T.stop = ID_INVARIANT;
-        } else {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) st.xc[k] = y[k];
-            st.t = x;
-            st.dt = 10. * MAX_TRACK_DEVIATION / __builtin_sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
-            st.deriv_init = false;
-            unsigned long long par = ray_parity(P.geom, y);
-            unsigned long long sk = sk_set(0ull, 0, -1, 0);
-            int ns = 1;
-            for (int m = 0; m < P.geom.n_solid; ++m) {
-                if ((par >> m) & 1ull) {
-                    if (ns >= MAX_SOLIDS) { T.stop = ID_UNSUPPORTED; break; }
-                    sk = sk_set(sk, ns, m, solid_ignored(P.geom, m, x));
-                    ++ns;
-                }
-            }
-            T.solids = sk; T.nsolid = ns;
-        }
-        flags = FL_STARTED;
+    Dopri st;
+    const double *ds = S.dense;
+    double xo8[8], dxo8[8], k38[8], k48[8], k58[8], k68[8], dxc8[8];
+    for (int k = 0; k < 8; ++k) {
+        xo8[k] = ds[(size_t)k * n + i];
+        dxo8[k] = ds[(size_t)(8 + k) * n + i];
+        k38[k] = ds[(size_t)(16 + k) * n + i];
+        k48[k] = ds[(size_t)(24 + k) * n + i];
+        k58[k] = ds[(size_t)(32 + k) * n + i];
+        k68[k] = ds[(size_t)(40 + k) * n + i];
+        dxc8[k] = ds[(size_t)(48 + k) * n + i];
     }
+    st.t = ds[(size_t)56 * n + i];
+    st.t_old = ds[(size_t)57 * n + i];
+    double polo = ds[(size_t)58 * n + i];
+    to9(xo8, polo, st.xo);
+    to9(dxo8, 0.0, st.dxo);
+    to9(k38, 0.0, st.k3);
+    to9(k48, 0.0, st.k4);
+    to9(k58, 0.0, st.k5);
+    to9(k68, 0.0, st.k6);
+    to9(dxc8, 0.0, st.dxc);
+    double x = S.t[i], y[9], spin[5], x1, y1[9], x2, y2[9];
+    for (int k = 0; k < 9; ++k) y[k] = S.y[(size_t)k * n + i];
+    for (int k = 0; k < 5; ++k) spin[k] = S.spin[(size_t)k * n + i];
+    const double *pd = S.pend;
+    x1 = pd[i];
+    x2 = pd[(size_t)10 * n + i];
+    for (int k = 0; k < 9; ++k) { y1[k] = pd[(size_t)(1 + k) * n + i]; y2[k] = pd[(size_t)(11 + k) * n + i]; }
 
-    int steps = 0;
-    while (T.stop == ID_UNKNOWN && (max_steps <= 0 || steps < max_steps)) {
-        if (flags & FL_RESET) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) st.xc[k] = y[k];
-            st.t = x;
-            st.deriv_init = false;
-        }
-        double x1 = x, y1[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) y1[k] = y[k];
-        if (!dopri_do_step<MIXED>(P, T.sp, st, T.nacc, T.nrej, T.c)) T.stop = ID_ODEINT_ERROR;
-        ++steps;
-        x = st.t;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) y[k] = st.xc[k];
-        if (tau > 0 && y[6] > tau) {
-            x = x1 + (x - x1) * (tau - y1[6]) / (y[6] - y1[6]);
-            dopri_calc_state(st, x, y);
-        }
-        if (x > tmax) {
-            x = tmax;
-            dopri_calc_state(st, x, y);
-        }
-        bool reset = (flags & FL_RESET) != 0;
-        while (x1 < x) {
-            double l2 = (y[8] - y1[8]) * (y[8] - y1[8]);
-            double d2 = (y[0] - y1[0]) * (y[0] - y1[0]) + (y[1] - y1[1]) * (y[1] - y1[1]) + (y[2] - y1[2]) * (y[2] - y1[2]);
-            double dev2 = 0.25 * (l2 - d2);
-            double x2 = x, y2[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) y2[k] = y[k];
-            if (dev2 > MAX_TRACK_DEVIATION * MAX_TRACK_DEVIATION) {
-                x2 = x1 + (x - x1) / __builtin_ceil(__builtin_sqrt(dev2) / MAX_TRACK_DEVIATION);
-                dopri_calc_state(st, x2, y2);
-            }
-            reset = check_hit<MIXED>(P, T, x1, y1, x2, y2, st);
-            if (reset) {
-                x = x2;
-#pragma unroll
-                for (int k = 0; k < 9; ++k) y[k] = y2[k];
-            }
-            x1 = x2;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) y1[k] = y2[k];
-        }
-        flags = reset ? (flags | FL_RESET) : (flags & ~FL_RESET);
-        double spinProjection = integrate_spin<MIXED>(P, T, spin, st, x, y);
-        bool spinHasFlipped = st.xo[7] != y[7];
-        if (spinProjection != st.xo[7]) {
-            do_polarize(T, y, spinProjection, T.sp.flipspin);
-            if (T.sp.flipspin && spinHasFlipped) y[7] *= -1;
-        } else if (spinHasFlipped) {
-            for (int k = 0; k < 5; ++k) spin[k] = 0;
-        }
-        if (T.stop == ID_UNKNOWN && y[6] >= tau) T.stop = ID_DECAYED;
-        else if (T.stop == ID_UNKNOWN && (x >= tmax || y[8] >= T.sp.lmax)) T.stop = ID_NOT_FINISH;
+    bool reset = check_hit<MIXED>(P, T, x1, y1, x2, y2, st);
+    if (reset) {
+        x = x2;
+        for (int k = 0; k < 9; ++k) y[k] = y2[k];
     }
+    x1 = x2;
+    for (int k = 0; k < 9; ++k) y1[k] = y2[k];
+    while (x1 < x) {
+        double l2 = (y[8] - y1[8]) * (y[8] - y1[8]);
+        double d2 = (y[0] - y1[0]) * (y[0] - y1[0]) + (y[1] - y1[1]) * (y[1] - y1[1]) + (y[2] - y1[2]) * (y[2] - y1[2]);
+        double dev2 = 0.25 * (l2 - d2);
+        x2 = x;
+        for (int k = 0; k < 9; ++k) y2[k] = y[k];
+        if (dev2 > MAX_TRACK_DEVIATION * MAX_TRACK_DEVIATION) {
+            x2 = x1 + (x - x1) / __builtin_ceil(__builtin_sqrt(dev2) / MAX_TRACK_DEVIATION);
+            dopri_calc_state(st, x2, y2);
+        }
+        reset = check_hit<MIXED>(P, T, x1, y1, x2, y2, st);
+        if (reset) {
+            x = x2;
+            for (int k = 0; k < 9; ++k) y[k] = y2[k];
+        }
+        x1 = x2;
+        for (int k = 0; k < 9; ++k) y1[k] = y2[k];
+    }
+    flags = reset ? (flags | FL_RESET) : (flags & ~FL_RESET);
+    double spinProjection = integrate_spin<MIXED>(P, T, spin, st, x, y);
+    bool spinHasFlipped = st.xo[7] != y[7];
+    if (spinProjection != st.xo[7]) {
+        do_polarize(T, y, spinProjection, T.sp.flipspin);
+        if (T.sp.flipspin && spinHasFlipped) y[7] *= -1;
+    } else if (spinHasFlipped) {
+        for (int k = 0; k < 5; ++k) spin[k] = 0;
+    }
+    finish_checks(T, x, y, tau, tmax);
 
-    /* write back */
-    if (st.deriv_init) flags |= FL_DERIV; else flags &= ~FL_DERIV;
     S.t[i] = x;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) { S.y[k * n + i] = y[k]; S.xc[k * n + i] = st.xc[k]; S.dxc[k * n + i] = st.dxc[k]; }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) S.spin[k * n + i] = spin[k];
-    S.ts[i] = st.t;
-    S.dt[i] = st.dt;
-    S.tau[i] = tau;
+    for (int k = 0; k < 9; ++k) S.y[(size_t)k * n + i] = y[k];
+    for (int k = 0; k < 5; ++k) S.spin[(size_t)k * n + i] = spin[k];
     S.flags[i] = flags;
     S.ctr[i] = T.ctr;
     S.stop[i] = T.stop;
@@ -1080,28 +1501,135 @@ PTD bool track_particle(const Params &P, int i, int max_steps, long long *acc_ou
     S.n_acc[i] = T.nacc; S.n_rej[i] = T.nrej;
     S.Hmax[i] = T.Hmax; S.noflip[i] = T.noflip;
     S.solids[i] = T.solids; S.nsolid[i] = T.nsolid;
-    if (acc_out) *acc_out = T.c.rhs;
-    return T.stop != ID_UNKNOWN;
+    c = T.c;
 }
 
-/* plain kernel: one thread per particle, max_steps accepted steps per launch */
+/*
+ * Particle set-up: the TParticle constructor's Hmax (src/particle.cpp:89) and the start
+ * of TTracker::IntegrateParticle (src/tracking.cpp:19-68): lifetime draw, initial
+ * step size, and the solid stack by +z ray parity.
+ */
 template <bool MIXED>
-__global__ void __launch_bounds__(256) track_kernel(Params P) {
+__global__ void __launch_bounds__(256) init_kernel(Params P) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.st.n) return;
-    if (*P.quit) return;
-    if (P.st.stop[i] != ID_UNKNOWN && (P.st.flags[i] & FL_STARTED)) return;
-    track_particle<MIXED>(P, i, P.max_steps, nullptr);
+    const State &S = P.st;
+    if (i >= S.n || (S.flags[i] & FL_STARTED)) return;
+    const int n = S.n;
+    Species sp = P.species[S.species[i]];
+    uint64_t key = S.key[i], ctr = S.ctr[i];
+    double x = S.t[i], y[9];
+    for (int k = 0; k < 9; ++k) y[k] = S.y[(size_t)k * n + i];
+    Ctr c = {0, 0, 0};
+    double Hmax = kinetic_energy(sp, y + 3) + potential_energy<MIXED>(P, sp, y, nullptr, nullptr, c);
+    double tau;
+    if (sp.tau > 0) tau = -ptm_log(1.0 - ptm_canonical(ptm_draw_u64(key, &ctr))) / (1. / sp.tau);
+    else tau = sp.tmax;
+    int stop = ID_UNKNOWN;
+    unsigned long long sk = sk_set(0ull, 0, -1, 0);
+    int ns = 1;
+    if (x > P.tmax_global) {
+        stop = ID_INVARIANT;
+    } else {
+        for (int k = 0; k < 9; ++k) S.xc[(size_t)k * n + i] = y[k];
+        S.ts[i] = x;
+        S.dt[i] = 10. * MAX_TRACK_DEVIATION / __builtin_sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+        unsigned long long par = ray_parity(P.geom, y[0], y[1], y[2]);
+        for (int m = 0; m < P.geom.n_solid; ++m) {
+            if ((par >> m) & 1ull) {
+                if (ns >= MAX_SOLIDS) { stop = ID_UNSUPPORTED; break; }
+                sk = sk_set(sk, ns, m, solid_ignored(P.geom, m, x));
+                ++ns;
+            }
+        }
+    }
+    S.Hmax[i] = Hmax;
+    S.tau[i] = tau;
+    S.ctr[i] = ctr;
+    S.stop[i] = stop;
+    S.solids[i] = sk;
+    S.nsolid[i] = ns;
+    S.flags[i] = FL_STARTED;
+}
+
+/*
+ * Step kernel: one thread per particle advances its particle by accepted steps
+ * until its step budget is spent, it stops, or one of its chord pieces meets the
+ * mesh.  In the last case the particle parks (piece, step end and dense-output
+ * state in HBM) and its index goes to the collision queue.
+ */
+#ifndef PT_STEP_MIN_WAVES
+#define PT_STEP_MIN_WAVES 2
+#endif
+template <bool MIXED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STEP_MIN_WAVES))) step_kernel(Params P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const State &S = P.st;
+    if (i >= S.n) return;
+    if (S.stop[i] != ID_UNKNOWN || *P.quit != 0) return;
+    int budget = S.budget[i];
+    if (budget <= 0) return;
+    Lane L;
+    load_lane(P, i, L);
+    Ctr c = {0, 0, 0};
+    bool pending = false;
+    do {
+        stepA<MIXED>(P, i, L, pending, c);
+        --budget;
+    } while (!pending && L.stop == ID_UNKNOWN && budget > 0);
+    store_lane(P, i, L);
+    S.budget[i] = budget;
+    if (pending) {
+        int slot = atomicAdd(S.qcount, 1);
+        S.queue[slot] = i;
+    }
+}
+
+/*
+ * Collision kernel: the parked particles of the queue, compacted into full
+ * waves, run the collision path (CheckHit's bisection, DoHit/OnHit) on their
+ * pending chord piece and finish the rest of that step.
+ */
+template <bool MIXED>
+__global__ void __launch_bounds__(256) collision_kernel(Params P) {
+    const int count = *P.st.qcount;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
+        Ctr c = {0, 0, 0};
+        collision_path<MIXED>(P, P.st.queue[j], c);
+    }
+}
+
+/* empties the collision queue (a kernel, not a memset, so launch loops can be captured) */
+__global__ void reset_queue_kernel(int *qcount) { *qcount = 0; }
+
+/* step budget of every particle for the next round of launches */
+__global__ void __launch_bounds__(256) budget_kernel(int *budget, int n, int steps) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) budget[i] = steps;
 }
 
 } // namespace ptd
 
 /* ---- launchers (host side, called from pt_host.cpp) ---------------------- */
 
-hipError_t pt_launch_track(const ptd::Params &P, int variant, int block, hipStream_t stream) {
+hipError_t pt_launch_init(const ptd::Params &P, int variant, int steps, hipStream_t stream) {
+    int grid = (P.st.n + 255) / 256;
+    if (grid == 0) return hipSuccess;
+    if (variant == 1) hipLaunchKernelGGL(ptd::init_kernel<true>, dim3(grid), dim3(256), 0, stream, P);
+    else hipLaunchKernelGGL(ptd::init_kernel<false>, dim3(grid), dim3(256), 0, stream, P);
+    hipLaunchKernelGGL(ptd::budget_kernel, dim3(grid), dim3(256), 0, stream, P.st.budget, P.st.n, steps);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_round(const ptd::Params &P, int variant, int block, int coll_grid, hipStream_t stream) {
     int grid = (P.st.n + block - 1) / block;
     if (grid == 0) return hipSuccess;
-    if (variant == 1) hipLaunchKernelGGL(ptd::track_kernel<true>, dim3(grid), dim3(block), 0, stream, P);
-    else hipLaunchKernelGGL(ptd::track_kernel<false>, dim3(grid), dim3(block), 0, stream, P);
+    if (variant == 1) {
+        hipLaunchKernelGGL(ptd::step_kernel<true>, dim3(grid), dim3(block), 0, stream, P);
+        hipLaunchKernelGGL(ptd::collision_kernel<true>, dim3(coll_grid), dim3(256), 0, stream, P);
+    } else {
+        hipLaunchKernelGGL(ptd::step_kernel<false>, dim3(grid), dim3(block), 0, stream, P);
+        hipLaunchKernelGGL(ptd::collision_kernel<false>, dim3(coll_grid), dim3(256), 0, stream, P);
+    }
+    hipLaunchKernelGGL(ptd::reset_queue_kernel, dim3(1), dim3(1), 0, stream, P.st.qcount);
     return hipGetLastError();
 }

@@ -22,12 +22,13 @@ class LaunchConfig:
     steps_per_launch: int = 0
     persistent: int = 0
     grid: int = 0
+    rounds: int = 0
     leaf_size: int = 4
 
     def native(self):
         c = _native.LaunchCfg()
-        c.variant, c.block, c.steps_per_launch, c.persistent, c.grid = (
-            self.variant, self.block, self.steps_per_launch, self.persistent, self.grid)
+        c.variant, c.block, c.steps_per_launch, c.persistent, c.grid, c.rounds = (
+            self.variant, self.block, self.steps_per_launch, self.persistent, self.grid, self.rounds)
         return c
 
 

@@ -28,10 +28,14 @@ def run_gpu(w, hit_cap=16, cfg=None, with_fp32=False):
 
 
 def assert_bit_exact(g, gh, o, oh):
+    msgs = []
     for f in FIELDS:
         if not np.array_equal(g[f], o[f]):
             bad = np.where(~np.all((g[f] == o[f]).reshape(len(g), -1), axis=1))[0]
-            raise AssertionError(f"{f} differs for {len(bad)} particles, first {bad[:5]}")
+            k = bad[0]
+            msgs.append(f"{f} differs for {len(bad)} particles, first {bad[:5]}: gpu {g[f][k]} oracle {o[f][k]}")
+    if msgs:
+        raise AssertionError("\n".join(msgs))
     if gh is not None:
         n = np.minimum(g["n_hits"], gh.shape[1])
         for k in np.where(n > 0)[0][:2000]:
