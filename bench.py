@@ -3,17 +3,28 @@
 Workload (the configuration the headline metric is quoted on): 1e6 neutrons in a
 closed cylinder (R = 0.25 m, H = 0.5 m, 19,600 triangles), 2% Lambert walls with
 1e-4 loss per bounce, B from a 64^3 FP64 tricubic map of 8 coaxial loops
-(|B| <= 1 T), low-field seekers, energies uniform in 0-250 neV.
+(|B| <= 1 T), low-field seekers, energies uniform in 0-250 neV, tmax 200 s.
 
-A bench step is one launch of the tracker over the whole resident batch,
-advancing every live particle by `--rk` accepted dopri5 steps.  `value` is
-accepted RK steps per second over the timed steps (inputs resident in HBM).
-The CPU baseline is the oracle (the C restatement of the reference algorithm)
-on a bounded sample of the same workload, on all host cores.
+A bench step is one `pt_batch_run` call over the whole resident batch: it
+advances every live particle by up to `--rk` accepted dopri5 steps.  `value` is
+accepted RK steps per second of GPU time over the timed steps (inputs resident
+in HBM; HIP events on the library's stream).  Per-kernel times come from HIP
+events the library records around each step-kernel and collision-kernel launch.
+
+Roofline of the dominant kernel (the step kernel): algorithmic bytes = field-map
+cell evaluations made by the step kernel (device counters) x one coefficient
+block per component (3 x 64 x 8 B = 1536 B FP64, 768 B FP32), divided by the
+step kernel's average launch duration, against HBM's 8 TB/s.  `traffic` is the
+measured HBM bytes per step-kernel launch from the committed rocprofv3 counter
+summary, when present.
+
+The CPU baseline is the oracle (the C restatement of the reference algorithm) on
+a bounded sample of the same workload, one thread per host core.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -23,8 +34,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_HBM_GBS = 8000.0
-PEAK_FP64_TFLOPS = 78.6
 BYTES_PER_FIELD_EVAL = {"fp64": 3 * 64 * 8, "mixed": 3 * 64 * 4}
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_step_kernel_fp64_cylinder_1e6.txt")
 
 
 def parse():
@@ -36,6 +47,7 @@ def parse():
     p.add_argument("--rk", type=int, default=16, help="accepted RK steps per particle per bench step")
     p.add_argument("--grid", type=int, default=64)
     p.add_argument("--block", type=int, default=256)
+    p.add_argument("--rounds", type=int, default=0, help="step/collision rounds per run call (0: library default)")
     p.add_argument("--variants", default="fp64,mixed")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
@@ -43,39 +55,50 @@ def parse():
 
 
 def cpu_baseline(w, budget_s):
-    """Oracle on the first particles of the workload, all host cores, scaled to ~budget_s."""
+    """Oracle on the first particles of the workload over the same flight time the
+    GPU run covers, one worker thread per host core, sized to about budget_s."""
     from oracle import ptref
     from pentrack_amd.workloads import Workload
-    cores = min(16, os.cpu_count() or 1)
+    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
     P = ptref.Problem(w.geometry, w.fields)
+    tmax = 2.0
 
-    def sub(m, tmax):
+    def sub(m):
         return Workload(w.name, w.geometry, w.fields, w.species, tmax, w.species_idx[:m], w.t0[:m], w.y0[:m],
                         w.spin0[:m], w.pid[:m], w.seed)
 
-    m, tmax = 4 * cores, 1.0
+    m = 4 * cores
+    for target in (0.25 * budget_s, budget_s):   # two calibration passes, then the sample
+        t = time.perf_counter()
+        out, _ = ptref.track(sub(m), "fp64", threads=cores, problem=P)
+        dt = time.perf_counter() - t
+        m = int(min(w.n, max(m, m * target / max(dt, 1e-3))))
     t = time.perf_counter()
-    out, _ = ptref.track(sub(m, tmax), "fp64", threads=cores, problem=P)
-    dt = time.perf_counter() - t
-    scale = max(1.0, min(50.0, budget_s / max(dt, 1e-3)))
-    tmax2 = min(w.tmax, tmax * scale)
-    t = time.perf_counter()
-    out, _ = ptref.track(sub(m, tmax2), "fp64", threads=cores, problem=P)
+    out, _ = ptref.track(sub(m), "fp64", threads=cores, problem=P)
     dt = time.perf_counter() - t
     acc = int(out["n_acc"].sum())
-    fields_per_step = float(out["n_field"].sum()) / max(acc, 1)
-    flops = None
     return dict(value=acc / dt, unit="RK steps/s", cores=cores, kind="port",
-                sample=f"{m} particles of the workload tracked for {tmax2:g} s of flight "
-                       f"({acc} accepted steps in {dt:.1f} s)",
-                fields_per_step=fields_per_step, rhs_per_step=float(out["n_rhs"].sum()) / max(acc, 1))
+                sample=f"first {m} particles of the workload tracked for {tmax:g} s of flight "
+                       f"({acc} accepted steps in {dt:.1f} s on {cores} threads)",
+                particles_per_s=m / dt)
+
+
+def read_traffic():
+    """HBM bytes per step-kernel launch from the committed counter summary, if any."""
+    try:
+        txt = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
+    return float(m.group(1)) if m else None
 
 
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the tracking path shards trivially "
-              "by particle index but multi-GPU runs are not part of this build.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); particles are independent and the "
+              "path would shard by particle index, but multi-GPU runs are not part of this build.",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     from pentrack_amd import _native, workloads
@@ -98,11 +121,12 @@ def main():
         vid = _native.PT_VARIANT_MIXED if variant == "mixed" else _native.PT_VARIANT_FP64
         b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
                   with_fp32=(variant == "mixed"))
-        cfg = LaunchConfig(variant=vid, block=a.block, steps_per_launch=a.rk)
+        cfg = LaunchConfig(variant=vid, block=a.block, steps_per_launch=a.rk, rounds=a.rounds)
         for _ in range(a.warmup):
             b.run(cfg, sptr)
         torch.cuda.synchronize()
-        acc0, rej0 = b.step_counts(sptr)
+        acc0, rej0, nf0 = b.step_counts(sptr, with_fields=True)
+        b.set_profiling(True)
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -114,22 +138,26 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         gpu_ms = ev0.elapsed_time(ev1)
-        acc1, rej1 = b.step_counts(sptr)
+        kt = b.kernel_times(sptr)
+        acc1, rej1, nf1 = b.step_counts(sptr, with_fields=True)
         active = b.active(sptr)
         steps = acc1 - acc0
-        results[variant] = dict(steps=steps, rejected=rej1 - rej0, gpu_ms=gpu_ms, wall_s=wall,
-                                rate=steps / (gpu_ms / 1e3), active=active)
+        results[variant] = dict(steps=steps, rejected=rej1 - rej0, field_evals=nf1 - nf0, gpu_ms=gpu_ms,
+                                wall_s=wall, rate=steps / (gpu_ms / 1e3), active=active, kernels=kt)
         print(f"[bench] {variant}: {steps} accepted steps in {gpu_ms:.1f} ms -> {steps / (gpu_ms / 1e3):.3e} steps/s "
-              f"(active {active}/{w.n})", file=sys.stderr, flush=True)
+              f"(active {active}/{w.n}); step kernel {kt['step'][0]:.1f} ms / {kt['step'][1]} launches, "
+              f"collision kernel {kt['collision'][0]:.1f} ms / {kt['collision'][1]} launches",
+              file=sys.stderr, flush=True)
         del b
 
     cpu = None if a.no_cpu else cpu_baseline(w, a.cpu_seconds)
     head = "fp64" if "fp64" in results else next(iter(results))
     r = results[head]
-    fields_per_step = cpu["fields_per_step"] if cpu else 8.0
-    bytes_per_step = fields_per_step * BYTES_PER_FIELD_EVAL[head]
-    launch_ms = r["gpu_ms"] / a.steps
-    achieved = bytes_per_step * (r["steps"] / a.steps) / (launch_ms / 1e3) / 1e9
+    step_ms, step_launches = r["kernels"]["step"]
+    per_launch_ms = step_ms / max(step_launches, 1)
+    bytes_per_launch = r["field_evals"] * BYTES_PER_FIELD_EVAL[head] / max(step_launches, 1)
+    achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
+    traffic = read_traffic() if head == "fp64" else None
     line = {
         "metric": "RK steps/s on 1 MI355X (cylinder, 1e6 neutrons, 64^3 tricubic B map)",
         "value": r["rate"],
@@ -137,20 +165,27 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": launch_ms,
+        "ms_per_step": r["gpu_ms"] / a.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "cylinder R=0.25 H=0.5 (19600 tris), 1e6 neutrons, 64^3 FP64 tricubic B map of 8 loops, "
-                               "2% Lambert, loss 1e-4, dopri5 rtol=atol=1e-9",
+                               "2% Lambert, loss 1e-4, dopri5 rtol=atol=1e-9, tmax 200 s",
                    "n_particles": w.n, "rk_steps_per_bench_step": a.rk, "variant": head, "block": a.block},
         "variants": {k: {"rk_steps_per_s": v["rate"], "gpu_ms": v["gpu_ms"], "accepted": v["steps"],
-                         "rejected": v["rejected"]} for k, v in results.items()},
+                         "rejected": v["rejected"], "particles_active": v["active"],
+                         "step_kernel_ms_per_launch": v["kernels"]["step"][0] / max(v["kernels"]["step"][1], 1),
+                         "collision_kernel_ms_per_launch":
+                             v["kernels"]["collision"][0] / max(v["kernels"]["collision"][1], 1)}
+                     for k, v in results.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "per_step_bytes": bytes_per_step, "field_evals_per_step": fields_per_step},
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "kernel": "step_kernel<fp64>", "kernel_ms_per_launch": per_launch_ms,
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "field_evals_per_accepted_step": r["field_evals"] / max(r["steps"], 1),
+                     "bytes_per_field_eval": BYTES_PER_FIELD_EVAL[head]},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }
     print(json.dumps(line), flush=True)

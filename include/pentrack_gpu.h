@@ -179,8 +179,18 @@ int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg);
 int pt_batch_active(pt_batch *b, void *stream, int *active);
 /* Copy results to the host (synchronises the stream).  hits may be NULL. */
 int pt_batch_download(pt_batch *b, void *stream, pt_particle_out *out, pt_hit *hits);
-/* Sum of accepted / rejected steps over the batch (synchronises the stream). */
-int pt_batch_step_counts(pt_batch *b, void *stream, int64_t *accepted, int64_t *rejected);
+/* Sums over the batch (synchronises the stream): accepted / rejected dopri5 steps and
+   field-map cell evaluations of the step kernel (each gathers one coefficient block per
+   component); any pointer may be NULL. */
+int pt_batch_step_counts(pt_batch *b, void *stream, int64_t *accepted, int64_t *rejected,
+                         int64_t *field_evals);
+/* Kernel timing (no counterpart in the reference): with profiling on, pt_batch_run records
+   HIP events around every step-kernel and collision-kernel launch on its stream;
+   pt_batch_kernel_times synchronises and returns the accumulated milliseconds and
+   launch counts, ms[0]/launches[0] for the step kernel, ms[1]/launches[1] for the
+   collision kernel (with its queue reset). */
+int pt_batch_set_profiling(pt_batch *b, int on);
+int pt_batch_kernel_times(pt_batch *b, void *stream, double *ms, int64_t *launches);
 
 /*
  * One-shot equivalent of calling TTracker::IntegrateParticle for every particle:
@@ -193,7 +203,8 @@ int pt_track_batch(void *stream, const pt_geometry *g, const pt_fields *f, int n
                    const pt_launch_cfg *cfg);
 
 /* Cancellation flag mirroring the reference's global `quit` (include/globals.h:33):
-   running kernels stop advancing particles at their next step boundary. */
+   pt_batch_run copies it to the device when it is called, and its step kernels
+   then advance no particle; state stays resumable (clear the flag and run again). */
 void pt_request_quit(int on);
 
 /* Last HIP error string of this thread (diagnostics). */

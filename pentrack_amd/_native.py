@@ -99,7 +99,8 @@ EXPORTS = [
     "pt_geometry_create", "pt_geometry_destroy", "pt_geometry_info",
     "pt_fields_create", "pt_fields_destroy", "pt_tricubic_precompute",
     "pt_batch_create", "pt_batch_destroy", "pt_batch_run", "pt_batch_active",
-    "pt_batch_download", "pt_batch_step_counts", "pt_track_batch",
+    "pt_batch_download", "pt_batch_step_counts", "pt_track_batch", "pt_batch_set_profiling",
+    "pt_batch_kernel_times",
     "pt_request_quit", "pt_last_error", "pt_clearance_grid", "pt_tricubic_matrix",
 ]
 
@@ -140,7 +141,9 @@ def lib():
     L.pt_batch_run.argtypes = [vp, vp, C.POINTER(LaunchCfg)]
     L.pt_batch_active.argtypes = [vp, vp, C.POINTER(C.c_int)]
     L.pt_batch_download.argtypes = [vp, vp, vp, vp]
-    L.pt_batch_step_counts.argtypes = [vp, vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.pt_batch_step_counts.argtypes = [vp, vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.pt_batch_set_profiling.argtypes = [vp, C.c_int]
+    L.pt_batch_kernel_times.argtypes = [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.pt_track_batch.argtypes = [vp, vp, vp, C.c_int, C.POINTER(Species), C.c_double, C.c_int, ip,
                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.c_uint64, C.POINTER(C.c_uint64), vp, vp, C.c_int, C.POINTER(LaunchCfg)]

@@ -113,6 +113,7 @@ struct State {
     double *Bc, *Vc;          /* field at the stepper's current state, [3][n] and [n] (valid if FL_BVALID) */
     double *pend;             /* pending wall-collision piece: x1, y1[9], x2, y2[9] -> [20][n] */
     double *dense;            /* dense-output state of the last step: [DENSE][n] */
+    long long *n_field;       /* field-map cell evaluations made by the step kernel */
     int *budget;              /* accepted steps left in the current run call */
     int *queue;               /* indices of particles parked on a wall collision */
     int *qcount;              /* queue length */

@@ -693,6 +693,15 @@ extern "C" int pt_tricubic_matrix(int32_t *out) {
 
 struct pt_batch {
     DevMem mem;
+    /* optional per-kernel timing: event pairs around step / collision launches */
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[2];
+    double ms[2] = {0, 0};
+    long long launches[2] = {0, 0};
+    ~pt_batch() {
+        for (auto &v : ev)
+            for (auto &e : v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    }
     ptd::Params P{};
     int n = 0;
     int hit_cap = 0;
@@ -767,6 +776,7 @@ extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_s
     if ((r = B->mem.alloc(&S.Vc, N))) return r;
     if ((r = B->mem.alloc(&S.pend, 20 * N))) return r;
     if ((r = B->mem.alloc(&S.dense, (size_t)ptd::DENSE * N))) return r;
+    if ((r = B->mem.upload(&S.n_field, lzero.data(), N))) return r;
     if ((r = B->mem.upload(&S.budget, izero.data(), N))) return r;
     if ((r = B->mem.alloc(&S.queue, N))) return r;
     if ((r = B->mem.upload(&S.qcount, izero.data(), 1))) return r;
@@ -803,7 +813,22 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
     int rounds = cfg->rounds > 0 ? cfg->rounds : 4;
     int coll_grid = std::max(1, std::min((b->n + 255) / 256, 2048));
     PT_HIP(pt_launch_init(b->P, cfg->variant, steps, s));
-    for (int r = 0; r < rounds; ++r) PT_HIP(pt_launch_round(b->P, cfg->variant, block, coll_grid, s));
+    for (int r = 0; r < rounds; ++r) {
+        for (int k = 0; k < 2; ++k) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (b->profiling) {
+                PT_HIP(hipEventCreate(&e0));
+                PT_HIP(hipEventCreate(&e1));
+                PT_HIP(hipEventRecord(e0, s));
+            }
+            if (k == 0) PT_HIP(pt_launch_step(b->P, cfg->variant, block, s));
+            else PT_HIP(pt_launch_collision(b->P, cfg->variant, coll_grid, s));
+            if (b->profiling) {
+                PT_HIP(hipEventRecord(e1, s));
+                b->ev[k].emplace_back(e0, e1);
+            }
+        }
+    }
     return PT_OK;
 }
 
@@ -893,19 +918,47 @@ extern "C" int pt_batch_download(pt_batch *b, void *stream, pt_particle_out *out
     return PT_OK;
 }
 
-extern "C" int pt_batch_step_counts(pt_batch *b, void *stream, int64_t *accepted, int64_t *rejected) {
+extern "C" int pt_batch_step_counts(pt_batch *b, void *stream, int64_t *accepted, int64_t *rejected,
+                                    int64_t *field_evals) {
     if (!b) return fail(PT_EINVAL, "pt_batch_step_counts: bad arguments");
     PT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
     int n = b->n;
-    std::vector<long long> acc(std::max(n, 1)), rej(std::max(n, 1));
+    std::vector<long long> acc(std::max(n, 1)), rej(std::max(n, 1)), nf(std::max(n, 1));
     if (n) {
         PT_HIP(hipMemcpy(acc.data(), b->P.st.n_acc, n * sizeof(long long), hipMemcpyDeviceToHost));
         PT_HIP(hipMemcpy(rej.data(), b->P.st.n_rej, n * sizeof(long long), hipMemcpyDeviceToHost));
+        PT_HIP(hipMemcpy(nf.data(), b->P.st.n_field, n * sizeof(long long), hipMemcpyDeviceToHost));
     }
-    long long sa = 0, sr = 0;
-    for (int i = 0; i < n; ++i) { sa += acc[i]; sr += rej[i]; }
+    long long sa = 0, sr = 0, sf = 0;
+    for (int i = 0; i < n; ++i) { sa += acc[i]; sr += rej[i]; sf += nf[i]; }
     if (accepted) *accepted = sa;
     if (rejected) *rejected = sr;
+    if (field_evals) *field_evals = sf;
+    return PT_OK;
+}
+
+extern "C" int pt_batch_set_profiling(pt_batch *b, int on) {
+    if (!b) return fail(PT_EINVAL, "pt_batch_set_profiling: bad arguments");
+    b->profiling = on != 0;
+    return PT_OK;
+}
+
+extern "C" int pt_batch_kernel_times(pt_batch *b, void *stream, double *ms, int64_t *launches) {
+    if (!b || !ms || !launches) return fail(PT_EINVAL, "pt_batch_kernel_times: bad arguments");
+    PT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    for (int k = 0; k < 2; ++k) {
+        for (auto &e : b->ev[k]) {
+            float t = 0;
+            PT_HIP(hipEventElapsedTime(&t, e.first, e.second));
+            b->ms[k] += t;
+            b->launches[k] += 1;
+            PT_HIP(hipEventDestroy(e.first));
+            PT_HIP(hipEventDestroy(e.second));
+        }
+        b->ev[k].clear();
+        ms[k] = b->ms[k];
+        launches[k] = b->launches[k];
+    }
     return PT_OK;
 }
 

@@ -11,7 +11,8 @@ namespace ptd { struct Params; }
 
 /* particle set-up (first call only) and the step budget of every particle */
 hipError_t pt_launch_init(const ptd::Params &P, int variant, int steps, hipStream_t stream);
-/* one round: step kernel, collision kernel over the queue, queue reset */
-hipError_t pt_launch_round(const ptd::Params &P, int variant, int block, int coll_grid, hipStream_t stream);
+/* one round = the step kernel, then the collision kernel over the queue and the queue reset */
+hipError_t pt_launch_step(const ptd::Params &P, int variant, int block, hipStream_t stream);
+hipError_t pt_launch_collision(const ptd::Params &P, int variant, int coll_grid, hipStream_t stream);
 
 #endif /* PT_KERNELS_H_ */

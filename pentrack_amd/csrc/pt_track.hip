@@ -1578,6 +1578,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
     } while (!pending && L.stop == ID_UNKNOWN && budget > 0);
     store_lane(P, i, L);
     S.budget[i] = budget;
+    S.n_field[i] += c.field;
     if (pending) {
         int slot = atomicAdd(S.qcount, 1);
         S.queue[slot] = i;
@@ -1620,16 +1621,18 @@ hipError_t pt_launch_init(const ptd::Params &P, int variant, int steps, hipStrea
     return hipGetLastError();
 }
 
-hipError_t pt_launch_round(const ptd::Params &P, int variant, int block, int coll_grid, hipStream_t stream) {
+hipError_t pt_launch_step(const ptd::Params &P, int variant, int block, hipStream_t stream) {
     int grid = (P.st.n + block - 1) / block;
     if (grid == 0) return hipSuccess;
-    if (variant == 1) {
-        hipLaunchKernelGGL(ptd::step_kernel<true>, dim3(grid), dim3(block), 0, stream, P);
-        hipLaunchKernelGGL(ptd::collision_kernel<true>, dim3(coll_grid), dim3(256), 0, stream, P);
-    } else {
-        hipLaunchKernelGGL(ptd::step_kernel<false>, dim3(grid), dim3(block), 0, stream, P);
-        hipLaunchKernelGGL(ptd::collision_kernel<false>, dim3(coll_grid), dim3(256), 0, stream, P);
-    }
+    if (variant == 1) hipLaunchKernelGGL(ptd::step_kernel<true>, dim3(grid), dim3(block), 0, stream, P);
+    else hipLaunchKernelGGL(ptd::step_kernel<false>, dim3(grid), dim3(block), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_collision(const ptd::Params &P, int variant, int coll_grid, hipStream_t stream) {
+    if (P.st.n == 0) return hipSuccess;
+    if (variant == 1) hipLaunchKernelGGL(ptd::collision_kernel<true>, dim3(coll_grid), dim3(256), 0, stream, P);
+    else hipLaunchKernelGGL(ptd::collision_kernel<false>, dim3(coll_grid), dim3(256), 0, stream, P);
     hipLaunchKernelGGL(ptd::reset_queue_kernel, dim3(1), dim3(1), 0, stream, P.st.qcount);
     return hipGetLastError();
 }

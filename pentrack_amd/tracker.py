@@ -73,10 +73,24 @@ class Batch:
                 return True
         return False
 
-    def step_counts(self, stream=None):
-        a, r = C.c_int64(), C.c_int64()
-        _native.check(_native.lib().pt_batch_step_counts(self.h, stream, C.byref(a), C.byref(r)), "pt_batch_step_counts")
-        return a.value, r.value
+    def step_counts(self, stream=None, with_fields=False):
+        """Accepted and rejected dopri5 steps summed over the batch (and, with
+        with_fields, the step kernel's field-map cell evaluations)."""
+        a, r, f = C.c_int64(), C.c_int64(), C.c_int64()
+        _native.check(_native.lib().pt_batch_step_counts(self.h, stream, C.byref(a), C.byref(r), C.byref(f)),
+                      "pt_batch_step_counts")
+        return (a.value, r.value, f.value) if with_fields else (a.value, r.value)
+
+    def set_profiling(self, on=True):
+        _native.check(_native.lib().pt_batch_set_profiling(self.h, int(bool(on))), "pt_batch_set_profiling")
+
+    def kernel_times(self, stream=None):
+        """Accumulated HIP-event milliseconds and launch counts of the step and
+        collision kernels since profiling was switched on."""
+        ms = (C.c_double * 2)()
+        n = (C.c_int64 * 2)()
+        _native.check(_native.lib().pt_batch_kernel_times(self.h, stream, ms, n), "pt_batch_kernel_times")
+        return {"step": (ms[0], n[0]), "collision": (ms[1], n[1])}
 
     def download(self, stream=None):
         out = np.zeros(self.n, dtype=_native.OUT_DTYPE)
