@@ -424,6 +424,116 @@ PTD_NOINLINE int2 seg_query_v(const Geom &G, double p0, double p1, double p2, do
     return make_int2(count, ntests);
 }
 
+/*
+ * Candidate list of one chord piece: the triangles whose bounding boxes meet a box
+ * that contains every sub-segment the collision search can test on that piece,
+ * collected once from the BVH in its depth-first leaf order.  A query whose two
+ * end points lie in the box is answered from the list: the box is convex, so the
+ * segment lies in it and only candidates can meet it; visiting them in BVH leaf
+ * order keeps the first error (coplanar overlap, too many hits) the same as the
+ * full traversal's.  n < 0: no list (too many candidates).
+ */
+constexpr int MAX_CAND = 24;
+struct Cand {
+    int n;
+    double lo[3], hi[3];
+    int idx[MAX_CAND];
+};
+
+PTD void cand_build(const Geom &G, const double *lo, const double *hi, Cand &C) {
+    C.n = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { C.lo[k] = lo[k]; C.hi[k] = hi[k]; }
+    if (G.n_node == 0) return;
+    int stack[BVH_STACK];
+    int sp = 0, node = 0;
+    for (;;) {
+        BvhNode nd;
+        load_node(G.node + node, nd);
+        bool hit = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double pad = 1e-9 + 1e-7 * (dabs((double)nd.lo[k]) + dabs((double)nd.hi[k]));
+            hit = hit && ((double)nd.lo[k] - pad <= hi[k]) && ((double)nd.hi[k] + pad >= lo[k]);
+        }
+        if (hit) {
+            if (nd.a < 0) {
+                int first = -nd.a - 1;
+                for (int k = 0; k < nd.b; ++k) {
+                    double a[3], b[3], c[3];
+                    int solid, orig;
+                    load_tri(G.tri + first + k, a, b, c, solid, orig);
+                    bool in = true;
+#pragma unroll
+                    for (int ax = 0; ax < 3; ++ax) {
+                        double tlo = a[ax] < b[ax] ? a[ax] : b[ax];
+                        tlo = tlo < c[ax] ? tlo : c[ax];
+                        double thi = a[ax] > b[ax] ? a[ax] : b[ax];
+                        thi = thi > c[ax] ? thi : c[ax];
+                        double pad = 1e-9 + 1e-7 * (dabs(tlo) + dabs(thi));
+                        in = in && (tlo - pad <= hi[ax]) && (thi + pad >= lo[ax]);
+                    }
+                    if (!in) continue;
+                    if (C.n >= MAX_CAND) { C.n = -1; return; }
+                    C.idx[C.n++] = first + k;
+                }
+            } else {
+                if (sp >= BVH_STACK) { C.n = -1; return; }
+                stack[sp++] = nd.b;
+                node = nd.a;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+}
+
+PTD bool cand_covers(const Cand &C, const double *p, const double *q) {
+    if (C.n < 0) return false;
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        in = in && p[k] >= C.lo[k] && p[k] <= C.hi[k] && q[k] >= C.lo[k] && q[k] <= C.hi[k];
+    return in;
+}
+
+/* seg_query_v restricted to the candidates (same tests, same sorted list, same errors) */
+PTD int cand_query(const Geom &G, const Cand &C, const double *p, const double *q, HitList *list, long long *tests) {
+    double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
+    double sl = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    int count = 0;
+    for (int k = 0; k < C.n; ++k) {
+        const int t = C.idx[k];
+        double a[3], b[3], c[3], pt[3];
+        int solid, orig;
+        load_tri(G.tri + t, a, b, c, solid, orig);
+        ++*tests;
+        int r = seg_tri(a, b, c, p, q, pt);
+        if (r == 0) continue;
+        if (r < 0) return -1;
+        if (count >= MAX_HITS) return -2;
+        ++count;
+        if (list) {
+            double v[3] = {pt[0] - p[0], pt[1] - p[1], pt[2] - p[2]};
+            double s = (v[0] * d[0] + v[1] * d[1] + v[2] * d[2]) / sl;
+            unsigned id = G.solid_id[solid];
+            int j = list->n;
+            while (j > 0) {
+                double so = list->s[j - 1];
+                unsigned io = list->id[j - 1];
+                bool before = (s == so) ? (id > io || (id == io && orig < list->orig[j - 1])) : (s < so);
+                if (!before) break;
+                list->s[j] = so; list->id[j] = io; list->tri[j] = list->tri[j - 1]; list->orig[j] = list->orig[j - 1];
+                --j;
+            }
+            list->s[j] = s; list->id[j] = id; list->tri[j] = t; list->orig[j] = orig;
+            list->n++;
+        }
+    }
+    return count;
+}
+
 PTD int seg_query(const Geom &G, const double *p, const double *q, HitList *list, long long *tests) {
     int2 r = seg_query_v(G, p[0], p[1], p[2], q[0], q[1], q[2], list);
     *tests += r.y;

@@ -486,6 +486,7 @@ struct Track {
     long long nacc, nrej;
     double Hmax, noflip;
     Ctr c;
+    Cand cand;                  /* candidate triangles of the chord piece under test */
 };
 
 PTD double rnd_uniform(Track &T, double a, double b) {
@@ -707,7 +708,7 @@ PTD_NOINLINE bool do_hit(const Params &P, Track &T, double x1, const double *y1,
     const Geom &G = P.geom;
     HitList L;
     L.n = 0;
-    int nc = seg_query(G, y1, y2, &L, &T.c.tri);
+    int nc = cand_covers(T.cand, y1, y2) ? cand_query(G, T.cand, y1, y2, &L, &T.c.tri) : seg_query(G, y1, y2, &L, &T.c.tri);
     if (nc <= 0) { T.stop = nc == -1 ? ID_CGAL_ERROR : (nc == -2 ? ID_UNSUPPORTED : ID_INVARIANT); return true; }
     double sv[3] = {y2[0] - y1[0], y2[1] - y1[1], y2[2] - y1[2]};
     unsigned long long ns = T.solids;
@@ -787,6 +788,15 @@ PTD_NOINLINE bool do_hit(const Params &P, Track &T, double x1, const double *y1,
 }
 
 /* number of intersections of segment p1->p2 (-1 coplanar, -2 overflow) */
+/* collision count of a segment on the collision path: from the piece's candidates when they cover it */
+PTD int count_hits(const Params &P, Track &T, const double *p1, const double *p2) {
+    if (clear_of_walls(P.geom, p1, p2)) return 0;
+    if (cand_covers(T.cand, p1, p2)) return cand_query(P.geom, T.cand, p1, p2, nullptr, &T.c.tri);
+    int2 r = seg_query_v(P.geom, p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], nullptr);
+    T.c.tri += r.y;
+    return r.x;
+}
+
 PTD int count_hits(const Params &P, Ctr &c, const double *p1, const double *p2) {
     if (clear_of_walls(P.geom, p1, p2)) return 0;
     int2 r = seg_query_v(P.geom, p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], nullptr);
@@ -820,7 +830,7 @@ PTD_NOINLINE int iterate_collision(const Params &P, Track &T, const Dopri &st, d
             mid = lo + (hi - lo) * 0.5;
             dopri_calc_state(st, mid, ymid);
             pmid[0] = ymid[0]; pmid[1] = ymid[1]; pmid[2] = ymid[2];
-            int n1 = count_hits(P, T.c, plo, pmid);
+            int n1 = count_hits(P, T, plo, pmid);
             if (n1 < 0) return -1;
             if (n1 > 0) {
                 ph[d >> 6] &= ~(1ull << (d & 63));
@@ -830,7 +840,7 @@ PTD_NOINLINE int iterate_collision(const Params &P, Track &T, const Dopri &st, d
             }
         }
         branch = 0;
-        int n2 = count_hits(P, T.c, pmid, phi);
+        int n2 = count_hits(P, T, pmid, phi);
         if (n2 < 0) return -1;
         if (n2 > 0) {
             ph[d >> 6] |= (1ull << (d & 63));
@@ -876,11 +886,27 @@ template <bool MIXED>
 PTD_NOINLINE bool check_hit(const Params &P, Track &T, double x1, const double *y1in, double &x2, double *y2, const Dopri &st) {
     double y1[9];
     for (int k = 0; k < 9; ++k) y1[k] = y1in[k];
+    {
+        /* every sub-segment of the search has its ends on the dense-output path between
+           y1 and y2; a path of length l with chord d strays at most sqrt(l^2 - d^2)/2 from
+           the chord, so the box of the chord widened by twice that contains them (ends
+           outside it are detected and fall back to the BVH) */
+        double l2 = (y2[8] - y1[8]) * (y2[8] - y1[8]);
+        double d2 = (y2[0] - y1[0]) * (y2[0] - y1[0]) + (y2[1] - y1[1]) * (y2[1] - y1[1]) + (y2[2] - y1[2]) * (y2[2] - y1[2]);
+        double dev = l2 > d2 ? __builtin_sqrt(l2 - d2) : 0.0;
+        double w = dev + 1e-6 + 1e-9 * __builtin_sqrt(d2);
+        double lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = (y1[k] < y2[k] ? y1[k] : y2[k]) - w;
+            hi[k] = (y1[k] < y2[k] ? y2[k] : y1[k]) + w;
+        }
+        cand_build(P.geom, lo, hi, T.cand);
+    }
     for (;;) {
         if (!check_segment(P.geom, y1, y2)) { T.stop = ID_HIT_BOUNDARIES; return true; }
         if (x2 == x1) return false;
         int cursolid = current_solid(P.geom, T.solids, T.nsolid);
-        int n = count_hits(P, T.c, y1, y2);
+        int n = count_hits(P, T, y1, y2);
         if (n < 0) { T.stop = n == -1 ? ID_CGAL_ERROR : ID_UNSUPPORTED; return true; }
         if (n == 0) return do_step_hook<MIXED>(P, T, x1, y1, x2, y2, st, cursolid);
         double xc1 = x1, xc2 = x2, yc1[9], yc2[9];
@@ -1412,6 +1438,7 @@ PTD void collision_path(const Params &P, int i, Ctr &c) {
     const int n = S.n;
     Track T;
     T.i = i;
+    T.cand.n = -1;
     T.sp = P.species[S.species[i]];
     T.key = S.key[i]; T.ctr = S.ctr[i];
     T.stop = S.stop[i];
