@@ -119,6 +119,9 @@ int pt_tricubic_precompute(int nx, int ny, int nz, const double *xg, const doubl
 /* the 64x64 integer matrix of libtricubic's tricubic_get_coeff (coefficients = A x),
    row-major into out[64*64]; for tests */
 int pt_tricubic_matrix(int32_t *out);
+/* spline derivatives dy/dx at the n grid points of one line, as ALGLIB's
+   spline1dgriddiffcubic with parabolic end conditions (src/field_3d.cpp:356); for tests */
+int pt_spline_grid_diff(int n, const double *x, const double *y, double *d);
 
 /* ---- particles --------------------------------------------------------- */
 typedef struct {

@@ -102,6 +102,7 @@ EXPORTS = [
     "pt_batch_download", "pt_batch_step_counts", "pt_track_batch", "pt_batch_set_profiling",
     "pt_batch_kernel_times",
     "pt_request_quit", "pt_last_error", "pt_clearance_grid", "pt_tricubic_matrix",
+    "pt_spline_grid_diff",
 ]
 
 _lib = None
@@ -148,6 +149,7 @@ def lib():
                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.c_uint64, C.POINTER(C.c_uint64), vp, vp, C.c_int, C.POINTER(LaunchCfg)]
     L.pt_request_quit.argtypes = [C.c_int]
+    L.pt_spline_grid_diff.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pt_clearance_grid.argtypes = [C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float),
                                     C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pt_request_quit.restype = None

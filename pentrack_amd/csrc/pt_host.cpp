@@ -679,6 +679,14 @@ extern "C" int pt_tricubic_precompute(int nx, int ny, int nz, const double *xg, 
     return PT_OK;
 }
 
+/* exported for tests: the spline derivatives of one grid line (ALGLIB spline1dgriddiffcubic) */
+extern "C" int pt_spline_grid_diff(int n, const double *x, const double *y, double *d) {
+    if (n < 2 || !x || !y || !d) return fail(PT_EINVAL, "pt_spline_grid_diff: bad arguments");
+    std::vector<double> buf;
+    spline_grid_diff(x, y, n, d, buf);
+    return PT_OK;
+}
+
 /* exported for tests: the 64x64 integer matrix */
 extern "C" int pt_tricubic_matrix(int32_t *out) {
     const TricubicMatrix &M = tricubic_matrix();
