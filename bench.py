@@ -51,6 +51,9 @@ def parse():
     p.add_argument("--variants", default="fp64,mixed")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--adj-n", type=int, default=100_000, help="differentiable leg: neutrons")
+    p.add_argument("--adj-steps", type=int, default=2000, help="differentiable leg: RK4 steps (dt = 1e-4 s)")
+    p.add_argument("--no-adjoint", action="store_true")
     return p.parse_args()
 
 
@@ -81,6 +84,38 @@ def cpu_baseline(w, budget_s):
                 sample=f"first {m} particles of the workload tracked for {tmax:g} s of flight "
                        f"({acc} accepted steps in {dt:.1f} s on {cores} threads)",
                 particles_per_s=m / dt)
+
+
+def adjoint_leg(a, sptr):
+    """Differentiable RK4 path: cube, B = sum of 8 coil maps (64^3 each), dt = 1e-4 s;
+    forward with checkpoints every 256 steps, then the adjoint of J = <E_kin> + <z>."""
+    import torch
+    from pentrack_amd import workloads
+    from pentrack_amd.adjoint import rk4_adjoint, rk4_forward
+    model, y0, s0 = workloads.adjoint_cube(n=a.adj_n, seed=5, n_grid=a.grid, nmaps=8, dt=1e-4)
+    dev = torch.device("cuda:0")
+    y0t = torch.tensor(y0, device=dev)
+    rk4_forward(model, s0, y0t[:256], 16)          # warm-up (module load)
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    yT, ck = rk4_forward(model, s0, y0t, a.adj_steps, 256)
+    e[1].record()
+    lamT = torch.zeros_like(yT)
+    lamT[:, 3:] = model.mass * yT[:, 3:] / len(yT)
+    lamT[:, 2] = 1.0 / len(yT)
+    e[2].record()
+    gy0, gs = rk4_adjoint(model, s0, ck, lamT, a.adj_steps, 256)
+    e[3].record()
+    torch.cuda.synchronize()
+    fwd, adj = e[0].elapsed_time(e[1]), e[2].elapsed_time(e[3])
+    steps = a.adj_n * a.adj_steps
+    print(f"[bench] adjoint leg: forward {fwd:.1f} ms, adjoint {adj:.1f} ms (ratio {adj / fwd:.2f}), "
+          f"{steps / (fwd / 1e3):.3e} RK4 steps/s forward", file=sys.stderr, flush=True)
+    return {"workload": f"cube, {a.adj_n} neutrons, B = sum of 8 coil maps ({a.grid}^3 each), RK4 dt=1e-4 s, "
+                        f"{a.adj_steps} steps, checkpoints every 256, J = <E_kin> + <z>",
+            "forward_ms": fwd, "adjoint_ms": adj, "adjoint_over_forward": adj / fwd,
+            "rk4_steps_per_s_forward": steps / (fwd / 1e3), "dJ_ds": gs.sum(0).cpu().tolist()}
 
 
 def read_traffic():
@@ -150,6 +185,7 @@ def main():
               file=sys.stderr, flush=True)
         del b
 
+    adj = None if a.no_adjoint else adjoint_leg(a, sptr)
     cpu = None if a.no_cpu else cpu_baseline(w, a.cpu_seconds)
     head = "fp64" if "fp64" in results else next(iter(results))
     r = results[head]
@@ -187,6 +223,7 @@ def main():
                      "field_evals_per_accepted_step": r["field_evals"] / max(r["steps"], 1),
                      "bytes_per_field_eval": BYTES_PER_FIELD_EVAL[head]},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "adjoint": adj,
     }
     print(json.dumps(line), flush=True)
 

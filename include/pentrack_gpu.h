@@ -205,6 +205,43 @@ int pt_track_batch(void *stream, const pt_geometry *g, const pt_fields *f, int n
                    const uint64_t *pid, pt_particle_out *out, pt_hit *hits, int hit_cap,
                    const pt_launch_cfg *cfg);
 
+/* ---- differentiable path: fixed-step RK4 and its adjoint ----------------- */
+/*
+ * No counterpart in the reference (its odeint stepper has no derivatives): smooth
+ * neutron motion (TParticle::EquationOfMotion, src/particle.cpp:105-135, in the
+ * non-relativistic limit) through B = sum_k s_k B_k, gravity, and specular box walls
+ * applied as fixed maps after each step.  All array pointers are DEVICE pointers;
+ * states are structure-of-arrays [6][n] (x y z vx vy vz).
+ */
+#define PT_ADJ_MAX_MAPS 8
+
+typedef struct {
+    int nmaps;                  /* number of maps k (<= PT_ADJ_MAX_MAPS), all on one grid */
+    int nx, ny, nz;             /* grid (strictly increasing coordinates) */
+    double x0[3], inv_h[3];     /* first coordinate and (n-1)/extent per axis (cell-search guess) */
+    const double *g[3];         /* grid coordinates (device) */
+    const double *coef[PT_ADJ_MAX_MAPS]; /* map k: [(nx-1)(ny-1)(nz-1)][3][64] blocks, device layout
+                                           a[16i+4j+k] = coefficient of x^i y^j z^k */
+    double s[PT_ADJ_MAX_MAPS];  /* scale factors s_k */
+} pt_adj_maps;
+
+typedef struct {
+    pt_adj_maps maps;
+    double lo[3], hi[3];        /* box walls */
+    double dt;                  /* RK4 step [s] */
+    double coupling;            /* pol * mu / m  [m^2 / (s^2 T)] */
+    double grav;                /* g [m/s^2] along -z */
+} pt_adj_params;
+
+/* Forward: yT = state after nsteps steps from y0; ckpt (may be NULL) receives the state
+   before every K-th step, [ceil(nsteps/K)][6][n]. */
+int pt_rk4_forward_ckpt(void *stream, const pt_adj_params *params, int n, const double *y0, double *yT,
+                        double *ckpt, int nsteps, int K);
+/* Adjoint: lam holds dJ/dyT on entry and dJ/dy0 on exit ([6][n]); sbar receives dJ/ds_k per
+   particle ([nmaps][n]); seg ([K][24][n] doubles) and flags ([K][n] ints) are workspace. */
+int pt_rk4_adjoint(void *stream, const pt_adj_params *params, int n, const double *ckpt, double *lam,
+                   double *sbar, double *seg, int *flags, int nsteps, int K);
+
 /* Cancellation flag mirroring the reference's global `quit` (include/globals.h:33):
    pt_batch_run copies it to the device when it is called, and its step kernels
    then advance no particle; state stays resumable (clear the flag and run again). */

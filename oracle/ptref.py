@@ -235,3 +235,59 @@ def field_eval(problem, pts, t=0.0, mode="fp64"):
     B = np.zeros((n, 3)); dB = np.zeros((n, 3, 3)); V = np.zeros(n); E = np.zeros((n, 3))
     L.ptref_field_eval(problem.n_fields, problem.fields, n, _dp(pts), float(t), _dp(B), _dp(dB), _dp(V), _dp(E))
     return B, dB, V, E
+
+
+# ---- differentiable RK4 path (oracle/ptadj.c) -------------------------------------
+
+class _AdjModelC(C.Structure):
+    _fields_ = [
+        ("nmaps", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("g", C.POINTER(C.c_double) * 3), ("coef", C.POINTER(C.c_double) * 8), ("s", C.c_double * 8),
+        ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+        ("dt", C.c_double), ("coupling", C.c_double), ("grav", C.c_double),
+    ]
+
+
+def _adj_lib():
+    if "adj" not in _libs:
+        path = os.path.join(HERE, "libptadj.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.ptadj_forward.argtypes = [C.POINTER(_AdjModelC), C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+        L.ptadj_adjoint.argtypes = [C.POINTER(_AdjModelC), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        _libs["adj"] = L
+    return _libs["adj"]
+
+
+class AdjModel:
+    """The RK4 model of pentrack_amd.adjoint.AdjointModel with scales s, for the CPU."""
+
+    def __init__(self, model, s):
+        self._keep = [np.ascontiguousarray(g, np.float64) for g in model.grid]
+        self._keep += [np.ascontiguousarray(c, np.float64) for c in model.coefs]
+        m = _AdjModelC()
+        m.nmaps = model.nmaps
+        m.nx, m.ny, m.nz = (len(g) for g in model.grid)
+        for a in range(3):
+            m.g[a] = _dp(self._keep[a])
+            m.lo[a], m.hi[a] = model.lo[a], model.hi[a]
+        for k in range(model.nmaps):
+            m.coef[k] = _dp(self._keep[3 + k])
+            m.s[k] = float(s[k])
+        m.dt, m.coupling, m.grav = model.dt, model.coupling, model.grav
+        self.m = m
+
+    def forward(self, y0, nsteps):
+        y0 = np.ascontiguousarray(y0, np.float64)
+        yT = np.empty_like(y0)
+        assert _adj_lib().ptadj_forward(C.byref(self.m), len(y0), y0.ctypes.data, yT.ctypes.data, nsteps) == 0
+        return yT
+
+    def adjoint(self, y0, lamT, nsteps):
+        y0 = np.ascontiguousarray(y0, np.float64)
+        lam = np.ascontiguousarray(lamT, np.float64).copy()
+        sbar = np.zeros((len(y0), self.m.nmaps))
+        assert _adj_lib().ptadj_adjoint(C.byref(self.m), len(y0), y0.ctypes.data, lam.ctypes.data,
+                                        sbar.ctypes.data, nsteps) == 0
+        return lam, sbar

@@ -102,7 +102,7 @@ EXPORTS = [
     "pt_batch_download", "pt_batch_step_counts", "pt_track_batch", "pt_batch_set_profiling",
     "pt_batch_kernel_times",
     "pt_request_quit", "pt_last_error", "pt_clearance_grid", "pt_tricubic_matrix",
-    "pt_spline_grid_diff",
+    "pt_spline_grid_diff", "pt_rk4_forward_ckpt", "pt_rk4_adjoint",
 ]
 
 _lib = None

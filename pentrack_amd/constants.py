@@ -13,6 +13,7 @@ M_N = 1.674927211E-27 / ELE_E    # neutron mass [eV/c^2]
 M_P = 1.672621637E-27 / ELE_E    # proton mass [eV/c^2]
 M_E = 9.10938215e-31 / ELE_E     # electron mass [eV/c^2]
 MU_N_SI = -0.96623641e-26        # neutron magnetic moment [J/T]
+MASS_N_KG = 1.674927211E-27      # neutron mass [kg]
 GAMMA_N = -1.83247185e8          # neutron gyromagnetic ratio [1/(T s)]
 
 MAX_TRACK_DEVIATION = 0.001      # include/particle.h:20

@@ -1,0 +1,386 @@
+/*
+ * pt_adjoint.hip -- differentiable tracking of the smooth inter-collision motion:
+ * fixed-step RK4 for neutrons in B = sum_k s_k B_k (tricubic maps on one grid) plus
+ * gravity, specular box walls applied as fixed maps after each step, and its
+ * discrete adjoint with respect to the initial state and the map scales s_k.
+ *
+ * The reference integrates with adaptive dopri5 and has no derivatives; this path
+ * follows its neutron equation of motion (TParticle::EquationOfMotion,
+ * src/particle.cpp:105-135, non-relativistic limit: dv/dt = -g z + pol mu grad|B| / m)
+ * and its tricubic interpolation (TabField3::Interpolate, src/field_3d.cpp:487-513)
+ * with second derivatives added for the adjoint.
+ *
+ * Forward: one thread per particle, state checkpointed every K steps.
+ * Adjoint: per checkpoint segment (last to first) the forward steps are recomputed
+ * from the checkpoint, the stage data of each step kept in HBM, then the steps are
+ * reversed: lambda <- (d y_{j+1} / d y_j)^T lambda and sbar += (d y_{j+1} / d s)^T lambda.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pt_adjoint.h"
+
+namespace pta {
+
+#define PTA __device__ __forceinline__
+
+/* ---- tricubic block evaluation with first and second derivatives --------- */
+/* block layout a[16 i + 4 j + k] = coefficient of x^i y^j z^k (see pt_host.cpp) */
+struct D2 { double f, fx, fy, fz, fxx, fxy, fxz, fyy, fyz, fzz; };
+
+PTA void tric_d2(const double *__restrict__ a, double x, double y, double z, bool hess, D2 &o) {
+    /* q*: y-Horner results per power of x */
+    double q[4], qy[4], qz[4], qyy[4], qyz[4], qzz[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double r = 0, ry = 0, rz = 0, ryy = 0, ryz = 0, rzz = 0;
+#pragma unroll
+        for (int j = 3; j >= 0; --j) {
+            const double *c = a + 16 * i + 4 * j;
+            /* z-Horner: p, p', p'' */
+            double p = ((c[3] * z + c[2]) * z + c[1]) * z + c[0];
+            double pz = (3.0 * c[3] * z + 2.0 * c[2]) * z + c[1];
+            double pzz = 6.0 * c[3] * z + 2.0 * c[2];
+            /* y-Horner with y-derivatives (second derivative via the product rule) */
+            ryy = ryy * y + 2.0 * ry;
+            ry = ry * y + r;
+            r = r * y + p;
+            ryz = ryz * y + rz;
+            rz = rz * y + pz;
+            rzz = rzz * y + pzz;
+        }
+        q[i] = r; qy[i] = ry; qz[i] = rz; qyy[i] = ryy; qyz[i] = ryz; qzz[i] = rzz;
+    }
+    /* x-Horner of every quantity, with x-derivatives */
+    double F = 0, Fx = 0, Fxx = 0, Fy = 0, Fxy = 0, Fz = 0, Fxz = 0, Fyy = 0, Fyz = 0, Fzz = 0;
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+        Fxx = Fxx * x + 2.0 * Fx;
+        Fx = Fx * x + F;
+        F = F * x + q[i];
+        Fxy = Fxy * x + Fy;
+        Fy = Fy * x + qy[i];
+        Fxz = Fxz * x + Fz;
+        Fz = Fz * x + qz[i];
+        if (hess) {
+            Fyy = Fyy * x + qyy[i];
+            Fyz = Fyz * x + qyz[i];
+            Fzz = Fzz * x + qzz[i];
+        }
+    }
+    o.f = F; o.fx = Fx; o.fy = Fy; o.fz = Fz;
+    o.fxx = Fxx; o.fxy = Fxy; o.fxz = Fxz; o.fyy = Fyy; o.fyz = Fyz; o.fzz = Fzz;
+}
+
+/* cell search (std::upper_bound semantics of TabField3::Interpolate) */
+PTA bool locate(const MapSet &M, const double *p, long &cell, double r[3], double inv[3]) {
+    const int n[3] = {M.nx, M.ny, M.nz};
+    int idx[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double *g = M.g[a];
+        double gs = (p[a] - M.x0[a]) * M.inv_h[a];
+        int k;
+        if (!(gs >= 0.0)) k = 0;
+        else if (gs >= (double)(n[a] - 2)) k = n[a] - 2;
+        else k = (int)gs;
+        double gk = g[k], gk1 = g[k + 1];
+        while (k > 0 && gk > p[a]) { --k; gk1 = gk; gk = g[k]; }
+        while (k < n[a] - 2 && gk1 <= p[a]) { ++k; gk = gk1; gk1 = g[k + 1]; }
+        if (!(gk <= p[a] && p[a] < gk1)) return false;
+        idx[a] = k;
+        inv[a] = 1.0 / (gk1 - gk);
+        r[a] = (p[a] - gk) * inv[a];
+    }
+    cell = ((long)idx[0] * (M.ny - 1) + idx[1]) * (M.nz - 1) + idx[2];
+    return true;
+}
+
+/* Total field B = sum_k s_k B_k: components, gradient G[c][j] = dB_c/dx_j and, with
+   hess, second derivatives H[c][jk] (jk = xx xy xz yy yz zz) */
+struct Field2 { double B[3], G[3][3], H[3][6]; bool in; long cell; double r[3], inv[3]; };
+
+PTA void field_total(const MapSet &M, const double *p, bool hess, Field2 &F) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        F.B[c] = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F.G[c][j] = 0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) F.H[c][j] = 0;
+    }
+    F.in = locate(M, p, F.cell, F.r, F.inv);
+    if (!F.in) return;
+    const double ix = F.inv[0], iy = F.inv[1], iz = F.inv[2];
+#pragma unroll 1
+    for (int k = 0; k < M.nmaps; ++k) {
+        const double sk = M.s[k];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            D2 d;
+            tric_d2(M.coef[k] + ((size_t)F.cell * 3 + c) * 64, F.r[0], F.r[1], F.r[2], hess, d);
+            F.B[c] += sk * d.f;
+            F.G[c][0] += sk * (d.fx * ix);
+            F.G[c][1] += sk * (d.fy * iy);
+            F.G[c][2] += sk * (d.fz * iz);
+            if (hess) {
+                F.H[c][0] += sk * (d.fxx * ix * ix);
+                F.H[c][1] += sk * (d.fxy * ix * iy);
+                F.H[c][2] += sk * (d.fxz * ix * iz);
+                F.H[c][3] += sk * (d.fyy * iy * iy);
+                F.H[c][4] += sk * (d.fyz * iy * iz);
+                F.H[c][5] += sk * (d.fzz * iz * iz);
+            }
+        }
+    }
+}
+
+/* grad|B| from the total field (zero where |B| = 0) */
+PTA void grad_absB(const Field2 &F, double &absB, double gB[3]) {
+    absB = __builtin_sqrt(F.B[0] * F.B[0] + F.B[1] * F.B[1] + F.B[2] * F.B[2]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        gB[j] = absB > 0 ? (F.B[0] * F.G[0][j] + F.B[1] * F.G[1][j] + F.B[2] * F.G[2][j]) / absB : 0.0;
+}
+
+/* right-hand side f(y) = (v, -g z + coupling * grad|B|) */
+PTA void rhs(const AdjParams &A, const double *y, double *f) {
+    Field2 F;
+    field_total(A.maps, y, false, F);
+    double absB, gB[3];
+    grad_absB(F, absB, gB);
+    f[0] = y[3]; f[1] = y[4]; f[2] = y[5];
+    f[3] = A.coupling * gB[0];
+    f[4] = A.coupling * gB[1];
+    f[5] = A.coupling * gB[2] - A.grav;
+}
+
+/* one RK4 step; stage slopes returned for the adjoint */
+PTA void rk4(const AdjParams &A, const double *y, double *yn, double k1[6], double k2[6], double k3[6]) {
+    double k4[6], u[6];
+    const double h = A.dt, h2 = 0.5 * A.dt;
+    rhs(A, y, k1);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) u[q] = y[q] + h2 * k1[q];
+    rhs(A, u, k2);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) u[q] = y[q] + h2 * k2[q];
+    rhs(A, u, k3);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) u[q] = y[q] + h * k3[q];
+    rhs(A, u, k4);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) yn[q] = y[q] + (h / 6.0) * (k1[q] + 2.0 * k2[q] + 2.0 * k3[q] + k4[q]);
+}
+
+/* specular box walls as a fixed map after each step: bit a (low wall) / a + 3 (high wall) */
+PTA int reflect(const AdjParams &A, double *y) {
+    int flags = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (y[a] < A.lo[a]) { y[a] = 2.0 * A.lo[a] - y[a]; y[3 + a] = -y[3 + a]; flags |= 1 << a; }
+        else if (y[a] > A.hi[a]) { y[a] = 2.0 * A.hi[a] - y[a]; y[3 + a] = -y[3 + a]; flags |= 1 << (a + 3); }
+    }
+    return flags;
+}
+
+/*
+ * Vector-Jacobian product of f at u: given w = (wx, wv) (adjoint of the slope),
+ * returns ubar = (df/du)^T w and adds (df/ds)^T w to sbar.
+ *   df_x/dv = I                       -> ubar_v += wx
+ *   df_v/dx = coupling * Hess|B|       -> ubar_x += coupling * Hess|B| wv
+ *   df_v/ds_k = coupling * d(grad|B|)/ds_k
+ * With b = B/|B| and g = grad|B|: Hess|B|_jk = (sum_c G_cj G_ck + B_c H_c,jk - g_j g_k)/|B|,
+ * d g_j/ds_k = (sum_c B^k_c G_cj + B_c G^k_cj - g_j b.B^k)/|B| for map k's field B^k, G^k,
+ * so sbar_k += sum_c alpha_c B^k_c + sum_cj beta_cj G^k_cj with
+ * alpha_c = (sum_j w'_j G_cj - (w'.g) b_c)/|B|, beta_cj = w'_j b_c, w' = coupling wv.
+ */
+PTA void vjp(const AdjParams &A, const double *u, const double *w, double *ubar, double *sbar) {
+    Field2 F;
+    field_total(A.maps, u, true, F);
+    double absB, g[3];
+    grad_absB(F, absB, g);
+    ubar[3] = w[0]; ubar[4] = w[1]; ubar[5] = w[2];
+    ubar[0] = 0; ubar[1] = 0; ubar[2] = 0;
+    if (!F.in || !(absB > 0)) return;
+    const double wp[3] = {A.coupling * w[3], A.coupling * w[4], A.coupling * w[5]};
+    const double inv = 1.0 / absB;
+    /* Hess|B| wp */
+    const int hidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        double acc = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double h = -g[j] * g[k];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) h += F.G[c][j] * F.G[c][k] + F.B[c] * F.H[c][hidx[j][k]];
+            acc += h * inv * wp[k];
+        }
+        ubar[j] = acc;
+    }
+    /* parameter part: second pass over the maps with value and gradient only */
+    double b[3] = {F.B[0] * inv, F.B[1] * inv, F.B[2] * inv};
+    double wg = wp[0] * g[0] + wp[1] * g[1] + wp[2] * g[2];
+    double alpha[3], beta[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        alpha[c] = (wp[0] * F.G[c][0] + wp[1] * F.G[c][1] + wp[2] * F.G[c][2] - wg * b[c]) * inv;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) beta[c][j] = wp[j] * b[c];
+    }
+#pragma unroll 1
+    for (int k = 0; k < A.maps.nmaps; ++k) {
+        double acc = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            D2 d;
+            tric_d2(A.maps.coef[k] + ((size_t)F.cell * 3 + c) * 64, F.r[0], F.r[1], F.r[2], false, d);
+            acc += alpha[c] * d.f + beta[c][0] * (d.fx * F.inv[0]) + beta[c][1] * (d.fy * F.inv[1]) +
+                   beta[c][2] * (d.fz * F.inv[2]);
+        }
+        sbar[k] += acc;
+    }
+}
+
+/* ---- kernels ---------------------------------------------------------------- */
+
+/* forward: y (SoA [6][n]) advanced nsteps; checkpoint c = state before step c*K ([nc][6][n]) */
+__global__ void __launch_bounds__(256) forward_kernel(AdjParams A, const double *y0, double *yT, double *ckpt,
+                                                      int n, int nsteps, int K) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double y[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) y[q] = y0[(size_t)q * n + i];
+    for (int s = 0; s < nsteps; ++s) {
+        if (ckpt && s % K == 0) {
+            size_t c = (size_t)(s / K);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) ckpt[(c * 6 + q) * n + i] = y[q];
+        }
+        double yn[6], k1[6], k2[6], k3[6];
+        rk4(A, y, yn, k1, k2, k3);
+        reflect(A, yn);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) y[q] = yn[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) yT[(size_t)q * n + i] = y[q];
+}
+
+/*
+ * adjoint: lam (SoA [6][n]) holds dJ/dy_T on entry and dJ/dy_0 on exit; sbar ([nmaps][n])
+ * receives each particle's dJ/ds_k.  seg: [K][24][n] stage data, flags [K][n].
+ */
+__global__ void __launch_bounds__(256) adjoint_kernel(AdjParams A, const double *ckpt, double *lam, double *sbar,
+                                                      double *seg, int *flags, int n, int nsteps, int K) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double l[6], sb[PTA_MAX_MAPS];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) l[q] = lam[(size_t)q * n + i];
+    for (int k = 0; k < PTA_MAX_MAPS; ++k) sb[k] = 0;
+    const int nc = (nsteps + K - 1) / K;
+    const double h = A.dt, h2 = 0.5 * A.dt;
+    for (int c = nc - 1; c >= 0; --c) {
+        const int s0 = c * K, s1 = (s0 + K < nsteps) ? s0 + K : nsteps;
+        /* recompute the segment, keeping y_j and k1..k3 of each step */
+        double y[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) y[q] = ckpt[((size_t)c * 6 + q) * n + i];
+        for (int s = s0; s < s1; ++s) {
+            const size_t j = (size_t)(s - s0);
+            double yn[6], k1[6], k2[6], k3[6];
+            rk4(A, y, yn, k1, k2, k3);
+            int f = reflect(A, yn);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                seg[(j * 24 + q) * n + i] = y[q];
+                seg[(j * 24 + 6 + q) * n + i] = k1[q];
+                seg[(j * 24 + 12 + q) * n + i] = k2[q];
+                seg[(j * 24 + 18 + q) * n + i] = k3[q];
+            }
+            flags[j * n + i] = f;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) y[q] = yn[q];
+        }
+        /* reverse the steps */
+        for (int s = s1 - 1; s >= s0; --s) {
+            const size_t j = (size_t)(s - s0);
+            /* the wall map after the step: x -> 2w - x, v -> -v on each flagged axis */
+            int f = flags[j * n + i];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (f & ((1 << a) | (1 << (a + 3)))) { l[a] = -l[a]; l[3 + a] = -l[3 + a]; }
+            double yj[6], k1[6], k2[6], k3[6], u[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                yj[q] = seg[(j * 24 + q) * n + i];
+                k1[q] = seg[(j * 24 + 6 + q) * n + i];
+                k2[q] = seg[(j * 24 + 12 + q) * n + i];
+                k3[q] = seg[(j * 24 + 18 + q) * n + i];
+            }
+            /* adjoints of the slopes from y_{j+1} = y + h/6 (k1 + 2k2 + 2k3 + k4) */
+            double a1[6], a2[6], a3[6], a4[6], ub[6], ly[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                a4[q] = (h / 6.0) * l[q];
+                a3[q] = (h / 6.0) * 2.0 * l[q];
+                a2[q] = (h / 6.0) * 2.0 * l[q];
+                a1[q] = (h / 6.0) * l[q];
+                ly[q] = l[q];
+            }
+            /* k4 = f(y + h k3) */
+#pragma unroll
+            for (int q = 0; q < 6; ++q) u[q] = yj[q] + h * k3[q];
+            vjp(A, u, a4, ub, sb);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) { ly[q] += ub[q]; a3[q] += h * ub[q]; }
+            /* k3 = f(y + h/2 k2) */
+#pragma unroll
+            for (int q = 0; q < 6; ++q) u[q] = yj[q] + h2 * k2[q];
+            vjp(A, u, a3, ub, sb);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) { ly[q] += ub[q]; a2[q] += h2 * ub[q]; }
+            /* k2 = f(y + h/2 k1) */
+#pragma unroll
+            for (int q = 0; q < 6; ++q) u[q] = yj[q] + h2 * k1[q];
+            vjp(A, u, a2, ub, sb);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) { ly[q] += ub[q]; a1[q] += h2 * ub[q]; }
+            /* k1 = f(y) */
+            vjp(A, yj, a1, ub, sb);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) l[q] = ly[q] + ub[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) lam[(size_t)q * n + i] = l[q];
+    for (int k = 0; k < A.maps.nmaps; ++k) sbar[(size_t)k * n + i] = sb[k];
+}
+
+} // namespace pta
+
+/* ---- C ABI ------------------------------------------------------------------ */
+
+extern "C" int pt_rk4_forward_ckpt(void *stream, const pt_adj_params *params, int n, const double *y0,
+                                   double *yT, double *ckpt, int nsteps, int K) {
+    if (!params || n < 0 || nsteps < 0 || K < 1 || (n > 0 && (!y0 || !yT))) return PT_EINVAL;
+    if (params->maps.nmaps < 0 || params->maps.nmaps > PTA_MAX_MAPS) return PT_EINVAL;
+    if (n == 0) return PT_OK;
+    hipLaunchKernelGGL(pta::forward_kernel, dim3((n + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       *params, y0, yT, ckpt, n, nsteps, K);
+    return hipGetLastError() == hipSuccess ? PT_OK : PT_EDEVICE;
+}
+
+extern "C" int pt_rk4_adjoint(void *stream, const pt_adj_params *params, int n, const double *ckpt, double *lam,
+                              double *sbar, double *seg, int *flags, int nsteps, int K) {
+    if (!params || n < 0 || nsteps < 0 || K < 1 || (n > 0 && (!ckpt || !lam || !sbar || !seg || !flags)))
+        return PT_EINVAL;
+    if (params->maps.nmaps < 0 || params->maps.nmaps > PTA_MAX_MAPS) return PT_EINVAL;
+    if (n == 0) return PT_OK;
+    hipLaunchKernelGGL(pta::adjoint_kernel, dim3((n + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       *params, ckpt, lam, sbar, seg, flags, n, nsteps, K);
+    return hipGetLastError() == hipSuccess ? PT_OK : PT_EDEVICE;
+}

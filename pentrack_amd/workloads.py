@@ -112,3 +112,24 @@ def cylinder(n=1000000, seed=2, tmax=200.0, Emax=250e-9, n_grid=64, Bmax=1.0, n_
     sp = [SpeciesConfig(kind=NEUTRON, tau=0.0, tmax=9e99, lmax=9e99)]
     return Workload("cylinder", geom, TFieldManager([fmap]), sp, tmax, np.zeros(n, np.int32), t0, y0, spin0,
                     np.arange(n, dtype=np.uint64), seed, meta)
+
+
+def adjoint_cube(n=100000, seed=5, n_grid=64, nmaps=8, Bmax=0.5, Emax=300e-9, dt=1e-4, threads=None):
+    """Differentiation workload: neutrons in the 1 m cube, B = sum_k s_k B_k with one
+    tricubic map per coaxial loop (loops centred on the cube axis), RK4 step dt.
+    Returns (AdjointModel, y0 [n][6], s0 [nmaps])."""
+    from .adjoint import AdjointModel
+    from .fields import coil_set, loop_field, tricubic_coefficients
+    rng = np.random.default_rng(seed)
+    lo, hi = np.zeros(3), np.ones(3)
+    h = (hi - lo) / (n_grid - 3)
+    axes = [lo[a] - h[a] + h[a] * np.arange(n_grid) for a in range(3)]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    pts = np.stack([X.ravel() - 0.5, Y.ravel() - 0.5, Z.ravel()], axis=1)
+    coils = [(0.6, -0.05 + 1.1 * k / max(nmaps - 1, 1), I) for k, (_, _, I) in enumerate(coil_set(seed, nmaps))]
+    Bk = [loop_field(pts, a, z0, I).T.reshape(3, n_grid, n_grid, n_grid) for a, z0, I in coils]
+    scale = Bmax / np.sqrt((sum(Bk) ** 2).sum(axis=0)).max()
+    coefs = [tricubic_coefficients(*axes, np.ascontiguousarray(B * scale), threads=threads) for B in Bk]
+    model = AdjointModel(grid=tuple(axes), coefs=coefs, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dt=dt)
+    _, y9, _ = neutron_source(rng, n, lambda p: np.ones(len(p), bool), 1e-3, 1 - 1e-3, Emax, pol=1.0)
+    return model, np.ascontiguousarray(y9[:, :6]), np.ones(nmaps)
