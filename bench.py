@@ -35,7 +35,18 @@ sys.path.insert(0, HERE)
 
 PEAK_HBM_GBS = 8000.0
 BYTES_PER_FIELD_EVAL = {"fp64": 3 * 64 * 8, "mixed": 3 * 64 * 4}
-TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_step_kernel_fp64_cylinder_1e6.txt")
+VARIANTS = {
+    "fp64": dict(mixed=False, persistent=0, graph=0),
+    "mixed": dict(mixed=True, persistent=0, graph=0),
+    "mixed_persistent": dict(mixed=True, persistent=1, graph=0),
+    "mixed_persistent_graph": dict(mixed=True, persistent=1, graph=1),
+}
+HEADLINE_ORDER = ["mixed_persistent_graph", "mixed_persistent", "mixed", "fp64"]
+# graph replays carry no per-kernel events: the roofline uses the same kernels timed eagerly
+PROFILED_FOR = {"mixed_persistent_graph": ["mixed_persistent"], "mixed_persistent": ["mixed_persistent"],
+                "mixed": ["mixed"], "fp64": ["fp64"]}
+KERNEL_NAMES = {"fp64": "step_kernel_fp64", "mixed": "step_kernel_mixed",
+                "mixed_persistent": "step_kernel_persistent_mixed"}
 
 
 def parse():
@@ -48,7 +59,13 @@ def parse():
     p.add_argument("--grid", type=int, default=64)
     p.add_argument("--block", type=int, default=256)
     p.add_argument("--rounds", type=int, default=0, help="step/collision rounds per run call (0: library default)")
-    p.add_argument("--variants", default="fp64,mixed")
+    p.add_argument("--variants", default="fp64,mixed,mixed_persistent,mixed_persistent_graph")
+    p.add_argument("--no-mixed-error", action="store_true")
+    p.add_argument("--err-n", type=int, default=10_000, help="mixed-precision error leg: neutrons")
+    p.add_argument("--err-tmax", type=float, default=2.0, help="mixed-precision error leg: horizon [s]")
+    p.add_argument("--no-uneven", action="store_true")
+    p.add_argument("--uneven-n", type=int, default=1_000_000)
+    p.add_argument("--uneven-tris", type=int, default=50_000)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--adj-n", type=int, default=100_000, help="differentiable leg: neutrons")
@@ -86,6 +103,71 @@ def cpu_baseline(w, budget_s):
                 particles_per_s=m / dt)
 
 
+def mixed_error_leg(a, sptr):
+    """Mixed precision against FP64 (the FP64 kernel is bit-exact with the FP64 oracle):
+    cube, 128^3 map of the coil field (FP32 table and FP32 interpolation on the mixed
+    side), neutrons tracked to a common horizon; max and p99 of the end-position and
+    end-velocity deviations over particles whose stop reason and hit sequence agree."""
+    import torch
+    from pentrack_amd import _native, workloads
+    from pentrack_amd.tracker import Batch, LaunchConfig
+    w = workloads.cube(n=a.err_n, seed=3, tmax=a.err_tmax, with_field=True, n_grid=128, Bmax=1.0)
+    out = {}
+    for name, vid in (("fp64", _native.PT_VARIANT_FP64), ("mixed", _native.PT_VARIANT_MIXED)):
+        b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
+                  hit_cap=16, with_fp32=True)
+        b.run_to_completion(LaunchConfig(variant=vid, steps_per_launch=256, rounds=8), max_launches=100000,
+                            stream=sptr)
+        out[name] = b.download(sptr)
+        del b
+    (g, gh), (m, mh) = out["fp64"], out["mixed"]
+    same = (g["stopID"] == m["stopID"]) & (g["Nhit"] == m["Nhit"])
+    n = np.minimum(g["n_hits"], 16)
+    for k in np.where(same)[0]:
+        same[k] &= np.array_equal(gh[k, :n[k]]["tri"], mh[k, :n[k]]["tri"])
+    dx = np.linalg.norm(g["y"][same, :3] - m["y"][same, :3], axis=1)
+    dv = np.linalg.norm(g["y"][same, 3:6] - m["y"][same, 3:6], axis=1)
+    res = {"workload": f"cube 1 m, {a.err_n} neutrons, 128^3 coil map (|B| <= 1 T), {a.err_tmax:g} s",
+           "same_collision_sequence": float(same.mean()),
+           "end_position_max_m": float(dx.max()), "end_position_p99_m": float(np.percentile(dx, 99)),
+           "end_velocity_max_m_s": float(dv.max()), "end_velocity_p99_m_s": float(np.percentile(dv, 99))}
+    print(f"[bench] mixed-precision error: {res}", file=sys.stderr, flush=True)
+    return res
+
+
+def uneven_leg(a, sptr):
+    """Mixed species with very uneven lifetimes: the autotuned launch shape (persistent
+    work-queue kernel or per-particle grid) against the default per-particle grid."""
+    import torch
+    from pentrack_amd import autotune, workloads
+    from pentrack_amd.tracker import Batch, LaunchConfig
+    w = workloads.uneven(n=a.uneven_n, seed=4, n_tris=a.uneven_tris, n_grid=a.grid)
+    best, entry = autotune.tune(w, n_sample=min(200_000, w.n), calls=2, blocks=(256,), grids=(2, 4), steps=(16, 64),
+                                leaf_sizes=(4,), cache=None, stream=sptr)
+    res = {"workload": f"cylinder ({w.geometry.n_triangles()} tris), {w.n} particles: 90% neutrons tau=880 s, "
+                       "8% protons 1-800 eV, 2% electrons 1-700 keV, uniform E 1e5 V/m + 64^3 B map",
+           "tuned_config": {k: getattr(best, k) for k in ("block", "steps_per_launch", "persistent", "grid",
+                                                          "leaf_size")}}
+    for name, cfg in (("default", LaunchConfig(steps_per_launch=16)), ("autotuned", best)):
+        b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
+                  leaf_size=cfg.leaf_size)
+        b.run(cfg, sptr)
+        torch.cuda.synchronize()
+        a0, _ = b.step_counts(sptr)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(4):
+            b.run(cfg, sptr)
+        e1.record()
+        torch.cuda.synchronize()
+        a1, _ = b.step_counts(sptr)
+        ms = e0.elapsed_time(e1)
+        res[name] = {"rk_steps_per_s": (a1 - a0) / (ms / 1e3), "gpu_ms": ms, "particles_active": b.active(sptr)}
+        del b
+    print(f"[bench] uneven lifetimes: {res}", file=sys.stderr, flush=True)
+    return res
+
+
 def adjoint_leg(a, sptr):
     """Differentiable RK4 path: cube, B = sum of 8 coil maps (64^3 each), dt = 1e-4 s;
     forward with checkpoints every 256 steps, then the adjoint of J = <E_kin> + <z>."""
@@ -118,10 +200,10 @@ def adjoint_leg(a, sptr):
             "rk4_steps_per_s_forward": steps / (fwd / 1e3), "dJ_ds": gs.sum(0).cpu().tolist()}
 
 
-def read_traffic():
-    """HBM bytes per step-kernel launch from the committed counter summary, if any."""
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary, if any."""
     try:
-        txt = open(TRAFFIC_FILE).read()
+        txt = open(os.path.join(HERE, "profiles", f"traffic_{kernel}_cylinder_1e6.txt")).read()
     except OSError:
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
@@ -142,7 +224,8 @@ def main():
     _native.lib()
     dev = torch.device("cuda:0")
     torch.cuda.init()
-    stream = torch.cuda.current_stream(dev)
+    stream = torch.cuda.Stream(dev)          # graph capture needs a non-default stream
+    torch.cuda.set_stream(stream)
     sptr = stream.cuda_stream
 
     t_build = time.perf_counter()
@@ -153,15 +236,19 @@ def main():
 
     results = {}
     for variant in [v for v in a.variants.split(",") if v]:
-        vid = _native.PT_VARIANT_MIXED if variant == "mixed" else _native.PT_VARIANT_FP64
+        if variant not in VARIANTS:
+            raise SystemExit(f"unknown variant {variant}; known: {', '.join(VARIANTS)}")
+        kw = VARIANTS[variant]
+        vid = _native.PT_VARIANT_MIXED if kw["mixed"] else _native.PT_VARIANT_FP64
         b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
-                  with_fp32=(variant == "mixed"))
-        cfg = LaunchConfig(variant=vid, block=a.block, steps_per_launch=a.rk, rounds=a.rounds)
+                  with_fp32=kw["mixed"])
+        cfg = LaunchConfig(variant=vid, block=a.block, steps_per_launch=a.rk, rounds=a.rounds,
+                           persistent=kw["persistent"], graph=kw["graph"])
         for _ in range(a.warmup):
             b.run(cfg, sptr)
         torch.cuda.synchronize()
         acc0, rej0, nf0 = b.step_counts(sptr, with_fields=True)
-        b.set_profiling(True)
+        b.set_profiling(not kw["graph"])
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -178,24 +265,31 @@ def main():
         active = b.active(sptr)
         steps = acc1 - acc0
         results[variant] = dict(steps=steps, rejected=rej1 - rej0, field_evals=nf1 - nf0, gpu_ms=gpu_ms,
-                                wall_s=wall, rate=steps / (gpu_ms / 1e3), active=active, kernels=kt)
+                                wall_s=wall, rate=steps / (gpu_ms / 1e3), active=active, kernels=kt,
+                                mixed=kw["mixed"])
         print(f"[bench] {variant}: {steps} accepted steps in {gpu_ms:.1f} ms -> {steps / (gpu_ms / 1e3):.3e} steps/s "
               f"(active {active}/{w.n}); step kernel {kt['step'][0]:.1f} ms / {kt['step'][1]} launches, "
               f"collision kernel {kt['collision'][0]:.1f} ms / {kt['collision'][1]} launches",
               file=sys.stderr, flush=True)
         del b
 
+    mixed_err = None if a.no_mixed_error else mixed_error_leg(a, sptr)
+    uneven_res = None if a.no_uneven else uneven_leg(a, sptr)
     adj = None if a.no_adjoint else adjoint_leg(a, sptr)
     cpu = None if a.no_cpu else cpu_baseline(w, a.cpu_seconds)
-    head = "fp64" if "fp64" in results else next(iter(results))
-    r = results[head]
-    step_ms, step_launches = r["kernels"]["step"]
+    head = next((v for v in HEADLINE_ORDER if v in results), next(iter(results)))
+    prof = next((v for v in PROFILED_FOR[head] if v in results), head)
+    r, rp = results[head], results[prof]
+    step_ms, step_launches = rp["kernels"]["step"]
     per_launch_ms = step_ms / max(step_launches, 1)
-    bytes_per_launch = r["field_evals"] * BYTES_PER_FIELD_EVAL[head] / max(step_launches, 1)
-    achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
-    traffic = read_traffic() if head == "fp64" else None
+    bpe = BYTES_PER_FIELD_EVAL["mixed" if rp["mixed"] else "fp64"]
+    bytes_per_launch = rp["field_evals"] * bpe / max(step_launches, 1)
+    achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
+    kernel_name = KERNEL_NAMES[prof]
+    traffic = read_traffic(kernel_name)
     line = {
-        "metric": "RK steps/s on 1 MI355X (cylinder, 1e6 neutrons, 64^3 tricubic B map)",
+        "metric": "RK steps/s on 1 MI355X (cylinder, 1e6 neutrons, 64^3 tricubic B map), "
+                  "mixed precision + persistent kernel",
         "value": r["rate"],
         "unit": "RK steps/s",
         "n_gpus": 1,
@@ -205,9 +299,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64 state, f32 field interpolation" if r["mixed"] else "f64",
         "data": "synthetic",
-        "config": {"workload": "cylinder R=0.25 H=0.5 (19600 tris), 1e6 neutrons, 64^3 FP64 tricubic B map of 8 loops, "
+        "config": {"workload": "cylinder R=0.25 H=0.5 (19600 tris), 1e6 neutrons, 64^3 tricubic B map of 8 loops, "
                                "2% Lambert, loss 1e-4, dopri5 rtol=atol=1e-9, tmax 200 s",
                    "n_particles": w.n, "rk_steps_per_bench_step": a.rk, "variant": head, "block": a.block},
         "variants": {k: {"rk_steps_per_s": v["rate"], "gpu_ms": v["gpu_ms"], "accepted": v["steps"],
@@ -218,11 +312,13 @@ def main():
                      for k, v in results.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": "step_kernel<fp64>", "kernel_ms_per_launch": per_launch_ms,
+                     "kernel": kernel_name, "kernel_ms_per_launch": per_launch_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "field_evals_per_accepted_step": r["field_evals"] / max(r["steps"], 1),
-                     "bytes_per_field_eval": BYTES_PER_FIELD_EVAL[head]},
+                     "field_evals_per_accepted_step": rp["field_evals"] / max(rp["steps"], 1),
+                     "bytes_per_field_eval": bpe},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "mixed_precision_error": mixed_err,
+        "uneven_lifetimes": uneven_res,
         "adjoint": adj,
     }
     print(json.dumps(line), flush=True)

@@ -161,6 +161,8 @@ typedef struct {
     int grid;                    /* persistent grid size in blocks (0 = automatic) */
     int rounds;                  /* step-kernel / collision-kernel rounds per pt_batch_run call
                                     (0 = 4); each round resolves at most one wall collision per particle */
+    int graph;                   /* 1: capture the call's launch sequence into a hipGraph on first use
+                                    and replay it (needs a non-default stream; no per-kernel timing) */
 } pt_launch_cfg;
 
 typedef struct pt_batch pt_batch;

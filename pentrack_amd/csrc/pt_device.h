@@ -117,6 +117,8 @@ struct State {
     int *budget;              /* accepted steps left in the current run call */
     int *queue;               /* indices of particles parked on a wall collision */
     int *qcount;              /* queue length */
+    int *work;                /* persistent variant: indices of particles with steps to do */
+    int *wcount;              /* [0] work list length, [1] next work item */
 };
 
 constexpr int DENSE = 59;     /* xo[8] dxo[8] k3..k6[32] dxc[8] t t_old pol */

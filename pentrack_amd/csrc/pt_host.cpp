@@ -701,12 +701,16 @@ extern "C" int pt_tricubic_matrix(int32_t *out) {
 
 struct pt_batch {
     DevMem mem;
+    /* captured launch sequences of pt_batch_run, keyed by the launch configuration */
+    struct GraphEntry { std::vector<int> key; hipGraph_t graph; hipGraphExec_t exec; };
+    std::vector<GraphEntry> graphs;
     /* optional per-kernel timing: event pairs around step / collision launches */
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[2];
     double ms[2] = {0, 0};
     long long launches[2] = {0, 0};
     ~pt_batch() {
+        for (auto &g : graphs) { (void)hipGraphExecDestroy(g.exec); (void)hipGraphDestroy(g.graph); }
         for (auto &v : ev)
             for (auto &e : v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     }
@@ -788,6 +792,8 @@ extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_s
     if ((r = B->mem.upload(&S.budget, izero.data(), N))) return r;
     if ((r = B->mem.alloc(&S.queue, N))) return r;
     if ((r = B->mem.upload(&S.qcount, izero.data(), 1))) return r;
+    if ((r = B->mem.alloc(&S.work, N))) return r;
+    if ((r = B->mem.upload(&S.wcount, izero.data(), 2))) return r;
     int zero = 0;
     if ((r = B->mem.upload(&B->d_quit, &zero, 1))) return r;
     if ((r = B->mem.upload(&B->d_count, &zero, 1))) return r;
@@ -820,23 +826,52 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
     int steps = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : INT_MAX;
     int rounds = cfg->rounds > 0 ? cfg->rounds : 4;
     int coll_grid = std::max(1, std::min((b->n + 255) / 256, 2048));
-    PT_HIP(pt_launch_init(b->P, cfg->variant, steps, s));
-    for (int r = 0; r < rounds; ++r) {
-        for (int k = 0; k < 2; ++k) {
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            if (b->profiling) {
-                PT_HIP(hipEventCreate(&e0));
-                PT_HIP(hipEventCreate(&e1));
-                PT_HIP(hipEventRecord(e0, s));
-            }
-            if (k == 0) PT_HIP(pt_launch_step(b->P, cfg->variant, block, s));
-            else PT_HIP(pt_launch_collision(b->P, cfg->variant, coll_grid, s));
-            if (b->profiling) {
-                PT_HIP(hipEventRecord(e1, s));
-                b->ev[k].emplace_back(e0, e1);
+    int pgrid = cfg->grid;
+    if (cfg->persistent && pgrid <= 0) {
+        int dev = 0, cus = 0;
+        PT_HIP(hipGetDevice(&dev));
+        PT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        pgrid = 2 * std::max(cus, 1);   /* two 256-thread blocks per CU (2 waves per SIMD) */
+    }
+    auto launch_all = [&](bool timed) -> int {
+        PT_HIP(pt_launch_init(b->P, cfg->variant, steps, s));
+        for (int r = 0; r < rounds; ++r) {
+            for (int k = 0; k < 2; ++k) {
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (timed) {
+                    PT_HIP(hipEventCreate(&e0));
+                    PT_HIP(hipEventCreate(&e1));
+                    PT_HIP(hipEventRecord(e0, s));
+                }
+                if (k == 1) PT_HIP(pt_launch_collision(b->P, cfg->variant, coll_grid, s));
+                else if (cfg->persistent) PT_HIP(pt_launch_step_persistent(b->P, cfg->variant, pgrid, s));
+                else PT_HIP(pt_launch_step(b->P, cfg->variant, block, s));
+                if (timed) {
+                    PT_HIP(hipEventRecord(e1, s));
+                    b->ev[k].emplace_back(e0, e1);
+                }
             }
         }
+        return PT_OK;
+    };
+    if (!cfg->graph) return launch_all(b->profiling);
+    if (s == nullptr) return fail(PT_EINVAL, "pt_batch_run: graph mode needs a non-default stream");
+    std::vector<int> key = {cfg->variant, block, steps, rounds, cfg->persistent, pgrid};
+    pt_batch::GraphEntry *ge = nullptr;
+    for (auto &g : b->graphs)
+        if (g.key == key) ge = &g;
+    if (!ge) {
+        pt_batch::GraphEntry g{key, nullptr, nullptr};
+        PT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int r = launch_all(false);
+        hipError_t ec = hipStreamEndCapture(s, &g.graph);
+        if (r) { if (g.graph) (void)hipGraphDestroy(g.graph); return r; }
+        if (ec != hipSuccess) return fail(PT_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        PT_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        b->graphs.push_back(g);
+        ge = &b->graphs.back();
     }
+    PT_HIP(hipGraphLaunch(ge->exec, s));
     return PT_OK;
 }
 
@@ -979,7 +1014,7 @@ extern "C" int pt_track_batch(void *stream, const pt_geometry *g, const pt_field
     int r = pt_batch_create(g, f, n_species, species, tmax_global, n, species_idx, t0, y0, spin0, seed, pid, hit_cap, &b);
     if (r) return r;
     std::unique_ptr<pt_batch> guard(b);
-    pt_launch_cfg c = cfg ? *cfg : pt_launch_cfg{PT_VARIANT_FP64, 256, 0, 0, 0, 0};
+    pt_launch_cfg c = cfg ? *cfg : pt_launch_cfg{PT_VARIANT_FP64, 256, 0, 0, 0, 0, 0};
     for (;;) {
         r = pt_batch_run(b, stream, &c);
         if (r) return r;

@@ -13,6 +13,8 @@ namespace ptd { struct Params; }
 hipError_t pt_launch_init(const ptd::Params &P, int variant, int steps, hipStream_t stream);
 /* one round = the step kernel, then the collision kernel over the queue and the queue reset */
 hipError_t pt_launch_step(const ptd::Params &P, int variant, int block, hipStream_t stream);
+/* persistent variant of the step kernel: work-list compaction, then `grid` blocks of 256 slots */
+hipError_t pt_launch_step_persistent(const ptd::Params &P, int variant, int grid, hipStream_t stream);
 hipError_t pt_launch_collision(const ptd::Params &P, int variant, int coll_grid, hipStream_t stream);
 
 #endif /* PT_KERNELS_H_ */

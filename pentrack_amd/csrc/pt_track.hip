@@ -1613,6 +1613,62 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
 }
 
 /*
+ * Persistent step kernel: a grid sized to the machine, each lane a slot that takes the
+ * next particle from the work list (the particles with steps to do, compacted by
+ * work_kernel) whenever its particle parks, stops or spends its step budget, so waves
+ * stay full while particles of very different lifetimes drop out.  Every lane leaves
+ * when the list is exhausted.
+ */
+template <bool MIXED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STEP_MIN_WAVES))) step_kernel_persistent(Params P) {
+    const State &S = P.st;
+    const int count = S.wcount[0];
+    const bool quit = *P.quit != 0;
+    Lane L;
+    Ctr c = {0, 0, 0};
+    int i = -1, budget = 0;
+    bool have = false, pending = false;
+    for (;;) {
+        if (!have) {
+            if (quit) break;
+            int w = atomicAdd(S.wcount + 1, 1);
+            if (w >= count) break;
+            i = S.work[w];
+            budget = S.budget[i];
+            load_lane(P, i, L);
+            c = Ctr{0, 0, 0};
+            pending = false;
+            have = true;
+        }
+        stepA<MIXED>(P, i, L, pending, c);
+        --budget;
+        if (pending || L.stop != ID_UNKNOWN || budget <= 0) {
+            store_lane(P, i, L);
+            S.budget[i] = budget;
+            S.n_field[i] += c.field;
+            if (pending) {
+                int slot = atomicAdd(S.qcount, 1);
+                S.queue[slot] = i;
+            }
+            have = false;
+        }
+    }
+}
+
+/* work list of the persistent kernel: particles that are running and have steps left */
+__global__ void __launch_bounds__(256) work_kernel(Params P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const State &S = P.st;
+    if (i >= S.n) return;
+    if (S.stop[i] == ID_UNKNOWN && (S.flags[i] & FL_STARTED) && S.budget[i] > 0) {
+        int w = atomicAdd(S.wcount, 1);
+        S.work[w] = i;
+    }
+}
+
+__global__ void reset_work_kernel(int *wcount) { wcount[0] = 0; wcount[1] = 0; }
+
+/*
  * Collision kernel: the parked particles of the queue, compacted into full
  * waves, run the collision path (CheckHit's bisection, DoHit/OnHit) on their
  * pending chord piece and finish the rest of that step.
@@ -1653,6 +1709,16 @@ hipError_t pt_launch_step(const ptd::Params &P, int variant, int block, hipStrea
     if (grid == 0) return hipSuccess;
     if (variant == 1) hipLaunchKernelGGL(ptd::step_kernel<true>, dim3(grid), dim3(block), 0, stream, P);
     else hipLaunchKernelGGL(ptd::step_kernel<false>, dim3(grid), dim3(block), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_step_persistent(const ptd::Params &P, int variant, int grid, hipStream_t stream) {
+    if (P.st.n == 0) return hipSuccess;
+    int wgrid = (P.st.n + 255) / 256;
+    hipLaunchKernelGGL(ptd::reset_work_kernel, dim3(1), dim3(1), 0, stream, P.st.wcount);
+    hipLaunchKernelGGL(ptd::work_kernel, dim3(wgrid), dim3(256), 0, stream, P);
+    if (variant == 1) hipLaunchKernelGGL(ptd::step_kernel_persistent<true>, dim3(grid), dim3(256), 0, stream, P);
+    else hipLaunchKernelGGL(ptd::step_kernel_persistent<false>, dim3(grid), dim3(256), 0, stream, P);
     return hipGetLastError();
 }
 

@@ -165,7 +165,8 @@ class TFieldManager:
         rc = L.pt_fields_create(len(self.containers), ds, int(bool(with_fp32)), C.byref(h))
         _native.check(rc, "pt_fields_create")
         if self._handle is not None:
-            L.pt_fields_destroy(self._handle[0])
+            # batches may still use the old tables: keep them until the manager is released
+            self._retired = getattr(self, "_retired", []) + [self._handle[0]]
         self._handle = (h, bool(with_fp32))
         return h
 
@@ -173,5 +174,7 @@ class TFieldManager:
         try:
             if self._handle is not None and _native._lib is not None:
                 _native._lib.pt_fields_destroy(self._handle[0])
+                for h in getattr(self, "_retired", []):
+                    _native._lib.pt_fields_destroy(h)
         except Exception:
             pass

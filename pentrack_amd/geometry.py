@@ -212,9 +212,12 @@ class TGeometry:
         return sum(len(s.triangles) for s in self.solids)
 
     def device(self, leaf_size=4):
-        """Native handle (uploads the BVH to the GPU)."""
-        if self._handle is not None and self._handle[1] == leaf_size:
-            return self._handle[0]
+        """Native handle (uploads the BVH to the GPU); one per BVH leaf size, kept until the
+        geometry is released, so batches built on any of them stay valid."""
+        if self._handle is None:
+            self._handle = {}
+        if leaf_size in self._handle:
+            return self._handle[leaf_size]
         L = _native.lib()
         a = self.arrays()
         self._keep = a
@@ -228,14 +231,13 @@ class TGeometry:
             a["ign"].ctypes.data_as(C.POINTER(C.c_double)), a["default_mat"], len(a["mats"]),
             a["mats"].ctypes.data_as(C.POINTER(C.c_double)), leaf_size, C.byref(h))
         _native.check(rc, "pt_geometry_create")
-        if self._handle is not None:
-            L.pt_geometry_destroy(self._handle[0])
-        self._handle = (h, leaf_size)
+        self._handle[leaf_size] = h
         return h
 
     def __del__(self):
         try:
-            if self._handle is not None and _native._lib is not None:
-                _native._lib.pt_geometry_destroy(self._handle[0])
+            if self._handle and _native._lib is not None:
+                for h in self._handle.values():
+                    _native._lib.pt_geometry_destroy(h)
         except Exception:
             pass

@@ -1,26 +1,33 @@
 """HBM traffic of the step kernel from rocprofv3 counters.
 
-`python -m pentrack_amd.profiling run` is the program profiled: a plain device
-copy of known size (the calibration: FETCH_SIZE and WRITE_SIZE of a kernel that
-reads and writes exactly COPY_BYTES), then the benchmark workload (1e6 neutrons,
-cylinder, 64^3 FP64 map) advanced by a few run calls of 16 RK steps.  Profile it
-twice, once per counter (one counter group per pass):
+`python -m pentrack_amd.profiling run KERNEL` is the program profiled: a plain
+device copy of known size (the calibration: FETCH_SIZE and WRITE_SIZE of a kernel
+that reads and writes exactly COPY_BYTES), then the benchmark workload (1e6
+neutrons, cylinder, 64^3 map) advanced by a few run calls of 16 RK steps with the
+step kernel KERNEL (step_kernel_fp64, step_kernel_mixed or
+step_kernel_persistent_mixed).  Profile it twice, once per counter:
 
-    rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 -m pentrack_amd.profiling run
-    rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 -m pentrack_amd.profiling run
+    rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 -m pentrack_amd.profiling run KERNEL
+    rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 -m pentrack_amd.profiling run KERNEL
 
-`python -m pentrack_amd.profiling summarize FETCH_DB WRITE_DB` prints the
+`python -m pentrack_amd.profiling summarize KERNEL FETCH_DB WRITE_DB` prints the
 summary: counter values (KiB per dispatch) of the copy and of the step kernel,
-the scale factor (bytes the copy moved / bytes the counter reports) and the
-step kernel's scaled HBM bytes per launch.
+the scale factor (bytes the copy moved / bytes the counter reports), the step
+kernel's scaled HBM bytes per launch and its algorithmic bytes per launch.
 """
 import sqlite3
 import sys
 
 COPY_BYTES = 1 << 30
+KERNELS = {
+    # name: (variant, persistent, bytes per field evaluation, dispatch-name pattern)
+    "step_kernel_fp64": (0, 0, 1536, "step_kernel<false>"),
+    "step_kernel_mixed": (1, 0, 768, "step_kernel<true>"),
+    "step_kernel_persistent_mixed": (1, 1, 768, "step_kernel_persistent<true>"),
+}
 
 
-def run():
+def run(kernel="step_kernel_fp64"):
     import torch
     from pentrack_amd import _native, workloads
     from pentrack_amd.tracker import Batch, LaunchConfig
@@ -34,9 +41,11 @@ def run():
     torch.cuda.synchronize()
     del src, dst
     w = workloads.cylinder(n=1_000_000, seed=2, tmax=200.0, n_grid=64)
-    b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid)
+    variant, persistent, bpe, _ = KERNELS[kernel]
+    b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
+              with_fp32=bool(variant))
     stream = torch.cuda.current_stream().cuda_stream
-    cfg = LaunchConfig(steps_per_launch=16)
+    cfg = LaunchConfig(variant=variant, persistent=persistent, steps_per_launch=16)
     b.set_profiling(True)
     for _ in range(3):
         b.run(cfg, stream)
@@ -45,7 +54,7 @@ def run():
     kt = b.kernel_times(stream)
     launches = kt["step"][1]
     print(f"profiling run done: {acc} accepted steps, {nf} step-kernel field evaluations over {launches} "
-          f"step-kernel launches = {nf * 1536 / max(launches, 1):.6e} algorithmic bytes per launch", flush=True)
+          f"step-kernel launches = {nf * bpe / max(launches, 1):.6e} algorithmic bytes per launch", flush=True)
 
 
 def _per_dispatch(db, counter):
@@ -56,13 +65,14 @@ def _per_dispatch(db, counter):
     return {d: (k, float(v)) for d, k, v in rows}
 
 
-def summarize(fetch_db, write_db):
+def summarize(kernel, fetch_db, write_db, algorithmic=None):
     f = _per_dispatch(fetch_db, "FETCH_SIZE")
     w = _per_dispatch(write_db, "WRITE_SIZE")
     kib = COPY_BYTES / 1024
     # the calibration copies: device-to-device copies that wrote COPY_BYTES
     copies = [d for d in f if "copy" in f[d][0].lower() and d in w and w[d][1] > 0.9 * kib]
-    step = [d for d in f if "step_kernel" in f[d][0]]
+    pattern = KERNELS[kernel][3]
+    step = [d for d in f if pattern in f[d][0]]
     if not copies or not step:
         raise RuntimeError("calibration copy or step kernel not found")
     fc = sum(f[d][1] for d in copies) / len(copies)
@@ -71,11 +81,11 @@ def summarize(fetch_db, write_db):
     fs = sum(f[d][1] for d in step) / len(step)
     ws = sum(w[d][1] for d in step if d in w) / len(step)
     fetch, write = fs * fscale * 1024, ws * wscale * 1024
-    print("# HBM traffic per launch of step_kernel<fp64>, cylinder 1e6 neutrons, 64^3 FP64 map, 16 RK steps per call")
+    print(f"# HBM traffic per launch of {kernel}, cylinder 1e6 neutrons, 64^3 map, 16 RK steps per call")
     print("commands:")
-    print("  rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 -m pentrack_amd.profiling run")
-    print("  rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 -m pentrack_amd.profiling run")
-    print(f"  python3 -m pentrack_amd.profiling summarize {fetch_db} {write_db}")
+    print(f"  rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 -m pentrack_amd.profiling run {kernel}")
+    print(f"  rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 -m pentrack_amd.profiling run {kernel}")
+    print(f"  python3 -m pentrack_amd.profiling summarize {kernel} {fetch_db} {write_db}")
     print()
     print(f"calibration: {len(copies)} device copies of {COPY_BYTES} B ({f[copies[0]][0]})")
     print(f"  FETCH_SIZE {fc:.1f} KiB, WRITE_SIZE {wc:.1f} KiB per copy; {kib:.0f} KiB moved each way")
@@ -84,14 +94,16 @@ def summarize(fetch_db, write_db):
     print(f"  raw per dispatch: FETCH_SIZE {fs:.1f} KiB, WRITE_SIZE {ws:.1f} KiB")
     print(f"  fetch_bytes_per_launch: {fetch:.6e}")
     print(f"  write_bytes_per_launch: {write:.6e}")
+    if algorithmic:
+        print(f"algorithmic_bytes_per_launch (field gathers, from the run's device counters): {algorithmic:.6e}")
     print(f"traffic_bytes_per_launch: {fetch + write:.6e}")
 
 
 if __name__ == "__main__":
     if len(sys.argv) >= 2 and sys.argv[1] == "run":
-        run()
-    elif len(sys.argv) == 4 and sys.argv[1] == "summarize":
-        summarize(sys.argv[2], sys.argv[3])
+        run(*sys.argv[2:3])
+    elif len(sys.argv) in (5, 6) and sys.argv[1] == "summarize":
+        summarize(sys.argv[2], sys.argv[3], sys.argv[4], float(sys.argv[5]) if len(sys.argv) == 6 else None)
     else:
         print(__doc__)
         sys.exit(2)

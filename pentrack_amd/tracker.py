@@ -23,12 +23,13 @@ class LaunchConfig:
     persistent: int = 0
     grid: int = 0
     rounds: int = 0
+    graph: int = 0
     leaf_size: int = 4
 
     def native(self):
         c = _native.LaunchCfg()
-        c.variant, c.block, c.steps_per_launch, c.persistent, c.grid, c.rounds = (
-            self.variant, self.block, self.steps_per_launch, self.persistent, self.grid, self.rounds)
+        c.variant, c.block, c.steps_per_launch, c.persistent, c.grid, c.rounds, c.graph = (
+            self.variant, self.block, self.steps_per_launch, self.persistent, self.grid, self.rounds, self.graph)
         return c
 
 

@@ -133,3 +133,43 @@ def adjoint_cube(n=100000, seed=5, n_grid=64, nmaps=8, Bmax=0.5, Emax=300e-9, dt
     model = AdjointModel(grid=tuple(axes), coefs=coefs, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dt=dt)
     _, y9, _ = neutron_source(rng, n, lambda p: np.ones(len(p), bool), 1e-3, 1 - 1e-3, Emax, pol=1.0)
     return model, np.ascontiguousarray(y9[:, :6]), np.ones(nmaps)
+
+
+def uneven(n=4000000, seed=4, n_tris=50000, tmax=200.0, n_grid=64, Bmax=1.0, E_field=1e5):
+    """Mixed species with very uneven lifetimes (SURVEY config 4): 90% neutrons with
+    exponential decay (tau = 880 s), 8% protons (1-800 eV), 2% electrons (1-700 keV) in
+    a closed cylinder tessellated to about n_tris triangles, B from the coil map plus a
+    uniform E = E_field V/m along z; species proper-time limits 3e-3 s (protons) and
+    1e-5 s (electrons) as in in/config.in."""
+    from .constants import ELECTRON, PROTON
+    from .fields import uniform_efield
+    rng = np.random.default_rng(seed)
+    R, H = 0.25, 0.5
+    # side quads 2 n_phi n_z plus caps ~ 2 n_phi (2 n_r - 1): n_z ~ 0.6 n_phi, n_r = 5
+    n_phi = max(12, int(np.sqrt(n_tris / 1.2 / 1.0 + 81) - 9))
+    n_z = max(4, int(0.6 * n_phi))
+    geom = cylinder_geometry(R, H, n_phi, n_z, 5)
+    r_in = R * np.cos(np.pi / n_phi) * (1 - 1e-6)
+    inside = lambda p: np.hypot(p[:, 0], p[:, 1]) < r_in
+    kinds = np.where(rng.uniform(size=n) < 0.9, NEUTRON, np.where(rng.uniform(size=n) < 0.8, PROTON, ELECTRON))
+    kinds = kinds.astype(np.int32)
+    _, y0, spin0 = neutron_source(rng, n, inside, (-R, -R, 1e-6), (R, R, H - 1e-6), 250e-9, pol=1.0)
+    pos = y0[:, :3]
+    E = np.where(kinds == PROTON, rng.uniform(1, 800, n), np.where(kinds == ELECTRON, rng.uniform(1e3, 7e5, n),
+                                                                  rng.uniform(0, 250e-9, n)))
+    phi = rng.uniform(0, 2 * np.pi, n)
+    theta = np.arccos(rng.uniform(-1, 1, n))
+    y = np.empty((n, 9))
+    t0 = np.zeros(n)
+    for k in (NEUTRON, PROTON, ELECTRON):
+        sel = kinds == k
+        if sel.any():
+            _, yk, _ = make_states(k, pos[sel], E[sel], phi[sel], theta[sel], 1.0)
+            y[sel] = yk
+    fmap, meta = coil_map(seed, n_grid, (-R, -R, 0.0), (R, R, H), Bmax=Bmax)
+    fm = TFieldManager([fmap, uniform_efield(0.0, 0.0, E_field)])
+    species = [SpeciesConfig(NEUTRON, tau=880.0, tmax=9e99), SpeciesConfig(PROTON, tmax=3e-3),
+               SpeciesConfig(ELECTRON, tmax=1e-5)]
+    meta = dict(meta, n_phi=n_phi, n_z=n_z)
+    return Workload("uneven", geom, fm, species, tmax, kinds, t0, y, spin0, np.arange(n, dtype=np.uint64), seed,
+                    meta)
