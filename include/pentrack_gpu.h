@@ -81,8 +81,15 @@ int pt_clearance_grid(int n_tri, const float *tri_vertices, int cells_long, floa
 int pt_geometry_info(const pt_geometry *g, int *n_nodes, int *depth);
 
 /* ---- fields ------------------------------------------------------------ */
-#define PT_FIELD_TRICUBIC 1  /* TabField3 (src/field_3d.cpp) */
-#define PT_FIELD_UNIFORM_E 2 /* TEDMStaticEField (src/edmfields.cpp:175-187) */
+#define PT_FIELD_TRICUBIC 1      /* TabField3 (src/field_3d.cpp) */
+#define PT_FIELD_UNIFORM_E 2     /* TEDMStaticEField (src/edmfields.cpp:175-187): e */
+#define PT_FIELD_CONDUCTOR 3     /* TConductorField (src/conductor.cpp:9-172): p = I x1 y1 z1 x2 y2 z2 */
+#define PT_FIELD_EXPONENTIAL_X 4 /* TExponentialFieldX (src/analyticFields.cpp:11-33): p = a1 a2 a3 c1 c2 */
+#define PT_FIELD_LINEAR_Z 5      /* TLinearFieldZ (src/analyticFields.cpp:37-57): p = a1 a2 */
+#define PT_FIELD_B0_GRAD_Z 6     /* TB0GradZ (src/analyticFields.cpp:61-81): p = a1 a2 z0 */
+#define PT_FIELD_B0_GRAD_X2 7    /* TB0GradX2 (src/analyticFields.cpp:85-105): p = a1 a2 a3 z0 */
+#define PT_FIELD_B0_GRAD_XY 8    /* TB0GradXY (src/analyticFields.cpp:109-129): p = a1 a2 z0 */
+#define PT_FIELD_B0_XY 9         /* TB0_XY (src/analyticFields.cpp:133-153): p = a1 z0 */
 
 typedef struct {
     int type;
@@ -94,6 +101,13 @@ typedef struct {
     int has_b, has_v;           /* coefficient blocks per cell: Bx By Bz if has_b, then V if has_v */
     const double *coef;         /* [(nx-1)(ny-1)(nz-1)][ncomp][64], cell (ix*(ny-1)+iy)*(nz-1)+iz */
     double e[3];                /* uniform E */
+    double p[8];                /* analytic field and conductor parameters (see the type codes) */
+    /* TFieldScaler formulas of the time t (src/field.cpp:7-45) in exprtk syntax, replacing
+       bscale / escale when not NULL; formula names already resolved (src/fields.cpp ResolveFormula).
+       pt_fields_create rejects, with PT_EINVAL and a message, the forms whose exprtk value
+       the compiler does not reproduce (chains of + - or * / with constants or divisions,
+       literal 0 operands of + * / on sub-expressions, '%', '&&', '||', other functions). */
+    const char *bscale_formula, *escale_formula;
 } pt_field_desc;
 
 typedef struct pt_fields pt_fields;
@@ -122,6 +136,9 @@ int pt_tricubic_matrix(int32_t *out);
 /* spline derivatives dy/dx at the n grid points of one line, as ALGLIB's
    spline1dgriddiffcubic with parabolic end conditions (src/field_3d.cpp:356); for tests */
 int pt_spline_grid_diff(int n, const double *x, const double *y, double *d);
+/* TFieldScaler::scalingFactor (src/field.cpp:7-10): the exprtk formula f(t) compiled
+   as for a field container and evaluated at n times on the host (for tests) */
+int pt_formula_eval(const char *formula, int n, const double *t, double *out);
 
 /* ---- particles --------------------------------------------------------- */
 typedef struct {

@@ -67,6 +67,7 @@ typedef long double ld;
 #define R_SIN(a) sin(a)
 #define R_COS(a) cos(a)
 #define R_ACOS(a) acos(a)
+#define R_EXP(a) exp(a)
 #define SQ(a) pow(a, 2)
 #else
 typedef double ld;
@@ -77,6 +78,7 @@ typedef double ld;
 #define R_SIN(a) ptm_sin(a)
 #define R_COS(a) ptm_cos(a)
 #define R_ACOS(a) ptm_acos(a)
+#define R_EXP(a) ptm_exp(a)
 #define SQ(a) ((a) * (a))
 #endif
 
@@ -642,13 +644,243 @@ static void scale_at_bounds(const ptref_field *f, double x, double y, double z, 
     }
 }
 
+/* ---- field-scaling formulas (TFieldScaler, src/field.cpp:7-45) ---------------
+   trees from oracle/formula.py, which restates exprtk's parsing and node synthesis;
+   evaluated here with exprtk's operator semantics (exprtk.hpp:786-900, 1463-1496) */
+enum { F_CONST, F_T, F_NEG, F_ADD, F_SUB, F_MUL, F_DIV, F_POW, F_IPOW, F_IPOWINV,
+       F_LT, F_LE, F_GT, F_GE, F_EQ, F_NE, F_AND, F_OR, F_NOT, F_SEL,
+       F_ABS, F_SQRT, F_EXP, F_LOG, F_SIN, F_COS, F_MIN, F_MAX };
+
+/* exprtk fast_exp<T,N>: fixed chains up to N = 10, binary powering above */
+static double f_fast_exp(double v, int n) {
+    if (n == 0) return 1.0;
+    if (n == 1) return v;
+    if (n == 2) return v * v;
+    if (n == 3) return v * v * v;
+    if (n == 4) { double v2 = v * v; return v2 * v2; }
+    if (n == 5) return f_fast_exp(v, 4) * v;
+    if (n == 6) { double v3 = f_fast_exp(v, 3); return v3 * v3; }
+    if (n == 7) return f_fast_exp(v, 6) * v;
+    if (n == 8) { double v4 = f_fast_exp(v, 4); return v4 * v4; }
+    if (n == 9) return f_fast_exp(v, 8) * v;
+    if (n == 10) { double v5 = f_fast_exp(v, 5); return v5 * v5; }
+    double l = 1.0;
+    unsigned k = (unsigned)n;
+    while (k) {
+        if (k & 1) { l *= v; --k; }
+        v *= v;
+        k >>= 1;
+    }
+    return l;
+}
+
+/* C pow for a non-integer or large exponent; the fp64 mode keeps the sign rules of
+   pow around the deterministic ptm_pow (x > 0) */
+static double f_pow(double x, double y) {
+#if PTREF_REFSEM
+    return pow(x, y);
+#else
+    if (y == 0.0) return 1.0;
+    if (isnan(x) || isnan(y)) return x + y;
+    if (x == 1.0) return 1.0;
+    if (x < 0.0) {
+        double ay = fabs(y);
+        int odd;
+        if (ay >= 4503599627370496.0) odd = ay < 9007199254740992.0 && (((uint64_t)ay) & 1u);
+        else {
+            if (nearbyint(y) != y) return NAN;
+            odd = (int)(((int64_t)y) & 1);
+        }
+        double v = ptm_pow(-x, y);
+        return odd ? -v : v;
+    }
+    return ptm_pow(x, y);
+#endif
+}
+
+static double f_eval(const ptref_fnode *nd, int k, double t) {
+    const ptref_fnode *x = nd + k;
+    double a, b;
+    switch (x->op) {
+    case F_CONST: return x->v;
+    case F_T: return t;
+    case F_SEL: return f_eval(nd, x->a, t) != 0.0 ? f_eval(nd, x->b, t) : f_eval(nd, x->c, t);
+    case F_NEG: return -f_eval(nd, x->a, t);
+    case F_NOT: return f_eval(nd, x->a, t) == 0.0 ? 1.0 : 0.0;
+    case F_ABS: a = f_eval(nd, x->a, t); return a < 0.0 ? -a : a;
+    case F_SQRT: return sqrt(f_eval(nd, x->a, t));
+    case F_EXP: return R_EXP(f_eval(nd, x->a, t));
+    case F_LOG: return R_LOG(f_eval(nd, x->a, t));
+    case F_SIN: return R_SIN(f_eval(nd, x->a, t));
+    case F_COS: return R_COS(f_eval(nd, x->a, t));
+    case F_IPOW: return f_fast_exp(f_eval(nd, x->a, t), (int)x->v);
+    case F_IPOWINV: return 1.0 / f_fast_exp(f_eval(nd, x->a, t), (int)x->v);
+    default: break;
+    }
+    a = f_eval(nd, x->a, t);
+    b = f_eval(nd, x->b, t);
+    switch (x->op) {
+    case F_ADD: return a + b;
+    case F_SUB: return a - b;
+    case F_MUL: return a * b;
+    case F_DIV: return a / b;
+    case F_POW: return f_pow(a, b);
+    case F_LT: return a < b ? 1.0 : 0.0;
+    case F_LE: return a <= b ? 1.0 : 0.0;
+    case F_GT: return a > b ? 1.0 : 0.0;
+    case F_GE: return a >= b ? 1.0 : 0.0;
+    case F_EQ: return a == b ? 1.0 : 0.0;
+    case F_NE: return a != b ? 1.0 : 0.0;
+    case F_AND: return (a != 0.0 && b != 0.0) ? 1.0 : 0.0;
+    case F_OR: return (a != 0.0 || b != 0.0) ? 1.0 : 0.0;
+    case F_MIN: return b < a ? b : a;
+    case F_MAX: return a < b ? b : a;
+    }
+    return NAN;
+}
+
+/* TFieldScaler::scalingFactor (src/field.cpp:7-10) */
+static double scale_B(const ptref_field *f, double t) { return f->n_bform ? f_eval(f->bform, f->n_bform - 1, t) : f->bscale; }
+static double scale_E(const ptref_field *f, double t) { return f->n_eform ? f_eval(f->eform, f->n_eform - 1, t) : f->escale; }
+
+/* ---- analytic fields (src/analyticFields.cpp:17-147), operations in the
+   reference's order ------------------------------------------------------- */
+static void analytic_B(const ptref_field *f, double x, double y, double z, double B[3], double dB[3][3]) {
+    const double *p = f->p;
+    for (int i = 0; i < 3; ++i) { B[i] = 0; if (dB) for (int j = 0; j < 3; ++j) dB[i][j] = 0; }
+    switch (f->type) {
+    case PTREF_FIELD_EXPONENTIAL_X: { /* TExponentialFieldX::BField :17-33 */
+        double a1 = p[0], a2 = p[1], a3 = p[2], c1 = p[3], c2 = p[4];
+        double e = R_EXP(-a2 * x + a3);
+        B[0] = a1 * e + c1;
+        B[1] = y * a1 * a2 / 2 * e + c2;
+        B[2] = z * a1 * a2 / 2 * e + c2;
+        if (dB) {
+            dB[0][0] = -a2 * a1 * e;
+            dB[1][0] = -a2 * y * a1 * a2 / 2 * e;
+            dB[1][1] = a1 * a2 / 2 * e;
+            dB[2][0] = -a2 * z * a1 * a2 / 2 * e;
+            dB[2][2] = a1 * a2 / 2 * e;
+        }
+        break;
+    }
+    case PTREF_FIELD_LINEAR_Z: /* TLinearFieldZ::BField :41-57 */
+        B[2] = p[0] * x + p[1];
+        if (dB) dB[2][0] = p[0];
+        break;
+    case PTREF_FIELD_B0_GRAD_Z: { /* TB0GradZ::BField :65-81 */
+        double a1 = p[0], a2 = p[1], z0 = p[2];
+        B[0] = -(a1 * z + a2) / 2 * x;
+        B[1] = -(a1 * z + a2) / 2 * y;
+        B[2] = a1 * z * z / 2 + a2 * z + z0;
+        if (dB) {
+            dB[0][0] = -(a1 * z + a2) / 2;
+            dB[0][2] = -a1 * x / 2;
+            dB[1][1] = -(a1 * z + a2) / 2;
+            dB[1][2] = -a1 * y / 2;
+            dB[2][2] = a1 * z + a2;
+        }
+        break;
+    }
+    case PTREF_FIELD_B0_GRAD_X2: { /* TB0GradX2::BField :89-105 */
+        double a1 = p[0], a2 = p[1], a3 = p[2], z0 = p[3];
+        B[0] = -a1 / 6 * x * x * x - a2 / 4 * x * x - a3 / 2 * x;
+        B[1] = -(a1 * x * x + a2 * x + a3) / 2 * y;
+        B[2] = (a1 * x * x + a2 * x + a3) * z + z0;
+        if (dB) {
+            dB[0][0] = -(a1 * x * x + a2 * x + a3) / 2;
+            dB[1][0] = (-a1 * x - a2 / 2) * y;
+            dB[1][1] = -(a1 * x * x + a2 * x + a3) / 2;
+            dB[2][0] = (2 * a1 * x + a2) * z;
+            dB[2][2] = a1 * x * x + a2 * x + a3;
+        }
+        break;
+    }
+    case PTREF_FIELD_B0_GRAD_XY: { /* TB0GradXY::BField :113-129 */
+        double a1 = p[0], a2 = p[1], z0 = p[2];
+        B[0] = -a1 * x * x * y / 4 - a2 * x / 2;
+        B[1] = -a1 * x * y * y / 4 - a2 * y / 2;
+        B[2] = a1 * x * y * z + a2 * z + z0;
+        if (dB) {
+            dB[0][0] = -(a1 * x * y + a2) / 2;
+            dB[0][1] = -a1 * x * x / 4;
+            dB[1][0] = -a1 * y * y / 4;
+            dB[1][1] = -(a1 * x * y + a2) / 2;
+            dB[2][0] = a1 * y * z;
+            dB[2][1] = a1 * x * z;
+            dB[2][2] = a1 * x * y + a2;
+        }
+        break;
+    }
+    case PTREF_FIELD_B0_XY: { /* TB0_XY::BField :137-153 */
+        double a1 = p[0], z0 = p[1];
+        B[0] = a1 * y * z;
+        B[1] = a1 * x * z;
+        B[2] = a1 * x * y + z0;
+        if (dB) {
+            dB[0][1] = a1 * z; dB[0][2] = a1 * y;
+            dB[1][0] = a1 * z; dB[1][2] = a1 * x;
+            dB[2][0] = a1 * y; dB[2][1] = a1 * x;
+        }
+        break;
+    }
+    }
+}
+
+/* ---- finite straight wire (TConductorField, src/conductor.cpp:9-172) --------
+   Biot-Savart law of a segment A->B carrying I: with e the unit vector along the
+   wire, r1 = P - A, r2 = P - B, w = e x r1 and d^2 = |w|^2,
+     B = k (c1 - c2) / d^2 w,  c_i = (e . r_i) / |r_i|,  k = mu0 I / (4 pi),
+   and its Jacobian from d(d^2)/dP = 2 (r1 - (e.r1) e), dc_i/dP = e/|r_i| - (e.r_i) r_i/|r_i|^3,
+   dw/dP = [e]x.  The reference evaluates the same field with a different
+   (computer-algebra) operation order; k is rounded from long double as there. */
+static double conductor_k(double current) {
+    long double pi = 3.1415926535897932384626L, mu0 = 4 * pi * 1e-7L;
+    return (double)(mu0 * current / (4 * pi));
+}
+
+static void conductor_B(const ptref_field *f, double x, double y, double z, double B[3], double dB[3][3]) {
+    const double *p = f->p;   /* I, x1 y1 z1, x2 y2 z2 */
+    for (int i = 0; i < 3; ++i) { B[i] = 0; if (dB) for (int j = 0; j < 3; ++j) dB[i][j] = 0; }
+    if (p[0] == 0) return;
+    const double k = conductor_k(p[0]);
+    double l0 = p[4] - p[1], l1 = p[5] - p[2], l2 = p[6] - p[3];
+    double L = sqrt(l0 * l0 + l1 * l1 + l2 * l2);
+    double e0 = l0 / L, e1 = l1 / L, e2 = l2 / L;
+    double r10 = x - p[1], r11 = y - p[2], r12 = z - p[3];
+    double r20 = x - p[4], r21 = y - p[5], r22 = z - p[6];
+    double n1 = sqrt(r10 * r10 + r11 * r11 + r12 * r12), n2 = sqrt(r20 * r20 + r21 * r21 + r22 * r22);
+    double s1 = e0 * r10 + e1 * r11 + e2 * r12, s2 = e0 * r20 + e1 * r21 + e2 * r22;
+    double w0 = e1 * r12 - e2 * r11, w1 = e2 * r10 - e0 * r12, w2 = e0 * r11 - e1 * r10;
+    double d2 = w0 * w0 + w1 * w1 + w2 * w2;
+    double c = s1 / n1 - s2 / n2;
+    double g = k * c / d2;
+    B[0] = g * w0; B[1] = g * w1; B[2] = g * w2;
+    if (dB) {
+        double q1 = s1 / (n1 * n1 * n1), q2 = s2 / (n2 * n2 * n2);
+        double dc[3], dd[3], dg[3];
+        double rp0 = r10 - s1 * e0, rp1 = r11 - s1 * e1, rp2 = r12 - s1 * e2;
+        dc[0] = (e0 / n1 - q1 * r10) - (e0 / n2 - q2 * r20);
+        dc[1] = (e1 / n1 - q1 * r11) - (e1 / n2 - q2 * r21);
+        dc[2] = (e2 / n1 - q1 * r12) - (e2 / n2 - q2 * r22);
+        dd[0] = 2 * rp0; dd[1] = 2 * rp1; dd[2] = 2 * rp2;
+        for (int j = 0; j < 3; ++j) dg[j] = k * (dc[j] / d2 - c * dd[j] / (d2 * d2));
+        double E[3][3] = {{0, -e2, e1}, {e2, 0, -e0}, {-e1, e0, 0}};
+        double w[3] = {w0, w1, w2};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) dB[i][j] = g * E[i][j] + w[i] * dg[j];
+    }
+}
+
 /* TFieldContainer::BField (src/field.cpp:139-153) */
 static void container_B(track_t *tp, const ptref_field *f, double x, double y, double z, double t, double B[3], double dB[3][3]) {
-    (void)t;
-    if (!in_bounds(f, x, y, z) || f->bscale == 0.) {
+    double bs = in_bounds(f, x, y, z) ? scale_B(f, t) : 0.;
+    if (!in_bounds(f, x, y, z) || bs == 0.) {
         for (int i = 0; i < 3; ++i) { B[i] = 0.; if (dB) for (int j = 0; j < 3; ++j) dB[i][j] = 0.; }
         return;
     }
+    if (f->type == PTREF_FIELD_CONDUCTOR) conductor_B(f, x, y, z, B, dB);
+    else if (f->type >= PTREF_FIELD_EXPONENTIAL_X) analytic_B(f, x, y, z, B, dB);
     if (f->type == PTREF_FIELD_TRICUBIC && f->has_b) {
         long cell; double r[3], dist[3];
         if (tric_locate(f, x, y, z, &cell, r, dist)) {
@@ -667,16 +899,16 @@ static void container_B(track_t *tp, const ptref_field *f, double x, double y, d
     }
     /* TFieldScaler::scaleVectorField */
     for (int i = 0; i < 3; ++i) {
-        B[i] *= f->bscale;
-        if (dB) { dB[i][0] *= f->bscale; dB[i][1] *= f->bscale; dB[i][2] *= f->bscale; }
+        B[i] *= bs;
+        if (dB) { dB[i][0] *= bs; dB[i][1] *= bs; dB[i][2] *= bs; }
     }
     for (int i = 0; i < 3; ++i) scale_at_bounds(f, x, y, z, &B[i], dB ? dB[i] : 0);
 }
 
 /* TFieldContainer::EField (src/field.cpp:155-169) */
 static void container_E(track_t *tp, const ptref_field *f, double x, double y, double z, double t, double *V, double E[3]) {
-    (void)t;
-    if (!in_bounds(f, x, y, z) || f->escale == 0.) {
+    double es = in_bounds(f, x, y, z) ? scale_E(f, t) : 0.;
+    if (!in_bounds(f, x, y, z) || es == 0.) {
         *V = 0.; E[0] = E[1] = E[2] = 0.;
         return;
     }
@@ -698,8 +930,8 @@ static void container_E(track_t *tp, const ptref_field *f, double x, double y, d
         for (int i = 0; i < 3; i++) E[i] = -dV[i];
     }
     /* TFieldScaler::scaleScalarField */
-    *V *= f->escale;
-    E[0] *= f->escale; E[1] *= f->escale; E[2] *= f->escale;
+    *V *= es;
+    E[0] *= es; E[1] *= es; E[2] *= es;
     scale_at_bounds(f, x, y, z, V, E);
 }
 

@@ -32,11 +32,26 @@ typedef struct {
                                  CorrelLength InternalBField ModifiedLambertProb LossPerBounce */
 } ptref_geom;
 
-enum { PTREF_FIELD_TRICUBIC = 1, PTREF_FIELD_UNIFORM_E = 2 };
+enum {
+    PTREF_FIELD_TRICUBIC = 1, PTREF_FIELD_UNIFORM_E = 2, PTREF_FIELD_CONDUCTOR = 3,
+    PTREF_FIELD_EXPONENTIAL_X = 4, PTREF_FIELD_LINEAR_Z = 5, PTREF_FIELD_B0_GRAD_Z = 6,
+    PTREF_FIELD_B0_GRAD_X2 = 7, PTREF_FIELD_B0_GRAD_XY = 8, PTREF_FIELD_B0_XY = 9
+};
+
+/* one node of a field-scaling formula tree (oracle/formula.py nodes()): children
+   precede parents, the root is the last node */
+typedef struct {
+    int op;                   /* oracle/formula.py codes: 0 const, 1 t, 2 neg, ... */
+    int a, b, c;              /* child node indices */
+    double v;                 /* constant value; exponent count of an integer power */
+} ptref_fnode;
 
 typedef struct {
     int type;
-    double bscale, escale;    /* constant scale factors of the container */
+    double bscale, escale;    /* constant scale factors of the container (used when no formula) */
+    int n_bform, n_eform;     /* formula trees of t for the scale factors (0: constant) */
+    const ptref_fnode *bform, *eform;
+    double p[8];              /* analytic field parameters (include/pentrack_gpu.h PT_FIELD_*) */
     double box[6];            /* xmax xmin ymax ymin zmax zmin; no bounds unless min<max on all axes */
     double bw;                /* boundary width of the smooth taper */
     /* tricubic map */

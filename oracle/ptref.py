@@ -21,9 +21,27 @@ class Geom(C.Structure):
     ]
 
 
+class FNode(C.Structure):
+    _fields_ = [("op", C.c_int), ("a", C.c_int), ("b", C.c_int), ("c", C.c_int), ("v", C.c_double)]
+
+
+def formula_nodes(formula):
+    """Formula tree of oracle/formula.py as a ctypes node array (None for no formula)."""
+    if formula is None:
+        return None
+    from . import formula as F
+    nd = F.nodes(F.parse(formula))
+    arr = (FNode * len(nd))()
+    for k, (op, a, b, c, v) in enumerate(nd):
+        arr[k].op, arr[k].a, arr[k].b, arr[k].c, arr[k].v = op, a, b, c, v
+    return arr
+
+
 class Field(C.Structure):
     _fields_ = [
         ("type", C.c_int), ("bscale", C.c_double), ("escale", C.c_double),
+        ("n_bform", C.c_int), ("n_eform", C.c_int), ("bform", C.POINTER(FNode)), ("eform", C.POINTER(FNode)),
+        ("p", C.c_double * 8),
         ("box", C.c_double * 6), ("bw", C.c_double),
         ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
         ("xg", C.POINTER(C.c_double)), ("yg", C.POINTER(C.c_double)), ("zg", C.POINTER(C.c_double)),
@@ -96,11 +114,20 @@ class Problem:
     """Geometry and fields of a workload in the oracle's layout (keeps buffers alive)."""
 
     def __init__(self, geometry, fields):
+        self._keep = []
+        if geometry is not None:
+            self._set_geometry(geometry)
+        n = max(1, len(fields.containers))
+        self.fields = (Field * n)()
+        self.n_fields = len(fields.containers)
+        self._set_fields(fields)
+
+    def _set_geometry(self, geometry):
         a = geometry.arrays()
         # the oracle takes the snapped double vertices
         tri = np.ascontiguousarray(a["tri"].astype(np.float64))
         tri[np.abs(tri) < 1e-8] = 0.0
-        self._keep = [tri, a["tri_solid"], a["solid_id"], a["solid_mat"], a["ign_off"], a["ign"], a["mats"]]
+        self._keep += [tri, a["tri_solid"], a["solid_id"], a["solid_mat"], a["ign_off"], a["ign"], a["mats"]]
         g = Geom()
         g.n_tri = len(tri)
         g.tri = _dp(tri)
@@ -114,9 +141,8 @@ class Problem:
         g.n_mat = len(a["mats"])
         g.mats = _dp(a["mats"])
         self.geom = g
-        n = max(1, len(fields.containers))
-        self.fields = (Field * n)()
-        self.n_fields = len(fields.containers)
+
+    def _set_fields(self, fields):
         for k, c in enumerate(fields.containers):
             f = self.fields[k]
             f.type, f.bscale, f.escale, f.bw = c.type, c.bscale, c.escale, c.bw
@@ -124,6 +150,14 @@ class Problem:
                 f.box[i] = c.box[i]
             for i in range(3):
                 f.e[i] = c.e[i]
+            for i, v in enumerate(c.params):
+                f.p[i] = v
+            for attr, nattr, form in (("bform", "n_bform", c.bformula), ("eform", "n_eform", c.eformula)):
+                arr = formula_nodes(form)
+                if arr is not None:
+                    self._keep.append(arr)
+                    setattr(f, attr, C.cast(arr, C.POINTER(FNode)))
+                    setattr(f, nattr, len(arr))
             if c.grid is not None:
                 xg, yg, zg = (np.ascontiguousarray(g_, dtype=np.float64) for g_ in c.grid)
                 coef = np.ascontiguousarray(c.coef, dtype=np.float64)
@@ -229,11 +263,22 @@ def get_solids(geom, t, p, mode="fp64"):
 
 
 def field_eval(problem, pts, t=0.0, mode="fp64"):
+    """TFieldManager::BField / EField at pts [n][3] and time t (a number or one per point);
+    problem: a Problem, or a TFieldManager (fields only)."""
+    if not isinstance(problem, Problem):
+        problem = Problem(None, problem)
     L = load(mode)
     pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
     n = len(pts)
     B = np.zeros((n, 3)); dB = np.zeros((n, 3, 3)); V = np.zeros(n); E = np.zeros((n, 3))
-    L.ptref_field_eval(problem.n_fields, problem.fields, n, _dp(pts), float(t), _dp(B), _dp(dB), _dp(V), _dp(E))
+    if np.ndim(t) == 0:
+        L.ptref_field_eval(problem.n_fields, problem.fields, n, _dp(pts), float(t), _dp(B), _dp(dB), _dp(V), _dp(E))
+    else:
+        for i in range(n):
+            b, db, v, e = (np.zeros(3), np.zeros(9), np.zeros(1), np.zeros(3))
+            L.ptref_field_eval(problem.n_fields, problem.fields, 1, _dp(pts[i]), float(t[i]), _dp(b), _dp(db), _dp(v),
+                               _dp(e))
+            B[i], dB[i], V[i], E[i] = b, db.reshape(3, 3), v[0], e
     return B, dB, V, E
 
 

@@ -1,8 +1,9 @@
 #!/bin/bash
 # Builds oracle/_ref/libref_islands.so from the reference's own Boost-free sources,
 # where they lie under /root/reference (ALGLIB 3.15.0 interpolation and its
-# dependencies, libtricubic 1.0), plus oracle/ref_wrap.cpp.  Test infrastructure
-# only; outputs go to oracle/_ref/ (git-ignored).
+# dependencies, libtricubic 1.0, src/field.cpp and src/analyticFields.cpp with
+# the vendored exprtk), plus oracle/ref_wrap.cpp and oracle/ref_wrap_fields.cpp.
+# Test infrastructure only; outputs go to oracle/_ref/ (git-ignored).
 set -e
 REF=${PT_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -21,6 +22,14 @@ done
 g++ $FLAGS -c "$TRI/libtricubic.cpp" -o "$OUT/obj/libtricubic.o" & pids+=($!)
 g++ $FLAGS -c "$TRI/tricubic_utils.cpp" -o "$OUT/obj/tricubic_utils.o" & pids+=($!)
 g++ $FLAGS -c "$HERE/ref_wrap.cpp" -o "$OUT/obj/ref_wrap.o" & pids+=($!)
+# field classes: the reference's include/ and exprtk/ on the include path only here
+FFLAGS="-O1 -fPIC -std=c++11 -ffp-contract=off -w -I$REF/include -I$REF/exprtk"
+for f in field analyticFields; do
+    if [ ! -f "$OUT/obj/$f.o" ] || [ "$REF/src/$f.cpp" -nt "$OUT/obj/$f.o" ]; then
+        g++ $FFLAGS -c "$REF/src/$f.cpp" -o "$OUT/obj/$f.o" & pids+=($!)
+    fi
+done
+g++ $FFLAGS -c "$HERE/ref_wrap_fields.cpp" -o "$OUT/obj/ref_wrap_fields.o" & pids+=($!)
 for p in "${pids[@]}"; do wait "$p"; done
 g++ -shared -o "$OUT/libref_islands.so" "$OUT"/obj/*.o
 echo "ref_build: $OUT/libref_islands.so"

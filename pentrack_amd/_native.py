@@ -12,8 +12,18 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpentrack_gpu.so")
 
 PT_OK = 0
+PT_EINVAL = -1
+PT_ENOMEM = -2
+PT_EDEVICE = -3
 PT_FIELD_TRICUBIC = 1
 PT_FIELD_UNIFORM_E = 2
+PT_FIELD_CONDUCTOR = 3
+PT_FIELD_EXPONENTIAL_X = 4
+PT_FIELD_LINEAR_Z = 5
+PT_FIELD_B0_GRAD_Z = 6
+PT_FIELD_B0_GRAD_X2 = 7
+PT_FIELD_B0_GRAD_XY = 8
+PT_FIELD_B0_XY = 9
 PT_VARIANT_FP64 = 0
 PT_VARIANT_MIXED = 1
 
@@ -41,6 +51,8 @@ class FieldDesc(C.Structure):
         ("has_b", C.c_int), ("has_v", C.c_int),
         ("coef", C.POINTER(C.c_double)),
         ("e", C.c_double * 3),
+        ("p", C.c_double * 8),
+        ("bscale_formula", C.c_char_p), ("escale_formula", C.c_char_p),
     ]
 
 
@@ -102,7 +114,7 @@ EXPORTS = [
     "pt_batch_download", "pt_batch_step_counts", "pt_track_batch", "pt_batch_set_profiling",
     "pt_batch_kernel_times",
     "pt_request_quit", "pt_last_error", "pt_clearance_grid", "pt_tricubic_matrix",
-    "pt_spline_grid_diff", "pt_rk4_forward_ckpt", "pt_rk4_adjoint",
+    "pt_spline_grid_diff", "pt_rk4_forward_ckpt", "pt_rk4_adjoint", "pt_formula_eval",
 ]
 
 _lib = None
@@ -152,6 +164,7 @@ def lib():
     L.pt_spline_grid_diff.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pt_clearance_grid.argtypes = [C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float),
                                     C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.pt_formula_eval.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pt_request_quit.restype = None
     L.pt_last_error.restype = C.c_char_p
     _lib = L

@@ -77,8 +77,12 @@ struct Geom {
 };
 
 struct Field {
-    int type;                 /* 1 tricubic, 2 uniform E */
+    int type;                 /* PT_FIELD_* of include/pentrack_gpu.h */
     int has_b, has_v, ncomp;
+    int blen, elen;           /* lengths of the scale-formula programs (0: constant bscale / escale) */
+    const uint32_t *bcode, *ecode;
+    const double *bcst, *ecst;
+    double p[8];              /* analytic parameters; conductor: I x1 y1 z1 x2 y2 z2, mu0 I / (4 pi) */
     int nx, ny, nz;
     double bscale, escale, bw;
     double box[6];

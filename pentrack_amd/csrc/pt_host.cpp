@@ -18,7 +18,12 @@
 
 #include "../../include/pentrack_gpu.h"
 #include "pt_device.h"
+#include "pt_formula.h"
 #include "pt_kernels.h"
+
+namespace ptf {
+bool compile(const char *formula, std::vector<uint32_t> &code, std::vector<double> &cst, std::string &err);
+}
 
 namespace {
 
@@ -444,6 +449,39 @@ extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32
         f.bscale = d.bscale; f.escale = d.escale; f.bw = d.bw;
         for (int a = 0; a < 6; ++a) f.box[a] = d.box[a];
         for (int a = 0; a < 3; ++a) f.e[a] = d.e[a];
+        for (int a = 0; a < 8; ++a) f.p[a] = d.p[a];
+        {
+            /* TFieldBoundaryBox constructor check (src/field.cpp:116-125) */
+            const double *b = d.box;
+            if (d.bw < 0 || d.bw > (b[0] - b[1]) / 2 || d.bw > (b[2] - b[3]) / 2 || d.bw > (b[4] - b[5]) / 2)
+                return fail(PT_EINVAL, "pt_fields_create: You defined a field with an invalid boundary. Check field parameters.");
+        }
+        /* TFieldScaler formulas (src/field.cpp:34-45): compiled to device programs */
+        const char *forms[2] = {d.bscale_formula, d.escale_formula};
+        for (int w = 0; w < 2; ++w) {
+            if (!forms[w]) continue;
+            std::vector<uint32_t> code;
+            std::vector<double> cst;
+            std::string err;
+            if (!ptf::compile(forms[w], code, cst, err))
+                return fail(PT_EINVAL, "pt_fields_create: field " + std::to_string(k) + ": " + err);
+            uint32_t *dcode;
+            double *dcst;
+            int r = F->mem.upload(&dcode, code.data(), code.size());
+            if (r) return r;
+            r = F->mem.upload(&dcst, cst.data(), cst.size());
+            if (r) return r;
+            if (w == 0) { f.bcode = dcode; f.bcst = dcst; f.blen = (int)code.size(); }
+            else { f.ecode = dcode; f.ecst = dcst; f.elen = (int)code.size(); }
+        }
+        if (d.type == PT_FIELD_CONDUCTOR) {
+            /* TConductorField constructor (src/conductor.cpp:9-19) and the prefactor
+               mu0 I / (4 pi) in long double (src/conductor.cpp:35, src/globals.cpp:13,19) */
+            if (d.p[1] == d.p[4] && d.p[2] == d.p[5] && d.p[3] == d.p[6])
+                return fail(PT_EINVAL, "pt_fields_create: You have defined a conductor with length zero! Aborting...");
+            const long double pi = 3.1415926535897932384626L, mu0 = 4 * pi * 1e-7L;
+            f.p[7] = (double)(mu0 * (long double)d.p[0] / (4 * pi));
+        }
         if (d.type == PT_FIELD_TRICUBIC) {
             if (d.nx < 2 || d.ny < 2 || d.nz < 2 || !d.xg || !d.yg || !d.zg || !d.coef || (!d.has_b && !d.has_v))
                 return fail(PT_EINVAL, "pt_fields_create: bad tricubic descriptor");
@@ -483,7 +521,7 @@ extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32
                 if (r) return r;
                 f.coef32 = dc32;
             }
-        } else if (d.type != PT_FIELD_UNIFORM_E) {
+        } else if (d.type < PT_FIELD_UNIFORM_E || d.type > PT_FIELD_B0_XY) {
             return fail(PT_EINVAL, "pt_fields_create: unknown field type");
         }
         F->host[k] = f;
@@ -684,6 +722,17 @@ extern "C" int pt_spline_grid_diff(int n, const double *x, const double *y, doub
     if (n < 2 || !x || !y || !d) return fail(PT_EINVAL, "pt_spline_grid_diff: bad arguments");
     std::vector<double> buf;
     spline_grid_diff(x, y, n, d, buf);
+    return PT_OK;
+}
+
+/* field-scaling formula f(t) evaluated on the host by the program the device runs */
+extern "C" int pt_formula_eval(const char *formula, int n, const double *t, double *out) {
+    if (!formula || n < 0 || (n > 0 && (!t || !out))) return fail(PT_EINVAL, "pt_formula_eval: bad arguments");
+    std::vector<uint32_t> code;
+    std::vector<double> cst;
+    std::string err;
+    if (!ptf::compile(formula, code, cst, err)) return fail(PT_EINVAL, "pt_formula_eval: " + err);
+    for (int i = 0; i < n; ++i) out[i] = ptf_run(code.data(), (int)code.size(), cst.data(), t[i]);
     return PT_OK;
 }
 

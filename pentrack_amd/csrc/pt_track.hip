@@ -15,7 +15,9 @@
  * the reference's order, with the reference's long double constants rounded
  * to double, so results match the FP64 CPU restatement bit for bit.
  */
+#include "../../include/pentrack_gpu.h"
 #include "pt_device.h"
+#include "pt_formula.h"
 #include "pt_kernels.h"
 
 namespace ptd {
@@ -196,14 +198,157 @@ PTD_NOINLINE FG scale_at_bounds(const Field &f, double x, double y, double z, do
 
 PTD bool has_box(const Field &f) { return f.box[1] < f.box[0] && f.box[3] < f.box[2] && f.box[5] < f.box[4]; }
 
-/* TFieldContainer::BField (src/field.cpp:139-153) accumulated into the manager sum
-   (TFieldManager::BField, src/fields.cpp:139-164) */
+/* TFieldScaler::scalingFactor (src/field.cpp:7-10) from the compiled formula */
+PTD double scale_formula(const uint32_t *code, int n, const double *cst, double t) {
+    return ptf_run(code, n, cst, t);
+}
+PTD double scale_B(const Field &f, double t) { return f.blen ? scale_formula(f.bcode, f.blen, f.bcst, t) : f.bscale; }
+PTD double scale_E(const Field &f, double t) { return f.elen ? scale_formula(f.ecode, f.elen, f.ecst, t) : f.escale; }
+
+/* analytic fields (src/analyticFields.cpp:17-147, operations in the reference's order) and
+   the finite straight wire (src/conductor.cpp:21-172) by the Biot-Savart law of a segment
+   A->B: with e the unit vector along the wire, r1 = P - A, r2 = P - B, w = e x r1,
+   d^2 = |w|^2, c_i = (e . r_i)/|r_i|:  B = k (c1 - c2)/d^2 w, k = mu0 I/(4 pi), and the
+   Jacobian from d(d^2)/dP = 2 (r1 - (e.r1) e), dc_i/dP = e/|r_i| - (e.r_i) r_i/|r_i|^3,
+   dw/dP = [e]x.  B and dB (row-major 3x3) are zeroed first. */
+PTD void analytic_B(const Field &f, double x, double y, double z, double *B, double *dB, bool grad) {
+    const double *p = f.p;
+    for (int i = 0; i < 3; ++i) B[i] = 0;
+    if (grad)
+        for (int i = 0; i < 9; ++i) dB[i] = 0;
+    switch (f.type) {
+    case PT_FIELD_CONDUCTOR: {
+        if (p[0] == 0) return;
+        const double k = p[7];
+        double l0 = p[4] - p[1], l1 = p[5] - p[2], l2 = p[6] - p[3];
+        double L = __builtin_sqrt(l0 * l0 + l1 * l1 + l2 * l2);
+        double e0 = l0 / L, e1 = l1 / L, e2 = l2 / L;
+        double r10 = x - p[1], r11 = y - p[2], r12 = z - p[3];
+        double r20 = x - p[4], r21 = y - p[5], r22 = z - p[6];
+        double n1 = __builtin_sqrt(r10 * r10 + r11 * r11 + r12 * r12), n2 = __builtin_sqrt(r20 * r20 + r21 * r21 + r22 * r22);
+        double s1 = e0 * r10 + e1 * r11 + e2 * r12, s2 = e0 * r20 + e1 * r21 + e2 * r22;
+        double w0 = e1 * r12 - e2 * r11, w1 = e2 * r10 - e0 * r12, w2 = e0 * r11 - e1 * r10;
+        double d2 = w0 * w0 + w1 * w1 + w2 * w2;
+        double c = s1 / n1 - s2 / n2;
+        double g = k * c / d2;
+        B[0] = g * w0; B[1] = g * w1; B[2] = g * w2;
+        if (grad) {
+            double q1 = s1 / (n1 * n1 * n1), q2 = s2 / (n2 * n2 * n2);
+            double rp0 = r10 - s1 * e0, rp1 = r11 - s1 * e1, rp2 = r12 - s1 * e2;
+            double dc0 = (e0 / n1 - q1 * r10) - (e0 / n2 - q2 * r20);
+            double dc1 = (e1 / n1 - q1 * r11) - (e1 / n2 - q2 * r21);
+            double dc2 = (e2 / n1 - q1 * r12) - (e2 / n2 - q2 * r22);
+            double dd0 = 2 * rp0, dd1 = 2 * rp1, dd2 = 2 * rp2;
+            double dg0 = k * (dc0 / d2 - c * dd0 / (d2 * d2));
+            double dg1 = k * (dc1 / d2 - c * dd1 / (d2 * d2));
+            double dg2 = k * (dc2 / d2 - c * dd2 / (d2 * d2));
+            const double E[9] = {0, -e2, e1, e2, 0, -e0, -e1, e0, 0};
+            const double w[3] = {w0, w1, w2};
+            const double dg[3] = {dg0, dg1, dg2};
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) dB[3 * i + j] = g * E[3 * i + j] + w[i] * dg[j];
+        }
+        return;
+    }
+    case PT_FIELD_EXPONENTIAL_X: {
+        double a1 = p[0], a2 = p[1], a3 = p[2], c1 = p[3], c2 = p[4];
+        double e = ptm_exp(-a2 * x + a3);
+        B[0] = a1 * e + c1;
+        B[1] = y * a1 * a2 / 2 * e + c2;
+        B[2] = z * a1 * a2 / 2 * e + c2;
+        if (grad) {
+            dB[0] = -a2 * a1 * e;
+            dB[3] = -a2 * y * a1 * a2 / 2 * e;
+            dB[4] = a1 * a2 / 2 * e;
+            dB[6] = -a2 * z * a1 * a2 / 2 * e;
+            dB[8] = a1 * a2 / 2 * e;
+        }
+        return;
+    }
+    case PT_FIELD_LINEAR_Z:
+        B[2] = p[0] * x + p[1];
+        if (grad) dB[6] = p[0];
+        return;
+    case PT_FIELD_B0_GRAD_Z: {
+        double a1 = p[0], a2 = p[1], z0 = p[2];
+        B[0] = -(a1 * z + a2) / 2 * x;
+        B[1] = -(a1 * z + a2) / 2 * y;
+        B[2] = a1 * z * z / 2 + a2 * z + z0;
+        if (grad) {
+            dB[0] = -(a1 * z + a2) / 2;
+            dB[2] = -a1 * x / 2;
+            dB[4] = -(a1 * z + a2) / 2;
+            dB[5] = -a1 * y / 2;
+            dB[8] = a1 * z + a2;
+        }
+        return;
+    }
+    case PT_FIELD_B0_GRAD_X2: {
+        double a1 = p[0], a2 = p[1], a3 = p[2], z0 = p[3];
+        B[0] = -a1 / 6 * x * x * x - a2 / 4 * x * x - a3 / 2 * x;
+        B[1] = -(a1 * x * x + a2 * x + a3) / 2 * y;
+        B[2] = (a1 * x * x + a2 * x + a3) * z + z0;
+        if (grad) {
+            dB[0] = -(a1 * x * x + a2 * x + a3) / 2;
+            dB[3] = (-a1 * x - a2 / 2) * y;
+            dB[4] = -(a1 * x * x + a2 * x + a3) / 2;
+            dB[6] = (2 * a1 * x + a2) * z;
+            dB[8] = a1 * x * x + a2 * x + a3;
+        }
+        return;
+    }
+    case PT_FIELD_B0_GRAD_XY: {
+        double a1 = p[0], a2 = p[1], z0 = p[2];
+        B[0] = -a1 * x * x * y / 4 - a2 * x / 2;
+        B[1] = -a1 * x * y * y / 4 - a2 * y / 2;
+        B[2] = a1 * x * y * z + a2 * z + z0;
+        if (grad) {
+            dB[0] = -(a1 * x * y + a2) / 2;
+            dB[1] = -a1 * x * x / 4;
+            dB[3] = -a1 * y * y / 4;
+            dB[4] = -(a1 * x * y + a2) / 2;
+            dB[6] = a1 * y * z;
+            dB[7] = a1 * x * z;
+            dB[8] = a1 * x * y + a2;
+        }
+        return;
+    }
+    case PT_FIELD_B0_XY: {
+        double a1 = p[0], z0 = p[1];
+        B[0] = a1 * y * z;
+        B[1] = a1 * x * z;
+        B[2] = a1 * x * y + z0;
+        if (grad) {
+            dB[1] = a1 * z; dB[2] = a1 * y;
+            dB[3] = a1 * z; dB[5] = a1 * x;
+            dB[6] = a1 * y; dB[7] = a1 * x;
+        }
+        return;
+    }
+    default:
+        return;
+    }
+}
+
+/* TFieldContainer::BField (src/field.cpp:139-153) with the scale factor bs already
+   evaluated, accumulated into the manager sum (TFieldManager::BField, src/fields.cpp:139-164);
+   tricubic maps and, through analytic_B, the analytic fields and conductors */
 template <bool MIXED>
-PTD void container_B(const Field &f, double x, double y, double z, double B[3], double dB[3][3], bool grad, long long &nfield) {
+PTD void container_B(const Field &f, double x, double y, double z, double bs, double B[3], double dB[3][3], bool grad,
+                     long long &nfield) {
     double Bt[3] = {0, 0, 0};
     double dBt[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    if (in_bounds(f, x, y, z) && f.bscale != 0.) {
-        if (f.type == 1 && f.has_b) {
+    if (in_bounds(f, x, y, z) && bs != 0.) {
+        if (f.type >= PT_FIELD_CONDUCTOR) {
+            double Ba[3], dBa[9];
+            analytic_B(f, x, y, z, Ba, dBa, grad);
+            Bt[0] = Ba[0]; Bt[1] = Ba[1]; Bt[2] = Ba[2];
+            if (grad) {
+                dBt[0][0] = dBa[0]; dBt[0][1] = dBa[1]; dBt[0][2] = dBa[2];
+                dBt[1][0] = dBa[3]; dBt[1][1] = dBa[4]; dBt[1][2] = dBa[5];
+                dBt[2][0] = dBa[6]; dBt[2][1] = dBa[7]; dBt[2][2] = dBa[8];
+            }
+        } else if (f.type == 1 && f.has_b) {
             long cell; double r[3], dist[3];
             if (tric_locate(f, x, y, z, cell, r, dist)) {
                 ++nfield;
@@ -227,8 +372,8 @@ PTD void container_B(const Field &f, double x, double y, double z, double B[3], 
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            Bt[i] *= f.bscale;
-            if (grad) { dBt[i][0] *= f.bscale; dBt[i][1] *= f.bscale; dBt[i][2] *= f.bscale; }
+            Bt[i] *= bs;
+            if (grad) { dBt[i][0] *= bs; dBt[i][1] *= bs; dBt[i][2] *= bs; }
         }
         if (has_box(f)) {
 #pragma unroll
@@ -250,49 +395,62 @@ PTD void container_B(const Field &f, double x, double y, double z, double B[3], 
 }
 
 template <bool MIXED>
-PTD void field_B(const Params &P, double x, double y, double z, double B[3], double dB[3][3], bool grad, long long &nfield) {
+PTD void field_B(const Params &P, double x, double y, double z, double t, double B[3], double dB[3][3], bool grad, long long &nfield) {
 #pragma unroll
     for (int i = 0; i < 3; i++) { B[i] = 0; if (grad) { dB[i][0] = 0; dB[i][1] = 0; dB[i][2] = 0; } }
-    for (int k = 0; k < P.n_fields; ++k) container_B<MIXED>(P.fields[k], x, y, z, B, dB, grad, nfield);
+    for (int k = 0; k < P.n_fields; ++k) {
+        const Field &f = P.fields[k];
+        /* TFieldScaler::scalingFactor (src/field.cpp:7-10), evaluated where the reference does */
+        const double bs = f.blen == 0 ? f.bscale : (in_bounds(f, x, y, z) ? scale_B(f, t) : 0.);
+        container_B<MIXED>(f, x, y, z, bs, B, dB, grad, nfield);
+    }
 }
 
-/* TFieldContainer::EField / TFieldManager::EField (src/field.cpp:155-169, src/fields.cpp:167-182) */
+/* TFieldContainer::EField (src/field.cpp:155-169) with the scale factor es already evaluated */
 template <bool MIXED>
-PTD void field_E(const Params &P, double x, double y, double z, double *V, double E[3], long long &nfield) {
+PTD void container_E(const Field &f, double x, double y, double z, double es, double &Vt, double Et[3], long long &nfield) {
+    Vt = 0; Et[0] = Et[1] = Et[2] = 0;
+    if (in_bounds(f, x, y, z) && es != 0.) {
+        if (f.type == 2) {
+            Et[0] = f.e[0]; Et[1] = f.e[1]; Et[2] = f.e[2];
+            Vt = Et[2] * z;
+        } else if (f.type == 1 && f.has_v) {
+            long cell; double r[3], dist[3];
+            double dV[3] = {0., 0., 0.};
+            if (tric_locate(f, x, y, z, cell, r, dist)) {
+                ++nfield;
+                double Fv, Fx, Fy, Fz;
+                int comp = f.has_b ? 3 : 0;
+                if (MIXED) {
+                    float fv, fx, fy, fz;
+                    tric_eval_grad32(f.coef32 + ((size_t)cell * f.ncomp + comp) * 64, (float)r[0], (float)r[1], (float)r[2], true, fv, fx, fy, fz);
+                    Fv = fv; Fx = fx; Fy = fy; Fz = fz;
+                } else {
+                    tric_eval_grad64(f.coef + ((size_t)cell * f.ncomp + comp) * 64, r[0], r[1], r[2], true, Fv, Fx, Fy, Fz);
+                }
+                Vt = Fv;
+                dV[0] = Fx / dist[0]; dV[1] = Fy / dist[1]; dV[2] = Fz / dist[2];
+            }
+            for (int i = 0; i < 3; i++) Et[i] = -dV[i];
+        }
+        Vt *= es;
+        Et[0] *= es; Et[1] *= es; Et[2] *= es;
+        if (has_box(f)) {
+            FG r = scale_at_bounds(f, x, y, z, Vt, Et[0], Et[1], Et[2], true);
+            Vt = r.F; Et[0] = r.d0; Et[1] = r.d1; Et[2] = r.d2;
+        }
+    }
+}
+
+/* TFieldManager::EField (src/fields.cpp:167-182) */
+template <bool MIXED>
+PTD void field_E(const Params &P, double x, double y, double z, double t, double *V, double E[3], long long &nfield) {
     *V = 0; E[0] = E[1] = E[2] = 0;
     for (int k = 0; k < P.n_fields; ++k) {
         const Field &f = P.fields[k];
-        double Vt = 0, Et[3] = {0, 0, 0};
-        if (in_bounds(f, x, y, z) && f.escale != 0.) {
-            if (f.type == 2) {
-                Et[0] = f.e[0]; Et[1] = f.e[1]; Et[2] = f.e[2];
-                Vt = Et[2] * z;
-            } else if (f.type == 1 && f.has_v) {
-                long cell; double r[3], dist[3];
-                double dV[3] = {0., 0., 0.};
-                if (tric_locate(f, x, y, z, cell, r, dist)) {
-                    ++nfield;
-                    double Fv, Fx, Fy, Fz;
-                    int comp = f.has_b ? 3 : 0;
-                    if (MIXED) {
-                        float fv, fx, fy, fz;
-                        tric_eval_grad32(f.coef32 + ((size_t)cell * f.ncomp + comp) * 64, (float)r[0], (float)r[1], (float)r[2], true, fv, fx, fy, fz);
-                        Fv = fv; Fx = fx; Fy = fy; Fz = fz;
-                    } else {
-                        tric_eval_grad64(f.coef + ((size_t)cell * f.ncomp + comp) * 64, r[0], r[1], r[2], true, Fv, Fx, Fy, Fz);
-                    }
-                    Vt = Fv;
-                    dV[0] = Fx / dist[0]; dV[1] = Fy / dist[1]; dV[2] = Fz / dist[2];
-                }
-                for (int i = 0; i < 3; i++) Et[i] = -dV[i];
-            }
-            Vt *= f.escale;
-            Et[0] *= f.escale; Et[1] *= f.escale; Et[2] *= f.escale;
-            if (has_box(f)) {
-                FG r = scale_at_bounds(f, x, y, z, Vt, Et[0], Et[1], Et[2], true);
-                Vt = r.F; Et[0] = r.d0; Et[1] = r.d1; Et[2] = r.d2;
-            }
-        }
+        double Vt, Et[3];
+        const double es = f.elen == 0 ? f.escale : (in_bounds(f, x, y, z) ? scale_E(f, t) : 0.);
+        container_E<MIXED>(f, x, y, z, es, Vt, Et, nfield);
         *V += Vt;
         for (int i = 0; i < 3; i++) E[i] += Et[i];
     }
@@ -351,19 +509,19 @@ PTD double kinetic_energy(const Species &sp, const double *v) {
 /* TParticle::GetPotentialEnergy, src/particle.cpp:254-269; Bc/Vc: field values already
    known at this point (from the equation-of-motion evaluation), or nullptr */
 template <bool MIXED>
-PTD double potential_energy(const Params &P, const Species &sp, const double *y, const double *Bc, const double *Vc, Ctr &c) {
+PTD double potential_energy(const Params &P, const Species &sp, const double *y, double t, const double *Bc, const double *Vc, Ctr &c) {
     double result = 0;
     if (sp.q != 0 || sp.mu != 0) {
         if (sp.mu != 0) {
             double B[3], dB[3][3];
             if (Bc) { B[0] = Bc[0]; B[1] = Bc[1]; B[2] = Bc[2]; }
-            else field_B<MIXED>(P, y[0], y[1], y[2], B, dB, false, c.field);
+            else field_B<MIXED>(P, y[0], y[1], y[2], t, B, dB, false, c.field);
             result += -y[7] * sp.mu / ELE_E * __builtin_sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
         }
         if (sp.q != 0) {
             double E[3], V;
             if (Vc) V = *Vc;
-            else field_E<MIXED>(P, y[0], y[1], y[2], &V, E, c.field);
+            else field_E<MIXED>(P, y[0], y[1], y[2], t, &V, E, c.field);
             result += sp.q / ELE_E * V;
         }
     }
@@ -422,16 +580,16 @@ PTD void to9(const double *s, double pol, double *y) {
    derivative is identically +0.0, is passed separately.  B and V return the
    field the equation of motion used (hasB: B was evaluated). */
 template <bool MIXED>
-PTD void derivs8(const Params &P, const Species &sp, const double *s, double pol, double *d, double B[3], double &V,
-                 bool &hasB, Ctr &c) {
+PTD void derivs8(const Params &P, const Species &sp, const double *s, double t, double pol, double *d, double B[3],
+                 double &V, bool &hasB, Ctr &c) {
     double y[9];
     to9(s, pol, y);
     double dB[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, E[3] = {0, 0, 0};
     B[0] = 0; B[1] = 0; B[2] = 0; V = 0;
     ++c.rhs;
     hasB = (sp.q != 0 || (sp.mu != 0 && pol != 0));
-    if (hasB) field_B<MIXED>(P, y[0], y[1], y[2], B, dB, true, c.field);
-    if (sp.q != 0) field_E<MIXED>(P, y[0], y[1], y[2], &V, E, c.field);
+    if (hasB) field_B<MIXED>(P, y[0], y[1], y[2], t, B, dB, true, c.field);
+    if (sp.q != 0) field_E<MIXED>(P, y[0], y[1], y[2], t, &V, E, c.field);
     double d9[9];
     eom(sp, y, B, dB, E, d9);
     d[0] = d9[0]; d[1] = d9[1]; d[2] = d9[2]; d[3] = d9[3]; d[4] = d9[4]; d[5] = d9[5]; d[6] = d9[6]; d[7] = d9[8];
@@ -696,7 +854,7 @@ PTD_NOINLINE bool do_step_hook(const Params &P, Track &T, double x1, const doubl
     double x2temp = x2, y2temp[9];
     for (int k = 0; k < 9; ++k) y2temp[k] = y2[k];
     on_step(P, T, x1, y1, x2, y2, st, cursolid);
-    double H = kinetic_energy(T.sp, y2 + 3) + potential_energy<MIXED>(P, T.sp, y2, nullptr, nullptr, T.c);
+    double H = kinetic_energy(T.sp, y2 + 3) + potential_energy<MIXED>(P, T.sp, y2, x2, nullptr, nullptr, T.c);
     T.Hmax = (H < T.Hmax) ? T.Hmax : H;
     if (y2temp[7] != y2[7]) T.Nspinflip++;
     T.Nstep++;
@@ -946,8 +1104,8 @@ PTD_NOINLINE double integrate_spin(const Params &P, Track &T, double *spin, cons
     double x1 = st.t_old;
     const double *y1 = st.xo;
     double B1[3], B2[3], dB[3][3];
-    field_B<MIXED>(P, y1[0], y1[1], y1[2], B1, dB, false, T.c.field);
-    field_B<MIXED>(P, y2[0], y2[1], y2[2], B2, dB, false, T.c.field);
+    field_B<MIXED>(P, y1[0], y1[1], y1[2], x1, B1, dB, false, T.c.field);
+    field_B<MIXED>(P, y2[0], y2[1], y2[2], x2, B2, dB, false, T.c.field);
     double Babs1 = __builtin_sqrt(B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]);
     double Babs2 = __builtin_sqrt(B2[0] * B2[0] + B2[1] * B2[1] + B2[2] * B2[2]);
     double Bmax = T.sp.Bmax;
@@ -1095,7 +1253,7 @@ PTD bool do_step_hook_A(const Params &P, const Species &sp, Lane &L, int i, cons
     bool changed = !(x2temp == x2 && state_eq(y2temp, y2));
     bool cached = at_end && !changed && (L.flags & FL_BVALID);
     double H = kinetic_energy(sp, y2 + 3) +
-               potential_energy<MIXED>(P, sp, y2, cached ? L.Bc : nullptr, cached ? &L.Vc : nullptr, c);
+               potential_energy<MIXED>(P, sp, y2, x2, cached ? L.Bc : nullptr, cached ? &L.Vc : nullptr, c);
     L.Hmax = (H < L.Hmax) ? L.Hmax : H;
     if (y2temp[7] != y2[7]) P.st.Nspinflip[i] += 1;
     L.Nstep++;
@@ -1114,9 +1272,9 @@ PTD void spin_step(const Params &P, int i, const Species &sp, double x1, double 
     const size_t n = (size_t)S.n;
     double B1[3], B2[3], dB[3][3];
     if (haveB1) { B1[0] = B10; B1[1] = B11; B1[2] = B12; }
-    else field_B<MIXED>(P, ys0, ys1, ys2, B1, dB, false, c.field);
+    else field_B<MIXED>(P, ys0, ys1, ys2, x1, B1, dB, false, c.field);
     if (haveB2) { B2[0] = B20; B2[1] = B21; B2[2] = B22; }
-    else field_B<MIXED>(P, y[0], y[1], y[2], B2, dB, false, c.field);
+    else field_B<MIXED>(P, y[0], y[1], y[2], x2, B2, dB, false, c.field);
     double Babs1 = __builtin_sqrt(B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]);
     double Babs2 = __builtin_sqrt(B2[0] * B2[0] + B2[1] * B2[1] + B2[2] * B2[2]);
     double spin[5];
@@ -1207,7 +1365,7 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
     }
     if (!(L.flags & FL_DERIV)) {
         bool hb;
-        derivs8<MIXED>(P, sp, xc, L.pol, L.dxc, L.Bc, L.Vc, hb, c);
+        derivs8<MIXED>(P, sp, xc, ts, L.pol, L.dxc, L.Bc, L.Vc, hb, c);
         L.flags = (L.flags | FL_DERIV);
         L.flags = hb ? (L.flags | FL_BVALID) : (L.flags & ~FL_BVALID);
     }
@@ -1234,23 +1392,23 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
         const double h21 = dt * b21;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h21 * L.dxc[k];
-        derivs8<MIXED>(P, sp, tmp, L.pol, k2, Bd, Vd, bd, c);
+        derivs8<MIXED>(P, sp, tmp, ts + dt * (1.0 / 5), L.pol, k2, Bd, Vd, bd, c);
         const double h31 = dt * b31, h32 = dt * b32;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h31 * L.dxc[k] + h32 * k2[k];
-        derivs8<MIXED>(P, sp, tmp, L.pol, k3, Bd, Vd, bd, c);
+        derivs8<MIXED>(P, sp, tmp, ts + dt * (3.0 / 10), L.pol, k3, Bd, Vd, bd, c);
         const double h41 = dt * b41, h42 = dt * b42, h43 = dt * b43;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h41 * L.dxc[k] + h42 * k2[k] + h43 * k3[k];
-        derivs8<MIXED>(P, sp, tmp, L.pol, k4, Bd, Vd, bd, c);
+        derivs8<MIXED>(P, sp, tmp, ts + dt * (4.0 / 5), L.pol, k4, Bd, Vd, bd, c);
         const double h51 = dt * b51, h52 = dt * b52, h53 = dt * b53, h54 = dt * b54;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h51 * L.dxc[k] + h52 * k2[k] + h53 * k3[k] + h54 * k4[k];
-        derivs8<MIXED>(P, sp, tmp, L.pol, k5, Bd, Vd, bd, c);
+        derivs8<MIXED>(P, sp, tmp, ts + dt * (8.0 / 9), L.pol, k5, Bd, Vd, bd, c);
         const double h61 = dt * b61, h62 = dt * b62, h63 = dt * b63, h64 = dt * b64, h65 = dt * b65;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h61 * L.dxc[k] + h62 * k2[k] + h63 * k3[k] + h64 * k4[k] + h65 * k5[k];
-        derivs8<MIXED>(P, sp, tmp, L.pol, k6, Bd, Vd, bd, c);
+        derivs8<MIXED>(P, sp, tmp, ts + dt, L.pol, k6, Bd, Vd, bd, c);
         const double g1 = dt * c1, g3 = dt * c3, g4 = dt * c4, g5 = dt * c5, g6 = dt * c6;
 #pragma unroll
         for (int k = 0; k < 8; ++k) xnew[k] = xc[k] + g1 * L.dxc[k] + g3 * k3[k] + g4 * k4[k] + g5 * k5[k] + g6 * k6[k];
@@ -1279,7 +1437,7 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
                 ds[(40 + k) * n + i] = k6[k];
             }
         }
-        derivs8<MIXED>(P, sp, xnew, polnew, k7, Bn, Vn, bn, c);
+        derivs8<MIXED>(P, sp, xnew, ts + dt, polnew, k7, Bn, Vn, bn, c);
         double err = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -1547,7 +1705,7 @@ __global__ void __launch_bounds__(256) init_kernel(Params P) {
     double x = S.t[i], y[9];
     for (int k = 0; k < 9; ++k) y[k] = S.y[(size_t)k * n + i];
     Ctr c = {0, 0, 0};
-    double Hmax = kinetic_energy(sp, y + 3) + potential_energy<MIXED>(P, sp, y, nullptr, nullptr, c);
+    double Hmax = kinetic_energy(sp, y + 3) + potential_energy<MIXED>(P, sp, y, x, nullptr, nullptr, c);
     double tau;
     if (sp.tau > 0) tau = -ptm_log(1.0 - ptm_canonical(ptm_draw_u64(key, &ctr))) / (1. / sp.tau);
     else tau = sp.tmax;

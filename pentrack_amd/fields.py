@@ -1,12 +1,16 @@
 """TFieldManager for the MI355X engine.
 
 Field containers mirror TFieldContainer (include/field.h, src/field.cpp):
-a field, constant B and E scale factors (the reference's exprtk time formulas
-reduced to constants) and an optional boundary box with a smooth taper.
-Supported fields on the tracking path:
+a field, B and E scale factors -- constants or exprtk formulas of the time t
+(TFieldScaler, src/field.cpp:7-45; compiled by pt_formula.cpp) -- and an
+optional boundary box with a smooth taper.  Supported fields on the tracking
+path:
   * TabField3 -- tricubic interpolation of a 3D table (src/field_3d.cpp);
     the 64 coefficients per cell come from pt_tricubic_precompute (host C++)
   * TEDMStaticEField -- uniform E with V = Ez*z (src/edmfields.cpp:175-187)
+  * TConductorField -- finite straight wire (src/conductor.cpp)
+  * TExponentialFieldX, TLinearFieldZ, TB0GradZ, TB0GradX2, TB0GradXY, TB0_XY
+    (src/analyticFields.cpp)
 Coil fields used to fill the synthetic tables are computed here on the host.
 """
 import ctypes as C
@@ -34,12 +38,19 @@ class FieldContainer:
     coef: Optional[np.ndarray] = None            # [cells][ncomp][64]
     # uniform E
     e: Sequence[float] = (0.0, 0.0, 0.0)
+    # analytic fields and conductors
+    params: Sequence[float] = ()
+    # exprtk formulas of t replacing bscale / escale
+    bformula: Optional[str] = None
+    eformula: Optional[str] = None
 
     def __post_init__(self):
+        if len(self.params) > 8:
+            raise ValueError("at most 8 field parameters")
         b = self.box
-        if b[1] < b[0] and b[3] < b[2] and b[5] < b[4]:
-            if self.bw < 0 or self.bw > (b[0] - b[1]) / 2 or self.bw > (b[2] - b[3]) / 2 or self.bw > (b[4] - b[5]) / 2:
-                raise ValueError("You defined a field with an invalid boundary. Check field parameters.")
+        # TFieldBoundaryBox constructor (src/field.cpp:116-125)
+        if self.bw < 0 or self.bw > (b[0] - b[1]) / 2 or self.bw > (b[2] - b[3]) / 2 or self.bw > (b[4] - b[5]) / 2:
+            raise ValueError("You defined a field with an invalid boundary. Check field parameters.")
 
 
 def tabfield3(xg, yg, zg, B=None, V=None, bscale=1.0, escale=0.0, box=(0, 0, 0, 0, 0, 0), bw=0.0, threads=None):
@@ -60,7 +71,10 @@ def tabfield3(xg, yg, zg, B=None, V=None, bscale=1.0, escale=0.0, box=(0, 0, 0, 
         raise ValueError("tabfield3 needs B or V samples")
     values = np.ascontiguousarray(np.stack(comps))
     coef = tricubic_coefficients(xg, yg, zg, values, threads=threads)
-    return FieldContainer(type=_native.PT_FIELD_TRICUBIC, bscale=bscale, escale=escale, box=tuple(box), bw=bw,
+    bs, bf = _scale(bscale)
+    es, ef = _scale(escale)
+    return FieldContainer(type=_native.PT_FIELD_TRICUBIC, bscale=bs, escale=es, bformula=bf, eformula=ef,
+                          box=tuple(box), bw=bw,
                           grid=(xg, yg, zg), has_b=int(B is not None), has_v=int(V is not None), coef=coef)
 
 
@@ -80,9 +94,49 @@ def tricubic_coefficients(xg, yg, zg, values, threads=None):
     return coef
 
 
+def _scale(s):
+    """A scale given as a number or as an exprtk formula of t."""
+    if isinstance(s, str):
+        return 1.0, s
+    return float(s), None
+
+
 def uniform_efield(ex, ey, ez, escale=1.0):
     """TEDMStaticEField in a container with B scale "0" (src/fields.cpp:100-104)."""
-    return FieldContainer(type=_native.PT_FIELD_UNIFORM_E, bscale=0.0, escale=escale, e=(ex, ey, ez))
+    es, ef = _scale(escale)
+    return FieldContainer(type=_native.PT_FIELD_UNIFORM_E, bscale=0.0, escale=es, eformula=ef, e=(ex, ey, ez))
+
+
+def conductor(I, p1, p2, bscale=1.0):
+    """TConductorField from p1 to p2 carrying I [A], in a container without bounds and
+    E scale "1" (src/fields.cpp:42-46, include/field.h:280-281)."""
+    if tuple(p1) == tuple(p2):
+        raise ValueError("You have defined a conductor with length zero! Aborting...")
+    bs, bf = _scale(bscale)
+    return FieldContainer(type=_native.PT_FIELD_CONDUCTOR, bscale=bs, bformula=bf, escale=1.0,
+                          params=(float(I),) + tuple(float(v) for v in p1) + tuple(float(v) for v in p2))
+
+
+_ANALYTIC = {
+    "ExponentialFieldX": (_native.PT_FIELD_EXPONENTIAL_X, 5),
+    "LinearFieldZ": (_native.PT_FIELD_LINEAR_Z, 2),
+    "B0GradZ": (_native.PT_FIELD_B0_GRAD_Z, 3),
+    "B0GradX2": (_native.PT_FIELD_B0_GRAD_X2, 4),
+    "B0GradXY": (_native.PT_FIELD_B0_GRAD_XY, 3),
+    "B0_XY": (_native.PT_FIELD_B0_XY, 2),
+}
+
+
+def analytic_field(name, params, box=(0, 0, 0, 0, 0, 0), bscale=1.0):
+    """One of the analytic B fields of src/analyticFields.cpp in a container with the
+    given bounds (xmax xmin ymax ymin zmax zmin), no taper and E scale "0"
+    (src/fields.cpp:106-134)."""
+    ty, npar = _ANALYTIC[name]
+    if len(params) != npar:
+        raise ValueError(f"{name} takes {npar} parameters")
+    bs, bf = _scale(bscale)
+    return FieldContainer(type=ty, bscale=bs, bformula=bf, escale=0.0, box=tuple(box), bw=0.0,
+                          params=tuple(float(v) for v in params))
 
 
 def loop_field(points, radius, z0, current):
@@ -146,6 +200,13 @@ class TFieldManager:
             d.bw = c.bw
             for i in range(3):
                 d.e[i] = c.e[i]
+            for i, v in enumerate(c.params):
+                d.p[i] = v
+            for attr, form in (("bscale_formula", c.bformula), ("escale_formula", c.eformula)):
+                if form is not None:
+                    b = C.create_string_buffer(form.encode())
+                    keep.append(b)
+                    setattr(d, attr, C.cast(b, C.c_char_p))
             if c.type == _native.PT_FIELD_TRICUBIC:
                 xg, yg, zg = (np.ascontiguousarray(g, dtype=np.float64) for g in c.grid)
                 coef = np.ascontiguousarray(c.coef, dtype=np.float64)

@@ -84,7 +84,8 @@ def coil_map(seed, n_grid, lo, hi, Bmax=1.0, threads=None):
 
 
 def cube(n=10000, seed=1, tmax=10.0, Emax=300e-9, with_field=False, n_grid=32, Bmax=0.1):
-    """Cube workload (SURVEY config 1; with a field map: the config-3 geometry and source)."""
+    """Neutrons in a 1 m cube under gravity with specular walls (335 neV); with_field adds an
+    n_grid^3 coil-field map over the cube."""
     rng = np.random.default_rng(seed)
     geom = cube_geometry()
     eps = 1e-6
@@ -100,7 +101,7 @@ def cube(n=10000, seed=1, tmax=10.0, Emax=300e-9, with_field=False, n_grid=32, B
 
 
 def cylinder(n=1000000, seed=2, tmax=200.0, Emax=250e-9, n_grid=64, Bmax=1.0, n_phi=200, n_z=40, n_r=5):
-    """Cylinder storage volume with a coil-field map (SURVEY config 2)."""
+    """Cylinder storage volume (R 0.25 m, H 0.5 m) with a 64^3 coil-field map."""
     rng = np.random.default_rng(seed)
     R, H = 0.25, 0.5
     geom = cylinder_geometry(R, H, n_phi, n_z, n_r)
@@ -136,7 +137,7 @@ def adjoint_cube(n=100000, seed=5, n_grid=64, nmaps=8, Bmax=0.5, Emax=300e-9, dt
 
 
 def uneven(n=4000000, seed=4, n_tris=50000, tmax=200.0, n_grid=64, Bmax=1.0, E_field=1e5):
-    """Mixed species with very uneven lifetimes (SURVEY config 4): 90% neutrons with
+    """Mixed species with very uneven lifetimes: 90% neutrons with
     exponential decay (tau = 880 s), 8% protons (1-800 eV), 2% electrons (1-700 keV) in
     a closed cylinder tessellated to about n_tris triangles, B from the coil map plus a
     uniform E = E_field V/m along z; species proper-time limits 3e-3 s (protons) and
@@ -173,3 +174,49 @@ def uneven(n=4000000, seed=4, n_tris=50000, tmax=200.0, n_grid=64, Bmax=1.0, E_f
     meta = dict(meta, n_phi=n_phi, n_z=n_z)
     return Workload("uneven", geom, fm, species, tmax, kinds, t0, y, spin0, np.arange(n, dtype=np.uint64), seed,
                     meta)
+
+
+def analytic(n=300, seed=6, tmax=0.1, n_grid=16):
+    """Neutrons, protons and electrons in a 1 m box under every analytic field type:
+    a finite wire (12.5 kA, outside the cube) switched on by a time ramp, the six analytic
+    B fields of src/analyticFields.cpp with bounds and time-dependent scales, a coil map
+    with a ramped scale and a tapered boundary, and a uniform E whose scale steps in time."""
+    from .constants import ELECTRON, PROTON
+    from .fields import analytic_field, conductor, uniform_efield
+    from .geometry import hollow_box_mesh
+    rng = np.random.default_rng(seed)
+    # a 1 mm shell (solid 2, 335 neV, 5% diffuse) around the unit cube; the inside is the
+    # default solid, where charged particles fly until they enter the shell
+    mats = [Material("vacuum"), Material("wall", FermiReal=335.0, DiffProb=0.05)]
+    geom = TGeometry(mats, "vacuum", [Solid(2, "wall", hollow_box_mesh((0, 0, 0), (1, 1, 1), 0.001))])
+    kinds = np.where(np.arange(n) % 3 == 0, PROTON, np.where(np.arange(n) % 3 == 1, ELECTRON, NEUTRON))
+    kinds = kinds.astype(np.int32)
+    _, y0, spin0 = neutron_source(rng, n, lambda p: np.ones(len(p), bool), 1e-3, 1 - 1e-3, 300e-9)
+    E = np.where(kinds == PROTON, rng.uniform(1, 800, n), np.where(kinds == ELECTRON, rng.uniform(1e3, 7e5, n),
+                                                                  rng.uniform(0, 300e-9, n)))
+    phi = rng.uniform(0, 2 * np.pi, n)
+    theta = np.arccos(rng.uniform(-1, 1, n))
+    y = y0.copy()
+    for k in (NEUTRON, PROTON, ELECTRON):
+        sel = kinds == k
+        _, yk, _ = make_states(k, y0[sel, :3], E[sel], phi[sel], theta[sel], y0[sel, 7])
+        y[sel] = yk
+    fmap, meta = coil_map(seed, n_grid, (0, 0, 0), (1, 1, 1), Bmax=0.05)
+    fmap.bformula = "t < 0.02 ? 0.2 : (t < 0.06 ? 0.2 + 20*(t - 0.02) : 1)"
+    fmap.box, fmap.bw = (1.0, 0.0, 1.0, 0.0, 1.0, 0.0), 0.1
+    containers = [
+        conductor(12500.0, (1.5, 0.5, -1.0), (1.5, 0.5, 2.0), bscale="t < 0.01 ? 0 : (t < 0.03 ? 50*(t-0.01) : 1)"),
+        analytic_field("B0GradZ", (2e-3, -1e-3, 1e-3), box=(1, 0, 1, 0, 1, 0), bscale="0.5 + 0.5*cos(2*pi*t/0.05)"),
+        analytic_field("ExponentialFieldX", (5e-3, 1.0, -0.5, 1e-4, 2e-4), box=(3, -3, 1, -1, 1.5, -1)),
+        analytic_field("LinearFieldZ", (2e-3, 1e-3), box=(2, -2, 2, -2, 2, -2), bscale="t/0.1"),
+        analytic_field("B0GradX2", (1e-3, 2e-4, -3e-4, 1e-3), box=(0.9, 0.1, 0.9, 0.1, 0.9, 0.1)),
+        analytic_field("B0GradXY", (1e-3, -2e-4, 5e-4)),
+        analytic_field("B0_XY", (1e-3, 2e-3), box=(1, 0, 1, 0, 0.7, 0.3), bscale="t > 0.05"),
+        fmap,
+        uniform_efield(0.0, 1e4, 1e5, escale="t < 0.02 ? 1 : 0.5"),
+    ]
+    species = [SpeciesConfig(NEUTRON, tau=0.0, tmax=9e99), SpeciesConfig(PROTON, tmax=3e-3),
+               SpeciesConfig(ELECTRON, tmax=1e-5)]
+    t0 = rng.uniform(0.0, 0.06, n)     # start times across the ramps of the scale formulas
+    return Workload("analytic", geom, TFieldManager(containers), species, tmax, kinds, t0, y, spin0,
+                    np.arange(n, dtype=np.uint64), seed, meta)

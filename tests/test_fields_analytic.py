@@ -1,0 +1,205 @@
+"""Field-scaling formulas, analytic fields and finite wires.
+
+Checked against the reference's own field classes -- TFieldScaler (exprtk formulas),
+TFieldContainer (scale, bounds, taper) and the fields of src/analyticFields.cpp --
+compiled from its sources by oracle/ref_build.sh.  The committed vectors
+(tests/golden/reference_fields.npz, written by tests/golden/make_golden_fields.py)
+run everywhere; where oracle/_ref has been built, fresh random formulas are also
+checked against exprtk directly.  The finite wire (src/conductor.cpp needs Boost
+headers to build) is checked against the Biot-Savart integral and finite differences.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oracle import formula as OF
+from pentrack_amd import _native
+from pentrack_amd.fields import FieldContainer, TFieldManager, analytic_field, conductor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+G = np.load(os.path.join(HERE, "golden", "reference_fields.npz"))
+REF_LIB = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "libref_islands.so")
+TYPE_NAMES = {4: "ExponentialFieldX", 5: "LinearFieldZ", 6: "B0GradZ", 7: "B0GradX2", 8: "B0GradXY", 9: "B0_XY"}
+NPAR = {4: 5, 5: 2, 6: 3, 7: 4, 8: 3, 9: 2}
+
+
+def dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def engine_formula(f, t):
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    out = np.empty_like(t)
+    rc = _native.lib().pt_formula_eval(f.encode(), len(t), dp(t), dp(out))
+    if rc != 0:
+        raise ValueError(_native.lib().pt_last_error().decode())
+    return out
+
+
+def same_bits(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return bool(np.all((a.view(np.uint64) == b.view(np.uint64)) | (np.isnan(a) & np.isnan(b))))
+
+
+def close(a, b, rtol):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    both_nan = np.isnan(a) & np.isnan(b)
+    ok = both_nan | (a == b) | (np.abs(a - b) <= rtol * np.maximum(np.abs(a), np.abs(b)))
+    return bool(np.all(ok))
+
+
+def test_formula_compiler_matches_exprtk_golden():
+    """Every committed formula: exact for formulas of exactly rounded operations, within
+    1e-13 where exp/log/sin/cos or a non-integer power is evaluated at run time (the
+    engine's deterministic functions against the C library's)."""
+    t = G["t"]
+    for f, ref, exact in zip(G["formulas"], G["values"], G["exact"]):
+        got = engine_formula(str(f), t)
+        if exact:
+            assert same_bits(got, ref), (f, got[:4], ref[:4])
+        else:
+            assert close(got, ref, 1e-13), f
+
+
+def test_oracle_formula_matches_exprtk_golden():
+    t = G["t"]
+    for f, ref in zip(G["formulas"], G["values"]):
+        tree = OF.parse(str(f))
+        got = np.array([OF.evaluate(tree, float(x)) for x in t])
+        assert same_bits(got, ref), f
+
+
+@pytest.mark.parametrize("mode", ["fp64", "refsem"])
+def test_oracle_c_evaluator_of_scale_formulas(oracle_lib, mode):
+    """The oracle's C evaluation of formula trees through a container whose field is 1:
+    B_z = scale(t) exactly (LinearFieldZ with a1 = 0, a2 = 1), +0 where the scale is
+    zero (the container returns a zero field then, src/field.cpp:140)."""
+    t = G["t"]
+    pts = np.full((len(t), 3), 0.5)
+    for f, ref, exact in zip(G["formulas"], G["values"], G["exact"]):
+        c = analytic_field("LinearFieldZ", (0.0, 1.0), bscale=str(f))
+        B, _, _, _ = oracle_lib.field_eval(TFieldManager([c]), pts, t, mode)
+        ref = np.where(ref == 0, 0.0, ref)
+        if mode == "refsem" or exact:
+            assert same_bits(B[:, 2], ref), (f, mode)
+        else:
+            assert close(B[:, 2], ref, 1e-13), (f, mode)
+
+
+@pytest.mark.parametrize("formula,what", [
+    ("(t*2)*(t*3)", "chain"), ("t*t/3", "chain"), ("1 - t/100 - 2", "chain"), ("(sin(t)+1)+2", "chain"),
+    ("(t+1)*0", "literal 0"), ("0/(t+1)", "literal 0"), ("t % 3", "'%'"), ("t < 1 && t > 0", "'&&'"),
+    ("t < 1 || t > 2", "'||'"), ("tan(t)", "unknown name"), ("2t", "implicit multiplication"), ("(t", "expected"),
+])
+def test_rejected_formula_forms(formula, what):
+    with pytest.raises(ValueError, match=what):
+        engine_formula(formula, np.zeros(1))
+
+
+def test_fields_create_reports_bad_formula_and_zero_length_wire():
+    """pt_fields_create rejects these before any device allocation."""
+    L = _native.lib()
+    for c, msg in ((analytic_field("LinearFieldZ", (1.0, 0.0), bscale="(t+1)+t*2+1"), "re-associated"),
+                   (FieldContainer(type=_native.PT_FIELD_CONDUCTOR, params=(1.0, 0, 0, 0, 0, 0, 0)), "length zero"),
+                   (FieldContainer(type=_native.PT_FIELD_B0_XY, params=(1.0, 0.0)), None)):
+        if msg is None:
+            continue
+        ds, keep = TFieldManager([c]).descs()
+        h = C.c_void_p()
+        assert L.pt_fields_create(1, ds, 0, C.byref(h)) == _native.PT_EINVAL
+        assert msg in L.pt_last_error().decode()
+    with pytest.raises(ValueError):
+        conductor(1.0, (0, 0, 0), (0, 0, 0))
+    with pytest.raises(ValueError):
+        FieldContainer(type=_native.PT_FIELD_B0_XY, box=(1, 0, 1, 0, 1, 0), bw=0.6)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_LIB), reason="oracle/_ref not built (reference sources absent)")
+def test_fresh_formulas_against_exprtk():
+    L = C.CDLL(REF_LIB)
+    L.ref_scaler_eval.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    rng = np.random.default_rng(2024)
+    t = np.concatenate([[0.0, 1e-3, 0.5, 1.0, 400.0], rng.uniform(0, 1000, 40), rng.uniform(0, 1, 20)])
+    frng = np.random.default_rng(99)
+    accepted = 0
+    for _ in range(600):
+        f = OF.random_formula(frng, depth=int(frng.integers(1, 5)))
+        ref = np.empty_like(t)
+        assert L.ref_scaler_eval(f.encode(), len(t), dp(t), dp(ref)) == 0, f
+        try:
+            got = engine_formula(f, t)
+        except ValueError:
+            continue
+        accepted += 1
+        assert same_bits(got, ref), f
+        tree = OF.parse(f)
+        assert same_bits([OF.evaluate(tree, float(x)) for x in t], ref), f
+    assert accepted > 300
+
+
+def golden_containers():
+    for k in range(len(G["c_type"])):
+        ty = int(G["c_type"][k])
+        par = tuple(G["c_par"][k][:NPAR[ty]])
+        box = tuple(G["c_box"][k])
+        c = analytic_field(TYPE_NAMES[ty], par, box=box, bscale=str(G["c_scale"][k]))
+        c.bw = float(G["c_bw"][k])
+        yield k, ty, c
+
+
+@pytest.mark.parametrize("mode", ["refsem", "fp64"])
+def test_analytic_containers_match_reference(oracle_lib, mode):
+    """TFieldContainer::BField around the six analytic fields, with constant and
+    time-dependent scales, bounds and tapers: bit for bit, except TExponentialFieldX in
+    the fp64 mode (deterministic exp within 2 ulp; 1e-14 of the field's scale)."""
+    for k, ty, c in golden_containers():
+        B, dB, _, _ = oracle_lib.field_eval(TFieldManager([c]), G["c_pts"][k], G["c_t"][k], mode)
+        if mode == "refsem" or ty != 4:
+            assert same_bits(B, G["c_B"][k]), (k, ty)
+            assert same_bits(dB, G["c_dB"][k]), (k, ty)
+        else:
+            for got, ref in ((B, G["c_B"][k]), (dB, G["c_dB"][k])):
+                assert np.allclose(got, ref, rtol=1e-14, atol=1e-14 * np.abs(ref).max()), (k, ty)
+        # the reference computes the same B with or without derivatives
+        assert same_bits(G["c_B0"][k], G["c_B"][k])
+
+
+def biot_savart(I, a, b, p, n=20001):
+    """B of the segment a->b by Simpson integration of mu0 I/(4 pi) dl x r / |r|^3."""
+    from scipy.integrate import simpson
+    s = np.linspace(0.0, 1.0, n)
+    a, b, p = (np.asarray(v, float) for v in (a, b, p))
+    pts = a[None, :] + s[:, None] * (b - a)[None, :]
+    r = p[None, :] - pts
+    integrand = np.cross(np.broadcast_to(b - a, r.shape), r) / np.linalg.norm(r, axis=1)[:, None] ** 3
+    return 1e-7 * I * simpson(integrand, x=s, axis=0)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "refsem"])
+def test_conductor_against_biot_savart_and_finite_differences(oracle_lib, mode):
+    rng = np.random.default_rng(7)
+    a, b, I = (0.2, -0.3, -1.0), (0.1, 0.4, 2.0), 12500.0
+    fm = TFieldManager([conductor(I, a, b)])
+    pts = rng.uniform(-1, 1, (20, 3)) + np.array([0.6, 0.0, 0.5])
+    B, dB, V, E = oracle_lib.field_eval(fm, pts, 0.0, mode)
+    assert np.all(V == 0) and np.all(E == 0)
+    for i, p in enumerate(pts):
+        ref = biot_savart(I, a, b, p)
+        assert np.allclose(B[i], ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max()), (i, B[i], ref)
+        h = 1e-6
+        for j in range(3):
+            e = np.zeros(3)
+            e[j] = h
+            Bp, _, _, _ = oracle_lib.field_eval(fm, (p + e)[None], 0.0, mode)
+            Bm, _, _, _ = oracle_lib.field_eval(fm, (p - e)[None], 0.0, mode)
+            fd = (Bp[0] - Bm[0]) / (2 * h)
+            assert np.allclose(dB[i][:, j], fd, rtol=1e-6, atol=1e-9 * np.abs(dB[i]).max()), (i, j)
+    # long wire: B -> mu0 I / (2 pi d) around it
+    fm = TFieldManager([conductor(I, (0, 0, -1e4), (0, 0, 1e4))])
+    B, _, _, _ = oracle_lib.field_eval(fm, np.array([[0.3, 0.0, 0.0]]), 0.0, mode)
+    assert np.isclose(B[0, 1], 2e-7 * I / 0.3, rtol=1e-8) and abs(B[0, 0]) < 1e-12 and abs(B[0, 2]) < 1e-12
+    # zero current: zero field
+    B, dB, _, _ = oracle_lib.field_eval(TFieldManager([conductor(0.0, a, b)]), pts, 0.0, mode)
+    assert np.all(B == 0) and np.all(dB == 0)

@@ -95,6 +95,20 @@ def test_charged_species_uniform_E_and_map(gpu, oracle_lib):
     assert set(np.unique(g["stopID"])) >= {-4, 1}
 
 
+def test_analytic_fields_and_time_dependent_scales_bit_exact(gpu, oracle_lib):
+    """A finite wire, the six analytic B fields, a tapered coil map and a uniform E, with
+    scale formulas of t (ramps, steps, a cosine), on neutrons, protons and electrons."""
+    w = workloads.analytic(n=600, seed=6, tmax=0.1)
+    print("case: analytic fields n=600", flush=True)
+    g, gh = run_gpu(w)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
+    assert_bit_exact(g, gh, o, oh)
+    print("case: analytic fields n=600 persistent", flush=True)
+    p, ph = run_gpu(w, cfg=LaunchConfig(persistent=1, steps_per_launch=16))
+    assert_bit_exact(p, ph, o, oh)
+    assert (g["stopID"][w.species_idx == 0] == -1).all() and g["n_acc"].sum() > 10000
+
+
 def test_edge_cases(gpu, oracle_lib):
     w = workloads.cube(n=8, seed=5, tmax=1.0)
     w.t0[0] = 2.0                  # starts past tmax: the reference throws (invariant stop)
