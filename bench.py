@@ -40,13 +40,20 @@ VARIANTS = {
     "mixed": dict(mixed=True, persistent=0, graph=0),
     "mixed_persistent": dict(mixed=True, persistent=1, graph=0),
     "mixed_persistent_graph": dict(mixed=True, persistent=1, graph=1),
+    # work list handed out in Morton order of the particle positions
+    "mixed_persistent_spatial": dict(mixed=True, persistent=1, graph=0, spatial=1),
+    "mixed_persistent_spatial_graph": dict(mixed=True, persistent=1, graph=1, spatial=1),
 }
-HEADLINE_ORDER = ["mixed_persistent_graph", "mixed_persistent", "mixed", "fp64"]
+HEADLINE_ORDER = ["mixed_persistent_spatial_graph", "mixed_persistent_spatial", "mixed_persistent_graph",
+                  "mixed_persistent", "mixed", "fp64"]
 # graph replays carry no per-kernel events: the roofline uses the same kernels timed eagerly
-PROFILED_FOR = {"mixed_persistent_graph": ["mixed_persistent"], "mixed_persistent": ["mixed_persistent"],
+PROFILED_FOR = {"mixed_persistent_spatial_graph": ["mixed_persistent_spatial"],
+                "mixed_persistent_spatial": ["mixed_persistent_spatial"],
+                "mixed_persistent_graph": ["mixed_persistent"], "mixed_persistent": ["mixed_persistent"],
                 "mixed": ["mixed"], "fp64": ["fp64"]}
 KERNEL_NAMES = {"fp64": "step_kernel_fp64", "mixed": "step_kernel_mixed",
-                "mixed_persistent": "step_kernel_persistent_mixed"}
+                "mixed_persistent": "step_kernel_persistent_mixed",
+                "mixed_persistent_spatial": "step_kernel_persistent_mixed_spatial"}
 
 
 def parse():
@@ -243,7 +250,7 @@ def main():
         b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
                   with_fp32=kw["mixed"])
         cfg = LaunchConfig(variant=vid, block=a.block, steps_per_launch=a.rk, rounds=a.rounds,
-                           persistent=kw["persistent"], graph=kw["graph"])
+                           persistent=kw["persistent"], graph=kw["graph"], spatial_order=kw.get("spatial", 0))
         for _ in range(a.warmup):
             b.run(cfg, sptr)
         torch.cuda.synchronize()

@@ -180,6 +180,9 @@ typedef struct {
                                     (0 = 4); each round resolves at most one wall collision per particle */
     int graph;                   /* 1: capture the call's launch sequence into a hipGraph on first use
                                     and replay it (needs a non-default stream; no per-kernel timing) */
+    int spatial_order;           /* persistent kernel: 1 hands out particles in Morton order of their
+                                    position (32^3 buckets over the mesh box) so concurrent waves
+                                    gather nearby field cells; 0 in index order.  Results are equal. */
 } pt_launch_cfg;
 
 typedef struct pt_batch pt_batch;

@@ -123,8 +123,12 @@ struct State {
     int *qcount;              /* queue length */
     int *work;                /* persistent variant: indices of particles with steps to do */
     int *wcount;              /* [0] work list length, [1] next work item */
+    int *pkey;                /* spatial bucket of each particle (work-list ordering) */
+    int *bhist;               /* [NBUCKET + 1] bucket counts, then running offsets */
 };
 
+constexpr int BUCKET_BITS = 5;                     /* 32 buckets per axis */
+constexpr int NBUCKET = 1 << (3 * BUCKET_BITS);     /* Morton-ordered spatial buckets */
 constexpr int DENSE = 59;     /* xo[8] dxo[8] k3..k6[32] dxc[8] t t_old pol */
 
 struct Params {

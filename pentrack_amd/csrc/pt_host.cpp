@@ -434,6 +434,7 @@ struct pt_fields {
     ptd::Field *dev = nullptr;
     int n = 0;
     bool has_fp32 = false;
+    bool general = false;   /* a scale formula, analytic field or conductor is present */
 };
 
 extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32, pt_fields **out) {
@@ -524,6 +525,7 @@ extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32
         } else if (d.type < PT_FIELD_UNIFORM_E || d.type > PT_FIELD_B0_XY) {
             return fail(PT_EINVAL, "pt_fields_create: unknown field type");
         }
+        F->general = F->general || f.blen > 0 || f.elen > 0 || f.type >= PT_FIELD_CONDUCTOR;
         F->host[k] = f;
     }
     int r = F->mem.upload(&F->dev, F->host.data(), F->host.size());
@@ -843,6 +845,8 @@ extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_s
     if ((r = B->mem.upload(&S.qcount, izero.data(), 1))) return r;
     if ((r = B->mem.alloc(&S.work, N))) return r;
     if ((r = B->mem.upload(&S.wcount, izero.data(), 2))) return r;
+    if ((r = B->mem.alloc(&S.pkey, N))) return r;
+    if ((r = B->mem.alloc(&S.bhist, (size_t)ptd::NBUCKET + 1))) return r;
     int zero = 0;
     if ((r = B->mem.upload(&B->d_quit, &zero, 1))) return r;
     if ((r = B->mem.upload(&B->d_count, &zero, 1))) return r;
@@ -869,6 +873,7 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
     }
     int block = cfg->block > 0 ? cfg->block : 256;
     if (block % 64 != 0 || block > 256) return fail(PT_EINVAL, "pt_batch_run: block must be a multiple of 64 and <= 256");
+    const int mode = (cfg->variant == PT_VARIANT_MIXED ? 1 : 0) | (b->fields->general ? 2 : 0);
     hipStream_t s = static_cast<hipStream_t>(stream);
     int q = g_quit.load();
     PT_HIP(hipMemcpyAsync(b->d_quit, &q, sizeof(int), hipMemcpyHostToDevice, s));
@@ -883,7 +888,7 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
         pgrid = 2 * std::max(cus, 1);   /* two 256-thread blocks per CU (2 waves per SIMD) */
     }
     auto launch_all = [&](bool timed) -> int {
-        PT_HIP(pt_launch_init(b->P, cfg->variant, steps, s));
+        PT_HIP(pt_launch_init(b->P, mode, steps, s));
         for (int r = 0; r < rounds; ++r) {
             for (int k = 0; k < 2; ++k) {
                 hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -892,9 +897,9 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
                     PT_HIP(hipEventCreate(&e1));
                     PT_HIP(hipEventRecord(e0, s));
                 }
-                if (k == 1) PT_HIP(pt_launch_collision(b->P, cfg->variant, coll_grid, s));
-                else if (cfg->persistent) PT_HIP(pt_launch_step_persistent(b->P, cfg->variant, pgrid, s));
-                else PT_HIP(pt_launch_step(b->P, cfg->variant, block, s));
+                if (k == 1) PT_HIP(pt_launch_collision(b->P, mode, coll_grid, s));
+                else if (cfg->persistent) PT_HIP(pt_launch_step_persistent(b->P, mode, pgrid, cfg->spatial_order, s));
+                else PT_HIP(pt_launch_step(b->P, mode, block, s));
                 if (timed) {
                     PT_HIP(hipEventRecord(e1, s));
                     b->ev[k].emplace_back(e0, e1);
@@ -905,7 +910,7 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
     };
     if (!cfg->graph) return launch_all(b->profiling);
     if (s == nullptr) return fail(PT_EINVAL, "pt_batch_run: graph mode needs a non-default stream");
-    std::vector<int> key = {cfg->variant, block, steps, rounds, cfg->persistent, pgrid};
+    std::vector<int> key = {cfg->variant, block, steps, rounds, cfg->persistent, pgrid, cfg->spatial_order};
     pt_batch::GraphEntry *ge = nullptr;
     for (auto &g : b->graphs)
         if (g.key == key) ge = &g;

@@ -9,12 +9,16 @@
 
 namespace ptd { struct Params; }
 
+/* mode: bit 0 FP32 field tables (PT_VARIANT_MIXED), bit 1 containers with scale formulas,
+   analytic fields or conductors present */
+
 /* particle set-up (first call only) and the step budget of every particle */
-hipError_t pt_launch_init(const ptd::Params &P, int variant, int steps, hipStream_t stream);
+hipError_t pt_launch_init(const ptd::Params &P, int mode, int steps, hipStream_t stream);
 /* one round = the step kernel, then the collision kernel over the queue and the queue reset */
-hipError_t pt_launch_step(const ptd::Params &P, int variant, int block, hipStream_t stream);
-/* persistent variant of the step kernel: work-list compaction, then `grid` blocks of 256 slots */
-hipError_t pt_launch_step_persistent(const ptd::Params &P, int variant, int grid, hipStream_t stream);
-hipError_t pt_launch_collision(const ptd::Params &P, int variant, int coll_grid, hipStream_t stream);
+hipError_t pt_launch_step(const ptd::Params &P, int mode, int block, hipStream_t stream);
+/* persistent variant of the step kernel: the work list (in index order, or with spatial != 0
+   in Morton order of the particle positions), then `grid` blocks of 256 slots */
+hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, int spatial, hipStream_t stream);
+hipError_t pt_launch_collision(const ptd::Params &P, int mode, int coll_grid, hipStream_t stream);
 
 #endif /* PT_KERNELS_H_ */

@@ -198,6 +198,11 @@ PTD_NOINLINE FG scale_at_bounds(const Field &f, double x, double y, double z, do
 
 PTD bool has_box(const Field &f) { return f.box[1] < f.box[0] && f.box[3] < f.box[2] && f.box[5] < f.box[4]; }
 
+/* kernel modes: FP32 field tables (mixed precision); containers with scale formulas,
+   analytic fields or conductors present (without it the field sum handles tricubic maps
+   and uniform E with constant scales only, the code of the common case) */
+constexpr int MODE_FP32 = 1, MODE_FIELDS = 2;
+
 /* TFieldScaler::scalingFactor (src/field.cpp:7-10) from the compiled formula */
 PTD double scale_formula(const uint32_t *code, int n, const double *cst, double t) {
     return ptf_run(code, n, cst, t);
@@ -333,13 +338,13 @@ PTD void analytic_B(const Field &f, double x, double y, double z, double *B, dou
 /* TFieldContainer::BField (src/field.cpp:139-153) with the scale factor bs already
    evaluated, accumulated into the manager sum (TFieldManager::BField, src/fields.cpp:139-164);
    tricubic maps and, through analytic_B, the analytic fields and conductors */
-template <bool MIXED>
+template <int MODE>
 PTD void container_B(const Field &f, double x, double y, double z, double bs, double B[3], double dB[3][3], bool grad,
                      long long &nfield) {
     double Bt[3] = {0, 0, 0};
     double dBt[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     if (in_bounds(f, x, y, z) && bs != 0.) {
-        if (f.type >= PT_FIELD_CONDUCTOR) {
+        if ((MODE & MODE_FIELDS) && f.type >= PT_FIELD_CONDUCTOR) {
             double Ba[3], dBa[9];
             analytic_B(f, x, y, z, Ba, dBa, grad);
             Bt[0] = Ba[0]; Bt[1] = Ba[1]; Bt[2] = Ba[2];
@@ -355,7 +360,7 @@ PTD void container_B(const Field &f, double x, double y, double z, double bs, do
 #pragma unroll 1
                 for (int i = 0; i < 3; ++i) {
                     double Fv, Fx = 0, Fy = 0, Fz = 0;
-                    if (MIXED) {
+                    if (MODE & MODE_FP32) {
                         float fv, fx, fy, fz;
                         tric_eval_grad32(f.coef32 + ((size_t)cell * f.ncomp + i) * 64, (float)r[0], (float)r[1], (float)r[2], grad, fv, fx, fy, fz);
                         Fv = fv; Fx = fx; Fy = fy; Fz = fz;
@@ -394,20 +399,20 @@ PTD void container_B(const Field &f, double x, double y, double z, double bs, do
     }
 }
 
-template <bool MIXED>
+template <int MODE>
 PTD void field_B(const Params &P, double x, double y, double z, double t, double B[3], double dB[3][3], bool grad, long long &nfield) {
 #pragma unroll
     for (int i = 0; i < 3; i++) { B[i] = 0; if (grad) { dB[i][0] = 0; dB[i][1] = 0; dB[i][2] = 0; } }
     for (int k = 0; k < P.n_fields; ++k) {
         const Field &f = P.fields[k];
         /* TFieldScaler::scalingFactor (src/field.cpp:7-10), evaluated where the reference does */
-        const double bs = f.blen == 0 ? f.bscale : (in_bounds(f, x, y, z) ? scale_B(f, t) : 0.);
-        container_B<MIXED>(f, x, y, z, bs, B, dB, grad, nfield);
+        const double bs = (!(MODE & MODE_FIELDS) || f.blen == 0) ? f.bscale : (in_bounds(f, x, y, z) ? scale_B(f, t) : 0.);
+        container_B<MODE>(f, x, y, z, bs, B, dB, grad, nfield);
     }
 }
 
 /* TFieldContainer::EField (src/field.cpp:155-169) with the scale factor es already evaluated */
-template <bool MIXED>
+template <int MODE>
 PTD void container_E(const Field &f, double x, double y, double z, double es, double &Vt, double Et[3], long long &nfield) {
     Vt = 0; Et[0] = Et[1] = Et[2] = 0;
     if (in_bounds(f, x, y, z) && es != 0.) {
@@ -421,7 +426,7 @@ PTD void container_E(const Field &f, double x, double y, double z, double es, do
                 ++nfield;
                 double Fv, Fx, Fy, Fz;
                 int comp = f.has_b ? 3 : 0;
-                if (MIXED) {
+                if (MODE & MODE_FP32) {
                     float fv, fx, fy, fz;
                     tric_eval_grad32(f.coef32 + ((size_t)cell * f.ncomp + comp) * 64, (float)r[0], (float)r[1], (float)r[2], true, fv, fx, fy, fz);
                     Fv = fv; Fx = fx; Fy = fy; Fz = fz;
@@ -443,14 +448,14 @@ PTD void container_E(const Field &f, double x, double y, double z, double es, do
 }
 
 /* TFieldManager::EField (src/fields.cpp:167-182) */
-template <bool MIXED>
+template <int MODE>
 PTD void field_E(const Params &P, double x, double y, double z, double t, double *V, double E[3], long long &nfield) {
     *V = 0; E[0] = E[1] = E[2] = 0;
     for (int k = 0; k < P.n_fields; ++k) {
         const Field &f = P.fields[k];
         double Vt, Et[3];
-        const double es = f.elen == 0 ? f.escale : (in_bounds(f, x, y, z) ? scale_E(f, t) : 0.);
-        container_E<MIXED>(f, x, y, z, es, Vt, Et, nfield);
+        const double es = (!(MODE & MODE_FIELDS) || f.elen == 0) ? f.escale : (in_bounds(f, x, y, z) ? scale_E(f, t) : 0.);
+        container_E<MODE>(f, x, y, z, es, Vt, Et, nfield);
         *V += Vt;
         for (int i = 0; i < 3; i++) E[i] += Et[i];
     }
@@ -508,20 +513,20 @@ PTD double kinetic_energy(const Species &sp, const double *v) {
 
 /* TParticle::GetPotentialEnergy, src/particle.cpp:254-269; Bc/Vc: field values already
    known at this point (from the equation-of-motion evaluation), or nullptr */
-template <bool MIXED>
+template <int MODE>
 PTD double potential_energy(const Params &P, const Species &sp, const double *y, double t, const double *Bc, const double *Vc, Ctr &c) {
     double result = 0;
     if (sp.q != 0 || sp.mu != 0) {
         if (sp.mu != 0) {
             double B[3], dB[3][3];
             if (Bc) { B[0] = Bc[0]; B[1] = Bc[1]; B[2] = Bc[2]; }
-            else field_B<MIXED>(P, y[0], y[1], y[2], t, B, dB, false, c.field);
+            else field_B<MODE>(P, y[0], y[1], y[2], t, B, dB, false, c.field);
             result += -y[7] * sp.mu / ELE_E * __builtin_sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
         }
         if (sp.q != 0) {
             double E[3], V;
             if (Vc) V = *Vc;
-            else field_E<MIXED>(P, y[0], y[1], y[2], t, &V, E, c.field);
+            else field_E<MODE>(P, y[0], y[1], y[2], t, &V, E, c.field);
             result += sp.q / ELE_E * V;
         }
     }
@@ -579,7 +584,7 @@ PTD void to9(const double *s, double pol, double *y) {
    (x, y, z, vx, vy, vz, proper time, path length); the polarisation, whose
    derivative is identically +0.0, is passed separately.  B and V return the
    field the equation of motion used (hasB: B was evaluated). */
-template <bool MIXED>
+template <int MODE>
 PTD void derivs8(const Params &P, const Species &sp, const double *s, double t, double pol, double *d, double B[3],
                  double &V, bool &hasB, Ctr &c) {
     double y[9];
@@ -588,8 +593,8 @@ PTD void derivs8(const Params &P, const Species &sp, const double *s, double t, 
     B[0] = 0; B[1] = 0; B[2] = 0; V = 0;
     ++c.rhs;
     hasB = (sp.q != 0 || (sp.mu != 0 && pol != 0));
-    if (hasB) field_B<MIXED>(P, y[0], y[1], y[2], t, B, dB, true, c.field);
-    if (sp.q != 0) field_E<MIXED>(P, y[0], y[1], y[2], t, &V, E, c.field);
+    if (hasB) field_B<MODE>(P, y[0], y[1], y[2], t, B, dB, true, c.field);
+    if (sp.q != 0) field_E<MODE>(P, y[0], y[1], y[2], t, &V, E, c.field);
     double d9[9];
     eom(sp, y, B, dB, E, d9);
     d[0] = d9[0]; d[1] = d9[1]; d[2] = d9[2]; d[3] = d9[3]; d[4] = d9[4]; d[5] = d9[5]; d[6] = d9[6]; d[7] = d9[8];
@@ -849,12 +854,12 @@ PTD void on_step(const Params &P, Track &T, double x1, const double *y1, double 
 }
 
 /* TTracker::DoStep + TParticle::DoStep (src/tracking.cpp:268-278, src/particle.cpp:196-206) */
-template <bool MIXED>
+template <int MODE>
 PTD_NOINLINE bool do_step_hook(const Params &P, Track &T, double x1, const double *y1, double &x2, double *y2, const Dopri &st, int cursolid) {
     double x2temp = x2, y2temp[9];
     for (int k = 0; k < 9; ++k) y2temp[k] = y2[k];
     on_step(P, T, x1, y1, x2, y2, st, cursolid);
-    double H = kinetic_energy(T.sp, y2 + 3) + potential_energy<MIXED>(P, T.sp, y2, x2, nullptr, nullptr, T.c);
+    double H = kinetic_energy(T.sp, y2 + 3) + potential_energy<MODE>(P, T.sp, y2, x2, nullptr, nullptr, T.c);
     T.Hmax = (H < T.Hmax) ? T.Hmax : H;
     if (y2temp[7] != y2[7]) T.Nspinflip++;
     T.Nstep++;
@@ -1040,7 +1045,7 @@ PTD_NOINLINE int iterate_collision(const Params &P, Track &T, const Dopri &st, d
 }
 
 /* TTracker::CheckHit (src/tracking.cpp:176-226); the tail recursion is a loop */
-template <bool MIXED>
+template <int MODE>
 PTD_NOINLINE bool check_hit(const Params &P, Track &T, double x1, const double *y1in, double &x2, double *y2, const Dopri &st) {
     double y1[9];
     for (int k = 0; k < 9; ++k) y1[k] = y1in[k];
@@ -1066,13 +1071,13 @@ PTD_NOINLINE bool check_hit(const Params &P, Track &T, double x1, const double *
         int cursolid = current_solid(P.geom, T.solids, T.nsolid);
         int n = count_hits(P, T, y1, y2);
         if (n < 0) { T.stop = n == -1 ? ID_CGAL_ERROR : ID_UNSUPPORTED; return true; }
-        if (n == 0) return do_step_hook<MIXED>(P, T, x1, y1, x2, y2, st, cursolid);
+        if (n == 0) return do_step_hook<MODE>(P, T, x1, y1, x2, y2, st, cursolid);
         double xc1 = x1, xc2 = x2, yc1[9], yc2[9];
         for (int k = 0; k < 9; ++k) { yc1[k] = y1[k]; yc2[k] = y2[k]; }
         int r = iterate_collision(P, T, st, xc1, yc1, xc2, yc2);
         if (r < 0) { T.stop = ID_CGAL_ERROR; return true; }
         if (r == 0) return false;
-        if (xc1 > x1 && do_step_hook<MIXED>(P, T, x1, y1, xc1, yc1, st, cursolid)) {
+        if (xc1 > x1 && do_step_hook<MODE>(P, T, x1, y1, xc1, yc1, st, cursolid)) {
             x2 = xc1;
             for (int k = 0; k < 9; ++k) y2[k] = yc1[k];
             return true;
@@ -1099,13 +1104,13 @@ PTD void do_polarize(Track &T, double *y, double polarization, int flipspin) {
 }
 
 /* TTracker::IntegrateSpin without the BMT branch (src/tracking.cpp:373-394, 473-483) */
-template <bool MIXED>
+template <int MODE>
 PTD_NOINLINE double integrate_spin(const Params &P, Track &T, double *spin, const Dopri &st, double x2, const double *y2) {
     double x1 = st.t_old;
     const double *y1 = st.xo;
     double B1[3], B2[3], dB[3][3];
-    field_B<MIXED>(P, y1[0], y1[1], y1[2], x1, B1, dB, false, T.c.field);
-    field_B<MIXED>(P, y2[0], y2[1], y2[2], x2, B2, dB, false, T.c.field);
+    field_B<MODE>(P, y1[0], y1[1], y1[2], x1, B1, dB, false, T.c.field);
+    field_B<MODE>(P, y2[0], y2[1], y2[2], x2, B2, dB, false, T.c.field);
     double Babs1 = __builtin_sqrt(B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]);
     double Babs2 = __builtin_sqrt(B2[0] * B2[0] + B2[1] * B2[1] + B2[2] * B2[2]);
     double Bmax = T.sp.Bmax;
@@ -1243,7 +1248,7 @@ PTD void on_step_A(const Params &P, const Species &sp, int i, int &stop, double 
 
 /* TParticle::DoStep on the step path (src/particle.cpp:196-206); at_end: (x2, y2) is the
    unmodified end of the step, where the last equation-of-motion evaluation left the field */
-template <bool MIXED>
+template <int MODE>
 PTD bool do_step_hook_A(const Params &P, const Species &sp, Lane &L, int i, const DenseRef &dref, double x1,
                         const double *y1, double &x2, double *y2, int cursolid, bool at_end, Ctr &c) {
     double x2temp = x2, y2temp[9];
@@ -1253,7 +1258,7 @@ PTD bool do_step_hook_A(const Params &P, const Species &sp, Lane &L, int i, cons
     bool changed = !(x2temp == x2 && state_eq(y2temp, y2));
     bool cached = at_end && !changed && (L.flags & FL_BVALID);
     double H = kinetic_energy(sp, y2 + 3) +
-               potential_energy<MIXED>(P, sp, y2, x2, cached ? L.Bc : nullptr, cached ? &L.Vc : nullptr, c);
+               potential_energy<MODE>(P, sp, y2, x2, cached ? L.Bc : nullptr, cached ? &L.Vc : nullptr, c);
     L.Hmax = (H < L.Hmax) ? L.Hmax : H;
     if (y2temp[7] != y2[7]) P.st.Nspinflip[i] += 1;
     L.Nstep++;
@@ -1264,7 +1269,7 @@ PTD bool do_step_hook_A(const Params &P, const Species &sp, Lane &L, int i, cons
    TParticle::DoPolarize (src/particle.cpp:225-234) on the step path.  B1/B2: the field at the
    step start / end when the stepper evaluated it there (else nullptr).  Returns the new
    tracker polarisation in y[7]. */
-template <bool MIXED>
+template <int MODE>
 PTD void spin_step(const Params &P, int i, const Species &sp, double x1, double ys0, double ys1, double ys2,
                             double polo, double x2, double *y, bool haveB1, double B10, double B11, double B12,
                             bool haveB2, double B20, double B21, double B22, Ctr &c) {
@@ -1272,9 +1277,9 @@ PTD void spin_step(const Params &P, int i, const Species &sp, double x1, double 
     const size_t n = (size_t)S.n;
     double B1[3], B2[3], dB[3][3];
     if (haveB1) { B1[0] = B10; B1[1] = B11; B1[2] = B12; }
-    else field_B<MIXED>(P, ys0, ys1, ys2, x1, B1, dB, false, c.field);
+    else field_B<MODE>(P, ys0, ys1, ys2, x1, B1, dB, false, c.field);
     if (haveB2) { B2[0] = B20; B2[1] = B21; B2[2] = B22; }
-    else field_B<MIXED>(P, y[0], y[1], y[2], x2, B2, dB, false, c.field);
+    else field_B<MODE>(P, y[0], y[1], y[2], x2, B2, dB, false, c.field);
     double Babs1 = __builtin_sqrt(B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]);
     double Babs2 = __builtin_sqrt(B2[0] * B2[0] + B2[1] * B2[1] + B2[2] * B2[2]);
     double spin[5];
@@ -1340,7 +1345,7 @@ PTD void spin_step(const Params &P, int i, const Species &sp, double x1, double 
  * a chord that comes near a wall).  Returns pending = true, with the chord piece in HBM,
  * when a chord piece intersects the mesh; the collision path then finishes the step.
  */
-template <bool MIXED>
+template <int MODE>
 PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
     const Geom &G = P.geom;
     const Species &sp = P.species[L.sidx];
@@ -1365,7 +1370,7 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
     }
     if (!(L.flags & FL_DERIV)) {
         bool hb;
-        derivs8<MIXED>(P, sp, xc, ts, L.pol, L.dxc, L.Bc, L.Vc, hb, c);
+        derivs8<MODE>(P, sp, xc, ts, L.pol, L.dxc, L.Bc, L.Vc, hb, c);
         L.flags = (L.flags | FL_DERIV);
         L.flags = hb ? (L.flags | FL_BVALID) : (L.flags & ~FL_BVALID);
     }
@@ -1392,23 +1397,23 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
         const double h21 = dt * b21;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h21 * L.dxc[k];
-        derivs8<MIXED>(P, sp, tmp, ts + dt * (1.0 / 5), L.pol, k2, Bd, Vd, bd, c);
+        derivs8<MODE>(P, sp, tmp, ts + dt * (1.0 / 5), L.pol, k2, Bd, Vd, bd, c);
         const double h31 = dt * b31, h32 = dt * b32;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h31 * L.dxc[k] + h32 * k2[k];
-        derivs8<MIXED>(P, sp, tmp, ts + dt * (3.0 / 10), L.pol, k3, Bd, Vd, bd, c);
+        derivs8<MODE>(P, sp, tmp, ts + dt * (3.0 / 10), L.pol, k3, Bd, Vd, bd, c);
         const double h41 = dt * b41, h42 = dt * b42, h43 = dt * b43;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h41 * L.dxc[k] + h42 * k2[k] + h43 * k3[k];
-        derivs8<MIXED>(P, sp, tmp, ts + dt * (4.0 / 5), L.pol, k4, Bd, Vd, bd, c);
+        derivs8<MODE>(P, sp, tmp, ts + dt * (4.0 / 5), L.pol, k4, Bd, Vd, bd, c);
         const double h51 = dt * b51, h52 = dt * b52, h53 = dt * b53, h54 = dt * b54;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h51 * L.dxc[k] + h52 * k2[k] + h53 * k3[k] + h54 * k4[k];
-        derivs8<MIXED>(P, sp, tmp, ts + dt * (8.0 / 9), L.pol, k5, Bd, Vd, bd, c);
+        derivs8<MODE>(P, sp, tmp, ts + dt * (8.0 / 9), L.pol, k5, Bd, Vd, bd, c);
         const double h61 = dt * b61, h62 = dt * b62, h63 = dt * b63, h64 = dt * b64, h65 = dt * b65;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h61 * L.dxc[k] + h62 * k2[k] + h63 * k3[k] + h64 * k4[k] + h65 * k5[k];
-        derivs8<MIXED>(P, sp, tmp, ts + dt, L.pol, k6, Bd, Vd, bd, c);
+        derivs8<MODE>(P, sp, tmp, ts + dt, L.pol, k6, Bd, Vd, bd, c);
         const double g1 = dt * c1, g3 = dt * c3, g4 = dt * c4, g5 = dt * c5, g6 = dt * c6;
 #pragma unroll
         for (int k = 0; k < 8; ++k) xnew[k] = xc[k] + g1 * L.dxc[k] + g3 * k3[k] + g4 * k4[k] + g5 * k5[k] + g6 * k6[k];
@@ -1437,7 +1442,7 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
                 ds[(40 + k) * n + i] = k6[k];
             }
         }
-        derivs8<MIXED>(P, sp, xnew, ts + dt, polnew, k7, Bn, Vn, bn, c);
+        derivs8<MODE>(P, sp, xnew, ts + dt, polnew, k7, Bn, Vn, bn, c);
         double err = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -1553,7 +1558,7 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
                 pending = true;
                 return;
             } else {
-                r = do_step_hook_A<MIXED>(P, sp, L, i, dref, xa, y1, x2, y2, cursolid, piece_end, c);
+                r = do_step_hook_A<MODE>(P, sp, L, i, dref, xa, y1, x2, y2, cursolid, piece_end, c);
             }
         }
         reset = r;
@@ -1569,7 +1574,7 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
     }
     L.flags = reset ? (L.flags | FL_RESET) : (L.flags & ~FL_RESET);
     const bool y_is_xc = at_end && !altered && (L.flags & FL_BVALID);
-    spin_step<MIXED>(P, i, sp, told, xc[0], xc[1], xc[2], polo, x, y, b1v, B1c[0], B1c[1], B1c[2],
+    spin_step<MODE>(P, i, sp, told, xc[0], xc[1], xc[2], polo, x, y, b1v, B1c[0], B1c[1], B1c[2],
                      y_is_xc, L.Bc[0], L.Bc[1], L.Bc[2], c);
     if (L.stop == ID_UNKNOWN && y[6] >= L.tau) L.stop = ID_DECAYED;
     else if (L.stop == ID_UNKNOWN && (x >= tmax || y[8] >= sp.lmax)) L.stop = ID_NOT_FINISH;
@@ -1590,7 +1595,7 @@ PTD void finish_checks(Track &T, double x, const double *y, double tau, double t
  * the loop body of TTracker::IntegrateParticle with CheckHit's recursion
  * (src/tracking.cpp:96-147).
  */
-template <bool MIXED>
+template <int MODE>
 PTD void collision_path(const Params &P, int i, Ctr &c) {
     const State &S = P.st;
     const int n = S.n;
@@ -1639,7 +1644,7 @@ PTD void collision_path(const Params &P, int i, Ctr &c) {
     x2 = pd[(size_t)10 * n + i];
     for (int k = 0; k < 9; ++k) { y1[k] = pd[(size_t)(1 + k) * n + i]; y2[k] = pd[(size_t)(11 + k) * n + i]; }
 
-    bool reset = check_hit<MIXED>(P, T, x1, y1, x2, y2, st);
+    bool reset = check_hit<MODE>(P, T, x1, y1, x2, y2, st);
     if (reset) {
         x = x2;
         for (int k = 0; k < 9; ++k) y[k] = y2[k];
@@ -1656,7 +1661,7 @@ PTD void collision_path(const Params &P, int i, Ctr &c) {
             x2 = x1 + (x - x1) / __builtin_ceil(__builtin_sqrt(dev2) / MAX_TRACK_DEVIATION);
             dopri_calc_state(st, x2, y2);
         }
-        reset = check_hit<MIXED>(P, T, x1, y1, x2, y2, st);
+        reset = check_hit<MODE>(P, T, x1, y1, x2, y2, st);
         if (reset) {
             x = x2;
             for (int k = 0; k < 9; ++k) y[k] = y2[k];
@@ -1665,7 +1670,7 @@ PTD void collision_path(const Params &P, int i, Ctr &c) {
         for (int k = 0; k < 9; ++k) y1[k] = y2[k];
     }
     flags = reset ? (flags | FL_RESET) : (flags & ~FL_RESET);
-    double spinProjection = integrate_spin<MIXED>(P, T, spin, st, x, y);
+    double spinProjection = integrate_spin<MODE>(P, T, spin, st, x, y);
     bool spinHasFlipped = st.xo[7] != y[7];
     if (spinProjection != st.xo[7]) {
         do_polarize(T, y, spinProjection, T.sp.flipspin);
@@ -1694,7 +1699,7 @@ PTD void collision_path(const Params &P, int i, Ctr &c) {
  * of TTracker::IntegrateParticle (src/tracking.cpp:19-68): lifetime draw, initial
  * step size, and the solid stack by +z ray parity.
  */
-template <bool MIXED>
+template <int MODE>
 __global__ void __launch_bounds__(256) init_kernel(Params P) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const State &S = P.st;
@@ -1705,7 +1710,7 @@ __global__ void __launch_bounds__(256) init_kernel(Params P) {
     double x = S.t[i], y[9];
     for (int k = 0; k < 9; ++k) y[k] = S.y[(size_t)k * n + i];
     Ctr c = {0, 0, 0};
-    double Hmax = kinetic_energy(sp, y + 3) + potential_energy<MIXED>(P, sp, y, x, nullptr, nullptr, c);
+    double Hmax = kinetic_energy(sp, y + 3) + potential_energy<MODE>(P, sp, y, x, nullptr, nullptr, c);
     double tau;
     if (sp.tau > 0) tau = -ptm_log(1.0 - ptm_canonical(ptm_draw_u64(key, &ctr))) / (1. / sp.tau);
     else tau = sp.tmax;
@@ -1745,7 +1750,7 @@ __global__ void __launch_bounds__(256) init_kernel(Params P) {
 #ifndef PT_STEP_MIN_WAVES
 #define PT_STEP_MIN_WAVES 2
 #endif
-template <bool MIXED>
+template <int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STEP_MIN_WAVES))) step_kernel(Params P) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const State &S = P.st;
@@ -1758,7 +1763,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
     Ctr c = {0, 0, 0};
     bool pending = false;
     do {
-        stepA<MIXED>(P, i, L, pending, c);
+        stepA<MODE>(P, i, L, pending, c);
         --budget;
     } while (!pending && L.stop == ID_UNKNOWN && budget > 0);
     store_lane(P, i, L);
@@ -1777,7 +1782,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
  * stay full while particles of very different lifetimes drop out.  Every lane leaves
  * when the list is exhausted.
  */
-template <bool MIXED>
+template <int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STEP_MIN_WAVES))) step_kernel_persistent(Params P) {
     const State &S = P.st;
     const int count = S.wcount[0];
@@ -1798,7 +1803,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
             pending = false;
             have = true;
         }
-        stepA<MIXED>(P, i, L, pending, c);
+        stepA<MODE>(P, i, L, pending, c);
         --budget;
         if (pending || L.stop != ID_UNKNOWN || budget <= 0) {
             store_lane(P, i, L);
@@ -1826,17 +1831,91 @@ __global__ void __launch_bounds__(256) work_kernel(Params P) {
 
 __global__ void reset_work_kernel(int *wcount) { wcount[0] = 0; wcount[1] = 0; }
 
+/* ---- work list in spatial order (counting sort into Morton-ordered buckets) ---- */
+
+/* clears the bucket counts and the work-list counters (kernels, not memsets, so the
+   launch sequence can be captured in a graph) */
+__global__ void __launch_bounds__(256) clear_buckets_kernel(int *bhist, int *wcount) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b <= NBUCKET) bhist[b] = 0;
+    if (b == 0) { wcount[0] = 0; wcount[1] = 0; }
+}
+
+/* spreads the low BUCKET_BITS bits of v to every third bit */
+PTD unsigned spread3(unsigned v) {
+    unsigned r = 0;
+#pragma unroll
+    for (int k = 0; k < BUCKET_BITS; ++k) r |= ((v >> k) & 1u) << (3 * k);
+    return r;
+}
+
+/* bucket of each particle with steps to do: its cell of a 32^3 grid over the mesh
+   bounding box (the clearance grid's box), in Morton order */
+__global__ void __launch_bounds__(256) bucket_count_kernel(Params P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const State &S = P.st;
+    if (i >= S.n) return;
+    if (S.stop[i] == ID_UNKNOWN && (S.flags[i] & FL_STARTED) && S.budget[i] > 0) {
+        const Geom &G = P.geom;
+        const size_t n = (size_t)S.n;
+        unsigned c[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double u = (S.y[a * n + i] - G.glo[a]) * G.ginv_h[a] * ((double)(1 << BUCKET_BITS) / G.gn[a]);
+            /* NaN and negatives to bucket 0, beyond the box to the last bucket */
+            c[a] = !(u >= 0.0) ? 0u : (u >= (double)((1 << BUCKET_BITS) - 1) ? (unsigned)((1 << BUCKET_BITS) - 1) : (unsigned)u);
+        }
+        const int key = (int)(spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2));
+        S.pkey[i] = key;
+        atomicAdd(S.bhist + key, 1);
+        atomicAdd(S.wcount, 1);
+    }
+}
+
+/* exclusive prefix sum of the NBUCKET counts in one block of 1024 threads */
+__global__ void __launch_bounds__(1024) bucket_scan_kernel(int *bhist) {
+    constexpr int PER = NBUCKET / 1024;
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    int local[PER];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) { local[k] = bhist[t * PER + k]; sum += local[k]; }
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;                 /* exclusive prefix of this thread's range */
+#pragma unroll
+    for (int k = 0; k < PER; ++k) { bhist[t * PER + k] = run; run += local[k]; }
+}
+
+/* places each particle with steps to do at the next free slot of its bucket */
+__global__ void __launch_bounds__(256) bucket_scatter_kernel(Params P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const State &S = P.st;
+    if (i >= S.n) return;
+    if (S.stop[i] == ID_UNKNOWN && (S.flags[i] & FL_STARTED) && S.budget[i] > 0) {
+        int slot = atomicAdd(S.bhist + S.pkey[i], 1);
+        S.work[slot] = i;
+    }
+}
+
 /*
  * Collision kernel: the parked particles of the queue, compacted into full
  * waves, run the collision path (CheckHit's bisection, DoHit/OnHit) on their
  * pending chord piece and finish the rest of that step.
  */
-template <bool MIXED>
+template <int MODE>
 __global__ void __launch_bounds__(256) collision_kernel(Params P) {
     const int count = *P.st.qcount;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
         Ctr c = {0, 0, 0};
-        collision_path<MIXED>(P, P.st.queue[j], c);
+        collision_path<MODE>(P, P.st.queue[j], c);
     }
 }
 
@@ -1853,37 +1932,62 @@ __global__ void __launch_bounds__(256) budget_kernel(int *budget, int n, int ste
 
 /* ---- launchers (host side, called from pt_host.cpp) ---------------------- */
 
-hipError_t pt_launch_init(const ptd::Params &P, int variant, int steps, hipStream_t stream) {
+hipError_t pt_launch_init(const ptd::Params &P, int mode, int steps, hipStream_t stream) {
     int grid = (P.st.n + 255) / 256;
     if (grid == 0) return hipSuccess;
-    if (variant == 1) hipLaunchKernelGGL(ptd::init_kernel<true>, dim3(grid), dim3(256), 0, stream, P);
-    else hipLaunchKernelGGL(ptd::init_kernel<false>, dim3(grid), dim3(256), 0, stream, P);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL(ptd::init_kernel<0>, dim3(grid), dim3(256), 0, stream, P); break;
+    case 1: hipLaunchKernelGGL(ptd::init_kernel<1>, dim3(grid), dim3(256), 0, stream, P); break;
+    case 2: hipLaunchKernelGGL(ptd::init_kernel<2>, dim3(grid), dim3(256), 0, stream, P); break;
+    default: hipLaunchKernelGGL(ptd::init_kernel<3>, dim3(grid), dim3(256), 0, stream, P); break;
+    }
     hipLaunchKernelGGL(ptd::budget_kernel, dim3(grid), dim3(256), 0, stream, P.st.budget, P.st.n, steps);
     return hipGetLastError();
 }
 
-hipError_t pt_launch_step(const ptd::Params &P, int variant, int block, hipStream_t stream) {
+hipError_t pt_launch_step(const ptd::Params &P, int mode, int block, hipStream_t stream) {
     int grid = (P.st.n + block - 1) / block;
     if (grid == 0) return hipSuccess;
-    if (variant == 1) hipLaunchKernelGGL(ptd::step_kernel<true>, dim3(grid), dim3(block), 0, stream, P);
-    else hipLaunchKernelGGL(ptd::step_kernel<false>, dim3(grid), dim3(block), 0, stream, P);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL(ptd::step_kernel<0>, dim3(grid), dim3(block), 0, stream, P); break;
+    case 1: hipLaunchKernelGGL(ptd::step_kernel<1>, dim3(grid), dim3(block), 0, stream, P); break;
+    case 2: hipLaunchKernelGGL(ptd::step_kernel<2>, dim3(grid), dim3(block), 0, stream, P); break;
+    default: hipLaunchKernelGGL(ptd::step_kernel<3>, dim3(grid), dim3(block), 0, stream, P); break;
+    }
     return hipGetLastError();
 }
 
-hipError_t pt_launch_step_persistent(const ptd::Params &P, int variant, int grid, hipStream_t stream) {
+hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, int spatial, hipStream_t stream) {
     if (P.st.n == 0) return hipSuccess;
     int wgrid = (P.st.n + 255) / 256;
-    hipLaunchKernelGGL(ptd::reset_work_kernel, dim3(1), dim3(1), 0, stream, P.st.wcount);
-    hipLaunchKernelGGL(ptd::work_kernel, dim3(wgrid), dim3(256), 0, stream, P);
-    if (variant == 1) hipLaunchKernelGGL(ptd::step_kernel_persistent<true>, dim3(grid), dim3(256), 0, stream, P);
-    else hipLaunchKernelGGL(ptd::step_kernel_persistent<false>, dim3(grid), dim3(256), 0, stream, P);
+    if (spatial) {
+        static_assert(ptd::NBUCKET % 1024 == 0, "bucket scan layout");
+        hipLaunchKernelGGL(ptd::clear_buckets_kernel, dim3(ptd::NBUCKET / 256 + 1), dim3(256), 0, stream, P.st.bhist,
+                           P.st.wcount);
+        hipLaunchKernelGGL(ptd::bucket_count_kernel, dim3(wgrid), dim3(256), 0, stream, P);
+        hipLaunchKernelGGL(ptd::bucket_scan_kernel, dim3(1), dim3(1024), 0, stream, P.st.bhist);
+        hipLaunchKernelGGL(ptd::bucket_scatter_kernel, dim3(wgrid), dim3(256), 0, stream, P);
+    } else {
+        hipLaunchKernelGGL(ptd::reset_work_kernel, dim3(1), dim3(1), 0, stream, P.st.wcount);
+        hipLaunchKernelGGL(ptd::work_kernel, dim3(wgrid), dim3(256), 0, stream, P);
+    }
+    switch (mode) {
+    case 0: hipLaunchKernelGGL(ptd::step_kernel_persistent<0>, dim3(grid), dim3(256), 0, stream, P); break;
+    case 1: hipLaunchKernelGGL(ptd::step_kernel_persistent<1>, dim3(grid), dim3(256), 0, stream, P); break;
+    case 2: hipLaunchKernelGGL(ptd::step_kernel_persistent<2>, dim3(grid), dim3(256), 0, stream, P); break;
+    default: hipLaunchKernelGGL(ptd::step_kernel_persistent<3>, dim3(grid), dim3(256), 0, stream, P); break;
+    }
     return hipGetLastError();
 }
 
-hipError_t pt_launch_collision(const ptd::Params &P, int variant, int coll_grid, hipStream_t stream) {
+hipError_t pt_launch_collision(const ptd::Params &P, int mode, int coll_grid, hipStream_t stream) {
     if (P.st.n == 0) return hipSuccess;
-    if (variant == 1) hipLaunchKernelGGL(ptd::collision_kernel<true>, dim3(coll_grid), dim3(256), 0, stream, P);
-    else hipLaunchKernelGGL(ptd::collision_kernel<false>, dim3(coll_grid), dim3(256), 0, stream, P);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL(ptd::collision_kernel<0>, dim3(coll_grid), dim3(256), 0, stream, P); break;
+    case 1: hipLaunchKernelGGL(ptd::collision_kernel<1>, dim3(coll_grid), dim3(256), 0, stream, P); break;
+    case 2: hipLaunchKernelGGL(ptd::collision_kernel<2>, dim3(coll_grid), dim3(256), 0, stream, P); break;
+    default: hipLaunchKernelGGL(ptd::collision_kernel<3>, dim3(coll_grid), dim3(256), 0, stream, P); break;
+    }
     hipLaunchKernelGGL(ptd::reset_queue_kernel, dim3(1), dim3(1), 0, stream, P.st.qcount);
     return hipGetLastError();
 }

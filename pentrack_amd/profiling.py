@@ -4,8 +4,9 @@
 device copy of known size (the calibration: FETCH_SIZE and WRITE_SIZE of a kernel
 that reads and writes exactly COPY_BYTES), then the benchmark workload (1e6
 neutrons, cylinder, 64^3 map) advanced by a few run calls of 16 RK steps with the
-step kernel KERNEL (step_kernel_fp64, step_kernel_mixed or
-step_kernel_persistent_mixed).  Profile it twice, once per counter:
+step kernel KERNEL (step_kernel_fp64, step_kernel_mixed,
+step_kernel_persistent_mixed, or step_kernel_persistent_mixed_spatial: the persistent
+kernel with its work list in Morton order of the particle positions).  Profile it twice, once per counter:
 
     rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 -m pentrack_amd.profiling run KERNEL
     rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 -m pentrack_amd.profiling run KERNEL
@@ -20,10 +21,11 @@ import sys
 
 COPY_BYTES = 1 << 30
 KERNELS = {
-    # name: (variant, persistent, bytes per field evaluation, dispatch-name pattern)
-    "step_kernel_fp64": (0, 0, 1536, "step_kernel<false>"),
-    "step_kernel_mixed": (1, 0, 768, "step_kernel<true>"),
-    "step_kernel_persistent_mixed": (1, 1, 768, "step_kernel_persistent<true>"),
+    # name: (variant, persistent, bytes per field evaluation, dispatch-name pattern, spatial order)
+    "step_kernel_fp64": (0, 0, 1536, "step_kernel<false>", 0),
+    "step_kernel_mixed": (1, 0, 768, "step_kernel<true>", 0),
+    "step_kernel_persistent_mixed": (1, 1, 768, "step_kernel_persistent<true>", 0),
+    "step_kernel_persistent_mixed_spatial": (1, 1, 768, "step_kernel_persistent<true>", 1),
 }
 
 
@@ -41,11 +43,11 @@ def run(kernel="step_kernel_fp64"):
     torch.cuda.synchronize()
     del src, dst
     w = workloads.cylinder(n=1_000_000, seed=2, tmax=200.0, n_grid=64)
-    variant, persistent, bpe, _ = KERNELS[kernel]
+    variant, persistent, bpe, _, spatial = KERNELS[kernel]
     b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
               with_fp32=bool(variant))
     stream = torch.cuda.current_stream().cuda_stream
-    cfg = LaunchConfig(variant=variant, persistent=persistent, steps_per_launch=16)
+    cfg = LaunchConfig(variant=variant, persistent=persistent, steps_per_launch=16, spatial_order=spatial)
     b.set_profiling(True)
     for _ in range(3):
         b.run(cfg, stream)

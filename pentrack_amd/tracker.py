@@ -24,12 +24,14 @@ class LaunchConfig:
     grid: int = 0
     rounds: int = 0
     graph: int = 0
+    spatial_order: int = 0
     leaf_size: int = 4
 
     def native(self):
         c = _native.LaunchCfg()
-        c.variant, c.block, c.steps_per_launch, c.persistent, c.grid, c.rounds, c.graph = (
-            self.variant, self.block, self.steps_per_launch, self.persistent, self.grid, self.rounds, self.graph)
+        c.variant, c.block, c.steps_per_launch, c.persistent, c.grid, c.rounds, c.graph, c.spatial_order = (
+            self.variant, self.block, self.steps_per_launch, self.persistent, self.grid, self.rounds, self.graph,
+            self.spatial_order)
         return c
 
 
