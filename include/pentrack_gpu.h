@@ -215,6 +215,37 @@ int pt_batch_step_counts(pt_batch *b, void *stream, int64_t *accepted, int64_t *
    launch counts, ms[0]/launches[0] for the step kernel, ms[1]/launches[1] for the
    collision kernel (with its queue reset). */
 int pt_batch_set_profiling(pt_batch *b, int on);
+
+/* a decay product: TNeutron::Decay (src/neutron.cpp:285-362) creates a proton and an
+   electron at the decay point with these arguments of the TProton / TElectron
+   constructors (src/particle.cpp:36-90) */
+typedef struct {
+    int32_t parent;              /* index of the decayed neutron in the batch */
+    int32_t kind;                /* 1 proton, 2 electron */
+    double t, pos[3];            /* decay time and point */
+    double E, phi, theta;        /* kinetic energy [eV], azimuth and polar angle of the velocity */
+    int32_t pol, pad;            /* polarisation (= spin projection), +-1 */
+} pt_secondary;
+
+/*
+ * TParticle::DoDecay / TNeutron::Decay (src/particle.cpp:219-223, src/neutron.cpp:285-362)
+ * for the neutrons of the batch that stopped with ID_DECAYED (the secondaries the
+ * reference appends, src/tracking.cpp:152-155): *count = 2 x decayed neutrons, the
+ * first min(cap, *count) records written to out, ordered by parent (proton, then
+ * electron).  The draws continue each neutron's random stream; the products are the
+ * same on every call.  Synchronises the stream.
+ */
+int pt_batch_secondaries(pt_batch *b, void *stream, int cap, pt_secondary *out, int *count);
+/* for tests: the proton recoil spectrum as std::piecewise_linear_distribution holds it
+   (n_int + 1 abscissas, spectrum values and normalised densities, n_int cumulative
+   probabilities and slopes); with x == NULL only *n_int is returned */
+int pt_proton_beta_table(int cap, double *x, double *raw, double *den, double *cp, double *m, int *n_int);
+/* for tests: the proton energies sampled for the variates p[0..n-1] in [0, 1) */
+int pt_beta_sample(int n, const double *p, double *out);
+/* for tests: the decay kinematics on the host for stream (key, ctr) and neutron velocity v[3]:
+   out6 = Ep phip thetap Ee phie thetae, pol2 = polarisations, *ctr_out = stream position after */
+int pt_neutron_decay_host(uint64_t key, uint64_t ctr, const double *v, double *out6, int32_t *pol2,
+                          uint64_t *ctr_out);
 int pt_batch_kernel_times(pt_batch *b, void *stream, double *ms, int64_t *launches);
 
 /*

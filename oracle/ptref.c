@@ -1606,6 +1606,7 @@ done_noint:
     out->n_hits_logged = tp->n_hits;
     out->n_rhs = tp->n_rhs; out->n_field = tp->n_field; out->n_tri_tests = tp->n_tri;
     out->n_bisect = tp->n_bisect; out->n_pieces = tp->n_pieces;
+    out->ctr_end = tp->ctr;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1760,6 +1761,175 @@ int ptref_track(const ptref_geom *g, int n_fields, const ptref_field *fields,
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* neutron decay (TNeutron::Decay, src/neutron.cpp:285-362)                   */
+/* ------------------------------------------------------------------------ */
+
+/* ProtonBetaSpectrum, src/mc.cpp:84-92: long double constants of src/globals.cpp,
+   double intermediates, pow(x, 2) evaluated as x*x, and 4/3 an integer quotient */
+static double beta_spectrum(double E) {
+    const long double e = 1.602176487E-19L, c0 = 299792458.L;
+    const long double mn = 1.674927211E-27L / e, mp = 1.672621637E-27L / e, me = 9.10938215e-31L / e;
+    double dm = mn - mp;
+    double xi = me / dm;
+    double sg = 1 - 2 * E * mn / (dm * dm) / (c0 * c0);
+    double w = (sg - xi * xi) / sg;
+    double root = sqrt(1 - sg);
+    double g1 = (w * w) * root * (4 * (1 + xi * xi / sg) - 4 / 3 * ((sg - xi * xi) / sg * (1 - sg)));
+    double g2 = (w * w) * root * (4 * (1 + xi * xi / sg - 2 * sg) - 4 / 3 * ((sg - xi * xi) / sg * (1 - sg)));
+    return 0.5 * (g1 + -0.1017 * g2);
+}
+
+/* interpolate_distribution / adaptive_linear_interpolation, include/mc.h:244-296 */
+typedef struct { double *x, *d; int n, cap; } beta_pts;
+static void beta_push(beta_pts *b, double x, double d) {
+    if (b->n == b->cap) {
+        b->cap = b->cap ? 2 * b->cap : 256;
+        b->x = (double *)realloc(b->x, b->cap * sizeof(double));
+        b->d = (double *)realloc(b->d, b->cap * sizeof(double));
+    }
+    b->x[b->n] = x; b->d[b->n] = d; ++b->n;
+}
+static void beta_refine(beta_pts *b, double nx, double nv, double tol, double maxd) {
+    double x0 = b->x[b->n - 1], v0 = b->d[b->n - 1];
+    double h = nx - x0;
+    double I0 = h * (v0 + nv) / 2.;
+    double mx = (x0 + nx) / 2.;
+    double mv = beta_spectrum(mx);
+    double I1 = (mx - x0) * (v0 + mv) / 2.;
+    double I2 = (nx - mx) * (mv + nv) / 2.;
+    if ((h > maxd) || ((fabs(I1 + I2 - I0) > tol) && (h > maxd / 100.))) {
+        beta_refine(b, mx, mv, tol / 2, maxd);
+        beta_refine(b, nx, nv, tol / 2, maxd);
+    } else {
+        beta_push(b, mx, mv);
+        beta_push(b, nx, nv);
+    }
+}
+
+/* std::piecewise_linear_distribution<double> parameter (libstdc++ _M_initialize) */
+static struct { double *x, *den, *cp, *m; int nint; } g_beta;
+static pthread_once_t g_beta_once = PTHREAD_ONCE_INIT;
+static void beta_build(void) {
+    beta_pts b = {0, 0, 0, 0};
+    beta_push(&b, 0., beta_spectrum(0.));
+    beta_refine(&b, 750., beta_spectrum(750.), DBL_EPS * 1000., (750. - 0.) / 10.);
+    int ni = b.n - 1;
+    double *cp = (double *)malloc(ni * sizeof(double)), *m = (double *)malloc(ni * sizeof(double));
+    double sum = 0.0;
+    for (int k = 0; k < ni; ++k) {
+        const double delta = b.x[k + 1] - b.x[k];
+        sum += 0.5 * (b.d[k + 1] + b.d[k]) * delta;
+        cp[k] = sum;
+        m[k] = (b.d[k + 1] - b.d[k]) / delta;
+    }
+    for (int k = 0; k <= ni; ++k) b.d[k] = b.d[k] / sum;
+    for (int k = 0; k < ni; ++k) { cp[k] = cp[k] / sum; m[k] = m[k] / sum; }
+    cp[ni - 1] = 1.0;
+    g_beta.x = b.x; g_beta.den = b.d; g_beta.cp = cp; g_beta.m = m; g_beta.nint = ni;
+}
+
+/* std::piecewise_linear_distribution<double>::operator() (libstdc++ random.tcc) */
+static double beta_sample(double p) {
+    int i = 0;
+    while (i < g_beta.nint && g_beta.cp[i] < p) ++i;          /* lower_bound */
+    const double pref = i > 0 ? g_beta.cp[i - 1] : 0.0;
+    const double a = 0.5 * g_beta.m[i];
+    const double b = g_beta.den[i];
+    const double cm = p - pref;
+    double x = g_beta.x[i];
+    if (a == 0) x += cm / b;
+    else {
+        const double d = b * b + 4.0 * a * cm;
+        x += 0.5 * (sqrt(d) - b) / a;
+    }
+    return x;
+}
+
+/* BOOST, src/globals.cpp:67-78 */
+static void lorentz_boost(const double beta[3], double p[4]) {
+    double b2 = beta[0] * beta[0] + beta[1] * beta[1] + beta[2] * beta[2];
+    double gamma = 1.0 / sqrt(1.0 - b2);
+    double bp = beta[0] * p[1] + beta[1] * p[2] + beta[2] * p[3];
+    double gamma2 = b2 > 0 ? (gamma - 1.0) / b2 : 0.0;
+    p[1] = (p[1] + gamma2 * bp * beta[0] + gamma * beta[0] * p[0]);
+    p[2] = (p[2] + gamma2 * bp * beta[1] + gamma * beta[1] * p[0]);
+    p[3] = (p[3] + gamma2 * bp * beta[2] + gamma * beta[2] * p[0]);
+    p[0] = (gamma * (p[0] + bp));
+}
+
+#if PTREF_REFSEM
+#define R_ATAN2(y, x) atan2(y, x)
+#else
+#define R_ATAN2(y, x) ptm_atan2(y, x)
+#endif
+
+/* TNeutron::Decay, src/neutron.cpp:285-362 (ld: the reference's long double constants) */
+static void neutron_decay(uint64_t key, uint64_t *ctr, const double v[3], double o6[6], int32_t pol[2]) {
+    const ld c0 = C_0, mp = MASS_P, mn = MASS_N, me = MASS_E;
+    double m_nue = 1 / (c0 * c0);
+    double twopi = 2 * PI, halfpi = PI / 2.;
+    double pp[4], pe[4];
+    double Ekin = beta_sample(ptm_canonical(ptm_draw_u64(key, ctr)));
+    pp[0] = Ekin / c0 + mp * c0;
+    double pabs = SQRT_LD(pp[0] * pp[0] - mp * mp * c0 * c0);
+    double beta1 = ptm_canonical(ptm_draw_u64(key, ctr)) * (twopi - 0.0) + 0.0;
+    double delta1 = R_ACOS(R_COS(0.0) - ptm_canonical(ptm_draw_u64(key, ctr)) * (R_COS(0.0) - R_COS(halfpi)));
+    pp[1] = pabs * R_SIN(delta1) * R_COS(beta1);
+    pp[2] = pabs * R_SIN(delta1) * R_SIN(beta1);
+    pp[3] = pabs * R_COS(delta1);
+    double virt[4] = {(double)mn * (double)c0 - pp[0], -pp[1], -pp[2], -pp[3]};
+    double m2v = virt[0] * virt[0] - virt[1] * virt[1] - virt[2] * virt[2] - virt[3] * virt[3];
+    pe[0] = (m2v + me * me * c0 * c0 - m_nue * m_nue * c0 * c0) / 2 / sqrt(m2v);
+    pabs = SQRT_LD(pe[0] * pe[0] - me * me * c0 * c0);
+    double beta2 = ptm_canonical(ptm_draw_u64(key, ctr)) * (twopi - 0.0) + 0.0;
+    double delta2 = R_ACOS(R_COS(0.0) - ptm_canonical(ptm_draw_u64(key, ctr)) * (R_COS(0.0) - R_COS(halfpi)));
+    pe[1] = pabs * R_SIN(delta2) * R_COS(beta2);
+    pe[2] = pabs * R_SIN(delta2) * R_SIN(beta2);
+    pe[3] = pabs * R_COS(delta2);
+    double bv[3] = {virt[1] / virt[0], virt[2] / virt[0], virt[3] / virt[0]};
+    lorentz_boost(bv, pe);
+    double pnue[4] = {virt[0] - pe[0], virt[1] - pe[1], virt[2] - pe[2], virt[3] - pe[3]};
+    double bn[3] = {v[0] / (double)c0, v[1] / (double)c0, v[2] / (double)c0};
+    lorentz_boost(bn, pe);
+    lorentz_boost(bn, pp);
+    lorentz_boost(bn, pnue);
+    double bp2 = (pp[1] * pp[1] + pp[2] * pp[2] + pp[3] * pp[3]) / pp[0] / pp[0];
+    o6[0] = mp * c0 * c0 * bp2 * (0.5 + bp2 * (3. / 8. + bp2 * (5. / 16.)));
+    o6[1] = R_ATAN2(pp[2], pp[1]);
+    o6[2] = R_ACOS(pp[3] / sqrt(pp[1] * pp[1] + pp[2] * pp[2] + pp[3] * pp[3]));
+    o6[3] = pe[0] * c0 - me * c0 * c0;
+    o6[4] = R_ATAN2(pe[2], pe[1]);
+    o6[5] = R_ACOS(pe[3] / sqrt(pe[1] * pe[1] + pe[2] * pe[2] + pe[3] * pe[3]));
+    pol[0] = ptm_canonical(ptm_draw_u64(key, ctr)) < 0.5 ? 1 : -1;
+    pol[1] = ptm_canonical(ptm_draw_u64(key, ctr)) < 0.5 ? 1 : -1;
+}
+
+/* decay products of n neutrons: stream (seed, pid[i]) at position ctr[i], velocity v[i];
+   out6 [n][6] = Ep phip thetap Ee phie thetae, pol [n][2] */
+int ptref_neutron_decay(int n, uint64_t seed, const uint64_t *pid, const uint64_t *ctr, const double *v,
+                        double *out6, int32_t *pol) {
+    pthread_once(&g_beta_once, beta_build);
+    for (int i = 0; i < n; ++i) {
+        uint64_t c = ctr[i];
+        neutron_decay(ptm_stream_key(seed, pid[i]), &c, v + 3 * i, out6 + 6 * i, pol + 2 * i);
+    }
+    return 0;
+}
+
+/* the sampled spectrum: returns the number of intervals; copies when x != NULL */
+int ptref_beta_table(int cap, double *x, double *den, double *cp, double *m) {
+    pthread_once(&g_beta_once, beta_build);
+    if (x) {
+        if (cap < g_beta.nint + 1) return -1;
+        memcpy(x, g_beta.x, (g_beta.nint + 1) * sizeof(double));
+        memcpy(den, g_beta.den, (g_beta.nint + 1) * sizeof(double));
+        memcpy(cp, g_beta.cp, g_beta.nint * sizeof(double));
+        memcpy(m, g_beta.m, g_beta.nint * sizeof(double));
+    }
+    return g_beta.nint;
+}
+
 /* ---- component entry points used by the oracle's own tests ------------- */
 
 int ptref_collisions(const ptref_geom *g, const double *p1, const double *p2, int cap,
@@ -1884,12 +2054,16 @@ int ptref_detmath(int which, int n, const double *x, const double *y, double *ou
             case 6: r = ptm_hypot(x[i], y[i]); break;
             case 7: ptm_csqrt(x[i], y[i], &r, &im); break;
             case 8: ptm_csqrt(x[i], y[i], &im, &r); break;
+            case 9: r = ptm_atan(x[i]); break;
+            case 10: r = ptm_atan2(x[i], y[i]); break;
             default: return -1;
         }
         out[i] = r;
     }
     return 0;
 }
+
+uint64_t ptref_stream_key(uint64_t seed, uint64_t id) { return ptm_stream_key(seed, id); }
 
 int ptref_rng(uint64_t seed, uint64_t id, int n, uint64_t *raw, double *canonical) {
     uint64_t key = ptm_stream_key(seed, id), ctr = 0;

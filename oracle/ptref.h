@@ -80,6 +80,7 @@ typedef struct {
     int32_t solidend;         /* ID of the solid the particle ended in */
     int32_t n_hits_logged;
     int64_t n_rhs, n_field, n_tri_tests, n_bisect, n_pieces;  /* op counters */
+    uint64_t ctr_end;         /* position of the particle's random stream at the end */
 } ptref_out;
 
 typedef struct {

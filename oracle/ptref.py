@@ -62,6 +62,7 @@ OUT_DTYPE = np.dtype([
     ("Hmax", "<f8"), ("noflipprob", "<f8"), ("tau", "<f8"),
     ("solidend", "<i4"), ("n_hits", "<i4"),
     ("n_rhs", "<i8"), ("n_field", "<i8"), ("n_tri_tests", "<i8"), ("n_bisect", "<i8"), ("n_pieces", "<i8"),
+    ("ctr_end", "<u8"),
 ], align=True)
 HIT_DTYPE = np.dtype([
     ("t", "<f8"), ("tri", "<i4"), ("leaving", "<i4"), ("entering", "<i4"),
@@ -94,6 +95,11 @@ def load(mode="fp64"):
     L.ptref_rotate_vector.argtypes = [dp, dp, dp]
     L.ptref_rotate_vector.restype = None
     L.ptref_field_eval.argtypes = [C.c_int, C.POINTER(Field), C.c_int, dp, C.c_double, dp, dp, dp, dp]
+    u64p = C.POINTER(C.c_uint64)
+    L.ptref_neutron_decay.argtypes = [C.c_int, C.c_uint64, u64p, u64p, dp, dp, C.POINTER(C.c_int32)]
+    L.ptref_beta_table.argtypes = [C.c_int, dp, dp, dp, dp]
+    L.ptref_stream_key.argtypes = [C.c_uint64, C.c_uint64]
+    L.ptref_stream_key.restype = C.c_uint64
     L.ptref_tricubic_eval.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
     L.ptref_tricubic_eval.restype = C.c_double
     L.ptref_integrate_free.argtypes = [C.c_int, C.POINTER(Field), C.c_int, C.c_double, dp, C.c_int, dp, dp, dp]
@@ -195,14 +201,46 @@ def track(workload, mode="fp64", hit_cap=0, threads=1, sel=None, problem=None):
 
 
 def detmath(which, x, y=None, mode="fp64"):
-    """Deterministic elementary functions of pt_detmath.h (log exp pow sin cos acos hypot csqrt)."""
-    names = {"log": 0, "exp": 1, "pow": 2, "sin": 3, "cos": 4, "acos": 5, "hypot": 6, "csqrt_re": 7, "csqrt_im": 8}
+    """Deterministic elementary functions of pt_detmath.h (log exp pow sin cos acos hypot csqrt atan
+    atan2(x, y))."""
+    names = {"log": 0, "exp": 1, "pow": 2, "sin": 3, "cos": 4, "acos": 5, "hypot": 6, "csqrt_re": 7, "csqrt_im": 8,
+             "atan": 9, "atan2": 10}
     L = load(mode)
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, dtype=np.float64)
     out = np.empty_like(x)
     assert L.ptref_detmath(names[which], len(x), _dp(x), _dp(y), _dp(out)) == 0
     return out
+
+
+def neutron_decay(seed, pid, ctr, v, mode="fp64"):
+    """TNeutron::Decay for neutrons with stream (seed, pid[i]) at position ctr[i] and velocity
+    v[i]: (out [n][6] = Ep phip thetap Ee phie thetae, pol [n][2])."""
+    L = load(mode)
+    pid = np.ascontiguousarray(pid, dtype=np.uint64)
+    ctr = np.ascontiguousarray(ctr, dtype=np.uint64)
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+    n = len(v)
+    out = np.zeros((n, 6))
+    pol = np.zeros((n, 2), np.int32)
+    u64 = C.POINTER(C.c_uint64)
+    assert L.ptref_neutron_decay(n, C.c_uint64(seed), pid.ctypes.data_as(u64), ctr.ctypes.data_as(u64), _dp(v),
+                                 _dp(out), pol.ctypes.data_as(C.POINTER(C.c_int32))) == 0
+    return out, pol
+
+
+def stream_key(seed, pid, mode="fp64"):
+    """Key of particle pid's random stream (pt_detmath.h ptm_stream_key)."""
+    return int(load(mode).ptref_stream_key(int(seed), int(pid)))
+
+
+def beta_table(mode="fp64"):
+    """The oracle's proton-spectrum parameter: (x, den, cp, m)."""
+    L = load(mode)
+    ni = L.ptref_beta_table(0, None, None, None, None)
+    x, den, cp, m = np.zeros(ni + 1), np.zeros(ni + 1), np.zeros(ni), np.zeros(ni)
+    assert L.ptref_beta_table(ni + 1, _dp(x), _dp(den), _dp(cp), _dp(m)) == ni
+    return x, den, cp, m
 
 
 def rng_stream(seed, pid, n, mode="fp64"):

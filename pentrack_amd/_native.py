@@ -83,6 +83,13 @@ class HitRec(C.Structure):
     ]
 
 
+class SecondaryRec(C.Structure):
+    _fields_ = [
+        ("parent", C.c_int32), ("kind", C.c_int32), ("t", C.c_double), ("pos", C.c_double * 3),
+        ("E", C.c_double), ("phi", C.c_double), ("theta", C.c_double), ("pol", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
 class LaunchCfg(C.Structure):
     _fields_ = [
         ("variant", C.c_int), ("block", C.c_int), ("steps_per_launch", C.c_int),
@@ -104,7 +111,12 @@ HIT_DTYPE = np.dtype([
     ("normal", "<f8", (3,)), ("v1", "<f8", (3,)), ("v2", "<f8", (3,)),
     ("pol1", "<f8"), ("pol2", "<f8"),
 ], align=True)
+SEC_DTYPE = np.dtype([
+    ("parent", "<i4"), ("kind", "<i4"), ("t", "<f8"), ("pos", "<f8", (3,)),
+    ("E", "<f8"), ("phi", "<f8"), ("theta", "<f8"), ("pol", "<i4"), ("pad", "<i4"),
+], align=True)
 assert OUT_DTYPE.itemsize == C.sizeof(ParticleOut), (OUT_DTYPE.itemsize, C.sizeof(ParticleOut))
+assert SEC_DTYPE.itemsize == C.sizeof(SecondaryRec), (SEC_DTYPE.itemsize, C.sizeof(SecondaryRec))
 assert HIT_DTYPE.itemsize == C.sizeof(HitRec), (HIT_DTYPE.itemsize, C.sizeof(HitRec))
 
 # every symbol include/pentrack_gpu.h declares (checked by the test suite)
@@ -116,6 +128,7 @@ EXPORTS = [
     "pt_batch_kernel_times",
     "pt_request_quit", "pt_last_error", "pt_clearance_grid", "pt_tricubic_matrix",
     "pt_spline_grid_diff", "pt_rk4_forward_ckpt", "pt_rk4_adjoint", "pt_formula_eval",
+    "pt_batch_secondaries", "pt_proton_beta_table", "pt_neutron_decay_host", "pt_beta_sample",
 ]
 
 _lib = None
@@ -166,6 +179,11 @@ def lib():
     L.pt_clearance_grid.argtypes = [C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float),
                                     C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pt_formula_eval.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    dp = C.POINTER(C.c_double)
+    L.pt_batch_secondaries.argtypes = [vp, vp, C.c_int, vp, C.POINTER(C.c_int)]
+    L.pt_proton_beta_table.argtypes = [C.c_int, dp, dp, dp, dp, dp, C.POINTER(C.c_int)]
+    L.pt_beta_sample.argtypes = [C.c_int, dp, dp]
+    L.pt_neutron_decay_host.argtypes = [C.c_uint64, C.c_uint64, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
     L.pt_request_quit.restype = None
     L.pt_last_error.restype = C.c_char_p
     _lib = L

@@ -251,6 +251,42 @@ PTM_FN double ptm_acos(double x) {
     }
 }
 
+/* ---- arc tangent --------------------------------------------------------- */
+/* atan on [0, inf): 1/x above 1, pi/4 - atan((1-x)/(1+x)) on (1/sqrt(3), 1], then
+   atan(u) = asin(u / sqrt(1 + u^2)) with |u| <= 1/sqrt(3), i.e. asin argument <= 1/2 */
+PTM_FN double ptm_atan_pos(double x) {
+    const double pio2_hi = 1.57079632679489655800e+00, pio2_lo = 6.12323399573676603587e-17;
+    const double pio4_hi = 7.85398163397448278999e-01, pio4_lo = 3.06161699786838301793e-17;
+    int inv = 0, fold = 0;
+    if (x > 1.0) { x = 1.0 / x; inv = 1; }
+    if (x > 0.57735026918962576451) { x = (1.0 - x) / (1.0 + x); fold = 1; }
+    double r = ptm_asin_small(x / __builtin_sqrt(1.0 + x * x));
+    if (fold) r = pio4_hi - (r - pio4_lo);
+    if (inv) r = pio2_hi - (r - pio2_lo);
+    return r;
+}
+PTM_FN double ptm_atan(double x) {
+    if (x != x) return x;
+    if (x < 0.0) return -ptm_atan_pos(-x);
+    if (x == 0.0) return x;
+    if (x > 1e300) return 1.57079632679489655800e+00;
+    return ptm_atan_pos(x);
+}
+/* atan2(y, x) with the C quadrant and signed-zero conventions */
+PTM_FN double ptm_atan2(double y, double x) {
+    const double pi_hi = 3.14159265358979311600e+00, pi_lo = 1.22464679914735317723e-16;
+    const double pio2 = 1.57079632679489655800e+00;
+    if (x != x || y != y) return x + y;
+    const int yneg = (ptm_to_bits(y) >> 63) != 0;
+    if (x == 0.0) {
+        if (y == 0.0) return (ptm_to_bits(x) >> 63) ? (yneg ? -pi_hi : pi_hi) : y;
+        return yneg ? -pio2 : pio2;
+    }
+    double r = ptm_atan(y / x);
+    if (x > 0.0) return r;
+    return yneg ? -pi_hi + (r - pi_lo) : pi_hi + (r + pi_lo);
+}
+
 /* ---- complex helpers (Fermi-potential reflection probability) ----------- */
 PTM_FN double ptm_hypot(double x, double y) {
     double a = ptm_fabs(x), b = ptm_fabs(y);

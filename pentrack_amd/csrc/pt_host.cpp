@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <climits>
 #include <atomic>
 #include <cmath>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "../../include/pentrack_gpu.h"
+#include "pt_decay.h"
 #include "pt_device.h"
 #include "pt_formula.h"
 #include "pt_kernels.h"
@@ -738,6 +740,123 @@ extern "C" int pt_formula_eval(const char *formula, int n, const double *t, doub
     return PT_OK;
 }
 
+/* ======================================================================== */
+/* neutron decay products (TNeutron::Decay, src/neutron.cpp:285-362)          */
+/* ======================================================================== */
+
+namespace {
+
+/* ProtonBetaSpectrum (src/mc.cpp:84-92) as the reference evaluates it on x86-64: long
+   double constants (src/globals.cpp:13-21), double intermediates, pow(x, 2) as x*x
+   (what the compilers make of it) and the integer quotient 4/3 == 1 */
+double proton_beta_spectrum(double E) {
+    const long double ele_e = 1.602176487E-19L, c_0 = 299792458.L;
+    const long double m_n = 1.674927211E-27L / ele_e, m_p = 1.672621637E-27L / ele_e, m_e = 9.10938215e-31L / ele_e;
+    double DeltaM = m_n - m_p;
+    double Xi = m_e / DeltaM;
+    double Sigma = 1 - 2 * E * m_n / (DeltaM * DeltaM) / (c_0 * c_0);
+    double q = (Sigma - Xi * Xi) / Sigma;
+    double g1 = (q * q) * std::sqrt(1 - Sigma) * (4 * (1 + Xi * Xi / Sigma) - 4 / 3 * ((Sigma - Xi * Xi) / Sigma * (1 - Sigma)));
+    double g2 = (q * q) * std::sqrt(1 - Sigma) * (4 * (1 + Xi * Xi / Sigma - 2 * Sigma) - 4 / 3 * ((Sigma - Xi * Xi) / Sigma * (1 - Sigma)));
+    double littlea = -0.1017;
+    return 0.5 * (g1 + littlea * g2);
+}
+
+/* adaptive_linear_interpolation (include/mc.h:244-270): bisect until the trapezoid
+   integral changes by less than the tolerance and the step is small enough */
+void adaptive_interpolation(double next_abscissa, double next_value, std::vector<double> &abscissas,
+                            std::vector<double> &values, double tolerance, double max_diff) {
+    double h = next_abscissa - abscissas.back();
+    double I0 = h * (values.back() + next_value) / 2.;
+    double ia = (abscissas.back() + next_abscissa) / 2.;
+    double iv = proton_beta_spectrum(ia);
+    double I1 = (ia - abscissas.back()) * (values.back() + iv) / 2.;
+    double I2 = (next_abscissa - ia) * (iv + next_value) / 2.;
+    if ((h > max_diff) || ((std::fabs(I1 + I2 - I0) > tolerance) && (h > max_diff / 100.))) {
+        adaptive_interpolation(ia, iv, abscissas, values, tolerance / 2, max_diff);
+        adaptive_interpolation(next_abscissa, next_value, abscissas, values, tolerance / 2, max_diff);
+    } else {
+        abscissas.push_back(ia);
+        values.push_back(iv);
+        abscissas.push_back(next_abscissa);
+        values.push_back(next_value);
+    }
+}
+
+struct BetaTable { std::vector<double> x, raw, den, cp, m; };
+
+/* interpolate_distribution(ProtonBetaSpectrum, 0., 750.) (include/mc.h:281-296, 337) and the
+   parameter std::piecewise_linear_distribution<double> builds from it (libstdc++
+   param_type::_M_initialize: trapezoid partial sums, slopes, all divided by the total,
+   last cumulative probability set to 1) */
+const BetaTable &proton_beta_table() {
+    static const BetaTable T = [] {
+        BetaTable t;
+        const double lo = 0., hi = 750.;
+        t.x.push_back(lo);
+        t.den.push_back(proton_beta_spectrum(lo));
+        adaptive_interpolation(hi, proton_beta_spectrum(hi), t.x, t.den, std::numeric_limits<double>::epsilon() * 1000.,
+                               (hi - lo) / 10.);
+        t.raw = t.den;
+        double sum = 0.0;
+        for (size_t k = 0; k + 1 < t.x.size(); ++k) {
+            const double delta = t.x[k + 1] - t.x[k];
+            sum += 0.5 * (t.den[k + 1] + t.den[k]) * delta;
+            t.cp.push_back(sum);
+            t.m.push_back((t.den[k + 1] - t.den[k]) / delta);
+        }
+        for (auto &v : t.den) v = v / sum;
+        for (auto &v : t.cp) v = v / sum;
+        for (auto &v : t.m) v = v / sum;
+        t.cp.back() = 1.0;
+        return t;
+    }();
+    return T;
+}
+
+pt_beta_table host_view(const BetaTable &T) {
+    return pt_beta_table{T.x.data(), T.den.data(), T.cp.data(), T.m.data(), (int)T.cp.size()};
+}
+
+}  // namespace
+
+/* the sampled proton spectrum (for tests): n_int intervals */
+extern "C" int pt_proton_beta_table(int cap, double *x, double *raw, double *den, double *cp, double *m, int *n_int) {
+    const BetaTable &T = proton_beta_table();
+    if (!n_int) return fail(PT_EINVAL, "pt_proton_beta_table: bad arguments");
+    *n_int = (int)T.cp.size();
+    if (!x) return PT_OK;
+    if (cap < (int)T.x.size() || !raw || !den || !cp || !m) return fail(PT_EINVAL, "pt_proton_beta_table: buffer too small");
+    std::copy(T.x.begin(), T.x.end(), x);
+    std::copy(T.raw.begin(), T.raw.end(), raw);
+    std::copy(T.den.begin(), T.den.end(), den);
+    std::copy(T.cp.begin(), T.cp.end(), cp);
+    std::copy(T.m.begin(), T.m.end(), m);
+    return PT_OK;
+}
+
+/* the proton energies the device samples for variates p[i] in [0, 1) (for tests) */
+extern "C" int pt_beta_sample(int n, const double *p, double *out) {
+    if (n < 0 || (n > 0 && (!p || !out))) return fail(PT_EINVAL, "pt_beta_sample: bad arguments");
+    const pt_beta_table T = host_view(proton_beta_table());
+    for (int i = 0; i < n; ++i) out[i] = ptk_beta_sample(&T, p[i]);
+    return PT_OK;
+}
+
+/* the decay kinematics the device runs, on the host (for tests): stream key and
+   position, neutron velocity -> Ep phip thetap Ee phie thetae, polarisations, new position */
+extern "C" int pt_neutron_decay_host(uint64_t key, uint64_t ctr, const double *v, double *out6, int32_t *pol2,
+                                     uint64_t *ctr_out) {
+    if (!v || !out6 || !pol2) return fail(PT_EINVAL, "pt_neutron_decay_host: bad arguments");
+    const pt_beta_table T = host_view(proton_beta_table());
+    pt_decay_out o;
+    ptk_neutron_decay(&T, key, &ctr, v, &o);
+    out6[0] = o.Ep; out6[1] = o.phip; out6[2] = o.thetap; out6[3] = o.Ee; out6[4] = o.phie; out6[5] = o.thetae;
+    pol2[0] = o.polp; pol2[1] = o.pole;
+    if (ctr_out) *ctr_out = ctr;
+    return PT_OK;
+}
+
 /* exported for tests: the 64x64 integer matrix */
 extern "C" int pt_tricubic_matrix(int32_t *out) {
     const TricubicMatrix &M = tricubic_matrix();
@@ -773,6 +892,11 @@ struct pt_batch {
     const pt_geometry *geom = nullptr;
     const pt_fields *fields = nullptr;
     ptd::Species *d_species = nullptr;
+    std::vector<ptd::Species> h_species;
+    /* decay products: the proton spectrum on the device and the record buffer (lazily) */
+    pt_beta_table d_beta{};
+    pt_secondary *d_sec = nullptr;
+    int *d_sec_count = nullptr;
 };
 
 extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_species, const pt_species *species,
@@ -798,6 +922,7 @@ extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_s
     }
     int r;
     if ((r = B->mem.upload(&B->d_species, sp.data(), sp.size()))) return r;
+    B->h_species = sp;
     ptd::State &S = B->P.st;
     S.n = n;
     size_t N = (size_t)std::max(n, 1);
@@ -950,6 +1075,42 @@ extern "C" int pt_batch_active(pt_batch *b, void *stream, int *active) {
     }
     PT_HIP(hipMemcpyAsync(active, b->d_count, sizeof(int), hipMemcpyDeviceToHost, s));
     PT_HIP(hipStreamSynchronize(s));
+    return PT_OK;
+}
+
+/* TParticle::DoDecay -> TNeutron::Decay (src/particle.cpp:219-223, src/neutron.cpp:285-362) for
+   every neutron of the batch that stopped with ID_DECAYED: the proton and the electron,
+   ordered by parent index (proton first).  Idempotent: the products are recomputed from
+   the final state and the neutron's stream each call. */
+extern "C" int pt_batch_secondaries(pt_batch *b, void *stream, int cap, pt_secondary *out, int *count) {
+    if (!b || !count || cap < 0 || (cap > 0 && !out)) return fail(PT_EINVAL, "pt_batch_secondaries: bad arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    *count = 0;
+    if (b->n == 0) return PT_OK;
+    int r;
+    if (!b->d_sec) {
+        const BetaTable &T = proton_beta_table();
+        double *x, *den, *cp, *m;
+        if ((r = b->mem.upload(&x, T.x.data(), T.x.size()))) return r;
+        if ((r = b->mem.upload(&den, T.den.data(), T.den.size()))) return r;
+        if ((r = b->mem.upload(&cp, T.cp.data(), T.cp.size()))) return r;
+        if ((r = b->mem.upload(&m, T.m.data(), T.m.size()))) return r;
+        b->d_beta = pt_beta_table{x, den, cp, m, (int)T.cp.size()};
+        if ((r = b->mem.alloc(&b->d_sec, 2 * (size_t)b->n))) return r;
+        if ((r = b->mem.alloc(&b->d_sec_count, 1))) return r;
+    }
+    PT_HIP(pt_launch_decay(b->P, b->d_beta, b->d_sec, b->d_sec_count, s));
+    int n2 = 0;
+    PT_HIP(hipMemcpyAsync(&n2, b->d_sec_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    PT_HIP(hipStreamSynchronize(s));
+    std::vector<pt_secondary> rec(n2);
+    if (n2) PT_HIP(hipMemcpy(rec.data(), b->d_sec, n2 * sizeof(pt_secondary), hipMemcpyDeviceToHost));
+    std::stable_sort(rec.begin(), rec.end(), [](const pt_secondary &a, const pt_secondary &c) {
+        return a.parent != c.parent ? a.parent < c.parent : a.kind < c.kind;
+    });
+    *count = n2;
+    if (out) std::copy(rec.begin(), rec.begin() + std::min(n2, cap), out);
+    if (out && cap < n2) return fail(PT_EINVAL, "pt_batch_secondaries: capacity below the number of products");
     return PT_OK;
 }
 

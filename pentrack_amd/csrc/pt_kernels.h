@@ -7,6 +7,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/pentrack_gpu.h"
+#include "pt_decay.h"
+
 namespace ptd { struct Params; }
 
 /* mode: bit 0 FP32 field tables (PT_VARIANT_MIXED), bit 1 containers with scale formulas,
@@ -20,5 +23,7 @@ hipError_t pt_launch_step(const ptd::Params &P, int mode, int block, hipStream_t
    in Morton order of the particle positions), then `grid` blocks of 256 slots */
 hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, int spatial, hipStream_t stream);
 hipError_t pt_launch_collision(const ptd::Params &P, int mode, int coll_grid, hipStream_t stream);
+/* decay products of the batch's decayed neutrons into out (count reset first) */
+hipError_t pt_launch_decay(const ptd::Params &P, pt_beta_table T, pt_secondary *out, int *count, hipStream_t stream);
 
 #endif /* PT_KERNELS_H_ */

@@ -16,6 +16,7 @@
  * to double, so results match the FP64 CPU restatement bit for bit.
  */
 #include "../../include/pentrack_gpu.h"
+#include "pt_decay.h"
 #include "pt_device.h"
 #include "pt_formula.h"
 #include "pt_kernels.h"
@@ -1831,6 +1832,29 @@ __global__ void __launch_bounds__(256) work_kernel(Params P) {
 
 __global__ void reset_work_kernel(int *wcount) { wcount[0] = 0; wcount[1] = 0; }
 
+/* ---- decay products (TParticle::DoDecay, src/particle.cpp:219-223) ---------------- */
+__global__ void reset_count_kernel(int *count) { *count = 0; }
+
+/* TNeutron::Decay (src/neutron.cpp:285-362) for every neutron stopped with ID_DECAYED, from
+   its final state and a copy of its stream position */
+__global__ void __launch_bounds__(256) decay_kernel(Params P, pt_beta_table T, pt_secondary *out, int *count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const State &S = P.st;
+    if (i >= S.n) return;
+    if (S.stop[i] != ID_DECAYED || P.species[S.species[i]].kind != 0) return;
+    const size_t n = (size_t)S.n;
+    uint64_t ctr = S.ctr[i];
+    const double v[3] = {S.y[3 * n + i], S.y[4 * n + i], S.y[5 * n + i]};
+    pt_decay_out o;
+    ptk_neutron_decay(&T, S.key[i], &ctr, v, &o);
+    const int slot = atomicAdd(count, 2);
+    const double t = S.t[i], x = S.y[i], y = S.y[n + i], z = S.y[2 * n + i];
+    pt_secondary p{i, 1, t, {x, y, z}, o.Ep, o.phip, o.thetap, o.polp, 0};
+    pt_secondary e{i, 2, t, {x, y, z}, o.Ee, o.phie, o.thetae, o.pole, 0};
+    out[slot] = p;
+    out[slot + 1] = e;
+}
+
 /* ---- work list in spatial order (counting sort into Morton-ordered buckets) ---- */
 
 /* clears the bucket counts and the work-list counters (kernels, not memsets, so the
@@ -1977,6 +2001,13 @@ hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, i
     case 2: hipLaunchKernelGGL(ptd::step_kernel_persistent<2>, dim3(grid), dim3(256), 0, stream, P); break;
     default: hipLaunchKernelGGL(ptd::step_kernel_persistent<3>, dim3(grid), dim3(256), 0, stream, P); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_decay(const ptd::Params &P, pt_beta_table T, pt_secondary *out, int *count, hipStream_t stream) {
+    hipLaunchKernelGGL(ptd::reset_count_kernel, dim3(1), dim3(1), 0, stream, count);
+    if (P.st.n > 0)
+        hipLaunchKernelGGL(ptd::decay_kernel, dim3((P.st.n + 255) / 256), dim3(256), 0, stream, P, T, out, count);
     return hipGetLastError();
 }
 

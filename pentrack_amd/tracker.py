@@ -12,7 +12,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native
-from .particle import SpeciesConfig
+from .particle import SpeciesConfig, make_states
 
 
 @dataclass
@@ -105,12 +105,50 @@ class Batch:
             hits = hits.reshape(self.n, self.hit_cap)
         return out, hits
 
+    def secondaries(self, stream=None):
+        """Decay products of the batch's decayed neutrons (TNeutron::Decay), ordered by parent
+        index, proton first: a SEC_DTYPE record array."""
+        L = _native.lib()
+        cnt = C.c_int()
+        _native.check(L.pt_batch_secondaries(self.h, stream, 0, None, C.byref(cnt)), "pt_batch_secondaries")
+        out = np.zeros(cnt.value, dtype=_native.SEC_DTYPE)
+        if cnt.value:
+            _native.check(L.pt_batch_secondaries(self.h, stream, cnt.value, out.ctypes.data_as(C.c_void_p),
+                                                 C.byref(cnt)), "pt_batch_secondaries")
+        return out
+
     def __del__(self):
         try:
             if getattr(self, "h", None) is not None and _native._lib is not None:
                 _native._lib.pt_batch_destroy(self.h)
         except Exception:
             pass
+
+
+SECONDARY_STREAM = {1: np.uint64(1) << np.uint64(62), 2: np.uint64(2) << np.uint64(62)}
+
+
+def secondary_states(species, rec, pid):
+    """Species list, indices and initial states of decay products (TProton / TElectron
+    constructors, src/particle.cpp:36-90; velocity from E, phi, theta as there).  The
+    species' settings come from `species` when it holds a proton or an electron entry,
+    else the defaults of the PARTICLES section (in/config.in: no decay, tmax, lmax 9e99).
+    A product's random stream id is its parent's id with bit 62 (proton) or 63 (electron) set."""
+    from .constants import ELECTRON, PROTON
+    by_kind = {s.kind: s for s in species}
+    sspecies = [by_kind.get(PROTON, SpeciesConfig(PROTON)), by_kind.get(ELECTRON, SpeciesConfig(ELECTRON))]
+    n = len(rec)
+    sidx = np.where(rec["kind"] == 1, 0, 1).astype(np.int32)
+    st0 = np.asarray(rec["t"], np.float64).copy()
+    sy0 = np.zeros((n, 9))
+    for k, kind in ((1, PROTON), (2, ELECTRON)):
+        sel = rec["kind"] == k
+        if sel.any():
+            _, yk, _ = make_states(kind, rec["pos"][sel], rec["E"][sel], rec["phi"][sel], rec["theta"][sel],
+                                   rec["pol"][sel].astype(np.float64))
+            sy0[sel] = yk
+    spid = pid[rec["parent"]] | np.array([SECONDARY_STREAM[int(k)] for k in rec["kind"]], np.uint64)
+    return sspecies, sidx, st0, sy0, np.zeros((n, 5)), spid
 
 
 class TTracker:
@@ -127,6 +165,25 @@ class TTracker:
                   with_fp32=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
         b.run_to_completion(cfg)
         return b.download()
+
+    def IntegrateWithSecondaries(self, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap=0):
+        """The reference's main loop for one batch (src/main.cpp:153-172): the primaries,
+        then the protons and electrons of their neutron decays.  Returns
+        ((out, hits), (sec_out, sec_hits), records): the products' results are in the
+        order of `records` (pentrack_amd.tracker.Batch.secondaries)."""
+        cfg = self.launch
+        b = Batch(self.geometry, self.fields, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap,
+                  with_fp32=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
+        b.run_to_completion(cfg)
+        prim = b.download()
+        rec = b.secondaries()
+        if len(rec) == 0:
+            return prim, (np.zeros(0, _native.OUT_DTYPE), None), rec
+        sspecies, sidx, st0, sy0, sspin0, spid = secondary_states(species, rec, np.asarray(pid, np.uint64))
+        s = Batch(self.geometry, self.fields, sspecies, tmax, sidx, st0, sy0, sspin0, seed, spid, hit_cap,
+                  with_fp32=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
+        s.run_to_completion(cfg)
+        return prim, s.download(), rec
 
     def IntegrateParticle(self, species: SpeciesConfig, tmax, t0, y0, spin0=None, seed=0, pid=0, hit_cap=0):
         out, hits = self.IntegrateBatch([species], tmax, np.zeros(1, np.int32), np.array([t0], np.float64),

@@ -53,6 +53,21 @@ def test_trig_on_sampling_range(P):
     assert ulps(P.detmath("acos", -u), np.arccos(-u)).max() <= 2
 
 
+def test_atan_and_atan2(P):
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-2, 2, 100000), 10 ** rng.uniform(-12, 12, 50000),
+                        -(10 ** rng.uniform(-12, 12, 50000)), [0.0, -0.0, 0.5773502691896257, 1.0, -1.0, 1e308]])
+    assert ulps(P.detmath("atan", x), np.arctan(x)).max() <= 3
+    y = rng.normal(size=200000) * 10 ** rng.uniform(-5, 5, 200000)
+    z = rng.normal(size=200000) * 10 ** rng.uniform(-5, 5, 200000)
+    assert ulps(P.detmath("atan2", y, z), np.arctan2(y, z)).max() <= 3
+    # quadrants, axes and signed zeros follow C atan2
+    sy = np.array([0.0, -0.0, 0.0, -0.0, 1.0, -1.0, 0.0, -0.0, 1.0, -1.0, 2.0, -2.0])
+    sx = np.array([0.0, 0.0, -0.0, -0.0, 0.0, 0.0, 1.0, 1.0, -1.0, -1.0, -3.0, -3.0])
+    got, ref = P.detmath("atan2", sy, sx), np.arctan2(sy, sx)
+    assert np.array_equal(np.signbit(got), np.signbit(ref)) and ulps(got, ref).max() <= 1
+
+
 def test_complex_sqrt_and_hypot(P):
     rng = np.random.default_rng(2)
     re = rng.normal(size=50000) * 1e-7

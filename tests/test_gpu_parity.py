@@ -109,6 +109,39 @@ def test_analytic_fields_and_time_dependent_scales_bit_exact(gpu, oracle_lib):
     assert (g["stopID"][w.species_idx == 0] == -1).all() and g["n_acc"].sum() > 10000
 
 
+def test_neutron_decay_products_and_their_tracks_bit_exact(gpu, oracle_lib):
+    """Neutrons with a 20 ms lifetime in the analytic-field box: the decay products of the
+    GPU batch against the oracle's TNeutron::Decay at the oracle's end states, then the
+    products tracked on the GPU (TTracker.IntegrateWithSecondaries) against the oracle."""
+    from pentrack_amd.tracker import TTracker, secondary_states
+    base = workloads.analytic(n=600, seed=13, tmax=0.1)
+    species = [SpeciesConfig(NEUTRON, tau=0.02, tmax=9e99), SpeciesConfig(PROTON, tmax=3e-3),
+               SpeciesConfig(ELECTRON, tmax=1e-5)]
+    w = Workload("decay", base.geometry, base.fields, species, base.tmax, base.species_idx, base.t0, base.y0,
+                 base.spin0, base.pid, 13)
+    print("case: decay products n=600", flush=True)
+    tr = TTracker(w.geometry, w.fields, LaunchConfig())
+    (g, gh), (sg, sgh), rec = tr.IntegrateWithSecondaries(w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0,
+                                                           w.seed, w.pid, hit_cap=16)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
+    assert_bit_exact(g, gh, o, oh)
+    dec = np.where((o["stopID"] == -4) & (w.species_idx == 0))[0]
+    assert len(dec) > 50 and len(rec) == 2 * len(dec)
+    out6, pol = oracle_lib.neutron_decay(w.seed, w.pid[dec], o["ctr_end"][dec], o["y"][dec, 3:6], "fp64")
+    assert np.array_equal(rec["parent"], np.repeat(dec, 2)) and np.array_equal(rec["kind"], np.tile([1, 2], len(dec)))
+    p, e = rec[0::2], rec[1::2]
+    for r in (p, e):
+        assert np.array_equal(r["t"], o["t"][dec]) and np.array_equal(r["pos"], o["y"][dec, :3])
+    assert np.array_equal(np.stack([p["E"], p["phi"], p["theta"], e["E"], e["phi"], e["theta"]], 1), out6)
+    assert np.array_equal(np.stack([p["pol"], e["pol"]], 1), pol)
+    print("case: decay products tracked", flush=True)
+    sspecies, sidx, st0, sy0, sspin0, spid = secondary_states(w.species, rec, w.pid)
+    sw = Workload("secondaries", w.geometry, w.fields, sspecies, w.tmax, sidx, st0, sy0, sspin0, spid, w.seed)
+    so, soh = oracle_lib.track(sw, "fp64", hit_cap=16, threads=8)
+    assert_bit_exact(sg, sgh, so, soh)
+    assert (sg["n_acc"] > 0).mean() > 0.9
+
+
 def test_edge_cases(gpu, oracle_lib):
     w = workloads.cube(n=8, seed=5, tmax=1.0)
     w.t0[0] = 2.0                  # starts past tmax: the reference throws (invariant stop)
