@@ -240,6 +240,22 @@ int pt_batch_secondaries(pt_batch *b, void *stream, int cap, pt_secondary *out, 
    (n_int + 1 abscissas, spectrum values and normalised densities, n_int cumulative
    probabilities and slopes); with x == NULL only *n_int is returned */
 int pt_proton_beta_table(int cap, double *x, double *raw, double *den, double *cp, double *m, int *n_int);
+/*
+ * parse_distribution (include/mc.h:310-335): the piecewise linear distribution
+ * (std::piecewise_linear_distribution parameter) of a formula of x on [lo, hi], as
+ * the reference builds the source's energy spectrum and angular distributions;
+ * the formula may call ProtonBetaSpectrum(x).  With x == NULL only *n_int.
+ */
+int pt_distribution_table(const char *formula, double lo, double hi, int cap, double *x, double *raw,
+                          double *den, double *cp, double *m, int *n_int);
+/* TFieldManager::BField / EField (src/fields.cpp:139-182) at n points pts[n][3] and times t[n]:
+   B [n][3], dB [n][3][3], V [n], E [n][3] (host arrays; FP64 tables; synchronises the stream) */
+int pt_fields_eval(const pt_fields *f, void *stream, int n, const double *pts, const double *t, double *B,
+                   double *dB, double *V, double *E);
+/* TGeometry::GetSolid (src/geometry.cpp:151-175) at n points and times: the ID of the
+   highest-priority solid not ignored at t (1: default solid; -1: too deeply nested) */
+int pt_geometry_solid_at(const pt_geometry *g, void *stream, int n, const double *pts, const double *t,
+                         int32_t *solid_id);
 /* for tests: the proton energies sampled for the variates p[0..n-1] in [0, 1) */
 int pt_beta_sample(int n, const double *p, double *out);
 /* for tests: the decay kinematics on the host for stream (key, ctr) and neutron velocity v[3]:

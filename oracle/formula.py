@@ -10,7 +10,8 @@ gives such a formula:
     (compute_pow10, exprtk.hpp:1400-1455), then one multiplication or division
     by 10^|e| for an exponent;
   * precedence (exprtk.hpp:19927-20005): or < and < comparisons < + - < * / < ^,
-    ^ right associative; unary minus applies to the power;
+    ^ right associative; unary minus applies to the power; two adjacent signs are
+    joined as exprtk's lexer does ('+-', '-+' -> '-', '--' -> '+', exprtk.hpp:3195-3235);
   * folding of constant sub-trees with the C library (Python's math calls it);
   * x^c, c integer, |c| <= 60 (exprtk.hpp:27105-27150): fast_exp chains
     (exprtk.hpp:1463-1496), 1/x^|c| for c < 0, 1 for c = 0; (t^c0)^c1 with a
@@ -283,15 +284,28 @@ class _Parser:
             else:
                 return a
 
+    def sign(self):
+        """A + or - operator with exprtk's joining of two adjacent ones ('+-', '-+' -> '-',
+        '--' -> '+'): +1, -1 or 0."""
+        self.ws()
+        if self.i >= len(self.s) or self.s[self.i] not in "+-":
+            return 0
+        c = self.s[self.i]
+        self.i += 1
+        self.ws()
+        if self.i < len(self.s) and self.s[self.i] in "+-":
+            d = self.s[self.i]
+            self.i += 1
+            return 1 if c == d else -1
+        return 1 if c == "+" else -1
+
     def add(self):
         a = self.mul()
         while True:
-            if self.take("+"):
-                a = self.bin(ADD, a, self.mul())
-            elif self.take("-"):
-                a = self.bin(SUB, a, self.mul())
-            else:
+            sg = self.sign()
+            if sg == 0:
                 return a
+            a = self.bin(ADD if sg > 0 else SUB, a, self.mul())
 
     def mul(self):
         a = self.unary()
@@ -304,9 +318,10 @@ class _Parser:
                 return a
 
     def unary(self):
-        if self.take("-"):
+        sg = self.sign()
+        if sg < 0:
             return self.un(NEG, self.unary())
-        if self.take("+"):
+        if sg > 0:
             return self.unary()
         a = self.primary()
         if self.take("^"):

@@ -129,6 +129,7 @@ EXPORTS = [
     "pt_request_quit", "pt_last_error", "pt_clearance_grid", "pt_tricubic_matrix",
     "pt_spline_grid_diff", "pt_rk4_forward_ckpt", "pt_rk4_adjoint", "pt_formula_eval",
     "pt_batch_secondaries", "pt_proton_beta_table", "pt_neutron_decay_host", "pt_beta_sample",
+    "pt_distribution_table", "pt_fields_eval", "pt_geometry_solid_at",
 ]
 
 _lib = None
@@ -183,6 +184,10 @@ def lib():
     L.pt_batch_secondaries.argtypes = [vp, vp, C.c_int, vp, C.POINTER(C.c_int)]
     L.pt_proton_beta_table.argtypes = [C.c_int, dp, dp, dp, dp, dp, C.POINTER(C.c_int)]
     L.pt_beta_sample.argtypes = [C.c_int, dp, dp]
+    L.pt_distribution_table.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp, dp,
+                                         C.POINTER(C.c_int)]
+    L.pt_fields_eval.argtypes = [vp, vp, C.c_int, dp, dp, dp, dp, dp, dp]
+    L.pt_geometry_solid_at.argtypes = [vp, vp, C.c_int, dp, dp, C.POINTER(C.c_int32)]
     L.pt_neutron_decay_host.argtypes = [C.c_uint64, C.c_uint64, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
     L.pt_request_quit.restype = None
     L.pt_last_error.restype = C.c_char_p

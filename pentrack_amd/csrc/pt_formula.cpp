@@ -22,11 +22,12 @@
  *     on for the 32 forms exprtk re-associates;
  *   - == and != compare exactly; and/or/not/?: treat any non-zero value (NaN
  *     included) as true.
- * exprtk re-associates further chains of + - or * / (with a constant, with a
- * division, or around a sub-expression); a formula containing such a chain is
- * rejected with an error naming it instead of being evaluated differently.
+ * exprtk re-associates further chains of + - or * / (an inner operation with a
+ * constant operand, or a division); a formula containing such a chain is rejected
+ * with an error naming it instead of being evaluated differently.
  * Accepted functions: abs sqrt exp log sin cos min max if not; constants pi,
- * epsilon, inf; the variable t.  Names are case-insensitive.
+ * epsilon, inf; the variable t (field scales) or x (distributions, which may also
+ * call ProtonBetaSpectrum(x) on the host).  Names are case-insensitive.
  */
 #include <cctype>
 #include <cmath>
@@ -75,7 +76,7 @@ double compute_pow10(double d, int exponent) {
 
 class Compiler {
 public:
-    explicit Compiler(const char *s) : s_(s) {}
+    Compiler(const char *s, const char *var, bool host_functions) : s_(s), var_(var), host_(host_functions) {}
 
     bool compile(std::vector<uint32_t> &code, std::vector<double> &cst, std::string &err) {
         int root = parse_cond();
@@ -91,6 +92,8 @@ public:
 
 private:
     const char *s_;
+    std::string var_;        /* the variable's name: t (field scales) or x (distributions) */
+    bool host_;              /* allow functions evaluated on the host only */
     size_t i_ = 0;
     bool ok_ = true;
     std::string err_;
@@ -139,6 +142,7 @@ private:
         case PF_LOG: return std::log(a);
         case PF_SIN: return std::sin(a);
         case PF_COS: return std::cos(a);
+        case PF_PBETA: return ptf_proton_beta_spectrum(a);
         }
         return a;
     }
@@ -292,9 +296,10 @@ private:
             int ch = side ? x.b : x.a, other = side ? x.a : x.b;
             const Node &c = n_[ch];
             if (c.kind != N_BIN || !same_class(x.op, c.op)) continue;
+            (void)other;
             bool has_div = x.op == PF_DIV || c.op == PF_DIV;
-            bool has_const = is_const(other) || is_const(c.a) || is_const(c.b);
-            if (has_div || has_const) {
+            bool child_const = is_const(c.a) || is_const(c.b);
+            if (has_div || child_const) {
                 ok_ = false;
                 err_ = std::string("formula '") + s_ +
                        "': a chain of " + (addsub(x.op) ? "+/-" : "*//") +
@@ -355,12 +360,30 @@ private:
         }
         return a;
     }
+    /* a + or - operator; exprtk's lexer joins two adjacent ones ('+ -' and '- +' to
+       '-', '- -' to '+'): returns +1, -1, or 0 when no sign follows */
+    int take_sign() {
+        skip();
+        char c = s_[i_];
+        if (c != '+' && c != '-') return 0;
+        ++i_;
+        skip();
+        char d = s_[i_];
+        if (d == '+' || d == '-') {
+            ++i_;
+            skip();
+            if (s_[i_] == '+' || s_[i_] == '-') { fail_at("three adjacent signs"); return 0; }
+            if (c == '+' && d == '+') return 2;          /* '+' then a unary '+' */
+            return (c == d) ? 1 : -1;
+        }
+        return c == '+' ? 1 : -1;
+    }
     int parse_add() {
         int a = parse_mul();
         while (ok_) {
-            if (accept("+")) a = mk_bin(PF_ADD, a, parse_mul());
-            else if (accept("-")) a = mk_bin(PF_SUB, a, parse_mul());
-            else break;
+            int sg = take_sign();
+            if (sg == 0) break;
+            a = mk_bin(sg > 0 ? PF_ADD : PF_SUB, a, parse_mul());
         }
         return a;
     }
@@ -375,8 +398,9 @@ private:
         return a;
     }
     int parse_unary() {
-        if (accept("-")) return mk_un(PF_NEG, parse_unary());
-        if (accept("+")) return parse_unary();
+        int sg = take_sign();
+        if (sg < 0) return mk_un(PF_NEG, parse_unary());
+        if (sg > 0) return parse_unary();
         return parse_pow();
     }
     int parse_pow() {
@@ -443,7 +467,12 @@ private:
             while (isalnum((unsigned char)s_[i_]) || s_[i_] == '_') ++i_;
             std::string w(s_ + st, i_ - st);
             for (auto &ch : w) ch = (char)tolower((unsigned char)ch);
-            if (w == "t") return mk_var();
+            if (w == var_) return mk_var();
+            if (host_ && w == "protonbetaspectrum") {
+                int a[1];
+                if (!parse_args(1, a)) return mk_const(0);
+                return mk_un(PF_PBETA, a[0]);
+            }
             if (w == "pi") return mk_const(3.141592653589793238462643383279502);
             if (w == "epsilon") return mk_const(0.0000000001);
             if (w == "inf") return mk_const(std::numeric_limits<double>::infinity());
@@ -507,12 +536,20 @@ private:
 }  // namespace
 
 namespace ptf {
-/* used by pt_fields_create; returns false and sets err on a rejected formula */
+/* used by pt_fields_create (variable t); returns false and sets err on a rejected formula */
 bool compile(const char *formula, std::vector<uint32_t> &code, std::vector<double> &cst, std::string &err) {
     code.clear();
     cst.clear();
     if (!formula) { err = "null formula"; return false; }
-    Compiler c(formula);
+    Compiler c(formula, "t", false);
+    return c.compile(code, cst, err);
+}
+/* distributions of the source and the spectra (variable x, host functions allowed) */
+bool compile_x(const char *formula, std::vector<uint32_t> &code, std::vector<double> &cst, std::string &err) {
+    code.clear();
+    cst.clear();
+    if (!formula) { err = "null formula"; return false; }
+    Compiler c(formula, "x", true);
     return c.compile(code, cst, err);
 }
 }  // namespace ptf

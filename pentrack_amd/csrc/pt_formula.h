@@ -38,8 +38,12 @@ enum {
     PF_SEL,         /* c a b -> c != 0 ? a : b */
     PF_ABS, PF_SQRT, PF_EXP, PF_LOG, PF_SIN, PF_COS,
     PF_MIN, PF_MAX,
+    PF_PBETA,       /* ProtonBetaSpectrum(x) (src/mc.cpp:84-92), host programs only */
     PF_NOPS
 };
+
+/* the proton recoil spectrum of free-neutron decay (a host function, pt_host.cpp) */
+double ptf_proton_beta_spectrum(double E);
 
 #define PT_FORMULA_MAX_STACK 16
 #define PT_FORMULA_MAX_CODE 256
@@ -114,6 +118,11 @@ PTF_FN double ptf_run(const uint32_t *code, int n, const double *cst, double t) 
         case PF_SIN: s[sp - 1] = ptm_sin(s[sp - 1]); break;
         case PF_COS: s[sp - 1] = ptm_cos(s[sp - 1]); break;
         case PF_IPOW: s[sp - 1] = ptf_fast_exp(s[sp - 1], (int)arg); break;
+#if defined(__HIP_DEVICE_COMPILE__)
+        case PF_PBETA: s[sp - 1] = __builtin_nan(""); break;
+#else
+        case PF_PBETA: s[sp - 1] = ptf_proton_beta_spectrum(s[sp - 1]); break;
+#endif
         case PF_IPOWINV: s[sp - 1] = 1.0 / ptf_fast_exp(s[sp - 1], (int)arg); break;
         case PF_SEL: {
             double c = s[sp - 3];

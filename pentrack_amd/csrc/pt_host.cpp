@@ -25,6 +25,7 @@
 
 namespace ptf {
 bool compile(const char *formula, std::vector<uint32_t> &code, std::vector<double> &cst, std::string &err);
+bool compile_x(const char *formula, std::vector<uint32_t> &code, std::vector<double> &cst, std::string &err);
 }
 
 namespace {
@@ -764,17 +765,18 @@ double proton_beta_spectrum(double E) {
 
 /* adaptive_linear_interpolation (include/mc.h:244-270): bisect until the trapezoid
    integral changes by less than the tolerance and the step is small enough */
-void adaptive_interpolation(double next_abscissa, double next_value, std::vector<double> &abscissas,
+template <class F>
+void adaptive_interpolation(const F &f, double next_abscissa, double next_value, std::vector<double> &abscissas,
                             std::vector<double> &values, double tolerance, double max_diff) {
     double h = next_abscissa - abscissas.back();
     double I0 = h * (values.back() + next_value) / 2.;
     double ia = (abscissas.back() + next_abscissa) / 2.;
-    double iv = proton_beta_spectrum(ia);
+    double iv = f(ia);
     double I1 = (ia - abscissas.back()) * (values.back() + iv) / 2.;
     double I2 = (next_abscissa - ia) * (iv + next_value) / 2.;
     if ((h > max_diff) || ((std::fabs(I1 + I2 - I0) > tolerance) && (h > max_diff / 100.))) {
-        adaptive_interpolation(ia, iv, abscissas, values, tolerance / 2, max_diff);
-        adaptive_interpolation(next_abscissa, next_value, abscissas, values, tolerance / 2, max_diff);
+        adaptive_interpolation(f, ia, iv, abscissas, values, tolerance / 2, max_diff);
+        adaptive_interpolation(f, next_abscissa, next_value, abscissas, values, tolerance / 2, max_diff);
     } else {
         abscissas.push_back(ia);
         values.push_back(iv);
@@ -785,30 +787,44 @@ void adaptive_interpolation(double next_abscissa, double next_value, std::vector
 
 struct BetaTable { std::vector<double> x, raw, den, cp, m; };
 
-/* interpolate_distribution(ProtonBetaSpectrum, 0., 750.) (include/mc.h:281-296, 337) and the
-   parameter std::piecewise_linear_distribution<double> builds from it (libstdc++
+/* interpolate_distribution(f, lo, hi) (include/mc.h:281-296) and the parameter
+   std::piecewise_linear_distribution<double> builds from it (libstdc++
    param_type::_M_initialize: trapezoid partial sums, slopes, all divided by the total,
    last cumulative probability set to 1) */
+template <class F>
+int distribution_table(const F &f, double lo, double hi, BetaTable &t) {
+    if (lo > hi) return fail(PT_EINVAL, "Invalid range used to parse distribution");
+    t.x.assign(1, lo);
+    t.den.assign(1, f(lo));
+    if (lo == hi) {
+        t.x.push_back(std::nextafter(hi, std::numeric_limits<double>::max()));
+        t.den.push_back(f(t.x.back()));
+    } else {
+        adaptive_interpolation(f, hi, f(hi), t.x, t.den, std::numeric_limits<double>::epsilon() * 1000., (hi - lo) / 10.);
+    }
+    t.raw = t.den;
+    double sum = 0.0;
+    t.cp.clear();
+    t.m.clear();
+    for (size_t k = 0; k + 1 < t.x.size(); ++k) {
+        const double delta = t.x[k + 1] - t.x[k];
+        sum += 0.5 * (t.den[k + 1] + t.den[k]) * delta;
+        t.cp.push_back(sum);
+        t.m.push_back((t.den[k + 1] - t.den[k]) / delta);
+    }
+    if (!(sum > 0)) return fail(PT_EINVAL, "distribution with zero or negative integral");
+    for (auto &v : t.den) v = v / sum;
+    for (auto &v : t.cp) v = v / sum;
+    for (auto &v : t.m) v = v / sum;
+    t.cp.back() = 1.0;
+    return PT_OK;
+}
+
+/* interpolate_distribution(ProtonBetaSpectrum, 0., 750.) (include/mc.h:337) */
 const BetaTable &proton_beta_table() {
     static const BetaTable T = [] {
         BetaTable t;
-        const double lo = 0., hi = 750.;
-        t.x.push_back(lo);
-        t.den.push_back(proton_beta_spectrum(lo));
-        adaptive_interpolation(hi, proton_beta_spectrum(hi), t.x, t.den, std::numeric_limits<double>::epsilon() * 1000.,
-                               (hi - lo) / 10.);
-        t.raw = t.den;
-        double sum = 0.0;
-        for (size_t k = 0; k + 1 < t.x.size(); ++k) {
-            const double delta = t.x[k + 1] - t.x[k];
-            sum += 0.5 * (t.den[k + 1] + t.den[k]) * delta;
-            t.cp.push_back(sum);
-            t.m.push_back((t.den[k + 1] - t.den[k]) / delta);
-        }
-        for (auto &v : t.den) v = v / sum;
-        for (auto &v : t.cp) v = v / sum;
-        for (auto &v : t.m) v = v / sum;
-        t.cp.back() = 1.0;
+        distribution_table(proton_beta_spectrum, 0., 750., t);
         return t;
     }();
     return T;
@@ -819,6 +835,8 @@ pt_beta_table host_view(const BetaTable &T) {
 }
 
 }  // namespace
+
+double ptf_proton_beta_spectrum(double E) { return proton_beta_spectrum(E); }
 
 /* the sampled proton spectrum (for tests): n_int intervals */
 extern "C" int pt_proton_beta_table(int cap, double *x, double *raw, double *den, double *cp, double *m, int *n_int) {
@@ -832,6 +850,30 @@ extern "C" int pt_proton_beta_table(int cap, double *x, double *raw, double *den
     std::copy(T.den.begin(), T.den.end(), den);
     std::copy(T.cp.begin(), T.cp.end(), cp);
     std::copy(T.m.begin(), T.m.end(), m);
+    return PT_OK;
+}
+
+/* parse_distribution (include/mc.h:310-335): the piecewise linear distribution of a
+   formula of x on [lo, hi] (the source's spectra and angular distributions) */
+extern "C" int pt_distribution_table(const char *formula, double lo, double hi, int cap, double *x, double *raw,
+                                     double *den, double *cp, double *m, int *n_int) {
+    if (!formula || !n_int) return fail(PT_EINVAL, "pt_distribution_table: bad arguments");
+    std::vector<uint32_t> code;
+    std::vector<double> cst;
+    std::string err;
+    if (!ptf::compile_x(formula, code, cst, err)) return fail(PT_EINVAL, "pt_distribution_table: " + err);
+    auto f = [&](double v) { return ptf_run(code.data(), (int)code.size(), cst.data(), v); };
+    BetaTable t;
+    int r = distribution_table(f, lo, hi, t);
+    if (r) return r;
+    *n_int = (int)t.cp.size();
+    if (!x) return PT_OK;
+    if (cap < (int)t.x.size() || !raw || !den || !cp || !m) return fail(PT_EINVAL, "pt_distribution_table: buffer too small");
+    std::copy(t.x.begin(), t.x.end(), x);
+    std::copy(t.raw.begin(), t.raw.end(), raw);
+    std::copy(t.den.begin(), t.den.end(), den);
+    std::copy(t.cp.begin(), t.cp.end(), cp);
+    std::copy(t.m.begin(), t.m.end(), m);
     return PT_OK;
 }
 
@@ -1111,6 +1153,49 @@ extern "C" int pt_batch_secondaries(pt_batch *b, void *stream, int cap, pt_secon
     *count = n2;
     if (out) std::copy(rec.begin(), rec.begin() + std::min(n2, cap), out);
     if (out && cap < n2) return fail(PT_EINVAL, "pt_batch_secondaries: capacity below the number of products");
+    return PT_OK;
+}
+
+/* TFieldManager::BField / EField at n points and times (host arrays), FP64 tables */
+extern "C" int pt_fields_eval(const pt_fields *f, void *stream, int n, const double *pts, const double *t, double *B,
+                              double *dB, double *V, double *E) {
+    if (!f || n < 0 || (n > 0 && (!pts || !t || !B || !dB || !V || !E))) return fail(PT_EINVAL, "pt_fields_eval: bad arguments");
+    if (n == 0) return PT_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevMem mem;
+    double *dp, *dt, *dB_, *ddB, *dV, *dE;
+    int r;
+    if ((r = mem.upload(&dp, pts, 3 * (size_t)n)) || (r = mem.upload(&dt, t, (size_t)n)) || (r = mem.alloc(&dB_, 3 * (size_t)n)) ||
+        (r = mem.alloc(&ddB, 9 * (size_t)n)) || (r = mem.alloc(&dV, (size_t)n)) || (r = mem.alloc(&dE, 3 * (size_t)n)))
+        return r;
+    ptd::Params P{};
+    P.n_fields = f->n;
+    P.fields = f->dev;
+    PT_HIP(pt_launch_fields_eval(P, f->general ? 1 : 0, n, dp, dt, dB_, ddB, dV, dE, s));
+    PT_HIP(hipStreamSynchronize(s));
+    PT_HIP(hipMemcpy(B, dB_, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(dB, ddB, 9 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(V, dV, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(E, dE, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+/* TGeometry::GetSolid (src/geometry.cpp:151-175) at n points and times: solid IDs (1 = default);
+   -1 where more solids nest than the tracker supports */
+extern "C" int pt_geometry_solid_at(const pt_geometry *g, void *stream, int n, const double *pts, const double *t,
+                                    int32_t *solid_id) {
+    if (!g || n < 0 || (n > 0 && (!pts || !t || !solid_id))) return fail(PT_EINVAL, "pt_geometry_solid_at: bad arguments");
+    if (n == 0) return PT_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevMem mem;
+    double *dp, *dt;
+    int *dout;
+    int r;
+    if ((r = mem.upload(&dp, pts, 3 * (size_t)n)) || (r = mem.upload(&dt, t, (size_t)n)) || (r = mem.alloc(&dout, (size_t)n)))
+        return r;
+    PT_HIP(pt_launch_solid_at(g->g, n, dp, dt, dout, s));
+    PT_HIP(hipStreamSynchronize(s));
+    PT_HIP(hipMemcpy(solid_id, dout, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
     return PT_OK;
 }
 

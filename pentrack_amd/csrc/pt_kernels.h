@@ -10,7 +10,7 @@
 #include "../../include/pentrack_gpu.h"
 #include "pt_decay.h"
 
-namespace ptd { struct Params; }
+namespace ptd { struct Params; struct Geom; }
 
 /* mode: bit 0 FP32 field tables (PT_VARIANT_MIXED), bit 1 containers with scale formulas,
    analytic fields or conductors present */
@@ -23,6 +23,11 @@ hipError_t pt_launch_step(const ptd::Params &P, int mode, int block, hipStream_t
    in Morton order of the particle positions), then `grid` blocks of 256 slots */
 hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, int spatial, hipStream_t stream);
 hipError_t pt_launch_collision(const ptd::Params &P, int mode, int coll_grid, hipStream_t stream);
+/* field sum (FP64 tables) at n points and times; general: containers with formulas or analytic fields */
+hipError_t pt_launch_fields_eval(const ptd::Params &P, int general, int n, const double *pts, const double *t, double *B,
+                                 double *dB, double *V, double *E, hipStream_t stream);
+/* ID of the solid with the highest priority at each point and time */
+hipError_t pt_launch_solid_at(const ptd::Geom &G, int n, const double *pts, const double *t, int *out, hipStream_t stream);
 /* decay products of the batch's decayed neutrons into out (count reset first) */
 hipError_t pt_launch_decay(const ptd::Params &P, pt_beta_table T, pt_secondary *out, int *count, hipStream_t stream);
 

@@ -1855,6 +1855,45 @@ __global__ void __launch_bounds__(256) decay_kernel(Params P, pt_beta_table T, p
     out[slot + 1] = e;
 }
 
+/* ---- field and solid queries (TFieldManager::BField/EField, TGeometry::GetSolid) ---- */
+
+/* field sum at n points and times, FP64 tables; dB row-major [n][3][3] */
+template <int MODE>
+__global__ void __launch_bounds__(256) fields_eval_kernel(Params P, int n, const double *pts, const double *t, double *B,
+                                                          double *dB, double *V, double *E) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long nf = 0;
+    double b[3], db[3][3], v, e[3];
+    const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    field_B<MODE>(P, x, y, z, t[i], b, db, true, nf);
+    field_E<MODE>(P, x, y, z, t[i], &v, e, nf);
+    for (int k = 0; k < 3; ++k) {
+        B[3 * i + k] = b[k];
+        E[3 * i + k] = e[k];
+        for (int j = 0; j < 3; ++j) dB[9 * i + 3 * k + j] = db[k][j];
+    }
+    V[i] = v;
+}
+
+/* TGeometry::GetSolid (src/geometry.cpp:151-175): the solids containing the point (+z ray
+   parity), the highest ID among those not ignored at t; -1 if more than MAX_SOLIDS nest */
+__global__ void __launch_bounds__(256) solid_at_kernel(Geom G, int n, const double *pts, const double *t, int *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long sk = sk_set(0ull, 0, -1, 0);
+    int ns = 1;
+    unsigned long long par = ray_parity(G, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+    for (int m = 0; m < G.n_solid; ++m) {
+        if ((par >> m) & 1ull) {
+            if (ns >= MAX_SOLIDS) { out[i] = -1; return; }
+            sk = sk_set(sk, ns, m, solid_ignored(G, m, t[i]));
+            ++ns;
+        }
+    }
+    out[i] = (int)solid_ID(G, current_solid(G, sk, ns));
+}
+
 /* ---- work list in spatial order (counting sort into Morton-ordered buckets) ---- */
 
 /* clears the bucket counts and the work-list counters (kernels, not memsets, so the
@@ -2001,6 +2040,21 @@ hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, i
     case 2: hipLaunchKernelGGL(ptd::step_kernel_persistent<2>, dim3(grid), dim3(256), 0, stream, P); break;
     default: hipLaunchKernelGGL(ptd::step_kernel_persistent<3>, dim3(grid), dim3(256), 0, stream, P); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_fields_eval(const ptd::Params &P, int general, int n, const double *pts, const double *t, double *B,
+                                 double *dB, double *V, double *E, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    dim3 g((n + 255) / 256), b(256);
+    if (general) hipLaunchKernelGGL(ptd::fields_eval_kernel<2>, g, b, 0, stream, P, n, pts, t, B, dB, V, E);
+    else hipLaunchKernelGGL(ptd::fields_eval_kernel<0>, g, b, 0, stream, P, n, pts, t, B, dB, V, E);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_solid_at(const ptd::Geom &G, int n, const double *pts, const double *t, int *out, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ptd::solid_at_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, G, n, pts, t, out);
     return hipGetLastError();
 }
 
