@@ -469,6 +469,10 @@ extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32
             std::string err;
             if (!ptf::compile(forms[w], code, cst, err))
                 return fail(PT_EINVAL, "pt_fields_create: field " + std::to_string(k) + ": " + err);
+            if (code.size() == 1 && (code[0] & 0xffu) == PF_CONST) {   /* a constant formula */
+                (w == 0 ? f.bscale : f.escale) = cst[0];
+                continue;
+            }
             uint32_t *dcode;
             double *dcst;
             int r = F->mem.upload(&dcode, code.data(), code.size());

@@ -1,0 +1,99 @@
+"""Text end log (TTextLogger, src/logger.cpp:42-134, 314-334).
+
+One file per species and log type, named `%012d<particle><suffix>.out` after the job
+number, a header line of the variable names, then one line per particle with the
+values printed at std::numeric_limits<double>::digits10 (15) significant digits, each
+followed by a space.  The variables are those of TLogger::Print; the start and end
+fields, potentials and solids are evaluated on the GPU (pt_fields_eval,
+pt_geometry_solid_at) at the particles' start and end states.
+"""
+import os
+
+import numpy as np
+
+from .constants import ELE_E, GRAVCONST, SPECIES_CONSTANTS, NEUTRON
+from .particle import kinetic_energy
+
+DEFAULT_ENDLOGVARS = ["jobnumber", "particle", "tstart", "xstart", "ystart", "zstart", "vxstart", "vystart",
+                      "vzstart", "polstart", "Sxstart", "Systart", "Szstart", "Hstart", "Estart", "Bstart", "Ustart",
+                      "solidstart", "tend", "xend", "yend", "zend", "vxend", "vyend", "vzend", "polend", "Sxend",
+                      "Syend", "Szend", "Hend", "Eend", "Bend", "Uend", "solidend", "stopID", "Nspinflip",
+                      "spinflipprob", "Nhit", "Nstep", "propert", "trajlength", "Hmax", "wL"]
+
+
+def _fmt(v):
+    """ostream << double with setprecision(15) (the %g style, 15 significant digits)."""
+    s = "%.15g" % v
+    if s in ("inf", "-inf", "nan", "-nan"):
+        return s
+    return s
+
+
+def potential_energy(kind, y, B, V, mats, solid_mat):
+    """TParticle::GetPotentialEnergy (src/particle.cpp:254-269), plus the material potential
+    of neutrons (TNeutron::GetPotentialEnergy, src/neutron.cpp:357-359, include/neutron.h:53-55)."""
+    q, m, mu, _ = SPECIES_CONSTANTS[kind]
+    res = np.zeros(len(y))
+    if mu != 0:
+        res += -y[:, 7] * mu / ELE_E * np.sqrt((B ** 2).sum(1))
+    if q != 0:
+        res += q / ELE_E * V
+    res += m * GRAVCONST * y[:, 2]
+    if kind == NEUTRON:
+        mat = mats[solid_mat]
+        res += mat[:, 0] * 1e-9 - mat[:, 6] * mu * y[:, 7] / ELE_E
+    return res
+
+
+def end_values(kind, t0, y0, spin0, out, geometry, fields, jobnumber=0, first_number=1):
+    """{variable: [n] values} of TLogger::Print for one species."""
+    from .queries import fields_eval, solid_at
+    n = len(out)
+    yend = out["y"]
+    tend = out["t"]
+    Bs, _, Vs, _ = fields_eval(fields, y0[:, :3], t0)
+    Be, _, Ve, _ = fields_eval(fields, yend[:, :3], tend)
+    sid_s = solid_at(geometry, y0[:, :3], t0)
+    sid_e = solid_at(geometry, yend[:, :3], tend)
+    mats = np.stack([m.as_array() for m in geometry.materials.values()])
+    names = list(geometry.materials)
+    id_to_mat = {1: names.index(geometry.default_material)}
+    for s in geometry.solids:
+        id_to_mat[s.ID] = names.index(s.material)
+    ms = np.array([id_to_mat.get(int(i), id_to_mat[1]) for i in sid_s])
+    me = np.array([id_to_mat.get(int(i), id_to_mat[1]) for i in sid_e])
+    Es = kinetic_energy(kind, y0[:, 3:6])
+    Ee = kinetic_energy(kind, yend[:, 3:6])
+    Hs = Es + potential_energy(kind, y0, Bs, Vs, mats, ms)
+    He = Ee + potential_energy(kind, yend, Be, Ve, mats, me)
+    spin = out["spin"]
+    wL = np.where(spin[:, 3] > 0, spin[:, 4] / np.where(spin[:, 3] > 0, spin[:, 3], 1.0), 0.0)
+    v = {
+        "jobnumber": np.full(n, float(jobnumber)), "particle": np.arange(first_number, first_number + n, dtype=float),
+        "tstart": t0, "xstart": y0[:, 0], "ystart": y0[:, 1], "zstart": y0[:, 2], "vxstart": y0[:, 3],
+        "vystart": y0[:, 4], "vzstart": y0[:, 5], "polstart": y0[:, 7], "Sxstart": spin0[:, 0],
+        "Systart": spin0[:, 1], "Szstart": spin0[:, 2], "Hstart": Hs, "Estart": Es,
+        "Bstart": np.sqrt((Bs ** 2).sum(1)), "Ustart": Vs, "solidstart": sid_s.astype(float),
+        "tend": tend, "xend": yend[:, 0], "yend": yend[:, 1], "zend": yend[:, 2], "vxend": yend[:, 3],
+        "vyend": yend[:, 4], "vzend": yend[:, 5], "polend": yend[:, 7], "Sxend": spin[:, 0], "Syend": spin[:, 1],
+        "Szend": spin[:, 2], "Hend": He, "Eend": Ee, "Bend": np.sqrt((Be ** 2).sum(1)), "Uend": Ve,
+        "solidend": out["solidend"].astype(float), "stopID": out["stopID"].astype(float),
+        "Nspinflip": out["Nspinflip"].astype(float), "spinflipprob": 1 - out["noflipprob"],
+        "Nhit": out["Nhit"].astype(float), "Nstep": out["Nstep"].astype(float), "propert": yend[:, 6],
+        "trajlength": yend[:, 8], "Hmax": out["Hmax"], "wL": wL,
+    }
+    return v
+
+
+def write_log(outdir, jobnumber, particle, suffix, titles, values):
+    """Append lines to `<outdir>/%012d<particle><suffix>.out`, writing the header first."""
+    os.makedirs(outdir, exist_ok=True)
+    path = os.path.join(outdir, "%012d%s%s.out" % (jobnumber, particle, suffix))
+    new = not os.path.exists(path)
+    cols = [np.asarray(values[t], dtype=np.float64) for t in titles]
+    with open(path, "a") as f:
+        if new:
+            f.write(" ".join(titles) + " \n")
+        for row in zip(*cols):
+            f.write(" ".join(_fmt(v) for v in row) + " \n")
+    return path

@@ -1,0 +1,68 @@
+"""GPU: field and solid queries through the C ABI against the oracle, and the
+PENTrack-style command line (config.in -> source -> tracking -> end logs)."""
+import os
+
+import numpy as np
+import pytest
+
+from pentrack_amd import queries, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def test_field_and_solid_queries_match_oracle(gpu, oracle_lib):
+    w = workloads.analytic(n=10, seed=3)
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(-0.2, 1.2, (4000, 3))
+    t = rng.uniform(0, 0.1, 4000)
+    print("case: field queries n=4000", flush=True)
+    B, dB, V, E = queries.fields_eval(w.fields, pts, t)
+    oB, odB, oV, oE = oracle_lib.field_eval(w.fields, pts, t, "fp64")
+    for a, b in ((B, oB), (dB, odB), (V, oV), (E, oE)):
+        assert np.array_equal(a, b)
+    print("case: solid queries n=4000", flush=True)
+    sid = queries.solid_at(w.geometry, pts, t)
+    P = oracle_lib.Problem(w.geometry, w.fields)
+    for i in range(0, 4000, 7):
+        ids, ign = oracle_lib.get_solids(P, t[i], pts[i])
+        live = [int(s) for s, g in zip(ids, ign) if not g]
+        assert sid[i] == (max(live) if live else 1), i
+
+
+def read_log(path):
+    lines = open(path).read().splitlines()
+    titles = lines[0].split()
+    vals = np.array([[float(x) for x in l.split()] for l in lines[1:]])
+    return titles, vals
+
+
+def test_command_line_run(gpu, oracle_lib, tmp_path):
+    from tests.test_config import write_config
+    from pentrack_amd.__main__ import main
+    from pentrack_amd import config as C
+    from pentrack_amd.source import VolumeSource
+    from pentrack_amd.workloads import Workload
+    cfgpath = write_config(str(tmp_path))
+    out = str(tmp_path / "out")
+    print("case: command line simcount 200", flush=True)
+    assert main(["3", cfgpath, out, "77"]) == 0
+    titles, nlog = read_log(os.path.join(out, "000000000003neutronend.out"))
+    assert len(nlog) == 200 and titles[:3] == ["jobnumber", "particle", "tstart"]
+    # the same particles through the oracle
+    cfg = C.TConfig(cfgpath)
+    geometry, fields = C.load_geometry(cfg, str(tmp_path)), C.load_fields(cfg)
+    t0, y0, spin0 = VolumeSource(cfg).sample(200, np.random.default_rng(77))
+    species = [C.species_config(cfg, n) for n in ("neutron", "proton", "electron")]
+    w = Workload("cli", geometry, fields, species, 0.5, np.zeros(200, np.int32), t0, y0, spin0,
+                 np.arange(1, 201, dtype=np.uint64), 77)
+    o, _ = oracle_lib.track(w, "fp64", threads=8)
+    col = {t: k for k, t in enumerate(titles)}
+    assert np.array_equal(nlog[:, col["stopID"]], o["stopID"])
+    assert np.array_equal(nlog[:, col["Nhit"]], o["Nhit"])
+    assert np.allclose(nlog[:, col["xend"]], o["y"][:, 0], rtol=1e-14, atol=1e-300)
+    assert np.allclose(nlog[:, col["Hmax"]], o["Hmax"], rtol=1e-14)
+    dec = int((o["stopID"] == -4).sum())
+    if dec:
+        _, plog = read_log(os.path.join(out, "000000000003protonend.out"))
+        _, elog = read_log(os.path.join(out, "000000000003electronend.out"))
+        assert len(plog) == dec and len(elog) == dec
