@@ -33,7 +33,7 @@ def main(argv=None):
     print("Random Seed:", seed)
     base = os.path.dirname(configpath)
     geometry = cfgmod.load_geometry(cfg, base)
-    fields = cfgmod.load_fields(cfg)
+    fields = cfgmod.load_fields(cfg, base)
     source = VolumeSource(cfg)
     species = [cfgmod.species_config(cfg, name) for name in ("neutron", "proton", "electron")]
     rng = np.random.default_rng(seed)

@@ -8,7 +8,8 @@ in which the reference loads solids (GEOMETRY) and sums fields (FIELDS).  From i
   * materials (operator>>(istream, material), src/geometry.cpp:10-52) and solids with
     their STL meshes and ignore times (src/geometry.cpp:60-137);
   * field containers of the FIELDS section (TFieldManager, src/fields.cpp:19-136) with
-    scale formulas resolved through FORMULAS (ResolveFormula);
+    scale formulas resolved through FORMULAS (ResolveFormula), table files (OPERA3D,
+    3Dtable, COMSOL) read by pentrack_amd.tables relative to the config file;
   * per-species settings: the PARTICLES defaults merged into each species' section
     (src/main.cpp:242-249);
   * the GLOBAL settings (simtype, simcount, simtime, secondaries).
@@ -160,8 +161,12 @@ def _need(vals, n, what):
     return vals
 
 
-def load_fields(cfg):
-    """The FIELDS section in key order (TFieldManager::TFieldManager, src/fields.cpp:19-136)."""
+def load_fields(cfg, base_dir=None):
+    """The FIELDS section in key order (TFieldManager::TFieldManager, src/fields.cpp:19-136).
+    Table files are found relative to base_dir (default: the config file's folder)."""
+    from . import tables
+    if base_dir is None:
+        base_dir = os.path.dirname(os.path.abspath(cfg.path)) if cfg.path else os.getcwd()
     formulas = cfg["FORMULAS"]
     out = []
     for key, rest in cfg.items("FIELDS"):
@@ -183,10 +188,21 @@ def load_fields(cfg):
             par = [float(x) for x in v[:npar]]
             box = tuple(float(x) for x in v[npar:npar + 6])
             out.append(analytic_field(ftype, par, box=box, bscale=resolve_formula(v[npar + 6], formulas)))
+        elif ftype in ("OPERA3D", "3Dtable"):
+            out.append(_table(tables.read_opera3d, ftype, args, formulas, base_dir))
+        elif ftype == "COMSOL":
+            out.append(_table(tables.read_comsol, ftype, args, formulas, base_dir))
         else:
-            raise ConfigError(f"Could not load field {ftype}: type not supported by this engine "
-                              "(tabulated maps come from pentrack_amd.fields.tabfield3)")
+            raise ConfigError(f"Could not load field {ftype}: type not supported by this engine")
     return TFieldManager(out)
+
+
+def _table(reader, ftype, args, formulas, base_dir):
+    from .tables import TableError
+    try:
+        return reader(ftype, args, formulas, base_dir)
+    except TableError as e:
+        raise ConfigError(str(e)) from None
 
 
 def _num(section, key, default, conv=float):

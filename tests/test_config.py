@@ -115,7 +115,7 @@ def test_materials_geometry_and_fields():
     with pytest.raises(ValueError):
         C.load_materials(C.TConfig(text="[MATERIALS]\nx 1 0 0.5 0 1e-9 1e-8\n"))
     with pytest.raises(C.ConfigError, match="not supported"):
-        C.load_fields(C.TConfig(text="[FIELDS]\n1 OPERA3D a.tab 1 1 0 1\n"))
+        C.load_fields(C.TConfig(text="[FIELDS]\n1 HarmonicExpandedBField 0 0 0 1\n"))
 
 
 @pytest.mark.skipif(not os.path.exists("/root/reference/in/config.in"), reason="reference inputs absent")
