@@ -90,6 +90,7 @@ int pt_geometry_info(const pt_geometry *g, int *n_nodes, int *depth);
 #define PT_FIELD_B0_GRAD_X2 7    /* TB0GradX2 (src/analyticFields.cpp:85-105): p = a1 a2 a3 z0 */
 #define PT_FIELD_B0_GRAD_XY 8    /* TB0GradXY (src/analyticFields.cpp:109-129): p = a1 a2 z0 */
 #define PT_FIELD_B0_XY 9         /* TB0_XY (src/analyticFields.cpp:133-153): p = a1 z0 */
+#define PT_FIELD_TAB2D 10        /* TabField (src/field_2d.cpp): axisymmetric r-z map, bicubic */
 
 typedef struct {
     int type;
@@ -100,6 +101,11 @@ typedef struct {
     const double *xg, *yg, *zg;
     int has_b, has_v;           /* coefficient blocks per cell: Bx By Bz if has_b, then V if has_v */
     const double *coef;         /* [(nx-1)(ny-1)(nz-1)][ncomp][64], cell (ix*(ny-1)+iy)*(nz-1)+iz */
+    /* PT_FIELD_TAB2D uses nx radial points in xg and nz axial points in zg (both strictly
+       increasing), has_b = mask of the Br (1), Bphi (2), Bz (4) columns, has_v = mask of
+       the V (1), Er (2), Ephi (4), Ez (8) columns (V not together with E: the reference
+       ignores the potential when E is tabulated), and coef from pt_bicubic_precompute
+       with the present columns in the order Br Bphi Bz V Er Ephi Ez. */
     double e[3];                /* uniform E */
     double p[8];                /* analytic field and conductor parameters (see the type codes) */
     /* TFieldScaler formulas of the time t (src/field.cpp:7-45) in exprtk syntax, replacing
@@ -130,6 +136,16 @@ void pt_fields_destroy(pt_fields *f);
  */
 int pt_tricubic_precompute(int nx, int ny, int nz, const double *xg, const double *yg, const double *zg,
                            int ncomp, const double *values, double *coef, int threads);
+/*
+ * TabField's bicubic splines (src/field_2d.cpp:235-285: ALGLIB spline2dbuildbicubicv, the
+ * derivative tables of spline2d_bicubiccalcderivatives).
+ * m, rg     radial grid, n, zg axial grid (strictly increasing)
+ * values    [ncomp][n][m] samples, values[c][iz][ir]
+ * coef      [(n-1)(m-1)][ncomp][16] output, cell iz*(m-1)+ir: f, df/dr, df/dz, d2f/drdz
+ *           at the corners (iz,ir), (iz,ir+1), (iz+1,ir), (iz+1,ir+1)
+ */
+int pt_bicubic_precompute(int m, int n, const double *rg, const double *zg, int ncomp, const double *values,
+                          double *coef);
 /* the 64x64 integer matrix of libtricubic's tricubic_get_coeff (coefficients = A x),
    row-major into out[64*64]; for tests */
 int pt_tricubic_matrix(int32_t *out);
@@ -249,7 +265,9 @@ int pt_proton_beta_table(int cap, double *x, double *raw, double *den, double *c
 int pt_distribution_table(const char *formula, double lo, double hi, int cap, double *x, double *raw,
                           double *den, double *cp, double *m, int *n_int);
 /* TFieldManager::BField / EField (src/fields.cpp:139-182) at n points pts[n][3] and times t[n]:
-   B [n][3], dB [n][3][3], V [n], E [n][3] (host arrays; FP64 tables; synchronises the stream) */
+   B [n][3], dB [n][3][3], V [n], E [n][3] (host arrays; FP64 tables; synchronises the stream).
+   dB may be NULL: then B is the value-only evaluation BField(x, y, z, t, B) (which for 2D
+   maps differs from the one with derivatives in the last bits, as in the reference). */
 int pt_fields_eval(const pt_fields *f, void *stream, int n, const double *pts, const double *t, double *B,
                    double *dB, double *V, double *E);
 /* TGeometry::GetSolid (src/geometry.cpp:151-175) at n points and times: the ID of the

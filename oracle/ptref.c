@@ -68,6 +68,7 @@ typedef long double ld;
 #define R_COS(a) cos(a)
 #define R_ACOS(a) acos(a)
 #define R_EXP(a) exp(a)
+#define R_ATAN2(y, x) atan2(y, x)
 #define SQ(a) pow(a, 2)
 #else
 typedef double ld;
@@ -79,6 +80,7 @@ typedef double ld;
 #define R_COS(a) ptm_cos(a)
 #define R_ACOS(a) ptm_acos(a)
 #define R_EXP(a) ptm_exp(a)
+#define R_ATAN2(y, x) ptm_atan2(y, x)
 #define SQ(a) ((a) * (a))
 #endif
 
@@ -872,6 +874,223 @@ static void conductor_B(const ptref_field *f, double x, double y, double z, doub
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* TabField (src/field_2d.cpp) with ALGLIB 3.15.0's bicubic splines          */
+/* ------------------------------------------------------------------------ */
+
+/* spline1d_spline1dgriddiffcubicinternal with parabolically terminated ends
+   (alglib interpolation.cpp:26709-26882) and spline1d_solvetridiagonal (:26970-27010):
+   node derivatives d[n] of the cubic spline through (x, y); w: 4n scratch */
+static void alg_griddiff(const double *x, const double *y, int n, double *d, double *w) {
+    if (n == 2) {
+        d[0] = (y[1] - y[0]) / (x[1] - x[0]);
+        d[1] = d[0];
+        return;
+    }
+    double *a = w, *b = w + n, *c = w + 2 * n, *r = w + 3 * n;
+    a[0] = 0; b[0] = 1; c[0] = 1;
+    r[0] = 2 * (y[1] - y[0]) / (x[1] - x[0]);
+    for (int i = 1; i <= n - 2; i++) {
+        a[i] = x[i + 1] - x[i];
+        b[i] = 2 * (x[i + 1] - x[i - 1]);
+        c[i] = x[i] - x[i - 1];
+        r[i] = 3 * (y[i] - y[i - 1]) / (x[i] - x[i - 1]) * (x[i + 1] - x[i]) +
+               3 * (y[i + 1] - y[i]) / (x[i + 1] - x[i]) * (x[i] - x[i - 1]);
+    }
+    a[n - 1] = 1; b[n - 1] = 1; c[n - 1] = 0;
+    r[n - 1] = 2 * (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
+    for (int k = 1; k <= n - 1; k++) {
+        double t = a[k] / b[k - 1];
+        b[k] = b[k] - t * c[k - 1];
+        r[k] = r[k] - t * r[k - 1];
+    }
+    d[n - 1] = r[n - 1] / b[n - 1];
+    for (int k = n - 2; k >= 0; k--) d[k] = (r[k] - c[k] * d[k + 1]) / b[k];
+}
+
+/* spline1dbuildcubic (parabolic ends) as a Hermite spline (spline1dbuildhermite,
+   :23642-23700), then spline1ddiff (:23914-23980) at every node: the bisection over
+   the n-1 pieces and the derivative c1 + 2 s c2 + 3 s^2 c3 of the piece found */
+static void alg_node_diff(const double *x, const double *y, int n, double *ds, double *w) {
+    double *d = w + 4 * n;
+    alg_griddiff(x, y, n, d, w);
+    for (int j = 0; j < n; ++j) {
+        int l = 0, r = n - 1;
+        while (l != r - 1) {
+            int m = (l + r) / 2;
+            if (x[m] >= x[j]) r = m;
+            else l = m;
+        }
+        double delta = x[l + 1] - x[l], delta2 = delta * delta, delta3 = delta * delta2;
+        double c1 = d[l];
+        double c2 = (3 * (y[l + 1] - y[l]) - 2 * d[l] * delta - d[l + 1] * delta) / delta2;
+        double c3 = (2 * (y[l] - y[l + 1]) + d[l] * delta + d[l + 1] * delta) / delta3;
+        double s = x[j] - x[l];
+        ds[j] = c1 + 2 * s * c2 + 3 * (s * s) * c3;
+    }
+}
+
+/* spline2dbuildbicubicv (:44925-45075) with spline2d_bicubiccalcderivatives (:46791-46886):
+   for every column c of values [ncomp][n][m] (values[c][iy][ix], x = r, y = z) the tables
+   out[c][0..3][n][m] = f, df/dx, df/dy, d2f/dxdy */
+void ptref_bicubic_tables(int m, int n, const double *xg, const double *yg, int ncomp, const double *values,
+                          double *out) {
+    int L = m > n ? m : n;
+    double *w = malloc(sizeof(double) * 7 * L);
+    double *line = w + 5 * L, *res = w + 6 * L;
+    size_t np = (size_t)m * n;
+    for (int c = 0; c < ncomp; ++c) {
+        const double *a = values + c * np;
+        double *f = out + (size_t)c * 4 * np, *fx = f + np, *fy = fx + np, *fxy = fy + np;
+        memcpy(f, a, sizeof(double) * np);
+        for (int i = 0; i < n; ++i) alg_node_diff(xg, a + (size_t)i * m, m, fx + (size_t)i * m, w);
+        for (int j = 0; j < m; ++j) {
+            for (int i = 0; i < n; ++i) line[i] = a[(size_t)i * m + j];
+            alg_node_diff(yg, line, n, res, w);
+            for (int i = 0; i < n; ++i) fy[(size_t)i * m + j] = res[i];
+        }
+        for (int i = 0; i < n; ++i) alg_node_diff(xg, fy + (size_t)i * m, m, fxy + (size_t)i * m, w);
+    }
+    free(w);
+}
+
+/* the interval of ALGLIB's bisection in spline2dcalc / spline2ddiff */
+static int alg_interval(const double *g, int n, double v) {
+    int l = 0, r = n - 1;
+    while (l != r - 1) {
+        int h = (l + r) / 2;
+        if (g[h] >= v) r = h;
+        else l = h;
+    }
+    return l;
+}
+
+/* spline2dcalc (:43381-43520, diff = 0) and spline2ddiff (:43538-43700, diff = 1) on the
+   tables tb[4][n][m] of one column; fx, fy only with diff */
+static void alg_spline2d(const ptref_field *f, const double *tb, double x, double y, int diff, double *F,
+                         double *Fx, double *Fy) {
+    const int m = f->nx, n = f->nz;
+    const double *X = f->xg, *Y = f->zg;
+    const size_t np = (size_t)m * n;
+    int ix = alg_interval(X, m, x), iy = alg_interval(Y, n, y);
+    double t, dt, u, du;
+    if (diff) {
+        t = (x - X[ix]) / (X[ix + 1] - X[ix]); dt = 1.0 / (X[ix + 1] - X[ix]);
+        u = (y - Y[iy]) / (Y[iy + 1] - Y[iy]); du = 1.0 / (Y[iy + 1] - Y[iy]);
+    } else {
+        dt = 1.0 / (X[ix + 1] - X[ix]); t = (x - X[ix]) * dt;
+        du = 1.0 / (Y[iy + 1] - Y[iy]); u = (y - Y[iy]) * du;
+    }
+    size_t s[4] = {(size_t)m * iy + ix, (size_t)m * iy + ix + 1, (size_t)m * (iy + 1) + ix, (size_t)m * (iy + 1) + ix + 1};
+    double t2 = t * t, t3 = t * t2, u2 = u * u, u3 = u * u2;
+    double ht00 = 2 * t3 - 3 * t2 + 1, ht10 = t3 - 2 * t2 + t, ht01 = -2 * t3 + 3 * t2, ht11 = t3 - t2;
+    double hu00 = 2 * u3 - 3 * u2 + 1, hu10 = u3 - 2 * u2 + u, hu01 = -2 * u3 + 3 * u2, hu11 = u3 - u2;
+    ht10 = ht10 / dt; ht11 = ht11 / dt; hu10 = hu10 / du; hu11 = hu11 / du;
+    /* per table k: the basis of (t at corner 0/1) x (u at corner 0/1), value and derivatives */
+    const double ht[2][2] = {{ht00, ht01}, {ht10, ht11}}, hu[2][2] = {{hu00, hu01}, {hu10, hu11}};
+    double dht[2][2] = {{6 * t2 - 6 * t, -6 * t2 + 6 * t}, {3 * t2 - 4 * t + 1, 3 * t2 - 2 * t}};
+    double dhu[2][2] = {{6 * u2 - 6 * u, -6 * u2 + 6 * u}, {3 * u2 - 4 * u + 1, 3 * u2 - 2 * u}};
+    dht[0][0] = dht[0][0] * dt; dht[0][1] = dht[0][1] * dt;
+    dhu[0][0] = dhu[0][0] * du; dhu[0][1] = dhu[0][1] * du;
+    double rf = 0, rx = 0, ry = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int a = k & 1, b = k >> 1;   /* table k: derivative order a in x, b in y */
+        const double *v = tb + k * np;
+        rf = rf + v[s[0]] * ht[a][0] * hu[b][0] + v[s[1]] * ht[a][1] * hu[b][0] + v[s[2]] * ht[a][0] * hu[b][1] +
+             v[s[3]] * ht[a][1] * hu[b][1];
+        if (diff) {
+            rx = rx + v[s[0]] * dht[a][0] * hu[b][0] + v[s[1]] * dht[a][1] * hu[b][0] + v[s[2]] * dht[a][0] * hu[b][1] +
+                 v[s[3]] * dht[a][1] * hu[b][1];
+            ry = ry + v[s[0]] * ht[a][0] * dhu[b][0] + v[s[1]] * ht[a][1] * dhu[b][0] + v[s[2]] * ht[a][0] * dhu[b][1] +
+                 v[s[3]] * ht[a][1] * dhu[b][1];
+        }
+    }
+    *F = rf;
+    if (diff) { *Fx = rx; *Fy = ry; }
+}
+
+/* exported for tests: spline2dcalc / spline2ddiff of column c of a 2D-map field */
+void ptref_spline2d(const ptref_field *f, int c, int n, const double *x, const double *y, int diff, double *F,
+                    double *Fx, double *Fy) {
+    const double *tb = f->bic + (size_t)c * 4 * f->nx * f->nz;
+    for (int i = 0; i < n; ++i) alg_spline2d(f, tb, x[i], y[i], diff, &F[i], diff ? &Fx[i] : 0, diff ? &Fy[i] : 0);
+}
+
+/* TabField::BField's range: r_mi + rdist*(m-1) from the first spacing (src/field_2d.cpp:199-206,
+   289-291) */
+static int tab2d_range(const ptref_field *f, double r, double z) {
+    double rlo = f->xg[0], rdist = f->xg[1] - f->xg[0], zlo = f->zg[0], zdist = f->zg[1] - f->zg[0];
+    return r >= rlo && r < rlo + rdist * (f->nx - 1) && z >= zlo && z < zlo + zdist * (f->nz - 1);
+}
+
+static const double *tab2d_column(const ptref_field *f, int k) {
+    return f->bic + (size_t)k * 4 * f->nx * f->nz;
+}
+
+/* TabField::BField (src/field_2d.cpp:289-337) into zeroed B, dB */
+static void tab2d_B(track_t *tp, const ptref_field *f, double x, double y, double z, double B[3], double dB[3][3]) {
+    double r = sqrt(x * x + y * y);
+    if (!tab2d_range(f, r, z)) return;
+    tp->n_field++;
+    double Br = 0, Bphi = 0, Bz = 0;
+    double phi = R_ATAN2(y, x);
+    int k = 0;
+    const double *col[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i)
+        if (f->has_b & (1 << i)) col[i] = tab2d_column(f, k++);
+    if (dB) {
+        double dBrdr = 0, dBrdz = 0, dBphidr = 0, dBphidz = 0, dBzdr = 0, dBzdz = 0;
+        if (col[0]) alg_spline2d(f, col[0], r, z, 1, &Br, &dBrdr, &dBrdz);
+        if (col[1]) alg_spline2d(f, col[1], r, z, 1, &Bphi, &dBphidr, &dBphidz);
+        double c = R_COS(phi), s = R_SIN(phi);
+        if (r > 0) {
+            dB[0][0] = dBrdr * c * c - dBphidr * c * s + (Br * s * s + Bphi * c * s) / r;
+            dB[0][1] = dBrdr * c * s - dBphidr * s * s - (Br * c * s + Bphi * c * c) / r;
+            dB[1][0] = dBrdr * c * s + dBphidr * c * c - (Br * c * s - Bphi * s * s) / r;
+            dB[1][1] = dBrdr * s * s + dBphidr * c * s + (Br * c * c - Bphi * c * s) / r;
+        }
+        dB[0][2] = dBrdz * c - dBphidz * s;
+        dB[1][2] = dBrdz * s + dBphidz * c;
+        if (col[2]) {
+            alg_spline2d(f, col[2], r, z, 1, &Bz, &dBzdr, &dBzdz);
+            dB[2][0] = dBzdr * c;
+            dB[2][1] = dBzdr * s;
+            dB[2][2] = dBzdz;
+        }
+    } else {
+        if (col[0]) alg_spline2d(f, col[0], r, z, 0, &Br, 0, 0);
+        if (col[1]) alg_spline2d(f, col[1], r, z, 0, &Bphi, 0, 0);
+        if (col[2]) alg_spline2d(f, col[2], r, z, 0, &Bz, 0, 0);
+    }
+    B[0] = Br * R_COS(phi) - Bphi * R_SIN(phi);
+    B[1] = Br * R_SIN(phi) + Bphi * R_COS(phi);
+    B[2] = Bz;
+}
+
+/* TabField::EField (src/field_2d.cpp:340-400) into zeroed V, E */
+static void tab2d_E(track_t *tp, const ptref_field *f, double x, double y, double z, double *V, double E[3]) {
+    double r = sqrt(x * x + y * y);
+    if (!tab2d_range(f, r, z)) return;
+    tp->n_field++;
+    int k = __builtin_popcount((unsigned)f->has_b);
+    if (f->has_v & 14) {
+        double phi = R_ATAN2(y, x);
+        double Ev[3] = {0, 0, 0};
+        for (int i = 0; i < 3; ++i)
+            if (f->has_v & (2 << i)) alg_spline2d(f, tab2d_column(f, k++), r, z, 0, &Ev[i], 0, 0);
+        E[0] = Ev[0] * R_COS(phi) - Ev[1] * R_SIN(phi);
+        E[1] = Ev[0] * R_SIN(phi) + Ev[1] * R_COS(phi);
+        E[2] = Ev[2];
+    } else if (f->has_v & 1) {
+        double dVdr, dVdz;
+        alg_spline2d(f, tab2d_column(f, k), r, z, 1, V, &dVdr, &dVdz);
+        double phi = R_ATAN2(y, x);
+        E[0] = -dVdr * R_COS(phi);
+        E[1] = -dVdr * R_SIN(phi);
+        E[2] = -dVdz;
+    }
+}
+
 /* TFieldContainer::BField (src/field.cpp:139-153) */
 static void container_B(track_t *tp, const ptref_field *f, double x, double y, double z, double t, double B[3], double dB[3][3]) {
     double bs = in_bounds(f, x, y, z) ? scale_B(f, t) : 0.;
@@ -880,6 +1099,7 @@ static void container_B(track_t *tp, const ptref_field *f, double x, double y, d
         return;
     }
     if (f->type == PTREF_FIELD_CONDUCTOR) conductor_B(f, x, y, z, B, dB);
+    else if (f->type == PTREF_FIELD_TAB2D) tab2d_B(tp, f, x, y, z, B, dB);
     else if (f->type >= PTREF_FIELD_EXPONENTIAL_X) analytic_B(f, x, y, z, B, dB);
     if (f->type == PTREF_FIELD_TRICUBIC && f->has_b) {
         long cell; double r[3], dist[3];
@@ -915,6 +1135,8 @@ static void container_E(track_t *tp, const ptref_field *f, double x, double y, d
     if (f->type == PTREF_FIELD_UNIFORM_E) { /* TEDMStaticEField::EField, src/edmfields.cpp:181-187 */
         E[0] = f->e[0]; E[1] = f->e[1]; E[2] = f->e[2];
         *V = E[2] * z;
+    } else if (f->type == PTREF_FIELD_TAB2D) {
+        tab2d_E(tp, f, x, y, z, V, E);
     } else if (f->type == PTREF_FIELD_TRICUBIC && f->has_v) {
         long cell; double r[3], dist[3];
         double dV[3] = {0., 0., 0.};
@@ -1858,12 +2080,6 @@ static void lorentz_boost(const double beta[3], double p[4]) {
     p[0] = (gamma * (p[0] + bp));
 }
 
-#if PTREF_REFSEM
-#define R_ATAN2(y, x) atan2(y, x)
-#else
-#define R_ATAN2(y, x) ptm_atan2(y, x)
-#endif
-
 /* TNeutron::Decay, src/neutron.cpp:285-362 (ld: the reference's long double constants) */
 static void neutron_decay(uint64_t key, uint64_t *ctr, const double v[3], double o6[6], int32_t pol[2]) {
     const ld c0 = C_0, mp = MASS_P, mn = MASS_N, me = MASS_E;
@@ -1974,8 +2190,9 @@ int ptref_field_eval(int n_fields, const ptref_field *fields, int n, const doubl
     for (int i = 0; i < n; ++i) {
         const double *p = pts + 3 * i;
         double dBl[3][3];
-        field_B(&tp, p[0], p[1], p[2], t, B + 3 * i, dBl);
-        for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) dB[9 * i + 3 * a + b] = dBl[a][b];
+        field_B(&tp, p[0], p[1], p[2], t, B + 3 * i, dB ? dBl : 0);   /* no dB: value-only BField */
+        if (dB)
+            for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) dB[9 * i + 3 * a + b] = dBl[a][b];
         field_E(&tp, p[0], p[1], p[2], t, V + i, E + 3 * i);
     }
     return 0;

@@ -35,7 +35,7 @@ typedef struct {
 enum {
     PTREF_FIELD_TRICUBIC = 1, PTREF_FIELD_UNIFORM_E = 2, PTREF_FIELD_CONDUCTOR = 3,
     PTREF_FIELD_EXPONENTIAL_X = 4, PTREF_FIELD_LINEAR_Z = 5, PTREF_FIELD_B0_GRAD_Z = 6,
-    PTREF_FIELD_B0_GRAD_X2 = 7, PTREF_FIELD_B0_GRAD_XY = 8, PTREF_FIELD_B0_XY = 9
+    PTREF_FIELD_B0_GRAD_X2 = 7, PTREF_FIELD_B0_GRAD_XY = 8, PTREF_FIELD_B0_XY = 9, PTREF_FIELD_TAB2D = 10
 };
 
 /* one node of a field-scaling formula tree (oracle/formula.py nodes()): children
@@ -61,6 +61,10 @@ typedef struct {
     const double *coef;         /* [(nx-1)(ny-1)(nz-1)][ncomp][64], cell index (ix*(ny-1)+iy)*(nz-1)+iz */
     /* uniform E */
     double e[3];
+    /* axisymmetric map: nx radial points in xg, nz axial points in zg; has_b = mask of Br (1),
+       Bphi (2), Bz (4), has_v = mask of V (1), Er (2), Ephi (4), Ez (8); bic = ALGLIB's four
+       tables per present column (ptref_bicubic_tables) */
+    const double *bic;          /* [ncomp][4][nz][nx]: f, df/dr, df/dz, d2f/drdz */
 } ptref_field;
 
 typedef struct {

@@ -47,6 +47,7 @@ class Field(C.Structure):
         ("xg", C.POINTER(C.c_double)), ("yg", C.POINTER(C.c_double)), ("zg", C.POINTER(C.c_double)),
         ("has_b", C.c_int), ("has_v", C.c_int), ("coef", C.POINTER(C.c_double)),
         ("e", C.c_double * 3),
+        ("bic", C.POINTER(C.c_double)),
     ]
 
 
@@ -101,6 +102,10 @@ def load(mode="fp64"):
     L.ptref_stream_key.argtypes = [C.c_uint64, C.c_uint64]
     L.ptref_stream_key.restype = C.c_uint64
     L.ptref_tricubic_eval.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
+    L.ptref_bicubic_tables.argtypes = [C.c_int, C.c_int, dp, dp, C.c_int, dp, dp]
+    L.ptref_bicubic_tables.restype = None
+    L.ptref_spline2d.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp, dp]
+    L.ptref_spline2d.restype = None
     L.ptref_tricubic_eval.restype = C.c_double
     L.ptref_integrate_free.argtypes = [C.c_int, C.POINTER(Field), C.c_int, C.c_double, dp, C.c_int, dp, dp, dp]
     L.ptref_dense_probe.argtypes = [C.c_int, C.c_double, dp, C.c_double, C.c_int, dp, dp, dp]
@@ -164,7 +169,15 @@ class Problem:
                     self._keep.append(arr)
                     setattr(f, attr, C.cast(arr, C.POINTER(FNode)))
                     setattr(f, nattr, len(arr))
-            if c.grid is not None:
+            if c.type == 10:     # 2D map: the oracle's own ALGLIB tables from the samples
+                rg, zg = (np.ascontiguousarray(c.grid[i], dtype=np.float64) for i in (0, 2))
+                bic = bicubic_tables(rg, zg, c.values)
+                self._keep.extend([rg, zg, bic])
+                f.nx, f.ny, f.nz = len(rg), 1, len(zg)
+                f.xg, f.zg = _dp(rg), _dp(zg)
+                f.has_b, f.has_v = c.has_b, c.has_v
+                f.bic = _dp(bic)
+            elif c.grid is not None:
                 xg, yg, zg = (np.ascontiguousarray(g_, dtype=np.float64) for g_ in c.grid)
                 coef = np.ascontiguousarray(c.coef, dtype=np.float64)
                 self._keep.extend([xg, yg, zg, coef])
@@ -172,6 +185,25 @@ class Problem:
                 f.xg, f.yg, f.zg = _dp(xg), _dp(yg), _dp(zg)
                 f.has_b, f.has_v = c.has_b, c.has_v
                 f.coef = _dp(coef)
+
+
+def bicubic_tables(rg, zg, values, mode="fp64"):
+    """ALGLIB spline2dbuildbicubicv tables [ncomp][4][nz][nr] of samples [ncomp][nz][nr]."""
+    L = load(mode)
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    rg, zg = np.ascontiguousarray(rg, dtype=np.float64), np.ascontiguousarray(zg, dtype=np.float64)
+    out = np.empty((values.shape[0], 4) + values.shape[1:])
+    L.ptref_bicubic_tables(len(rg), len(zg), _dp(rg), _dp(zg), values.shape[0], _dp(values), _dp(out))
+    return out
+
+
+def spline2d(problem, k, c, x, y, diff, mode="fp64"):
+    """spline2dcalc (diff False) / spline2ddiff of column c of 2D-map field k at points (x, y)."""
+    L = load(mode)
+    x, y = np.ascontiguousarray(x, dtype=np.float64), np.ascontiguousarray(y, dtype=np.float64)
+    F, Fx, Fy = np.zeros(len(x)), np.zeros(len(x)), np.zeros(len(x))
+    L.ptref_spline2d(C.byref(problem.fields[k]), c, len(x), _dp(x), _dp(y), int(bool(diff)), _dp(F), _dp(Fx), _dp(Fy))
+    return (F, Fx, Fy) if diff else F
 
 
 def track(workload, mode="fp64", hit_cap=0, threads=1, sel=None, problem=None):
@@ -300,9 +332,10 @@ def get_solids(geom, t, p, mode="fp64"):
     return list(ids[:n]), list(ign[:n])
 
 
-def field_eval(problem, pts, t=0.0, mode="fp64"):
+def field_eval(problem, pts, t=0.0, mode="fp64", grad=True):
     """TFieldManager::BField / EField at pts [n][3] and time t (a number or one per point);
-    problem: a Problem, or a TFieldManager (fields only)."""
+    problem: a Problem, or a TFieldManager (fields only).  grad False: the value-only
+    BField(x, y, z, t, B), dB returned as None."""
     if not isinstance(problem, Problem):
         problem = Problem(None, problem)
     L = load(mode)
@@ -310,14 +343,15 @@ def field_eval(problem, pts, t=0.0, mode="fp64"):
     n = len(pts)
     B = np.zeros((n, 3)); dB = np.zeros((n, 3, 3)); V = np.zeros(n); E = np.zeros((n, 3))
     if np.ndim(t) == 0:
-        L.ptref_field_eval(problem.n_fields, problem.fields, n, _dp(pts), float(t), _dp(B), _dp(dB), _dp(V), _dp(E))
+        L.ptref_field_eval(problem.n_fields, problem.fields, n, _dp(pts), float(t), _dp(B), _dp(dB) if grad else None,
+                           _dp(V), _dp(E))
     else:
         for i in range(n):
             b, db, v, e = (np.zeros(3), np.zeros(9), np.zeros(1), np.zeros(3))
-            L.ptref_field_eval(problem.n_fields, problem.fields, 1, _dp(pts[i]), float(t[i]), _dp(b), _dp(db), _dp(v),
-                               _dp(e))
+            L.ptref_field_eval(problem.n_fields, problem.fields, 1, _dp(pts[i]), float(t[i]), _dp(b),
+                               _dp(db) if grad else None, _dp(v), _dp(e))
             B[i], dB[i], V[i], E[i] = b, db.reshape(3, 3), v[0], e
-    return B, dB, V, E
+    return B, dB if grad else None, V, E
 
 
 # ---- differentiable RK4 path (oracle/ptadj.c) -------------------------------------

@@ -6,6 +6,7 @@
  * product links or loads this.
  *
  *   ALGLIB 3.15.0 spline1dgriddiffcubic (alglib-3.15.0/cpp/src/interpolation.cpp)
+ *   ALGLIB 3.15.0 spline2dbuildbicubicv / spline2dcalc / spline2ddiff (the 2D maps)
  *   libtricubic 1.0 tricubic_get_coeff / tricubic_eval
  *     (tricubic-1.0/src/libtricubic/libtricubic.cpp)
  *
@@ -104,3 +105,31 @@ int ref_preinterpol(int nx, int ny, int nz, const double *xg, const double *yg, 
 }
 
 } // extern "C"
+
+extern "C" {
+
+/*
+ * TabField's splines (src/field_2d.cpp:235-285, 289-337): alglib::spline2dbuildbicubicv
+ * over the radial grid x[m] and axial grid y[n] of values[n][m], then at the points
+ * (px, py): spline2dcalc (diff == 0; F only) or spline2ddiff (diff == 1; F, dF/dx, dF/dy).
+ */
+int ref_spline2d(int m, int n, const double *x, const double *y, const double *values, int np, const double *px,
+                 const double *py, int diff, double *F, double *Fx, double *Fy) {
+    alglib::real_1d_array X, Y, V;
+    X.setcontent(m, x);
+    Y.setcontent(n, y);
+    V.setcontent((alglib::ae_int_t)m * n, values);
+    alglib::spline2dinterpolant c;
+    alglib::spline2dbuildbicubicv(X, m, Y, n, V, 1, c);
+    for (int i = 0; i < np; ++i) {
+        if (diff) {
+            double fxy;
+            alglib::spline2ddiff(c, px[i], py[i], F[i], Fx[i], Fy[i], fxy);
+        } else {
+            F[i] = alglib::spline2dcalc(c, px[i], py[i]);
+        }
+    }
+    return 0;
+}
+
+}

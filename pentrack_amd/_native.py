@@ -24,6 +24,7 @@ PT_FIELD_B0_GRAD_Z = 6
 PT_FIELD_B0_GRAD_X2 = 7
 PT_FIELD_B0_GRAD_XY = 8
 PT_FIELD_B0_XY = 9
+PT_FIELD_TAB2D = 10
 PT_VARIANT_FP64 = 0
 PT_VARIANT_MIXED = 1
 
@@ -129,7 +130,7 @@ EXPORTS = [
     "pt_request_quit", "pt_last_error", "pt_clearance_grid", "pt_tricubic_matrix",
     "pt_spline_grid_diff", "pt_rk4_forward_ckpt", "pt_rk4_adjoint", "pt_formula_eval",
     "pt_batch_secondaries", "pt_proton_beta_table", "pt_neutron_decay_host", "pt_beta_sample",
-    "pt_distribution_table", "pt_fields_eval", "pt_geometry_solid_at",
+    "pt_distribution_table", "pt_fields_eval", "pt_geometry_solid_at", "pt_bicubic_precompute",
 ]
 
 _lib = None
@@ -162,6 +163,8 @@ def lib():
                                          C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), C.c_int]
     L.pt_tricubic_matrix.argtypes = [ip]
+    L.pt_bicubic_precompute.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
+                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pt_batch_create.argtypes = [vp, vp, C.c_int, C.POINTER(Species), C.c_double, C.c_int, ip,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.c_uint64, C.POINTER(C.c_uint64), C.c_int, C.POINTER(vp)]

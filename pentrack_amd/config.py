@@ -8,8 +8,9 @@ in which the reference loads solids (GEOMETRY) and sums fields (FIELDS).  From i
   * materials (operator>>(istream, material), src/geometry.cpp:10-52) and solids with
     their STL meshes and ignore times (src/geometry.cpp:60-137);
   * field containers of the FIELDS section (TFieldManager, src/fields.cpp:19-136) with
-    scale formulas resolved through FORMULAS (ResolveFormula), table files (OPERA3D,
-    3Dtable, COMSOL) read by pentrack_amd.tables relative to the config file;
+    scale formulas resolved through FORMULAS (ResolveFormula), table files (OPERA2D,
+    2Dtable, OPERA3D, 3Dtable, COMSOL) read by pentrack_amd.tables relative to the config
+    file;
   * per-species settings: the PARTICLES defaults merged into each species' section
     (src/main.cpp:242-249);
   * the GLOBAL settings (simtype, simcount, simtime, secondaries).
@@ -192,6 +193,8 @@ def load_fields(cfg, base_dir=None):
             out.append(_table(tables.read_opera3d, ftype, args, formulas, base_dir))
         elif ftype == "COMSOL":
             out.append(_table(tables.read_comsol, ftype, args, formulas, base_dir))
+        elif ftype in ("OPERA2D", "2Dtable"):
+            out.append(_table(tables.read_opera2d, ftype, args, formulas, base_dir))
         else:
             raise ConfigError(f"Could not load field {ftype}: type not supported by this engine")
     return TFieldManager(out)

@@ -134,6 +134,7 @@ constexpr int DENSE = 59;     /* xo[8] dxo[8] k3..k6[32] dxc[8] t t_old pol */
 struct Params {
     Geom geom;
     int n_fields;
+    int b_reuse;              /* 0 if a 2D map tabulates B: its value-only B differs from the one with derivatives */
     const Field *fields;
     const Species *species;
     double tmax_global;

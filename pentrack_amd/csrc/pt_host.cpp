@@ -438,6 +438,7 @@ struct pt_fields {
     int n = 0;
     bool has_fp32 = false;
     bool general = false;   /* a scale formula, analytic field or conductor is present */
+    bool b_reuse = true;    /* no 2D map tabulates B (value-only B equals B with derivatives) */
 };
 
 extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32, pt_fields **out) {
@@ -529,10 +530,46 @@ extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32
                 if (r) return r;
                 f.coef32 = dc32;
             }
+        } else if (d.type == PT_FIELD_TAB2D) {
+            /* TabField (src/field_2d.cpp): r grid in xg[nx], z grid in zg[nz] */
+            if (d.nx < 2 || d.nz < 2 || !d.xg || !d.zg || !d.coef || (d.has_b & ~7) || (d.has_v & ~15) ||
+                (!d.has_b && !d.has_v))
+                return fail(PT_EINVAL, "pt_fields_create: bad 2D table descriptor");
+            if ((d.has_v & 1) && (d.has_v & 14))
+                return fail(PT_EINVAL, "pt_fields_create: 2D table with both V and E columns (the potential is ignored "
+                                       "when E is tabulated: leave it out)");
+            const double *gs[2] = {d.xg, d.zg};
+            int ns[2] = {d.nx, d.nz};
+            for (int a = 0; a < 2; ++a) {
+                for (int i = 1; i < ns[a]; ++i)
+                    if (!(gs[a][i] > gs[a][i - 1])) return fail(PT_EINVAL, "pt_fields_create: grid not strictly increasing");
+                double *dg;
+                int r = F->mem.upload(&dg, gs[a], (size_t)ns[a]);
+                if (r) return r;
+                f.g[2 * a] = dg;
+                f.x0[2 * a] = gs[a][0];
+                f.inv_h[2 * a] = (double)(ns[a] - 1) / (gs[a][ns[a] - 1] - gs[a][0]);
+            }
+            f.nx = d.nx; f.ny = 1; f.nz = d.nz;
+            /* TabField::CheckTab (src/field_2d.cpp:199-206): the map's range from the first
+               coordinate and the first spacing */
+            f.p[0] = d.xg[0];
+            f.p[1] = d.xg[0] + (d.xg[1] - d.xg[0]) * (d.nx - 1);
+            f.p[2] = d.zg[0];
+            f.p[3] = d.zg[0] + (d.zg[1] - d.zg[0]) * (d.nz - 1);
+            f.has_b = d.has_b;
+            f.has_v = d.has_v;
+            f.ncomp = __builtin_popcount((unsigned)d.has_b) + __builtin_popcount((unsigned)d.has_v);
+            size_t ncoef = (size_t)(d.nx - 1) * (d.nz - 1) * f.ncomp * 16;
+            double *dc;
+            int r = F->mem.upload(&dc, d.coef, ncoef);
+            if (r) return r;
+            f.coef = dc;
         } else if (d.type < PT_FIELD_UNIFORM_E || d.type > PT_FIELD_B0_XY) {
             return fail(PT_EINVAL, "pt_fields_create: unknown field type");
         }
         F->general = F->general || f.blen > 0 || f.elen > 0 || f.type >= PT_FIELD_CONDUCTOR;
+        F->b_reuse = F->b_reuse && !(f.type == PT_FIELD_TAB2D && f.has_b);
         F->host[k] = f;
     }
     int r = F->mem.upload(&F->dev, F->host.data(), F->host.size());
@@ -731,6 +768,84 @@ extern "C" int pt_spline_grid_diff(int n, const double *x, const double *y, doub
     if (n < 2 || !x || !y || !d) return fail(PT_EINVAL, "pt_spline_grid_diff: bad arguments");
     std::vector<double> buf;
     spline_grid_diff(x, y, n, d, buf);
+    return PT_OK;
+}
+
+/* ======================================================================== */
+/* bicubic tables of axisymmetric maps (TabField, src/field_2d.cpp:235-285)  */
+/* ======================================================================== */
+
+namespace {
+
+/* Derivatives at the nodes of the cubic spline through (x, y): ALGLIB spline1dbuildcubic
+   with parabolic ends (node derivatives d as spline1dgriddiffcubic), its Hermite pieces
+   (spline1dbuildhermite, interpolation.cpp:23642-23700) and spline1ddiff at each node
+   (:23914-23980).  The node x[j], j > 0, lies at the right end of piece j-1, so the
+   value is that piece's derivative at its length, not d[j] itself. */
+void spline_node_diff(const double *x, const double *y, int n, double *ds, std::vector<double> &d,
+                      std::vector<double> &buf) {
+    d.resize(n);
+    spline_grid_diff(x, y, n, d.data(), buf);
+    for (int j = 0; j < n; ++j) {
+        int l = j > 0 ? j - 1 : 0;
+        double delta = x[l + 1] - x[l];
+        double delta2 = delta * delta;
+        double delta3 = delta * delta2;
+        double c1 = d[l];
+        double c2 = (3 * (y[l + 1] - y[l]) - 2 * d[l] * delta - d[l + 1] * delta) / delta2;
+        double c3 = (2 * (y[l] - y[l + 1]) + d[l] * delta + d[l + 1] * delta) / delta3;
+        double s = x[j] - x[l];
+        ds[j] = c1 + 2 * s * c2 + 3 * (s * s) * c3;
+    }
+}
+
+} // namespace
+
+/*
+ * ALGLIB spline2dbuildbicubicv (interpolation.cpp:44925-45075) and its derivative tables
+ * (spline2d_bicubiccalcderivatives, :46791-46886): df/dr along every z row, df/dz along
+ * every r column, d2f/drdz along every z row of df/dz.  Output per cell (iz, ir) and
+ * component: the four tables (f, df/dr, df/dz, d2f/drdz) at the corners (iz,ir),
+ * (iz,ir+1), (iz+1,ir), (iz+1,ir+1) -- the 16 numbers spline2dcalc/spline2ddiff read.
+ */
+extern "C" int pt_bicubic_precompute(int m, int n, const double *rg, const double *zg, int ncomp,
+                                     const double *values, double *coef) {
+    if (m < 2 || n < 2 || ncomp < 1 || !rg || !zg || !values || !coef)
+        return fail(PT_EINVAL, "pt_bicubic_precompute: bad arguments");
+    for (int i = 1; i < m; ++i)
+        if (!(rg[i] > rg[i - 1])) return fail(PT_EINVAL, "pt_bicubic_precompute: r grid not strictly increasing");
+    for (int i = 1; i < n; ++i)
+        if (!(zg[i] > zg[i - 1])) return fail(PT_EINVAL, "pt_bicubic_precompute: z grid not strictly increasing");
+    const size_t np = (size_t)m * n;
+    std::vector<double> dr(np), dz(np), drz(np), line(std::max(m, n)), out(std::max(m, n)), d, buf;
+    for (int comp = 0; comp < ncomp; ++comp) {
+        const double *a = values + comp * np;        /* a[iz * m + ir] */
+        for (int iz = 0; iz < n; ++iz) {
+            spline_node_diff(rg, a + (size_t)iz * m, m, out.data(), d, buf);
+            for (int ir = 0; ir < m; ++ir) dr[(size_t)iz * m + ir] = out[ir];
+        }
+        for (int ir = 0; ir < m; ++ir) {
+            for (int iz = 0; iz < n; ++iz) line[iz] = a[(size_t)iz * m + ir];
+            spline_node_diff(zg, line.data(), n, out.data(), d, buf);
+            for (int iz = 0; iz < n; ++iz) dz[(size_t)iz * m + ir] = out[iz];
+        }
+        for (int iz = 0; iz < n; ++iz) {
+            spline_node_diff(rg, dz.data() + (size_t)iz * m, m, out.data(), d, buf);
+            for (int ir = 0; ir < m; ++ir) drz[(size_t)iz * m + ir] = out[ir];
+        }
+        const double *tab[4] = {a, dr.data(), dz.data(), drz.data()};
+        for (int iz = 0; iz < n - 1; ++iz)
+            for (int ir = 0; ir < m - 1; ++ir) {
+                double *c = coef + (((size_t)iz * (m - 1) + ir) * ncomp + comp) * 16;
+                const size_t s1 = (size_t)iz * m + ir, s2 = s1 + 1, s3 = s1 + m, s4 = s3 + 1;
+                for (int k = 0; k < 4; ++k) {
+                    c[4 * k + 0] = tab[k][s1];
+                    c[4 * k + 1] = tab[k][s2];
+                    c[4 * k + 2] = tab[k][s3];
+                    c[4 * k + 3] = tab[k][s4];
+                }
+            }
+    }
     return PT_OK;
 }
 
@@ -1024,6 +1139,7 @@ extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_s
     ptd::Params &P = B->P;
     P.geom = g->g;
     P.n_fields = f->n;
+    P.b_reuse = f->b_reuse ? 1 : 0;
     P.fields = f->dev;
     P.species = B->d_species;
     P.tmax_global = tmax_global;
@@ -1163,22 +1279,23 @@ extern "C" int pt_batch_secondaries(pt_batch *b, void *stream, int cap, pt_secon
 /* TFieldManager::BField / EField at n points and times (host arrays), FP64 tables */
 extern "C" int pt_fields_eval(const pt_fields *f, void *stream, int n, const double *pts, const double *t, double *B,
                               double *dB, double *V, double *E) {
-    if (!f || n < 0 || (n > 0 && (!pts || !t || !B || !dB || !V || !E))) return fail(PT_EINVAL, "pt_fields_eval: bad arguments");
+    if (!f || n < 0 || (n > 0 && (!pts || !t || !B || !V || !E))) return fail(PT_EINVAL, "pt_fields_eval: bad arguments");
     if (n == 0) return PT_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     DevMem mem;
     double *dp, *dt, *dB_, *ddB, *dV, *dE;
     int r;
     if ((r = mem.upload(&dp, pts, 3 * (size_t)n)) || (r = mem.upload(&dt, t, (size_t)n)) || (r = mem.alloc(&dB_, 3 * (size_t)n)) ||
-        (r = mem.alloc(&ddB, 9 * (size_t)n)) || (r = mem.alloc(&dV, (size_t)n)) || (r = mem.alloc(&dE, 3 * (size_t)n)))
+        (dB && (r = mem.alloc(&ddB, 9 * (size_t)n))) || (r = mem.alloc(&dV, (size_t)n)) || (r = mem.alloc(&dE, 3 * (size_t)n)))
         return r;
+    if (!dB) ddB = nullptr;
     ptd::Params P{};
     P.n_fields = f->n;
     P.fields = f->dev;
     PT_HIP(pt_launch_fields_eval(P, f->general ? 1 : 0, n, dp, dt, dB_, ddB, dV, dE, s));
     PT_HIP(hipStreamSynchronize(s));
     PT_HIP(hipMemcpy(B, dB_, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
-    PT_HIP(hipMemcpy(dB, ddB, 9 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    if (dB) PT_HIP(hipMemcpy(dB, ddB, 9 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     PT_HIP(hipMemcpy(V, dV, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     PT_HIP(hipMemcpy(E, dE, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     return PT_OK;

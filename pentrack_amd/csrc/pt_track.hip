@@ -336,6 +336,129 @@ PTD void analytic_B(const Field &f, double x, double y, double z, double *B, dou
     }
 }
 
+/* Axisymmetric maps (TabField, src/field_2d.cpp:289-400) with ALGLIB's bicubic splines.
+   The cell is ALGLIB's bisection result: the last node below the coordinate, clamped to
+   [0, n-2] (found from a uniform-grid guess).  Value-only evaluations follow spline2dcalc
+   (t = (x - x_l) * (1/h)), evaluations with derivatives spline2ddiff (t = (x - x_l) / h),
+   as the reference calls one or the other; sums in ALGLIB's order. */
+PTD int bic_cell(const double *g, int n, double x0, double inv_h, double v) {
+    const double gd = (v - x0) * inv_h;
+    int l = gd > 0 ? (gd < (double)(n - 2) ? (int)gd : n - 2) : 0;
+    while (l > 0 && !(g[l] < v)) --l;
+    while (l < n - 2 && g[l + 1] < v) ++l;
+    return l;
+}
+
+struct BicAt { double t, u, dt, du; size_t cell; };
+
+PTD BicAt bic_locate(const Field &f, double r, double z, bool diff) {
+    BicAt b;
+    const int ir = bic_cell(f.g[0], f.nx, f.x0[0], f.inv_h[0], r);
+    const int iz = bic_cell(f.g[2], f.nz, f.x0[2], f.inv_h[2], z);
+    const double r0 = f.g[0][ir], hr = f.g[0][ir + 1] - r0;
+    const double z0 = f.g[2][iz], hz = f.g[2][iz + 1] - z0;
+    b.dt = 1.0 / hr;
+    b.du = 1.0 / hz;
+    if (diff) { b.t = (r - r0) / hr; b.u = (z - z0) / hz; }
+    else { b.t = (r - r0) * b.dt; b.u = (z - z0) * b.du; }
+    b.cell = (size_t)iz * (f.nx - 1) + ir;
+    return b;
+}
+
+/* one component: c[16] = f, df/dr, df/dz, d2f/drdz at the corners s1..s4 */
+PTD void bic_eval(const double *c, const BicAt &b, bool diff, double &F, double &Fr, double &Fz) {
+    const double t = b.t, u = b.u;
+    const double t2 = t * t, t3 = t * t2, u2 = u * u, u3 = u * u2;
+    const double ht00 = 2 * t3 - 3 * t2 + 1, ht01 = -2 * t3 + 3 * t2;
+    const double ht10 = (t3 - 2 * t2 + t) / b.dt, ht11 = (t3 - t2) / b.dt;
+    const double hu00 = 2 * u3 - 3 * u2 + 1, hu01 = -2 * u3 + 3 * u2;
+    const double hu10 = (u3 - 2 * u2 + u) / b.du, hu11 = (u3 - u2) / b.du;
+    F = 0.0 + c[0] * ht00 * hu00 + c[1] * ht01 * hu00 + c[2] * ht00 * hu01 + c[3] * ht01 * hu01;
+    F = F + c[4] * ht10 * hu00 + c[5] * ht11 * hu00 + c[6] * ht10 * hu01 + c[7] * ht11 * hu01;
+    F = F + c[8] * ht00 * hu10 + c[9] * ht01 * hu10 + c[10] * ht00 * hu11 + c[11] * ht01 * hu11;
+    F = F + c[12] * ht10 * hu10 + c[13] * ht11 * hu10 + c[14] * ht10 * hu11 + c[15] * ht11 * hu11;
+    if (!diff) return;
+    const double dht00 = (6 * t2 - 6 * t) * b.dt, dht01 = (-6 * t2 + 6 * t) * b.dt;
+    const double dht10 = 3 * t2 - 4 * t + 1, dht11 = 3 * t2 - 2 * t;
+    const double dhu00 = (6 * u2 - 6 * u) * b.du, dhu01 = (-6 * u2 + 6 * u) * b.du;
+    const double dhu10 = 3 * u2 - 4 * u + 1, dhu11 = 3 * u2 - 2 * u;
+    Fr = 0.0 + c[0] * dht00 * hu00 + c[1] * dht01 * hu00 + c[2] * dht00 * hu01 + c[3] * dht01 * hu01;
+    Fr = Fr + c[4] * dht10 * hu00 + c[5] * dht11 * hu00 + c[6] * dht10 * hu01 + c[7] * dht11 * hu01;
+    Fr = Fr + c[8] * dht00 * hu10 + c[9] * dht01 * hu10 + c[10] * dht00 * hu11 + c[11] * dht01 * hu11;
+    Fr = Fr + c[12] * dht10 * hu10 + c[13] * dht11 * hu10 + c[14] * dht10 * hu11 + c[15] * dht11 * hu11;
+    Fz = 0.0 + c[0] * ht00 * dhu00 + c[1] * ht01 * dhu00 + c[2] * ht00 * dhu01 + c[3] * ht01 * dhu01;
+    Fz = Fz + c[4] * ht10 * dhu00 + c[5] * ht11 * dhu00 + c[6] * ht10 * dhu01 + c[7] * ht11 * dhu01;
+    Fz = Fz + c[8] * ht00 * dhu10 + c[9] * ht01 * dhu10 + c[10] * ht00 * dhu11 + c[11] * ht01 * dhu11;
+    Fz = Fz + c[12] * ht10 * dhu10 + c[13] * ht11 * dhu10 + c[14] * ht10 * dhu11 + c[15] * ht11 * dhu11;
+}
+
+PTD bool tab2d_in_range(const Field &f, double r, double z) {
+    return r >= f.p[0] && r < f.p[1] && z >= f.p[2] && z < f.p[3];
+}
+
+/* TabField::BField (src/field_2d.cpp:289-337): B and dB (row-major) zeroed first */
+PTD void tab2d_B(const Field &f, double x, double y, double z, double *B, double *dB, bool grad, long long &nfield) {
+    for (int i = 0; i < 3; ++i) B[i] = 0;
+    if (grad)
+        for (int i = 0; i < 9; ++i) dB[i] = 0;
+    const double r = __builtin_sqrt(x * x + y * y);
+    if (!tab2d_in_range(f, r, z)) return;
+    ++nfield;
+    const double phi = ptm_atan2(y, x);
+    const double cp = ptm_cos(phi), sp = ptm_sin(phi);
+    const BicAt b = bic_locate(f, r, z, grad);
+    const double *c = f.coef + b.cell * f.ncomp * 16;
+    double Br = 0, Bphi = 0, Bz = 0, dBrdr = 0, dBrdz = 0, dBphidr = 0, dBphidz = 0, dBzdr = 0, dBzdz = 0;
+    if (f.has_b & 1) { bic_eval(c, b, grad, Br, dBrdr, dBrdz); c += 16; }
+    if (f.has_b & 2) { bic_eval(c, b, grad, Bphi, dBphidr, dBphidz); c += 16; }
+    if (f.has_b & 4) bic_eval(c, b, grad, Bz, dBzdr, dBzdz);
+    if (grad) {
+        if (r > 0) {
+            dB[0] = dBrdr * cp * cp - dBphidr * cp * sp + (Br * sp * sp + Bphi * cp * sp) / r;
+            dB[1] = dBrdr * cp * sp - dBphidr * sp * sp - (Br * cp * sp + Bphi * cp * cp) / r;
+            dB[3] = dBrdr * cp * sp + dBphidr * cp * cp - (Br * cp * sp - Bphi * sp * sp) / r;
+            dB[4] = dBrdr * sp * sp + dBphidr * cp * sp + (Br * cp * cp - Bphi * cp * sp) / r;
+        }
+        dB[2] = dBrdz * cp - dBphidz * sp;           /* CylToCart (src/field_2d.cpp:32-35) */
+        dB[5] = dBrdz * sp + dBphidz * cp;
+        if (f.has_b & 4) { dB[6] = dBzdr * cp; dB[7] = dBzdr * sp; dB[8] = dBzdz; }
+    }
+    B[0] = Br * cp - Bphi * sp;
+    B[1] = Br * sp + Bphi * cp;
+    B[2] = Bz;
+}
+
+/* TabField::EField (src/field_2d.cpp:340-400): a tabulated E (value-only splines) takes
+   precedence over the potential (splines with derivatives, E = -grad V) */
+PTD void tab2d_E(const Field &f, double x, double y, double z, double &V, double E[3], long long &nfield) {
+    V = 0; E[0] = E[1] = E[2] = 0;
+    const double r = __builtin_sqrt(x * x + y * y);
+    if (!tab2d_in_range(f, r, z)) return;
+    ++nfield;
+    const double *c0 = f.coef + __builtin_popcount((unsigned)f.has_b) * 16;
+    if (f.has_v & 14) {
+        const double phi = ptm_atan2(y, x);
+        const BicAt b = bic_locate(f, r, z, false);
+        const double *c = c0 + b.cell * f.ncomp * 16;
+        double Er = 0, Ephi = 0, Ez = 0, dummy;
+        if (f.has_v & 2) { bic_eval(c, b, false, Er, dummy, dummy); c += 16; }
+        if (f.has_v & 4) { bic_eval(c, b, false, Ephi, dummy, dummy); c += 16; }
+        if (f.has_v & 8) bic_eval(c, b, false, Ez, dummy, dummy);
+        const double cp = ptm_cos(phi), sp = ptm_sin(phi);
+        E[0] = Er * cp - Ephi * sp;
+        E[1] = Er * sp + Ephi * cp;
+        E[2] = Ez;
+    } else if (f.has_v & 1) {
+        const BicAt b = bic_locate(f, r, z, true);
+        double dVdr = 0, dVdz = 0;
+        bic_eval(c0 + b.cell * f.ncomp * 16, b, true, V, dVdr, dVdz);
+        const double phi = ptm_atan2(y, x);
+        E[0] = -dVdr * ptm_cos(phi);
+        E[1] = -dVdr * ptm_sin(phi);
+        E[2] = -dVdz;
+    }
+}
+
 /* TFieldContainer::BField (src/field.cpp:139-153) with the scale factor bs already
    evaluated, accumulated into the manager sum (TFieldManager::BField, src/fields.cpp:139-164);
    tricubic maps and, through analytic_B, the analytic fields and conductors */
@@ -347,7 +470,8 @@ PTD void container_B(const Field &f, double x, double y, double z, double bs, do
     if (in_bounds(f, x, y, z) && bs != 0.) {
         if ((MODE & MODE_FIELDS) && f.type >= PT_FIELD_CONDUCTOR) {
             double Ba[3], dBa[9];
-            analytic_B(f, x, y, z, Ba, dBa, grad);
+            if (f.type == PT_FIELD_TAB2D) tab2d_B(f, x, y, z, Ba, dBa, grad, nfield);
+            else analytic_B(f, x, y, z, Ba, dBa, grad);
             Bt[0] = Ba[0]; Bt[1] = Ba[1]; Bt[2] = Ba[2];
             if (grad) {
                 dBt[0][0] = dBa[0]; dBt[0][1] = dBa[1]; dBt[0][2] = dBa[2];
@@ -420,6 +544,8 @@ PTD void container_E(const Field &f, double x, double y, double z, double es, do
         if (f.type == 2) {
             Et[0] = f.e[0]; Et[1] = f.e[1]; Et[2] = f.e[2];
             Vt = Et[2] * z;
+        } else if ((MODE & MODE_FIELDS) && f.type == PT_FIELD_TAB2D) {
+            tab2d_E(f, x, y, z, Vt, Et, nfield);
         } else if (f.type == 1 && f.has_v) {
             long cell; double r[3], dist[3];
             double dV[3] = {0., 0., 0.};
@@ -593,8 +719,11 @@ PTD void derivs8(const Params &P, const Species &sp, const double *s, double t, 
     double dB[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, E[3] = {0, 0, 0};
     B[0] = 0; B[1] = 0; B[2] = 0; V = 0;
     ++c.rhs;
-    hasB = (sp.q != 0 || (sp.mu != 0 && pol != 0));
-    if (hasB) field_B<MODE>(P, y[0], y[1], y[2], t, B, dB, true, c.field);
+    const bool needB = (sp.q != 0 || (sp.mu != 0 && pol != 0));
+    if (needB) field_B<MODE>(P, y[0], y[1], y[2], t, B, dB, true, c.field);
+    /* B may stand in for later value-only evaluations at this point unless a 2D map
+       computes those differently (TabField: spline2dcalc vs spline2ddiff) */
+    hasB = needB && (!(MODE & MODE_FIELDS) || P.b_reuse);
     if (sp.q != 0) field_E<MODE>(P, y[0], y[1], y[2], t, &V, E, c.field);
     double d9[9];
     eom(sp, y, B, dB, E, d9);
@@ -1866,12 +1995,14 @@ __global__ void __launch_bounds__(256) fields_eval_kernel(Params P, int n, const
     long long nf = 0;
     double b[3], db[3][3], v, e[3];
     const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
-    field_B<MODE>(P, x, y, z, t[i], b, db, true, nf);
+    const bool grad = dB != nullptr;       /* without dB: BField(x, y, z, t, B), value-only */
+    field_B<MODE>(P, x, y, z, t[i], b, db, grad, nf);
     field_E<MODE>(P, x, y, z, t[i], &v, e, nf);
     for (int k = 0; k < 3; ++k) {
         B[3 * i + k] = b[k];
         E[3 * i + k] = e[k];
-        for (int j = 0; j < 3; ++j) dB[9 * i + 3 * k + j] = db[k][j];
+        if (grad)
+            for (int j = 0; j < 3; ++j) dB[9 * i + 3 * k + j] = db[k][j];
     }
     V[i] = v;
 }

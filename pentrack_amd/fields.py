@@ -11,6 +11,8 @@ path:
   * TConductorField -- finite straight wire (src/conductor.cpp)
   * TExponentialFieldX, TLinearFieldZ, TB0GradZ, TB0GradX2, TB0GradXY, TB0_XY
     (src/analyticFields.cpp)
+  * TabField -- axisymmetric r-z maps with ALGLIB's bicubic splines (src/field_2d.cpp);
+    the per-cell Hermite data come from pt_bicubic_precompute (host C++)
 Coil fields used to fill the synthetic tables are computed here on the host.
 """
 import ctypes as C
@@ -43,6 +45,8 @@ class FieldContainer:
     # exprtk formulas of t replacing bscale / escale
     bformula: Optional[str] = None
     eformula: Optional[str] = None
+    # 2D maps: the samples [ncomp][nz][nr] the coefficients were made from
+    values: Optional[np.ndarray] = None
 
     def __post_init__(self):
         if len(self.params) > 8:
@@ -76,6 +80,41 @@ def tabfield3(xg, yg, zg, B=None, V=None, bscale=1.0, escale=0.0, box=(0, 0, 0, 
     return FieldContainer(type=_native.PT_FIELD_TRICUBIC, bscale=bs, escale=es, bformula=bf, eformula=ef,
                           box=tuple(box), bw=bw,
                           grid=(xg, yg, zg), has_b=int(B is not None), has_v=int(V is not None), coef=coef)
+
+
+TAB2D_B = ("Br", "Bphi", "Bz")
+TAB2D_V = ("V", "Er", "Ephi", "Ez")
+
+
+def tabfield2(rg, zg, bscale=1.0, escale=1.0, **cols):
+    """TabField container (src/field_2d.cpp) of an axisymmetric map: samples on an r-z grid
+    (rg, zg strictly increasing), each column given as [len(zg)][len(rg)] by name (Br,
+    Bphi, Bz in T; V in V or Er, Ephi, Ez in V/m).  A tabulated E replaces the potential
+    (src/field_2d.cpp:262-284).  No bounds box: TFieldContainer(field, Bscale, Escale)
+    (src/field_2d.cpp:68)."""
+    rg, zg = np.asarray(rg, dtype=np.float64), np.asarray(zg, dtype=np.float64)
+    unknown = set(cols) - set(TAB2D_B + TAB2D_V)
+    if unknown:
+        raise ValueError(f"unknown 2D table columns {sorted(unknown)}")
+    if any(cols.get(k) is not None for k in ("Er", "Ephi", "Ez")):
+        cols = dict(cols, V=None)
+    names = [k for k in TAB2D_B + TAB2D_V if cols.get(k) is not None]
+    if not names:
+        raise ValueError("tabfield2 needs at least one column")
+    has_b = sum(1 << i for i, k in enumerate(TAB2D_B) if cols.get(k) is not None)
+    has_v = sum(1 << i for i, k in enumerate(TAB2D_V) if cols.get(k) is not None)
+    values = np.ascontiguousarray(np.stack([np.asarray(cols[k], dtype=np.float64).reshape(len(zg), len(rg))
+                                            for k in names]))
+    m, n = len(rg), len(zg)
+    coef = np.empty(((n - 1) * (m - 1), len(names), 16))
+    rc = _native.lib().pt_bicubic_precompute(m, n, _native.dptr(np.ascontiguousarray(rg)),
+                                             _native.dptr(np.ascontiguousarray(zg)), len(names),
+                                             _native.dptr(values), _native.dptr(coef))
+    _native.check(rc, "pt_bicubic_precompute")
+    bs, bf = _scale(bscale)
+    es, ef = _scale(escale)
+    return FieldContainer(type=_native.PT_FIELD_TAB2D, bscale=bs, escale=es, bformula=bf, eformula=ef,
+                          grid=(rg, None, zg), has_b=has_b, has_v=has_v, coef=coef, values=values)
 
 
 def tricubic_coefficients(xg, yg, zg, values, threads=None):
@@ -213,6 +252,14 @@ class TFieldManager:
                 keep.extend([xg, yg, zg, coef])
                 d.nx, d.ny, d.nz = len(xg), len(yg), len(zg)
                 d.xg, d.yg, d.zg = _native.dptr(xg), _native.dptr(yg), _native.dptr(zg)
+                d.has_b, d.has_v = c.has_b, c.has_v
+                d.coef = _native.dptr(coef)
+            elif c.type == _native.PT_FIELD_TAB2D:
+                rg, zg = (np.ascontiguousarray(c.grid[i], dtype=np.float64) for i in (0, 2))
+                coef = np.ascontiguousarray(c.coef, dtype=np.float64)
+                keep.extend([rg, zg, coef])
+                d.nx, d.ny, d.nz = len(rg), 1, len(zg)
+                d.xg, d.zg = _native.dptr(rg), _native.dptr(zg)
                 d.has_b, d.has_v = c.has_b, c.has_v
                 d.coef = _native.dptr(coef)
         return ds, keep

@@ -51,8 +51,8 @@ def end_values(kind, t0, y0, spin0, out, geometry, fields, jobnumber=0, first_nu
     n = len(out)
     yend = out["y"]
     tend = out["t"]
-    Bs, _, Vs, _ = fields_eval(fields, y0[:, :3], t0)
-    Be, _, Ve, _ = fields_eval(fields, yend[:, :3], tend)
+    Bs, _, Vs, _ = fields_eval(fields, y0[:, :3], t0, grad=False)      # BField(x, y, z, t, B) (src/logger.cpp:55,63)
+    Be, _, Ve, _ = fields_eval(fields, yend[:, :3], tend, grad=False)
     sid_s = solid_at(geometry, y0[:, :3], t0)
     sid_e = solid_at(geometry, yend[:, :3], tend)
     mats = np.stack([m.as_array() for m in geometry.materials.values()])

@@ -6,17 +6,18 @@ import numpy as np
 from . import _native
 
 
-def fields_eval(fields, pts, t, stream=None):
+def fields_eval(fields, pts, t, stream=None, grad=True):
     """TFieldManager::BField / EField at pts [n][3], times t (scalar or [n]): B [n][3],
-    dB [n][3][3], V [n], E [n][3]."""
+    dB [n][3][3], V [n], E [n][3].  grad False: the value-only BField(x, y, z, t, B) the
+    reference uses for energies and logs, dB returned as None."""
     pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
     n = len(pts)
     t = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (n,)))
     B, dB, V, E = np.zeros((n, 3)), np.zeros((n, 3, 3)), np.zeros(n), np.zeros((n, 3))
     d = _native.dptr
-    _native.check(_native.lib().pt_fields_eval(fields.device(), stream, n, d(pts), d(t), d(B), d(dB), d(V), d(E)),
-                  "pt_fields_eval")
-    return B, dB, V, E
+    _native.check(_native.lib().pt_fields_eval(fields.device(), stream, n, d(pts), d(t), d(B), d(dB) if grad else None,
+                                               d(V), d(E)), "pt_fields_eval")
+    return B, dB if grad else None, V, E
 
 
 def solid_at(geometry, pts, t, stream=None, leaf_size=4):

@@ -1,6 +1,7 @@
-"""Readers of tabulated 3D field maps: the OPERA3D (and older 3Dtable) and COMSOL text
-formats of the FIELDS section (src/field_3d.cpp:124-310), plain or compressed with gzip
-(`.gz`) or bzip2 (`.bz2`), chosen by the file name's extension as the reference does.
+"""Readers of tabulated field maps: the OPERA3D (and older 3Dtable) and COMSOL text
+formats of 3D maps (src/field_3d.cpp:124-310) and the OPERA2D (and older 2Dtable) format
+of axisymmetric r-z maps (src/field_2d.cpp:38-196), plain or compressed with gzip (`.gz`)
+or bzip2 (`.bz2`), chosen by the file name's extension as the reference does.
 
 Both build the TabField3 container the reference builds (src/field_3d.cpp:417-470):
 the grid is the sorted list of distinct x, y and z coordinates, each sample goes to the
@@ -17,7 +18,7 @@ import re
 
 import numpy as np
 
-from .fields import tabfield3
+from .fields import tabfield2, tabfield3
 
 _NUM = re.compile(r"\s*([-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?)")
 _INT = re.compile(r"\s*([-+]?\d+)")
@@ -207,3 +208,89 @@ def read_opera3d(fieldtype, args, formulas, base_dir):
             col += 1
     V = a[:, col] if has_v else None
     return _container(x, y, z, comps, V, bscale, escale, bw)
+
+
+def read_opera2d(fieldtype, args, formulas, base_dir):
+    """ReadOperaField2 and TabField::ReadTabFile (src/field_2d.cpp:38-196).
+
+    `OPERA2D file Bscale Escale lengthconv`, or `2Dtable file Bscale Escale` in cm, Gauss
+    and V.  The file starts with `m 1 n` (radial and axial sizes) and column descriptions:
+    X, then Z (or Y and Z: a third line " 2 Y [LENGU]" adds a column that is skipped), then
+    lines naming RBX, RBY, RBZ, EX, EY, EZ, RV open the Br, Bphi, Bz, Er, Ephi, Ez and V
+    columns, closed by a line beginning " 0".  Rows run over z for each r: a z smaller than
+    the previous one starts the next r.
+    """
+    from .config import resolve_formula
+    if fieldtype == "2Dtable":
+        args = _params(args, 3, fieldtype)
+        bscale = "(" + resolve_formula(args[1], formulas) + ")*0.0001"
+        escale = "(" + resolve_formula(args[2], formulas) + ")*100"
+        lconv = 0.01
+    elif fieldtype == "OPERA2D":
+        args = _params(args, 4, fieldtype)
+        bscale, escale = resolve_formula(args[1], formulas), resolve_formula(args[2], formulas)
+        lconv = _number(args[3], fieldtype)
+    else:
+        raise TableError(f"Tried to load 2D table file for unknown field type {fieldtype}!")
+    path = os.path.join(base_dir, args[0])
+    text = open_table(path)
+    pos = 0
+    sizes = []
+    for _ in range(3):
+        mm = _INT.match(text, pos)
+        if not mm:
+            raise TableError(f"{path} not found or corrupt!")
+        sizes.append(int(mm.group(1)))
+        pos = mm.end()
+    m, n = sizes[0], sizes[2]
+
+    def getline():
+        nonlocal pos
+        e = text.find("\n", pos)
+        if e < 0:
+            line, pos = text[pos:], len(text)
+        else:
+            line, pos = text[pos:e], e + 1
+        return line
+
+    for _ in range(3):
+        line = getline()
+    with_y = line[:12] == " 2 Y [LENGU]"
+    line = getline()
+    if with_y:
+        line = getline()
+    present = []
+    for key, name in (("RBX", "Br"), ("RBY", "Bphi"), ("RBZ", "Bz"), ("EX", "Er"), ("EY", "Ephi"), ("EZ", "Ez"),
+                      ("RV", "V")):
+        if key in line:
+            present.append(name)
+            line = getline()
+    if line[:2] != " 0":
+        raise TableError(f"{path} not found or corrupt!")
+    ncoord = 3 if with_y else 2
+    ncol = ncoord + len(present)
+    vals = _to_doubles(text[pos:].split())
+    if len(vals) % ncol or not len(vals):
+        raise TableError(f"{path}: incomplete rows")
+    a = vals.reshape(-1, ncol)
+    r = a[:, 0] * lconv
+    z = a[:, ncoord - 1] * lconv
+    new_r = np.concatenate([[False], z[1:] < z[:-1]])
+    ri = np.cumsum(new_r)
+    starts = np.flatnonzero(np.concatenate([[True], new_r[1:]]))
+    zi = np.arange(len(z)) - starts[ri]
+    if ri[-1] + 1 != m or zi[-1] + 1 != n or zi.max() >= n or ri.max() >= m or len(z) != m * n:
+        raise TableError(f"The header says the size is {m} by {n}, actually it is {ri[-1] + 1} by {zi[-1] + 1}! "
+                         "Exiting...")
+    rind = np.empty(m)
+    zind = np.empty(n)
+    rind[ri] = r                       # the last row of each r block, as the reference assigns
+    zind[zi] = z
+    cols = {}
+    for k, name in enumerate(present):
+        tab = np.zeros((n, m))
+        tab[zi, ri] = a[:, ncoord + k]
+        cols[name] = tab
+    if not (np.diff(rind) > 0).all() or not (np.diff(zind) > 0).all():
+        raise TableError(f"{path}: r and z must increase through the table")
+    return tabfield2(rind, zind, bscale=bscale, escale=escale, **cols)
