@@ -3,13 +3,14 @@ formats of 3D maps (src/field_3d.cpp:124-310) and the OPERA2D (and older 2Dtable
 of axisymmetric r-z maps (src/field_2d.cpp:38-196), plain or compressed with gzip (`.gz`)
 or bzip2 (`.bz2`), chosen by the file name's extension as the reference does.
 
-Both build the TabField3 container the reference builds (src/field_3d.cpp:417-470):
-the grid is the sorted list of distinct x, y and z coordinates, each sample goes to the
-grid point its coordinates name (lower_bound), grid points without a sample keep zero
-field, and the container's bounding box is the extent of the samples with the boundary
-width of the config line.  Field components missing from an OPERA3D file are tabulated
-as zeros: the reference leaves such a component at the zero the field sum starts from,
-which gives the same values and derivatives.
+The 3D readers build the TabField3 container the reference builds (src/field_3d.cpp:
+417-470): the grid is the sorted list of distinct x, y and z coordinates, each sample goes
+to the grid point its coordinates name (lower_bound), grid points without a sample keep
+zero field, and the container's bounding box is the extent of the samples with the
+boundary width of the config line.  Field components missing from an OPERA3D file are
+tabulated as zeros: the reference leaves such a component at the zero the field sum
+starts from, which gives the same values and derivatives.  The 2D reader builds a
+TabField container (pentrack_amd.fields.tabfield2) without bounds box.
 """
 import bz2
 import gzip
