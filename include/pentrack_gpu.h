@@ -114,6 +114,9 @@ typedef struct {
        the compiler does not reproduce (chains of + - or * / with constants or divisions,
        literal 0 operands of + * / on sub-expressions, '%', '&&', '||', other functions). */
     const char *bscale_formula, *escale_formula;
+    /* PT_FIELD_TRICUBIC with coef == NULL: the samples [ncomp][nx][ny][nz] (host memory),
+       pre-interpolated on the GPU by pt_fields_create (bit-identical to pt_tricubic_precompute) */
+    const double *values;
 } pt_field_desc;
 
 typedef struct pt_fields pt_fields;
@@ -136,6 +139,11 @@ void pt_fields_destroy(pt_fields *f);
  */
 int pt_tricubic_precompute(int nx, int ny, int nz, const double *xg, const double *yg, const double *zg,
                            int ncomp, const double *values, double *coef, int threads);
+/* pt_tricubic_precompute on the GPU (seven batched spline-derivative passes, one thread per
+   grid line, then the 64 x 64 products per cell); host arrays in and out, same layout and
+   bit-identical results; synchronous */
+int pt_tricubic_precompute_gpu(int nx, int ny, int nz, const double *xg, const double *yg, const double *zg,
+                               int ncomp, const double *values, double *coef);
 /*
  * TabField's bicubic splines (src/field_2d.cpp:235-285: ALGLIB spline2dbuildbicubicv, the
  * derivative tables of spline2d_bicubiccalcderivatives).

@@ -179,7 +179,11 @@ class Problem:
                 f.bic = _dp(bic)
             elif c.grid is not None:
                 xg, yg, zg = (np.ascontiguousarray(g_, dtype=np.float64) for g_ in c.grid)
-                coef = np.ascontiguousarray(c.coef, dtype=np.float64)
+                if c.coef is None:        # a map kept as samples: the host pre-interpolation
+                    from pentrack_amd.fields import tricubic_coefficients
+                    coef = tricubic_coefficients(xg, yg, zg, c.values)
+                else:
+                    coef = np.ascontiguousarray(c.coef, dtype=np.float64)
                 self._keep.extend([xg, yg, zg, coef])
                 f.nx, f.ny, f.nz = len(xg), len(yg), len(zg)
                 f.xg, f.yg, f.zg = _dp(xg), _dp(yg), _dp(zg)

@@ -54,6 +54,7 @@ class FieldDesc(C.Structure):
         ("e", C.c_double * 3),
         ("p", C.c_double * 8),
         ("bscale_formula", C.c_char_p), ("escale_formula", C.c_char_p),
+        ("values", C.POINTER(C.c_double)),
     ]
 
 
@@ -131,6 +132,7 @@ EXPORTS = [
     "pt_spline_grid_diff", "pt_rk4_forward_ckpt", "pt_rk4_adjoint", "pt_formula_eval",
     "pt_batch_secondaries", "pt_proton_beta_table", "pt_neutron_decay_host", "pt_beta_sample",
     "pt_distribution_table", "pt_fields_eval", "pt_geometry_solid_at", "pt_bicubic_precompute",
+    "pt_tricubic_precompute_gpu",
 ]
 
 _lib = None
@@ -163,6 +165,9 @@ def lib():
                                          C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), C.c_int]
     L.pt_tricubic_matrix.argtypes = [ip]
+    L.pt_tricubic_precompute_gpu.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double)]
     L.pt_bicubic_precompute.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pt_batch_create.argtypes = [vp, vp, C.c_int, C.POINTER(Species), C.c_double, C.c_int, ip,

@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpentrack_gpu.so")
 
-SOURCES = ["pt_track.hip", "pt_adjoint.hip", "pt_host.cpp", "pt_formula.cpp"]
+SOURCES = ["pt_track.hip", "pt_adjoint.hip", "pt_precompute.hip", "pt_host.cpp", "pt_formula.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
     # bit-reproducible FP64: no contraction, no reassociation

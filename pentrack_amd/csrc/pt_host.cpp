@@ -441,6 +441,11 @@ struct pt_fields {
     bool b_reuse = true;    /* no 2D map tabulates B (value-only B equals B with derivatives) */
 };
 
+namespace {
+int tricubic_on_device(int nx, int ny, int nz, const double *const gh[3], int ncomp, const double *values_h,
+                       int device_order, double *coef_dev, float *coef32_dev);
+}
+
 extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32, pt_fields **out) {
     if (!out || n < 0 || (n > 0 && !descs)) return fail(PT_EINVAL, "pt_fields_create: bad arguments");
     std::unique_ptr<pt_fields> F(new pt_fields());
@@ -492,7 +497,8 @@ extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32
             f.p[7] = (double)(mu0 * (long double)d.p[0] / (4 * pi));
         }
         if (d.type == PT_FIELD_TRICUBIC) {
-            if (d.nx < 2 || d.ny < 2 || d.nz < 2 || !d.xg || !d.yg || !d.zg || !d.coef || (!d.has_b && !d.has_v))
+            if (d.nx < 2 || d.ny < 2 || d.nz < 2 || !d.xg || !d.yg || !d.zg || (!d.coef && !d.values) ||
+                (!d.has_b && !d.has_v))
                 return fail(PT_EINVAL, "pt_fields_create: bad tricubic descriptor");
             const double *gs[3] = {d.xg, d.yg, d.zg};
             int ns[3] = {d.nx, d.ny, d.nz};
@@ -511,6 +517,21 @@ extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32
             f.has_v = d.has_v ? 1 : 0;
             f.ncomp = 3 * f.has_b + f.has_v;
             size_t ncoef = (size_t)(d.nx - 1) * (d.ny - 1) * (d.nz - 1) * f.ncomp * 64;
+            if (!d.coef) {
+                /* samples only: pre-interpolate on the device, straight into the tracker's order */
+                double *dc;
+                float *dc32 = nullptr;
+                int r = F->mem.alloc(&dc, ncoef);
+                if (r) return r;
+                if (with_fp32 && (r = F->mem.alloc(&dc32, ncoef))) return r;
+                r = tricubic_on_device(d.nx, d.ny, d.nz, gs, f.ncomp, d.values, 1, dc, dc32);
+                if (r) return r;
+                f.coef = dc;
+                f.coef32 = dc32;
+                F->general = F->general || f.blen > 0 || f.elen > 0;
+                F->host[k] = f;
+                continue;
+            }
             /* libtricubic order a[i + 4j + 16k] (coefficient of x^i y^j z^k) becomes
                a[16i + 4j + k]: the 16 coefficients of one power of x are contiguous */
             std::vector<double> cdev(ncoef);
@@ -760,6 +781,69 @@ extern "C" int pt_tricubic_precompute(int nx, int ny, int nz, const double *xg, 
             for (auto &t : th) t.join();
         }
     }
+    return PT_OK;
+}
+
+namespace {
+
+/* TabField3::PreInterpol on the device for samples values_h [ncomp][nx][ny][nz] (host) into
+   coef_dev [cells][ncomp][64] (and coef32_dev if not NULL); synchronous */
+int tricubic_on_device(int nx, int ny, int nz, const double *const gh[3], int ncomp, const double *values_h,
+                       int device_order, double *coef_dev, float *coef32_dev) {
+    DevMem tmp;
+    const size_t np = (size_t)nx * ny * nz;
+    const int n[3] = {nx, ny, nz};
+    double *g[3], *der[8], *scr, *A;
+    double **der_dev;
+    int r;
+    for (int a = 0; a < 3; ++a)
+        if ((r = tmp.upload(&g[a], gh[a], (size_t)n[a]))) return r;
+    for (int k = 0; k < 8; ++k)
+        if ((r = tmp.alloc(&der[k], np))) return r;
+    if ((r = tmp.alloc(&scr, 2 * np)) || (r = tmp.upload(&der_dev, der, 8))) return r;
+    {
+        const TricubicMatrix &M = tricubic_matrix();
+        std::vector<double> Ad(64 * 64);
+        for (int i = 0; i < 64; ++i)
+            for (int j = 0; j < 64; ++j) Ad[64 * i + j] = (double)M.A[i][j];
+        if ((r = tmp.upload(&A, Ad.data(), Ad.size()))) return r;
+    }
+    for (int comp = 0; comp < ncomp; ++comp) {
+        PT_HIP(hipMemcpy(der[0], values_h + comp * np, np * sizeof(double), hipMemcpyHostToDevice));
+        PT_HIP(pt_launch_tricubic_derivs(g, nx, ny, nz, der, scr, 0));
+        PT_HIP(pt_launch_tricubic_coef(g[0], g[1], g[2], nx, ny, nz, der_dev, A, ncomp, comp, device_order, coef_dev,
+                                       coef32_dev, 0));
+        PT_HIP(hipStreamSynchronize(0));
+    }
+    return PT_OK;
+}
+
+int check_grid(const double *const g[3], const int n[3], const char *who) {
+    for (int a = 0; a < 3; ++a) {
+        if (n[a] < 2 || !g[a]) return fail(PT_EINVAL, std::string(who) + ": bad grid");
+        for (int i = 1; i < n[a]; ++i)
+            if (!(g[a][i] > g[a][i - 1])) return fail(PT_EINVAL, std::string(who) + ": grid not strictly increasing");
+    }
+    return PT_OK;
+}
+
+} // namespace
+
+/* TabField3::PreInterpol on the GPU, host arrays in and out (libtricubic coefficient order,
+   as pt_tricubic_precompute); synchronous on the null stream */
+extern "C" int pt_tricubic_precompute_gpu(int nx, int ny, int nz, const double *xg, const double *yg, const double *zg,
+                                          int ncomp, const double *values, double *coef) {
+    const double *gh[3] = {xg, yg, zg};
+    const int n[3] = {nx, ny, nz};
+    if (ncomp < 1 || !values || !coef) return fail(PT_EINVAL, "pt_tricubic_precompute_gpu: bad arguments");
+    int r = check_grid(gh, n, "pt_tricubic_precompute_gpu");
+    if (r) return r;
+    const size_t ncoef = (size_t)(nx - 1) * (ny - 1) * (nz - 1) * ncomp * 64;
+    DevMem mem;
+    double *dc;
+    if ((r = mem.alloc(&dc, ncoef))) return r;
+    if ((r = tricubic_on_device(nx, ny, nz, gh, ncomp, values, 0, dc, nullptr))) return r;
+    PT_HIP(hipMemcpy(coef, dc, ncoef * sizeof(double), hipMemcpyDeviceToHost));
     return PT_OK;
 }
 

@@ -26,6 +26,15 @@ hipError_t pt_launch_collision(const ptd::Params &P, int mode, int coll_grid, hi
 /* field sum (FP64 tables) at n points and times; general: containers with formulas or analytic fields */
 hipError_t pt_launch_fields_eval(const ptd::Params &P, int general, int n, const double *pts, const double *t, double *B,
                                  double *dB, double *V, double *E, hipStream_t stream);
+/* TabField3::PreInterpol on the device: the seven spline-derivative tables der[1..7] of the
+   samples der[0] (each [nx][ny][nz]; scratch: 2 nx ny nz doubles), then the 64 coefficients
+   per cell of component comp (der_dev: the 8 table pointers in device memory, A: the 64 x 64
+   matrix as doubles; device_order 1: the tracker's coefficient order) */
+hipError_t pt_launch_tricubic_derivs(const double *const g[3], int nx, int ny, int nz, double *const der[8],
+                                     double *scratch, hipStream_t stream);
+hipError_t pt_launch_tricubic_coef(const double *xg, const double *yg, const double *zg, int nx, int ny, int nz,
+                                   const double *const *der_dev, const double *A, int ncomp, int comp, int device_order,
+                                   double *coef, float *coef32, hipStream_t stream);
 /* ID of the solid with the highest priority at each point and time */
 hipError_t pt_launch_solid_at(const ptd::Geom &G, int n, const double *pts, const double *t, int *out, hipStream_t stream);
 /* decay products of the batch's decayed neutrons into out (count reset first) */

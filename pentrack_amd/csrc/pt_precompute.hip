@@ -1,0 +1,160 @@
+/*
+ * pt_precompute.hip -- TabField3::PreInterpol (src/field_3d.cpp:336-414) on the GPU.
+ *
+ * Seven spline-derivative passes over the sample table (CalcDerivs: d/dx, d/dy, d/dz
+ * of F, then d/dy and d/dz of dF/dx, d/dz of dF/dy and of d2F/dxdy), each one thread
+ * per grid line solving ALGLIB's tridiagonal system for parabolically terminated
+ * cubic splines (spline1dgriddiffcubic), then one thread per cell forming the 64
+ * corner data scaled by the cell sizes and multiplying them by libtricubic's 64 x 64
+ * integer matrix.  Every operation follows the host routine pt_tricubic_precompute in
+ * the same order (no contraction, all 64 products of each sum, zeros included), so
+ * the coefficients are bit-identical to it.
+ *
+ * Work per cell: 4096 multiply-adds in FP64 and 8 x 8 B of gathered derivative data
+ * per corner, 512 B of coefficients written per component (256 B more for the FP32
+ * copy): the coefficient kernel is bound by FP64 issue rate, the line passes by HBM.
+ */
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace ptp {
+
+/* derivatives along `axis` of tab [nx][ny][nz] for every grid line; scr: [2][len][nlines] */
+__global__ void __launch_bounds__(256) spline_lines_kernel(const double *__restrict__ g, int nx, int ny, int nz, int axis,
+                                                           const double *__restrict__ tab, double *__restrict__ out,
+                                                           double *__restrict__ scr) {
+    const int n[3] = {nx, ny, nz};
+    const long long stride[3] = {(long long)ny * nz, nz, 1};
+    const int len = n[axis];
+    /* lines are numbered so that consecutive threads touch consecutive z where z is not the axis */
+    const int da = axis == 0 ? 1 : 0, db = axis == 2 ? 1 : 2;
+    const long long nlines = (long long)n[da] * n[db];
+    const long long line = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= nlines) return;
+    const long long ia = line / n[db], ib = line % n[db];
+    const long long base = ia * stride[da] + ib * stride[db];
+    const long long st = stride[axis];
+    const double *y = tab + base;
+    double *d = out + base;
+    if (len == 2) {
+        const double d0 = (y[st] - y[0]) / (g[1] - g[0]);
+        d[0] = d0;
+        d[st] = d0;
+        return;
+    }
+    double *a2s = scr, *bs = scr + (size_t)len * nlines;
+    /* row 0: a1 = 0, a2 = 1, a3 = 1, b = 2 (y1 - y0) / (x1 - x0) */
+    double a2p = 1.0, a3p = 1.0;
+    double yk = y[0], yk1 = y[st];
+    double bp = 2 * (yk1 - yk) / (g[1] - g[0]);
+    a2s[line] = a2p;
+    bs[line] = bp;
+    for (int k = 1; k <= len - 2; ++k) {
+        const double ykm = yk;
+        yk = yk1;
+        yk1 = y[(long long)(k + 1) * st];
+        const double a1 = g[k + 1] - g[k];
+        const double a2 = 2 * (g[k + 1] - g[k - 1]);
+        const double a3 = g[k] - g[k - 1];
+        const double b = 3 * (yk - ykm) / (g[k] - g[k - 1]) * (g[k + 1] - g[k]) +
+                         3 * (yk1 - yk) / (g[k + 1] - g[k]) * (g[k] - g[k - 1]);
+        const double t = a1 / a2p;
+        a2p = a2 - t * a3p;
+        bp = b - t * bp;
+        a3p = a3;
+        a2s[(size_t)k * nlines + line] = a2p;
+        bs[(size_t)k * nlines + line] = bp;
+    }
+    {   /* row n-1: a1 = 1, a2 = 1, a3 = 0, b = 2 (y[n-1] - y[n-2]) / (x[n-1] - x[n-2]) */
+        const double b = 2 * (yk1 - yk) / (g[len - 1] - g[len - 2]);
+        const double t = 1.0 / a2p;
+        a2p = 1.0 - t * a3p;
+        bp = b - t * bp;
+    }
+    double dk = bp / a2p;
+    d[(long long)(len - 1) * st] = dk;
+    for (int k = len - 2; k >= 0; --k) {
+        const double a3 = k == 0 ? 1.0 : g[k] - g[k - 1];
+        dk = (bs[(size_t)k * nlines + line] - a3 * dk) / a2s[(size_t)k * nlines + line];
+        d[(long long)k * st] = dk;
+    }
+}
+
+/* one thread per cell: corner data (F, Fx, Fy, Fz, Fxy, Fxz, Fyz, Fxyz at the 8 corners,
+   derivatives scaled by the cell sizes) times the 64 x 64 matrix; der[8] -> tables of
+   np values.  Output in libtricubic order a[i + 4j + 16k] (device_order 0) or in the
+   tracker's order a[16i + 4j + k] (device_order 1), at coef[(cell * ncomp + comp) * 64] */
+__global__ void __launch_bounds__(256) tricubic_coef_kernel(const double *__restrict__ xg, const double *__restrict__ yg,
+                                                            const double *__restrict__ zg, int nx, int ny, int nz,
+                                                            const double *const *__restrict__ der,
+                                                            const double *__restrict__ A, int ncomp, int comp,
+                                                            int device_order, double *__restrict__ coef,
+                                                            float *__restrict__ coef32) {
+    __shared__ double As[64 * 64];
+    for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) As[k] = A[k];
+    __syncthreads();
+    const long long ncell = (long long)(nx - 1) * (ny - 1) * (nz - 1);
+    const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= ncell) return;
+    const int iz = (int)(cell % (nz - 1));
+    const int iy = (int)((cell / (nz - 1)) % (ny - 1));
+    const int ix = (int)(cell / ((long long)(nz - 1) * (ny - 1)));
+    const double cellx = xg[ix + 1] - xg[ix];
+    const double celly = yg[iy + 1] - yg[iy];
+    const double cellz = zg[iz + 1] - zg[iz];
+    double xv[64];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const long long p = ((long long)(ix + (c & 1)) * ny + (iy + ((c >> 1) & 1))) * nz + (iz + ((c >> 2) & 1));
+        xv[c] = der[0][p];
+        xv[8 + c] = der[1][p] * cellx;
+        xv[16 + c] = der[2][p] * celly;
+        xv[24 + c] = der[3][p] * cellz;
+        xv[32 + c] = der[4][p] * cellx * celly;
+        xv[40 + c] = der[5][p] * cellx * cellz;
+        xv[48 + c] = der[6][p] * celly * cellz;
+        xv[56 + c] = der[7][p] * cellx * celly * cellz;
+    }
+    double *a = coef + ((size_t)cell * ncomp + comp) * 64;
+    float *a32 = coef32 ? coef32 + ((size_t)cell * ncomp + comp) * 64 : nullptr;
+#pragma unroll 1
+    for (int i = 0; i < 64; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 64; ++j) s += As[64 * i + j] * xv[j];
+        const int o = device_order ? 16 * (i & 3) + 4 * ((i >> 2) & 3) + (i >> 4) : i;
+        a[o] = s;
+        if (a32) a32[o] = (float)s;
+    }
+}
+
+} // namespace ptp
+
+/* the seven derivative tables of one component (der[1..7]; der[0] = the samples) */
+hipError_t pt_launch_tricubic_derivs(const double *const g[3], int nx, int ny, int nz, double *const der[8],
+                                     double *scratch, hipStream_t stream) {
+    const int n[3] = {nx, ny, nz};
+    /* (source table, axis, output): dx, dy, dz, dxy, dxz, dyz, dxyz as CalcDerivs is called */
+    const int plan[7][3] = {{0, 0, 1}, {0, 1, 2}, {0, 2, 3}, {1, 1, 4}, {1, 2, 5}, {2, 2, 6}, {4, 2, 7}};
+    for (int s = 0; s < 7; ++s) {
+        const int axis = plan[s][1];
+        const long long nlines = (long long)nx * ny * nz / n[axis];
+        const unsigned blocks = (unsigned)((nlines + 255) / 256);
+        hipLaunchKernelGGL(ptp::spline_lines_kernel, dim3(blocks), dim3(256), 0, stream, g[axis], nx, ny, nz, axis,
+                           der[plan[s][0]], der[plan[s][2]], scratch);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t pt_launch_tricubic_coef(const double *xg, const double *yg, const double *zg, int nx, int ny, int nz,
+                                   const double *const *der_dev, const double *A, int ncomp, int comp, int device_order,
+                                   double *coef, float *coef32, hipStream_t stream) {
+    const long long ncell = (long long)(nx - 1) * (ny - 1) * (nz - 1);
+    if (ncell <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ptp::tricubic_coef_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, stream, xg, yg,
+                       zg, nx, ny, nz, der_dev, A, ncomp, comp, device_order, coef, coef32);
+    return hipGetLastError();
+}

@@ -57,12 +57,15 @@ class FieldContainer:
             raise ValueError("You defined a field with an invalid boundary. Check field parameters.")
 
 
-def tabfield3(xg, yg, zg, B=None, V=None, bscale=1.0, escale=0.0, box=(0, 0, 0, 0, 0, 0), bw=0.0, threads=None):
+def tabfield3(xg, yg, zg, B=None, V=None, bscale=1.0, escale=0.0, box=(0, 0, 0, 0, 0, 0), bw=0.0, threads=None,
+              precompute="host"):
     """TabField3 container from samples on a grid.
 
     B: [3][nx][ny][nz] (T) or None; V: [nx][ny][nz] (V) or None.
     Grid coordinates are made sorted and unique as in TabField3::TabField3
-    (src/field_3d.cpp:417-426).
+    (src/field_3d.cpp:417-426).  precompute "host": coefficients now, on the host
+    (pt_tricubic_precompute); "gpu": the samples are kept and pre-interpolated on the GPU
+    when the field manager is uploaded (bit-identical coefficients).
     """
     xg, yg, zg = (np.unique(np.asarray(g, dtype=np.float64)) for g in (xg, yg, zg))
     comps = []
@@ -74,12 +77,28 @@ def tabfield3(xg, yg, zg, B=None, V=None, bscale=1.0, escale=0.0, box=(0, 0, 0, 
     if not comps:
         raise ValueError("tabfield3 needs B or V samples")
     values = np.ascontiguousarray(np.stack(comps))
-    coef = tricubic_coefficients(xg, yg, zg, values, threads=threads)
+    if precompute not in ("host", "gpu"):
+        raise ValueError("precompute is 'host' or 'gpu'")
+    coef = tricubic_coefficients(xg, yg, zg, values, threads=threads) if precompute == "host" else None
     bs, bf = _scale(bscale)
     es, ef = _scale(escale)
     return FieldContainer(type=_native.PT_FIELD_TRICUBIC, bscale=bs, escale=es, bformula=bf, eformula=ef,
-                          box=tuple(box), bw=bw,
-                          grid=(xg, yg, zg), has_b=int(B is not None), has_v=int(V is not None), coef=coef)
+                          box=tuple(box), bw=bw, grid=(xg, yg, zg), has_b=int(B is not None),
+                          has_v=int(V is not None), coef=coef, values=None if coef is not None else values)
+
+
+def tricubic_coefficients_gpu(xg, yg, zg, values):
+    """TabField3::PreInterpol on the GPU (pt_tricubic_precompute_gpu): [cells][ncomp][64],
+    the same layout and values as tricubic_coefficients."""
+    L = _native.lib()
+    xg, yg, zg = (np.ascontiguousarray(g, dtype=np.float64) for g in (xg, yg, zg))
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    ncomp = values.shape[0]
+    coef = np.empty(((len(xg) - 1) * (len(yg) - 1) * (len(zg) - 1), ncomp, 64), dtype=np.float64)
+    d = _native.dptr
+    _native.check(L.pt_tricubic_precompute_gpu(len(xg), len(yg), len(zg), d(xg), d(yg), d(zg), ncomp, d(values),
+                                               d(coef)), "pt_tricubic_precompute_gpu")
+    return coef
 
 
 TAB2D_B = ("Br", "Bphi", "Bz")
@@ -248,12 +267,18 @@ class TFieldManager:
                     setattr(d, attr, C.cast(b, C.c_char_p))
             if c.type == _native.PT_FIELD_TRICUBIC:
                 xg, yg, zg = (np.ascontiguousarray(g, dtype=np.float64) for g in c.grid)
-                coef = np.ascontiguousarray(c.coef, dtype=np.float64)
-                keep.extend([xg, yg, zg, coef])
+                keep.extend([xg, yg, zg])
                 d.nx, d.ny, d.nz = len(xg), len(yg), len(zg)
                 d.xg, d.yg, d.zg = _native.dptr(xg), _native.dptr(yg), _native.dptr(zg)
                 d.has_b, d.has_v = c.has_b, c.has_v
-                d.coef = _native.dptr(coef)
+                if c.coef is not None:
+                    coef = np.ascontiguousarray(c.coef, dtype=np.float64)
+                    keep.append(coef)
+                    d.coef = _native.dptr(coef)
+                else:                      # samples only: pre-interpolated on the GPU
+                    vals = np.ascontiguousarray(c.values, dtype=np.float64)
+                    keep.append(vals)
+                    d.values = _native.dptr(vals)
             elif c.type == _native.PT_FIELD_TAB2D:
                 rg, zg = (np.ascontiguousarray(c.grid[i], dtype=np.float64) for i in (0, 2))
                 coef = np.ascontiguousarray(c.coef, dtype=np.float64)
