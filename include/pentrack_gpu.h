@@ -189,6 +189,7 @@ typedef struct {
     int32_t tri;                 /* triangle index (input order) whose normal was used */
     int32_t leaving, entering;   /* solid IDs */
     double normal[3], v1[3], v2[3], pol1, pol2;
+    double pos[3];               /* position of the bracketing point before the wall */
 } pt_hit;
 
 #define PT_VARIANT_FP64 0        /* plain FP64, one thread per particle */

@@ -1611,7 +1611,9 @@ static int do_hit(track_t *tp, double x1, const double *y1, double *x2, double *
             ptref_hit *h = &tp->hits[tp->n_hits];
             h->t = x1; h->tri = cl->tri;
             h->leaving = (int32_t)solid_ID(c, leaving); h->entering = (int32_t)solid_ID(c, entering);
-            for (int i = 0; i < 3; ++i) { h->normal[i] = cl->normal[i]; h->v1[i] = y1[3 + i]; h->v2[i] = y2[3 + i]; }
+            for (int i = 0; i < 3; ++i) {
+                h->normal[i] = cl->normal[i]; h->v1[i] = y1[3 + i]; h->v2[i] = y2[3 + i]; h->pos[i] = y1[i];
+            }
             h->pol1 = y1[7]; h->pol2 = y2[7];
         }
         tp->n_hits++;

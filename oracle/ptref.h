@@ -91,6 +91,7 @@ typedef struct {
     double t;
     int32_t tri, leaving, entering;
     double normal[3], v1[3], v2[3], pol1, pol2;
+    double pos[3];
 } ptref_hit;
 
 #endif /* PTREF_H_ */

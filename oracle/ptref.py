@@ -68,7 +68,7 @@ OUT_DTYPE = np.dtype([
 HIT_DTYPE = np.dtype([
     ("t", "<f8"), ("tri", "<i4"), ("leaving", "<i4"), ("entering", "<i4"),
     ("normal", "<f8", (3,)), ("v1", "<f8", (3,)), ("v2", "<f8", (3,)),
-    ("pol1", "<f8"), ("pol2", "<f8"),
+    ("pol1", "<f8"), ("pol2", "<f8"), ("pos", "<f8", (3,)),
 ], align=True)
 
 _libs = {}

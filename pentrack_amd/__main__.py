@@ -3,14 +3,17 @@
 Mirrors the reference's `PENTrack jobnumber configpath outpath seed` for simtype 1
 (particle simulation, src/main.cpp:52-200): reads config.in, builds geometry, fields and
 the source, tracks `simcount` particles on the GPU (and their decay products when
-`secondaries 1`), prints the stopID histogram per species and writes the text end logs
-of species whose `endlog` is 1.
+`secondaries 1`), prints the stopID histogram per species and writes the text end and
+hit logs of species whose `endlog` / `hitlog` is 1.  Hits are recorded up to HIT_CAP per
+particle; a run where a particle hit more often says so (the reference logs every hit).
 """
 import os
 import sys
 import time
 
 import numpy as np
+
+HIT_CAP = 256
 
 
 def main(argv=None):
@@ -20,7 +23,7 @@ def main(argv=None):
     outpath = os.path.abspath(argv[2] if len(argv) > 2 else "out")
     seed = int(argv[3]) if len(argv) > 3 else 0
     from . import config as cfgmod
-    from .logger import DEFAULT_ENDLOGVARS, end_values, write_log
+    from .logger import DEFAULT_ENDLOGVARS, DEFAULT_HITLOGVARS, end_values, hit_values, write_log
     from .source import VolumeSource
     from .tracker import LaunchConfig, TTracker, secondary_states
 
@@ -42,33 +45,49 @@ def main(argv=None):
     kind_idx = np.full(n, source.kind, np.int32)
     pid = np.arange(1, n + 1, dtype=np.uint64)
     print(f"Simulating {n} {source.particle}s...")
+    names = {0: "neutron", 1: "proton", 2: "electron"}
+    settings = {k: cfgmod.particle_settings(cfg, v) for k, v in names.items()}
+    on = lambda kind, log: settings[kind].get(log, "").split()[:1] == ["1"]
+    for kind in names:
+        for log in ("end", "hit"):
+            if on(kind, log + "log") and settings[kind].get(log + "logfilter", "").strip():
+                raise SystemExit(f"{log}logfilter is not supported by this engine")
+    hit_cap = HIT_CAP if any(on(k, "hitlog") for k in names) else 0
     tr = TTracker(geometry, fields, LaunchConfig())
     if g.secondaries == 1:
-        (out, _), (sout, _), rec = tr.IntegrateWithSecondaries(species, g.simtime, kind_idx, t0, y0, spin0, seed, pid)
+        (out, hits), (sout, shits), rec = tr.IntegrateWithSecondaries(species, g.simtime, kind_idx, t0, y0, spin0, seed,
+                                                                      pid, hit_cap=hit_cap)
     else:
-        out, _ = tr.IntegrateBatch(species, g.simtime, kind_idx, t0, y0, spin0, seed, pid)
-        sout, rec = np.zeros(0, out.dtype), None
-    names = {0: "neutron", 1: "proton", 2: "electron"}
-    groups = [(source.kind, out, t0, y0, spin0, pid.astype(float))]
+        out, hits = tr.IntegrateBatch(species, g.simtime, kind_idx, t0, y0, spin0, seed, pid, hit_cap=hit_cap)
+        sout, shits, rec = np.zeros(0, out.dtype), None, None
+    groups = [(source.kind, out, hits, t0, y0, spin0, pid.astype(float))]
     if rec is not None and len(rec):
         _, sidx, st0, sy0, sspin0, _ = secondary_states(species, rec, pid)
         for k, kind in ((1, 1), (2, 2)):
             sel = rec["kind"] == k
-            groups.append((kind, sout[sel], st0[sel], sy0[sel], sspin0[sel], (rec["parent"][sel] + 1).astype(float)))
-    for kind, o, a, b, c, numbers in groups:
+            groups.append((kind, sout[sel], shits[sel] if shits is not None else None, st0[sel], sy0[sel], sspin0[sel],
+                           (rec["parent"][sel] + 1).astype(float)))
+    for kind, o, h, a, b, c, numbers in groups:
         if len(o) == 0:
             continue
         ids, counts = np.unique(o["stopID"], return_counts=True)
         print(f"{names[kind]}: " + ", ".join(f"{i}: {c_}" for i, c_ in zip(ids, counts)))
-        settings = cfgmod.particle_settings(cfg, names[kind])
-        if settings.get("endlog", "").split()[:1] == ["1"]:
-            titles = settings.get("endlogvars", "").split() or DEFAULT_ENDLOGVARS
-            vals = end_values(kind, a, b, c, o, geometry, fields, jobnumber)
-            vals["particle"] = numbers
+        for log, defaults in (("end", DEFAULT_ENDLOGVARS), ("hit", DEFAULT_HITLOGVARS)):
+            if not on(kind, log + "log"):
+                continue
+            titles = settings[kind].get(log + "logvars", "").split() or defaults
+            if log == "end":
+                vals = end_values(kind, a, b, c, o, geometry, fields, jobnumber)
+                vals["particle"] = numbers
+            else:
+                if (o["n_hits"] > hit_cap).any():
+                    print(f"warning: {int((o['n_hits'] > hit_cap).sum())} {names[kind]}s hit walls more than "
+                          f"{hit_cap} times; their hit log keeps the first {hit_cap}")
+                vals = hit_values(h, o["n_hits"], numbers, jobnumber)
             missing = [t for t in titles if t not in vals]
             if missing:
-                raise SystemExit(f"endlogvars not supported: {missing}")
-            path = write_log(outpath, jobnumber, names[kind], "end", titles, vals)
+                raise SystemExit(f"{log}logvars not supported: {missing}")
+            path = write_log(outpath, jobnumber, names[kind], log, titles, vals)
             print("wrote", path)
     steps = int(out["Nstep"].sum() + (sout["Nstep"].sum() if len(sout) else 0))
     print(f"{steps} steps")

@@ -81,7 +81,7 @@ class HitRec(C.Structure):
     _fields_ = [
         ("t", C.c_double), ("tri", C.c_int32), ("leaving", C.c_int32), ("entering", C.c_int32),
         ("normal", C.c_double * 3), ("v1", C.c_double * 3), ("v2", C.c_double * 3),
-        ("pol1", C.c_double), ("pol2", C.c_double),
+        ("pol1", C.c_double), ("pol2", C.c_double), ("pos", C.c_double * 3),
     ]
 
 
@@ -111,7 +111,7 @@ OUT_DTYPE = np.dtype([
 HIT_DTYPE = np.dtype([
     ("t", "<f8"), ("tri", "<i4"), ("leaving", "<i4"), ("entering", "<i4"),
     ("normal", "<f8", (3,)), ("v1", "<f8", (3,)), ("v2", "<f8", (3,)),
-    ("pol1", "<f8"), ("pol2", "<f8"),
+    ("pol1", "<f8"), ("pol2", "<f8"), ("pos", "<f8", (3,)),
 ], align=True)
 SEC_DTYPE = np.dtype([
     ("parent", "<i4"), ("kind", "<i4"), ("t", "<f8"), ("pos", "<f8", (3,)),

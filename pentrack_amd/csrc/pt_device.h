@@ -103,6 +103,7 @@ struct Hit {
     double t;
     int tri, leaving, entering;
     double normal[3], v1[3], v2[3], pol1, pol2;
+    double pos[3];
 };
 
 /* resumable per-particle state, structure of arrays */

@@ -1459,7 +1459,9 @@ extern "C" int pt_batch_download(pt_batch *b, void *stream, pt_particle_out *out
         for (size_t k = 0; k < h.size(); ++k) {
             pt_hit &o = hits[k];
             o.t = h[k].t; o.tri = h[k].tri; o.leaving = h[k].leaving; o.entering = h[k].entering;
-            for (int a = 0; a < 3; ++a) { o.normal[a] = h[k].normal[a]; o.v1[a] = h[k].v1[a]; o.v2[a] = h[k].v2[a]; }
+            for (int a = 0; a < 3; ++a) {
+                o.normal[a] = h[k].normal[a]; o.v1[a] = h[k].v1[a]; o.v2[a] = h[k].v2[a]; o.pos[a] = h[k].pos[a];
+            }
             o.pol1 = h[k].pol1; o.pol2 = h[k].pol2;
         }
     }

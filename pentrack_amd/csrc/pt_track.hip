@@ -1071,7 +1071,9 @@ PTD_NOINLINE bool do_hit(const Params &P, Track &T, double x1, const double *y1,
         if (T.nhitlog < P.st.hit_cap) {
             Hit &h = P.st.hits[(size_t)T.i * P.st.hit_cap + T.nhitlog];
             h.t = x1; h.tri = L.orig[sel]; h.leaving = (int)lid; h.entering = (int)eid;
-            for (int k = 0; k < 3; ++k) { h.normal[k] = normal[k]; h.v1[k] = y1[3 + k]; h.v2[k] = y2[3 + k]; }
+            for (int k = 0; k < 3; ++k) {
+                h.normal[k] = normal[k]; h.v1[k] = y1[3 + k]; h.v2[k] = y2[3 + k]; h.pos[k] = y1[k];
+            }
             h.pol1 = y1[7]; h.pol2 = y2[7];
         }
         T.nhitlog++;

@@ -1,11 +1,13 @@
-"""Text end log (TTextLogger, src/logger.cpp:42-134, 314-334).
+"""Text end and hit logs (TTextLogger, src/logger.cpp:42-134, 198-232, 314-334).
 
 One file per species and log type, named `%012d<particle><suffix>.out` after the job
-number, a header line of the variable names, then one line per particle with the
-values printed at std::numeric_limits<double>::digits10 (15) significant digits, each
-followed by a space.  The variables are those of TLogger::Print; the start and end
-fields, potentials and solids are evaluated on the GPU (pt_fields_eval,
-pt_geometry_solid_at) at the particles' start and end states.
+number, a header line of the variable names, then one line per particle (end log) or
+per wall hit (hit log, particles in order, hits in time order) with the values printed
+at std::numeric_limits<double>::digits10 (15) significant digits, each followed by a
+space.  The variables are those of TLogger::Print and TLogger::PrintHit; the start and
+end fields, potentials and solids are evaluated on the GPU (pt_fields_eval,
+pt_geometry_solid_at) at the particles' start and end states, the hits come from the
+tracker's hit records.
 """
 import os
 
@@ -19,6 +21,10 @@ DEFAULT_ENDLOGVARS = ["jobnumber", "particle", "tstart", "xstart", "ystart", "zs
                       "solidstart", "tend", "xend", "yend", "zend", "vxend", "vyend", "vzend", "polend", "Sxend",
                       "Syend", "Szend", "Hend", "Eend", "Bend", "Uend", "solidend", "stopID", "Nspinflip",
                       "spinflipprob", "Nhit", "Nstep", "propert", "trajlength", "Hmax", "wL"]
+
+
+DEFAULT_HITLOGVARS = ["jobnumber", "particle", "t", "x", "y", "z", "v1x", "v1y", "v1z", "pol1", "v2x", "v2y", "v2z",
+                      "pol2", "nx", "ny", "nz", "solid1", "solid2"]
 
 
 def _fmt(v):
@@ -97,3 +103,19 @@ def write_log(outdir, jobnumber, particle, suffix, titles, values):
         for row in zip(*cols):
             f.write(" ".join(_fmt(v) for v in row) + " \n")
     return path
+
+
+def hit_values(hits, n_hits, numbers, jobnumber=0):
+    """{variable: values} of TLogger::PrintHit (src/logger.cpp:198-232) for the logged hits
+    hits [n][cap] (n_hits[k] recorded for particle k, at most cap kept)."""
+    cap = hits.shape[1] if hits.ndim == 2 else 0
+    k = np.minimum(np.asarray(n_hits), cap)
+    rows = np.concatenate([hits[i, :k[i]] for i in range(len(k))]) if cap else hits[:0]
+    part = np.repeat(np.asarray(numbers, dtype=float), k)
+    v = {"jobnumber": np.full(len(rows), float(jobnumber)), "particle": part, "t": rows["t"],
+         "x": rows["pos"][:, 0], "y": rows["pos"][:, 1], "z": rows["pos"][:, 2],
+         "v1x": rows["v1"][:, 0], "v1y": rows["v1"][:, 1], "v1z": rows["v1"][:, 2], "pol1": rows["pol1"],
+         "v2x": rows["v2"][:, 0], "v2y": rows["v2"][:, 1], "v2z": rows["v2"][:, 2], "pol2": rows["pol2"],
+         "nx": rows["normal"][:, 0], "ny": rows["normal"][:, 1], "nz": rows["normal"][:, 2],
+         "solid1": rows["leaving"].astype(float), "solid2": rows["entering"].astype(float)}
+    return v

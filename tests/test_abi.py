@@ -31,7 +31,7 @@ def test_struct_layouts_match_header():
     from pentrack_amd import _native
     # pt_particle_out: 1+9+5 doubles, 4 ints, 2 int64, 3 doubles, 2 ints
     assert ctypes.sizeof(_native.ParticleOut) == 8 * 15 + 16 + 16 + 24 + 8
-    assert ctypes.sizeof(_native.HitRec) == 8 + 12 + 4 + 9 * 8 + 16
+    assert ctypes.sizeof(_native.HitRec) == 8 + 12 + 4 + 9 * 8 + 16 + 24
 
 
 def test_host_entry_points_reject_bad_arguments():

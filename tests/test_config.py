@@ -67,6 +67,7 @@ Bmax 0.1
 
 [neutron]
 tau 0.05
+hitlog 1
 
 [proton]
 tmax 3e-3
@@ -174,3 +175,20 @@ def test_end_log_number_format_matches_iostream():
     vals = [0.0, -0.0, 1.0, 1e-9, 123456789.123456789, 3.141592653589793, -2.5e-300, 1e300, 0.1, 2.0 / 3.0, 7e15,
             1234567890123456.0]
     assert "".join(_fmt(v) + " " for v in vals) == ref
+
+
+def test_hit_log_rows_and_format(tmp_path):
+    """TLogger::PrintHit variables (src/logger.cpp:198-232): one row per logged hit,
+    particles in order; a particle's hits beyond the record capacity are dropped."""
+    from pentrack_amd.logger import DEFAULT_HITLOGVARS, hit_values, write_log
+    hits = np.zeros((3, 2), dtype=_native.HIT_DTYPE)
+    hits["t"] = [[0.1, 0.2], [0.3, 0.0], [0.5, 0.6]]
+    hits["pos"][..., 2] = hits["t"] * 10
+    hits["entering"] = [[2, 1], [10, 0], [2, 2]]
+    v = hit_values(hits, np.array([2, 1, 5]), np.array([7.0, 8.0, 9.0]), jobnumber=4)
+    assert list(v["particle"]) == [7, 7, 8, 9, 9] and list(v["t"]) == [0.1, 0.2, 0.3, 0.5, 0.6]
+    assert list(v["z"]) == [1.0, 2.0, 3.0, 5.0, 6.0] and list(v["solid2"]) == [2, 1, 10, 2, 2]
+    p = write_log(str(tmp_path), 4, "neutron", "hit", DEFAULT_HITLOGVARS, v)
+    lines = open(p).read().split("\n")
+    assert os.path.basename(p) == "000000000004neutronhit.out"
+    assert lines[0] == " ".join(DEFAULT_HITLOGVARS) + " " and lines[1].startswith("4 7 0.1 0 0 1 ")

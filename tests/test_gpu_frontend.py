@@ -61,6 +61,20 @@ def test_command_line_run(gpu, oracle_lib, tmp_path):
     assert np.array_equal(nlog[:, col["Nhit"]], o["Nhit"])
     assert np.allclose(nlog[:, col["xend"]], o["y"][:, 0], rtol=1e-14, atol=1e-300)
     assert np.allclose(nlog[:, col["Hmax"]], o["Hmax"], rtol=1e-14)
+    # hit log: every hit of every neutron, particles in order, as the oracle records them
+    htitles, hlog = read_log(os.path.join(out, "000000000003neutronhit.out"))
+    assert htitles == ["jobnumber", "particle", "t", "x", "y", "z", "v1x", "v1y", "v1z", "pol1", "v2x", "v2y", "v2z",
+                       "pol2", "nx", "ny", "nz", "solid1", "solid2"]
+    o2, oh = oracle_lib.track(w, "fp64", hit_cap=256, threads=8)
+    assert (o2["n_hits"] <= 256).all()
+    rows = np.concatenate([oh[k, :o2["n_hits"][k]] for k in range(200)])
+    assert len(hlog) == len(rows) == o["Nhit"].sum()
+    hc = {t: k for k, t in enumerate(htitles)}
+    assert np.array_equal(hlog[:, hc["particle"]], np.repeat(np.arange(1, 201), o2["n_hits"]))
+    for name, want in (("t", rows["t"]), ("x", rows["pos"][:, 0]), ("z", rows["pos"][:, 2]),
+                       ("v2y", rows["v2"][:, 1]), ("nz", rows["normal"][:, 2]), ("pol2", rows["pol2"])):
+        assert np.allclose(hlog[:, hc[name]], want, rtol=1e-14, atol=1e-300), name
+    assert np.array_equal(hlog[:, hc["solid2"]], rows["entering"])
     dec = int((o["stopID"] == -4).sum())
     if dec:
         _, plog = read_log(os.path.join(out, "000000000003protonend.out"))

@@ -39,7 +39,7 @@ def assert_bit_exact(g, gh, o, oh):
     if gh is not None:
         n = np.minimum(g["n_hits"], gh.shape[1])
         for k in np.where(n > 0)[0][:2000]:
-            for f in ("t", "tri", "leaving", "entering", "normal", "v1", "v2", "pol1", "pol2"):
+            for f in ("t", "tri", "leaving", "entering", "normal", "v1", "v2", "pol1", "pol2", "pos"):
                 assert np.array_equal(gh[k, :n[k]][f], oh[k, :n[k]][f]), (k, f)
 
 
