@@ -24,7 +24,7 @@ def main(argv=None):
     seed = int(argv[3]) if len(argv) > 3 else 0
     from . import config as cfgmod
     from .logger import DEFAULT_ENDLOGVARS, DEFAULT_HITLOGVARS, end_values, hit_values, write_log
-    from .source import VolumeSource
+    from .source import create_source
     from .tracker import LaunchConfig, TTracker, secondary_states
 
     cfg = cfgmod.TConfig(configpath)
@@ -37,11 +37,11 @@ def main(argv=None):
     base = os.path.dirname(configpath)
     geometry = cfgmod.load_geometry(cfg, base)
     fields = cfgmod.load_fields(cfg, base)
-    source = VolumeSource(cfg)
+    source = create_source(cfg, base)
     species = [cfgmod.species_config(cfg, name) for name in ("neutron", "proton", "electron")]
     rng = np.random.default_rng(seed)
     n = g.simcount
-    t0, y0, spin0 = source.sample(n, rng)
+    t0, y0, spin0 = source.sample(n, rng, geometry, fields)
     kind_idx = np.full(n, source.kind, np.int32)
     pid = np.arange(1, n + 1, dtype=np.uint64)
     print(f"Simulating {n} {source.particle}s...")

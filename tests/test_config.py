@@ -192,3 +192,39 @@ def test_hit_log_rows_and_format(tmp_path):
     lines = open(p).read().split("\n")
     assert os.path.basename(p) == "000000000004neutronhit.out"
     assert lines[0] == " ".join(DEFAULT_HITLOGVARS) + " " and lines[1].startswith("4 7 0.1 0 0 1 ")
+
+
+def test_rotate_vectors_match_oracle(oracle_lib):
+    """RotateVector without a reference direction (src/globals.cpp:40-64), as the surface
+    sources use it: CGAL's base1 cases (zero components) and general normals, bit for bit."""
+    from pentrack_amd.source import rotate_vectors
+    rng = np.random.default_rng(4)
+    n = np.concatenate([rng.normal(size=(200, 3)), [[0, 0, 1], [0, 1, 0], [1, 0, 0], [0, 2, -1], [3, 0, 1],
+                                                    [1, -1, 0], [-1e-3, 1, 1e3]]])
+    v = rng.normal(size=(len(n), 3))
+    got = rotate_vectors(v, n)
+    from oracle import ptref
+    L = ptref.load("fp64")
+    for k in range(len(n)):
+        w = np.ascontiguousarray(v[k].copy())
+        L.ptref_rotate_vector(ptref._dp(w), ptref._dp(np.ascontiguousarray(n[k])), None)
+        assert np.array_equal(got[k], w), k
+
+
+def test_source_modes_and_parameters(tmp_path):
+    from pentrack_amd.source import SurfaceSource, VolumeSource, create_source
+    base = CONFIG.split("[SOURCE]")[0]
+    def cfg_with(src):
+        return C.TConfig(text=base + "[SOURCE]\nparticle neutron\nEmin 1e-7\nEmax 2e-7\n" + src)
+    assert isinstance(create_source(cfg_with("sourcemode cylvolume\nparameters 0 1 0 360 0 1\n")), VolumeSource)
+    with pytest.raises(C.ConfigError, match="Could not load source"):
+        create_source(cfg_with("sourcemode pointsource\n"))
+    with pytest.raises(C.ConfigError, match="needs 6 parameters"):
+        create_source(cfg_with("sourcemode boxvolume\nparameters 0 1 0 1\n"))
+    with pytest.raises(C.ConfigError, match="needs an STLfile"):
+        create_source(cfg_with("sourcemode STLvolume\n"))
+    s = create_source(cfg_with("sourcemode cylsurface\nparameters 0.1 0.5 -90 90 0 1\nEnormal 5e-8\n"))
+    assert isinstance(s, SurfaceSource) and s.Enormal == 5e-8 and np.allclose(s.cyl[2:4], [-np.pi / 2, np.pi / 2])
+    write_stl(str(tmp_path / "src.STL"), box_mesh((0, 0, 0), (1, 2, 3)), "src")
+    s = create_source(cfg_with("sourcemode STLvolume\nSTLfile src.STL\n"), str(tmp_path))
+    assert np.array_equal(s.bbox[0], [0, 0, 0]) and np.array_equal(s.bbox[1], [1, 2, 3])
