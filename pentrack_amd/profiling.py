@@ -22,10 +22,11 @@ import sys
 COPY_BYTES = 1 << 30
 KERNELS = {
     # name: (variant, persistent, bytes per field evaluation, dispatch-name pattern, spatial order)
-    "step_kernel_fp64": (0, 0, 1536, "step_kernel<false>", 0),
-    "step_kernel_mixed": (1, 0, 768, "step_kernel<true>", 0),
-    "step_kernel_persistent_mixed": (1, 1, 768, "step_kernel_persistent<true>", 0),
-    "step_kernel_persistent_mixed_spatial": (1, 1, 768, "step_kernel_persistent<true>", 1),
+    # (the kernels' template argument is the mode: 0 FP64 tables, 1 FP32 tables)
+    "step_kernel_fp64": (0, 0, 1536, "step_kernel<0>", 0),
+    "step_kernel_mixed": (1, 0, 768, "step_kernel<1>", 0),
+    "step_kernel_persistent_mixed": (1, 1, 768, "step_kernel_persistent<1>", 0),
+    "step_kernel_persistent_mixed_spatial": (1, 1, 768, "step_kernel_persistent<1>", 1),
 }
 
 
