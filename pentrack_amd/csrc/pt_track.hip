@@ -2106,8 +2106,15 @@ __global__ void __launch_bounds__(256) bucket_scatter_kernel(Params P) {
  * waves, run the collision path (CheckHit's bisection, DoHit/OnHit) on their
  * pending chord piece and finish the rest of that step.
  */
+/* three waves per SIMD: the collision path's registers (256 VGPRs + 192 AGPRs at one wave)
+   fit in 168 VGPRs with 138 spilled values, and the extra waves hide the latency of its
+   BVH walks and dense-output reads (cylinder, 1e6 neutrons: 18.6 -> 12.6 ms per launch;
+   two waves 13.1 ms, four 13.6 ms) */
+#ifndef PT_COLL_MIN_WAVES
+#define PT_COLL_MIN_WAVES 3
+#endif
 template <int MODE>
-__global__ void __launch_bounds__(256) collision_kernel(Params P) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_COLL_MIN_WAVES))) collision_kernel(Params P) {
     const int count = *P.st.qcount;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
         Ctr c = {0, 0, 0};
