@@ -229,6 +229,23 @@ int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg);
 int pt_batch_active(pt_batch *b, void *stream, int *active);
 /* Copy results to the host (synchronises the stream).  hits may be NULL. */
 int pt_batch_download(pt_batch *b, void *stream, pt_particle_out *out, pt_hit *hits);
+
+/* a particle's state at a snapshot time, with its spin and counters at that point */
+typedef struct {
+    double t, y[9], spin[5], Hmax, noflipprob;
+    int32_t stopID, Nhit, Nstep, Nspinflip;
+} pt_snapshot;
+/*
+ * TLogger::PrintSnapshot (src/logger.cpp:128-140, called after each step's collision
+ * checks, src/tracking.cpp:126): after every step from x1 (the stepper's step start) to x
+ * (the step's end after collisions), the first snapshot time in [x1, x) is recorded from
+ * the stepper's dense output.  Times are sorted and de-duplicated; cap records are kept
+ * per particle (more are counted).  Call before the first pt_batch_run; n_times = 0
+ * switches snapshots off.
+ */
+int pt_batch_set_snapshots(pt_batch *b, int n_times, const double *times, int cap);
+/* out [n][cap] records, count [n] records taken (may exceed cap; synchronises the stream) */
+int pt_batch_snapshots(pt_batch *b, void *stream, pt_snapshot *out, int32_t *count);
 /* Sums over the batch (synchronises the stream): accepted / rejected dopri5 steps and
    field-map cell evaluations of the step kernel (each gathers one coefficient block per
    component); any pointer may be NULL. */

@@ -1695,6 +1695,37 @@ static void do_polarize(track_t *tp, double *y, double polarization, int flipspi
 }
 
 /* TTracker::IntegrateSpin without the BMT integration branch (src/tracking.cpp:373-394,473-483) */
+/* snapshots (ptref_set_snapshots): sorted times, records [n][cap] and counts [n] of the
+   batch; the particle's index in the batch is the worker's current one */
+static int g_snap_n, g_snap_cap;
+static const double *g_snap_t;
+static ptref_snap *g_snaps;
+static int32_t *g_nsnap;
+static __thread int g_particle;
+
+void ptref_set_snapshots(int n, const double *times, int cap, ptref_snap *out, int32_t *count) {
+    g_snap_n = n; g_snap_t = times; g_snap_cap = cap; g_snaps = out; g_nsnap = count;
+}
+
+/* TLogger::PrintSnapshot (src/logger.cpp:128-140): find_if over the times for the first
+   with x1 <= t < x2; the state from the stepper's dense output */
+static void print_snapshot(track_t *tp, const dopri_t *st, double x1, double x2, const double *spin) {
+    for (int j = 0; j < g_snap_n; ++j) {
+        double ts = g_snap_t[j];
+        if (x1 <= ts && ts < x2) {
+            int k = g_nsnap[g_particle]++;
+            if (k >= g_snap_cap) return;
+            ptref_snap *r = &g_snaps[(size_t)g_particle * g_snap_cap + k];
+            r->t = ts;
+            dopri_calc_state(tp, st, ts, r->y);
+            memcpy(r->spin, spin, sizeof(r->spin));
+            r->Hmax = tp->Hmax; r->noflipprob = (double)tp->noflipprob;
+            r->stopID = tp->stopID; r->Nhit = tp->Nhit; r->Nstep = tp->Nstep; r->Nspinflip = tp->Nspinflip;
+            return;
+        }
+    }
+}
+
 static double integrate_spin(track_t *tp, double *spin, const dopri_t *st, double x2, const double *y2, double Bmax) {
     double x1 = st->t_old;
     const double *y1 = st->xo;
@@ -1805,6 +1836,7 @@ static void integrate_particle(const ctx_t *c, const ptref_species *cfg, double 
             x1 = x2;
             memcpy(y1, y2, sizeof(y1));
         }
+        if (g_snap_n) print_snapshot(tp, &st, st.t_old, x, spin);   /* src/tracking.cpp:126 */
         double spinProjection = integrate_spin(tp, spin, &st, x, y, cfg->Bmax);
         int spinHasFlipped = st.xo[7] != y[7];
         if (spinProjection != st.xo[7]) {
@@ -1948,6 +1980,7 @@ typedef struct {
 static void *worker(void *arg) {
     job_t *j = (job_t *)arg;
     for (int i = j->begin; i < j->end; ++i) {
+        g_particle = i;
         integrate_particle(j->c, &j->species[j->species_idx[i]], j->tmax_global, j->t0[i], j->y0 + 9 * i,
                            j->spin0 + 5 * i, j->seed, j->pid[i], &j->out[i],
                            j->hits ? j->hits + (size_t)i * j->hit_cap : 0, j->hit_cap);

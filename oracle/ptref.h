@@ -94,4 +94,10 @@ typedef struct {
     double pos[3];
 } ptref_hit;
 
+/* a particle's state at a snapshot time (TLogger::PrintSnapshot) and its counters then */
+typedef struct {
+    double t, y[9], spin[5], Hmax, noflipprob;
+    int32_t stopID, Nhit, Nstep, Nspinflip;
+} ptref_snap;
+
 #endif /* PTREF_H_ */

@@ -102,6 +102,8 @@ def load(mode="fp64"):
     L.ptref_stream_key.argtypes = [C.c_uint64, C.c_uint64]
     L.ptref_stream_key.restype = C.c_uint64
     L.ptref_tricubic_eval.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
+    L.ptref_set_snapshots.argtypes = [C.c_int, dp, C.c_int, C.c_void_p, C.POINTER(C.c_int32)]
+    L.ptref_set_snapshots.restype = None
     L.ptref_bicubic_tables.argtypes = [C.c_int, C.c_int, dp, dp, C.c_int, dp, dp]
     L.ptref_bicubic_tables.restype = None
     L.ptref_spline2d.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp, dp]
@@ -208,6 +210,25 @@ def spline2d(problem, k, c, x, y, diff, mode="fp64"):
     F, Fx, Fy = np.zeros(len(x)), np.zeros(len(x)), np.zeros(len(x))
     L.ptref_spline2d(C.byref(problem.fields[k]), c, len(x), _dp(x), _dp(y), int(bool(diff)), _dp(F), _dp(Fx), _dp(Fy))
     return (F, Fx, Fy) if diff else F
+
+
+SNAP_DTYPE = np.dtype([("t", "<f8"), ("y", "<f8", (9,)), ("spin", "<f8", (5,)), ("Hmax", "<f8"), ("noflipprob", "<f8"),
+                       ("stopID", "<i4"), ("Nhit", "<i4"), ("Nstep", "<i4"), ("Nspinflip", "<i4")], align=True)
+
+
+def track_snapshots(workload, times, cap, mode="fp64", threads=1):
+    """The oracle run with snapshot times: (out, records [n][cap], counts [n])."""
+    L = load(mode)
+    t = np.ascontiguousarray(np.unique(np.asarray(times, dtype=np.float64)))
+    snaps = np.zeros(workload.n * cap, dtype=SNAP_DTYPE)
+    counts = np.zeros(workload.n, dtype=np.int32)
+    L.ptref_set_snapshots(len(t), _dp(t), cap, snaps.ctypes.data_as(C.c_void_p),
+                          counts.ctypes.data_as(C.POINTER(C.c_int32)))
+    try:
+        out, _ = track(workload, mode, threads=threads)
+    finally:
+        L.ptref_set_snapshots(0, None, 0, None, None)
+    return out, snaps.reshape(workload.n, cap), counts
 
 
 def track(workload, mode="fp64", hit_cap=0, threads=1, sel=None, problem=None):

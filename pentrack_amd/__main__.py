@@ -23,7 +23,7 @@ def main(argv=None):
     outpath = os.path.abspath(argv[2] if len(argv) > 2 else "out")
     seed = int(argv[3]) if len(argv) > 3 else 0
     from . import config as cfgmod
-    from .logger import DEFAULT_ENDLOGVARS, DEFAULT_HITLOGVARS, end_values, hit_values, write_log
+    from .logger import DEFAULT_ENDLOGVARS, DEFAULT_HITLOGVARS, end_values, hit_values, snapshot_values, write_log
     from .source import create_source
     from .tracker import LaunchConfig, TTracker, secondary_states
 
@@ -49,17 +49,34 @@ def main(argv=None):
     settings = {k: cfgmod.particle_settings(cfg, v) for k, v in names.items()}
     on = lambda kind, log: settings[kind].get(log, "").split()[:1] == ["1"]
     for kind in names:
-        for log in ("end", "hit"):
+        for log in ("end", "hit", "snapshot"):
             if on(kind, log + "log") and settings[kind].get(log + "logfilter", "").strip():
                 raise SystemExit(f"{log}logfilter is not supported by this engine")
     hit_cap = HIT_CAP if any(on(k, "hitlog") for k in names) else 0
+    snaps = None
+    if on(source.kind, "snapshotlog"):
+        times = cfgmod._floats(settings[source.kind].get("snapshots", ""))
+        snaps = (times, max(len(times), 1)) if times else None
+    if any(on(k, "snapshotlog") for k in names if k != source.kind) and g.secondaries == 1:
+        raise SystemExit("snapshot logs of decay products are not supported by this engine")
     tr = TTracker(geometry, fields, LaunchConfig())
-    if g.secondaries == 1:
+    if g.secondaries == 1 and snaps is None:
         (out, hits), (sout, shits), rec = tr.IntegrateWithSecondaries(species, g.simtime, kind_idx, t0, y0, spin0, seed,
                                                                       pid, hit_cap=hit_cap)
     else:
-        out, hits = tr.IntegrateBatch(species, g.simtime, kind_idx, t0, y0, spin0, seed, pid, hit_cap=hit_cap)
+        if g.secondaries == 1:
+            raise SystemExit("snapshot logs together with secondaries are not supported by this engine")
+        res = tr.IntegrateBatch(species, g.simtime, kind_idx, t0, y0, spin0, seed, pid, hit_cap=hit_cap, snapshots=snaps)
+        out, hits = res[0], res[1]
         sout, shits, rec = np.zeros(0, out.dtype), None, None
+        if snaps is not None:
+            vals = snapshot_values(source.kind, t0, y0, spin0, res[2], res[3], pid.astype(float), geometry, fields,
+                                   jobnumber)
+            titles = settings[source.kind].get("snapshotlogvars", "").split() or DEFAULT_ENDLOGVARS
+            missing = [t for t in titles if t not in vals]
+            if missing:
+                raise SystemExit(f"snapshotlogvars not supported: {missing}")
+            print("wrote", write_log(outpath, jobnumber, names[source.kind], "snapshot", titles, vals))
     groups = [(source.kind, out, hits, t0, y0, spin0, pid.astype(float))]
     if rec is not None and len(rec):
         _, sidx, st0, sy0, sspin0, _ = secondary_states(species, rec, pid)

@@ -117,6 +117,11 @@ SEC_DTYPE = np.dtype([
     ("parent", "<i4"), ("kind", "<i4"), ("t", "<f8"), ("pos", "<f8", (3,)),
     ("E", "<f8"), ("phi", "<f8"), ("theta", "<f8"), ("pol", "<i4"), ("pad", "<i4"),
 ], align=True)
+SNAP_DTYPE = np.dtype([
+    ("t", "<f8"), ("y", "<f8", (9,)), ("spin", "<f8", (5,)), ("Hmax", "<f8"), ("noflipprob", "<f8"),
+    ("stopID", "<i4"), ("Nhit", "<i4"), ("Nstep", "<i4"), ("Nspinflip", "<i4"),
+], align=True)
+assert SNAP_DTYPE.itemsize == 152
 assert OUT_DTYPE.itemsize == C.sizeof(ParticleOut), (OUT_DTYPE.itemsize, C.sizeof(ParticleOut))
 assert SEC_DTYPE.itemsize == C.sizeof(SecondaryRec), (SEC_DTYPE.itemsize, C.sizeof(SecondaryRec))
 assert HIT_DTYPE.itemsize == C.sizeof(HitRec), (HIT_DTYPE.itemsize, C.sizeof(HitRec))
@@ -132,7 +137,7 @@ EXPORTS = [
     "pt_spline_grid_diff", "pt_rk4_forward_ckpt", "pt_rk4_adjoint", "pt_formula_eval",
     "pt_batch_secondaries", "pt_proton_beta_table", "pt_neutron_decay_host", "pt_beta_sample",
     "pt_distribution_table", "pt_fields_eval", "pt_geometry_solid_at", "pt_bicubic_precompute",
-    "pt_tricubic_precompute_gpu",
+    "pt_tricubic_precompute_gpu", "pt_batch_set_snapshots", "pt_batch_snapshots",
 ]
 
 _lib = None
@@ -179,6 +184,8 @@ def lib():
     L.pt_batch_download.argtypes = [vp, vp, vp, vp]
     L.pt_batch_step_counts.argtypes = [vp, vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.pt_batch_set_profiling.argtypes = [vp, C.c_int]
+    L.pt_batch_set_snapshots.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.c_int]
+    L.pt_batch_snapshots.argtypes = [vp, vp, vp, C.POINTER(C.c_int32)]
     L.pt_batch_kernel_times.argtypes = [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.pt_track_batch.argtypes = [vp, vp, vp, C.c_int, C.POINTER(Species), C.c_double, C.c_int, ip,
                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),

@@ -106,6 +106,12 @@ struct Hit {
     double pos[3];
 };
 
+/* a particle's state at a snapshot time (TLogger::PrintSnapshot) with its counters then */
+struct Snap {
+    double t, y[9], spin[5], Hmax, noflip;
+    int stop, Nhit, Nstep, Nspinflip;
+};
+
 /* resumable per-particle state, structure of arrays */
 struct State {
     int n;
@@ -115,6 +121,11 @@ struct State {
     unsigned long long *ctr, *key, *solids;
     Hit *hits;
     int hit_cap;
+    /* snapshots: sorted times, records [n][snap_cap], records per particle */
+    const double *snap_t;
+    int snap_n, snap_cap;
+    Snap *snaps;
+    int *nsnap;
     double *Bc, *Vc;          /* field at the stepper's current state, [3][n] and [n] (valid if FL_BVALID) */
     double *pend;             /* pending wall-collision piece: x1, y1[9], x2, y2[9] -> [20][n] */
     double *dense;            /* dense-output state of the last step: [DENSE][n] */

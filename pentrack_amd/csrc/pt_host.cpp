@@ -1468,6 +1468,53 @@ extern "C" int pt_batch_download(pt_batch *b, void *stream, pt_particle_out *out
     return PT_OK;
 }
 
+/* TLogger::PrintSnapshot (src/logger.cpp:128-140): snapshot times (sorted and de-duplicated
+   here) and the records kept per particle; n_times = 0 switches snapshots off */
+extern "C" int pt_batch_set_snapshots(pt_batch *b, int n_times, const double *times, int cap) {
+    if (!b || n_times < 0 || (n_times > 0 && (!times || cap <= 0))) return fail(PT_EINVAL, "pt_batch_set_snapshots: bad arguments");
+    ptd::State &S = b->P.st;
+    std::vector<double> t(times, times + n_times);
+    std::sort(t.begin(), t.end());
+    t.erase(std::unique(t.begin(), t.end()), t.end());
+    S.snap_n = 0;
+    if (!t.empty()) {
+        double *dt;
+        ptd::Snap *ds;
+        int *dn;
+        int r;
+        if ((r = b->mem.upload(&dt, t.data(), t.size())) || (r = b->mem.alloc(&ds, (size_t)b->n * cap))) return r;
+        std::vector<int> zero((size_t)std::max(b->n, 1), 0);
+        if ((r = b->mem.upload(&dn, zero.data(), zero.size()))) return r;
+        S.snap_t = dt; S.snaps = ds; S.nsnap = dn; S.snap_cap = cap;
+        S.snap_n = (int)t.size();
+    }
+    /* launch sequences captured before hold the old state by value */
+    for (auto &g : b->graphs) { (void)hipGraphExecDestroy(g.exec); (void)hipGraphDestroy(g.graph); }
+    b->graphs.clear();
+    return PT_OK;
+}
+
+/* the snapshot records: out [n][cap] (cap of pt_batch_set_snapshots), count [n] (may exceed cap) */
+extern "C" int pt_batch_snapshots(pt_batch *b, void *stream, pt_snapshot *out, int32_t *count) {
+    if (!b || !out || !count) return fail(PT_EINVAL, "pt_batch_snapshots: bad arguments");
+    const ptd::State &S = b->P.st;
+    if (S.snap_n == 0) return fail(PT_EINVAL, "pt_batch_snapshots: no snapshot times set");
+    PT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    const size_t n = (size_t)b->n, cap = (size_t)S.snap_cap;
+    std::vector<ptd::Snap> h(n * cap);
+    PT_HIP(hipMemcpy(h.data(), S.snaps, h.size() * sizeof(ptd::Snap), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(count, S.nsnap, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < h.size(); ++k) {
+        pt_snapshot &o = out[k];
+        o.t = h[k].t;
+        for (int q = 0; q < 9; ++q) o.y[q] = h[k].y[q];
+        for (int q = 0; q < 5; ++q) o.spin[q] = h[k].spin[q];
+        o.Hmax = h[k].Hmax; o.noflipprob = h[k].noflip;
+        o.stopID = h[k].stop; o.Nhit = h[k].Nhit; o.Nstep = h[k].Nstep; o.Nspinflip = h[k].Nspinflip;
+    }
+    return PT_OK;
+}
+
 extern "C" int pt_batch_step_counts(pt_batch *b, void *stream, int64_t *accepted, int64_t *rejected,
                                     int64_t *field_evals) {
     if (!b) return fail(PT_EINVAL, "pt_batch_step_counts: bad arguments");

@@ -769,6 +769,40 @@ PTD void dense_calc(const DenseRef &r, double t, double *y) {
 
 PTD void dense_calc(const Dopri &s, double t, double *y) { dopri_calc_state(s, t, y); }
 
+/* TLogger::PrintSnapshot (src/logger.cpp:128-140): the first snapshot time in [x1, x2),
+   with the snapshot times sorted (-1: none) */
+#define NOINLINE_SNAP __attribute__((noinline))
+PTD int snapshot_in(const State &S, double x1, double x2) {
+    int lo = 0, hi = S.snap_n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (S.snap_t[mid] < x1) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < S.snap_n && S.snap_t[lo] < x2) ? lo : -1;
+}
+
+/* record the stepper's dense-output state at the snapshot time in [x1, x2), if any, with
+   the particle's spin and counters at that point of the step */
+template <class DenseSrc>
+PTD NOINLINE_SNAP void record_snapshot(const State &S, int i, const DenseSrc &d, double x1, double x2,
+                                       const double *spin_or_null, int stop, int Nhit, int Nstep, int Nspinflip,
+                                       double Hmax, double noflip) {
+    const int j = snapshot_in(S, x1, x2);
+    if (j < 0) return;
+    const int k = S.nsnap[i];
+    S.nsnap[i] = k + 1;
+    if (k >= S.snap_cap) return;
+    Snap &r = S.snaps[(size_t)i * S.snap_cap + k];
+    r.t = S.snap_t[j];
+    double y[9];
+    dense_calc(d, r.t, y);
+    for (int q = 0; q < 9; ++q) r.y[q] = y[q];
+    for (int q = 0; q < 5; ++q) r.spin[q] = spin_or_null ? spin_or_null[q] : S.spin[(size_t)q * S.n + i];
+    r.Hmax = Hmax; r.noflip = noflip;
+    r.stop = stop; r.Nhit = Nhit; r.Nstep = Nstep; r.Nspinflip = Nspinflip;
+}
+
 struct Track {
     int i;                      /* particle index */
     Species sp;
@@ -1561,7 +1595,8 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
             bool split = 0.25 * (l2 - d2) > MAX_TRACK_DEVIATION * MAX_TRACK_DEVIATION;
             bool trunc = (L.tau > 0 && xnew[6] > L.tau) || (ts + dt > tmax);
             /* detached: the chord starts at the tracker's point, not at xc */
-            need_dense = trunc || split || absorb || (L.flags & FL_DETACHED) || !clear_of_walls(G, xc, xnew);
+            need_dense = trunc || split || absorb || (L.flags & FL_DETACHED) || !clear_of_walls(G, xc, xnew) ||
+                         (P.st.snap_n > 0 && snapshot_in(P.st, ts, ts + dt) >= 0);
         }
         if (need_dense) {
 #pragma unroll
@@ -1705,6 +1740,9 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
         for (int k = 0; k < 9; ++k) y1[k] = y2[k];
     }
     L.flags = reset ? (L.flags | FL_RESET) : (L.flags & ~FL_RESET);
+    if (P.st.snap_n > 0)
+        record_snapshot(P.st, i, dref, told, x, nullptr, L.stop, P.st.Nhit[i], L.Nstep, P.st.Nspinflip[i], L.Hmax,
+                        P.st.noflip[i]);
     const bool y_is_xc = at_end && !altered && (L.flags & FL_BVALID);
     spin_step<MODE>(P, i, sp, told, xc[0], xc[1], xc[2], polo, x, y, b1v, B1c[0], B1c[1], B1c[2],
                      y_is_xc, L.Bc[0], L.Bc[1], L.Bc[2], c);
@@ -1802,6 +1840,8 @@ PTD void collision_path(const Params &P, int i, Ctr &c) {
         for (int k = 0; k < 9; ++k) y1[k] = y2[k];
     }
     flags = reset ? (flags | FL_RESET) : (flags & ~FL_RESET);
+    if (S.snap_n > 0)
+        record_snapshot(S, i, st, st.t_old, x, spin, T.stop, T.Nhit, T.Nstep, T.Nspinflip, T.Hmax, T.noflip);
     double spinProjection = integrate_spin<MODE>(P, T, spin, st, x, y);
     bool spinHasFlipped = st.xo[7] != y[7];
     if (spinProjection != st.xo[7]) {

@@ -1,4 +1,4 @@
-"""Text end and hit logs (TTextLogger, src/logger.cpp:42-134, 198-232, 314-334).
+"""Text end, snapshot and hit logs (TTextLogger, src/logger.cpp:42-140, 198-232, 314-334).
 
 One file per species and log type, named `%012d<particle><suffix>.out` after the job
 number, a header line of the variable names, then one line per particle (end log) or
@@ -51,28 +51,28 @@ def potential_energy(kind, y, B, V, mats, solid_mat):
     return res
 
 
-def end_values(kind, t0, y0, spin0, out, geometry, fields, jobnumber=0, first_number=1):
-    """{variable: [n] values} of TLogger::Print for one species."""
+def print_values(kind, t0, y0, spin0, t, y, spin, rec, geometry, fields, jobnumber=0, first_number=1):
+    """{variable: [n] values} of TLogger::Print (src/logger.cpp:42-126) for particles that
+    started at (t0, y0, spin0) and are now at (t, y, spin) with the counters of `rec`
+    (stopID, Nspinflip, noflipprob, Nhit, Nstep, Hmax).  The solid at the end point is
+    TGeometry::GetSolid there, as Print evaluates it."""
     from .queries import fields_eval, solid_at
-    n = len(out)
-    yend = out["y"]
-    tend = out["t"]
+    n = len(t)
     Bs, _, Vs, _ = fields_eval(fields, y0[:, :3], t0, grad=False)      # BField(x, y, z, t, B) (src/logger.cpp:55,63)
-    Be, _, Ve, _ = fields_eval(fields, yend[:, :3], tend, grad=False)
+    Be, _, Ve, _ = fields_eval(fields, y[:, :3], t, grad=False)
     sid_s = solid_at(geometry, y0[:, :3], t0)
-    sid_e = solid_at(geometry, yend[:, :3], tend)
+    sid_e = solid_at(geometry, y[:, :3], t)
     mats = np.stack([m.as_array() for m in geometry.materials.values()])
     names = list(geometry.materials)
     id_to_mat = {1: names.index(geometry.default_material)}
     for s in geometry.solids:
         id_to_mat[s.ID] = names.index(s.material)
-    ms = np.array([id_to_mat.get(int(i), id_to_mat[1]) for i in sid_s])
-    me = np.array([id_to_mat.get(int(i), id_to_mat[1]) for i in sid_e])
+    ms = np.array([id_to_mat.get(int(i), id_to_mat[1]) for i in sid_s], dtype=int)
+    me = np.array([id_to_mat.get(int(i), id_to_mat[1]) for i in sid_e], dtype=int)
     Es = kinetic_energy(kind, y0[:, 3:6])
-    Ee = kinetic_energy(kind, yend[:, 3:6])
+    Ee = kinetic_energy(kind, y[:, 3:6])
     Hs = Es + potential_energy(kind, y0, Bs, Vs, mats, ms)
-    He = Ee + potential_energy(kind, yend, Be, Ve, mats, me)
-    spin = out["spin"]
+    He = Ee + potential_energy(kind, y, Be, Ve, mats, me)
     wL = np.where(spin[:, 3] > 0, spin[:, 4] / np.where(spin[:, 3] > 0, spin[:, 3], 1.0), 0.0)
     v = {
         "jobnumber": np.full(n, float(jobnumber)), "particle": np.arange(first_number, first_number + n, dtype=float),
@@ -80,14 +80,33 @@ def end_values(kind, t0, y0, spin0, out, geometry, fields, jobnumber=0, first_nu
         "vystart": y0[:, 4], "vzstart": y0[:, 5], "polstart": y0[:, 7], "Sxstart": spin0[:, 0],
         "Systart": spin0[:, 1], "Szstart": spin0[:, 2], "Hstart": Hs, "Estart": Es,
         "Bstart": np.sqrt((Bs ** 2).sum(1)), "Ustart": Vs, "solidstart": sid_s.astype(float),
-        "tend": tend, "xend": yend[:, 0], "yend": yend[:, 1], "zend": yend[:, 2], "vxend": yend[:, 3],
-        "vyend": yend[:, 4], "vzend": yend[:, 5], "polend": yend[:, 7], "Sxend": spin[:, 0], "Syend": spin[:, 1],
+        "tend": t, "xend": y[:, 0], "yend": y[:, 1], "zend": y[:, 2], "vxend": y[:, 3],
+        "vyend": y[:, 4], "vzend": y[:, 5], "polend": y[:, 7], "Sxend": spin[:, 0], "Syend": spin[:, 1],
         "Szend": spin[:, 2], "Hend": He, "Eend": Ee, "Bend": np.sqrt((Be ** 2).sum(1)), "Uend": Ve,
-        "solidend": out["solidend"].astype(float), "stopID": out["stopID"].astype(float),
-        "Nspinflip": out["Nspinflip"].astype(float), "spinflipprob": 1 - out["noflipprob"],
-        "Nhit": out["Nhit"].astype(float), "Nstep": out["Nstep"].astype(float), "propert": yend[:, 6],
-        "trajlength": yend[:, 8], "Hmax": out["Hmax"], "wL": wL,
+        "solidend": sid_e.astype(float), "stopID": rec["stopID"].astype(float),
+        "Nspinflip": rec["Nspinflip"].astype(float), "spinflipprob": 1 - rec["noflipprob"],
+        "Nhit": rec["Nhit"].astype(float), "Nstep": rec["Nstep"].astype(float), "propert": y[:, 6],
+        "trajlength": y[:, 8], "Hmax": rec["Hmax"], "wL": wL,
     }
+    return v
+
+
+def end_values(kind, t0, y0, spin0, out, geometry, fields, jobnumber=0, first_number=1):
+    """{variable: [n] values} of TLogger::Print for the particles' end states."""
+    return print_values(kind, t0, y0, spin0, out["t"], out["y"], out["spin"], out, geometry, fields, jobnumber,
+                        first_number)
+
+
+def snapshot_values(kind, t0, y0, spin0, snaps, counts, numbers, geometry, fields, jobnumber=0):
+    """TLogger::Print of the snapshot records snaps [n][cap] (counts[k] taken by particle k,
+    at most cap kept): one row per record, particles in order, times ascending."""
+    cap = snaps.shape[1]
+    k = np.minimum(np.asarray(counts), cap)
+    idx = np.repeat(np.arange(len(k)), k)
+    rows = np.concatenate([snaps[i, :k[i]] for i in range(len(k))]) if len(idx) else snaps.reshape(-1)[:0]
+    v = print_values(kind, t0[idx], y0[idx], spin0[idx], rows["t"], rows["y"], rows["spin"], rows, geometry, fields,
+                     jobnumber)
+    v["particle"] = np.asarray(numbers, dtype=float)[idx]
     return v
 
 

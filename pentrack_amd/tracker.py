@@ -43,6 +43,7 @@ class Batch:
         L = _native.lib()
         self.n = len(t0)
         self.hit_cap = hit_cap
+        self.snap_cap = 0
         sp = (_native.Species * len(species))(*[s.native() for s in species])
         self._keep = [np.ascontiguousarray(species_idx, dtype=np.int32), np.ascontiguousarray(t0, dtype=np.float64),
                       np.ascontiguousarray(y0, dtype=np.float64).reshape(-1, 9),
@@ -94,6 +95,22 @@ class Batch:
         n = (C.c_int64 * 2)()
         _native.check(_native.lib().pt_batch_kernel_times(self.h, stream, ms, n), "pt_batch_kernel_times")
         return {"step": (ms[0], n[0]), "collision": (ms[1], n[1])}
+
+    def set_snapshots(self, times, cap):
+        """TLogger::PrintSnapshot times (sorted, de-duplicated) and the records kept per
+        particle; call before the first run."""
+        t = np.ascontiguousarray(np.asarray(times, dtype=np.float64))
+        self.snap_cap = int(cap) if len(t) else 0
+        _native.check(_native.lib().pt_batch_set_snapshots(self.h, len(t), _native.dptr(t), self.snap_cap),
+                      "pt_batch_set_snapshots")
+
+    def snapshots(self, stream=None):
+        """(records [n][cap] SNAP_DTYPE, counts [n]): the particles' snapshot states."""
+        rec = np.zeros(self.n * self.snap_cap, dtype=_native.SNAP_DTYPE)
+        cnt = np.zeros(self.n, dtype=np.int32)
+        _native.check(_native.lib().pt_batch_snapshots(self.h, stream, rec.ctypes.data_as(C.c_void_p),
+                                                       cnt.ctypes.data_as(C.POINTER(C.c_int32))), "pt_batch_snapshots")
+        return rec.reshape(self.n, self.snap_cap), cnt
 
     def download(self, stream=None):
         out = np.zeros(self.n, dtype=_native.OUT_DTYPE)
@@ -159,12 +176,18 @@ class TTracker:
         self.fields = fields
         self.launch = launch or LaunchConfig()
 
-    def IntegrateBatch(self, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap=0):
+    def IntegrateBatch(self, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap=0, snapshots=None):
+        """(out, hits), or with snapshots = (times, cap) (out, hits, records, counts)."""
         cfg = self.launch
         b = Batch(self.geometry, self.fields, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap,
                   with_fp32=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
+        if snapshots is not None:
+            b.set_snapshots(*snapshots)
         b.run_to_completion(cfg)
-        return b.download()
+        out, hits = b.download()
+        if snapshots is None:
+            return out, hits
+        return (out, hits) + b.snapshots()
 
     def IntegrateWithSecondaries(self, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap=0):
         """The reference's main loop for one batch (src/main.cpp:153-172): the primaries,

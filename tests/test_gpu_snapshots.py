@@ -1,0 +1,58 @@
+"""GPU: snapshots (TLogger::PrintSnapshot, src/logger.cpp:128-140, src/tracking.cpp:126):
+the stepper's dense-output state at the first snapshot time of each step, with the
+particle's counters, bit for bit against the oracle, and on a free-fall parabola."""
+import numpy as np
+import pytest
+
+from pentrack_amd import workloads
+from pentrack_amd.constants import NEUTRON
+from pentrack_amd.tracker import Batch, LaunchConfig
+
+pytestmark = pytest.mark.gpu
+
+TIMES = [2.5, 0.1, 0.5, 1.0, 1.0 + 1e-9, 0.5]      # unsorted, a duplicate, two in one step
+
+
+def run_snapshots(w, times, cap, cfg=None):
+    b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid)
+    b.set_snapshots(times, cap)
+    assert b.run_to_completion(cfg or LaunchConfig(), max_launches=100000)
+    out, _ = b.download()
+    return out, *b.snapshots()
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_snapshots_match_oracle(gpu, oracle_lib, persistent):
+    w = workloads.cylinder(n=1500, seed=7, tmax=3.0, n_grid=24)
+    print("case: snapshots cylinder n=1500 persistent", persistent, flush=True)
+    out, rec, cnt = run_snapshots(w, TIMES, 4, LaunchConfig(persistent=persistent))
+    o, orec, ocnt = oracle_lib.track_snapshots(w, TIMES, 4, threads=8)
+    assert np.array_equal(out["y"], o["y"]) and np.array_equal(cnt, ocnt)
+    assert cnt.max() <= 4 and (cnt > 0).mean() > 0.9
+    for k in range(w.n):
+        a, b = rec[k, :cnt[k]], orec[k, :ocnt[k]]
+        for f in ("t", "y", "spin", "Hmax", "noflipprob", "stopID", "Nhit", "Nstep", "Nspinflip"):
+            assert np.array_equal(a[f], b[f]), (k, f)
+    # at most one record per step: 1.0 and 1.0 + 1e-9 fall into one step of every particle
+    t = rec["t"][rec["t"] > 0]
+    assert (np.isin(t, [0.1, 0.5, 1.0, 1.0 + 1e-9, 2.5])).all()
+
+
+def test_snapshots_on_a_parabola(gpu):
+    """No walls in reach: the snapshot state of a falling neutron is the parabola."""
+    w = workloads.cube(n=200, seed=3, tmax=0.2)
+    w.y0[:, :3] = [0.5, 0.5, 0.9]
+    w.y0[:, 3:6] = np.random.default_rng(1).uniform(-0.5, 0.5, (200, 3))
+    w.y0[:, 8] = 0
+    w.t0[:] = 0
+    times = [0.05, 0.1, 0.15]
+    print("case: snapshots parabola", flush=True)
+    out, rec, cnt = run_snapshots(w, times, 3)
+    assert (cnt == 3).all()
+    g = 9.80665
+    for j, t in enumerate(times):
+        r = rec[:, j]
+        assert (r["t"] == t).all()
+        want_z = w.y0[:, 2] + w.y0[:, 5] * t - 0.5 * g * t * t
+        assert np.allclose(r["y"][:, 2], want_z, rtol=0, atol=1e-9)
+        assert np.allclose(r["y"][:, 0], w.y0[:, 0] + w.y0[:, 3] * t, rtol=0, atol=1e-9)
