@@ -246,6 +246,22 @@ typedef struct {
 int pt_batch_set_snapshots(pt_batch *b, int n_times, const double *times, int cap);
 /* out [n][cap] records, count [n] records taken (may exceed cap; synchronises the stream) */
 int pt_batch_snapshots(pt_batch *b, void *stream, pt_snapshot *out, int32_t *count);
+
+/* a step's end state for the track log and the ID of the particle's current solid */
+typedef struct {
+    double t, y[9];
+    int32_t solid, pad;
+} pt_track_point;
+/*
+ * TLogger::PrintTrack (src/logger.cpp:142-196) as IntegrateParticle calls it after each step
+ * (src/tracking.cpp:140): the step's end state (after the spin step) is recorded when
+ * int(l1 / interval) != int(l / interval) for the trajectory lengths l1 at the stepper's
+ * step start and l at the end.  The start point (src/tracking.cpp:43) is not recorded here.
+ * cap records are kept per particle (more are counted); interval = 0 switches it off.
+ */
+int pt_batch_set_track(pt_batch *b, double interval, int cap);
+/* out [n][cap] records, count [n] records taken (may exceed cap; synchronises the stream) */
+int pt_batch_track(pt_batch *b, void *stream, pt_track_point *out, int32_t *count);
 /* Sums over the batch (synchronises the stream): accepted / rejected dopri5 steps and
    field-map cell evaluations of the step kernel (each gathers one coefficient block per
    component); any pointer may be NULL. */

@@ -1707,6 +1707,28 @@ void ptref_set_snapshots(int n, const double *times, int cap, ptref_snap *out, i
     g_snap_n = n; g_snap_t = times; g_snap_cap = cap; g_snaps = out; g_nsnap = count;
 }
 
+/* track log (ptref_set_track): interval, records [n][cap], counts [n] */
+static double g_trk_interval;
+static int g_trk_cap;
+static ptref_trackpt *g_trk;
+static int32_t *g_ntrk;
+
+void ptref_set_track(double interval, int cap, ptref_trackpt *out, int32_t *count) {
+    g_trk_interval = interval; g_trk_cap = cap; g_trk = out; g_ntrk = count;
+}
+
+/* TLogger::PrintTrack (src/logger.cpp:142-153) after a step (src/tracking.cpp:140) */
+static void print_track(double l1, double x, const double *y, unsigned solid) {
+    if (y[8] > 0 && (int)(l1 / g_trk_interval) == (int)(y[8] / g_trk_interval)) return;
+    int k = g_ntrk[g_particle]++;
+    if (k >= g_trk_cap) return;
+    ptref_trackpt *r = &g_trk[(size_t)g_particle * g_trk_cap + k];
+    r->t = x;
+    memcpy(r->y, y, sizeof(r->y));
+    r->solid = (int32_t)solid;
+    r->pad = 0;
+}
+
 /* TLogger::PrintSnapshot (src/logger.cpp:128-140): find_if over the times for the first
    with x1 <= t < x2; the state from the stepper's dense output */
 static void print_snapshot(track_t *tp, const dopri_t *st, double x1, double x2, const double *spin) {
@@ -1845,6 +1867,7 @@ static void integrate_particle(const ctx_t *c, const ptref_species *cfg, double 
         } else if (spinHasFlipped) {
             for (int i = 0; i < 5; ++i) spin[i] = 0;
         }
+        if (g_trk_interval > 0) print_track(st.xo[8], x, y, solid_ID(c, current_solid(c, &tp->cur)));
         if (tp->stopID == ID_UNKNOWN && y[6] >= tau) tp->stopID = ID_DECAYED;
         else if (tp->stopID == ID_UNKNOWN && (x >= tmax_global || y[8] >= maxtraj)) tp->stopID = ID_NOT_FINISH;
     }

@@ -94,6 +94,12 @@ typedef struct {
     double pos[3];
 } ptref_hit;
 
+/* a step's end state for the track log (TLogger::PrintTrack) and the current solid's ID */
+typedef struct {
+    double t, y[9];
+    int32_t solid, pad;
+} ptref_trackpt;
+
 /* a particle's state at a snapshot time (TLogger::PrintSnapshot) and its counters then */
 typedef struct {
     double t, y[9], spin[5], Hmax, noflipprob;

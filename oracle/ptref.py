@@ -104,6 +104,8 @@ def load(mode="fp64"):
     L.ptref_tricubic_eval.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
     L.ptref_set_snapshots.argtypes = [C.c_int, dp, C.c_int, C.c_void_p, C.POINTER(C.c_int32)]
     L.ptref_set_snapshots.restype = None
+    L.ptref_set_track.argtypes = [C.c_double, C.c_int, C.c_void_p, C.POINTER(C.c_int32)]
+    L.ptref_set_track.restype = None
     L.ptref_bicubic_tables.argtypes = [C.c_int, C.c_int, dp, dp, C.c_int, dp, dp]
     L.ptref_bicubic_tables.restype = None
     L.ptref_spline2d.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp, dp]
@@ -229,6 +231,22 @@ def track_snapshots(workload, times, cap, mode="fp64", threads=1):
     finally:
         L.ptref_set_snapshots(0, None, 0, None, None)
     return out, snaps.reshape(workload.n, cap), counts
+
+
+TRACK_DTYPE = np.dtype([("t", "<f8"), ("y", "<f8", (9,)), ("solid", "<i4"), ("pad", "<i4")], align=True)
+
+
+def track_tracklog(workload, interval, cap, mode="fp64", threads=1):
+    """The oracle run with the track log: (out, records [n][cap], counts [n])."""
+    L = load(mode)
+    rec = np.zeros(workload.n * cap, dtype=TRACK_DTYPE)
+    counts = np.zeros(workload.n, dtype=np.int32)
+    L.ptref_set_track(float(interval), cap, rec.ctypes.data_as(C.c_void_p), counts.ctypes.data_as(C.POINTER(C.c_int32)))
+    try:
+        out, _ = track(workload, mode, threads=threads)
+    finally:
+        L.ptref_set_track(0.0, 0, None, None)
+    return out, rec.reshape(workload.n, cap), counts
 
 
 def track(workload, mode="fp64", hit_cap=0, threads=1, sel=None, problem=None):

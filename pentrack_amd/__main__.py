@@ -23,7 +23,8 @@ def main(argv=None):
     outpath = os.path.abspath(argv[2] if len(argv) > 2 else "out")
     seed = int(argv[3]) if len(argv) > 3 else 0
     from . import config as cfgmod
-    from .logger import DEFAULT_ENDLOGVARS, DEFAULT_HITLOGVARS, end_values, hit_values, snapshot_values, write_log
+    from .logger import (DEFAULT_ENDLOGVARS, DEFAULT_HITLOGVARS, DEFAULT_TRACKLOGVARS, end_values, hit_values,
+                         snapshot_values, track_values, write_log)
     from .source import create_source
     from .tracker import LaunchConfig, TTracker, secondary_states
 
@@ -49,7 +50,7 @@ def main(argv=None):
     settings = {k: cfgmod.particle_settings(cfg, v) for k, v in names.items()}
     on = lambda kind, log: settings[kind].get(log, "").split()[:1] == ["1"]
     for kind in names:
-        for log in ("end", "hit", "snapshot"):
+        for log in ("end", "hit", "snapshot", "track"):
             if on(kind, log + "log") and settings[kind].get(log + "logfilter", "").strip():
                 raise SystemExit(f"{log}logfilter is not supported by this engine")
     hit_cap = HIT_CAP if any(on(k, "hitlog") for k in names) else 0
@@ -57,16 +58,36 @@ def main(argv=None):
     if on(source.kind, "snapshotlog"):
         times = cfgmod._floats(settings[source.kind].get("snapshots", ""))
         snaps = (times, max(len(times), 1)) if times else None
-    if any(on(k, "snapshotlog") for k in names if k != source.kind) and g.secondaries == 1:
-        raise SystemExit("snapshot logs of decay products are not supported by this engine")
+    trk = None
+    if on(source.kind, "tracklog"):
+        interval = cfgmod._floats(settings[source.kind].get("trackloginterval", ""))
+        if interval and interval[0] > 0:
+            trk = (interval[0], int(min(4096, max(16, 4e9 // (88 * max(n, 1))))))
+    for log in ("snapshot", "track"):
+        if any(on(k, log + "log") for k in names if k != source.kind) and g.secondaries == 1:
+            raise SystemExit(f"{log} logs of decay products are not supported by this engine")
     tr = TTracker(geometry, fields, LaunchConfig())
-    if g.secondaries == 1 and snaps is None:
+    if g.secondaries == 1 and snaps is None and trk is None:
         (out, hits), (sout, shits), rec = tr.IntegrateWithSecondaries(species, g.simtime, kind_idx, t0, y0, spin0, seed,
                                                                       pid, hit_cap=hit_cap)
     else:
         if g.secondaries == 1:
-            raise SystemExit("snapshot logs together with secondaries are not supported by this engine")
-        res = tr.IntegrateBatch(species, g.simtime, kind_idx, t0, y0, spin0, seed, pid, hit_cap=hit_cap, snapshots=snaps)
+            raise SystemExit("snapshot or track logs together with secondaries are not supported by this engine")
+        res = tr.IntegrateBatch(species, g.simtime, kind_idx, t0, y0, spin0, seed, pid, hit_cap=hit_cap, snapshots=snaps,
+                                track=trk)
+        if trk is not None:
+            trec, tcnt = res[-2], res[-1]
+            if (tcnt > trk[1]).any():
+                print(f"warning: {int((tcnt > trk[1]).sum())} particles logged more than {trk[1]} track points; "
+                      "their track log keeps the first ones")
+            from .queries import solid_at
+            vals = track_values(source.kind, t0, y0, solid_at(geometry, y0[:, :3], t0), trec, tcnt, pid.astype(float),
+                                geometry, fields, jobnumber)
+            titles = settings[source.kind].get("tracklogvars", "").split() or DEFAULT_TRACKLOGVARS
+            missing = [t for t in titles if t not in vals]
+            if missing:
+                raise SystemExit(f"tracklogvars not supported: {missing}")
+            print("wrote", write_log(outpath, jobnumber, names[source.kind], "track", titles, vals))
         out, hits = res[0], res[1]
         sout, shits, rec = np.zeros(0, out.dtype), None, None
         if snaps is not None:

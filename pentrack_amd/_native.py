@@ -122,6 +122,8 @@ SNAP_DTYPE = np.dtype([
     ("stopID", "<i4"), ("Nhit", "<i4"), ("Nstep", "<i4"), ("Nspinflip", "<i4"),
 ], align=True)
 assert SNAP_DTYPE.itemsize == 152
+TRACK_DTYPE = np.dtype([("t", "<f8"), ("y", "<f8", (9,)), ("solid", "<i4"), ("pad", "<i4")], align=True)
+assert TRACK_DTYPE.itemsize == 88
 assert OUT_DTYPE.itemsize == C.sizeof(ParticleOut), (OUT_DTYPE.itemsize, C.sizeof(ParticleOut))
 assert SEC_DTYPE.itemsize == C.sizeof(SecondaryRec), (SEC_DTYPE.itemsize, C.sizeof(SecondaryRec))
 assert HIT_DTYPE.itemsize == C.sizeof(HitRec), (HIT_DTYPE.itemsize, C.sizeof(HitRec))
@@ -138,6 +140,7 @@ EXPORTS = [
     "pt_batch_secondaries", "pt_proton_beta_table", "pt_neutron_decay_host", "pt_beta_sample",
     "pt_distribution_table", "pt_fields_eval", "pt_geometry_solid_at", "pt_bicubic_precompute",
     "pt_tricubic_precompute_gpu", "pt_batch_set_snapshots", "pt_batch_snapshots",
+    "pt_batch_set_track", "pt_batch_track",
 ]
 
 _lib = None
@@ -186,6 +189,8 @@ def lib():
     L.pt_batch_set_profiling.argtypes = [vp, C.c_int]
     L.pt_batch_set_snapshots.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.c_int]
     L.pt_batch_snapshots.argtypes = [vp, vp, vp, C.POINTER(C.c_int32)]
+    L.pt_batch_set_track.argtypes = [vp, C.c_double, C.c_int]
+    L.pt_batch_track.argtypes = [vp, vp, vp, C.POINTER(C.c_int32)]
     L.pt_batch_kernel_times.argtypes = [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.pt_track_batch.argtypes = [vp, vp, vp, C.c_int, C.POINTER(Species), C.c_double, C.c_int, ip,
                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),

@@ -106,6 +106,12 @@ struct Hit {
     double pos[3];
 };
 
+/* a step's end state for the track log (TLogger::PrintTrack) and the current solid's ID */
+struct TrackRec {
+    double t, y[9];
+    int solid, pad;
+};
+
 /* a particle's state at a snapshot time (TLogger::PrintSnapshot) with its counters then */
 struct Snap {
     double t, y[9], spin[5], Hmax, noflip;
@@ -126,6 +132,11 @@ struct State {
     int snap_n, snap_cap;
     Snap *snaps;
     int *nsnap;
+    /* track log: length interval (0: off), records [n][trk_cap], records per particle */
+    double trk_interval;
+    int trk_cap;
+    TrackRec *trk;
+    int *ntrk;
     double *Bc, *Vc;          /* field at the stepper's current state, [3][n] and [n] (valid if FL_BVALID) */
     double *pend;             /* pending wall-collision piece: x1, y1[9], x2, y2[9] -> [20][n] */
     double *dense;            /* dense-output state of the last step: [DENSE][n] */

@@ -1494,6 +1494,43 @@ extern "C" int pt_batch_set_snapshots(pt_batch *b, int n_times, const double *ti
     return PT_OK;
 }
 
+/* TLogger::PrintTrack: records of step ends where the trajectory length crossed a multiple of
+   interval; interval = 0 switches the track log off */
+extern "C" int pt_batch_set_track(pt_batch *b, double interval, int cap) {
+    if (!b || interval < 0 || (interval > 0 && cap <= 0)) return fail(PT_EINVAL, "pt_batch_set_track: bad arguments");
+    ptd::State &S = b->P.st;
+    S.trk_interval = 0;
+    if (interval > 0) {
+        ptd::TrackRec *d;
+        int *dn;
+        int r;
+        std::vector<int> zero((size_t)std::max(b->n, 1), 0);
+        if ((r = b->mem.alloc(&d, (size_t)b->n * cap)) || (r = b->mem.upload(&dn, zero.data(), zero.size()))) return r;
+        S.trk = d; S.ntrk = dn; S.trk_cap = cap; S.trk_interval = interval;
+    }
+    for (auto &g : b->graphs) { (void)hipGraphExecDestroy(g.exec); (void)hipGraphDestroy(g.graph); }
+    b->graphs.clear();
+    return PT_OK;
+}
+
+extern "C" int pt_batch_track(pt_batch *b, void *stream, pt_track_point *out, int32_t *count) {
+    if (!b || !out || !count) return fail(PT_EINVAL, "pt_batch_track: bad arguments");
+    const ptd::State &S = b->P.st;
+    if (!(S.trk_interval > 0)) return fail(PT_EINVAL, "pt_batch_track: no track log set");
+    PT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    const size_t n = (size_t)b->n, cap = (size_t)S.trk_cap;
+    std::vector<ptd::TrackRec> h(n * cap);
+    PT_HIP(hipMemcpy(h.data(), S.trk, h.size() * sizeof(ptd::TrackRec), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(count, S.ntrk, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < h.size(); ++k) {
+        out[k].t = h[k].t;
+        for (int q = 0; q < 9; ++q) out[k].y[q] = h[k].y[q];
+        out[k].solid = h[k].solid;
+        out[k].pad = 0;
+    }
+    return PT_OK;
+}
+
 /* the snapshot records: out [n][cap] (cap of pt_batch_set_snapshots), count [n] (may exceed cap) */
 extern "C" int pt_batch_snapshots(pt_batch *b, void *stream, pt_snapshot *out, int32_t *count) {
     if (!b || !out || !count) return fail(PT_EINVAL, "pt_batch_snapshots: bad arguments");

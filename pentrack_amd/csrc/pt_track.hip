@@ -782,6 +782,21 @@ PTD int snapshot_in(const State &S, double x1, double x2) {
     return (lo < S.snap_n && S.snap_t[lo] < x2) ? lo : -1;
 }
 
+/* TLogger::PrintTrack (src/logger.cpp:142-153, src/tracking.cpp:140): the step's end state
+   when the trajectory length crossed a multiple of trackloginterval since the stepper's step
+   start (l1) */
+PTD NOINLINE_SNAP void record_track(const State &S, int i, double l1, double x, const double *y, unsigned solid) {
+    if (y[8] > 0 && (int)(l1 / S.trk_interval) == (int)(y[8] / S.trk_interval)) return;
+    const int k = S.ntrk[i];
+    S.ntrk[i] = k + 1;
+    if (k >= S.trk_cap) return;
+    TrackRec &r = S.trk[(size_t)i * S.trk_cap + k];
+    r.t = x;
+    for (int q = 0; q < 9; ++q) r.y[q] = y[q];
+    r.solid = (int)solid;
+    r.pad = 0;
+}
+
 /* record the stepper's dense-output state at the snapshot time in [x1, x2), if any, with
    the particle's spin and counters at that point of the step */
 template <class DenseSrc>
@@ -1746,6 +1761,8 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
     const bool y_is_xc = at_end && !altered && (L.flags & FL_BVALID);
     spin_step<MODE>(P, i, sp, told, xc[0], xc[1], xc[2], polo, x, y, b1v, B1c[0], B1c[1], B1c[2],
                      y_is_xc, L.Bc[0], L.Bc[1], L.Bc[2], c);
+    if (P.st.trk_interval > 0)
+        record_track(P.st, i, xc[7], x, y, solid_ID(G, current_solid(G, L.solids, L.nsolid)));
     if (L.stop == ID_UNKNOWN && y[6] >= L.tau) L.stop = ID_DECAYED;
     else if (L.stop == ID_UNKNOWN && (x >= tmax || y[8] >= sp.lmax)) L.stop = ID_NOT_FINISH;
     L.x = x;
@@ -1850,6 +1867,8 @@ PTD void collision_path(const Params &P, int i, Ctr &c) {
     } else if (spinHasFlipped) {
         for (int k = 0; k < 5; ++k) spin[k] = 0;
     }
+    if (S.trk_interval > 0)
+        record_track(S, i, st.xo[8], x, y, solid_ID(P.geom, current_solid(P.geom, T.solids, T.nsolid)));
     finish_checks(T, x, y, tau, tmax);
 
     S.t[i] = x;

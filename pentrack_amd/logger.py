@@ -1,4 +1,4 @@
-"""Text end, snapshot and hit logs (TTextLogger, src/logger.cpp:42-140, 198-232, 314-334).
+"""Text end, snapshot, track and hit logs (TTextLogger, src/logger.cpp:42-232, 314-334).
 
 One file per species and log type, named `%012d<particle><suffix>.out` after the job
 number, a header line of the variable names, then one line per particle (end log) or
@@ -23,6 +23,9 @@ DEFAULT_ENDLOGVARS = ["jobnumber", "particle", "tstart", "xstart", "ystart", "zs
                       "spinflipprob", "Nhit", "Nstep", "propert", "trajlength", "Hmax", "wL"]
 
 
+DEFAULT_TRACKLOGVARS = ["jobnumber", "particle", "polarisation", "t", "x", "y", "z", "vx", "vy", "vz", "H", "E",
+                        "Bx", "dBxdx", "dBxdy", "dBxdz", "By", "dBydx", "dBydy", "dBydz", "Bz", "dBzdx", "dBzdy",
+                        "dBzdz", "Ex", "Ey", "Ez", "V"]
 DEFAULT_HITLOGVARS = ["jobnumber", "particle", "t", "x", "y", "z", "v1x", "v1y", "v1z", "pol1", "v2x", "v2y", "v2z",
                       "pol2", "nx", "ny", "nz", "solid1", "solid2"]
 
@@ -137,4 +140,41 @@ def hit_values(hits, n_hits, numbers, jobnumber=0):
          "v2x": rows["v2"][:, 0], "v2y": rows["v2"][:, 1], "v2z": rows["v2"][:, 2], "pol2": rows["pol2"],
          "nx": rows["normal"][:, 0], "ny": rows["normal"][:, 1], "nz": rows["normal"][:, 2],
          "solid1": rows["leaving"].astype(float), "solid2": rows["entering"].astype(float)}
+    return v
+
+
+def track_values(kind, t0, y0, sid0, recs, counts, numbers, geometry, fields, jobnumber=0):
+    """TLogger::PrintTrack variables (src/logger.cpp:142-196): the start point (logged when
+    its trajectory length is 0, src/tracking.cpp:43) and the recorded step ends, particles in
+    order.  H uses the potential energy in the particle's current solid."""
+    from .queries import fields_eval
+    cap = recs.shape[1]
+    k = np.minimum(np.asarray(counts), cap)
+    rows_t, rows_y, rows_s, part = [], [], [], []
+    numbers = np.asarray(numbers, dtype=float)
+    for i in range(len(k)):
+        if not y0[i, 8] > 0:
+            rows_t.append([t0[i]]); rows_y.append(y0[i:i + 1]); rows_s.append([sid0[i]]); part.append([numbers[i]])
+        r = recs[i, :k[i]]
+        rows_t.append(r["t"]); rows_y.append(r["y"]); rows_s.append(r["solid"]); part.append(np.full(k[i], numbers[i]))
+    t = np.concatenate(rows_t) if rows_t else np.zeros(0)
+    y = np.concatenate(rows_y) if rows_y else np.zeros((0, 9))
+    sid = np.concatenate(rows_s).astype(int) if rows_s else np.zeros(0, int)
+    B, dB, V, E = fields_eval(fields, y[:, :3], t, grad=True)
+    Bv, _, _, _ = fields_eval(fields, y[:, :3], t, grad=False)
+    mats = np.stack([m.as_array() for m in geometry.materials.values()])
+    names = list(geometry.materials)
+    id_to_mat = {1: names.index(geometry.default_material)}
+    for s in geometry.solids:
+        id_to_mat[s.ID] = names.index(s.material)
+    ms = np.array([id_to_mat.get(int(i), id_to_mat[1]) for i in sid], dtype=int)
+    Ek = kinetic_energy(kind, y[:, 3:6])
+    H = Ek + potential_energy(kind, y, Bv, V, mats, ms)
+    v = {"jobnumber": np.full(len(t), float(jobnumber)), "particle": np.concatenate(part) if part else np.zeros(0),
+         "polarisation": y[:, 7], "t": t, "x": y[:, 0], "y": y[:, 1], "z": y[:, 2], "vx": y[:, 3], "vy": y[:, 4],
+         "vz": y[:, 5], "H": H, "E": Ek, "Ex": E[:, 0], "Ey": E[:, 1], "Ez": E[:, 2], "V": V}
+    for a, ax in enumerate("xyz"):
+        v["B" + ax] = B[:, a]
+        for b, bx in enumerate("xyz"):
+            v[f"dB{ax}d{bx}"] = dB[:, a, b]
     return v

@@ -44,6 +44,7 @@ class Batch:
         self.n = len(t0)
         self.hit_cap = hit_cap
         self.snap_cap = 0
+        self.trk_cap = 0
         sp = (_native.Species * len(species))(*[s.native() for s in species])
         self._keep = [np.ascontiguousarray(species_idx, dtype=np.int32), np.ascontiguousarray(t0, dtype=np.float64),
                       np.ascontiguousarray(y0, dtype=np.float64).reshape(-1, 9),
@@ -112,6 +113,20 @@ class Batch:
                                                        cnt.ctypes.data_as(C.POINTER(C.c_int32))), "pt_batch_snapshots")
         return rec.reshape(self.n, self.snap_cap), cnt
 
+    def set_track(self, interval, cap):
+        """TLogger::PrintTrack records: step ends where the trajectory length crossed a
+        multiple of `interval` (m), at most cap per particle; call before the first run."""
+        self.trk_cap = int(cap) if interval > 0 else 0
+        _native.check(_native.lib().pt_batch_set_track(self.h, float(interval), self.trk_cap), "pt_batch_set_track")
+
+    def track(self, stream=None):
+        """(records [n][cap] TRACK_DTYPE, counts [n]): the particles' track-log points."""
+        rec = np.zeros(self.n * self.trk_cap, dtype=_native.TRACK_DTYPE)
+        cnt = np.zeros(self.n, dtype=np.int32)
+        _native.check(_native.lib().pt_batch_track(self.h, stream, rec.ctypes.data_as(C.c_void_p),
+                                                   cnt.ctypes.data_as(C.POINTER(C.c_int32))), "pt_batch_track")
+        return rec.reshape(self.n, self.trk_cap), cnt
+
     def download(self, stream=None):
         out = np.zeros(self.n, dtype=_native.OUT_DTYPE)
         hits = np.zeros(self.n * self.hit_cap, dtype=_native.HIT_DTYPE) if self.hit_cap else None
@@ -176,18 +191,24 @@ class TTracker:
         self.fields = fields
         self.launch = launch or LaunchConfig()
 
-    def IntegrateBatch(self, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap=0, snapshots=None):
-        """(out, hits), or with snapshots = (times, cap) (out, hits, records, counts)."""
+    def IntegrateBatch(self, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap=0, snapshots=None,
+                       track=None):
+        """(out, hits); with snapshots = (times, cap) also (records, counts) of the snapshots,
+        with track = (interval, cap) also (records, counts) of the track log."""
         cfg = self.launch
         b = Batch(self.geometry, self.fields, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap,
                   with_fp32=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
         if snapshots is not None:
             b.set_snapshots(*snapshots)
+        if track is not None:
+            b.set_track(*track)
         b.run_to_completion(cfg)
-        out, hits = b.download()
-        if snapshots is None:
-            return out, hits
-        return (out, hits) + b.snapshots()
+        res = b.download()
+        if snapshots is not None:
+            res = res + b.snapshots()
+        if track is not None:
+            res = res + b.track()
+        return res
 
     def IntegrateWithSecondaries(self, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap=0):
         """The reference's main loop for one batch (src/main.cpp:153-172): the primaries,

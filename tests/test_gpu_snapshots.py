@@ -56,3 +56,26 @@ def test_snapshots_on_a_parabola(gpu):
         want_z = w.y0[:, 2] + w.y0[:, 5] * t - 0.5 * g * t * t
         assert np.allclose(r["y"][:, 2], want_z, rtol=0, atol=1e-9)
         assert np.allclose(r["y"][:, 0], w.y0[:, 0] + w.y0[:, 3] * t, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_track_log_matches_oracle(gpu, oracle_lib, persistent):
+    """TLogger::PrintTrack records (src/tracking.cpp:140): step ends where the trajectory
+    length crossed a multiple of the interval, with the current solid, bit for bit."""
+    w = workloads.cylinder(n=1000, seed=8, tmax=1.0, n_grid=24)
+    b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid)
+    b.set_track(0.05, 64)
+    print("case: track log cylinder n=1000 persistent", persistent, flush=True)
+    assert b.run_to_completion(LaunchConfig(persistent=persistent), max_launches=100000)
+    out, _ = b.download()
+    rec, cnt = b.track()
+    o, orec, ocnt = oracle_lib.track_tracklog(w, 0.05, 64, threads=8)
+    assert np.array_equal(out["y"], o["y"]) and np.array_equal(cnt, ocnt)
+    assert (cnt > 3).mean() > 0.9                    # counts go on past the 64 records kept
+    kept = np.minimum(cnt, 64)
+    for k in range(w.n):
+        assert np.array_equal(rec[k, :kept[k]], orec[k, :kept[k]]), k
+    # consecutive records are at least one interval apart in trajectory length
+    for k in range(0, w.n, 50):
+        L = rec["y"][k, :kept[k], 8]
+        assert (np.floor(L / 0.05)[1:] > np.floor(L / 0.05)[:-1]).all()
