@@ -785,7 +785,7 @@ PTD int snapshot_in(const State &S, double x1, double x2) {
 /* TLogger::PrintTrack (src/logger.cpp:142-153, src/tracking.cpp:140): the step's end state
    when the trajectory length crossed a multiple of trackloginterval since the stepper's step
    start (l1) */
-PTD NOINLINE_SNAP void record_track(const State &S, int i, double l1, double x, const double *y, unsigned solid) {
+PTD void record_track(const State &S, int i, double l1, double x, const double *y, unsigned solid) {
     if (y[8] > 0 && (int)(l1 / S.trk_interval) == (int)(y[8] / S.trk_interval)) return;
     const int k = S.ntrk[i];
     S.ntrk[i] = k + 1;
