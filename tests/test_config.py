@@ -126,6 +126,11 @@ def test_reference_config_file_parses():
     assert C.global_settings(cfg).simcount > 0
     mats = C.load_materials(cfg)
     assert len(mats) > 10 and all(m.name for m in mats)
+    # its PARTICLES section asks for BMT spin tracking between 500 and 700 s, which this
+    # engine does not do: the species settings are refused, not silently changed
+    with pytest.raises(C.ConfigError, match="spintimes"):
+        C.species_config(cfg, "neutron")
+    cfg["PARTICLES"]["spintimes"] = ""
     assert C.species_config(cfg, "neutron").tau > 800
 
 
@@ -228,3 +233,16 @@ def test_source_modes_and_parameters(tmp_path):
     write_stl(str(tmp_path / "src.STL"), box_mesh((0, 0, 0), (1, 2, 3)), "src")
     s = create_source(cfg_with("sourcemode STLvolume\nSTLfile src.STL\n"), str(tmp_path))
     assert np.array_equal(s.bbox[0], [0, 0, 0]) and np.array_equal(s.bbox[1], [1, 2, 3])
+
+
+def test_bmt_spin_tracking_is_rejected():
+    """Spin-tracking windows would switch the reference to BMT integration; the engine
+    refuses them instead of silently transporting the spin along B."""
+    cfg = C.TConfig(text=CONFIG + "\n[neutron]\n")
+    cfg["neutron"]["spintimes"] = " 500 700"
+    with pytest.raises(C.ConfigError, match="spintimes"):
+        C.species_config(cfg, "neutron")
+    cfg["neutron"]["spintimes"] = " "
+    cfg["neutron"]["spinlog"] = " 1"
+    with pytest.raises(C.ConfigError, match="spinlog"):
+        C.species_config(cfg, "neutron")
