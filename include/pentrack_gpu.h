@@ -173,6 +173,8 @@ typedef struct {
     double lmax;       /* particleconf "lmax" */
     int flipspin;      /* particleconf "flipspin" */
     double Bmax;       /* particleconf "Bmax" */
+    int n_spintimes;   /* particleconf "spintimes": BMT spin tracking in [spintimes[2k], spintimes[2k+1]) */
+    double spintimes[8];
 } pt_species;
 
 typedef struct {

@@ -1748,6 +1748,162 @@ static void print_snapshot(track_t *tp, const dopri_t *st, double x1, double x2,
     }
 }
 
+/* ---- BMT spin tracking (src/tracking.cpp:395-450, src/particle.cpp:141-180) ---- */
+
+/* TParticle::SpinPrecessionAxis (src/particle.cpp:150-174) for the field and the
+   trajectory's time derivative dydt */
+static void spin_precession_axis(const track_t *tp, const double B[3], const double E[3], const double *dydt,
+                                 double O[3]) {
+    const ld gamma = tp->sp.gamma;                       /* the member is a long double */
+    double v2 = dydt[0] * dydt[0] + dydt[1] * dydt[1] + dydt[2] * dydt[2];
+    double gamma_rel = 1. / SQRT_LD(1. - v2 / C_0 / C_0);
+    double Bdotv = B[0] * dydt[0] + B[1] * dydt[1] + B[2] * dydt[2];
+    double Bp[3];
+    for (int j = 0; j < 3; j++) Bp[j] = Bdotv * dydt[j] / v2;
+    double OB[3], OT[3];
+    OB[0] = -gamma / gamma_rel * ((1 - gamma_rel) * Bp[0] + gamma_rel * B[0] - gamma_rel * (dydt[1] * E[2] - dydt[2] * E[1]) / C_0 / C_0);
+    OB[1] = -gamma / gamma_rel * ((1 - gamma_rel) * Bp[1] + gamma_rel * B[1] - gamma_rel * (dydt[2] * E[0] - dydt[0] * E[2]) / C_0 / C_0);
+    OB[2] = -gamma / gamma_rel * ((1 - gamma_rel) * Bp[2] + gamma_rel * B[2] - gamma_rel * (dydt[0] * E[1] - dydt[1] * E[0]) / C_0 / C_0);
+    OT[0] = gamma_rel * gamma_rel / (gamma_rel + 1) * (dydt[4] * dydt[2] - dydt[5] * dydt[1]) / C_0 / C_0;
+    OT[1] = gamma_rel * gamma_rel / (gamma_rel + 1) * (dydt[5] * dydt[0] - dydt[3] * dydt[2]) / C_0 / C_0;
+    OT[2] = gamma_rel * gamma_rel / (gamma_rel + 1) * (dydt[3] * dydt[1] - dydt[4] * dydt[0]) / C_0 / C_0;
+    O[0] = OB[0] + OT[0];
+    O[1] = OB[1] + OT[1];
+    O[2] = OB[2] + OT[2];
+}
+
+/* TParticle::SpinDerivs without field interpolation (src/particle.cpp:177-190): the
+   precession axis from the trajectory's dense output at t (SpinPrecessionAxis with the
+   stepper, src/particle.cpp:141-148) */
+static void spin_derivs(track_t *tp, const dopri_t *traj, const double *y, double *dydx, double t) {
+    double yt[9], dydt[9], B[3], dB[3][3], E[3] = {0, 0, 0}, V = 0, O[3];
+    dopri_calc_state(tp, traj, t, yt);
+    field_B(tp, yt[0], yt[1], yt[2], t, B, dB);
+    field_E(tp, yt[0], yt[1], yt[2], t, &V, E);
+    eom(&tp->sp, yt, B, dB, E, dydt);
+    spin_precession_axis(tp, B, E, dydt, O);
+    dydx[0] = O[1] * y[2] - O[2] * y[1];
+    dydx[1] = O[2] * y[0] - O[0] * y[2];
+    dydx[2] = O[0] * y[1] - O[1] * y[0];
+    dydx[3] = 1.;
+    dydx[4] = sqrt(O[0] * O[0] + O[1] * O[1] + O[2] * O[2]);
+}
+
+/* the spin stepper: make_dense_output(1e-12, 1e-12, runge_kutta_dopri5) on the five spin
+   variables, the same algorithm as the trajectory's */
+typedef struct {
+    double xc[5], dxc[5], xo[5], dxo[5], k3[5], k4[5], k5[5], k6[5];
+    double t, t_old, dt;
+} sdopri_t;
+
+static int sdopri_try(track_t *tp, const dopri_t *traj, sdopri_t *s, double xnew[5], double dxnew[5]) {
+    const double a2 = 1.0 / 5, a3 = 3.0 / 10, a4 = 4.0 / 5, a5 = 8.0 / 9;
+    const double b21 = 1.0 / 5;
+    const double b31 = 3.0 / 40, b32 = 9.0 / 40;
+    const double b41 = 44.0 / 45, b42 = -56.0 / 15, b43 = 32.0 / 9;
+    const double b51 = 19372.0 / 6561, b52 = -25360.0 / 2187, b53 = 64448.0 / 6561, b54 = -212.0 / 729;
+    const double b61 = 9017.0 / 3168, b62 = -355.0 / 33, b63 = 46732.0 / 5247, b64 = 49.0 / 176, b65 = -5103.0 / 18656;
+    const double c1 = 35.0 / 384, c3 = 500.0 / 1113, c4 = 125.0 / 192, c5 = -2187.0 / 6784, c6 = 11.0 / 84;
+    const double dc1 = c1 - 5179.0 / 57600, dc3 = c3 - 7571.0 / 16695, dc4 = c4 - 393.0 / 640,
+                 dc5 = c5 - (-92097.0 / 339200), dc6 = c6 - 187.0 / 2100, dc7 = -1.0 / 40;
+    const double *x = s->xc, *k1 = s->dxc;
+    double dt = s->dt, t = s->t;
+    double tmp[5], k2[5], xerr[5];
+    int i;
+    for (i = 0; i < 5; ++i) tmp[i] = 1.0 * x[i] + (dt * b21) * k1[i];
+    spin_derivs(tp, traj, tmp, k2, t + dt * a2);
+    for (i = 0; i < 5; ++i) tmp[i] = 1.0 * x[i] + (dt * b31) * k1[i] + (dt * b32) * k2[i];
+    spin_derivs(tp, traj, tmp, s->k3, t + dt * a3);
+    for (i = 0; i < 5; ++i) tmp[i] = 1.0 * x[i] + (dt * b41) * k1[i] + (dt * b42) * k2[i] + (dt * b43) * s->k3[i];
+    spin_derivs(tp, traj, tmp, s->k4, t + dt * a4);
+    for (i = 0; i < 5; ++i) tmp[i] = 1.0 * x[i] + (dt * b51) * k1[i] + (dt * b52) * k2[i] + (dt * b53) * s->k3[i] + (dt * b54) * s->k4[i];
+    spin_derivs(tp, traj, tmp, s->k5, t + dt * a5);
+    for (i = 0; i < 5; ++i) tmp[i] = 1.0 * x[i] + (dt * b61) * k1[i] + (dt * b62) * k2[i] + (dt * b63) * s->k3[i] + (dt * b64) * s->k4[i] + (dt * b65) * s->k5[i];
+    spin_derivs(tp, traj, tmp, s->k6, t + dt);
+    for (i = 0; i < 5; ++i) xnew[i] = 1.0 * x[i] + (dt * c1) * k1[i] + (dt * c3) * s->k3[i] + (dt * c4) * s->k4[i] + (dt * c5) * s->k5[i] + (dt * c6) * s->k6[i];
+    spin_derivs(tp, traj, xnew, dxnew, t + dt);
+    for (i = 0; i < 5; ++i) xerr[i] = (dt * dc1) * k1[i] + (dt * dc3) * s->k3[i] + (dt * dc4) * s->k4[i] + (dt * dc5) * s->k5[i] + (dt * dc6) * s->k6[i] + (dt * dc7) * dxnew[i];
+    double err = 0;
+    for (i = 0; i < 5; ++i) {
+        double e = fabs(xerr[i]) / (1e-12 + 1e-12 * (1.0 * fabs(x[i]) + dt * fabs(k1[i])));
+        err = (err < e) ? e : err;
+    }
+    if (err > 1.0) {
+        double f = 9.0 / 10 * R_POW(err, -1.0 / 3);
+        double m = 1.0 / 5;
+        s->dt = dt * ((f < m) ? m : f);
+        return 0;
+    }
+    s->t_old = t;
+    s->t = t + dt;
+    if (err < 0.5) {
+        double emin = R_POW(5.0, -5.0);
+        err = (emin < err) ? err : emin;
+        s->dt = dt * (9.0 / 10 * R_POW(err, -1.0 / 5));
+    }
+    return 1;
+}
+
+static void sdopri_calc_state(const sdopri_t *s, double t, double *x) {
+    const double b1 = 35.0 / 384, b3 = 500.0 / 1113, b4 = 125.0 / 192, b5 = -2187.0 / 6784, b6 = 11.0 / 84;
+    const double dt = (s->t - s->t_old);
+    const double theta = (t - s->t_old) / dt;
+    const double X1 = 5.0 * (2558722523.0 - 31403016.0 * theta) / 11282082432.0;
+    const double X3 = 100.0 * (882725551.0 - 15701508.0 * theta) / 32700410799.0;
+    const double X4 = 25.0 * (443332067.0 - 31403016.0 * theta) / 1880347072.0;
+    const double X5 = 32805.0 * (23143187.0 - 3489224.0 * theta) / 199316789632.0;
+    const double X6 = 55.0 * (29972135.0 - 7076736.0 * theta) / 822651844.0;
+    const double X7 = 10.0 * (7414447.0 - 829305.0 * theta) / 29380423.0;
+    const double theta_m_1 = theta - 1.0;
+    const double theta_sq = theta * theta;
+    const double A = theta_sq * (3.0 - 2.0 * theta);
+    const double B = theta_sq * theta_m_1;
+    const double C = theta_sq * theta_m_1 * theta_m_1;
+    const double D = theta * theta_m_1 * theta_m_1;
+    const double b1t = A * b1 - C * X1 + D;
+    const double b3t = A * b3 + C * X3;
+    const double b4t = A * b4 - C * X4;
+    const double b5t = A * b5 + C * X5;
+    const double b6t = A * b6 - C * X6;
+    const double b7t = B + C * X7;
+    for (int i = 0; i < 5; ++i)
+        x[i] = 1.0 * s->xo[i] + (dt * b1t) * s->dxo[i] + (dt * b3t) * s->k3[i] + (dt * b4t) * s->k4[i] + (dt * b5t) * s->k5[i] + (dt * b6t) * s->k6[i] + (dt * b7t) * s->dxc[i];
+}
+
+/* the spin integration of IntegrateSpin (src/tracking.cpp:424-450): the spin stepper from x1
+   with a first step of half a Larmor period, steps until it passes x2, the state at x2 from
+   its dense output; returns -1 if a step fails 500 times */
+static int bmt_integrate(track_t *tp, double *spin, const dopri_t *traj, double x1, double x2, double Babs1) {
+    sdopri_t s;
+    memcpy(s.xc, spin, sizeof(s.xc));
+    s.t = x1;
+    s.dt = fabs(PI / (double)tp->sp.gamma / Babs1);    /* GetGyromagneticRatio() is a double */
+    spin_derivs(tp, traj, s.xc, s.dxc, s.t);
+    for (;;) {
+        int tries = 0, ok;
+        for (;;) {
+            double xnew[5], dxnew[5];
+            ok = sdopri_try(tp, traj, &s, xnew, dxnew);
+            if (ok) {
+                memcpy(s.xo, s.xc, sizeof(s.xo));
+                memcpy(s.dxo, s.dxc, sizeof(s.dxo));
+                memcpy(s.xc, xnew, sizeof(s.xc));
+                memcpy(s.dxc, dxnew, sizeof(s.dxc));
+                break;
+            }
+            if (tries++ >= 500) return -1;
+        }
+        double t = s.t;
+        if (t > x2) {
+            t = x2;
+            sdopri_calc_state(&s, t, spin);
+        } else {
+            memcpy(spin, s.xc, 5 * sizeof(double));
+        }
+        if (t >= x2) return 0;
+    }
+}
+
 static double integrate_spin(track_t *tp, double *spin, const dopri_t *st, double x2, const double *y2, double Bmax) {
     double x1 = st->t_old;
     const double *y1 = st->xo;
@@ -1764,8 +1920,21 @@ static double integrate_spin(track_t *tp, double *spin, const dopri_t *st, doubl
         for (int i = 0; i < 5; ++i) spin[i] = 0;
         return sp;
     } else if ((x2 > x1) && (Babs1 < Bmax || Babs2 < Bmax)) {
-        /* no spin-tracking time windows configured: parallel transport */
-        if (spin[0] != 0 || spin[1] != 0 || spin[2] != 0) {
+        int integrate1 = 0, integrate2 = 0;
+        const ptref_species *cfg = tp->cfg;
+        for (int i = 0; i + 1 < cfg->n_spintimes; i += 2) {
+            integrate1 |= (x1 >= cfg->spintimes[i] && x1 < cfg->spintimes[i + 1]);
+            integrate2 |= (x2 >= cfg->spintimes[i] && x2 < cfg->spintimes[i + 1]);
+        }
+        if (integrate1 || integrate2) {
+            if (spin[0] == 0 && spin[1] == 0 && spin[2] == 0) {
+                spin[0] = B1[0] * y1[7] / Babs1;
+                spin[1] = B1[1] * y1[7] / Babs1;
+                spin[2] = B1[2] * y1[7] / Babs1;
+            }
+            if (bmt_integrate(tp, spin, st, x1, x2, Babs1) < 0) tp->stopID = ID_ODEINT_ERROR;
+        } else if (spin[0] != 0 || spin[1] != 0 || spin[2] != 0) {
+            /* outside the windows: parallel transport along B */
             double sp = (spin[0] * B1[0] + spin[1] * B1[1] + spin[2] * B1[2]) / Babs1 / sqrt(spin[0] * spin[0] + spin[1] * spin[1] + spin[2] * spin[2]);
             if (sp * sp >= 1) { spin[0] = 0; spin[1] = 0; }
             else {

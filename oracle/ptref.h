@@ -74,6 +74,8 @@ typedef struct {
     double lmax;              /* max trajectory length */
     int flipspin;
     double Bmax;
+    int n_spintimes;          /* spin-tracking windows [spintimes[2k], spintimes[2k+1]) (BMT) */
+    double spintimes[8];
 } ptref_species;
 
 typedef struct {

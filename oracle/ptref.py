@@ -53,7 +53,7 @@ class Field(C.Structure):
 
 class Species(C.Structure):
     _fields_ = [("kind", C.c_int), ("tau", C.c_double), ("tmax", C.c_double), ("lmax", C.c_double),
-                ("flipspin", C.c_int), ("Bmax", C.c_double)]
+                ("flipspin", C.c_int), ("Bmax", C.c_double), ("n_spintimes", C.c_int), ("spintimes", C.c_double * 8)]
 
 
 OUT_DTYPE = np.dtype([
@@ -259,6 +259,10 @@ def track(workload, mode="fp64", hit_cap=0, threads=1, sel=None, problem=None):
     for k, s in enumerate(workload.species):
         sp[k].kind, sp[k].tau, sp[k].tmax, sp[k].lmax, sp[k].flipspin, sp[k].Bmax = (
             s.kind, s.tau, s.tmax, s.lmax, s.flipspin, s.Bmax)
+        st = list(getattr(s, "spintimes", ()))[:8]
+        sp[k].n_spintimes = len(st)
+        for q, v in enumerate(st):
+            sp[k].spintimes[q] = v
     si = np.ascontiguousarray(workload.species_idx[idx], dtype=np.int32)
     t0 = np.ascontiguousarray(workload.t0[idx])
     y0 = np.ascontiguousarray(workload.y0[idx])

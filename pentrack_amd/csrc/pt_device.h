@@ -97,6 +97,8 @@ struct Species {
     int kind;
     double q, m, mu, gamma, tau, tmax, lmax, Bmax;
     int flipspin;
+    int n_spintimes;          /* BMT spin-tracking windows [spintimes[2k], spintimes[2k+1]) */
+    double spintimes[8];
 };
 
 struct Hit {

@@ -1164,6 +1164,10 @@ extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_s
         const pt_species &q = species[s];
         sp[s].kind = q.kind; sp[s].q = q.q; sp[s].m = q.m; sp[s].mu = q.mu; sp[s].gamma = q.gamma;
         sp[s].tau = q.tau; sp[s].tmax = q.tmax; sp[s].lmax = q.lmax; sp[s].Bmax = q.Bmax; sp[s].flipspin = q.flipspin;
+        if (q.n_spintimes < 0 || q.n_spintimes > 8 || (q.n_spintimes & 1))
+            return fail(PT_EINVAL, "pt_batch_create: spintimes needs pairs of times, at most four windows");
+        sp[s].n_spintimes = q.n_spintimes;
+        for (int k = 0; k < 8; ++k) sp[s].spintimes[k] = k < q.n_spintimes ? q.spintimes[k] : 0.0;
     }
     int r;
     if ((r = B->mem.upload(&B->d_species, sp.data(), sp.size()))) return r;

@@ -1284,7 +1284,130 @@ PTD void do_polarize(Track &T, double *y, double polarization, int flipspin) {
     }
 }
 
-/* TTracker::IntegrateSpin without the BMT branch (src/tracking.cpp:373-394, 473-483) */
+/* spin-tracking windows of IntegrateSpin (src/tracking.cpp:397-400) */
+PTD bool spin_window(const Species &sp, double t) {
+    bool in = false;
+    for (int k = 0; k + 1 < sp.n_spintimes; k += 2) in |= (t >= sp.spintimes[k] && t < sp.spintimes[k + 1]);
+    return in;
+}
+
+/* TParticle::SpinDerivs without field interpolation (src/particle.cpp:141-190): the
+   precession axis (BMT with Thomas precession) from the trajectory's dense output at t */
+template <int MODE>
+PTD void spin_derivs(const Params &P, Track &T, const Dopri &traj, const double *y, double *dydx, double t) {
+    double yt[9], d9[9], B[3], dB[3][3], E[3] = {0, 0, 0}, V = 0;
+    dopri_calc_state(traj, t, yt);
+    field_B<MODE>(P, yt[0], yt[1], yt[2], t, B, dB, true, T.c.field);
+    field_E<MODE>(P, yt[0], yt[1], yt[2], t, &V, E, T.c.field);
+    eom(T.sp, yt, B, dB, E, d9);
+    const double gamma = T.sp.gamma;
+    const double v2 = d9[0] * d9[0] + d9[1] * d9[1] + d9[2] * d9[2];
+    const double gamma_rel = 1. / __builtin_sqrt(1. - v2 / C_0 / C_0);
+    const double Bdotv = B[0] * d9[0] + B[1] * d9[1] + B[2] * d9[2];
+    double Bp[3];
+    for (int j = 0; j < 3; j++) Bp[j] = Bdotv * d9[j] / v2;
+    double O[3];
+    O[0] = -gamma / gamma_rel * ((1 - gamma_rel) * Bp[0] + gamma_rel * B[0] - gamma_rel * (d9[1] * E[2] - d9[2] * E[1]) / C_0 / C_0) +
+           gamma_rel * gamma_rel / (gamma_rel + 1) * (d9[4] * d9[2] - d9[5] * d9[1]) / C_0 / C_0;
+    O[1] = -gamma / gamma_rel * ((1 - gamma_rel) * Bp[1] + gamma_rel * B[1] - gamma_rel * (d9[2] * E[0] - d9[0] * E[2]) / C_0 / C_0) +
+           gamma_rel * gamma_rel / (gamma_rel + 1) * (d9[5] * d9[0] - d9[3] * d9[2]) / C_0 / C_0;
+    O[2] = -gamma / gamma_rel * ((1 - gamma_rel) * Bp[2] + gamma_rel * B[2] - gamma_rel * (d9[0] * E[1] - d9[1] * E[0]) / C_0 / C_0) +
+           gamma_rel * gamma_rel / (gamma_rel + 1) * (d9[3] * d9[1] - d9[4] * d9[0]) / C_0 / C_0;
+    dydx[0] = O[1] * y[2] - O[2] * y[1];
+    dydx[1] = O[2] * y[0] - O[0] * y[2];
+    dydx[2] = O[0] * y[1] - O[1] * y[0];
+    dydx[3] = 1.;
+    dydx[4] = __builtin_sqrt(O[0] * O[0] + O[1] * O[1] + O[2] * O[2]);
+}
+
+/* the spin integration of IntegrateSpin (src/tracking.cpp:424-450): a dense-output dopri5
+   (rtol = atol = 1e-12) on the five spin variables from x1, first step half a Larmor
+   period, until it passes x2; the state at x2 from its dense output.  false if a step
+   fails 500 times. */
+template <int MODE>
+PTD_NOINLINE bool bmt_integrate(const Params &P, Track &T, const Dopri &traj, double *spin, double x1, double x2,
+                                double Babs1) {
+    constexpr double a2 = 1.0 / 5, a3 = 3.0 / 10, a4 = 4.0 / 5, a5 = 8.0 / 9;
+    constexpr double b21 = 1.0 / 5, b31 = 3.0 / 40, b32 = 9.0 / 40, b41 = 44.0 / 45, b42 = -56.0 / 15, b43 = 32.0 / 9;
+    constexpr double b51 = 19372.0 / 6561, b52 = -25360.0 / 2187, b53 = 64448.0 / 6561, b54 = -212.0 / 729;
+    constexpr double b61 = 9017.0 / 3168, b62 = -355.0 / 33, b63 = 46732.0 / 5247, b64 = 49.0 / 176, b65 = -5103.0 / 18656;
+    constexpr double c1 = 35.0 / 384, c3 = 500.0 / 1113, c4 = 125.0 / 192, c5 = -2187.0 / 6784, c6 = 11.0 / 84;
+    constexpr double dc1 = c1 - 5179.0 / 57600, dc3 = c3 - 7571.0 / 16695, dc4 = c4 - 393.0 / 640,
+                     dc5 = c5 - (-92097.0 / 339200), dc6 = c6 - 187.0 / 2100, dc7 = -1.0 / 40;
+    double x[5], k1[5], xo[5], k1o[5], k3[5], k4[5], k5[5], k6[5];
+    for (int i = 0; i < 5; ++i) x[i] = spin[i];
+    double t = x1, told = x1;
+    double dt = __builtin_fabs(PI / T.sp.gamma / Babs1);
+    spin_derivs<MODE>(P, T, traj, x, k1, t);
+    for (;;) {
+        int tries = 0;
+        for (;;) {
+            double tmp[5], k2[5], xn[5], kn[5];
+            for (int i = 0; i < 5; ++i) tmp[i] = 1.0 * x[i] + (dt * b21) * k1[i];
+            spin_derivs<MODE>(P, T, traj, tmp, k2, t + dt * a2);
+            for (int i = 0; i < 5; ++i) tmp[i] = 1.0 * x[i] + (dt * b31) * k1[i] + (dt * b32) * k2[i];
+            spin_derivs<MODE>(P, T, traj, tmp, k3, t + dt * a3);
+            for (int i = 0; i < 5; ++i) tmp[i] = 1.0 * x[i] + (dt * b41) * k1[i] + (dt * b42) * k2[i] + (dt * b43) * k3[i];
+            spin_derivs<MODE>(P, T, traj, tmp, k4, t + dt * a4);
+            for (int i = 0; i < 5; ++i)
+                tmp[i] = 1.0 * x[i] + (dt * b51) * k1[i] + (dt * b52) * k2[i] + (dt * b53) * k3[i] + (dt * b54) * k4[i];
+            spin_derivs<MODE>(P, T, traj, tmp, k5, t + dt * a5);
+            for (int i = 0; i < 5; ++i)
+                tmp[i] = 1.0 * x[i] + (dt * b61) * k1[i] + (dt * b62) * k2[i] + (dt * b63) * k3[i] + (dt * b64) * k4[i] + (dt * b65) * k5[i];
+            spin_derivs<MODE>(P, T, traj, tmp, k6, t + dt);
+            for (int i = 0; i < 5; ++i)
+                xn[i] = 1.0 * x[i] + (dt * c1) * k1[i] + (dt * c3) * k3[i] + (dt * c4) * k4[i] + (dt * c5) * k5[i] + (dt * c6) * k6[i];
+            spin_derivs<MODE>(P, T, traj, xn, kn, t + dt);
+            double err = 0;
+            for (int i = 0; i < 5; ++i) {
+                double xe = (dt * dc1) * k1[i] + (dt * dc3) * k3[i] + (dt * dc4) * k4[i] + (dt * dc5) * k5[i] + (dt * dc6) * k6[i] + (dt * dc7) * kn[i];
+                double e = __builtin_fabs(xe) / (1e-12 + 1e-12 * (1.0 * __builtin_fabs(x[i]) + dt * __builtin_fabs(k1[i])));
+                err = (err < e) ? e : err;
+            }
+            if (err > 1.0) {
+                double f = 9.0 / 10 * ptm_pow(err, -1.0 / 3);
+                double m = 1.0 / 5;
+                dt = dt * ((f < m) ? m : f);
+                if (tries++ >= 500) return false;
+                continue;
+            }
+            told = t;
+            t = t + dt;
+            if (err < 0.5) {
+                double emin = ptm_pow(5.0, -5.0);
+                err = (emin < err) ? err : emin;
+                dt = dt * (9.0 / 10 * ptm_pow(err, -1.0 / 5));
+            }
+            for (int i = 0; i < 5; ++i) { xo[i] = x[i]; k1o[i] = k1[i]; x[i] = xn[i]; k1[i] = kn[i]; }
+            break;
+        }
+        if (t > x2) {
+            /* runge_kutta_dopri5::calc_state at x2 */
+            constexpr double e1 = 35.0 / 384, e3 = 500.0 / 1113, e4 = 125.0 / 192, e5 = -2187.0 / 6784, e6 = 11.0 / 84;
+            const double h = (t - told);
+            const double theta = (x2 - told) / h;
+            const double X1 = 5.0 * (2558722523.0 - 31403016.0 * theta) / 11282082432.0;
+            const double X3 = 100.0 * (882725551.0 - 15701508.0 * theta) / 32700410799.0;
+            const double X4 = 25.0 * (443332067.0 - 31403016.0 * theta) / 1880347072.0;
+            const double X5 = 32805.0 * (23143187.0 - 3489224.0 * theta) / 199316789632.0;
+            const double X6 = 55.0 * (29972135.0 - 7076736.0 * theta) / 822651844.0;
+            const double X7 = 10.0 * (7414447.0 - 829305.0 * theta) / 29380423.0;
+            const double tm1 = theta - 1.0, tsq = theta * theta;
+            const double A = tsq * (3.0 - 2.0 * theta), B = tsq * tm1, C = tsq * tm1 * tm1, D = theta * tm1 * tm1;
+            const double b1t = A * e1 - C * X1 + D, b3t = A * e3 + C * X3, b4t = A * e4 - C * X4;
+            const double b5t = A * e5 + C * X5, b6t = A * e6 - C * X6, b7t = B + C * X7;
+            for (int i = 0; i < 5; ++i)
+                spin[i] = 1.0 * xo[i] + (h * b1t) * k1o[i] + (h * b3t) * k3[i] + (h * b4t) * k4[i] + (h * b5t) * k5[i] +
+                          (h * b6t) * k6[i] + (h * b7t) * k1[i];
+            return true;
+        }
+        for (int i = 0; i < 5; ++i) spin[i] = x[i];
+        if (t >= x2) return true;
+    }
+}
+
+/* TTracker::IntegrateSpin (src/tracking.cpp:373-483) on the collision path: the BMT
+   integration inside the spin-tracking windows, parallel transport outside */
 template <int MODE>
 PTD_NOINLINE double integrate_spin(const Params &P, Track &T, double *spin, const Dopri &st, double x2, const double *y2) {
     double x1 = st.t_old;
@@ -1303,7 +1426,14 @@ PTD_NOINLINE double integrate_spin(const Params &P, Track &T, double *spin, cons
         for (int i = 0; i < 5; ++i) spin[i] = 0;
         return sp;
     } else if ((x2 > x1) && (Babs1 < Bmax || Babs2 < Bmax)) {
-        if (spin[0] != 0 || spin[1] != 0 || spin[2] != 0) {
+        if (T.sp.n_spintimes > 0 && (spin_window(T.sp, x1) || spin_window(T.sp, x2))) {
+            if (spin[0] == 0 && spin[1] == 0 && spin[2] == 0) {
+                spin[0] = B1[0] * y1[7] / Babs1;
+                spin[1] = B1[1] * y1[7] / Babs1;
+                spin[2] = B1[2] * y1[7] / Babs1;
+            }
+            if (!bmt_integrate<MODE>(P, T, st, spin, x1, x2, Babs1)) T.stop = ID_ODEINT_ERROR;
+        } else if (spin[0] != 0 || spin[1] != 0 || spin[2] != 0) {
             double sp = (spin[0] * B1[0] + spin[1] * B1[1] + spin[2] * B1[2]) / Babs1 / __builtin_sqrt(spin[0] * spin[0] + spin[1] * spin[1] + spin[2] * spin[2]);
             if (sp * sp >= 1) { spin[0] = 0; spin[1] = 0; }
             else {
@@ -1449,9 +1579,10 @@ PTD bool do_step_hook_A(const Params &P, const Species &sp, Lane &L, int i, cons
 /* TTracker::IntegrateSpin without the BMT branch (src/tracking.cpp:373-394, 473-483) and
    TParticle::DoPolarize (src/particle.cpp:225-234) on the step path.  B1/B2: the field at the
    step start / end when the stepper evaluated it there (else nullptr).  Returns the new
-   tracker polarisation in y[7]. */
+   tracker polarisation in y[7], or false, with nothing changed, when the step needs the BMT
+   integration (a spin-tracking window): the collision path then does the spin step. */
 template <int MODE>
-PTD void spin_step(const Params &P, int i, const Species &sp, double x1, double ys0, double ys1, double ys2,
+PTD bool spin_step(const Params &P, int i, const Species &sp, double x1, double ys0, double ys1, double ys2,
                             double polo, double x2, double *y, bool haveB1, double B10, double B11, double B12,
                             bool haveB2, double B20, double B21, double B22, Ctr &c) {
     const State &S = P.st;
@@ -1479,6 +1610,7 @@ PTD void spin_step(const Params &P, int i, const Species &sp, double x1, double 
         spinProjection = pr;
         done = true;
     } else if ((x2 > x1) && (Babs1 < sp.Bmax || Babs2 < sp.Bmax)) {
+        if (sp.n_spintimes > 0 && (spin_window(sp, x1) || spin_window(sp, x2))) return false;
         if (spin[0] != 0 || spin[1] != 0 || spin[2] != 0) {
             double pr = (spin[0] * B1[0] + spin[1] * B1[1] + spin[2] * B1[2]) / Babs1 / __builtin_sqrt(spin[0] * spin[0] + spin[1] * spin[1] + spin[2] * spin[2]);
             if (pr * pr >= 1) { spin[0] = 0; spin[1] = 0; }
@@ -1517,6 +1649,7 @@ PTD void spin_step(const Params &P, int i, const Species &sp, double x1, double 
     S.ctr[i] = ctr;
     for (int k = 0; k < 5; ++k) S.spin[k * n + i] = spin[k];
     (void)spin_in;
+    return true;
 }
 
 /*
@@ -1611,7 +1744,8 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
             bool trunc = (L.tau > 0 && xnew[6] > L.tau) || (ts + dt > tmax);
             /* detached: the chord starts at the tracker's point, not at xc */
             need_dense = trunc || split || absorb || (L.flags & FL_DETACHED) || !clear_of_walls(G, xc, xnew) ||
-                         (P.st.snap_n > 0 && snapshot_in(P.st, ts, ts + dt) >= 0);
+                         (P.st.snap_n > 0 && snapshot_in(P.st, ts, ts + dt) >= 0) ||
+                         (sp.n_spintimes > 0 && (spin_window(sp, ts) || spin_window(sp, ts + dt)));
         }
         if (need_dense) {
 #pragma unroll
@@ -1759,8 +1893,21 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
         record_snapshot(P.st, i, dref, told, x, nullptr, L.stop, P.st.Nhit[i], L.Nstep, P.st.Nspinflip[i], L.Hmax,
                         P.st.noflip[i]);
     const bool y_is_xc = at_end && !altered && (L.flags & FL_BVALID);
-    spin_step<MODE>(P, i, sp, told, xc[0], xc[1], xc[2], polo, x, y, b1v, B1c[0], B1c[1], B1c[2],
-                     y_is_xc, L.Bc[0], L.Bc[1], L.Bc[2], c);
+    if (!spin_step<MODE>(P, i, sp, told, xc[0], xc[1], xc[2], polo, x, y, b1v, B1c[0], B1c[1], B1c[2],
+                         y_is_xc, L.Bc[0], L.Bc[1], L.Bc[2], c)) {
+        /* a spin-tracking window: the collision path finishes the step from its spin step
+           (a chord piece of zero length at the step's end) */
+        double *pd = P.st.pend;
+        pd[i] = x;
+        pd[10 * n + i] = x;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { pd[(1 + k) * n + i] = y[k]; pd[(11 + k) * n + i] = y[k]; }
+        L.x = x;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) L.y[k] = y[k];
+        pending = true;
+        return;
+    }
     if (P.st.trk_interval > 0)
         record_track(P.st, i, xc[7], x, y, solid_ID(G, current_solid(G, L.solids, L.nsolid)));
     if (L.stop == ID_UNKNOWN && y[6] >= L.tau) L.stop = ID_DECAYED;
@@ -1831,8 +1978,10 @@ PTD void collision_path(const Params &P, int i, Ctr &c) {
     x2 = pd[(size_t)10 * n + i];
     for (int k = 0; k < 9; ++k) { y1[k] = pd[(size_t)(1 + k) * n + i]; y2[k] = pd[(size_t)(11 + k) * n + i]; }
 
-    bool reset = check_hit<MODE>(P, T, x1, y1, x2, y2, st);
-    if (reset) {
+    /* a zero-length piece: the step path handed over only the spin step */
+    const bool spin_only = !(x2 > x1);
+    bool reset = spin_only ? (flags & FL_RESET) != 0 : check_hit<MODE>(P, T, x1, y1, x2, y2, st);
+    if (reset && !spin_only) {
         x = x2;
         for (int k = 0; k < 9; ++k) y[k] = y2[k];
     }
@@ -1857,7 +2006,7 @@ PTD void collision_path(const Params &P, int i, Ctr &c) {
         for (int k = 0; k < 9; ++k) y1[k] = y2[k];
     }
     flags = reset ? (flags | FL_RESET) : (flags & ~FL_RESET);
-    if (S.snap_n > 0)
+    if (S.snap_n > 0 && !spin_only)
         record_snapshot(S, i, st, st.t_old, x, spin, T.stop, T.Nhit, T.Nstep, T.Nspinflip, T.Hmax, T.noflip);
     double spinProjection = integrate_spin<MODE>(P, T, spin, st, x, y);
     bool spinHasFlipped = st.xo[7] != y[7];

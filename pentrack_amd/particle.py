@@ -25,12 +25,19 @@ class SpeciesConfig:
     lmax: float = 9e99
     flipspin: int = 0
     Bmax: float = 0.0
+    spintimes: tuple = ()     # BMT spin-tracking windows t0 t1 [t2 t3 ...] (at most four)
 
     def native(self):
         q, m, mu, gamma = SPECIES_CONSTANTS[self.kind]
         s = _native.Species()
         s.kind = self.kind
         s.q, s.m, s.mu, s.gamma = q, m, mu, gamma
+        st = tuple(float(v) for v in self.spintimes)
+        if len(st) % 2 or len(st) > 8:
+            raise ValueError("spintimes: pairs of times, at most four windows")
+        s.n_spintimes = len(st)
+        for k, v in enumerate(st):
+            s.spintimes[k] = v
         s.tau, s.tmax, s.lmax = self.tau, self.tmax, self.lmax
         s.flipspin = self.flipspin
         s.Bmax = self.Bmax

@@ -126,12 +126,8 @@ def test_reference_config_file_parses():
     assert C.global_settings(cfg).simcount > 0
     mats = C.load_materials(cfg)
     assert len(mats) > 10 and all(m.name for m in mats)
-    # its PARTICLES section asks for BMT spin tracking between 500 and 700 s, which this
-    # engine does not do: the species settings are refused, not silently changed
-    with pytest.raises(C.ConfigError, match="spintimes"):
-        C.species_config(cfg, "neutron")
-    cfg["PARTICLES"]["spintimes"] = ""
-    assert C.species_config(cfg, "neutron").tau > 800
+    nc = C.species_config(cfg, "neutron")
+    assert nc.tau > 800 and nc.spintimes == (500.0, 700.0)            # BMT between 500 and 700 s
 
 
 def test_volume_source_distributions():
@@ -235,14 +231,18 @@ def test_source_modes_and_parameters(tmp_path):
     assert np.array_equal(s.bbox[0], [0, 0, 0]) and np.array_equal(s.bbox[1], [1, 2, 3])
 
 
-def test_bmt_spin_tracking_is_rejected():
-    """Spin-tracking windows would switch the reference to BMT integration; the engine
-    refuses them instead of silently transporting the spin along B."""
-    cfg = C.TConfig(text=CONFIG + "\n[neutron]\n")
-    cfg["neutron"]["spintimes"] = " 500 700"
-    with pytest.raises(C.ConfigError, match="spintimes"):
+def test_spin_tracking_settings():
+    cfg = C.TConfig(text=CONFIG)
+    cfg["neutron"]["spintimes"] = " 500 700 800 900"
+    assert C.species_config(cfg, "neutron").spintimes == (500.0, 700.0, 800.0, 900.0)
+    cfg["neutron"]["spintimes"] = " 500"
+    with pytest.raises(C.ConfigError, match="pairs"):
         C.species_config(cfg, "neutron")
-    cfg["neutron"]["spintimes"] = " "
+    cfg["neutron"]["spintimes"] = " 0 1"
+    cfg["neutron"]["interpolatefields"] = " 1"
+    with pytest.raises(C.ConfigError, match="interpolatefields"):
+        C.species_config(cfg, "neutron")
+    cfg["neutron"]["interpolatefields"] = " 0"
     cfg["neutron"]["spinlog"] = " 1"
     with pytest.raises(C.ConfigError, match="spinlog"):
         C.species_config(cfg, "neutron")

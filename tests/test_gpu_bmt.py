@@ -1,0 +1,41 @@
+"""GPU: BMT spin tracking (IntegrateSpin inside spin-tracking windows) against the
+oracle, bit for bit, and on the Larmor precession."""
+import numpy as np
+import pytest
+
+from pentrack_amd import workloads
+from pentrack_amd.particle import SpeciesConfig
+from pentrack_amd.workloads import Workload
+from tests.test_gpu_parity import FIELDS, run_gpu
+from tests.test_oracle_bmt import GAMMA_N, larmor_workload
+
+pytestmark = pytest.mark.gpu
+
+
+def test_larmor_precession_gpu(gpu, oracle_lib):
+    w = larmor_workload(1e-6, (0.0, 1.0), 0.05, n=64)
+    print("case: BMT Larmor n=64", flush=True)
+    g, _ = run_gpu(w, hit_cap=0)
+    o, _ = oracle_lib.track(w, "fp64")
+    for f in FIELDS:
+        assert np.array_equal(g[f], o[f]), f
+    wl = -GAMMA_N * 1e-6
+    assert np.allclose(g["spin"][:, 0], np.cos(wl * 0.05), atol=1e-8)
+
+
+def test_bmt_with_walls_and_map(gpu, oracle_lib):
+    """Cylinder with the coil map and wall hits; spin tracking in two windows and Bmax
+    below part of the map: the step and collision paths both hand spin steps over."""
+    # |B| <= 20 uT keeps the Larmor period above 1.7 ms: a few hundred spin steps per window
+    base = workloads.cylinder(n=300, seed=9, tmax=0.3, n_grid=24, Bmax=2e-5)
+    sp = [SpeciesConfig(kind=0, tau=0.0, tmax=9e99, lmax=9e99, Bmax=1e-5, spintimes=(0.0, 0.1, 0.2, 0.25))]
+    spin0 = base.spin0.copy()
+    spin0[:, 0] = 1.0
+    w = Workload("bmt", base.geometry, base.fields, sp, base.tmax, base.species_idx, base.t0, base.y0, spin0,
+                 base.pid, base.seed)
+    print("case: BMT cylinder n=400", flush=True)
+    g, gh = run_gpu(w, hit_cap=8)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=8, threads=8)
+    for f in FIELDS:
+        assert np.array_equal(g[f], o[f]), f
+    assert g["Nhit"].sum() > 100 and (g["spin"][:, 3] > 0).mean() > 0.5      # spin integrated in the windows
