@@ -175,6 +175,7 @@ typedef struct {
     double Bmax;       /* particleconf "Bmax" */
     int n_spintimes;   /* particleconf "spintimes": BMT spin tracking in [spintimes[2k], spintimes[2k+1]) */
     double spintimes[8];
+    int interpolatefields; /* particleconf "interpolatefields": precession axis from splines over each step */
 } pt_species;
 
 typedef struct {

@@ -173,6 +173,7 @@ typedef struct {
     double Hmax;
     ld noflipprob;
     ptref_hit *hits; int hit_cap; int n_hits;
+    const struct spin_splines_s *spl;   /* BMT with interpolatefields: this step's axis splines */
 } track_t;
 
 static double rnd_uniform(track_t *p, double a, double b) {
@@ -1775,13 +1776,65 @@ static void spin_precession_axis(const track_t *tp, const double B[3], const dou
 /* TParticle::SpinDerivs without field interpolation (src/particle.cpp:177-190): the
    precession axis from the trajectory's dense output at t (SpinPrecessionAxis with the
    stepper, src/particle.cpp:141-148) */
-static void spin_derivs(track_t *tp, const dopri_t *traj, const double *y, double *dydx, double t) {
-    double yt[9], dydt[9], B[3], dB[3][3], E[3] = {0, 0, 0}, V = 0, O[3];
+static void spin_axis_at(track_t *tp, const dopri_t *traj, double t, double O[3]) {
+    double yt[9], dydt[9], B[3], dB[3][3], E[3] = {0, 0, 0}, V = 0;
     dopri_calc_state(tp, traj, t, yt);
     field_B(tp, yt[0], yt[1], yt[2], t, B, dB);
     field_E(tp, yt[0], yt[1], yt[2], t, &V, E);
     eom(&tp->sp, yt, B, dB, E, dydt);
     spin_precession_axis(tp, B, E, dydt, O);
+}
+
+/* interpolatefields: ALGLIB spline1dbuildcubic (parabolic ends) of each axis component over
+   11 points of the step (src/tracking.cpp:404-420) and spline1dcalc (interpolation.cpp:
+   22980-23010): the piece found by bisection, c0 + s (c1 + s (c2 + s c3)) */
+#define SPIN_INT_POINTS 10
+typedef struct spin_splines_s { double ts[SPIN_INT_POINTS + 1]; double c[3][4 * SPIN_INT_POINTS]; } spin_splines;
+
+static void spin_splines_build(track_t *tp, const dopri_t *traj, double x1, double x2, spin_splines *sp) {
+    const int n = SPIN_INT_POINTS + 1;
+    double om[3][SPIN_INT_POINTS + 1], d[SPIN_INT_POINTS + 1], w[4 * (SPIN_INT_POINTS + 1)];
+    for (int i = 0; i < n; i++) {
+        double t = x1 + i * (x2 - x1) / SPIN_INT_POINTS;
+        double O[3];
+        sp->ts[i] = t;
+        spin_axis_at(tp, traj, t, O);
+        om[0][i] = O[0]; om[1][i] = O[1]; om[2][i] = O[2];
+    }
+    for (int k = 0; k < 3; ++k) {
+        const double *x = sp->ts, *y = om[k];
+        alg_griddiff(x, y, n, d, w);
+        for (int i = 0; i <= n - 2; i++) {
+            double delta = x[i + 1] - x[i], delta2 = delta * delta, delta3 = delta * delta2;
+            sp->c[k][4 * i + 0] = y[i];
+            sp->c[k][4 * i + 1] = d[i];
+            sp->c[k][4 * i + 2] = (3 * (y[i + 1] - y[i]) - 2 * d[i] * delta - d[i + 1] * delta) / delta2;
+            sp->c[k][4 * i + 3] = (2 * (y[i] - y[i + 1]) + d[i] * delta + d[i + 1] * delta) / delta3;
+        }
+    }
+}
+
+static double spin_spline_calc(const spin_splines *sp, int k, double x) {
+    int l = 0, r = SPIN_INT_POINTS;
+    while (l != r - 1) {
+        int m = (l + r) / 2;
+        if (sp->ts[m] >= x) r = m;
+        else l = m;
+    }
+    x = x - sp->ts[l];
+    const double *c = sp->c[k] + 4 * l;
+    return c[0] + x * (c[1] + x * (c[2] + x * c[3]));
+}
+
+static void spin_derivs(track_t *tp, const dopri_t *traj, const double *y, double *dydx, double t) {
+    double O[3];
+    if (tp->spl) {
+        O[0] = spin_spline_calc(tp->spl, 0, t);
+        O[1] = spin_spline_calc(tp->spl, 1, t);
+        O[2] = spin_spline_calc(tp->spl, 2, t);
+    } else {
+        spin_axis_at(tp, traj, t, O);
+    }
     dydx[0] = O[1] * y[2] - O[2] * y[1];
     dydx[1] = O[2] * y[0] - O[0] * y[2];
     dydx[2] = O[0] * y[1] - O[1] * y[0];
@@ -1873,7 +1926,7 @@ static void sdopri_calc_state(const sdopri_t *s, double t, double *x) {
 /* the spin integration of IntegrateSpin (src/tracking.cpp:424-450): the spin stepper from x1
    with a first step of half a Larmor period, steps until it passes x2, the state at x2 from
    its dense output; returns -1 if a step fails 500 times */
-static int bmt_integrate(track_t *tp, double *spin, const dopri_t *traj, double x1, double x2, double Babs1) {
+static int bmt_steps(track_t *tp, double *spin, const dopri_t *traj, double x1, double x2, double Babs1) {
     sdopri_t s;
     memcpy(s.xc, spin, sizeof(s.xc));
     s.t = x1;
@@ -1902,6 +1955,18 @@ static int bmt_integrate(track_t *tp, double *spin, const dopri_t *traj, double 
         }
         if (t >= x2) return 0;
     }
+}
+
+static int bmt_integrate(track_t *tp, double *spin, const dopri_t *traj, double x1, double x2, double Babs1) {
+    spin_splines spl;
+    tp->spl = 0;
+    if (tp->cfg->interpolatefields) {
+        spin_splines_build(tp, traj, x1, x2, &spl);
+        tp->spl = &spl;
+    }
+    int r = bmt_steps(tp, spin, traj, x1, x2, Babs1);
+    tp->spl = 0;
+    return r;
 }
 
 static double integrate_spin(track_t *tp, double *spin, const dopri_t *st, double x2, const double *y2, double Bmax) {

@@ -76,6 +76,7 @@ typedef struct {
     double Bmax;
     int n_spintimes;          /* spin-tracking windows [spintimes[2k], spintimes[2k+1]) (BMT) */
     double spintimes[8];
+    int interpolatefields;    /* precession axis from cubic splines over 11 points of each step */
 } ptref_species;
 
 typedef struct {

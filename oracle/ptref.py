@@ -53,7 +53,8 @@ class Field(C.Structure):
 
 class Species(C.Structure):
     _fields_ = [("kind", C.c_int), ("tau", C.c_double), ("tmax", C.c_double), ("lmax", C.c_double),
-                ("flipspin", C.c_int), ("Bmax", C.c_double), ("n_spintimes", C.c_int), ("spintimes", C.c_double * 8)]
+                ("flipspin", C.c_int), ("Bmax", C.c_double), ("n_spintimes", C.c_int), ("spintimes", C.c_double * 8),
+                ("interpolatefields", C.c_int)]
 
 
 OUT_DTYPE = np.dtype([
@@ -261,6 +262,7 @@ def track(workload, mode="fp64", hit_cap=0, threads=1, sel=None, problem=None):
             s.kind, s.tau, s.tmax, s.lmax, s.flipspin, s.Bmax)
         st = list(getattr(s, "spintimes", ()))[:8]
         sp[k].n_spintimes = len(st)
+        sp[k].interpolatefields = int(getattr(s, "interpolatefields", 0))
         for q, v in enumerate(st):
             sp[k].spintimes[q] = v
     si = np.ascontiguousarray(workload.species_idx[idx], dtype=np.int32)

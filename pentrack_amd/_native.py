@@ -64,7 +64,7 @@ class Species(C.Structure):
         ("q", C.c_double), ("m", C.c_double), ("mu", C.c_double), ("gamma", C.c_double),
         ("tau", C.c_double), ("tmax", C.c_double), ("lmax", C.c_double),
         ("flipspin", C.c_int), ("Bmax", C.c_double),
-        ("n_spintimes", C.c_int), ("spintimes", C.c_double * 8),
+        ("n_spintimes", C.c_int), ("spintimes", C.c_double * 8), ("interpolatefields", C.c_int),
     ]
 
 

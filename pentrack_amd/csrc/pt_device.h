@@ -99,6 +99,7 @@ struct Species {
     int flipspin;
     int n_spintimes;          /* BMT spin-tracking windows [spintimes[2k], spintimes[2k+1]) */
     double spintimes[8];
+    int interpolatefields;    /* precession axis from cubic splines over 11 points of each step */
 };
 
 struct Hit {

@@ -1167,6 +1167,7 @@ extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_s
         if (q.n_spintimes < 0 || q.n_spintimes > 8 || (q.n_spintimes & 1))
             return fail(PT_EINVAL, "pt_batch_create: spintimes needs pairs of times, at most four windows");
         sp[s].n_spintimes = q.n_spintimes;
+        sp[s].interpolatefields = q.interpolatefields;
         for (int k = 0; k < 8; ++k) sp[s].spintimes[k] = k < q.n_spintimes ? q.spintimes[k] : 0.0;
     }
     int r;

@@ -1294,7 +1294,7 @@ PTD bool spin_window(const Species &sp, double t) {
 /* TParticle::SpinDerivs without field interpolation (src/particle.cpp:141-190): the
    precession axis (BMT with Thomas precession) from the trajectory's dense output at t */
 template <int MODE>
-PTD void spin_derivs(const Params &P, Track &T, const Dopri &traj, const double *y, double *dydx, double t) {
+PTD void spin_axis(const Params &P, Track &T, const Dopri &traj, double t, double O[3]) {
     double yt[9], d9[9], B[3], dB[3][3], E[3] = {0, 0, 0}, V = 0;
     dopri_calc_state(traj, t, yt);
     field_B<MODE>(P, yt[0], yt[1], yt[2], t, B, dB, true, T.c.field);
@@ -1306,13 +1306,89 @@ PTD void spin_derivs(const Params &P, Track &T, const Dopri &traj, const double 
     const double Bdotv = B[0] * d9[0] + B[1] * d9[1] + B[2] * d9[2];
     double Bp[3];
     for (int j = 0; j < 3; j++) Bp[j] = Bdotv * d9[j] / v2;
-    double O[3];
     O[0] = -gamma / gamma_rel * ((1 - gamma_rel) * Bp[0] + gamma_rel * B[0] - gamma_rel * (d9[1] * E[2] - d9[2] * E[1]) / C_0 / C_0) +
            gamma_rel * gamma_rel / (gamma_rel + 1) * (d9[4] * d9[2] - d9[5] * d9[1]) / C_0 / C_0;
     O[1] = -gamma / gamma_rel * ((1 - gamma_rel) * Bp[1] + gamma_rel * B[1] - gamma_rel * (d9[2] * E[0] - d9[0] * E[2]) / C_0 / C_0) +
            gamma_rel * gamma_rel / (gamma_rel + 1) * (d9[5] * d9[0] - d9[3] * d9[2]) / C_0 / C_0;
     O[2] = -gamma / gamma_rel * ((1 - gamma_rel) * Bp[2] + gamma_rel * B[2] - gamma_rel * (d9[0] * E[1] - d9[1] * E[0]) / C_0 / C_0) +
            gamma_rel * gamma_rel / (gamma_rel + 1) * (d9[3] * d9[1] - d9[4] * d9[0]) / C_0 / C_0;
+}
+
+/* interpolatefields: ALGLIB spline1dbuildcubic (parabolic ends) of each axis component over
+   SPIN_PTS + 1 points of the step (src/tracking.cpp:404-420), spline1dcalc by bisection and
+   the Hermite piece c0 + s (c1 + s (c2 + s c3)) */
+constexpr int SPIN_PTS = 10;
+struct SpinSplines { double ts[SPIN_PTS + 1]; double c[3][4 * SPIN_PTS]; };
+
+template <int MODE>
+PTD void spin_splines_build(const Params &P, Track &T, const Dopri &traj, double x1, double x2, SpinSplines &S) {
+    constexpr int n = SPIN_PTS + 1;
+    double om[3][n];
+    for (int i = 0; i < n; i++) {
+        const double t = x1 + i * (x2 - x1) / SPIN_PTS;
+        double O[3];
+        S.ts[i] = t;
+        spin_axis<MODE>(P, T, traj, t, O);
+        om[0][i] = O[0]; om[1][i] = O[1]; om[2][i] = O[2];
+    }
+    const double *x = S.ts;
+    for (int k = 0; k < 3; ++k) {
+        const double *y = om[k];
+        /* spline1dgriddiffcubic: tridiagonal system with parabolic ends (Thomas algorithm) */
+        double a2[n], r[n], d[n];
+        a2[0] = 1; r[0] = 2 * (y[1] - y[0]) / (x[1] - x[0]);
+        double a3p = 1;
+        for (int i = 1; i <= n - 2; i++) {
+            const double a1 = x[i + 1] - x[i], a2i = 2 * (x[i + 1] - x[i - 1]), a3 = x[i] - x[i - 1];
+            const double b = 3 * (y[i] - y[i - 1]) / (x[i] - x[i - 1]) * (x[i + 1] - x[i]) +
+                             3 * (y[i + 1] - y[i]) / (x[i + 1] - x[i]) * (x[i] - x[i - 1]);
+            const double tt = a1 / a2[i - 1];
+            a2[i] = a2i - tt * a3p;
+            r[i] = b - tt * r[i - 1];
+            a3p = a3;
+        }
+        {
+            const double b = 2 * (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
+            const double tt = 1.0 / a2[n - 2];
+            a2[n - 1] = 1.0 - tt * a3p;
+            r[n - 1] = b - tt * r[n - 2];
+        }
+        d[n - 1] = r[n - 1] / a2[n - 1];
+        for (int i = n - 2; i >= 0; --i) d[i] = (r[i] - (i == 0 ? 1.0 : x[i] - x[i - 1]) * d[i + 1]) / a2[i];
+        for (int i = 0; i <= n - 2; i++) {
+            const double delta = x[i + 1] - x[i], delta2 = delta * delta, delta3 = delta * delta2;
+            S.c[k][4 * i + 0] = y[i];
+            S.c[k][4 * i + 1] = d[i];
+            S.c[k][4 * i + 2] = (3 * (y[i + 1] - y[i]) - 2 * d[i] * delta - d[i + 1] * delta) / delta2;
+            S.c[k][4 * i + 3] = (2 * (y[i] - y[i + 1]) + d[i] * delta + d[i + 1] * delta) / delta3;
+        }
+    }
+}
+
+PTD double spin_spline_calc(const SpinSplines &S, int k, double x) {
+    int l = 0, r = SPIN_PTS;
+    while (l != r - 1) {
+        const int m = (l + r) / 2;
+        if (S.ts[m] >= x) r = m;
+        else l = m;
+    }
+    x = x - S.ts[l];
+    const double *c = S.c[k] + 4 * l;
+    return c[0] + x * (c[1] + x * (c[2] + x * c[3]));
+}
+
+/* TParticle::SpinDerivs (src/particle.cpp:177-190) */
+template <int MODE>
+PTD void spin_derivs(const Params &P, Track &T, const Dopri &traj, const SpinSplines *spl, const double *y,
+                     double *dydx, double t) {
+    double O[3];
+    if (spl) {
+        O[0] = spin_spline_calc(*spl, 0, t);
+        O[1] = spin_spline_calc(*spl, 1, t);
+        O[2] = spin_spline_calc(*spl, 2, t);
+    } else {
+        spin_axis<MODE>(P, T, traj, t, O);
+    }
     dydx[0] = O[1] * y[2] - O[2] * y[1];
     dydx[1] = O[2] * y[0] - O[0] * y[2];
     dydx[2] = O[0] * y[1] - O[1] * y[0];
@@ -1338,26 +1414,32 @@ PTD_NOINLINE bool bmt_integrate(const Params &P, Track &T, const Dopri &traj, do
     for (int i = 0; i < 5; ++i) x[i] = spin[i];
     double t = x1, told = x1;
     double dt = __builtin_fabs(PI / T.sp.gamma / Babs1);
-    spin_derivs<MODE>(P, T, traj, x, k1, t);
+    SpinSplines splines;
+    const SpinSplines *spl = nullptr;
+    if (T.sp.interpolatefields) {
+        spin_splines_build<MODE>(P, T, traj, x1, x2, splines);
+        spl = &splines;
+    }
+    spin_derivs<MODE>(P, T, traj, spl, x, k1, t);
     for (;;) {
         int tries = 0;
         for (;;) {
             double tmp[5], k2[5], xn[5], kn[5];
             for (int i = 0; i < 5; ++i) tmp[i] = 1.0 * x[i] + (dt * b21) * k1[i];
-            spin_derivs<MODE>(P, T, traj, tmp, k2, t + dt * a2);
+            spin_derivs<MODE>(P, T, traj, spl, tmp, k2, t + dt * a2);
             for (int i = 0; i < 5; ++i) tmp[i] = 1.0 * x[i] + (dt * b31) * k1[i] + (dt * b32) * k2[i];
-            spin_derivs<MODE>(P, T, traj, tmp, k3, t + dt * a3);
+            spin_derivs<MODE>(P, T, traj, spl, tmp, k3, t + dt * a3);
             for (int i = 0; i < 5; ++i) tmp[i] = 1.0 * x[i] + (dt * b41) * k1[i] + (dt * b42) * k2[i] + (dt * b43) * k3[i];
-            spin_derivs<MODE>(P, T, traj, tmp, k4, t + dt * a4);
+            spin_derivs<MODE>(P, T, traj, spl, tmp, k4, t + dt * a4);
             for (int i = 0; i < 5; ++i)
                 tmp[i] = 1.0 * x[i] + (dt * b51) * k1[i] + (dt * b52) * k2[i] + (dt * b53) * k3[i] + (dt * b54) * k4[i];
-            spin_derivs<MODE>(P, T, traj, tmp, k5, t + dt * a5);
+            spin_derivs<MODE>(P, T, traj, spl, tmp, k5, t + dt * a5);
             for (int i = 0; i < 5; ++i)
                 tmp[i] = 1.0 * x[i] + (dt * b61) * k1[i] + (dt * b62) * k2[i] + (dt * b63) * k3[i] + (dt * b64) * k4[i] + (dt * b65) * k5[i];
-            spin_derivs<MODE>(P, T, traj, tmp, k6, t + dt);
+            spin_derivs<MODE>(P, T, traj, spl, tmp, k6, t + dt);
             for (int i = 0; i < 5; ++i)
                 xn[i] = 1.0 * x[i] + (dt * c1) * k1[i] + (dt * c3) * k3[i] + (dt * c4) * k4[i] + (dt * c5) * k5[i] + (dt * c6) * k6[i];
-            spin_derivs<MODE>(P, T, traj, xn, kn, t + dt);
+            spin_derivs<MODE>(P, T, traj, spl, xn, kn, t + dt);
             double err = 0;
             for (int i = 0; i < 5; ++i) {
                 double xe = (dt * dc1) * k1[i] + (dt * dc3) * k3[i] + (dt * dc4) * k4[i] + (dt * dc5) * k5[i] + (dt * dc6) * k6[i] + (dt * dc7) * kn[i];

@@ -26,6 +26,7 @@ class SpeciesConfig:
     flipspin: int = 0
     Bmax: float = 0.0
     spintimes: tuple = ()     # BMT spin-tracking windows t0 t1 [t2 t3 ...] (at most four)
+    interpolatefields: int = 0  # BMT precession axis from splines over 11 points of each step
 
     def native(self):
         q, m, mu, gamma = SPECIES_CONSTANTS[self.kind]
@@ -36,6 +37,7 @@ class SpeciesConfig:
         if len(st) % 2 or len(st) > 8:
             raise ValueError("spintimes: pairs of times, at most four windows")
         s.n_spintimes = len(st)
+        s.interpolatefields = int(self.interpolatefields)
         for k, v in enumerate(st):
             s.spintimes[k] = v
         s.tau, s.tmax, s.lmax = self.tau, self.tmax, self.lmax

@@ -240,9 +240,7 @@ def test_spin_tracking_settings():
         C.species_config(cfg, "neutron")
     cfg["neutron"]["spintimes"] = " 0 1"
     cfg["neutron"]["interpolatefields"] = " 1"
-    with pytest.raises(C.ConfigError, match="interpolatefields"):
-        C.species_config(cfg, "neutron")
-    cfg["neutron"]["interpolatefields"] = " 0"
+    assert C.species_config(cfg, "neutron").interpolatefields == 1
     cfg["neutron"]["spinlog"] = " 1"
     with pytest.raises(C.ConfigError, match="spinlog"):
         C.species_config(cfg, "neutron")

@@ -7,7 +7,7 @@ from pentrack_amd import workloads
 from pentrack_amd.particle import SpeciesConfig
 from pentrack_amd.workloads import Workload
 from tests.test_gpu_parity import FIELDS, run_gpu
-from tests.test_oracle_bmt import GAMMA_N, larmor_workload
+from tests.test_oracle_bmt import GAMMA_N, falling_workload, larmor_workload
 
 pytestmark = pytest.mark.gpu
 
@@ -39,3 +39,15 @@ def test_bmt_with_walls_and_map(gpu, oracle_lib):
     for f in FIELDS:
         assert np.array_equal(g[f], o[f]), f
     assert g["Nhit"].sum() > 100 and (g["spin"][:, 3] > 0).mean() > 0.5      # spin integrated in the windows
+
+
+@pytest.mark.parametrize("name,params", [("B0GradZ", (0.0, 1e-5, 1e-6)), ("ExponentialFieldX", (2e-6, 3.0, 0.0, 1e-6, 0.0))])
+def test_interpolated_axis_gpu(gpu, oracle_lib, name, params):
+    """interpolatefields 1: the precession axis from splines over 11 points of each step."""
+    w = falling_workload(1, name, params, n=64)
+    print(f"case: BMT interpolatefields {name} n=64", flush=True)
+    g, _ = run_gpu(w, hit_cap=0)
+    o, _ = oracle_lib.track(w, "fp64")
+    for f in FIELDS:
+        assert np.array_equal(g[f], o[f]), f
+    assert (g["spin"][:, 4] > 1).all()
