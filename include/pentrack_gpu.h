@@ -265,6 +265,24 @@ typedef struct {
 int pt_batch_set_track(pt_batch *b, double interval, int cap);
 /* out [n][cap] records, count [n] records taken (may exceed cap; synchronises the stream) */
 int pt_batch_track(pt_batch *b, void *stream, pt_track_point *out, int32_t *count);
+
+/* a spin-log row: time, position, spin and the BMT precession axis W there */
+typedef struct {
+    double t, pos[3], spin[3], W[3];
+} pt_spin_point;
+/*
+ * TLogger::PrintSpin (src/logger.cpp:237-284) as IntegrateSpin calls it (src/tracking.cpp:431,
+ * :448): inside the BMT spin-tracking windows a row is recorded at the start of each spin
+ * integration and after each spin step whose end x satisfies int(x1 / interval) !=
+ * int(x / interval) for the spin stepper's previous time x1 (the end is clamped to the
+ * trajectory step's end, where the row holds the dense-output spin).  At the first call the
+ * reference reads the previous time of a stepper that has made no step (an uninitialised
+ * value); that row is always written here.  cap records are kept per particle (more are
+ * counted); interval = 0 switches it off.
+ */
+int pt_batch_set_spinlog(pt_batch *b, double interval, int cap);
+/* out [n][cap] records, count [n] records taken (may exceed cap; synchronises the stream) */
+int pt_batch_spinlog(pt_batch *b, void *stream, pt_spin_point *out, int32_t *count);
 /* Sums over the batch (synchronises the stream): accepted / rejected dopri5 steps and
    field-map cell evaluations of the step kernel (each gathers one coefficient block per
    component); any pointer may be NULL. */

@@ -174,6 +174,7 @@ typedef struct {
     ld noflipprob;
     ptref_hit *hits; int hit_cap; int n_hits;
     const struct spin_splines_s *spl;   /* BMT with interpolatefields: this step's axis splines */
+    double *spin_pos;                   /* spin_axis_at also stores the position here when set */
 } track_t;
 
 static double rnd_uniform(track_t *p, double a, double b) {
@@ -1779,6 +1780,7 @@ static void spin_precession_axis(const track_t *tp, const double B[3], const dou
 static void spin_axis_at(track_t *tp, const dopri_t *traj, double t, double O[3]) {
     double yt[9], dydt[9], B[3], dB[3][3], E[3] = {0, 0, 0}, V = 0;
     dopri_calc_state(tp, traj, t, yt);
+    if (tp->spin_pos) { tp->spin_pos[0] = yt[0]; tp->spin_pos[1] = yt[1]; tp->spin_pos[2] = yt[2]; }
     field_B(tp, yt[0], yt[1], yt[2], t, B, dB);
     field_E(tp, yt[0], yt[1], yt[2], t, &V, E);
     eom(&tp->sp, yt, B, dB, E, dydt);
@@ -1926,11 +1928,40 @@ static void sdopri_calc_state(const sdopri_t *s, double t, double *x) {
 /* the spin integration of IntegrateSpin (src/tracking.cpp:424-450): the spin stepper from x1
    with a first step of half a Larmor period, steps until it passes x2, the state at x2 from
    its dense output; returns -1 if a step fails 500 times */
+/* spin log (ptref_set_spinlog): interval, records [n][cap], counts [n] */
+static double g_slog_interval;
+static int g_slog_cap;
+static ptref_spinpt *g_slog;
+static int32_t *g_nslog;
+
+void ptref_set_spinlog(double interval, int cap, ptref_spinpt *out, int32_t *count) {
+    g_slog_interval = interval; g_slog_cap = cap; g_slog = out; g_nslog = count;
+}
+
+/* TLogger::PrintSpin (src/logger.cpp:237-284) as IntegrateSpin calls it (src/tracking.cpp:431,
+   :448): at x unless x lies after the spin stepper's previous time x1 in the same multiple
+   of spinloginterval.  At the call after initialize the stepper's previous time is
+   uninitialised in the reference; that row (x1 < 0 here) is always written.  The spin at x
+   is the integrator's (current state, or the dense output at the clamped end). */
+static void print_spin(track_t *tp, const dopri_t *traj, double x1, double x, const double *spin) {
+    if (x1 >= 0 && x > x1 && (int)(x1 / g_slog_interval) == (int)(x / g_slog_interval)) return;
+    int k = g_nslog[g_particle]++;
+    if (k >= g_slog_cap) return;
+    ptref_spinpt *r = &g_slog[(size_t)g_particle * g_slog_cap + k];
+    double O[3], pos[3];
+    tp->spin_pos = pos;
+    spin_axis_at(tp, traj, x, O);     /* SpinPrecessionAxis, never the interpolated axis */
+    tp->spin_pos = 0;
+    r->t = x;
+    for (int q = 0; q < 3; ++q) { r->pos[q] = pos[q]; r->spin[q] = spin[q]; r->W[q] = O[q]; }
+}
+
 static int bmt_steps(track_t *tp, double *spin, const dopri_t *traj, double x1, double x2, double Babs1) {
     sdopri_t s;
     memcpy(s.xc, spin, sizeof(s.xc));
     s.t = x1;
     s.dt = fabs(PI / (double)tp->sp.gamma / Babs1);    /* GetGyromagneticRatio() is a double */
+    if (g_slog_interval > 0) print_spin(tp, traj, -1.0, x1, s.xc);
     spin_derivs(tp, traj, s.xc, s.dxc, s.t);
     for (;;) {
         int tries = 0, ok;
@@ -1953,6 +1984,7 @@ static int bmt_steps(track_t *tp, double *spin, const dopri_t *traj, double x1, 
         } else {
             memcpy(spin, s.xc, 5 * sizeof(double));
         }
+        if (g_slog_interval > 0) print_spin(tp, traj, s.t_old, t, spin);
         if (t >= x2) return 0;
     }
 }

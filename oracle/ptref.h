@@ -103,6 +103,11 @@ typedef struct {
     int32_t solid, pad;
 } ptref_trackpt;
 
+/* a spin-log row (TLogger::PrintSpin): time, position, spin, BMT precession axis */
+typedef struct {
+    double t, pos[3], spin[3], W[3];
+} ptref_spinpt;
+
 /* a particle's state at a snapshot time (TLogger::PrintSnapshot) and its counters then */
 typedef struct {
     double t, y[9], spin[5], Hmax, noflipprob;

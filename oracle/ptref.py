@@ -107,6 +107,8 @@ def load(mode="fp64"):
     L.ptref_set_snapshots.restype = None
     L.ptref_set_track.argtypes = [C.c_double, C.c_int, C.c_void_p, C.POINTER(C.c_int32)]
     L.ptref_set_track.restype = None
+    L.ptref_set_spinlog.argtypes = [C.c_double, C.c_int, C.c_void_p, C.POINTER(C.c_int32)]
+    L.ptref_set_spinlog.restype = None
     L.ptref_bicubic_tables.argtypes = [C.c_int, C.c_int, dp, dp, C.c_int, dp, dp]
     L.ptref_bicubic_tables.restype = None
     L.ptref_spline2d.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp, dp]
@@ -247,6 +249,22 @@ def track_tracklog(workload, interval, cap, mode="fp64", threads=1):
         out, _ = track(workload, mode, threads=threads)
     finally:
         L.ptref_set_track(0.0, 0, None, None)
+    return out, rec.reshape(workload.n, cap), counts
+
+
+SPIN_DTYPE = np.dtype([("t", "<f8"), ("pos", "<f8", (3,)), ("spin", "<f8", (3,)), ("W", "<f8", (3,))])
+
+
+def track_spinlog(workload, interval, cap, mode="fp64", threads=1):
+    """The oracle run with the spin log: (out, records [n][cap], counts [n])."""
+    L = load(mode)
+    rec = np.zeros(workload.n * cap, dtype=SPIN_DTYPE)
+    counts = np.zeros(workload.n, dtype=np.int32)
+    L.ptref_set_spinlog(float(interval), cap, rec.ctypes.data_as(C.c_void_p), counts.ctypes.data_as(C.POINTER(C.c_int32)))
+    try:
+        out, _ = track(workload, mode, threads=threads)
+    finally:
+        L.ptref_set_spinlog(0.0, 0, None, None)
     return out, rec.reshape(workload.n, cap), counts
 
 

@@ -23,8 +23,8 @@ def main(argv=None):
     outpath = os.path.abspath(argv[2] if len(argv) > 2 else "out")
     seed = int(argv[3]) if len(argv) > 3 else 0
     from . import config as cfgmod
-    from .logger import (DEFAULT_ENDLOGVARS, DEFAULT_HITLOGVARS, DEFAULT_TRACKLOGVARS, end_values, hit_values,
-                         snapshot_values, track_values, write_log)
+    from .logger import (DEFAULT_ENDLOGVARS, DEFAULT_HITLOGVARS, DEFAULT_SPINLOGVARS, DEFAULT_TRACKLOGVARS, end_values,
+                         hit_values, snapshot_values, spin_values, track_values, write_log)
     from .source import create_source
     from .tracker import LaunchConfig, TTracker, secondary_states
 
@@ -50,7 +50,7 @@ def main(argv=None):
     settings = {k: cfgmod.particle_settings(cfg, v) for k, v in names.items()}
     on = lambda kind, log: settings[kind].get(log, "").split()[:1] == ["1"]
     for kind in names:
-        for log in ("end", "hit", "snapshot", "track"):
+        for log in ("end", "hit", "snapshot", "track", "spin"):
             if on(kind, log + "log") and settings[kind].get(log + "logfilter", "").strip():
                 raise SystemExit(f"{log}logfilter is not supported by this engine")
     hit_cap = HIT_CAP if any(on(k, "hitlog") for k in names) else 0
@@ -63,18 +63,35 @@ def main(argv=None):
         interval = cfgmod._floats(settings[source.kind].get("trackloginterval", ""))
         if interval and interval[0] > 0:
             trk = (interval[0], int(min(4096, max(16, 4e9 // (88 * max(n, 1))))))
-    for log in ("snapshot", "track"):
+    slog = None
+    if on(source.kind, "spinlog"):
+        interval = cfgmod._floats(settings[source.kind].get("spinloginterval", ""))
+        if interval and interval[0] > 0:
+            slog = (interval[0], int(min(4096, max(16, 4e9 // (80 * max(n, 1))))))
+    for log in ("snapshot", "track", "spin"):
         if any(on(k, log + "log") for k in names if k != source.kind) and g.secondaries == 1:
             raise SystemExit(f"{log} logs of decay products are not supported by this engine")
     tr = TTracker(geometry, fields, LaunchConfig())
-    if g.secondaries == 1 and snaps is None and trk is None:
+    if g.secondaries == 1 and snaps is None and trk is None and slog is None:
         (out, hits), (sout, shits), rec = tr.IntegrateWithSecondaries(species, g.simtime, kind_idx, t0, y0, spin0, seed,
                                                                       pid, hit_cap=hit_cap)
     else:
         if g.secondaries == 1:
-            raise SystemExit("snapshot or track logs together with secondaries are not supported by this engine")
+            raise SystemExit("snapshot, track or spin logs together with secondaries are not supported by this engine")
         res = tr.IntegrateBatch(species, g.simtime, kind_idx, t0, y0, spin0, seed, pid, hit_cap=hit_cap, snapshots=snaps,
-                                track=trk)
+                                track=trk, spinlog=slog)
+        if slog is not None:
+            srec, scnt = res[-2], res[-1]
+            res = res[:-2]
+            if (scnt > slog[1]).any():
+                print(f"warning: {int((scnt > slog[1]).sum())} particles logged more than {slog[1]} spin points; "
+                      "their spin log keeps the first ones")
+            vals = spin_values(srec, scnt, pid.astype(float), fields, jobnumber)
+            titles = settings[source.kind].get("spinlogvars", "").split() or DEFAULT_SPINLOGVARS
+            missing = [t for t in titles if t not in vals]
+            if missing:
+                raise SystemExit(f"spinlogvars not supported: {missing}")
+            print("wrote", write_log(outpath, jobnumber, names[source.kind], "spin", titles, vals))
         if trk is not None:
             trec, tcnt = res[-2], res[-1]
             if (tcnt > trk[1]).any():

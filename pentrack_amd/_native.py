@@ -125,6 +125,8 @@ SNAP_DTYPE = np.dtype([
 assert SNAP_DTYPE.itemsize == 152
 TRACK_DTYPE = np.dtype([("t", "<f8"), ("y", "<f8", (9,)), ("solid", "<i4"), ("pad", "<i4")], align=True)
 assert TRACK_DTYPE.itemsize == 88
+SPIN_DTYPE = np.dtype([("t", "<f8"), ("pos", "<f8", (3,)), ("spin", "<f8", (3,)), ("W", "<f8", (3,))])
+assert SPIN_DTYPE.itemsize == 80
 assert OUT_DTYPE.itemsize == C.sizeof(ParticleOut), (OUT_DTYPE.itemsize, C.sizeof(ParticleOut))
 assert SEC_DTYPE.itemsize == C.sizeof(SecondaryRec), (SEC_DTYPE.itemsize, C.sizeof(SecondaryRec))
 assert HIT_DTYPE.itemsize == C.sizeof(HitRec), (HIT_DTYPE.itemsize, C.sizeof(HitRec))
@@ -141,7 +143,7 @@ EXPORTS = [
     "pt_batch_secondaries", "pt_proton_beta_table", "pt_neutron_decay_host", "pt_beta_sample",
     "pt_distribution_table", "pt_fields_eval", "pt_geometry_solid_at", "pt_bicubic_precompute",
     "pt_tricubic_precompute_gpu", "pt_batch_set_snapshots", "pt_batch_snapshots",
-    "pt_batch_set_track", "pt_batch_track",
+    "pt_batch_set_track", "pt_batch_track", "pt_batch_set_spinlog", "pt_batch_spinlog",
 ]
 
 _lib = None
@@ -192,6 +194,8 @@ def lib():
     L.pt_batch_snapshots.argtypes = [vp, vp, vp, C.POINTER(C.c_int32)]
     L.pt_batch_set_track.argtypes = [vp, C.c_double, C.c_int]
     L.pt_batch_track.argtypes = [vp, vp, vp, C.POINTER(C.c_int32)]
+    L.pt_batch_set_spinlog.argtypes = [vp, C.c_double, C.c_int]
+    L.pt_batch_spinlog.argtypes = [vp, vp, vp, C.POINTER(C.c_int32)]
     L.pt_batch_kernel_times.argtypes = [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.pt_track_batch.argtypes = [vp, vp, vp, C.c_int, C.POINTER(Species), C.c_double, C.c_int, ip,
                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),

@@ -224,14 +224,11 @@ def particle_settings(cfg, name):
 def species_config(cfg, name):
     """The species' tracking settings, with the BMT spin-tracking windows `spintimes`
     (read as the reference reads them, pairs of times, src/tracking.cpp:51-60) and the
-    interpolation of the precession axis.  The spin log is not implemented by this engine
-    and is rejected."""
+    interpolation of the precession axis; the spin log is read by the command line."""
     s = particle_settings(cfg, name)
     st = _floats(s.get("spintimes", ""))
     if len(st) % 2 or len(st) > 8:
         raise ConfigError(f"{name}: spintimes needs pairs of times (at most four windows)")
-    if s.get("spinlog", "").split()[:1] == ["1"]:
-        raise ConfigError(f"{name}: spinlog is not supported by this engine")
     return SpeciesConfig(kind=SPECIES_BY_NAME[name], tau=_num(s, "tau", 0.0), tmax=_num(s, "tmax", 9e99),
                          lmax=_num(s, "lmax", 9e99), flipspin=_num(s, "flipspin", 0, lambda x: int(float(x))),
                          Bmax=_num(s, "Bmax", 0.0), spintimes=tuple(st),

@@ -115,6 +115,11 @@ struct TrackRec {
     int solid, pad;
 };
 
+/* a spin-log row (TLogger::PrintSpin): time, position, spin and BMT precession axis */
+struct SpinRec {
+    double t, pos[3], spin[3], W[3];
+};
+
 /* a particle's state at a snapshot time (TLogger::PrintSnapshot) with its counters then */
 struct Snap {
     double t, y[9], spin[5], Hmax, noflip;
@@ -140,6 +145,11 @@ struct State {
     int trk_cap;
     TrackRec *trk;
     int *ntrk;
+    /* spin log: time interval (0: off), records [n][slog_cap], records per particle */
+    double slog_interval;
+    int slog_cap;
+    SpinRec *slog;
+    int *nslog;
     double *Bc, *Vc;          /* field at the stepper's current state, [3][n] and [n] (valid if FL_BVALID) */
     double *pend;             /* pending wall-collision piece: x1, y1[9], x2, y2[9] -> [20][n] */
     double *dense;            /* dense-output state of the last step: [DENSE][n] */

@@ -1536,6 +1536,37 @@ extern "C" int pt_batch_track(pt_batch *b, void *stream, pt_track_point *out, in
     return PT_OK;
 }
 
+/* TLogger::PrintSpin: a row after each BMT spin step that crossed a multiple of interval and
+   at the start of every spin integration; interval = 0 switches the spin log off */
+extern "C" int pt_batch_set_spinlog(pt_batch *b, double interval, int cap) {
+    if (!b || interval < 0 || (interval > 0 && cap <= 0)) return fail(PT_EINVAL, "pt_batch_set_spinlog: bad arguments");
+    ptd::State &S = b->P.st;
+    S.slog_interval = 0;
+    if (interval > 0) {
+        ptd::SpinRec *d;
+        int *dn;
+        int r;
+        std::vector<int> zero((size_t)std::max(b->n, 1), 0);
+        if ((r = b->mem.alloc(&d, (size_t)b->n * cap)) || (r = b->mem.upload(&dn, zero.data(), zero.size()))) return r;
+        S.slog = d; S.nslog = dn; S.slog_cap = cap; S.slog_interval = interval;
+    }
+    for (auto &g : b->graphs) { (void)hipGraphExecDestroy(g.exec); (void)hipGraphDestroy(g.graph); }
+    b->graphs.clear();
+    return PT_OK;
+}
+
+extern "C" int pt_batch_spinlog(pt_batch *b, void *stream, pt_spin_point *out, int32_t *count) {
+    if (!b || !out || !count) return fail(PT_EINVAL, "pt_batch_spinlog: bad arguments");
+    const ptd::State &S = b->P.st;
+    if (!(S.slog_interval > 0)) return fail(PT_EINVAL, "pt_batch_spinlog: no spin log set");
+    PT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    const size_t n = (size_t)b->n, cap = (size_t)S.slog_cap;
+    static_assert(sizeof(ptd::SpinRec) == sizeof(pt_spin_point), "spin-log record layout");
+    PT_HIP(hipMemcpy(out, S.slog, n * cap * sizeof(ptd::SpinRec), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(count, S.nslog, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
 /* the snapshot records: out [n][cap] (cap of pt_batch_set_snapshots), count [n] (may exceed cap) */
 extern "C" int pt_batch_snapshots(pt_batch *b, void *stream, pt_snapshot *out, int32_t *count) {
     if (!b || !out || !count) return fail(PT_EINVAL, "pt_batch_snapshots: bad arguments");

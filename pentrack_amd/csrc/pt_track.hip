@@ -1294,9 +1294,10 @@ PTD bool spin_window(const Species &sp, double t) {
 /* TParticle::SpinDerivs without field interpolation (src/particle.cpp:141-190): the
    precession axis (BMT with Thomas precession) from the trajectory's dense output at t */
 template <int MODE>
-PTD void spin_axis(const Params &P, Track &T, const Dopri &traj, double t, double O[3]) {
+PTD void spin_axis(const Params &P, Track &T, const Dopri &traj, double t, double O[3], double *pos = nullptr) {
     double yt[9], d9[9], B[3], dB[3][3], E[3] = {0, 0, 0}, V = 0;
     dopri_calc_state(traj, t, yt);
+    if (pos) { pos[0] = yt[0]; pos[1] = yt[1]; pos[2] = yt[2]; }
     field_B<MODE>(P, yt[0], yt[1], yt[2], t, B, dB, true, T.c.field);
     field_E<MODE>(P, yt[0], yt[1], yt[2], t, &V, E, T.c.field);
     eom(T.sp, yt, B, dB, E, d9);
@@ -1312,6 +1313,23 @@ PTD void spin_axis(const Params &P, Track &T, const Dopri &traj, double t, doubl
            gamma_rel * gamma_rel / (gamma_rel + 1) * (d9[5] * d9[0] - d9[3] * d9[2]) / C_0 / C_0;
     O[2] = -gamma / gamma_rel * ((1 - gamma_rel) * Bp[2] + gamma_rel * B[2] - gamma_rel * (d9[0] * E[1] - d9[1] * E[0]) / C_0 / C_0) +
            gamma_rel * gamma_rel / (gamma_rel + 1) * (d9[3] * d9[1] - d9[4] * d9[0]) / C_0 / C_0;
+}
+
+/* TLogger::PrintSpin (src/logger.cpp:237-284): a row at time x unless x lies after the spin
+   stepper's previous time x1 in the same multiple of spinloginterval; the first row of a
+   spin integration (x1 = -1 here) is always written.  The spin at x is the integrator's. */
+template <int MODE>
+PTD void record_spin(const Params &P, Track &T, const Dopri &traj, double x1, double x, const double *spin) {
+    const State &S = P.st;
+    if (x1 >= 0 && x > x1 && (int)(x1 / S.slog_interval) == (int)(x / S.slog_interval)) return;
+    const int k = S.nslog[T.i];
+    S.nslog[T.i] = k + 1;
+    if (k >= S.slog_cap) return;
+    SpinRec &r = S.slog[(size_t)T.i * S.slog_cap + k];
+    double O[3], pos[3];
+    spin_axis<MODE>(P, T, traj, x, O, pos);
+    r.t = x;
+    for (int q = 0; q < 3; ++q) { r.pos[q] = pos[q]; r.spin[q] = spin[q]; r.W[q] = O[q]; }
 }
 
 /* interpolatefields: ALGLIB spline1dbuildcubic (parabolic ends) of each axis component over
@@ -1420,6 +1438,8 @@ PTD_NOINLINE bool bmt_integrate(const Params &P, Track &T, const Dopri &traj, do
         spin_splines_build<MODE>(P, T, traj, x1, x2, splines);
         spl = &splines;
     }
+    const bool slog = P.st.slog_interval > 0;
+    if (slog) record_spin<MODE>(P, T, traj, -1.0, x1, x);
     spin_derivs<MODE>(P, T, traj, spl, x, k1, t);
     for (;;) {
         int tries = 0;
@@ -1481,9 +1501,11 @@ PTD_NOINLINE bool bmt_integrate(const Params &P, Track &T, const Dopri &traj, do
             for (int i = 0; i < 5; ++i)
                 spin[i] = 1.0 * xo[i] + (h * b1t) * k1o[i] + (h * b3t) * k3[i] + (h * b4t) * k4[i] + (h * b5t) * k5[i] +
                           (h * b6t) * k6[i] + (h * b7t) * k1[i];
+            if (slog) record_spin<MODE>(P, T, traj, told, x2, spin);
             return true;
         }
         for (int i = 0; i < 5; ++i) spin[i] = x[i];
+        if (slog) record_spin<MODE>(P, T, traj, told, t, spin);
         if (t >= x2) return true;
     }
 }

@@ -26,6 +26,7 @@ DEFAULT_ENDLOGVARS = ["jobnumber", "particle", "tstart", "xstart", "ystart", "zs
 DEFAULT_TRACKLOGVARS = ["jobnumber", "particle", "polarisation", "t", "x", "y", "z", "vx", "vy", "vz", "H", "E",
                         "Bx", "dBxdx", "dBxdy", "dBxdz", "By", "dBydx", "dBydy", "dBydz", "Bz", "dBzdx", "dBzdy",
                         "dBzdz", "Ex", "Ey", "Ez", "V"]
+DEFAULT_SPINLOGVARS = ["jobnumber", "particle", "t", "x", "y", "z", "Sx", "Sy", "Sz", "Wx", "Wy", "Wz", "Bx", "By", "Bz"]
 DEFAULT_HITLOGVARS = ["jobnumber", "particle", "t", "x", "y", "z", "v1x", "v1y", "v1z", "pol1", "v2x", "v2y", "v2z",
                       "pol2", "nx", "ny", "nz", "solid1", "solid2"]
 
@@ -140,6 +141,24 @@ def hit_values(hits, n_hits, numbers, jobnumber=0):
          "v2x": rows["v2"][:, 0], "v2y": rows["v2"][:, 1], "v2z": rows["v2"][:, 2], "pol2": rows["pol2"],
          "nx": rows["normal"][:, 0], "ny": rows["normal"][:, 1], "nz": rows["normal"][:, 2],
          "solid1": rows["leaving"].astype(float), "solid2": rows["entering"].astype(float)}
+    return v
+
+
+def spin_values(recs, counts, numbers, fields, jobnumber=0):
+    """TLogger::PrintSpin variables (src/logger.cpp:250-281), particles in order: the recorded
+    time, position, spin and precession axis, and B there (value-only, as field.BField)."""
+    from .queries import fields_eval
+    cap = recs.shape[1]
+    k = np.minimum(np.asarray(counts), cap)
+    r = np.concatenate([recs[i, :k[i]] for i in range(len(k))]) if len(k) else recs.reshape(-1)[:0]
+    part = np.repeat(np.asarray(numbers, dtype=float), k)
+    B, _, _, _ = fields_eval(fields, np.ascontiguousarray(r["pos"]), np.ascontiguousarray(r["t"]), grad=False)
+    v = {"jobnumber": np.full(len(r), float(jobnumber)), "particle": part, "t": r["t"]}
+    for a, ax in enumerate("xyz"):
+        v[ax] = r["pos"][:, a]
+        v["S" + ax] = r["spin"][:, a]
+        v["W" + ax] = r["W"][:, a]
+        v["B" + ax] = B[:, a]
     return v
 
 

@@ -45,6 +45,7 @@ class Batch:
         self.hit_cap = hit_cap
         self.snap_cap = 0
         self.trk_cap = 0
+        self.slog_cap = 0
         sp = (_native.Species * len(species))(*[s.native() for s in species])
         self._keep = [np.ascontiguousarray(species_idx, dtype=np.int32), np.ascontiguousarray(t0, dtype=np.float64),
                       np.ascontiguousarray(y0, dtype=np.float64).reshape(-1, 9),
@@ -127,6 +128,22 @@ class Batch:
                                                    cnt.ctypes.data_as(C.POINTER(C.c_int32))), "pt_batch_track")
         return rec.reshape(self.n, self.trk_cap), cnt
 
+    def set_spinlog(self, interval, cap):
+        """TLogger::PrintSpin records inside the BMT spin-tracking windows: the start of each
+        spin integration and spin-step ends that crossed a multiple of `interval` (s), at
+        most cap per particle; call before the first run."""
+        self.slog_cap = int(cap) if interval > 0 else 0
+        _native.check(_native.lib().pt_batch_set_spinlog(self.h, float(interval), self.slog_cap),
+                      "pt_batch_set_spinlog")
+
+    def spinlog(self, stream=None):
+        """(records [n][cap] SPIN_DTYPE, counts [n]): the particles' spin-log rows."""
+        rec = np.zeros(self.n * self.slog_cap, dtype=_native.SPIN_DTYPE)
+        cnt = np.zeros(self.n, dtype=np.int32)
+        _native.check(_native.lib().pt_batch_spinlog(self.h, stream, rec.ctypes.data_as(C.c_void_p),
+                                                     cnt.ctypes.data_as(C.POINTER(C.c_int32))), "pt_batch_spinlog")
+        return rec.reshape(self.n, self.slog_cap), cnt
+
     def download(self, stream=None):
         out = np.zeros(self.n, dtype=_native.OUT_DTYPE)
         hits = np.zeros(self.n * self.hit_cap, dtype=_native.HIT_DTYPE) if self.hit_cap else None
@@ -192,9 +209,10 @@ class TTracker:
         self.launch = launch or LaunchConfig()
 
     def IntegrateBatch(self, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap=0, snapshots=None,
-                       track=None):
+                       track=None, spinlog=None):
         """(out, hits); with snapshots = (times, cap) also (records, counts) of the snapshots,
-        with track = (interval, cap) also (records, counts) of the track log."""
+        with track = (interval, cap) also (records, counts) of the track log, with
+        spinlog = (interval, cap) also (records, counts) of the spin log, in that order."""
         cfg = self.launch
         b = Batch(self.geometry, self.fields, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap,
                   with_fp32=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
@@ -202,12 +220,16 @@ class TTracker:
             b.set_snapshots(*snapshots)
         if track is not None:
             b.set_track(*track)
+        if spinlog is not None:
+            b.set_spinlog(*spinlog)
         b.run_to_completion(cfg)
         res = b.download()
         if snapshots is not None:
             res = res + b.snapshots()
         if track is not None:
             res = res + b.track()
+        if spinlog is not None:
+            res = res + b.spinlog()
         return res
 
     def IntegrateWithSecondaries(self, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap=0):

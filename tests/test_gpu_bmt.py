@@ -51,3 +51,31 @@ def test_interpolated_axis_gpu(gpu, oracle_lib, name, params):
     for f in FIELDS:
         assert np.array_equal(g[f], o[f]), f
     assert (g["spin"][:, 4] > 1).all()
+
+
+@pytest.mark.parametrize("which", ["larmor", "cylinder"])
+def test_spin_log_matches_oracle(gpu, oracle_lib, which):
+    """TLogger::PrintSpin rows recorded inside bmt_integrate, bit for bit with the oracle."""
+    from pentrack_amd.tracker import Batch, LaunchConfig
+    if which == "larmor":
+        w, iv = larmor_workload(1e-6, (0.0, 1.0), 0.05, n=64), 2e-3
+    else:
+        base = workloads.cylinder(n=200, seed=9, tmax=0.15, n_grid=24, Bmax=2e-5)
+        sp = [SpeciesConfig(kind=0, tau=0.0, tmax=9e99, lmax=9e99, Bmax=1e-5, spintimes=(0.0, 0.1))]
+        spin0 = base.spin0.copy()
+        spin0[:, 0] = 1.0
+        w = Workload("bmt", base.geometry, base.fields, sp, base.tmax, base.species_idx, base.t0, base.y0, spin0,
+                     base.pid, base.seed)
+        iv = 5e-3
+    b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid)
+    b.set_spinlog(iv, 128)
+    print(f"case: spin log {which} n={w.n}", flush=True)
+    assert b.run_to_completion(LaunchConfig(), max_launches=100000)
+    out, _ = b.download()
+    rec, cnt = b.spinlog()
+    o, orec, ocnt = oracle_lib.track_spinlog(w, iv, 128, threads=8)
+    assert np.array_equal(out["spin"], o["spin"]) and np.array_equal(cnt, ocnt)
+    assert cnt.sum() > 10 * w.n
+    kept = np.minimum(cnt, 128)
+    for k in range(w.n):
+        assert np.array_equal(rec[k, :kept[k]], orec[k, :kept[k]]), k

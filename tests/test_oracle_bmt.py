@@ -100,3 +100,33 @@ def test_interpolated_axis_tracks_direct_axis(oracle_lib):
     assert np.allclose(i["spin"][:, 4], d["spin"][:, 4], rtol=1e-6, atol=0)
     assert np.allclose(i["spin"][:, :3], d["spin"][:, :3], rtol=0, atol=2e-5)
     assert not np.array_equal(i["spin"], d["spin"])
+
+
+def test_spin_log_larmor(oracle_lib):
+    """TLogger::PrintSpin rows (src/logger.cpp:237-284): one at the start of every spin
+    integration (each trajectory step) and one after each spin step that crossed a multiple
+    of the interval; in the uniform field each row holds the Larmor spin, the axis
+    W = -gamma B0 z_hat and the free-fall position at its time."""
+    from pentrack_amd.constants import GRAVCONST
+    B0, T, iv = 1e-6, 0.05, 5e-3
+    w = larmor_workload(B0, (0.0, 1.0), T)
+    o, rec, cnt = oracle_lib.track_spinlog(w, iv, 256)
+    d, _ = oracle_lib.track(w, "fp64")
+    assert np.array_equal(o["spin"], d["spin"])                 # logging changes nothing
+    wl = -GAMMA_N * B0
+    for k in range(w.n):
+        r = rec[k, :cnt[k]]
+        assert 0 < cnt[k] <= 256 and r["t"][0] == 0 and r["t"][-1] == T
+        assert (np.diff(r["t"]) >= 0).all()
+        assert np.allclose(r["spin"][:, 0], np.cos(wl * r["t"]), atol=1e-8)
+        assert np.allclose(r["spin"][:, 1], np.sin(wl * r["t"]), atol=1e-8)
+        assert np.allclose(r["W"][:, 2], wl, rtol=1e-12) and np.abs(r["W"][:, :2]).max() < 1e-12 * wl
+        p = w.y0[k, :3] + np.outer(r["t"], w.y0[k, 3:6])
+        p[:, 2] -= GRAVCONST * r["t"] ** 2 / 2
+        assert np.allclose(r["pos"], p, atol=1e-9)
+        # every multiple of the interval is followed by a row within one spin step (< 3 ms
+        # here), and a row either starts a trajectory step or ends a spin step that crossed
+        # a multiple, which no other spin step crosses
+        for b in iv * np.arange(1, int(T / iv) + 1):
+            assert (r["t"] - b)[r["t"] >= b].min() < 3e-3, b
+        assert cnt[k] <= int(T / iv) + o["Nstep"][k] + 1
