@@ -6,6 +6,7 @@ the source, tracks `simcount` particles on the GPU (and their decay products whe
 `secondaries 1`), prints the stopID histogram per species and writes the text end and
 hit logs of species whose `endlog` / `hitlog` is 1.  Hits are recorded up to HIT_CAP per
 particle; a run where a particle hit more often says so (the reference logs every hit).
+Log filters and formula log variables are evaluated from [FORMULAS] as the reference does.
 """
 import os
 import sys
@@ -23,6 +24,7 @@ def main(argv=None):
     outpath = os.path.abspath(argv[2] if len(argv) > 2 else "out")
     seed = int(argv[3]) if len(argv) > 3 else 0
     from . import config as cfgmod
+    from . import logformula
     from .logger import (DEFAULT_ENDLOGVARS, DEFAULT_HITLOGVARS, DEFAULT_SPINLOGVARS, DEFAULT_TRACKLOGVARS, end_values,
                          hit_values, snapshot_values, spin_values, track_values, write_log)
     from .source import create_source
@@ -49,10 +51,17 @@ def main(argv=None):
     names = {0: "neutron", 1: "proton", 2: "electron"}
     settings = {k: cfgmod.particle_settings(cfg, v) for k, v in names.items()}
     on = lambda kind, log: settings[kind].get(log, "").split()[:1] == ["1"]
-    for kind in names:
-        for log in ("end", "hit", "snapshot", "track", "spin"):
-            if on(kind, log + "log") and settings[kind].get(log + "logfilter", "").strip():
-                raise SystemExit(f"{log}logfilter is not supported by this engine")
+    formulas = {k: v.strip() for k, v in cfg.items("FORMULAS")}
+
+    def emit(particle, log, titles, vals):
+        """TLogger::Log (src/logger.cpp:286-311): the filter formula and formula columns."""
+        kind = {v: k for k, v in names.items()}[particle]
+        filt = (settings[kind].get(log + "logfilter", "").split() or [""])[0]
+        try:
+            cols = logformula.apply(formulas, filt, titles, vals)
+        except logformula.FormulaError as e:
+            raise SystemExit(f"{log}log: {e}")
+        return write_log(outpath, jobnumber, particle, log, titles, cols)
     hit_cap = HIT_CAP if any(on(k, "hitlog") for k in names) else 0
     snaps = None
     if on(source.kind, "snapshotlog"):
@@ -88,10 +97,7 @@ def main(argv=None):
                       "their spin log keeps the first ones")
             vals = spin_values(srec, scnt, pid.astype(float), fields, jobnumber)
             titles = settings[source.kind].get("spinlogvars", "").split() or DEFAULT_SPINLOGVARS
-            missing = [t for t in titles if t not in vals]
-            if missing:
-                raise SystemExit(f"spinlogvars not supported: {missing}")
-            print("wrote", write_log(outpath, jobnumber, names[source.kind], "spin", titles, vals))
+            print("wrote", emit(names[source.kind], "spin", titles, vals))
         if trk is not None:
             trec, tcnt = res[-2], res[-1]
             if (tcnt > trk[1]).any():
@@ -101,20 +107,14 @@ def main(argv=None):
             vals = track_values(source.kind, t0, y0, solid_at(geometry, y0[:, :3], t0), trec, tcnt, pid.astype(float),
                                 geometry, fields, jobnumber)
             titles = settings[source.kind].get("tracklogvars", "").split() or DEFAULT_TRACKLOGVARS
-            missing = [t for t in titles if t not in vals]
-            if missing:
-                raise SystemExit(f"tracklogvars not supported: {missing}")
-            print("wrote", write_log(outpath, jobnumber, names[source.kind], "track", titles, vals))
+            print("wrote", emit(names[source.kind], "track", titles, vals))
         out, hits = res[0], res[1]
         sout, shits, rec = np.zeros(0, out.dtype), None, None
         if snaps is not None:
             vals = snapshot_values(source.kind, t0, y0, spin0, res[2], res[3], pid.astype(float), geometry, fields,
                                    jobnumber)
             titles = settings[source.kind].get("snapshotlogvars", "").split() or DEFAULT_ENDLOGVARS
-            missing = [t for t in titles if t not in vals]
-            if missing:
-                raise SystemExit(f"snapshotlogvars not supported: {missing}")
-            print("wrote", write_log(outpath, jobnumber, names[source.kind], "snapshot", titles, vals))
+            print("wrote", emit(names[source.kind], "snapshot", titles, vals))
     groups = [(source.kind, out, hits, t0, y0, spin0, pid.astype(float))]
     if rec is not None and len(rec):
         _, sidx, st0, sy0, sspin0, _ = secondary_states(species, rec, pid)
@@ -139,11 +139,7 @@ def main(argv=None):
                     print(f"warning: {int((o['n_hits'] > hit_cap).sum())} {names[kind]}s hit walls more than "
                           f"{hit_cap} times; their hit log keeps the first {hit_cap}")
                 vals = hit_values(h, o["n_hits"], numbers, jobnumber)
-            missing = [t for t in titles if t not in vals]
-            if missing:
-                raise SystemExit(f"{log}logvars not supported: {missing}")
-            path = write_log(outpath, jobnumber, names[kind], log, titles, vals)
-            print("wrote", path)
+            print("wrote", emit(names[kind], log, titles, vals))
     steps = int(out["Nstep"].sum() + (sout["Nstep"].sum() if len(sout) else 0))
     print(f"{steps} steps")
     return 0

@@ -84,7 +84,8 @@ def test_command_line_run(gpu, oracle_lib, tmp_path):
 
 def test_command_line_spin_log(gpu, oracle_lib, tmp_path):
     """spinlog 1 on the command line: the spin log file holds the rows the oracle records
-    (src/logger.cpp:237-284), with B at each row's position and time."""
+    (src/logger.cpp:237-284), with B at each row's position and time, filtered by a
+    spinlogfilter formula and with a formula column (src/logger.cpp:286-308)."""
     from tests.test_config import CONFIG, write_config
     from pentrack_amd.__main__ import main
     from pentrack_amd import config as C
@@ -93,13 +94,16 @@ def test_command_line_spin_log(gpu, oracle_lib, tmp_path):
     from oracle import ptref
     cfgpath = write_config(str(tmp_path))
     text = CONFIG.replace("secondaries 1", "secondaries 0").replace("simcount 200", "simcount 20")
-    text = text.replace("[neutron]\ntau 0.05\n", "[neutron]\ntau 0.05\nspintimes 0 0.02\nspinlog 1\nspinloginterval 1e-3\n")
+    text = text.replace("[neutron]\ntau 0.05\n", "[neutron]\ntau 0.05\nspintimes 0 0.02\nspinlog 1\nspinloginterval 1e-3\n"
+                        "spinlogfilter late\nspinlogvars jobnumber particle t x y z Sx Sy Sz Wx Wy Wz Bx By Bz Sperp\n")
+    text = text.replace("[FORMULAS]\n", "[FORMULAS]\nlate t >= 0.005 and z > 0\nSperp sqrt(Sx^2 + Sy^2)\n")
     open(cfgpath, "w").write(text)
     out = str(tmp_path / "out")
     print("case: command line spin log simcount 20", flush=True)
     assert main(["3", cfgpath, out, "77"]) == 0
     titles, slog = read_log(os.path.join(out, "000000000003neutronspin.out"))
-    assert titles == ["jobnumber", "particle", "t", "x", "y", "z", "Sx", "Sy", "Sz", "Wx", "Wy", "Wz", "Bx", "By", "Bz"]
+    assert titles == ["jobnumber", "particle", "t", "x", "y", "z", "Sx", "Sy", "Sz", "Wx", "Wy", "Wz", "Bx", "By", "Bz",
+                      "Sperp"]
     cfg = C.TConfig(cfgpath)
     geometry, fields = C.load_geometry(cfg, str(tmp_path)), C.load_fields(cfg)
     t0, y0, spin0 = VolumeSource(cfg).sample(20, np.random.default_rng(77))
@@ -107,10 +111,15 @@ def test_command_line_spin_log(gpu, oracle_lib, tmp_path):
     w = Workload("cli", geometry, fields, species, 0.5, np.zeros(20, np.int32), t0, y0, spin0,
                  np.arange(1, 21, dtype=np.uint64), 77)
     o, rec, cnt = oracle_lib.track_spinlog(w, 1e-3, 4096, threads=8)
-    assert (cnt <= 4096).all() and len(slog) == cnt.sum() > 20 * 10
+    assert (cnt <= 4096).all()
     rows = np.concatenate([rec[k, :cnt[k]] for k in range(20)])
+    num = np.repeat(np.arange(1, 21), cnt)
+    keep = (rows["t"] >= 0.005) & (rows["pos"][:, 2] > 0)
+    assert 20 * 5 < keep.sum() < len(rows)
+    rows, num = rows[keep], num[keep]
     c = {t: k for k, t in enumerate(titles)}
-    assert np.array_equal(slog[:, c["particle"]], np.repeat(np.arange(1, 21), cnt))
+    assert len(slog) == len(rows) and np.array_equal(slog[:, c["particle"]], num)
+    assert np.allclose(slog[:, c["Sperp"]], np.hypot(rows["spin"][:, 0], rows["spin"][:, 1]), rtol=1e-14)
     for name, want in (("t", rows["t"]), ("x", rows["pos"][:, 0]), ("z", rows["pos"][:, 2]), ("Sx", rows["spin"][:, 0]),
                        ("Sz", rows["spin"][:, 2]), ("Wz", rows["W"][:, 2])):
         assert np.allclose(slog[:, c[name]], want, rtol=1e-14, atol=1e-300), name
