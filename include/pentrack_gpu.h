@@ -37,8 +37,8 @@ extern "C" {
 #define PT_ID_ABSORBED_ON_SURFACE 2
 /* extensions: a reference runtime_error (src/tracking.cpp:38,286,333,350) becomes a
    per-particle invariant stop; a configuration outside the implemented path
-   (micro-roughness model, BMT spin integration, more than PT_MAX_HITS
-   intersections of one segment) stops the particle as unsupported */
+   (micro-roughness model, more than PT_MAX_HITS intersections of one segment, more
+   than PT_MAX_SOLIDS nested solids) stops the particle as unsupported */
 #define PT_ID_INVARIANT (-8)
 #define PT_ID_UNSUPPORTED (-9)
 
@@ -275,9 +275,9 @@ typedef struct {
  * :448): inside the BMT spin-tracking windows a row is recorded at the start of each spin
  * integration and after each spin step whose end x satisfies int(x1 / interval) !=
  * int(x / interval) for the spin stepper's previous time x1 (the end is clamped to the
- * trajectory step's end, where the row holds the dense-output spin).  At the first call the
- * reference reads the previous time of a stepper that has made no step (an uninitialised
- * value); that row is always written here.  cap records are kept per particle (more are
+ * trajectory step's end, where the row holds the dense-output spin).  At the first call of
+ * an integration the fresh stepper's previous time is 0.0, so its start row is skipped when
+ * the integration starts in (0, interval).  cap records are kept per particle (more are
  * counted); interval = 0 switches it off.
  */
 int pt_batch_set_spinlog(pt_batch *b, double interval, int cap);

@@ -1940,11 +1940,12 @@ void ptref_set_spinlog(double interval, int cap, ptref_spinpt *out, int32_t *cou
 
 /* TLogger::PrintSpin (src/logger.cpp:237-284) as IntegrateSpin calls it (src/tracking.cpp:431,
    :448): at x unless x lies after the spin stepper's previous time x1 in the same multiple
-   of spinloginterval.  At the call after initialize the stepper's previous time is
-   uninitialised in the reference; that row (x1 < 0 here) is always written.  The spin at x
-   is the integrator's (current state, or the dense output at the clamped end). */
+   of spinloginterval.  At the call right after initialize (src/tracking.cpp:430-431) the
+   fresh stepper has made no step and its previous time is the value-initialised 0.0, so
+   the start row of an integration starting in (0, spinloginterval) is skipped.  The spin
+   at x is the integrator's (current state, or the dense output at the clamped end). */
 static void print_spin(track_t *tp, const dopri_t *traj, double x1, double x, const double *spin) {
-    if (x1 >= 0 && x > x1 && (int)(x1 / g_slog_interval) == (int)(x / g_slog_interval)) return;
+    if (x > x1 && (int)(x1 / g_slog_interval) == (int)(x / g_slog_interval)) return;
     int k = g_nslog[g_particle]++;
     if (k >= g_slog_cap) return;
     ptref_spinpt *r = &g_slog[(size_t)g_particle * g_slog_cap + k];
@@ -1961,7 +1962,7 @@ static int bmt_steps(track_t *tp, double *spin, const dopri_t *traj, double x1, 
     memcpy(s.xc, spin, sizeof(s.xc));
     s.t = x1;
     s.dt = fabs(PI / (double)tp->sp.gamma / Babs1);    /* GetGyromagneticRatio() is a double */
-    if (g_slog_interval > 0) print_spin(tp, traj, -1.0, x1, s.xc);
+    if (g_slog_interval > 0) print_spin(tp, traj, 0.0, x1, s.xc);
     spin_derivs(tp, traj, s.xc, s.dxc, s.t);
     for (;;) {
         int tries = 0, ok;

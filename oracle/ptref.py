@@ -222,9 +222,10 @@ SNAP_DTYPE = np.dtype([("t", "<f8"), ("y", "<f8", (9,)), ("spin", "<f8", (5,)), 
 
 
 def track_snapshots(workload, times, cap, mode="fp64", threads=1):
-    """The oracle run with snapshot times: (out, records [n][cap], counts [n])."""
+    """The oracle run with snapshot times in the given order (PrintSnapshot's find_if,
+    src/logger.cpp:134): (out, records [n][cap], counts [n])."""
     L = load(mode)
-    t = np.ascontiguousarray(np.unique(np.asarray(times, dtype=np.float64)))
+    t = np.ascontiguousarray(np.asarray(times, dtype=np.float64))
     snaps = np.zeros(workload.n * cap, dtype=SNAP_DTYPE)
     counts = np.zeros(workload.n, dtype=np.int32)
     L.ptref_set_snapshots(len(t), _dp(t), cap, snaps.ctypes.data_as(C.c_void_p),

@@ -98,11 +98,6 @@ class AdjointModel:
         P.dt, P.coupling, P.grav = self.dt, self.coupling, self.grav
         return P
 
-    def oracle_model(self, s):
-        """The same model for the CPU restatement (oracle/ptadj.c); libtricubic layout."""
-        from oracle import ptref
-        return ptref.AdjModel(self, s)
-
 
 def _lib():
     L = _native.lib()

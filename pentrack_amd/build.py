@@ -1,15 +1,25 @@
-"""Build libpentrack_gpu.so in-tree with hipcc for gfx950."""
+"""Build libpentrack_gpu.so in-tree with hipcc for gfx950.
+
+Every source is compiled to its own object in parallel; the tracking kernels
+(pt_track.hip) are compiled three times per kernel mode (-DPT_MODE_TU=m: step,
+persistent-step and collision kernels) plus once for
+the mode-independent kernels and launchers, then everything is linked.
+"""
 import os
 import subprocess
 import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpentrack_gpu.so")
+OBJ = os.path.join(HERE, "_obj")
 
-SOURCES = ["pt_track.hip", "pt_adjoint.hip", "pt_precompute.hip", "pt_host.cpp", "pt_formula.cpp"]
+N_MODES = 4
+SOURCES = ["pt_adjoint.hip", "pt_precompute.hip", "pt_host.cpp", "pt_formula.cpp"]
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     # bit-reproducible FP64: no contraction, no reassociation
     "-ffp-contract=off", "-fno-fast-math",
     # packed FP32 instructions disabled (hipcc only)
@@ -18,13 +28,43 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
+def _units():
+    """(object name, source, extra flags) of every translation unit."""
+    units = [("pt_track_common.o", "pt_track.hip", [])]
+    units += [(f"pt_track_m{m}{'scp'[k]}.o", "pt_track.hip", [f"-DPT_MODE_TU={m}", f"-DPT_KERNEL_SET={k}"])
+              for m in reversed(range(N_MODES)) for k in (0, 2, 1)]
+    units += [(os.path.splitext(s)[0] + ".o", s, []) for s in SOURCES]
+    return units
+
+
+def build(verbose=False, jobs=None):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc] + FLAGS + ["-o", OUT + ".tmp", "-x", "hip"] + [os.path.join(CSRC, s) for s in SOURCES]
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    os.makedirs(OBJ, exist_ok=True)
+    units = _units()
+
+    def compile_one(u):
+        obj, src, extra = u
+        cmd = [hipcc] + FLAGS + extra + ["-c", "-x", "hip", os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
+        t = time.perf_counter()
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if verbose:
+            print(f"  {obj}: {time.perf_counter() - t:.0f} s", flush=True)
+        return obj, r
+
+    jobs = jobs or min(len(units), max(1, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(compile_one, units))
+    failed = [(o, r) for o, r in results if r.returncode != 0]
+    for o, r in failed:
+        sys.stderr.write(f"--- {o}\n" + r.stdout + r.stderr)
+    if failed:
+        raise RuntimeError("hipcc failed building libpentrack_gpu.so: " + ", ".join(o for o, _ in failed))
+    link = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp"] + \
+        [os.path.join(OBJ, u[0]) for u in units]
+    r = subprocess.run(link, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libpentrack_gpu.so")
+        raise RuntimeError("linking libpentrack_gpu.so failed")
     os.replace(OUT + ".tmp", OUT)
     if verbose:
         print("built", OUT)

@@ -137,7 +137,7 @@ struct State {
     int hit_cap;
     /* snapshots: sorted times, records [n][snap_cap], records per particle */
     const double *snap_t;
-    int snap_n, snap_cap;
+    int snap_n, snap_cap, snap_sorted;
     Snap *snaps;
     int *nsnap;
     /* track log: length interval (0: off), records [n][trk_cap], records per particle */

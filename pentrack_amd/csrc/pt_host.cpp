@@ -1473,14 +1473,16 @@ extern "C" int pt_batch_download(pt_batch *b, void *stream, pt_particle_out *out
     return PT_OK;
 }
 
-/* TLogger::PrintSnapshot (src/logger.cpp:128-140): snapshot times (sorted and de-duplicated
-   here) and the records kept per particle; n_times = 0 switches snapshots off */
+/* TLogger::PrintSnapshot (src/logger.cpp:128-140): snapshot times in the caller's order (a
+   non-decreasing list is de-duplicated and searched by bisection, which finds the same
+   time as find_if) and the records kept per particle; n_times = 0 switches snapshots off */
 extern "C" int pt_batch_set_snapshots(pt_batch *b, int n_times, const double *times, int cap) {
     if (!b || n_times < 0 || (n_times > 0 && (!times || cap <= 0))) return fail(PT_EINVAL, "pt_batch_set_snapshots: bad arguments");
     ptd::State &S = b->P.st;
     std::vector<double> t(times, times + n_times);
-    std::sort(t.begin(), t.end());
-    t.erase(std::unique(t.begin(), t.end()), t.end());
+    const bool sorted = std::is_sorted(t.begin(), t.end());
+    if (sorted) t.erase(std::unique(t.begin(), t.end()), t.end());
+    S.snap_sorted = sorted ? 1 : 0;
     S.snap_n = 0;
     if (!t.empty()) {
         double *dt;

@@ -15,6 +15,15 @@ namespace ptd { struct Params; struct Geom; }
 /* mode: bit 0 FP32 field tables (PT_VARIANT_MIXED), bit 1 containers with scale formulas,
    analytic fields or conductors present */
 
+/* the mode-templated launches, each instantiated in the translation unit of its mode */
+template <int MODE> hipError_t pt_launch_init_mode(const ptd::Params &P, int grid, hipStream_t stream);
+template <int MODE> hipError_t pt_launch_step_mode(const ptd::Params &P, int grid, int block, hipStream_t stream);
+template <int MODE> hipError_t pt_launch_step_persistent_mode(const ptd::Params &P, int grid, hipStream_t stream);
+template <int MODE> hipError_t pt_launch_collision_mode(const ptd::Params &P, int grid, hipStream_t stream);
+template <int MODE>
+hipError_t pt_launch_fields_eval_mode(const ptd::Params &P, int n, const double *pts, const double *t, double *B,
+                                      double *dB, double *V, double *E, hipStream_t stream);
+
 /* particle set-up (first call only) and the step budget of every particle */
 hipError_t pt_launch_init(const ptd::Params &P, int mode, int steps, hipStream_t stream);
 /* one round = the step kernel, then the collision kernel over the queue and the queue reset */

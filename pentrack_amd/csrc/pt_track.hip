@@ -21,6 +21,21 @@
 #include "pt_formula.h"
 #include "pt_kernels.h"
 
+/* This file is compiled three times per kernel mode (-DPT_MODE_TU=0..3 with
+   -DPT_KERNEL_SET=0: the mode's set-up and step kernels, =2: its persistent step kernel,
+   =1: its collision kernel and field queries) and once
+   without PT_MODE_TU (the mode-independent kernels and the launchers that dispatch on
+   the mode), so the modes build in parallel. */
+#ifndef PT_MODE_TU
+#define PT_MODE_TU (-1)
+#define PT_COMMON_TU 1
+#else
+#define PT_COMMON_TU 0
+#endif
+#ifndef PT_KERNEL_SET
+#define PT_KERNEL_SET 0
+#endif
+
 namespace ptd {
 
 /* ------------------------------------------------------------------------ */
@@ -769,10 +784,15 @@ PTD void dense_calc(const DenseRef &r, double t, double *y) {
 
 PTD void dense_calc(const Dopri &s, double t, double *y) { dopri_calc_state(s, t, y); }
 
-/* TLogger::PrintSnapshot (src/logger.cpp:128-140): the first snapshot time in [x1, x2),
-   with the snapshot times sorted (-1: none) */
+/* TLogger::PrintSnapshot (src/logger.cpp:128-140): the index of the first snapshot time in
+   the configured order (find_if) with x1 <= t < x2, -1 if none; a sorted list by bisection */
 #define NOINLINE_SNAP __attribute__((noinline))
 PTD int snapshot_in(const State &S, double x1, double x2) {
+    if (!S.snap_sorted) {
+        for (int j = 0; j < S.snap_n; ++j)
+            if (x1 <= S.snap_t[j] && S.snap_t[j] < x2) return j;
+        return -1;
+    }
     int lo = 0, hi = S.snap_n;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -1316,12 +1336,13 @@ PTD void spin_axis(const Params &P, Track &T, const Dopri &traj, double t, doubl
 }
 
 /* TLogger::PrintSpin (src/logger.cpp:237-284): a row at time x unless x lies after the spin
-   stepper's previous time x1 in the same multiple of spinloginterval; the first row of a
-   spin integration (x1 = -1 here) is always written.  The spin at x is the integrator's. */
+   stepper's previous time x1 in the same multiple of spinloginterval.  For the row right
+   after the stepper's initialize (src/tracking.cpp:430-431) x1 is the fresh stepper's
+   value-initialised previous time, 0.0.  The spin at x is the integrator's. */
 template <int MODE>
 PTD void record_spin(const Params &P, Track &T, const Dopri &traj, double x1, double x, const double *spin) {
     const State &S = P.st;
-    if (x1 >= 0 && x > x1 && (int)(x1 / S.slog_interval) == (int)(x / S.slog_interval)) return;
+    if (x > x1 && (int)(x1 / S.slog_interval) == (int)(x / S.slog_interval)) return;
     const int k = S.nslog[T.i];
     S.nslog[T.i] = k + 1;
     if (k >= S.slog_cap) return;
@@ -1439,7 +1460,7 @@ PTD_NOINLINE bool bmt_integrate(const Params &P, Track &T, const Dopri &traj, do
         spl = &splines;
     }
     const bool slog = P.st.slog_interval > 0;
-    if (slog) record_spin<MODE>(P, T, traj, -1.0, x1, x);
+    if (slog) record_spin<MODE>(P, T, traj, 0.0, x1, x);
     spin_derivs<MODE>(P, T, traj, spl, x, k1, t);
     for (;;) {
         int tries = 0;
@@ -2263,6 +2284,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
 }
 
 /* work list of the persistent kernel: particles that are running and have steps left */
+#if PT_COMMON_TU
 __global__ void __launch_bounds__(256) work_kernel(Params P) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const State &S = P.st;
@@ -2272,14 +2294,20 @@ __global__ void __launch_bounds__(256) work_kernel(Params P) {
         S.work[w] = i;
     }
 }
+#endif
 
+#if PT_COMMON_TU
 __global__ void reset_work_kernel(int *wcount) { wcount[0] = 0; wcount[1] = 0; }
+#endif
 
 /* ---- decay products (TParticle::DoDecay, src/particle.cpp:219-223) ---------------- */
+#if PT_COMMON_TU
 __global__ void reset_count_kernel(int *count) { *count = 0; }
+#endif
 
 /* TNeutron::Decay (src/neutron.cpp:285-362) for every neutron stopped with ID_DECAYED, from
    its final state and a copy of its stream position */
+#if PT_COMMON_TU
 __global__ void __launch_bounds__(256) decay_kernel(Params P, pt_beta_table T, pt_secondary *out, int *count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const State &S = P.st;
@@ -2297,6 +2325,7 @@ __global__ void __launch_bounds__(256) decay_kernel(Params P, pt_beta_table T, p
     out[slot] = p;
     out[slot + 1] = e;
 }
+#endif
 
 /* ---- field and solid queries (TFieldManager::BField/EField, TGeometry::GetSolid) ---- */
 
@@ -2323,6 +2352,7 @@ __global__ void __launch_bounds__(256) fields_eval_kernel(Params P, int n, const
 
 /* TGeometry::GetSolid (src/geometry.cpp:151-175): the solids containing the point (+z ray
    parity), the highest ID among those not ignored at t; -1 if more than MAX_SOLIDS nest */
+#if PT_COMMON_TU
 __global__ void __launch_bounds__(256) solid_at_kernel(Geom G, int n, const double *pts, const double *t, int *out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -2338,16 +2368,19 @@ __global__ void __launch_bounds__(256) solid_at_kernel(Geom G, int n, const doub
     }
     out[i] = (int)solid_ID(G, current_solid(G, sk, ns));
 }
+#endif
 
 /* ---- work list in spatial order (counting sort into Morton-ordered buckets) ---- */
 
 /* clears the bucket counts and the work-list counters (kernels, not memsets, so the
    launch sequence can be captured in a graph) */
+#if PT_COMMON_TU
 __global__ void __launch_bounds__(256) clear_buckets_kernel(int *bhist, int *wcount) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b <= NBUCKET) bhist[b] = 0;
     if (b == 0) { wcount[0] = 0; wcount[1] = 0; }
 }
+#endif
 
 /* spreads the low BUCKET_BITS bits of v to every third bit */
 PTD unsigned spread3(unsigned v) {
@@ -2359,6 +2392,7 @@ PTD unsigned spread3(unsigned v) {
 
 /* bucket of each particle with steps to do: its cell of a 32^3 grid over the mesh
    bounding box (the clearance grid's box), in Morton order */
+#if PT_COMMON_TU
 __global__ void __launch_bounds__(256) bucket_count_kernel(Params P) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const State &S = P.st;
@@ -2379,8 +2413,10 @@ __global__ void __launch_bounds__(256) bucket_count_kernel(Params P) {
         atomicAdd(S.wcount, 1);
     }
 }
+#endif
 
 /* exclusive prefix sum of the NBUCKET counts in one block of 1024 threads */
+#if PT_COMMON_TU
 __global__ void __launch_bounds__(1024) bucket_scan_kernel(int *bhist) {
     constexpr int PER = NBUCKET / 1024;
     __shared__ int part[1024];
@@ -2401,8 +2437,10 @@ __global__ void __launch_bounds__(1024) bucket_scan_kernel(int *bhist) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) { bhist[t * PER + k] = run; run += local[k]; }
 }
+#endif
 
 /* places each particle with steps to do at the next free slot of its bucket */
+#if PT_COMMON_TU
 __global__ void __launch_bounds__(256) bucket_scatter_kernel(Params P) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const State &S = P.st;
@@ -2412,6 +2450,7 @@ __global__ void __launch_bounds__(256) bucket_scatter_kernel(Params P) {
         S.work[slot] = i;
     }
 }
+#endif
 
 /*
  * Collision kernel: the parked particles of the queue, compacted into full
@@ -2435,27 +2474,83 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_COL
 }
 
 /* empties the collision queue (a kernel, not a memset, so launch loops can be captured) */
+#if PT_COMMON_TU
 __global__ void reset_queue_kernel(int *qcount) { *qcount = 0; }
+#endif
 
 /* step budget of every particle for the next round of launches */
+#if PT_COMMON_TU
 __global__ void __launch_bounds__(256) budget_kernel(int *budget, int n, int steps) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) budget[i] = steps;
 }
+#endif
 
 } // namespace ptd
 
 /* ---- launchers (host side, called from pt_host.cpp) ---------------------- */
 
+#if PT_MODE_TU >= 0
+/* the kernels of one mode (this translation unit is built once per mode) */
+template <int MODE>
+hipError_t pt_launch_init_mode(const ptd::Params &P, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL(ptd::init_kernel<MODE>, dim3(grid), dim3(256), 0, stream, P);
+    return hipGetLastError();
+}
+template <int MODE>
+hipError_t pt_launch_step_mode(const ptd::Params &P, int grid, int block, hipStream_t stream) {
+    hipLaunchKernelGGL(ptd::step_kernel<MODE>, dim3(grid), dim3(block), 0, stream, P);
+    return hipGetLastError();
+}
+template <int MODE>
+hipError_t pt_launch_step_persistent_mode(const ptd::Params &P, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL(ptd::step_kernel_persistent<MODE>, dim3(grid), dim3(256), 0, stream, P);
+    return hipGetLastError();
+}
+template <int MODE>
+hipError_t pt_launch_collision_mode(const ptd::Params &P, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL(ptd::collision_kernel<MODE>, dim3(grid), dim3(256), 0, stream, P);
+    return hipGetLastError();
+}
+template <int MODE>
+hipError_t pt_launch_fields_eval_mode(const ptd::Params &P, int n, const double *pts, const double *t, double *B,
+                                      double *dB, double *V, double *E, hipStream_t stream) {
+    hipLaunchKernelGGL(ptd::fields_eval_kernel<MODE>, dim3((n + 255) / 256), dim3(256), 0, stream, P, n, pts, t, B, dB,
+                       V, E);
+    return hipGetLastError();
+}
+/* PT_KERNEL_SET 0: the set-up and step kernels of the mode, 2: its persistent step kernel,
+   1: its collision kernel and field queries (three translation units per mode) */
+#if PT_KERNEL_SET == 0
+template hipError_t pt_launch_init_mode<PT_MODE_TU>(const ptd::Params &, int, hipStream_t);
+template hipError_t pt_launch_step_mode<PT_MODE_TU>(const ptd::Params &, int, int, hipStream_t);
+#elif PT_KERNEL_SET == 2
+template hipError_t pt_launch_step_persistent_mode<PT_MODE_TU>(const ptd::Params &, int, hipStream_t);
+#else
+template hipError_t pt_launch_collision_mode<PT_MODE_TU>(const ptd::Params &, int, hipStream_t);
+#if (PT_MODE_TU & 1) == 0 /* fields_eval runs on FP64 tables only */
+template hipError_t pt_launch_fields_eval_mode<PT_MODE_TU>(const ptd::Params &, int, const double *, const double *,
+                                                           double *, double *, double *, double *, hipStream_t);
+#endif
+#endif
+#endif /* PT_MODE_TU >= 0 */
+
+#if PT_COMMON_TU
+/* dispatch on the mode to the translation unit that holds its kernels */
+static hipError_t init_dispatch(const ptd::Params &P, int mode, int grid, hipStream_t s) {
+    switch (mode) {
+    case 0: return pt_launch_init_mode<0>(P, grid, s);
+    case 1: return pt_launch_init_mode<1>(P, grid, s);
+    case 2: return pt_launch_init_mode<2>(P, grid, s);
+    default: return pt_launch_init_mode<3>(P, grid, s);
+    }
+}
+
 hipError_t pt_launch_init(const ptd::Params &P, int mode, int steps, hipStream_t stream) {
     int grid = (P.st.n + 255) / 256;
     if (grid == 0) return hipSuccess;
-    switch (mode) {
-    case 0: hipLaunchKernelGGL(ptd::init_kernel<0>, dim3(grid), dim3(256), 0, stream, P); break;
-    case 1: hipLaunchKernelGGL(ptd::init_kernel<1>, dim3(grid), dim3(256), 0, stream, P); break;
-    case 2: hipLaunchKernelGGL(ptd::init_kernel<2>, dim3(grid), dim3(256), 0, stream, P); break;
-    default: hipLaunchKernelGGL(ptd::init_kernel<3>, dim3(grid), dim3(256), 0, stream, P); break;
-    }
+    hipError_t e = init_dispatch(P, mode, grid, stream);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(ptd::budget_kernel, dim3(grid), dim3(256), 0, stream, P.st.budget, P.st.n, steps);
     return hipGetLastError();
 }
@@ -2464,12 +2559,11 @@ hipError_t pt_launch_step(const ptd::Params &P, int mode, int block, hipStream_t
     int grid = (P.st.n + block - 1) / block;
     if (grid == 0) return hipSuccess;
     switch (mode) {
-    case 0: hipLaunchKernelGGL(ptd::step_kernel<0>, dim3(grid), dim3(block), 0, stream, P); break;
-    case 1: hipLaunchKernelGGL(ptd::step_kernel<1>, dim3(grid), dim3(block), 0, stream, P); break;
-    case 2: hipLaunchKernelGGL(ptd::step_kernel<2>, dim3(grid), dim3(block), 0, stream, P); break;
-    default: hipLaunchKernelGGL(ptd::step_kernel<3>, dim3(grid), dim3(block), 0, stream, P); break;
+    case 0: return pt_launch_step_mode<0>(P, grid, block, stream);
+    case 1: return pt_launch_step_mode<1>(P, grid, block, stream);
+    case 2: return pt_launch_step_mode<2>(P, grid, block, stream);
+    default: return pt_launch_step_mode<3>(P, grid, block, stream);
     }
-    return hipGetLastError();
 }
 
 hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, int spatial, hipStream_t stream) {
@@ -2486,22 +2580,21 @@ hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, i
         hipLaunchKernelGGL(ptd::reset_work_kernel, dim3(1), dim3(1), 0, stream, P.st.wcount);
         hipLaunchKernelGGL(ptd::work_kernel, dim3(wgrid), dim3(256), 0, stream, P);
     }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     switch (mode) {
-    case 0: hipLaunchKernelGGL(ptd::step_kernel_persistent<0>, dim3(grid), dim3(256), 0, stream, P); break;
-    case 1: hipLaunchKernelGGL(ptd::step_kernel_persistent<1>, dim3(grid), dim3(256), 0, stream, P); break;
-    case 2: hipLaunchKernelGGL(ptd::step_kernel_persistent<2>, dim3(grid), dim3(256), 0, stream, P); break;
-    default: hipLaunchKernelGGL(ptd::step_kernel_persistent<3>, dim3(grid), dim3(256), 0, stream, P); break;
+    case 0: return pt_launch_step_persistent_mode<0>(P, grid, stream);
+    case 1: return pt_launch_step_persistent_mode<1>(P, grid, stream);
+    case 2: return pt_launch_step_persistent_mode<2>(P, grid, stream);
+    default: return pt_launch_step_persistent_mode<3>(P, grid, stream);
     }
-    return hipGetLastError();
 }
 
 hipError_t pt_launch_fields_eval(const ptd::Params &P, int general, int n, const double *pts, const double *t, double *B,
                                  double *dB, double *V, double *E, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    dim3 g((n + 255) / 256), b(256);
-    if (general) hipLaunchKernelGGL(ptd::fields_eval_kernel<2>, g, b, 0, stream, P, n, pts, t, B, dB, V, E);
-    else hipLaunchKernelGGL(ptd::fields_eval_kernel<0>, g, b, 0, stream, P, n, pts, t, B, dB, V, E);
-    return hipGetLastError();
+    if (general) return pt_launch_fields_eval_mode<2>(P, n, pts, t, B, dB, V, E, stream);
+    return pt_launch_fields_eval_mode<0>(P, n, pts, t, B, dB, V, E, stream);
 }
 
 hipError_t pt_launch_solid_at(const ptd::Geom &G, int n, const double *pts, const double *t, int *out, hipStream_t stream) {
@@ -2519,12 +2612,15 @@ hipError_t pt_launch_decay(const ptd::Params &P, pt_beta_table T, pt_secondary *
 
 hipError_t pt_launch_collision(const ptd::Params &P, int mode, int coll_grid, hipStream_t stream) {
     if (P.st.n == 0) return hipSuccess;
+    hipError_t e;
     switch (mode) {
-    case 0: hipLaunchKernelGGL(ptd::collision_kernel<0>, dim3(coll_grid), dim3(256), 0, stream, P); break;
-    case 1: hipLaunchKernelGGL(ptd::collision_kernel<1>, dim3(coll_grid), dim3(256), 0, stream, P); break;
-    case 2: hipLaunchKernelGGL(ptd::collision_kernel<2>, dim3(coll_grid), dim3(256), 0, stream, P); break;
-    default: hipLaunchKernelGGL(ptd::collision_kernel<3>, dim3(coll_grid), dim3(256), 0, stream, P); break;
+    case 0: e = pt_launch_collision_mode<0>(P, coll_grid, stream); break;
+    case 1: e = pt_launch_collision_mode<1>(P, coll_grid, stream); break;
+    case 2: e = pt_launch_collision_mode<2>(P, coll_grid, stream); break;
+    default: e = pt_launch_collision_mode<3>(P, coll_grid, stream); break;
     }
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(ptd::reset_queue_kernel, dim3(1), dim3(1), 0, stream, P.st.qcount);
     return hipGetLastError();
 }
+#endif /* PT_COMMON_TU */

@@ -114,13 +114,21 @@ def snapshot_values(kind, t0, y0, spin0, snaps, counts, numbers, geometry, field
     return v
 
 
+# log files this process has opened: TTextLogger::DoLog (src/logger.cpp:316-329) truncates
+# a log file when it first opens it in a run and appends to it afterwards
+_opened = set()
+
+
 def write_log(outdir, jobnumber, particle, suffix, titles, values):
-    """Append lines to `<outdir>/%012d<particle><suffix>.out`, writing the header first."""
+    """Write lines to `<outdir>/%012d<particle><suffix>.out`: the first write of this run
+    truncates the file and writes the header, later writes append."""
     os.makedirs(outdir, exist_ok=True)
     path = os.path.join(outdir, "%012d%s%s.out" % (jobnumber, particle, suffix))
-    new = not os.path.exists(path)
+    key = os.path.abspath(path)
+    new = key not in _opened
+    _opened.add(key)
     cols = [np.asarray(values[t], dtype=np.float64) for t in titles]
-    with open(path, "a") as f:
+    with open(path, "w" if new else "a") as f:
         if new:
             f.write(" ".join(titles) + " \n")
         for row in zip(*cols):

@@ -5,6 +5,7 @@ differences of the first derivatives is covered through the state gradient."""
 import numpy as np
 import pytest
 
+from oracle.ptref import AdjModel as oracle_model  # the RK4 model on the CPU (oracle/ptadj.c)
 from pentrack_amd import workloads
 
 
@@ -29,7 +30,7 @@ def dJ(yT, m_kg):
 def test_adjoint_matches_central_differences(case):
     model, y0, s0 = case
     nsteps = 600          # 0.12 s: several wall reflections for the fast particles
-    M = model.oracle_model(s0)
+    M = oracle_model(model, s0)
     yT = M.forward(y0, nsteps)
     lam, sbar = M.adjoint(y0, dJ(yT, model.mass), nsteps)
     gs = sbar.sum(0)
@@ -38,8 +39,8 @@ def test_adjoint_matches_central_differences(case):
         sp, sm = s0.copy(), s0.copy()
         sp[k] += h
         sm[k] -= h
-        fd = (objective(model.oracle_model(sp).forward(y0, nsteps), model.mass) -
-              objective(model.oracle_model(sm).forward(y0, nsteps), model.mass)) / (2 * h)
+        fd = (objective(oracle_model(model, sp).forward(y0, nsteps), model.mass) -
+              objective(oracle_model(model, sm).forward(y0, nsteps), model.mass)) / (2 * h)
         assert abs(gs[k] - fd) <= 1e-6 * abs(fd) + 1e-12, (k, gs[k], fd)
     for i in range(len(y0)):
         for q in range(6):
@@ -56,7 +57,7 @@ def test_reflection_map_is_exact_for_free_flight():
     wall map applied after each step reproduces the fold exactly."""
     model, y0, s0 = workloads.adjoint_cube(n=4, seed=9, n_grid=8, nmaps=1, dt=1e-3)
     model.grav = 0.0
-    M = model.oracle_model([0.0])
+    M = oracle_model(model, [0.0])
     nsteps = 700
     yT = M.forward(y0, nsteps)
     T = nsteps * model.dt

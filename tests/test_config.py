@@ -195,6 +195,20 @@ def test_hit_log_rows_and_format(tmp_path):
     assert lines[0] == " ".join(DEFAULT_HITLOGVARS) + " " and lines[1].startswith("4 7 0.1 0 0 1 ")
 
 
+def test_log_truncated_on_first_write_of_a_run(tmp_path):
+    """TTextLogger::DoLog (src/logger.cpp:316-329) truncates a log when it first opens it in a
+    run and appends afterwards: a second write in the same run adds rows after one header, a
+    new run (the process's set of opened logs cleared) starts the file over."""
+    from pentrack_amd import logger
+    v = {"a": np.array([1.0, 2.0]), "b": np.array([3.0, 4.0])}
+    p = logger.write_log(str(tmp_path), 1, "neutron", "end", ["a", "b"], v)
+    logger.write_log(str(tmp_path), 1, "neutron", "end", ["a", "b"], v)
+    assert open(p).read().split("\n")[:-1] == ["a b ", "1 3 ", "2 4 ", "1 3 ", "2 4 "]
+    logger._opened.clear()
+    logger.write_log(str(tmp_path), 1, "neutron", "end", ["a", "b"], {"a": np.array([5.0]), "b": np.array([6.0])})
+    assert open(p).read().split("\n")[:-1] == ["a b ", "5 6 "]
+
+
 def test_rotate_vectors_match_oracle(oracle_lib):
     """RotateVector without a reference direction (src/globals.cpp:40-64), as the surface
     sources use it: CGAL's base1 cases (zero components) and general normals, bit for bit."""

@@ -4,6 +4,7 @@ and the adjoint gradient against central finite differences of the GPU forward."
 import numpy as np
 import pytest
 
+from oracle.ptref import AdjModel as oracle_model  # the RK4 model on the CPU (oracle/ptadj.c)
 from pentrack_amd import workloads
 from pentrack_amd.adjoint import rk4_adjoint, rk4_forward, rk4_tracker
 
@@ -22,7 +23,7 @@ def test_forward_and_adjoint_match_cpu(case, gpu):
     print("case: adjoint n=500 maps=4 grid=16 steps=600 K=64", flush=True)
     nsteps, K = 600, 64
     yT, ck = rk4_forward(model, s0, torch.tensor(y0, device=gpu), nsteps, K)
-    M = model.oracle_model(s0)
+    M = oracle_model(model, s0)
     yc = M.forward(y0, nsteps)
     assert np.allclose(yT.cpu().numpy(), yc, rtol=1e-12, atol=1e-12)
     lamT = np.zeros_like(yc)

@@ -10,7 +10,10 @@ from pentrack_amd.tracker import Batch, LaunchConfig
 
 pytestmark = pytest.mark.gpu
 
-TIMES = [2.5, 0.1, 0.5, 1.0, 1.0 + 1e-9, 0.5]      # unsorted, a duplicate, two in one step
+# unsorted, a duplicate, and two in one step with the later time first: find_if over the
+# configured order (src/logger.cpp:134) records 1.0 + 1e-9 there, not 1.0
+TIMES = [2.5, 0.1, 0.5, 1.0 + 1e-9, 1.0, 0.5]
+SORTED_TIMES = [0.1, 0.5, 1.0, 1.0 + 1e-9, 2.5]
 
 
 def run_snapshots(w, times, cap, cfg=None):
@@ -33,10 +36,19 @@ def test_snapshots_match_oracle(gpu, oracle_lib, persistent):
         a, b = rec[k, :cnt[k]], orec[k, :ocnt[k]]
         for f in ("t", "y", "spin", "Hmax", "noflipprob", "stopID", "Nhit", "Nstep", "Nspinflip"):
             assert np.array_equal(a[f], b[f]), (k, f)
-    # at most one record per step: 1.0 and 1.0 + 1e-9 fall into one step of every particle
-    # (slots past a particle's count hold no record)
+    # at most one record per step: 1.0 and 1.0 + 1e-9 fall into one step of almost every
+    # particle, where the configured order picks 1.0 + 1e-9 (slots past a particle's count
+    # hold no record)
     t = np.concatenate([rec["t"][k, :min(cnt[k], 4)] for k in range(w.n)])
     assert (np.isin(t, [0.1, 0.5, 1.0, 1.0 + 1e-9, 2.5])).all()
+    assert (t == 1.0 + 1e-9).sum() > 10 * (t == 1.0).sum()
+    # a sorted list is searched by bisection and gives find_if's answer for that order
+    out2, rec2, cnt2 = run_snapshots(w, SORTED_TIMES, 4, LaunchConfig(persistent=persistent))
+    o2, orec2, ocnt2 = oracle_lib.track_snapshots(w, SORTED_TIMES, 4, threads=8)
+    assert np.array_equal(cnt2, ocnt2)
+    for k in range(w.n):
+        m = min(cnt2[k], 4)
+        assert np.array_equal(rec2["t"][k, :m], orec2["t"][k, :m]) and np.array_equal(rec2["y"][k, :m], orec2["y"][k, :m])
 
 
 def test_snapshots_on_a_parabola(gpu):

@@ -130,3 +130,30 @@ def test_spin_log_larmor(oracle_lib):
         for b in iv * np.arange(1, int(T / iv) + 1):
             assert (r["t"] - b)[r["t"] >= b].min() < 3e-3, b
         assert cnt[k] <= int(T / iv) + o["Nstep"][k] + 1
+
+
+def test_spin_log_start_row_uses_fresh_stepper_time(oracle_lib):
+    """PrintSpin right after the spin stepper's initialize (src/tracking.cpp:430-431) sees
+    the fresh stepper's previous time 0.0 (odeint value-initialises it), so the start row of
+    an integration that starts in (0, spinloginterval) is not written.  Particles starting at
+    t0 = 3 ms with an interval longer than the flight: no row at all (every integration
+    starts in (0, interval) and no step crosses a multiple); from t0 = 0 only the row at 0."""
+    B0, T = 1e-6, 0.05
+    for t0, expect in ((3e-3, 0), (0.0, 1)):
+        w = larmor_workload(B0, (0.0, 1.0), T)
+        w.t0[:] = t0
+        o, rec, cnt = oracle_lib.track_spinlog(w, 1.0, 64)
+        assert (o["stopID"] == -1).all() and (o["Nstep"] > 3).all()
+        assert (cnt == expect).all(), (t0, cnt)
+        if expect:
+            assert (rec["t"][:, 0] == 0).all()
+    # interval 5 ms from t0 = 3 ms: start rows of integrations starting before 5 ms are
+    # skipped, later ones are written; so no row lies in (3 ms, 5 ms) except spin-step ends
+    # that crossed no multiple, which are never written either
+    w = larmor_workload(B0, (0.0, 1.0), T)
+    w.t0[:] = 3e-3
+    o, rec, cnt = oracle_lib.track_spinlog(w, 5e-3, 256)
+    for k in range(w.n):
+        r = rec[k, :cnt[k]]
+        assert not ((r["t"] > 3e-3) & (r["t"] < 5e-3)).any()
+        assert r["t"].min() >= 5e-3
