@@ -391,6 +391,20 @@ int pt_rk4_forward_ckpt(void *stream, const pt_adj_params *params, int n, const 
    particle ([nmaps][n]); seg ([K][24][n] doubles) and flags ([K][n] ints) are workspace. */
 int pt_rk4_adjoint(void *stream, const pt_adj_params *params, int n, const double *ckpt, double *lam,
                    double *sbar, double *seg, int *flags, int nsteps, int K);
+/*
+ * No counterpart in the reference.  pt_rk4_adjoint for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same kernel, with its workspace (seg and flags above) in a
+ * device array of PT_ADJ_NOALLOC_MAX_BYTES that the library holds on every GPU it runs
+ * on until the process ends.  The call allocates nothing; the caller still passes ckpt,
+ * lam and sbar.  Returns PT_EINVAL, with a message in pt_last_error(), when
+ * min(K, nsteps) * n * 196 bytes (plus alignment) exceed PT_ADJ_NOALLOC_MAX_BYTES.
+ * Calls on one GPU share that array: calls that overlap in time on one GPU (different
+ * streams or host threads) race and must be serialised by the caller.
+ */
+#define PT_ADJ_NOALLOC_MAX_BYTES (64u << 20)
+int pt_rk4_adjoint_noalloc(void *stream, const pt_adj_params *params, int n, const double *ckpt, double *lam,
+                           double *sbar, int nsteps, int K);
 
 /* Cancellation flag mirroring the reference's global `quit` (include/globals.h:33):
    pt_batch_run copies it to the device when it is called, and its step kernels

@@ -144,6 +144,7 @@ EXPORTS = [
     "pt_distribution_table", "pt_fields_eval", "pt_geometry_solid_at", "pt_bicubic_precompute",
     "pt_tricubic_precompute_gpu", "pt_batch_set_snapshots", "pt_batch_snapshots",
     "pt_batch_set_track", "pt_batch_track", "pt_batch_set_spinlog", "pt_batch_spinlog",
+    "pt_rk4_adjoint_noalloc",
 ]
 
 _lib = None

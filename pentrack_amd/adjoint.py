@@ -103,6 +103,7 @@ def _lib():
     L = _native.lib()
     vp = C.c_void_p
     L.pt_rk4_forward_ckpt.argtypes = [vp, C.POINTER(AdjParams), C.c_int, vp, vp, vp, C.c_int, C.c_int]
+    L.pt_rk4_adjoint_noalloc.argtypes = [vp, C.POINTER(AdjParams), C.c_int, vp, vp, vp, C.c_int, C.c_int]
     L.pt_rk4_adjoint.argtypes = [vp, C.POINTER(AdjParams), C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_int]
     return L
 

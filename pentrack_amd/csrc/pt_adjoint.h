@@ -9,6 +9,9 @@
 
 #define PTA_MAX_MAPS PT_ADJ_MAX_MAPS
 
+/* records msg as the calling thread's pt_last_error() and returns code (pt_host.cpp) */
+int pt_fail(int code, const char *msg);
+
 namespace pta {
 typedef pt_adj_maps MapSet;
 typedef pt_adj_params AdjParams;

@@ -18,6 +18,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <string>
+
 #include "pt_adjoint.h"
 
 namespace pta {
@@ -383,4 +386,70 @@ extern "C" int pt_rk4_adjoint(void *stream, const pt_adj_params *params, int n, 
     hipLaunchKernelGGL(pta::adjoint_kernel, dim3((n + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream),
                        *params, ckpt, lam, sbar, seg, flags, n, nsteps, K);
     return hipGetLastError() == hipSuccess ? PT_OK : PT_EDEVICE;
+}
+
+/* ---- the adjoint without allocation ---------------------------------------- */
+/*
+ * pt_rk4_adjoint_noalloc keeps the workspace pt_rk4_adjoint takes from its caller (the
+ * stage data of one checkpoint segment, [rows][24][n] doubles, and the wall flags,
+ * [rows][n] ints, rows = min(K, nsteps)) in this file-scope device array instead.
+ * Every GPU the library runs on holds its own copy until the process ends.  In host code
+ * the name `scratch` is not a device address: the address of a device's copy comes from
+ * hipGetSymbolAddress with that device current, once per device, and only that address
+ * is handed to kernels.  The adjoint kernel writes every workspace entry of a segment
+ * (forward recomputation) before it reads it (reverse sweep), so nothing needs clearing.
+ */
+static __device__ __align__(256) unsigned char scratch[PT_ADJ_NOALLOC_MAX_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+/* device address of `scratch` on device dev (set-up on the first call for that device) */
+int scratch_address(int dev, void **out) {
+    if (dev < 0 || dev >= MAX_DEVICES) return pt_fail(PT_EDEVICE, "pt_rk4_adjoint_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != dev) e = hipSetDevice(dev);
+        void *p = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur >= 0 && cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return pt_fail(PT_EDEVICE, (std::string("pt_rk4_adjoint_noalloc: hipGetSymbolAddress: ") +
+                                        hipGetErrorString(e)).c_str());
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PT_OK;
+}
+} // namespace
+
+extern "C" int pt_rk4_adjoint_noalloc(void *stream, const pt_adj_params *params, int n, const double *ckpt,
+                                      double *lam, double *sbar, int nsteps, int K) {
+    if (!params || n < 0 || nsteps < 0 || K < 1 || (n > 0 && (!ckpt || !lam || !sbar))) return PT_EINVAL;
+    if (params->maps.nmaps < 0 || params->maps.nmaps > PTA_MAX_MAPS) return PT_EINVAL;
+    if (n == 0) return PT_OK;
+    const size_t rows = (size_t)(nsteps < K ? nsteps : K);
+    const size_t seg_bytes = rows * 24 * (size_t)n * sizeof(double);
+    const size_t flag_off = (seg_bytes + 255) / 256 * 256;
+    const size_t need = flag_off + rows * (size_t)n * sizeof(int);
+    if (need > (size_t)PT_ADJ_NOALLOC_MAX_BYTES)
+        return pt_fail(PT_EINVAL, ("pt_rk4_adjoint_noalloc: workspace of " + std::to_string(need) +
+                                   " bytes (min(K, nsteps) x n x 196) exceeds PT_ADJ_NOALLOC_MAX_BYTES = " +
+                                   std::to_string((size_t)PT_ADJ_NOALLOC_MAX_BYTES)).c_str());
+    int dev = -1;
+    hipError_t e = stream ? hipStreamGetDevice(static_cast<hipStream_t>(stream), &dev) : hipGetDevice(&dev);
+    if (e != hipSuccess) return pt_fail(PT_EDEVICE, hipGetErrorString(e));
+    void *base = nullptr;
+    int rc = scratch_address(dev, &base);
+    if (rc != PT_OK) return rc;
+    double *seg = static_cast<double *>(base);
+    int *flags = reinterpret_cast<int *>(static_cast<unsigned char *>(base) + flag_off);
+    hipLaunchKernelGGL(pta::adjoint_kernel, dim3((n + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       *params, ckpt, lam, sbar, seg, flags, n, nsteps, K);
+    e = hipGetLastError();
+    return e == hipSuccess ? PT_OK : pt_fail(PT_EDEVICE, hipGetErrorString(e));
 }

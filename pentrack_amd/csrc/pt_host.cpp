@@ -1658,3 +1658,6 @@ extern "C" int pt_track_batch(void *stream, const pt_geometry *g, const pt_field
 extern "C" void pt_request_quit(int on) { g_quit.store(on ? 1 : 0); }
 
 extern "C" const char *pt_last_error(void) { return g_last_error.c_str(); }
+
+/* for the library's other translation units: record msg as this thread's last error */
+int pt_fail(int code, const char *msg) { return fail(code, msg); }
