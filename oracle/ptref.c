@@ -43,6 +43,21 @@
  * Random numbers: one stream per particle (pt_detmath.h), drawn at the same
  * points and through the same libstdc++ transforms as the reference draws from
  * its global std::mt19937_64.
+ *
+ * What pins each part (tests/), besides agreement with the GPU:
+ *   stepper          tableau by its order conditions in exact rationals; the error
+ *                    norm and every controller branch against the exact error polynomial
+ *                    on dx/dt = lambda x; quadrature exactness, fifth-order convergence
+ *                    (Kepler), fourth-order error estimate and dense output
+ *                    (test_oracle_stepper.py, through ptref_probe_steps)
+ *   chord splitting  piece counts replayed on the closed-form parabola (test_oracle_chords.py)
+ *   bisection        bracket within 1e-8 m before the closed-form wall crossing, first
+ *                    half searched first (test_oracle_chords.py)
+ *   equation of motion, fields, collisions, wall physics: closed forms, exact rational
+ *                    predicates, HitTest known answers and the reference's own ALGLIB,
+ *                    libtricubic and field classes compiled into oracle/_ref
+ *   parity unpinned: odeint's exact step-size bits beyond the published controller, CGAL
+ *                    traversal order for ties between equal-s hits, x87 and glibc last bits
  */
 #include <math.h>
 #include <complex.h>
@@ -159,6 +174,7 @@ typedef struct {
     double k3[9], k4[9], k5[9], k6[9];
     double t, t_old, dt;
     int deriv_init;
+    double last_err;             /* error norm of the last trial step */
 } dopri_t;
 
 typedef struct {
@@ -175,6 +191,8 @@ typedef struct {
     ptref_hit *hits; int hit_cap; int n_hits;
     const struct spin_splines_s *spl;   /* BMT with interpolatefields: this step's axis splines */
     double *spin_pos;                   /* spin_axis_at also stores the position here when set */
+    int probe_kind;                     /* > 0: derivs evaluates a test ODE (ptref_probe_steps) */
+    const double *probe_par;
 } track_t;
 
 static double rnd_uniform(track_t *p, double a, double b) {
@@ -1218,9 +1236,29 @@ static void eom(const species_t *sp, const double *y, const double B[3], const d
 }
 
 /* TParticle::derivs, src/particle.cpp:96-103 */
+/* test ODEs of the stepper probe (ptref_probe_steps): 1 linear dx_i/dt = lam_i x_i (par[i]);
+   2 polynomial in t, dx_i/dt = sum_k par[5 i + k] t^k; 3 the Kepler problem in the plane,
+   (x, y, vx, vy)' = (vx, vy, -x/r^3, -y/r^3), other components constant */
+static void probe_rhs(int kind, const double *par, const double *y, double *dydx, double t) {
+    for (int i = 0; i < 9; ++i) dydx[i] = 0;
+    if (kind == 1) {
+        for (int i = 0; i < 9; ++i) dydx[i] = par[i] * y[i];
+    } else if (kind == 2) {
+        for (int i = 0; i < 9; ++i) {
+            double p = 0;
+            for (int k = 4; k >= 0; --k) p = p * t + par[5 * i + k];
+            dydx[i] = p;
+        }
+    } else {
+        double r2 = y[0] * y[0] + y[1] * y[1], r3 = r2 * sqrt(r2);
+        dydx[0] = y[2]; dydx[1] = y[3]; dydx[2] = -y[0] / r3; dydx[3] = -y[1] / r3;
+    }
+}
+
 static void derivs(track_t *tp, const double *y, double *dydx, double t) {
     double B[3] = {0, 0, 0}, dB[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, E[3] = {0, 0, 0}, V = 0;
     tp->n_rhs++;
+    if (tp->probe_kind > 0) { probe_rhs(tp->probe_kind, tp->probe_par, y, dydx, t); return; }
     if (tp->sp.q != 0 || (tp->sp.mu != 0 && y[7] != 0)) field_B(tp, y[0], y[1], y[2], t, B, dB);
     if (tp->sp.q != 0) field_E(tp, y[0], y[1], y[2], t, &V, E);
     eom(&tp->sp, y, B, dB, E, dydx);
@@ -1301,6 +1339,7 @@ static int dopri_try(track_t *tp, dopri_t *s, double xnew[9], double dxnew[9]) {
         double e = fabs(xerr[i]) / (1e-9 + 1e-9 * (1.0 * fabs(x[i]) + dt * fabs(k1[i])));
         err = (err < e) ? e : err;
     }
+    s->last_err = err;
     if (err > 1.0) {
         /* default_step_adjuster::decrease_step, error_order 4 */
         double f = 9.0 / 10 * R_POW(err, -1.0 / 3);
@@ -2570,6 +2609,60 @@ int ptref_dense_probe(int kind, double t0, const double *y0, double dt0, int n_t
         out_t[k] = t;
         dopri_calc_state(&tp, &st, t, out_y + 9 * k);
     }
+    return 0;
+}
+
+/*
+ * The oracle's dopri5 (dopri_try: stages, error norm, controller; dopri_do_step; dense
+ * output) on a test ODE (probe_rhs kinds 1-3), for tests.  Adaptive (fixed = 0): n_steps
+ * accepted steps from (t0, x0) with first step dt0; per accepted step k: t_out[k],
+ * x_out[9k..], the proposed next step dt_out[k], the error norm of the accepted trial
+ * err_out[k] and the rejected trials before it rej_out[k].  Fixed (fixed = 1): n_steps
+ * steps of exactly dt0, each trial taken whatever its error (err_out: its error norm,
+ * dt_out: the step the controller proposed).  n_theta dense-output points of the last
+ * step at t_old + theta (t - t_old) into dense_out [n_theta][9].
+ */
+int ptref_probe_steps(int kind, const double *par, const double *x0, double t0, double dt0, int n_steps, int fixed,
+                      double *t_out, double *x_out, double *dt_out, double *err_out, int *rej_out,
+                      int n_theta, const double *theta, double *dense_out) {
+    if (kind < 1 || kind > 3 || n_steps < 1) return -1;
+    ctx_t c;
+    track_t tp;
+    dopri_t st;
+    memset(&c, 0, sizeof(c));
+    memset(&tp, 0, sizeof(tp));
+    memset(&st, 0, sizeof(st));
+    tp.c = &c;
+    tp.probe_kind = kind;
+    tp.probe_par = par;
+    dopri_initialize(&st, x0, t0, dt0);
+    for (int k = 0; k < n_steps; ++k) {
+        int rej = 0;
+        if (fixed) {
+            if (!st.deriv_init) { derivs(&tp, st.xc, st.dxc, st.t); st.deriv_init = 1; }
+            double xnew[9], dxnew[9];
+            st.dt = dt0;
+            const double t = st.t;
+            (void)dopri_try(&tp, &st, xnew, dxnew);
+            dt_out[k] = st.dt;
+            memcpy(st.xo, st.xc, sizeof(st.xo));
+            memcpy(st.dxo, st.dxc, sizeof(st.dxo));
+            memcpy(st.xc, xnew, sizeof(st.xc));
+            memcpy(st.dxc, dxnew, sizeof(st.dxc));
+            st.t_old = t;
+            st.t = t + dt0;
+        } else {
+            long long r0 = tp.n_rej;
+            if (dopri_do_step(&tp, &st) < 0) return -2;
+            rej = (int)(tp.n_rej - r0);
+            dt_out[k] = st.dt;
+        }
+        t_out[k] = st.t;
+        memcpy(x_out + 9 * k, st.xc, 9 * sizeof(double));
+        err_out[k] = st.last_err;
+        rej_out[k] = rej;
+    }
+    for (int q = 0; q < n_theta; ++q) dopri_calc_state(&tp, &st, st.t_old + (st.t - st.t_old) * theta[q], dense_out + 9 * q);
     return 0;
 }
 

@@ -116,6 +116,8 @@ def load(mode="fp64"):
     L.ptref_tricubic_eval.restype = C.c_double
     L.ptref_integrate_free.argtypes = [C.c_int, C.POINTER(Field), C.c_int, C.c_double, dp, C.c_int, dp, dp, dp]
     L.ptref_dense_probe.argtypes = [C.c_int, C.c_double, dp, C.c_double, C.c_int, dp, dp, dp]
+    L.ptref_probe_steps.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, dp, dp,
+                                    C.POINTER(C.c_int), C.c_int, dp, dp]
     L.ptref_species_constants.argtypes = [C.c_int, dp]
     L.ptref_species_constants.restype = None
     L.ptref_detmath.argtypes = [C.c_int, C.c_int, dp, dp, dp]
@@ -476,3 +478,25 @@ class AdjModel:
         assert _adj_lib().ptadj_adjoint(C.byref(self.m), len(y0), y0.ctypes.data, lam.ctypes.data,
                                         sbar.ctypes.data, nsteps) == 0
         return lam, sbar
+
+
+PROBE_LINEAR, PROBE_POLY_T, PROBE_KEPLER = 1, 2, 3
+
+
+def probe_steps(kind, par, x0, t0, dt0, n_steps, fixed=False, theta=(), mode="fp64"):
+    """The oracle's dopri5 on a test ODE (ptref_probe_steps): dict of t, x [n][9], dt (the
+    proposed next step), err (error norm of the accepted or forced trial), rej (rejected
+    trials before each accepted step) and dense [len(theta)][9] of the last step."""
+    L = load(mode)
+    par = np.ascontiguousarray(np.asarray(par, dtype=np.float64).ravel())
+    x0 = np.ascontiguousarray(np.asarray(x0, dtype=np.float64).ravel())
+    assert x0.size == 9
+    th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64).ravel())
+    t = np.zeros(n_steps); x = np.zeros((n_steps, 9)); dt = np.zeros(n_steps); err = np.zeros(n_steps)
+    rej = np.zeros(n_steps, dtype=np.int32); dense = np.zeros((max(len(th), 1), 9))
+    rc = L.ptref_probe_steps(kind, _dp(par), _dp(x0), float(t0), float(dt0), int(n_steps), int(bool(fixed)),
+                             _dp(t), _dp(x), _dp(dt), _dp(err), rej.ctypes.data_as(C.POINTER(C.c_int)),
+                             len(th), _dp(th), _dp(dense))
+    if rc != 0:
+        raise RuntimeError(f"ptref_probe_steps failed ({rc})")
+    return dict(t=t, x=x, dt=dt, err=err, rej=rej, dense=dense[:len(th)])
