@@ -37,6 +37,8 @@ PEAK_HBM_GBS = 8000.0
 BYTES_PER_FIELD_EVAL = {"fp64": 3 * 64 * 8, "mixed": 3 * 64 * 4}
 VARIANTS = {
     "fp64": dict(mixed=False, persistent=0, graph=0),
+    "fp64_persistent": dict(mixed=False, persistent=1, graph=0),
+    "fp64_persistent_graph": dict(mixed=False, persistent=1, graph=1),
     "mixed": dict(mixed=True, persistent=0, graph=0),
     "mixed_persistent": dict(mixed=True, persistent=1, graph=0),
     "mixed_persistent_graph": dict(mixed=True, persistent=1, graph=1),
@@ -50,8 +52,10 @@ HEADLINE_ORDER = ["mixed_persistent_spatial_graph", "mixed_persistent_spatial", 
 PROFILED_FOR = {"mixed_persistent_spatial_graph": ["mixed_persistent_spatial"],
                 "mixed_persistent_spatial": ["mixed_persistent_spatial"],
                 "mixed_persistent_graph": ["mixed_persistent"], "mixed_persistent": ["mixed_persistent"],
-                "mixed": ["mixed"], "fp64": ["fp64"]}
+                "mixed": ["mixed"], "fp64": ["fp64"], "fp64_persistent_graph": ["fp64_persistent"],
+                "fp64_persistent": ["fp64_persistent"]}
 KERNEL_NAMES = {"fp64": "step_kernel_fp64", "mixed": "step_kernel_mixed",
+                "fp64_persistent": "step_kernel_persistent_fp64",
                 "mixed_persistent": "step_kernel_persistent_mixed",
                 "mixed_persistent_spatial": "step_kernel_persistent_mixed_spatial"}
 
@@ -122,7 +126,7 @@ def mixed_error_leg(a, sptr):
     out = {}
     for name, vid in (("fp64", _native.PT_VARIANT_FP64), ("mixed", _native.PT_VARIANT_MIXED)):
         b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
-                  hit_cap=16, with_fp32=True)
+                  hit_cap=16, with_mixed=True)
         b.run_to_completion(LaunchConfig(variant=vid, steps_per_launch=256, rounds=8), max_launches=100000,
                             stream=sptr)
         out[name] = b.download(sptr)
@@ -248,7 +252,7 @@ def main():
         kw = VARIANTS[variant]
         vid = _native.PT_VARIANT_MIXED if kw["mixed"] else _native.PT_VARIANT_FP64
         b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
-                  with_fp32=kw["mixed"])
+                  with_mixed=kw["mixed"])
         cfg = LaunchConfig(variant=vid, block=a.block, steps_per_launch=a.rk, rounds=a.rounds,
                            persistent=kw["persistent"], graph=kw["graph"], spatial_order=kw.get("spatial", 0))
         for _ in range(a.warmup):

@@ -117,18 +117,31 @@ typedef struct {
     /* PT_FIELD_TRICUBIC with coef == NULL: the samples [ncomp][nx][ny][nz] (host memory),
        pre-interpolated on the GPU by pt_fields_create (bit-identical to pt_tricubic_precompute) */
     const double *values;
+    /* PT_FIELD_TRICUBIC, mixed-precision tables: the largest gradient error [field unit / m]
+       (value error: times the smallest cell size) that a cell component's FP32 remainder may
+       carry by its worst-case bound; components above it keep their FP64 coefficients
+       (0: PT_MIXED_TOL_DEFAULT) */
+    double mixed_tol;
 } pt_field_desc;
+
+#define PT_MIXED_TOL_DEFAULT 1e-10
 
 typedef struct pt_fields pt_fields;
 
 /*
  * Replaces TFieldManager::TFieldManager (src/fields.cpp:19-136): descriptors are
  * kept in the given order (the reference's lexicographic FIELDS key order) and
- * summed in that order.  with_fp32 also uploads an FP32 copy of every
- * coefficient table for the mixed-precision tracker.
+ * summed in that order.  with_mixed also builds, on the GPU, the mixed-precision
+ * table of every tricubic map for PT_VARIANT_MIXED: per cell and component the
+ * constant and linear coefficients in FP64 and the other 60 in FP32 (288 B instead of
+ * 512 B), or the FP64 block where the FP32 remainder's error bound exceeds the map's
+ * mixed_tol.  The FP64 tables are kept as well.
  */
-int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32, pt_fields **out);
+int pt_fields_create(int n, const pt_field_desc *descs, int with_mixed, pt_fields **out);
 void pt_fields_destroy(pt_fields *f);
+/* No counterpart in the reference: blocks (cells x components) of the mixed-precision
+   tables and how many of them kept their FP64 coefficients */
+int pt_fields_mixed_info(const pt_fields *f, int64_t *blocks, int64_t *fp64_blocks);
 
 /*
  * TabField3::PreInterpol (src/field_3d.cpp:367-414): spline derivatives along each
@@ -196,7 +209,8 @@ typedef struct {
 } pt_hit;
 
 #define PT_VARIANT_FP64 0        /* plain FP64, one thread per particle */
-#define PT_VARIANT_MIXED 1       /* FP32 field tables and interpolation, FP64 state and control */
+#define PT_VARIANT_MIXED 1       /* mixed-precision field tables (FP64 linear part, FP32 remainder
+                                    within mixed_tol), FP64 state, control, bisection, triangle test */
 
 typedef struct {
     int variant;

@@ -55,6 +55,7 @@ class FieldDesc(C.Structure):
         ("p", C.c_double * 8),
         ("bscale_formula", C.c_char_p), ("escale_formula", C.c_char_p),
         ("values", C.POINTER(C.c_double)),
+        ("mixed_tol", C.c_double),
     ]
 
 
@@ -144,7 +145,7 @@ EXPORTS = [
     "pt_distribution_table", "pt_fields_eval", "pt_geometry_solid_at", "pt_bicubic_precompute",
     "pt_tricubic_precompute_gpu", "pt_batch_set_snapshots", "pt_batch_snapshots",
     "pt_batch_set_track", "pt_batch_track", "pt_batch_set_spinlog", "pt_batch_spinlog",
-    "pt_rk4_adjoint_noalloc",
+    "pt_rk4_adjoint_noalloc", "pt_fields_mixed_info",
 ]
 
 _lib = None
@@ -173,6 +174,7 @@ def lib():
     L.pt_geometry_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.pt_fields_create.argtypes = [C.c_int, C.POINTER(FieldDesc), C.c_int, C.POINTER(vp)]
     L.pt_fields_destroy.argtypes = [vp]
+    L.pt_fields_mixed_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.pt_tricubic_precompute.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), C.c_int]

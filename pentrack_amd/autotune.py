@@ -60,7 +60,7 @@ def tune(w, n_sample=200_000, calls=3, blocks=(64, 128, 256), grids=(1, 2, 4), s
         for shape, k in itertools.product(shapes, steps):
             cfg = LaunchConfig(variant=variant, steps_per_launch=k, leaf_size=leaf, **shape)
             b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx[sel], w.t0[sel], w.y0[sel],
-                      w.spin0[sel], w.seed, w.pid[sel], with_fp32=(variant == 1), leaf_size=leaf)
+                      w.spin0[sel], w.seed, w.pid[sel], with_mixed=(variant == 1), leaf_size=leaf)
             for _ in range(calls):
                 b.run(cfg, s)
             torch.cuda.synchronize()

@@ -37,14 +37,19 @@ def _units():
     return units
 
 
-def build(verbose=False, jobs=None):
+def build(verbose=False, jobs=None, out=OUT, defines=()):
+    """Compile and link the library to `out`; `defines` are extra -D flags (launch-shape
+    experiments, e.g. PT_STEP_MIN_WAVES=1), built in their own object folder."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    os.makedirs(OBJ, exist_ok=True)
+    obj_dir = OBJ if not defines else OBJ + "_" + "_".join(d.replace("=", "") for d in defines)
+    os.makedirs(obj_dir, exist_ok=True)
+    extra_d = [f"-D{d}" for d in defines]
     units = _units()
 
     def compile_one(u):
         obj, src, extra = u
-        cmd = [hipcc] + FLAGS + extra + ["-c", "-x", "hip", os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
+        cmd = [hipcc] + FLAGS + extra_d + extra + ["-c", "-x", "hip", os.path.join(CSRC, src),
+                                                  "-o", os.path.join(obj_dir, obj)]
         t = time.perf_counter()
         r = subprocess.run(cmd, capture_output=True, text=True)
         if verbose:
@@ -59,16 +64,16 @@ def build(verbose=False, jobs=None):
         sys.stderr.write(f"--- {o}\n" + r.stdout + r.stderr)
     if failed:
         raise RuntimeError("hipcc failed building libpentrack_gpu.so: " + ", ".join(o for o, _ in failed))
-    link = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp"] + \
-        [os.path.join(OBJ, u[0]) for u in units]
+    link = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp"] + \
+        [os.path.join(obj_dir, u[0]) for u in units]
     r = subprocess.run(link, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("linking libpentrack_gpu.so failed")
-    os.replace(OUT + ".tmp", OUT)
+    os.replace(out + ".tmp", out)
     if verbose:
-        print("built", OUT)
-    return OUT
+        print("built", out)
+    return out
 
 
 if __name__ == "__main__":

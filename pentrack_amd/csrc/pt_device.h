@@ -90,8 +90,14 @@ struct Field {
     double x0[3], inv_h[3];   /* first grid coordinate and 1/spacing guess for the cell search */
     const double *g[3];       /* grid coordinates */
     const double *coef;       /* FP64 table */
-    const float *coef32;      /* FP32 table (mixed precision) */
+    const unsigned char *mix; /* mixed-precision table: MIX_BLOCK_BYTES per cell and component */
 };
+
+/* A block of the mixed-precision table: the constant and linear coefficients in FP64
+   (device order: slots 0 = 1, 1 = z, 4 = y, 16 = x) and the 64 coefficients with those four
+   slots zeroed in FP32; a component whose FP32 remainder may err by more than the map's
+   tolerance keeps its FP64 block instead, marked by a NaN constant term. */
+constexpr int MIX_BLOCK_BYTES = 4 * 8 + 64 * 4;
 
 struct Species {
     int kind;

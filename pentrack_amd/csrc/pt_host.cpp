@@ -436,14 +436,15 @@ struct pt_fields {
     std::vector<ptd::Field> host;
     ptd::Field *dev = nullptr;
     int n = 0;
-    bool has_fp32 = false;
+    bool has_fp32 = false;  /* mixed-precision tables built */
+    long long mix_blocks = 0, mix_fp64 = 0;   /* blocks of the mixed tables, of which kept in FP64 */
     bool general = false;   /* a scale formula, analytic field or conductor is present */
     bool b_reuse = true;    /* no 2D map tabulates B (value-only B equals B with derivatives) */
 };
 
 namespace {
 int tricubic_on_device(int nx, int ny, int nz, const double *const gh[3], int ncomp, const double *values_h,
-                       int device_order, double *coef_dev, float *coef32_dev);
+                       int device_order, double *coef_dev);
 }
 
 extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32, pt_fields **out) {
@@ -517,39 +518,39 @@ extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32
             f.has_v = d.has_v ? 1 : 0;
             f.ncomp = 3 * f.has_b + f.has_v;
             size_t ncoef = (size_t)(d.nx - 1) * (d.ny - 1) * (d.nz - 1) * f.ncomp * 64;
+            double *dc;
+            int r;
             if (!d.coef) {
                 /* samples only: pre-interpolate on the device, straight into the tracker's order */
-                double *dc;
-                float *dc32 = nullptr;
-                int r = F->mem.alloc(&dc, ncoef);
-                if (r) return r;
-                if (with_fp32 && (r = F->mem.alloc(&dc32, ncoef))) return r;
-                r = tricubic_on_device(d.nx, d.ny, d.nz, gs, f.ncomp, d.values, 1, dc, dc32);
-                if (r) return r;
-                f.coef = dc;
-                f.coef32 = dc32;
-                F->general = F->general || f.blen > 0 || f.elen > 0;
-                F->host[k] = f;
-                continue;
+                if ((r = F->mem.alloc(&dc, ncoef))) return r;
+                if ((r = tricubic_on_device(d.nx, d.ny, d.nz, gs, f.ncomp, d.values, 1, dc))) return r;
+            } else {
+                /* libtricubic order a[i + 4j + 16k] (coefficient of x^i y^j z^k) becomes
+                   a[16i + 4j + k]: the 16 coefficients of one power of x are contiguous */
+                std::vector<double> cdev(ncoef);
+                for (size_t b = 0; b < ncoef; b += 64)
+                    for (int i = 0; i < 4; ++i)
+                        for (int j = 0; j < 4; ++j)
+                            for (int k = 0; k < 4; ++k) cdev[b + 16 * i + 4 * j + k] = d.coef[b + i + 4 * j + 16 * k];
+                if ((r = F->mem.upload(&dc, cdev.data(), ncoef))) return r;
             }
-            /* libtricubic order a[i + 4j + 16k] (coefficient of x^i y^j z^k) becomes
-               a[16i + 4j + k]: the 16 coefficients of one power of x are contiguous */
-            std::vector<double> cdev(ncoef);
-            for (size_t b = 0; b < ncoef; b += 64)
-                for (int i = 0; i < 4; ++i)
-                    for (int j = 0; j < 4; ++j)
-                        for (int k = 0; k < 4; ++k) cdev[b + 16 * i + 4 * j + k] = d.coef[b + i + 4 * j + 16 * k];
-            double *dc;
-            int r = F->mem.upload(&dc, cdev.data(), ncoef);
-            if (r) return r;
             f.coef = dc;
             if (with_fp32) {
-                std::vector<float> c32(ncoef);
-                for (size_t i = 0; i < ncoef; ++i) c32[i] = (float)cdev[i];
-                float *dc32;
-                r = F->mem.upload(&dc32, c32.data(), ncoef);
-                if (r) return r;
-                f.coef32 = dc32;
+                /* the mixed-precision table, built on the device from the FP64 table */
+                if (!(d.mixed_tol >= 0)) return fail(PT_EINVAL, "pt_fields_create: negative mixed_tol");
+                const double tol = d.mixed_tol > 0 ? d.mixed_tol : PT_MIXED_TOL_DEFAULT;
+                const size_t nblk = ncoef / 64;
+                unsigned char *dm;
+                int *dcount;
+                const int zero = 0;
+                if ((r = F->mem.alloc(&dm, nblk * ptd::MIX_BLOCK_BYTES)) || (r = F->mem.upload(&dcount, &zero, 1)))
+                    return r;
+                PT_HIP(pt_launch_mixed_table(f.g[0], f.g[1], f.g[2], d.nx, d.ny, d.nz, f.ncomp, dc, tol, dm, dcount, 0));
+                int nfp64 = 0;
+                PT_HIP(hipMemcpy(&nfp64, dcount, sizeof(int), hipMemcpyDeviceToHost));
+                f.mix = dm;
+                F->mix_blocks += (long long)nblk;
+                F->mix_fp64 += nfp64;
             }
         } else if (d.type == PT_FIELD_TAB2D) {
             /* TabField (src/field_2d.cpp): r grid in xg[nx], z grid in zg[nz] */
@@ -600,6 +601,14 @@ extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32
 }
 
 extern "C" void pt_fields_destroy(pt_fields *f) { delete f; }
+
+/* blocks (cell x component) of the mixed-precision tables and those kept in FP64 */
+extern "C" int pt_fields_mixed_info(const pt_fields *f, int64_t *blocks, int64_t *fp64_blocks) {
+    if (!f) return fail(PT_EINVAL, "pt_fields_mixed_info: bad arguments");
+    if (blocks) *blocks = f->mix_blocks;
+    if (fp64_blocks) *fp64_blocks = f->mix_fp64;
+    return PT_OK;
+}
 
 /* ======================================================================== */
 /* tricubic pre-interpolation (TabField3::PreInterpol, src/field_3d.cpp:336-414) */
@@ -787,9 +796,9 @@ extern "C" int pt_tricubic_precompute(int nx, int ny, int nz, const double *xg, 
 namespace {
 
 /* TabField3::PreInterpol on the device for samples values_h [ncomp][nx][ny][nz] (host) into
-   coef_dev [cells][ncomp][64] (and coef32_dev if not NULL); synchronous */
+   coef_dev [cells][ncomp][64]; synchronous */
 int tricubic_on_device(int nx, int ny, int nz, const double *const gh[3], int ncomp, const double *values_h,
-                       int device_order, double *coef_dev, float *coef32_dev) {
+                       int device_order, double *coef_dev) {
     DevMem tmp;
     const size_t np = (size_t)nx * ny * nz;
     const int n[3] = {nx, ny, nz};
@@ -812,7 +821,7 @@ int tricubic_on_device(int nx, int ny, int nz, const double *const gh[3], int nc
         PT_HIP(hipMemcpy(der[0], values_h + comp * np, np * sizeof(double), hipMemcpyHostToDevice));
         PT_HIP(pt_launch_tricubic_derivs(g, nx, ny, nz, der, scr, 0));
         PT_HIP(pt_launch_tricubic_coef(g[0], g[1], g[2], nx, ny, nz, der_dev, A, ncomp, comp, device_order, coef_dev,
-                                       coef32_dev, 0));
+                                       0));
         PT_HIP(hipStreamSynchronize(0));
     }
     return PT_OK;
@@ -842,7 +851,7 @@ extern "C" int pt_tricubic_precompute_gpu(int nx, int ny, int nz, const double *
     DevMem mem;
     double *dc;
     if ((r = mem.alloc(&dc, ncoef))) return r;
-    if ((r = tricubic_on_device(nx, ny, nz, gh, ncomp, values, 0, dc, nullptr))) return r;
+    if ((r = tricubic_on_device(nx, ny, nz, gh, ncomp, values, 0, dc))) return r;
     PT_HIP(hipMemcpy(coef, dc, ncoef * sizeof(double), hipMemcpyDeviceToHost));
     return PT_OK;
 }
@@ -1249,6 +1258,15 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
     }
     int block = cfg->block > 0 ? cfg->block : 256;
     if (block % 64 != 0 || block > 256) return fail(PT_EINVAL, "pt_batch_run: block must be a multiple of 64 and <= 256");
+    {
+        /* the step kernels hold their dopri5 stage vectors in PT_STEP_LDS_BYTES of LDS per block */
+        int dev = 0, lds = 0;
+        PT_HIP(hipGetDevice(&dev));
+        PT_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        if (lds < PT_STEP_LDS_BYTES)
+            return fail(PT_EDEVICE, "pt_batch_run: the step kernels need " + std::to_string(PT_STEP_LDS_BYTES) +
+                                        " bytes of LDS per block, the device allows " + std::to_string(lds));
+    }
     const int mode = (cfg->variant == PT_VARIANT_MIXED ? 1 : 0) | (b->fields->general ? 2 : 0);
     hipStream_t s = static_cast<hipStream_t>(stream);
     int q = g_quit.load();

@@ -12,8 +12,12 @@
 
 namespace ptd { struct Params; struct Geom; }
 
-/* mode: bit 0 FP32 field tables (PT_VARIANT_MIXED), bit 1 containers with scale formulas,
+/* mode: bit 0 mixed-precision field tables (PT_VARIANT_MIXED), bit 1 containers with scale formulas,
    analytic fields or conductors present */
+
+/* static LDS of the step kernels: the dopri5 stage vectors k2..k5, 4 x 8 doubles for each of
+   at most 256 threads (must match stage_lds in pt_track.hip) */
+#define PT_STEP_LDS_BYTES (4 * 8 * 256 * 8)
 
 /* the mode-templated launches, each instantiated in the translation unit of its mode */
 template <int MODE> hipError_t pt_launch_init_mode(const ptd::Params &P, int grid, hipStream_t stream);
@@ -43,7 +47,13 @@ hipError_t pt_launch_tricubic_derivs(const double *const g[3], int nx, int ny, i
                                      double *scratch, hipStream_t stream);
 hipError_t pt_launch_tricubic_coef(const double *xg, const double *yg, const double *zg, int nx, int ny, int nz,
                                    const double *const *der_dev, const double *A, int ncomp, int comp, int device_order,
-                                   double *coef, float *coef32, hipStream_t stream);
+                                   double *coef, hipStream_t stream);
+/* the mixed-precision table (ptd::MIX_BLOCK_BYTES per cell and component) of a map from its
+   FP64 table in the tracker's order; *n_fp64 counts the blocks kept in FP64 (tol: bound on
+   the FP32 remainder's gradient error, field unit per metre) */
+hipError_t pt_launch_mixed_table(const double *xg, const double *yg, const double *zg, int nx, int ny, int nz,
+                                 int ncomp, const double *coef, double tol, unsigned char *mix, int *n_fp64,
+                                 hipStream_t stream);
 /* ID of the solid with the highest priority at each point and time */
 hipError_t pt_launch_solid_at(const ptd::Geom &G, int n, const double *pts, const double *t, int *out, hipStream_t stream);
 /* decay products of the batch's decayed neutrons into out (count reset first) */

@@ -89,8 +89,7 @@ __global__ void __launch_bounds__(256) tricubic_coef_kernel(const double *__rest
                                                             const double *__restrict__ zg, int nx, int ny, int nz,
                                                             const double *const *__restrict__ der,
                                                             const double *__restrict__ A, int ncomp, int comp,
-                                                            int device_order, double *__restrict__ coef,
-                                                            float *__restrict__ coef32) {
+                                                            int device_order, double *__restrict__ coef) {
     __shared__ double As[64 * 64];
     for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) As[k] = A[k];
     __syncthreads();
@@ -117,7 +116,6 @@ __global__ void __launch_bounds__(256) tricubic_coef_kernel(const double *__rest
         xv[56 + c] = der[7][p] * cellx * celly * cellz;
     }
     double *a = coef + ((size_t)cell * ncomp + comp) * 64;
-    float *a32 = coef32 ? coef32 + ((size_t)cell * ncomp + comp) * 64 : nullptr;
 #pragma unroll 1
     for (int i = 0; i < 64; ++i) {
         double s = 0.0;
@@ -125,8 +123,51 @@ __global__ void __launch_bounds__(256) tricubic_coef_kernel(const double *__rest
         for (int j = 0; j < 64; ++j) s += As[64 * i + j] * xv[j];
         const int o = device_order ? 16 * (i & 3) + 4 * ((i >> 2) & 3) + (i >> 4) : i;
         a[o] = s;
-        if (a32) a32[o] = (float)s;
     }
+}
+
+/*
+ * The mixed-precision table of a map from its FP64 table in the tracker's order: one thread
+ * per cell and component writes a block of 4 doubles (the constant and linear coefficients,
+ * slots 0, 1, 4, 16: 1, z, y, x) and 64 floats (all coefficients, those four zeroed).  The
+ * FP32 remainder's error is bounded by 2^-20 sum_{i+j+k>=2} |a_ijk| for the value and by
+ * 2^-20 sum |a_ijk| (i/hx + j/hy + k/hz) for the gradient (coefficients rounded to FP32 and
+ * an FMA Horner over [0,1]^3, with room for the rounding of the cell coordinates); where the
+ * gradient bound exceeds tol [field unit / m] or the value bound tol * min(hx, hy, hz), the
+ * block is marked by a NaN constant term and the tracker reads the FP64 block instead.
+ */
+__global__ void __launch_bounds__(256) mixed_table_kernel(const double *__restrict__ xg, const double *__restrict__ yg,
+                                                          const double *__restrict__ zg, int nx, int ny, int nz,
+                                                          int ncomp, const double *__restrict__ coef, double tol,
+                                                          unsigned char *__restrict__ mix, int *__restrict__ n_fp64) {
+    const long long ncell = (long long)(nx - 1) * (ny - 1) * (nz - 1);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncell * ncomp) return;
+    const long long cell = t / ncomp;
+    const int iz = (int)(cell % (nz - 1));
+    const int iy = (int)((cell / (nz - 1)) % (ny - 1));
+    const int ix = (int)(cell / ((long long)(nz - 1) * (ny - 1)));
+    const double hx = xg[ix + 1] - xg[ix], hy = yg[iy + 1] - yg[iy], hz = zg[iz + 1] - zg[iz];
+    const double *a = coef + (size_t)t * 64;
+    double vb = 0.0, gb = 0.0;
+    for (int o = 0; o < 64; ++o) {
+        const int i = o >> 4, j = (o >> 2) & 3, k = o & 3;
+        if (i + j + k < 2) continue;
+        const double m = fabs(a[o]);
+        vb += m;
+        gb += m * ((double)i / hx + (double)j / hy + (double)k / hz);
+    }
+    const double u = 1.0 / 1048576.0;   /* 2^-20 */
+    const double hmin = fmin(hx, fmin(hy, hz));
+    const bool keep64 = !(gb * u <= tol) || !(vb * u <= tol * hmin);
+    double *lin = reinterpret_cast<double *>(mix + (size_t)t * (4 * 8 + 64 * 4));
+    float *rem = reinterpret_cast<float *>(lin + 4);
+    lin[0] = keep64 ? __builtin_nan("") : a[0];
+    lin[1] = a[1];
+    lin[2] = a[4];
+    lin[3] = a[16];
+    for (int o = 0; o < 64; ++o) rem[o] = (o == 0 || o == 1 || o == 4 || o == 16) ? 0.0f : (float)a[o];
+    if (keep64) atomicAdd(n_fp64, 1);
 }
 
 } // namespace ptp
@@ -151,10 +192,20 @@ hipError_t pt_launch_tricubic_derivs(const double *const g[3], int nx, int ny, i
 
 hipError_t pt_launch_tricubic_coef(const double *xg, const double *yg, const double *zg, int nx, int ny, int nz,
                                    const double *const *der_dev, const double *A, int ncomp, int comp, int device_order,
-                                   double *coef, float *coef32, hipStream_t stream) {
+                                   double *coef, hipStream_t stream) {
     const long long ncell = (long long)(nx - 1) * (ny - 1) * (nz - 1);
     if (ncell <= 0) return hipSuccess;
     hipLaunchKernelGGL(ptp::tricubic_coef_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, stream, xg, yg,
-                       zg, nx, ny, nz, der_dev, A, ncomp, comp, device_order, coef, coef32);
+                       zg, nx, ny, nz, der_dev, A, ncomp, comp, device_order, coef);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_mixed_table(const double *xg, const double *yg, const double *zg, int nx, int ny, int nz,
+                                 int ncomp, const double *coef, double tol, unsigned char *mix, int *n_fp64,
+                                 hipStream_t stream) {
+    const long long nblk = (long long)(nx - 1) * (ny - 1) * (nz - 1) * ncomp;
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ptp::mixed_table_kernel, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, stream, xg, yg, zg,
+                       nx, ny, nz, ncomp, coef, tol, mix, n_fp64);
     return hipGetLastError();
 }

@@ -132,16 +132,14 @@ PTD void tric_eval_grad64(const double *__restrict__ a, double x, double y, doub
     }
 }
 
-/* FP32 variant for the mixed-precision tracker: coefficients and Horner in FP32 (with FMA) */
-PTD void tric_eval_grad32(const float *__restrict__ a, float x, float y, float z, bool grad,
+/* FP32 Horner (with FMA) of a block held in registers, for the remainder of the
+   mixed-precision tables */
+PTD void tric_eval_grad32(const float a[64], float x, float y, float z, bool grad,
                           float &F, float &Fx, float &Fy, float &Fz) {
     float ry[4], ryd[4], ryz[4];
 #pragma unroll
     for (int i = 3; i >= 0; --i) {
-        const float4 *c4 = reinterpret_cast<const float4 *>(a + 16 * i);
-        float c[16];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) { float4 v = c4[q]; c[4 * q] = v.x; c[4 * q + 1] = v.y; c[4 * q + 2] = v.z; c[4 * q + 3] = v.w; }
+        const float *c = a + 16 * i;
         float rY = 0.f, rYd = 0.f, rYz = 0.f;
 #pragma unroll
         for (int j = 3; j >= 0; --j) {
@@ -161,6 +159,38 @@ PTD void tric_eval_grad32(const float *__restrict__ a, float x, float y, float z
         Fx = __builtin_fmaf(__builtin_fmaf(3.0f * ry[3], x, 2.0f * ry[2]), x, ry[1]);
         Fy = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ryd[3], x, ryd[2]), x, ryd[1]), x, ryd[0]);
         Fz = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ryz[3], x, ryz[2]), x, ryz[1]), x, ryz[0]);
+    }
+}
+
+/* Mixed precision: value and gradient of component comp of a cell from the mixed table,
+   the FP64 linear part (in FP64 arithmetic on the FP64 cell coordinates) plus the FP32
+   Horner of the remainder; or the FP64 block where the component kept it. */
+PTD void tric_eval_mixed(const Field &f, size_t cell, int comp, const double r[3], bool grad, double &F, double &Fx,
+                         double &Fy, double &Fz) {
+    const size_t blk = cell * (size_t)f.ncomp + (size_t)comp;
+    const double *lin = reinterpret_cast<const double *>(f.mix + blk * MIX_BLOCK_BYTES);
+    /* the whole block is loaded before the marker is tested, so the loads of the common
+       (mixed) case do not wait for the marker's round trip to memory */
+    const double2 l01 = reinterpret_cast<const double2 *>(lin)[0];
+    const double2 l23 = reinterpret_cast<const double2 *>(lin)[1];
+    float c[64];
+    const float4 *c4 = reinterpret_cast<const float4 *>(lin + 4);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const float4 v = c4[q];
+        c[4 * q] = v.x; c[4 * q + 1] = v.y; c[4 * q + 2] = v.z; c[4 * q + 3] = v.w;
+    }
+    if (l01.x != l01.x) {
+        tric_eval_grad64(f.coef + blk * 64, r[0], r[1], r[2], grad, F, Fx, Fy, Fz);
+        return;
+    }
+    float fv, fx = 0.f, fy = 0.f, fz = 0.f;
+    tric_eval_grad32(c, (float)r[0], (float)r[1], (float)r[2], grad, fv, fx, fy, fz);
+    F = l01.x + l01.y * r[2] + l23.x * r[1] + l23.y * r[0] + (double)fv;
+    if (grad) {
+        Fx = l23.y + (double)fx;
+        Fy = l23.x + (double)fy;
+        Fz = l01.y + (double)fz;
     }
 }
 
@@ -214,10 +244,10 @@ PTD_NOINLINE FG scale_at_bounds(const Field &f, double x, double y, double z, do
 
 PTD bool has_box(const Field &f) { return f.box[1] < f.box[0] && f.box[3] < f.box[2] && f.box[5] < f.box[4]; }
 
-/* kernel modes: FP32 field tables (mixed precision); containers with scale formulas,
+/* kernel modes: mixed-precision field tables; containers with scale formulas,
    analytic fields or conductors present (without it the field sum handles tricubic maps
    and uniform E with constant scales only, the code of the common case) */
-constexpr int MODE_FP32 = 1, MODE_FIELDS = 2;
+constexpr int MODE_MIXED = 1, MODE_FIELDS = 2;
 
 /* TFieldScaler::scalingFactor (src/field.cpp:7-10) from the compiled formula */
 PTD double scale_formula(const uint32_t *code, int n, const double *cst, double t) {
@@ -500,10 +530,8 @@ PTD void container_B(const Field &f, double x, double y, double z, double bs, do
 #pragma unroll 1
                 for (int i = 0; i < 3; ++i) {
                     double Fv, Fx = 0, Fy = 0, Fz = 0;
-                    if (MODE & MODE_FP32) {
-                        float fv, fx, fy, fz;
-                        tric_eval_grad32(f.coef32 + ((size_t)cell * f.ncomp + i) * 64, (float)r[0], (float)r[1], (float)r[2], grad, fv, fx, fy, fz);
-                        Fv = fv; Fx = fx; Fy = fy; Fz = fz;
+                    if (MODE & MODE_MIXED) {
+                        tric_eval_mixed(f, (size_t)cell, i, r, grad, Fv, Fx, Fy, Fz);
                     } else {
                         tric_eval_grad64(f.coef + ((size_t)cell * f.ncomp + i) * 64, r[0], r[1], r[2], grad, Fv, Fx, Fy, Fz);
                     }
@@ -568,10 +596,8 @@ PTD void container_E(const Field &f, double x, double y, double z, double es, do
                 ++nfield;
                 double Fv, Fx, Fy, Fz;
                 int comp = f.has_b ? 3 : 0;
-                if (MODE & MODE_FP32) {
-                    float fv, fx, fy, fz;
-                    tric_eval_grad32(f.coef32 + ((size_t)cell * f.ncomp + comp) * 64, (float)r[0], (float)r[1], (float)r[2], true, fv, fx, fy, fz);
-                    Fv = fv; Fx = fx; Fy = fy; Fz = fz;
+                if (MODE & MODE_MIXED) {
+                    tric_eval_mixed(f, (size_t)cell, comp, r, true, Fv, Fx, Fy, Fz);
                 } else {
                     tric_eval_grad64(f.coef + ((size_t)cell * f.ncomp + comp) * 64, r[0], r[1], r[2], true, Fv, Fx, Fy, Fz);
                 }
@@ -1777,6 +1803,15 @@ PTD bool spin_step(const Params &P, int i, const Species &sp, double x1, double 
     return true;
 }
 
+/* The dopri5 stage vectors k2..k5 of the step kernels' trial steps live in LDS, one slice per
+   lane: stage_lds[stage][component][threadIdx.x] (k2 = stage 0 ... k5 = stage 3; 64 KB per
+   block of at most 256 threads, pt_batch_run's limit).  Keeping them out of the registers
+   leaves the register file to the field evaluations between the stages; each lane touches
+   only its own slice, so no barrier is needed. */
+constexpr int STAGE_LDS = 4, STAGE_LANES = 256;
+__shared__ double stage_lds[STAGE_LDS][8][STAGE_LANES];
+static_assert(sizeof(stage_lds) == PT_STEP_LDS_BYTES, "the host checks PT_STEP_LDS_BYTES against the device");
+
 /*
  * One step of TTracker::IntegrateParticle (src/tracking.cpp:71-148) on the step path.
  * The dopri5 trial keeps its stage vectors in registers; the dense output is written to
@@ -1829,38 +1864,53 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
     double xnew[8], k7[8], Bn[3], Vn, polnew, dt_used;
     bool bn, need_dense;
     int tries = 0;
+    /* this lane's slices of the stage vectors k2..k5 */
+    const int lane = threadIdx.x;
+#define K2(k) stage_lds[0][k][lane]
+#define K3(k) stage_lds[1][k][lane]
+#define K4(k) stage_lds[2][k][lane]
+#define K5(k) stage_lds[3][k][lane]
     for (;;) {
         const double dt = L.dt;
-        double tmp[8], k2[8], k3[8], k4[8], k5[8], k6[8], Bd[3], Vd;
+        double tmp[8], kst[8], k6[8], Bd[3], Vd;
         bool bd;
         const double h21 = dt * b21;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h21 * L.dxc[k];
-        derivs8<MODE>(P, sp, tmp, ts + dt * (1.0 / 5), L.pol, k2, Bd, Vd, bd, c);
+        derivs8<MODE>(P, sp, tmp, ts + dt * (1.0 / 5), L.pol, kst, Bd, Vd, bd, c);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) K2(k) = kst[k];
         const double h31 = dt * b31, h32 = dt * b32;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h31 * L.dxc[k] + h32 * k2[k];
-        derivs8<MODE>(P, sp, tmp, ts + dt * (3.0 / 10), L.pol, k3, Bd, Vd, bd, c);
+        for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h31 * L.dxc[k] + h32 * K2(k);
+        derivs8<MODE>(P, sp, tmp, ts + dt * (3.0 / 10), L.pol, kst, Bd, Vd, bd, c);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) K3(k) = kst[k];
         const double h41 = dt * b41, h42 = dt * b42, h43 = dt * b43;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h41 * L.dxc[k] + h42 * k2[k] + h43 * k3[k];
-        derivs8<MODE>(P, sp, tmp, ts + dt * (4.0 / 5), L.pol, k4, Bd, Vd, bd, c);
+        for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h41 * L.dxc[k] + h42 * K2(k) + h43 * K3(k);
+        derivs8<MODE>(P, sp, tmp, ts + dt * (4.0 / 5), L.pol, kst, Bd, Vd, bd, c);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) K4(k) = kst[k];
         const double h51 = dt * b51, h52 = dt * b52, h53 = dt * b53, h54 = dt * b54;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h51 * L.dxc[k] + h52 * k2[k] + h53 * k3[k] + h54 * k4[k];
-        derivs8<MODE>(P, sp, tmp, ts + dt * (8.0 / 9), L.pol, k5, Bd, Vd, bd, c);
+        for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h51 * L.dxc[k] + h52 * K2(k) + h53 * K3(k) + h54 * K4(k);
+        derivs8<MODE>(P, sp, tmp, ts + dt * (8.0 / 9), L.pol, kst, Bd, Vd, bd, c);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) K5(k) = kst[k];
         const double h61 = dt * b61, h62 = dt * b62, h63 = dt * b63, h64 = dt * b64, h65 = dt * b65;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h61 * L.dxc[k] + h62 * k2[k] + h63 * k3[k] + h64 * k4[k] + h65 * k5[k];
+        for (int k = 0; k < 8; ++k)
+            tmp[k] = xc[k] + h61 * L.dxc[k] + h62 * K2(k) + h63 * K3(k) + h64 * K4(k) + h65 * K5(k);
         derivs8<MODE>(P, sp, tmp, ts + dt, L.pol, k6, Bd, Vd, bd, c);
         const double g1 = dt * c1, g3 = dt * c3, g4 = dt * c4, g5 = dt * c5, g6 = dt * c6;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) xnew[k] = xc[k] + g1 * L.dxc[k] + g3 * k3[k] + g4 * k4[k] + g5 * k5[k] + g6 * k6[k];
+        for (int k = 0; k < 8; ++k) xnew[k] = xc[k] + g1 * L.dxc[k] + g3 * K3(k) + g4 * K4(k) + g5 * K5(k) + g6 * k6[k];
         polnew = L.pol + g1 * 0.0 + g3 * 0.0 + g4 * 0.0 + g5 * 0.0 + g6 * 0.0;
         const double e1 = dt * dc1, e3 = dt * dc3, e4 = dt * dc4, e5 = dt * dc5, e6 = dt * dc6, e7 = dt * dc7;
         double errp[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) errp[k] = e1 * L.dxc[k] + e3 * k3[k] + e4 * k4[k] + e5 * k5[k] + e6 * k6[k];
+        for (int k = 0; k < 8; ++k) errp[k] = e1 * L.dxc[k] + e3 * K3(k) + e4 * K4(k) + e5 * K5(k) + e6 * k6[k];
         /* can the rest of this step need the dense output? */
         {
             double l2 = (xnew[7] - xc[7]) * (xnew[7] - xc[7]);
@@ -1877,9 +1927,9 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
             for (int k = 0; k < 8; ++k) {
                 ds[k * n + i] = xc[k];
                 ds[(8 + k) * n + i] = L.dxc[k];
-                ds[(16 + k) * n + i] = k3[k];
-                ds[(24 + k) * n + i] = k4[k];
-                ds[(32 + k) * n + i] = k5[k];
+                ds[(16 + k) * n + i] = K3(k);
+                ds[(24 + k) * n + i] = K4(k);
+                ds[(32 + k) * n + i] = K5(k);
                 ds[(40 + k) * n + i] = k6[k];
             }
         }
@@ -1913,6 +1963,10 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
         }
         if (ok) break;
     }
+#undef K2
+#undef K3
+#undef K4
+#undef K5
     const double polo = L.pol;
     const double told = ts;
     const double tnew = ts + dt_used;

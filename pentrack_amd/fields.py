@@ -47,6 +47,9 @@ class FieldContainer:
     eformula: Optional[str] = None
     # 2D maps: the samples [ncomp][nz][nr] the coefficients were made from
     values: Optional[np.ndarray] = None
+    # tricubic, mixed-precision tables: bound on the FP32 remainder's gradient error of a
+    # cell component [field unit / m]; components above it keep FP64 (0: library default)
+    mixed_tol: float = 0.0
 
     def __post_init__(self):
         if len(self.params) > 8:
@@ -271,6 +274,7 @@ class TFieldManager:
                 d.nx, d.ny, d.nz = len(xg), len(yg), len(zg)
                 d.xg, d.yg, d.zg = _native.dptr(xg), _native.dptr(yg), _native.dptr(zg)
                 d.has_b, d.has_v = c.has_b, c.has_v
+                d.mixed_tol = c.mixed_tol
                 if c.coef is not None:
                     coef = np.ascontiguousarray(c.coef, dtype=np.float64)
                     keep.append(coef)
@@ -289,19 +293,28 @@ class TFieldManager:
                 d.coef = _native.dptr(coef)
         return ds, keep
 
-    def device(self, with_fp32=False):
-        if self._handle is not None and (self._handle[1] or not with_fp32):
+    def device(self, with_mixed=False):
+        if self._handle is not None and (self._handle[1] or not with_mixed):
             return self._handle[0]
         L = _native.lib()
         ds, keep = self.descs()
         h = C.c_void_p()
-        rc = L.pt_fields_create(len(self.containers), ds, int(bool(with_fp32)), C.byref(h))
+        rc = L.pt_fields_create(len(self.containers), ds, int(bool(with_mixed)), C.byref(h))
         _native.check(rc, "pt_fields_create")
         if self._handle is not None:
             # batches may still use the old tables: keep them until the manager is released
             self._retired = getattr(self, "_retired", []) + [self._handle[0]]
-        self._handle = (h, bool(with_fp32))
+        self._handle = (h, bool(with_mixed))
         return h
+
+    def mixed_info(self):
+        """(blocks, blocks kept in FP64) of the mixed-precision tables on the device."""
+        if self._handle is None or not self._handle[1]:
+            return 0, 0
+        b, f = C.c_int64(), C.c_int64()
+        _native.check(_native.lib().pt_fields_mixed_info(self._handle[0], C.byref(b), C.byref(f)),
+                      "pt_fields_mixed_info")
+        return b.value, f.value
 
     def __del__(self):
         try:

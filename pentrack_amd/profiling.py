@@ -46,7 +46,7 @@ def run(kernel="step_kernel_fp64"):
     w = workloads.cylinder(n=1_000_000, seed=2, tmax=200.0, n_grid=64)
     variant, persistent, bpe, _, spatial = KERNELS[kernel]
     b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
-              with_fp32=bool(variant))
+              with_mixed=bool(variant))
     stream = torch.cuda.current_stream().cuda_stream
     cfg = LaunchConfig(variant=variant, persistent=persistent, steps_per_launch=16, spatial_order=spatial)
     b.set_profiling(True)

@@ -39,7 +39,7 @@ class Batch:
     """Particles resident in device memory with their resumable tracker state."""
 
     def __init__(self, geometry, fields, species, tmax, species_idx, t0, y0, spin0, seed, pid,
-                 hit_cap=0, with_fp32=False, leaf_size=4):
+                 hit_cap=0, with_mixed=False, leaf_size=4):
         L = _native.lib()
         self.n = len(t0)
         self.hit_cap = hit_cap
@@ -54,7 +54,7 @@ class Batch:
         si, t, y, s, p, _ = self._keep
         self.geometry, self.fields = geometry, fields
         g = geometry.device(leaf_size)
-        f = fields.device(with_fp32)
+        f = fields.device(with_mixed)
         h = C.c_void_p()
         rc = L.pt_batch_create(g, f, len(species), sp, float(tmax), self.n, si.ctypes.data_as(C.POINTER(C.c_int32)),
                                _native.dptr(t), _native.dptr(y), _native.dptr(s), int(seed) & (2 ** 64 - 1),
@@ -215,7 +215,7 @@ class TTracker:
         spinlog = (interval, cap) also (records, counts) of the spin log, in that order."""
         cfg = self.launch
         b = Batch(self.geometry, self.fields, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap,
-                  with_fp32=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
+                  with_mixed=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
         if snapshots is not None:
             b.set_snapshots(*snapshots)
         if track is not None:
@@ -239,7 +239,7 @@ class TTracker:
         order of `records` (pentrack_amd.tracker.Batch.secondaries)."""
         cfg = self.launch
         b = Batch(self.geometry, self.fields, species, tmax, species_idx, t0, y0, spin0, seed, pid, hit_cap,
-                  with_fp32=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
+                  with_mixed=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
         b.run_to_completion(cfg)
         prim = b.download()
         rec = b.secondaries()
@@ -247,7 +247,7 @@ class TTracker:
             return prim, (np.zeros(0, _native.OUT_DTYPE), None), rec
         sspecies, sidx, st0, sy0, sspin0, spid = secondary_states(species, rec, np.asarray(pid, np.uint64))
         s = Batch(self.geometry, self.fields, sspecies, tmax, sidx, st0, sy0, sspin0, seed, spid, hit_cap,
-                  with_fp32=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
+                  with_mixed=cfg.variant == _native.PT_VARIANT_MIXED, leaf_size=cfg.leaf_size)
         s.run_to_completion(cfg)
         return prim, s.download(), rec
 

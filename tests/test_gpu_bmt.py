@@ -6,7 +6,7 @@ import pytest
 from pentrack_amd import workloads
 from pentrack_amd.particle import SpeciesConfig
 from pentrack_amd.workloads import Workload
-from tests.test_gpu_parity import FIELDS, run_gpu
+from tests.test_gpu_parity import FIELDS, HITS, assert_bit_exact, run_gpu
 from tests.test_oracle_bmt import GAMMA_N, falling_workload, larmor_workload
 
 pytestmark = pytest.mark.gpu
@@ -34,10 +34,9 @@ def test_bmt_with_walls_and_map(gpu, oracle_lib):
     w = Workload("bmt", base.geometry, base.fields, sp, base.tmax, base.species_idx, base.t0, base.y0, spin0,
                  base.pid, base.seed)
     print("case: BMT cylinder n=400", flush=True)
-    g, gh = run_gpu(w, hit_cap=8)
-    o, oh = oracle_lib.track(w, "fp64", hit_cap=8, threads=8)
-    for f in FIELDS:
-        assert np.array_equal(g[f], o[f]), f
+    g, gh = run_gpu(w)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=HITS, threads=8)
+    assert_bit_exact(g, gh, o, oh)
     assert g["Nhit"].sum() > 100 and (g["spin"][:, 3] > 0).mean() > 0.5      # spin integrated in the windows
 
 

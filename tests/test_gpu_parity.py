@@ -20,14 +20,19 @@ FIELDS = ("t", "y", "spin", "stopID", "Nhit", "Nstep", "Nspinflip", "n_acc", "n_
           "solidend", "n_hits")
 
 
-def run_gpu(w, hit_cap=16, cfg=None, with_fp32=False):
+HITS = 512      # hit-log capacity: every test's particles hit fewer walls (checked)
+
+
+def run_gpu(w, hit_cap=HITS, cfg=None, with_mixed=False):
     b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
-              hit_cap=hit_cap, with_fp32=with_fp32)
+              hit_cap=hit_cap, with_mixed=with_mixed)
     assert b.run_to_completion(cfg or LaunchConfig(), max_launches=100000)
     return b.download()
 
 
 def assert_bit_exact(g, gh, o, oh):
+    """Every output field bit for bit, and the whole hit log of every particle: the logs
+    must hold every hit (no particle hit the walls more often than the log capacity)."""
     msgs = []
     for f in FIELDS:
         if not np.array_equal(g[f], o[f]):
@@ -37,17 +42,19 @@ def assert_bit_exact(g, gh, o, oh):
     if msgs:
         raise AssertionError("\n".join(msgs))
     if gh is not None:
-        n = np.minimum(g["n_hits"], gh.shape[1])
-        for k in np.where(n > 0)[0][:2000]:
-            for f in ("t", "tri", "leaving", "entering", "normal", "v1", "v2", "pol1", "pol2", "pos"):
-                assert np.array_equal(gh[k, :n[k]][f], oh[k, :n[k]][f]), (k, f)
+        cap = gh.shape[1]
+        assert oh.shape[1] == cap and (g["n_hits"] <= cap).all(), "hit log shorter than a particle's hits"
+        logged = np.arange(cap)[None, :] < g["n_hits"][:, None]
+        for f in ("t", "tri", "leaving", "entering", "normal", "v1", "v2", "pol1", "pol2", "pos"):
+            a, b = gh[f][logged], oh[f][logged]
+            assert np.array_equal(a, b), f"hit log field {f} differs in {int((a != b).reshape(len(a), -1).any(1).sum())} hits"
 
 
 def test_cube_specular_bit_exact(gpu, oracle_lib):
     w = workloads.cube(n=3000, seed=1, tmax=10.0)
     print("case: cube n=3000 tmax=10 seed=1", flush=True)
     g, gh = run_gpu(w)
-    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=HITS, threads=8)
     assert_bit_exact(g, gh, o, oh)
     assert (g["stopID"] == -1).mean() > 0.99 and g["Nhit"].mean() > 50
 
@@ -61,10 +68,10 @@ def test_sliced_launches_equal_single_launch(gpu):
 
 
 def test_cylinder_field_map_lambert_bit_exact(gpu, oracle_lib):
-    w = workloads.cylinder(n=2000, seed=2, tmax=3.0, n_grid=32)
-    print("case: cylinder n=2000 tmax=3 grid=32", flush=True)
+    w = workloads.cylinder(n=2000, seed=2, tmax=3.0, n_grid=64)      # the 64^3 FP64 map of the benchmark
+    print("case: cylinder n=2000 tmax=3 grid=64", flush=True)
     g, gh = run_gpu(w)
-    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=HITS, threads=8)
     assert_bit_exact(g, gh, o, oh)
     assert g["Nhit"].sum() > 1000
 
@@ -90,7 +97,7 @@ def test_charged_species_uniform_E_and_map(gpu, oracle_lib):
     w = Workload("mixed-species", base.geometry, fm, species, base.tmax, kinds, base.t0, y0, base.spin0, base.pid, 4)
     print("case: charged species n=600", flush=True)
     g, gh = run_gpu(w)
-    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=HITS, threads=8)
     assert_bit_exact(g, gh, o, oh)
     assert set(np.unique(g["stopID"])) >= {-4, 1}
 
@@ -101,7 +108,7 @@ def test_analytic_fields_and_time_dependent_scales_bit_exact(gpu, oracle_lib):
     w = workloads.analytic(n=600, seed=6, tmax=0.1)
     print("case: analytic fields n=600", flush=True)
     g, gh = run_gpu(w)
-    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=HITS, threads=8)
     assert_bit_exact(g, gh, o, oh)
     print("case: analytic fields n=600 persistent", flush=True)
     p, ph = run_gpu(w, cfg=LaunchConfig(persistent=1, steps_per_launch=16))
@@ -122,8 +129,8 @@ def test_neutron_decay_products_and_their_tracks_bit_exact(gpu, oracle_lib):
     print("case: decay products n=600", flush=True)
     tr = TTracker(w.geometry, w.fields, LaunchConfig())
     (g, gh), (sg, sgh), rec = tr.IntegrateWithSecondaries(w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0,
-                                                           w.seed, w.pid, hit_cap=16)
-    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
+                                                           w.seed, w.pid, hit_cap=HITS)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=HITS, threads=8)
     assert_bit_exact(g, gh, o, oh)
     dec = np.where((o["stopID"] == -4) & (w.species_idx == 0))[0]
     assert len(dec) > 50 and len(rec) == 2 * len(dec)
@@ -137,7 +144,7 @@ def test_neutron_decay_products_and_their_tracks_bit_exact(gpu, oracle_lib):
     print("case: decay products tracked", flush=True)
     sspecies, sidx, st0, sy0, sspin0, spid = secondary_states(w.species, rec, w.pid)
     sw = Workload("secondaries", w.geometry, w.fields, sspecies, w.tmax, sidx, st0, sy0, sspin0, spid, w.seed)
-    so, soh = oracle_lib.track(sw, "fp64", hit_cap=16, threads=8)
+    so, soh = oracle_lib.track(sw, "fp64", hit_cap=HITS, threads=8)
     assert_bit_exact(sg, sgh, so, soh)
     assert (sg["n_acc"] > 0).mean() > 0.9
 
@@ -149,7 +156,7 @@ def test_edge_cases(gpu, oracle_lib):
     w.y0[2, 2] = 1.0 - 1e-9        # just below the ceiling
     print("case: edge cases", flush=True)
     g, gh = run_gpu(w)
-    o, oh = oracle_lib.track(w, "fp64", hit_cap=16)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=HITS)
     assert_bit_exact(g, gh, o, oh)
     assert g["stopID"][0] == _native.ID_INVARIANT and g["stopID"][1] == _native.ID_HIT_BOUNDARIES
     # an empty batch is valid
@@ -160,20 +167,35 @@ def test_edge_cases(gpu, oracle_lib):
     assert b.active() == 0
 
 
-def test_mixed_precision_error_bound(gpu, oracle_lib):
-    """FP32 field interpolation, FP64 state/control: end positions within 1e-6 m of the
-    FP64 oracle over a short horizon with identical collision sequences."""
-    w = workloads.cylinder(n=1000, seed=6, tmax=0.5, n_grid=32)
-    print("case: mixed precision n=1000", flush=True)
-    g, gh = run_gpu(w, cfg=LaunchConfig(variant=_native.PT_VARIANT_MIXED), with_fp32=True)
-    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
-    same = (g["stopID"] == o["stopID"]) & (g["Nhit"] == o["Nhit"])
-    n = np.minimum(g["n_hits"], 16)
-    for k in np.where(same)[0]:
-        same[k] &= np.array_equal(gh[k, :n[k]]["tri"], oh[k, :n[k]]["tri"])
-    assert same.mean() > 0.99
-    d = np.linalg.norm(g["y"][same, :3] - o["y"][same, :3], axis=1)
-    assert np.percentile(d, 99) < 1e-6
+def test_mixed_precision_config3(gpu, oracle_lib):
+    """The mixed-precision workload as specified: 1 m cube, 128^3 coil map (|B| <= 1 T),
+    neutrons from the config-1 source.  The FP64 kernel is bit-exact with the FP64 oracle on
+    it; the mixed-precision kernel (FP64 constant and linear terms, FP32 remainder, FP64
+    blocks where the remainder's bound exceeds the default tolerance) keeps every particle's
+    stop reason and full hit-triangle sequence and its end position within 1e-6 m of the
+    oracle over 0.25 s.  (The dynamics in this field are chaotic: deviations of any origin
+    grow about e-fold every 0.8 s, so the bound holds over a horizon, not over 50 s.)"""
+    w = workloads.cube(n=2000, seed=3, tmax=0.25, with_field=True, n_grid=128, Bmax=1.0)
+    print("case: mixed precision, cube, 128^3 map, n=2000, 0.25 s", flush=True)
+    cap = 256
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=cap, threads=8)
+    assert o["n_hits"].max() <= cap and o["Nhit"].sum() > w.n
+    g, gh = run_gpu(w, hit_cap=cap)
+    assert_bit_exact(g, gh, o, oh)
+    m, mh = run_gpu(w, hit_cap=cap, cfg=LaunchConfig(variant=_native.PT_VARIANT_MIXED), with_mixed=True)
+    blocks, fp64 = w.fields.mixed_info()
+    assert 0 < fp64 < 0.5 * blocks
+    assert np.array_equal(m["stopID"], o["stopID"]) and np.array_equal(m["Nhit"], o["Nhit"])
+    assert np.array_equal(m["n_hits"], o["n_hits"])
+    for k in range(w.n):
+        nk = o["n_hits"][k]
+        assert np.array_equal(mh[k, :nk]["tri"], oh[k, :nk]["tri"]), k
+    dx = np.linalg.norm(m["y"][:, :3] - o["y"][:, :3], axis=1)
+    dv = np.linalg.norm(m["y"][:, 3:6] - o["y"][:, 3:6], axis=1)
+    print(f"mixed vs oracle: end position max {dx.max():.2e} p99 {np.percentile(dx, 99):.2e} m, "
+          f"end velocity max {dv.max():.2e} m/s, FP64 blocks {fp64}/{blocks}", flush=True)
+    assert dx.max() <= 1e-6
+    assert not np.array_equal(m["y"], o["y"])           # the mixed arithmetic is in use
 
 
 def test_persistent_work_queue_bit_exact(gpu, oracle_lib):
@@ -182,7 +204,7 @@ def test_persistent_work_queue_bit_exact(gpu, oracle_lib):
     w = workloads.cylinder(n=1500, seed=8, tmax=2.0, n_grid=24)
     print("case: persistent cylinder n=1500", flush=True)
     g, gh = run_gpu(w, cfg=LaunchConfig(persistent=1, steps_per_launch=9, grid=64))
-    o, oh = oracle_lib.track(w, "fp64", hit_cap=16, threads=8)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=HITS, threads=8)
     assert_bit_exact(g, gh, o, oh)
     print("case: persistent cylinder n=1500 spatial order", flush=True)
     s, sh = run_gpu(w, cfg=LaunchConfig(persistent=1, steps_per_launch=9, grid=64, spatial_order=1))
@@ -196,7 +218,7 @@ def test_graph_replays_match_eager_calls(gpu):
     w = workloads.cube(n=2000, seed=12, tmax=3.0)
     print("case: graph replay cube n=2000", flush=True)
     mk = lambda: Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
-                       hit_cap=8)
+                       hit_cap=HITS)
     a, b = mk(), mk()
     s = torch.cuda.Stream()
     sp = s.cuda_stream

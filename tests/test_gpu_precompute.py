@@ -41,8 +41,8 @@ def test_maps_uploaded_as_samples(gpu, oracle_lib):
     o = oracle_lib.field_eval(dev, pts, 0.0, "fp64")
     assert all(np.array_equal(u, v) for u, v in zip(a, o))
     # the FP32 copy made on the device equals the host's rounding of the coefficients
-    host.device(with_fp32=True)
-    dev.device(with_fp32=True)
+    host.device(with_mixed=True)
+    dev.device(with_mixed=True)
     from pentrack_amd.workloads import Workload
     from pentrack_amd.particle import SpeciesConfig
     from pentrack_amd.constants import NEUTRON
@@ -53,8 +53,8 @@ def test_maps_uploaded_as_samples(gpu, oracle_lib):
     ws = [Workload("m", w0.geometry, f, [SpeciesConfig(NEUTRON, tmax=9e99)], 0.05, w0.species_idx, w0.t0, w0.y0,
                    w0.spin0, w0.pid, 5) for f in (host, dev)]
     cfg = LaunchConfig(variant=_native.PT_VARIANT_MIXED)
-    ga, _ = run_gpu(ws[0], cfg=cfg, with_fp32=True)
-    gb, _ = run_gpu(ws[1], cfg=cfg, with_fp32=True)
+    ga, _ = run_gpu(ws[0], cfg=cfg, with_mixed=True)
+    gb, _ = run_gpu(ws[1], cfg=cfg, with_mixed=True)
     assert np.array_equal(ga["y"], gb["y"]) and np.array_equal(ga["Nstep"], gb["Nstep"])
 
 
