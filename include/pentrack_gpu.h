@@ -420,6 +420,18 @@ int pt_rk4_adjoint(void *stream, const pt_adj_params *params, int n, const doubl
 int pt_rk4_adjoint_noalloc(void *stream, const pt_adj_params *params, int n, const double *ckpt, double *lam,
                            double *sbar, int nsteps, int K);
 
+/* ---- measurement aid ------------------------------------------------------ */
+/*
+ * No counterpart in the reference: the bandwidth ceilings the tracking kernels are compared
+ * with.  gather_gbs: `threads` lanes each read `gathers` blocks of block_bytes (contiguous per
+ * lane, 16-byte loads) from uniformly random blocks of a table of table_bytes, the access
+ * pattern of the tricubic field evaluation; copy_gbs: a streaming copy of the table (bytes
+ * read + written).  Both averaged over `reps` launches after a warm-up; allocates and frees
+ * 2 x table_bytes of device memory; synchronous on the default stream.
+ */
+int pt_gather_bandwidth(uint64_t table_bytes, int block_bytes, int threads, int gathers, int reps,
+                        double *gather_gbs, double *copy_gbs);
+
 /* Cancellation flag mirroring the reference's global `quit` (include/globals.h:33):
    pt_batch_run copies it to the device when it is called, and its step kernels
    then advance no particle; state stays resumable (clear the flag and run again). */

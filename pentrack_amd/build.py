@@ -17,7 +17,7 @@ OUT = os.path.join(HERE, "libpentrack_gpu.so")
 OBJ = os.path.join(HERE, "_obj")
 
 N_MODES = 4
-SOURCES = ["pt_adjoint.hip", "pt_precompute.hip", "pt_host.cpp", "pt_formula.cpp"]
+SOURCES = ["pt_adjoint.hip", "pt_precompute.hip", "pt_membench.hip", "pt_host.cpp", "pt_formula.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     # bit-reproducible FP64: no contraction, no reassociation

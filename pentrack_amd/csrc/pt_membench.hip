@@ -1,0 +1,112 @@
+/*
+ * pt_membench.hip -- the bandwidth ceilings the tracking kernels are measured against.
+ *
+ * No counterpart in the reference (a measurement aid).  The step kernel's dominant memory
+ * traffic is one 64-coefficient block per field component gathered from a random cell by
+ * each lane; this file measures what the device sustains for exactly that pattern (each
+ * lane reads its own contiguous block of block_bytes from a uniformly random block of a
+ * table of table_bytes), and for a plain streaming copy.  With tables that fit the 256 MB
+ * Infinity Cache the gather is served from there; with larger ones from HBM.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/pentrack_gpu.h"
+#include "pt_adjoint.h"
+
+namespace ptm_bench {
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+/* each thread gathers `gathers` random blocks of `d2` double2 words and keeps a sum */
+__global__ void __launch_bounds__(256) gather_kernel(const double2 *__restrict__ table, uint64_t nblocks, int d2,
+                                                     int gathers, uint64_t seed, double *__restrict__ sink) {
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    for (int g = 0; g < gathers; ++g) {
+        /* uniform in [0, nblocks) by a multiply-high of 32 random bits (nblocks < 2^32) */
+        const uint64_t b = ((mix64(seed ^ (tid * 0x100000001b3ull + (uint64_t)g)) >> 32) * nblocks) >> 32;
+        const double2 *p = table + b * (uint64_t)d2;
+        for (int q = 0; q < d2; ++q) {
+            const double2 v = p[q];
+            s += v.x + v.y;
+        }
+    }
+    sink[tid] = s;
+}
+
+__global__ void __launch_bounds__(256) copy_kernel(const double2 *__restrict__ a, double2 *__restrict__ b, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+__global__ void __launch_bounds__(256) fill_kernel(double2 *__restrict__ a, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        a[i] = make_double2(1e-3 * (double)(i & 1023), 1.0);
+}
+
+} // namespace ptm_bench
+
+#define PTB_HIP(call)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) { rc = pt_fail(PT_EDEVICE, (std::string(#call ": ") + hipGetErrorString(e_)).c_str()); goto done; } \
+    } while (0)
+
+extern "C" int pt_gather_bandwidth(uint64_t table_bytes, int block_bytes, int threads, int gathers, int reps,
+                                   double *gather_gbs, double *copy_gbs) {
+    if (block_bytes < 16 || block_bytes % 16 != 0 || table_bytes < (uint64_t)block_bytes ||
+        table_bytes / (uint64_t)block_bytes >= (1ull << 32) || threads < 256 ||
+        threads % 256 != 0 || gathers < 1 || reps < 1)
+        return pt_fail(PT_EINVAL, "pt_gather_bandwidth: bad arguments");
+    int rc = PT_OK;
+    const uint64_t nblocks = table_bytes / (uint64_t)block_bytes;
+    const uint64_t n2 = nblocks * (uint64_t)block_bytes / 16;
+    double2 *table = nullptr, *copy = nullptr;
+    double *sink = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    PTB_HIP(hipMalloc(&table, n2 * 16));
+    PTB_HIP(hipMalloc(&copy, n2 * 16));
+    PTB_HIP(hipMalloc(&sink, (size_t)threads * sizeof(double)));
+    PTB_HIP(hipEventCreate(&e0));
+    PTB_HIP(hipEventCreate(&e1));
+    hipLaunchKernelGGL(ptm_bench::fill_kernel, dim3(4096), dim3(256), 0, 0, table, n2);
+    PTB_HIP(hipGetLastError());
+    /* warm-up, then the timed repetitions of the gather */
+    hipLaunchKernelGGL(ptm_bench::gather_kernel, dim3(threads / 256), dim3(256), 0, 0, table, nblocks, block_bytes / 16,
+                       gathers, 12345ull, sink);
+    PTB_HIP(hipGetLastError());
+    PTB_HIP(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(ptm_bench::gather_kernel, dim3(threads / 256), dim3(256), 0, 0, table, nblocks,
+                           block_bytes / 16, gathers, 777ull + (uint64_t)r, sink);
+    PTB_HIP(hipEventRecord(e1, 0));
+    PTB_HIP(hipEventSynchronize(e1));
+    PTB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (gather_gbs) *gather_gbs = (double)threads * gathers * block_bytes * reps / (ms * 1e-3) / 1e9;
+    /* streaming copy of the table (read + write bytes) */
+    hipLaunchKernelGGL(ptm_bench::copy_kernel, dim3(8192), dim3(256), 0, 0, table, copy, n2);
+    PTB_HIP(hipGetLastError());
+    PTB_HIP(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(ptm_bench::copy_kernel, dim3(8192), dim3(256), 0, 0, table, copy, n2);
+    PTB_HIP(hipEventRecord(e1, 0));
+    PTB_HIP(hipEventSynchronize(e1));
+    PTB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (copy_gbs) *copy_gbs = 2.0 * (double)n2 * 16 * reps / (ms * 1e-3) / 1e9;
+done:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (sink) (void)hipFree(sink);
+    if (copy) (void)hipFree(copy);
+    if (table) (void)hipFree(table);
+    return rc;
+}

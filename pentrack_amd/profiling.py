@@ -1,33 +1,53 @@
-"""HBM traffic of the step kernel from rocprofv3 counters.
+"""Measurements of the step kernel from rocprofv3: HBM traffic, FP64 instructions, kernel table.
 
 `python -m pentrack_amd.profiling run KERNEL` is the program profiled: a plain
 device copy of known size (the calibration: FETCH_SIZE and WRITE_SIZE of a kernel
 that reads and writes exactly COPY_BYTES), then the benchmark workload (1e6
 neutrons, cylinder, 64^3 map) advanced by a few run calls of 16 RK steps with the
-step kernel KERNEL (step_kernel_fp64, step_kernel_mixed,
+step kernel KERNEL (step_kernel_fp64, step_kernel_persistent_fp64, step_kernel_mixed,
 step_kernel_persistent_mixed, or step_kernel_persistent_mixed_spatial: the persistent
-kernel with its work list in Morton order of the particle positions).  Profile it twice, once per counter:
+kernel with its work list in Morton order of the particle positions).  Profile it three
+times, once per counter set (a pass holds at most 4 TCC and 8 SQ counters):
 
     rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 -m pentrack_amd.profiling run KERNEL
     rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 -m pentrack_amd.profiling run KERNEL
+    rocprofv3 --pmc SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 SQ_INSTS_VALU_FMA_F64 SQ_INSTS_VALU_TRANS_F64 \
+        -d gpurun_out/pmc_flops -o run -- python3 -m pentrack_amd.profiling run KERNEL
 
-`python -m pentrack_amd.profiling summarize KERNEL FETCH_DB WRITE_DB` prints the
-summary: counter values (KiB per dispatch) of the copy and of the step kernel,
-the scale factor (bytes the copy moved / bytes the counter reports), the step
-kernel's scaled HBM bytes per launch and its algorithmic bytes per launch.
+`python -m pentrack_amd.profiling summarize KERNEL FETCH_DB WRITE_DB [ALGORITHMIC [FLOPS_DB]]`
+prints the summary: counter values (KiB per dispatch) of the copy and of the step kernel,
+the scale factor (bytes the copy moved / bytes the counter reports), the step kernel's
+scaled HBM bytes per launch, its algorithmic bytes per launch and its issued FP64 lane
+operations per launch.  `python -m pentrack_amd.profiling trace CSV TITLE COMMAND` prints
+the kernel table of a `rocprofv3 --kernel-trace --output-format csv` run.
 """
+import ctypes as C
 import sqlite3
 import sys
 
 COPY_BYTES = 1 << 30
 KERNELS = {
-    # name: (variant, persistent, bytes per field evaluation, dispatch-name pattern, spatial order)
-    # (the kernels' template argument is the mode: 0 FP64 tables, 1 FP32 tables)
-    "step_kernel_fp64": (0, 0, 1536, "step_kernel<0>", 0),
-    "step_kernel_mixed": (1, 0, 768, "step_kernel<1>", 0),
-    "step_kernel_persistent_mixed": (1, 1, 768, "step_kernel_persistent<1>", 0),
-    "step_kernel_persistent_mixed_spatial": (1, 1, 768, "step_kernel_persistent<1>", 1),
+    # name: (variant, persistent, dispatch-name pattern, spatial order)
+    # (the kernels' template argument is the mode: 0 FP64 tables, 1 mixed-precision tables)
+    "step_kernel_fp64": (0, 0, "step_kernel<0>", 0),
+    "step_kernel_persistent_fp64": (0, 1, "step_kernel_persistent<0>", 0),
+    "step_kernel_mixed": (1, 0, "step_kernel<1>", 0),
+    "step_kernel_persistent_mixed": (1, 1, "step_kernel_persistent<1>", 0),
+    "step_kernel_persistent_mixed_spatial": (1, 1, "step_kernel_persistent<1>", 1),
 }
+# the FP64 VALU instruction counters of a third pass (wave-level instruction counts)
+FLOP_COUNTERS = ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64")
+
+
+def gather_ceiling(table_bytes, block_bytes=512, threads=1 << 19, gathers=16, reps=5):
+    """(random-gather GB/s, streaming-copy GB/s) of pt_gather_bandwidth for a table of
+    table_bytes: lanes reading their own block_bytes block from random blocks, as the
+    tricubic evaluation does."""
+    from . import _native
+    g, c = C.c_double(), C.c_double()
+    _native.check(_native.lib().pt_gather_bandwidth(int(table_bytes), int(block_bytes), int(threads), int(gathers),
+                                                    int(reps), C.byref(g), C.byref(c)), "pt_gather_bandwidth")
+    return g.value, c.value
 
 
 def run(kernel="step_kernel_fp64"):
@@ -44,9 +64,13 @@ def run(kernel="step_kernel_fp64"):
     torch.cuda.synchronize()
     del src, dst
     w = workloads.cylinder(n=1_000_000, seed=2, tmax=200.0, n_grid=64)
-    variant, persistent, bpe, _, spatial = KERNELS[kernel]
+    variant, persistent, _, spatial = KERNELS[kernel]
     b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
               with_mixed=bool(variant))
+    blocks, fp64 = w.fields.mixed_info()
+    # bytes one field evaluation gathers: 3 blocks of 512 B (FP64), or of 288 B plus 512 B
+    # for the components kept in FP64 (mixed)
+    bpe = 3 * 512 if not variant else 3 * (288 + 512 * fp64 / max(blocks, 1))
     stream = torch.cuda.current_stream().cuda_stream
     cfg = LaunchConfig(variant=variant, persistent=persistent, steps_per_launch=16, spatial_order=spatial)
     b.set_profiling(True)
@@ -68,13 +92,29 @@ def _per_dispatch(db, counter):
     return {d: (k, float(v)) for d, k, v in rows}
 
 
-def summarize(kernel, fetch_db, write_db, algorithmic=None):
+def _flops_per_dispatch(db, pattern):
+    """Issued FP64 lane operations per dispatch of the kernels matching pattern: 64 lanes x
+    (ADD + MUL + TRANS + 2 FMA) wave instructions (inactive lanes of a wave included, so an
+    upper bound of the useful FP64 FLOPs)."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select dispatch_id, kernel_name, counter_name, sum(value) from counters_collection "
+                       "group by dispatch_id, counter_name").fetchall()
+    per = {}
+    for d, k, c, v in rows:
+        if pattern in k and c in FLOP_COUNTERS:
+            per.setdefault(d, {})[c] = float(v)
+    w = {"SQ_INSTS_VALU_ADD_F64": 1, "SQ_INSTS_VALU_MUL_F64": 1, "SQ_INSTS_VALU_FMA_F64": 2,
+         "SQ_INSTS_VALU_TRANS_F64": 1}
+    return [64 * sum(w[c] * v for c, v in cs.items()) for cs in per.values()], per
+
+
+def summarize(kernel, fetch_db, write_db, algorithmic=None, flops_db=None):
     f = _per_dispatch(fetch_db, "FETCH_SIZE")
     w = _per_dispatch(write_db, "WRITE_SIZE")
     kib = COPY_BYTES / 1024
     # the calibration copies: device-to-device copies that wrote COPY_BYTES
     copies = [d for d in f if "copy" in f[d][0].lower() and d in w and w[d][1] > 0.9 * kib]
-    pattern = KERNELS[kernel][3]
+    pattern = KERNELS[kernel][2]
     step = [d for d in f if pattern in f[d][0]]
     if not copies or not step:
         raise RuntimeError("calibration copy or step kernel not found")
@@ -100,13 +140,56 @@ def summarize(kernel, fetch_db, write_db, algorithmic=None):
     if algorithmic:
         print(f"algorithmic_bytes_per_launch (field gathers, from the run's device counters): {algorithmic:.6e}")
     print(f"traffic_bytes_per_launch: {fetch + write:.6e}")
+    if flops_db:
+        fl, per = _flops_per_dispatch(flops_db, pattern)
+        if fl:
+            one = next(iter(per.values()))
+            print(f"FP64 instructions (third pass, {' '.join(FLOP_COUNTERS)}), first dispatch: "
+                  + ", ".join(f"{c} {v:.4e}" for c, v in sorted(one.items())))
+            print(f"  rocprofv3 --pmc {' '.join(FLOP_COUNTERS)} -d gpurun_out/pmc_flops -o run -- "
+                  f"python3 -m pentrack_amd.profiling run {kernel}")
+            print(f"fp64_flops_per_launch: {sum(fl) / len(fl):.6e}")
+
+
+def summarize_trace(csv_path, title, command):
+    """The kernel table of a rocprofv3 --kernel-trace --output-format csv run: calls, average
+    and total duration, registers, scratch and LDS of every kernel, slowest first."""
+    import csv
+
+    def col(row, *names):
+        for n in names:
+            if n in row and row[n] != "":
+                return row[n]
+        return ""
+    stats = {}
+    with open(csv_path) as fh:
+        for row in csv.DictReader(fh):
+            name = col(row, "Kernel_Name", "Kernel-Name", "KernelName")
+            t = (int(col(row, "End_Timestamp", "End-Timestamp")) - int(col(row, "Start_Timestamp", "Start-Timestamp")))
+            e = stats.setdefault(name, {"calls": 0, "ns": 0, "row": row})
+            e["calls"] += 1
+            e["ns"] += t
+    print(f"# rocprofv3 --kernel-trace: {title}")
+    print(f"command: {command}")
+    print()
+    print("kernel | calls | avg ms | total ms | arch VGPR | AGPR | SGPR | scratch B/lane | LDS B | block | grid")
+    for name, e in sorted(stats.items(), key=lambda kv: -kv[1]["ns"]):
+        r = e["row"]
+        print(" | ".join([name, str(e["calls"]), f"{e['ns'] / e['calls'] / 1e6:.3f}", f"{e['ns'] / 1e6:.2f}",
+                          col(r, "Arch_VGPR_Count", "VGPR_Count"), col(r, "Accum_VGPR_Count"), col(r, "SGPR_Count"),
+                          col(r, "Scratch_Size", "Private_Segment_Size"), col(r, "LDS_Block_Size", "LDS_Size",
+                                                                               "Group_Segment_Size"),
+                          col(r, "Workgroup_Size_X", "Workgroup_Size"), col(r, "Grid_Size_X", "Grid_Size")]))
 
 
 if __name__ == "__main__":
     if len(sys.argv) >= 2 and sys.argv[1] == "run":
         run(*sys.argv[2:3])
-    elif len(sys.argv) in (5, 6) and sys.argv[1] == "summarize":
-        summarize(sys.argv[2], sys.argv[3], sys.argv[4], float(sys.argv[5]) if len(sys.argv) == 6 else None)
+    elif len(sys.argv) in (5, 6, 7) and sys.argv[1] == "summarize":
+        summarize(sys.argv[2], sys.argv[3], sys.argv[4], float(sys.argv[5]) if len(sys.argv) >= 6 else None,
+                  sys.argv[6] if len(sys.argv) == 7 else None)
+    elif len(sys.argv) == 5 and sys.argv[1] == "trace":
+        summarize_trace(sys.argv[2], sys.argv[3], sys.argv[4])
     else:
         print(__doc__)
         sys.exit(2)
