@@ -63,8 +63,125 @@ class Solid:
         return any(a <= t < b for a, b in self.ignoretimes)
 
 
-def read_stl(path):
-    """Binary STL reader (TTriangleMesh::ReadFile, src/trianglemesh.cpp:17-61).
+def repair_soup(tris):
+    """The polygon-soup repair TTriangleMesh::ReadFile applies to every STL file
+    (src/trianglemesh.cpp:63-64): PMP::repair_polygon_soup then PMP::orient_polygon_soup of
+    the vendored CGAL 4.14.1, restated for triangles.
+
+    tris: [n][3][3] float32 after the 1e-8 vertex snap.  Returns (triangles, source index of
+    each kept triangle).  The steps, in CGAL's order:
+      * merge_duplicate_points_in_polygon_soup (repair_polygon_soup.h:511-582): vertices are
+        identified by exact coordinates;
+      * simplify_polygons_in_polygon_soup and remove_invalid_polygons_in_polygon_soup
+        (:123-243, :339-372): a triangle with two identical vertices collapses below three
+        vertices and is erased, keeping the order of the others (triangles have no pinched
+        form, :246-337);
+      * merge_duplicate_polygons_in_polygon_soup (:906-1003): triangles on the same three
+        vertices, in either orientation, are duplicates; the first of a group stays, the
+        others are swapped one by one to the end of the list and cut off.  The order in which
+        several groups are visited is that of an unordered_map in CGAL (parity unpinned with
+        more than one group); here the group with the largest first index goes first;
+      * remove_isolated_points_in_polygon_soup changes no triangle;
+      * Polygon_soup_orienter::orient (orient_polygon_soup.h:188-284): from the first
+        triangle not yet oriented, a depth-first walk over edges shared by at most two
+        triangles; a neighbour that runs the shared edge in the same direction is reversed,
+        one that runs it the other way is taken as it is; an edge met by two already oriented
+        triangles of the same direction is not crossed.  The first triangle of each connected
+        component keeps its orientation (the soup is made consistent, not outward).  The
+        duplication of singular vertices that follows changes no triangle.
+    """
+    t = np.asarray(tris, dtype=np.float32).reshape(-1, 3, 3)
+    n = len(t)
+    _, vid = np.unique(t.reshape(-1, 3), axis=0, return_inverse=True)
+    F = vid.reshape(n, 3).astype(np.int64)
+    keep = (F[:, 0] != F[:, 1]) & (F[:, 1] != F[:, 2]) & (F[:, 0] != F[:, 2])
+    src = np.nonzero(keep)[0]
+    F = [list(f) for f in F[keep]]
+    # duplicate triangles
+    groups = {}
+    for k, f in enumerate(F):
+        groups.setdefault(tuple(sorted(f)), []).append(k)
+    dups = [g for g in groups.values() if len(g) > 1]
+    if dups:
+        m = len(F)
+        pos = list(range(m))          # position of each polygon id
+        at = list(range(m))           # polygon id at each position
+        order = list(F)
+        srcl = list(src)
+        swap = m - 1
+        for g in sorted(dups, key=lambda g: g[0], reverse=True):      # largest first index first
+            for pid in g[1:]:
+                p = pos[pid]
+                other = at[swap]
+                pos[other], pos[pid] = p, swap
+                at[p], at[swap] = other, pid
+                order[swap], order[p] = order[p], order[swap]
+                srcl[swap], srcl[p] = srcl[p], srcl[swap]
+                swap -= 1
+        F, src = order[:swap + 1], np.asarray(srcl[:swap + 1])
+    # orientation
+    m = len(F)
+    edges = {}
+    for k, f in enumerate(F):
+        for j in range(3):
+            edges.setdefault((f[j], f[(j + 1) % 3]), set()).add(k)
+    marked = set()
+    for f in F:
+        for j in range(3):
+            a, b = f[j], f[(j + 1) % 3]
+            if len(edges.get((a, b), ())) + len(edges.get((b, a), ())) > 2:
+                marked.add((min(a, b), max(a, b)))
+    oriented = [False] * m
+    for start in range(m):
+        if oriented[start]:
+            continue
+        oriented[start] = True
+        stack = [start]
+        while stack:
+            cur = stack.pop()
+            for j in range(3):
+                i1, i2 = F[cur][j], F[cur][(j + 1) % 3]
+                if (min(i1, i2), max(i1, i2)) in marked:
+                    continue
+                same = edges[(i1, i2)]
+                if len(same) > 1:
+                    idx = min(same)
+                    if idx == cur:
+                        idx = sorted(same)[1]
+                    if oriented[idx]:
+                        marked.add((min(i1, i2), max(i1, i2)))
+                        continue
+                    f = F[idx]
+                    for q in range(3):
+                        e = (f[q], f[(q + 1) % 3])
+                        edges[e].discard(idx)
+                        if not edges[e]:
+                            del edges[e]
+                    F[idx] = f[::-1]
+                    f = F[idx]
+                    for q in range(3):
+                        edges.setdefault((f[q], f[(q + 1) % 3]), set()).add(idx)
+                    oriented[idx] = True
+                    stack.append(idx)
+                else:
+                    other = edges.get((i2, i1))
+                    if other:
+                        idx = min(other)
+                        if not oriented[idx]:
+                            oriented[idx] = True
+                            stack.append(idx)
+    # triangles from the (possibly reversed) vertex cycles: a reversed triangle (a, b, c)
+    # is (c, b, a) in CGAL's std::reverse
+    out = np.empty((m, 3, 3), dtype=np.float32)
+    uniq = np.unique(t.reshape(-1, 3), axis=0)
+    for k, f in enumerate(F):
+        out[k] = uniq[f]
+    return out, np.asarray(src, dtype=np.int64)
+
+
+def read_stl(path, repair=True):
+    """Binary STL reader (TTriangleMesh::ReadFile, src/trianglemesh.cpp:17-64): the 1e-8 vertex
+    snap and, with repair, the polygon-soup repair and orientation of repair_soup.
 
     Returns (header name, [n][3][3] float32 vertices); normals in the file are ignored.
     """
@@ -80,7 +197,11 @@ def read_stl(path):
     if 84 + count * rec.itemsize > len(data):
         raise ValueError(f"{path} should contain {count} triangles but is truncated")
     arr = np.frombuffer(data, dtype=rec, count=count, offset=84)
-    return header, np.ascontiguousarray(arr["v"], dtype=np.float32)
+    tris = np.ascontiguousarray(arr["v"], dtype=np.float32)
+    if repair:
+        tris = np.where(np.abs(tris) < 1e-8, np.float32(0), tris)
+        tris, _ = repair_soup(tris)
+    return header, tris
 
 
 def write_stl(path, tris, name="pentrack_amd"):
