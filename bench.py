@@ -85,8 +85,8 @@ def parse():
     p.add_argument("--err-speed-n", type=int, default=1_000_000,
                    help="neutrons of the FP64-vs-mixed speed comparison on the error leg's map (0: skip)")
     p.add_argument("--no-uneven", action="store_true")
-    p.add_argument("--uneven-n", type=int, default=1_000_000)
-    p.add_argument("--uneven-tris", type=int, default=50_000)
+    p.add_argument("--uneven-n", type=int, default=4_000_000)
+    p.add_argument("--uneven-meshes", default="guide,storage,cylinder")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-ceiling", action="store_true")
@@ -253,35 +253,47 @@ def mixed_error_leg(a, sptr):
 
 
 def uneven_leg(a, sptr):
-    """Mixed species with very uneven lifetimes: the autotuned launch shape (persistent
-    work-queue kernel or per-particle grid) against the default per-particle grid."""
+    """Config 4: mixed species with very uneven lifetimes in vessels of the reference's
+    1032-triangle guide and 61468-triangle storage volume and of a synthetic 200k-triangle
+    cylinder, a loss per bounce per neutron: the launch shape the autotuner picks (persistent
+    work-queue kernel or per-particle grid, steps per call, BVH leaf size) against the default
+    per-particle grid."""
     import torch
     from pentrack_amd import autotune, workloads
     from pentrack_amd.tracker import Batch, LaunchConfig
-    w = workloads.uneven(n=a.uneven_n, seed=4, n_tris=a.uneven_tris, n_grid=a.grid)
-    best, entry = autotune.tune(w, n_sample=min(200_000, w.n), calls=2, blocks=(256,), grids=(2, 4), steps=(16, 64),
-                                leaf_sizes=(4,), cache=None, stream=sptr)
-    res = {"workload": f"cylinder ({w.geometry.n_triangles()} tris), {w.n} particles: 90% neutrons tau=880 s, "
-                       "8% protons 1-800 eV, 2% electrons 1-700 keV, uniform E 1e5 V/m + 64^3 B map",
-           "tuned_config": {k: getattr(best, k) for k in ("block", "steps_per_launch", "persistent", "grid",
-                                                          "leaf_size")}}
-    for name, cfg in (("default", LaunchConfig(steps_per_launch=16)), ("autotuned", best)):
-        b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
-                  leaf_size=cfg.leaf_size)
-        b.run(cfg, sptr)
-        torch.cuda.synchronize()
-        a0, _ = b.step_counts(sptr)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(4):
+    res = {}
+    for mesh in [m for m in a.uneven_meshes.split(",") if m]:
+        w = workloads.uneven_vessel(mesh, n=a.uneven_n, seed=4, n_grid=a.grid, n_tris=200_000)
+        best, entry = autotune.tune(w, n_sample=min(200_000, w.n), calls=2, blocks=(256,), grids=(2, 4),
+                                    steps=(16, 64), leaf_sizes=(2, 4, 8), cache=None, stream=sptr)
+        r = {"workload": f"{mesh} vessel ({w.meta['mesh_triangles']} mesh triangles, {w.geometry.n_triangles()} "
+                         f"with the shell box), {w.n} particles: 90% neutrons tau=880 s and loss per bounce "
+                         "U(1e-5, 1e-3), 8% protons 1-800 eV, 2% electrons 1-700 keV, uniform E 1e5 V/m + "
+                         f"{a.grid}^3 B map",
+             "tuned_config": {k: getattr(best, k) for k in ("block", "steps_per_launch", "persistent", "grid",
+                                                            "leaf_size")},
+             "autotune_trials": entry.get("trials")}
+        for name, cfg in (("default", LaunchConfig(steps_per_launch=16)), ("autotuned", best)):
+            b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
+                      leaf_size=cfg.leaf_size)
+            b.set_loss(w.loss)
             b.run(cfg, sptr)
-        e1.record()
-        torch.cuda.synchronize()
-        a1, _ = b.step_counts(sptr)
-        ms = e0.elapsed_time(e1)
-        res[name] = {"rk_steps_per_s": (a1 - a0) / (ms / 1e3), "gpu_ms": ms, "particles_active": b.active(sptr)}
-        del b
-    print(f"[bench] uneven lifetimes: {res}", file=sys.stderr, flush=True)
+            torch.cuda.synchronize()
+            a0, _ = b.step_counts(sptr)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(4):
+                b.run(cfg, sptr)
+            e1.record()
+            torch.cuda.synchronize()
+            a1, _ = b.step_counts(sptr)
+            ms = e0.elapsed_time(e1)
+            r[name] = {"rk_steps_per_s": (a1 - a0) / (ms / 1e3), "gpu_ms": ms, "particles_active": b.active(sptr)}
+            del b
+        r["autotuned_over_default"] = r["autotuned"]["rk_steps_per_s"] / r["default"]["rk_steps_per_s"]
+        res[mesh] = r
+        print(f"[bench] uneven lifetimes, {mesh}: {r}", file=sys.stderr, flush=True)
+        del w
     return res
 
 

@@ -247,6 +247,15 @@ int pt_batch_active(pt_batch *b, void *stream, int *active);
 /* Copy results to the host (synchronises the stream).  hits may be NULL. */
 int pt_batch_download(pt_batch *b, void *stream, pt_particle_out *out, pt_hit *hits);
 
+/*
+ * No counterpart in the reference (which has LossPerBounce per material only,
+ * include/geometry.h:30): a loss probability per bounce for each of the n particles, replacing
+ * the material's LossPerBounce in TNeutron::OnHit's absorption probability on total reflection
+ * (src/neutron.cpp:241) at every wall that particle meets; values in [0, 1].  loss == NULL
+ * switches it off.  Call before pt_batch_run.
+ */
+int pt_batch_set_loss(pt_batch *b, const double *loss);
+
 /* a particle's state at a snapshot time, with its spin and counters at that point */
 typedef struct {
     double t, y[9], spin[5], Hmax, noflipprob;

@@ -59,6 +59,7 @@
  *   parity unpinned: odeint's exact step-size bits beyond the published controller, CGAL
  *                    traversal order for ties between equal-s hits, x87 and glibc last bits
  */
+#include <float.h>
 #include <math.h>
 #include <complex.h>
 #include <string.h>
@@ -194,6 +195,14 @@ typedef struct {
     int probe_kind;                     /* > 0: derivs evaluates a test ODE (ptref_probe_steps) */
     const double *probe_par;
 } track_t;
+
+/* per-particle loss per bounce (a build extension: the reference has the material's
+   LossPerBounce only), NULL when off; g_particle is the batch index of the particle a worker
+   thread tracks */
+static const double *g_loss;
+static __thread int g_particle;
+
+void ptref_set_loss(const double *loss) { g_loss = loss; }
 
 static double rnd_uniform(track_t *p, double a, double b) {
     uint64_t r = ptm_draw_u64(p->key, &p->ctr);
@@ -1358,6 +1367,13 @@ static int dopri_try(track_t *tp, dopri_t *s, double xnew[9], double dxnew[9]) {
     return 1;
 }
 
+/* all nine components finite (0 and subnormals count; NaN and +-inf do not) */
+static int state_finite(const double *x) {
+    for (int i = 0; i < 9; ++i)
+        if (!(fabs(x[i]) <= DBL_MAX)) return 0;
+    return 1;
+}
+
 /* dense_output_runge_kutta::do_step; returns 0 on success, -1 when the
    failed-step checker throws (500 tries) */
 static int dopri_do_step(track_t *tp, dopri_t *s) {
@@ -1525,7 +1541,8 @@ static void neutron_onhit(track_t *tp, double x1, const double *y1, double *x2, 
             else neutron_transmit(tp, y1, y2, normal, Estep);
         }
     } else {
-        double absprob = 1 - reflprob + mat[8];
+        /* LossPerBounce of the material, or the particle's own where a per-particle loss is set */
+        double absprob = 1 - reflprob + (g_loss ? g_loss[g_particle] : mat[8]);
         if (prob < 0 + 0 + absprob * (1 - 0 - 0)) {
             tp->stopID = ID_ABSORBED_ON_SURFACE;
         } else {
@@ -1742,7 +1759,6 @@ static int g_snap_n, g_snap_cap;
 static const double *g_snap_t;
 static ptref_snap *g_snaps;
 static int32_t *g_nsnap;
-static __thread int g_particle;
 
 void ptref_set_snapshots(int n, const double *times, int cap, ptref_snap *out, int32_t *count) {
     g_snap_n = n; g_snap_t = times; g_snap_cap = cap; g_snaps = out; g_nsnap = count;
@@ -2133,8 +2149,11 @@ static void integrate_particle(const ctx_t *c, const ptref_species *cfg, double 
         if (resetintegration) dopri_initialize(&st, y, x, st.dt);
         double x1 = x, y1[9];
         memcpy(y1, y, sizeof(y1));
-        if (dopri_do_step(tp, &st) < 0) {
-            /* odeint threw: x and y keep the step-start values (src/tracking.cpp:78-85) */
+        if (dopri_do_step(tp, &st) < 0 || !state_finite(st.xc)) {
+            /* odeint threw: x and y keep the step-start values (src/tracking.cpp:78-85).  A
+               step accepted with a non-finite state (a stage beyond the speed of light makes
+               1/gamma a NaN, which the maximum of the error norm passes over) stops the same
+               way here; the reference would go on into CGAL with NaN coordinates. */
             tp->stopID = ID_ODEINT_ERROR;
             break;
         }

@@ -105,6 +105,8 @@ def load(mode="fp64"):
     L.ptref_tricubic_eval.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
     L.ptref_set_snapshots.argtypes = [C.c_int, dp, C.c_int, C.c_void_p, C.POINTER(C.c_int32)]
     L.ptref_set_snapshots.restype = None
+    L.ptref_set_loss.argtypes = [dp]
+    L.ptref_set_loss.restype = None
     L.ptref_set_track.argtypes = [C.c_double, C.c_int, C.c_void_p, C.POINTER(C.c_int32)]
     L.ptref_set_track.restype = None
     L.ptref_set_spinlog.argtypes = [C.c_double, C.c_int, C.c_void_p, C.POINTER(C.c_int32)]
@@ -293,10 +295,16 @@ def track(workload, mode="fp64", hit_cap=0, threads=1, sel=None, problem=None):
     pid = np.ascontiguousarray(workload.pid[idx], dtype=np.uint64)
     out = np.zeros(n, dtype=OUT_DTYPE)
     hits = np.zeros(n * hit_cap, dtype=HIT_DTYPE) if hit_cap else None
-    rc = L.ptref_track(C.byref(P.geom), P.n_fields, P.fields, sp, si.ctypes.data_as(C.POINTER(C.c_int32)),
-                       float(workload.tmax), n, _dp(t0), _dp(y0), _dp(s0), int(workload.seed),
-                       pid.ctypes.data_as(C.POINTER(C.c_uint64)), out.ctypes.data_as(C.c_void_p),
-                       hits.ctypes.data_as(C.c_void_p) if hits is not None else None, hit_cap, threads)
+    loss = getattr(workload, "loss", None)
+    loss = None if loss is None else np.ascontiguousarray(np.asarray(loss, dtype=np.float64)[idx])
+    L.ptref_set_loss(_dp(loss) if loss is not None else None)
+    try:
+        rc = L.ptref_track(C.byref(P.geom), P.n_fields, P.fields, sp, si.ctypes.data_as(C.POINTER(C.c_int32)),
+                           float(workload.tmax), n, _dp(t0), _dp(y0), _dp(s0), int(workload.seed),
+                           pid.ctypes.data_as(C.POINTER(C.c_uint64)), out.ctypes.data_as(C.c_void_p),
+                           hits.ctypes.data_as(C.c_void_p) if hits is not None else None, hit_cap, threads)
+    finally:
+        L.ptref_set_loss(None)
     if rc != 0:
         raise RuntimeError(f"ptref_track failed ({rc})")
     return out, (hits.reshape(n, hit_cap) if hits is not None else None)

@@ -145,7 +145,7 @@ EXPORTS = [
     "pt_distribution_table", "pt_fields_eval", "pt_geometry_solid_at", "pt_bicubic_precompute",
     "pt_tricubic_precompute_gpu", "pt_batch_set_snapshots", "pt_batch_snapshots",
     "pt_batch_set_track", "pt_batch_track", "pt_batch_set_spinlog", "pt_batch_spinlog",
-    "pt_rk4_adjoint_noalloc", "pt_fields_mixed_info", "pt_gather_bandwidth",
+    "pt_rk4_adjoint_noalloc", "pt_fields_mixed_info", "pt_gather_bandwidth", "pt_batch_set_loss",
 ]
 
 _lib = None
@@ -175,6 +175,7 @@ def lib():
     L.pt_fields_create.argtypes = [C.c_int, C.POINTER(FieldDesc), C.c_int, C.POINTER(vp)]
     L.pt_fields_destroy.argtypes = [vp]
     L.pt_fields_mixed_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.pt_batch_set_loss.argtypes = [vp, C.POINTER(C.c_double)]
     L.pt_gather_bandwidth.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]
     L.pt_tricubic_precompute.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),

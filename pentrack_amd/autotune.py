@@ -61,6 +61,8 @@ def tune(w, n_sample=200_000, calls=3, blocks=(64, 128, 256), grids=(1, 2, 4), s
             cfg = LaunchConfig(variant=variant, steps_per_launch=k, leaf_size=leaf, **shape)
             b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx[sel], w.t0[sel], w.y0[sel],
                       w.spin0[sel], w.seed, w.pid[sel], with_mixed=(variant == 1), leaf_size=leaf)
+            if getattr(w, "loss", None) is not None:
+                b.set_loss(w.loss[sel])
             for _ in range(calls):
                 b.run(cfg, s)
             torch.cuda.synchronize()

@@ -32,6 +32,7 @@ constexpr double HBAR = 1.05457266e-34;
 constexpr double MAX_TRACK_DEVIATION = 0.001;   /* include/particle.h:20 */
 constexpr double REFLECT_TOLERANCE = 1e-8;      /* include/trianglemesh.h:39 */
 constexpr double DBL_EPS = 2.220446049250313e-16;
+constexpr double DBL_MAX_V = 1.7976931348623157e308;   /* largest finite double */
 
 enum {
     ID_UNKNOWN = 0, ID_NOT_FINISH = -1, ID_HIT_BOUNDARIES = -2, ID_ODEINT_ERROR = -3,
@@ -141,6 +142,7 @@ struct State {
     unsigned long long *ctr, *key, *solids;
     Hit *hits;
     int hit_cap;
+    const double *loss;       /* per-particle loss per bounce replacing the materials' (nullptr: off) */
     /* snapshots: sorted times, records [n][snap_cap], records per particle */
     const double *snap_t;
     int snap_n, snap_cap, snap_sorted;

@@ -1519,6 +1519,28 @@ extern "C" int pt_batch_set_snapshots(pt_batch *b, int n_times, const double *ti
     return PT_OK;
 }
 
+/* per-particle loss per bounce (n values; NULL switches it off): replaces the materials'
+   LossPerBounce in TNeutron::OnHit's absorption probability (src/neutron.cpp:241) */
+extern "C" int pt_batch_set_loss(pt_batch *b, const double *loss) {
+    if (!b) return fail(PT_EINVAL, "pt_batch_set_loss: bad arguments");
+    ptd::State &S = b->P.st;
+    S.loss = nullptr;
+    if (loss) {
+        for (int i = 0; i < b->n; ++i)
+            if (!(loss[i] >= 0.0 && loss[i] <= 1.0))
+                return fail(PT_EINVAL, "pt_batch_set_loss: loss per bounce " + std::to_string(loss[i]) +
+                                           " of particle " + std::to_string(i) + " outside [0, 1]");
+        double *d;
+        int r = b->mem.upload(&d, loss, (size_t)std::max(b->n, 1));
+        if (r) return r;
+        S.loss = d;
+    }
+    /* launch sequences captured before hold the old state by value */
+    for (auto &g : b->graphs) { (void)hipGraphExecDestroy(g.exec); (void)hipGraphDestroy(g.graph); }
+    b->graphs.clear();
+    return PT_OK;
+}
+
 /* TLogger::PrintTrack: records of step ends where the trajectory length crossed a multiple of
    interval; interval = 0 switches the track log off */
 extern "C" int pt_batch_set_track(pt_batch *b, double interval, int cap) {

@@ -1038,7 +1038,9 @@ PTD_NOINLINE void neutron_onhit(const Params &P, Track &T, double x1, const doub
             else neutron_transmit(T.sp, y1, y2, normal, Estep);
         }
     } else {
-        double absprob = 1 - reflprob + mat[8];
+        /* LossPerBounce of the material, or the particle's own where a per-particle loss is set */
+        const double loss = P.st.loss ? P.st.loss[T.i] : mat[8];
+        double absprob = 1 - reflprob + loss;
         if (prob < 0 + 0 + absprob * (1 - 0 - 0)) {
             T.stop = ID_ABSORBED_ON_SURFACE;
         } else {
@@ -1967,6 +1969,18 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
 #undef K3
 #undef K4
 #undef K5
+    {
+        /* an accepted step with a non-finite state (a stage beyond the speed of light makes
+           1/gamma a NaN, which the maximum of the error norm passes over) stops as a stepper
+           failure, x and y kept at the step start (src/tracking.cpp:78-85) */
+        bool finite = polnew == polnew;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) finite = finite && (dabs(xnew[k]) <= DBL_MAX_V);
+        if (!finite) {
+            L.stop = ID_ODEINT_ERROR;
+            return;
+        }
+    }
     const double polo = L.pol;
     const double told = ts;
     const double tnew = ts + dt_used;

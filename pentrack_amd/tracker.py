@@ -98,6 +98,18 @@ class Batch:
         _native.check(_native.lib().pt_batch_kernel_times(self.h, stream, ms, n), "pt_batch_kernel_times")
         return {"step": (ms[0], n[0]), "collision": (ms[1], n[1])}
 
+    def set_loss(self, loss):
+        """Per-particle loss per bounce replacing the materials' LossPerBounce (None: off);
+        call before the first run."""
+        if loss is None:
+            _native.check(_native.lib().pt_batch_set_loss(self.h, None), "pt_batch_set_loss")
+            return
+        v = np.ascontiguousarray(np.asarray(loss, dtype=np.float64))
+        if v.shape != (self.n,):
+            raise ValueError("one loss per particle")
+        self._loss = v
+        _native.check(_native.lib().pt_batch_set_loss(self.h, _native.dptr(v)), "pt_batch_set_loss")
+
     def set_snapshots(self, times, cap):
         """TLogger::PrintSnapshot times (sorted, de-duplicated) and the records kept per
         particle; call before the first run."""

@@ -9,8 +9,9 @@ energy uniform in [0, Emax], isotropic directions (src/source.cpp:157-202 draws
 the same quantities).  Host-side sampling uses numpy's PCG64 seeded per
 workload; the tracker's own random draws use the per-particle streams.
 """
+import os
 from dataclasses import dataclass, field
-from typing import List
+from typing import List, Optional
 
 import numpy as np
 
@@ -34,6 +35,7 @@ class Workload:
     pid: np.ndarray
     seed: int
     meta: dict = field(default_factory=dict)
+    loss: Optional[np.ndarray] = None       # per-particle loss per bounce (None: the materials')
 
     @property
     def n(self):
@@ -174,6 +176,81 @@ def uneven(n=4000000, seed=4, n_tris=50000, tmax=200.0, n_grid=64, Bmax=1.0, E_f
     meta = dict(meta, n_phi=n_phi, n_z=n_z)
     return Workload("uneven", geom, fm, species, tmax, kinds, t0, y, spin0, np.arange(n, dtype=np.uint64), seed,
                     meta)
+
+
+# the reference's own meshes used by the uneven-lifetime workloads (binary STL, metres)
+REFERENCE_MESHES = {"guide": "guide_experiment.STL",                    # in/guide_experiment.STL, 1032 tris
+                    "storage": "storagevolume_60degfeeder_bellow.STL"}  # in/storagevolume_..., 61468 tris
+MESH_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "golden")
+
+
+def vessel_geometry(mesh, wall, margin=0.05):
+    """A vessel whose inside is the default solid (vacuum, ID 1), where charged particles
+    fly (TProton/TElectron::OnStep absorb them inside any solid with ID > 1,
+    src/proton.cpp:25-33): solid 2 of the wall material is the shell between the closed
+    mesh (reversed, so its normals face the vacuum) and a box around it, as
+    test/HollowUnitCube.STL is built."""
+    mesh = np.asarray(mesh, dtype=np.float32)
+    lo, hi = mesh.reshape(-1, 3).min(0) - margin, mesh.reshape(-1, 3).max(0) + margin
+    shell = np.concatenate([box_mesh(tuple(lo), tuple(hi)), mesh[:, ::-1, :]]).astype(np.float32)
+    return TGeometry([Material("vacuum"), wall], "vacuum", [Solid(2, wall.name, shell)])
+
+
+def uneven_vessel(mesh="storage", n=4000000, seed=4, tmax=200.0, n_grid=64, Bmax=1.0, E_field=1e5, n_tris=200000,
+                  loss=(1e-5, 1e-3)):
+    """Config 4: mixed species with very uneven lifetimes in a vessel of the mesh `mesh`
+    ("guide", "storage": the reference's in/guide_experiment.STL and
+    in/storagevolume_60degfeeder_bellow.STL; "cylinder": a synthetic closed cylinder of about
+    n_tris triangles): 90% neutrons with exponential decay (tau = 880 s) and a loss per
+    bounce drawn per particle from U(loss) (a build extension, Workload.loss), 8% protons
+    (1-800 eV) and 2% electrons (1-700 keV), in a uniform E = E_field V/m along z plus the
+    coil B map (n_grid^3 over the vessel); species proper-time limits 3e-3 s (protons) and
+    1e-5 s (electrons) as in in/config.in.  Walls: 335 neV, 2% Lambert.  Start points are
+    uniform inside the mesh, found with the GPU's TGeometry::GetSolid (needs a GPU)."""
+    from .constants import ELECTRON, PROTON
+    from .fields import uniform_efield
+    from .geometry import read_stl
+    from .queries import solid_at
+    rng = np.random.default_rng(seed)
+    if mesh in REFERENCE_MESHES:
+        _, tris = read_stl(os.path.join(MESH_DIR, REFERENCE_MESHES[mesh]))
+    elif mesh == "cylinder":
+        n_phi = max(12, int(np.sqrt(n_tris / 1.2 + 81) - 9))
+        tris = cylinder_mesh(0.25, 0.5, n_phi, max(4, int(0.6 * n_phi)), 5)
+    else:
+        raise ValueError(f"unknown mesh {mesh}")
+    wall = Material("wall", FermiReal=335.0, DiffProb=0.02)
+    geom = vessel_geometry(tris, wall)
+    lo, hi = tris.reshape(-1, 3).min(0).astype(float), tris.reshape(-1, 3).max(0).astype(float)
+    pos = np.empty((n, 3))
+    k = 0
+    while k < n:
+        p = rng.uniform(lo, hi, size=(max(2 * (n - k), 1024), 3))
+        p = p[solid_at(geom, p, 0.0) == 1]
+        take = min(len(p), n - k)
+        pos[k:k + take] = p[:take]
+        k += take
+    kinds = np.where(rng.uniform(size=n) < 0.9, NEUTRON, np.where(rng.uniform(size=n) < 0.8, PROTON, ELECTRON))
+    kinds = kinds.astype(np.int32)
+    E = np.where(kinds == PROTON, rng.uniform(1, 800, n), np.where(kinds == ELECTRON, rng.uniform(1e3, 7e5, n),
+                                                                  rng.uniform(0, 250e-9, n)))
+    phi = rng.uniform(0, 2 * np.pi, n)
+    theta = np.arccos(rng.uniform(-1, 1, n))
+    y = np.empty((n, 9))
+    spin0 = np.zeros((n, 5))
+    for kd in (NEUTRON, PROTON, ELECTRON):
+        sel = kinds == kd
+        if sel.any():
+            _, yk, _ = make_states(kd, pos[sel], E[sel], phi[sel], theta[sel], 1.0)
+            y[sel] = yk
+    fmap, meta = coil_map(seed, n_grid, lo, hi, Bmax=Bmax)
+    fm = TFieldManager([fmap, uniform_efield(0.0, 0.0, E_field)])
+    species = [SpeciesConfig(NEUTRON, tau=880.0, tmax=9e99), SpeciesConfig(PROTON, tmax=3e-3),
+               SpeciesConfig(ELECTRON, tmax=1e-5)]
+    lpb = np.where(kinds == NEUTRON, rng.uniform(loss[0], loss[1], n), 0.0)
+    meta = dict(meta, mesh=mesh, mesh_triangles=len(tris))
+    return Workload(f"uneven_{mesh}", geom, fm, species, tmax, kinds, np.zeros(n), y, spin0,
+                    np.arange(n, dtype=np.uint64), seed, meta, loss=lpb)
 
 
 def analytic(n=300, seed=6, tmax=0.1, n_grid=16):
