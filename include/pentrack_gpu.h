@@ -357,7 +357,8 @@ int pt_distribution_table(const char *formula, double lo, double hi, int cap, do
 int pt_fields_eval(const pt_fields *f, void *stream, int n, const double *pts, const double *t, double *B,
                    double *dB, double *V, double *E);
 /* TGeometry::GetSolid (src/geometry.cpp:151-175) at n points and times: the ID of the
-   highest-priority solid not ignored at t (1: default solid; -1: too deeply nested) */
+   highest-priority solid not ignored at t (1: default solid; -1: too deeply nested;
+   -2: BVH traversal stack exhausted, which geometries built by pt_geometry_create exclude) */
 int pt_geometry_solid_at(const pt_geometry *g, void *stream, int n, const double *pts, const double *t,
                          int32_t *solid_id);
 /* for tests: the proton energies sampled for the variates p[0..n-1] in [0, 1) */

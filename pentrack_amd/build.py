@@ -71,9 +71,29 @@ def build(verbose=False, jobs=None, out=OUT, defines=()):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("linking libpentrack_gpu.so failed")
     os.replace(out + ".tmp", out)
+    if out == OUT:
+        build_c_example(verbose)
     if verbose:
         print("built", out)
     return out
+
+
+C_EXAMPLE = os.path.join(HERE, "pt_capi_example")
+
+
+def build_c_example(verbose=False):
+    """The plain C caller of the library (csrc/pt_capi_example.c), linked against the
+    in-tree libpentrack_gpu.so with an $ORIGIN run path."""
+    cc = os.environ.get("CC", "gcc")
+    cmd = [cc, "-O2", "-std=c11", "-Wall", "-Wextra", "-o", C_EXAMPLE + ".tmp", os.path.join(CSRC, "pt_capi_example.c"),
+           "-L" + HERE, "-lpentrack_gpu", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building pt_capi_example failed")
+    os.replace(C_EXAMPLE + ".tmp", C_EXAMPLE)
+    if verbose:
+        print("built", C_EXAMPLE)
 
 
 if __name__ == "__main__":

@@ -596,10 +596,14 @@ PTD int seg_query(const Geom &G, const double *p, const double *q, HitList *list
 }
 
 /* +z ray parity per solid (TTriangleMesh::GetSolids, include/trianglemesh.h:181-188):
-   bit m of the result is set if the ray from p crosses solid m an odd number of times */
-PTD_NOINLINE unsigned long long ray_parity(const Geom &G, double p0, double p1, double p2) {
+   bit m of the result is set if the ray from p crosses solid m an odd number of times.
+   *overflow is set (and the parity is not to be used) if the traversal stack ran out;
+   pt_geometry_create rejects BVHs deeper than BVH_STACK - 2, so that does not happen for a
+   geometry built there, but a traversal never drops a subtree silently either. */
+PTD_NOINLINE unsigned long long ray_parity(const Geom &G, double p0, double p1, double p2, int *overflow) {
     const double p[3] = {p0, p1, p2};
     unsigned long long par = 0;
+    *overflow = 0;
     int stack[BVH_STACK];
     int sp = 0, node = 0;
     if (G.n_node == 0) return 0;
@@ -622,7 +626,8 @@ PTD_NOINLINE unsigned long long ray_parity(const Geom &G, double p0, double p1, 
                     if (ray_tri(a, b, c, p)) par ^= (1ull << solid);
                 }
             } else {
-                if (sp < BVH_STACK) stack[sp++] = nd.b;
+                if (sp >= BVH_STACK) { *overflow = 1; return 0; }
+                stack[sp++] = nd.b;
                 node = nd.a;
                 continue;
             }

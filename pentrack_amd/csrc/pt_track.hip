@@ -2256,8 +2256,10 @@ __global__ void __launch_bounds__(256) init_kernel(Params P) {
         for (int k = 0; k < 9; ++k) S.xc[(size_t)k * n + i] = y[k];
         S.ts[i] = x;
         S.dt[i] = 10. * MAX_TRACK_DEVIATION / __builtin_sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
-        unsigned long long par = ray_parity(P.geom, y[0], y[1], y[2]);
-        for (int m = 0; m < P.geom.n_solid; ++m) {
+        int ovf;
+        unsigned long long par = ray_parity(P.geom, y[0], y[1], y[2], &ovf);
+        if (ovf) stop = ID_INVARIANT;
+        for (int m = 0; m < P.geom.n_solid && !ovf; ++m) {
             if ((par >> m) & 1ull) {
                 if (ns >= MAX_SOLIDS) { stop = ID_UNSUPPORTED; break; }
                 sk = sk_set(sk, ns, m, solid_ignored(P.geom, m, x));
@@ -2419,14 +2421,17 @@ __global__ void __launch_bounds__(256) fields_eval_kernel(Params P, int n, const
 }
 
 /* TGeometry::GetSolid (src/geometry.cpp:151-175): the solids containing the point (+z ray
-   parity), the highest ID among those not ignored at t; -1 if more than MAX_SOLIDS nest */
+   parity), the highest ID among those not ignored at t; -1 if more than MAX_SOLIDS nest,
+   -2 if the BVH traversal stack ran out */
 #if PT_COMMON_TU
 __global__ void __launch_bounds__(256) solid_at_kernel(Geom G, int n, const double *pts, const double *t, int *out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     unsigned long long sk = sk_set(0ull, 0, -1, 0);
     int ns = 1;
-    unsigned long long par = ray_parity(G, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+    int ovf;
+    unsigned long long par = ray_parity(G, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], &ovf);
+    if (ovf) { out[i] = -2; return; }
     for (int m = 0; m < G.n_solid; ++m) {
         if ((par >> m) & 1ull) {
             if (ns >= MAX_SOLIDS) { out[i] = -1; return; }
