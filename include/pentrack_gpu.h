@@ -131,7 +131,7 @@ typedef struct pt_fields pt_fields;
 /*
  * Replaces TFieldManager::TFieldManager (src/fields.cpp:19-136): descriptors are
  * kept in the given order (the reference's lexicographic FIELDS key order) and
- * summed in that order.  with_mixed also builds, on the GPU, the mixed-precision
+ * summed in that order (at most 16 containers).  with_mixed also builds, on the GPU, the mixed-precision
  * table of every tricubic map for PT_VARIANT_MIXED: per cell and component the
  * constant and linear coefficients in FP64 and the other 60 in FP32 (288 B instead of
  * 512 B), or the FP64 block where the FP32 remainder's error bound exceeds the map's
@@ -231,7 +231,7 @@ typedef struct pt_batch pt_batch;
 
 /*
  * Uploads n particles (the state TTracker::IntegrateParticle starts from) and
- * allocates their resumable tracker state in device memory.
+ * allocates their resumable tracker state in device memory (at most 8 species).
  * species_idx [n], t0 [n], y0 [n][9], spin0 [n][5], pid [n] (random-stream ids).
  * hit_cap: hits logged per particle (0 = none).
  */

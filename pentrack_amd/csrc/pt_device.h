@@ -21,6 +21,30 @@
 #define PTD_NOINLINE __device__ __noinline__
 #define PTHD __host__ __device__ __forceinline__
 
+/* Loads and stores through a pointer to the library's device allocations, issued as
+   global (not flat) memory instructions: flat accesses also count against the wait
+   counter of LDS and scalar loads, so every LDS read would wait for them too */
+template <class T> PTD T gload(const T *p) { return *(const __attribute__((address_space(1))) T *)(p); }
+template <class T> PTD void gstore(T *p, T v) { *(__attribute__((address_space(1))) T *)(p) = v; }
+/* relaxed device-scope atomic add on global memory: with the address space known the
+   compiler combines the lanes of a wave that add to one address into a single atomic
+   (a flat atomic would be issued lane by lane) */
+template <class T> PTD T gatomic_add(T *p, T v) {
+    return __hip_atomic_fetch_add((__attribute__((address_space(1))) T *)(p), v, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+}
+/* HIP's vector structs copy through a generic reference: load their native vector type */
+typedef double pt_d2v __attribute__((ext_vector_type(2)));
+typedef float pt_f4v __attribute__((ext_vector_type(4)));
+PTD double2 gload(const double2 *p) {
+    const pt_d2v v = *(const __attribute__((address_space(1))) pt_d2v *)(p);
+    return make_double2(v.x, v.y);
+}
+PTD float4 gload(const float4 *p) {
+    const pt_f4v v = *(const __attribute__((address_space(1))) pt_f4v *)(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 namespace ptd {
 
 /* ---- constants (src/globals.cpp:13-32, double-rounded) ------------------- */
@@ -175,15 +199,24 @@ constexpr int BUCKET_BITS = 5;                     /* 32 buckets per axis */
 constexpr int NBUCKET = 1 << (3 * BUCKET_BITS);     /* Morton-ordered spatial buckets */
 constexpr int DENSE = 59;     /* xo[8] dxo[8] k3..k6[32] dxc[8] t t_old pol */
 
+/* field containers and species held in the parameter block itself, so that the kernels
+   read them through the parameter pointer (uniform scalar loads) */
+constexpr int MAX_FIELDS = 16;
+constexpr int MAX_SPECIES = 8;
+
 struct Params {
     Geom geom;
     int n_fields;
     int b_reuse;              /* 0 if a 2D map tabulates B: its value-only B differs from the one with derivatives */
-    const Field *fields;
-    const Species *species;
+    Field fields[MAX_FIELDS];
+    Species species[MAX_SPECIES];
     double tmax_global;
     State st;
     const int *quit;
+    /* the device copy of these parameters, which the kernels read through one pointer
+       argument: a kernel argument struct whose address escapes to a non-inlined function
+       would be copied into per-lane scratch memory and read back through flat loads */
+    const Params *self;
 };
 
 /* ---- small helpers ------------------------------------------------------- */

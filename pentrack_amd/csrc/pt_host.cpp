@@ -449,6 +449,8 @@ int tricubic_on_device(int nx, int ny, int nz, const double *const gh[3], int nc
 
 extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32, pt_fields **out) {
     if (!out || n < 0 || (n > 0 && !descs)) return fail(PT_EINVAL, "pt_fields_create: bad arguments");
+    if (n > ptd::MAX_FIELDS)
+        return fail(PT_EINVAL, "pt_fields_create: at most " + std::to_string(ptd::MAX_FIELDS) + " field containers");
     std::unique_ptr<pt_fields> F(new pt_fields());
     F->n = n;
     F->has_fp32 = with_fp32 != 0;
@@ -1143,6 +1145,7 @@ struct pt_batch {
     int hit_cap = 0;
     int *d_quit = nullptr;
     int *d_count = nullptr;
+    ptd::Params *d_params = nullptr;   /* device copy of P, uploaded before each launch sequence */
     const pt_geometry *geom = nullptr;
     const pt_fields *fields = nullptr;
     ptd::Species *d_species = nullptr;
@@ -1158,6 +1161,8 @@ extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_s
                                const double *y0, const double *spin0, uint64_t seed, const uint64_t *pid,
                                int hit_cap, pt_batch **out) {
     if (!g || !f || !out || n < 0 || n_species <= 0 || !species || hit_cap < 0) return fail(PT_EINVAL, "pt_batch_create: bad arguments");
+    if (n_species > ptd::MAX_SPECIES)
+        return fail(PT_EINVAL, "pt_batch_create: at most " + std::to_string(ptd::MAX_SPECIES) + " species");
     if (n > 0 && (!species_idx || !t0 || !y0 || !spin0 || !pid)) return fail(PT_EINVAL, "pt_batch_create: null particle arrays");
     for (int i = 0; i < n; ++i)
         if (species_idx[i] < 0 || species_idx[i] >= n_species) return fail(PT_EINVAL, "pt_batch_create: species index out of range");
@@ -1234,12 +1239,14 @@ extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_s
     int zero = 0;
     if ((r = B->mem.upload(&B->d_quit, &zero, 1))) return r;
     if ((r = B->mem.upload(&B->d_count, &zero, 1))) return r;
+    if ((r = B->mem.alloc(&B->d_params, 1))) return r;
     ptd::Params &P = B->P;
+    P.self = B->d_params;
     P.geom = g->g;
     P.n_fields = f->n;
     P.b_reuse = f->b_reuse ? 1 : 0;
-    P.fields = f->dev;
-    P.species = B->d_species;
+    for (int k = 0; k < f->n; ++k) P.fields[k] = f->host[k];
+    for (int k = 0; k < n_species; ++k) P.species[k] = B->h_species[k];
     P.tmax_global = tmax_global;
     P.quit = B->d_quit;
     *out = B.release();
@@ -1271,6 +1278,9 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
     hipStream_t s = static_cast<hipStream_t>(stream);
     int q = g_quit.load();
     PT_HIP(hipMemcpyAsync(b->d_quit, &q, sizeof(int), hipMemcpyHostToDevice, s));
+    /* the kernels read the parameters from their device copy (also under graph replay:
+       the captured launches take its address, this copy stays outside the graph) */
+    PT_HIP(hipMemcpyAsync(b->d_params, &b->P, sizeof(ptd::Params), hipMemcpyHostToDevice, s));
     int steps = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : INT_MAX;
     int rounds = cfg->rounds > 0 ? cfg->rounds : 4;
     int coll_grid = std::max(1, std::min((b->n + 255) / 256, 2048));
@@ -1368,6 +1378,7 @@ extern "C" int pt_batch_secondaries(pt_batch *b, void *stream, int cap, pt_secon
         if ((r = b->mem.alloc(&b->d_sec, 2 * (size_t)b->n))) return r;
         if ((r = b->mem.alloc(&b->d_sec_count, 1))) return r;
     }
+    PT_HIP(hipMemcpyAsync(b->d_params, &b->P, sizeof(ptd::Params), hipMemcpyHostToDevice, s));
     PT_HIP(pt_launch_decay(b->P, b->d_beta, b->d_sec, b->d_sec_count, s));
     int n2 = 0;
     PT_HIP(hipMemcpyAsync(&n2, b->d_sec_count, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1398,7 +1409,11 @@ extern "C" int pt_fields_eval(const pt_fields *f, void *stream, int n, const dou
     if (!dB) ddB = nullptr;
     ptd::Params P{};
     P.n_fields = f->n;
-    P.fields = f->dev;
+    for (int k = 0; k < f->n; ++k) P.fields[k] = f->host[k];
+    ptd::Params *dpar;
+    if ((r = mem.alloc(&dpar, 1))) return r;
+    P.self = dpar;
+    PT_HIP(hipMemcpy(dpar, &P, sizeof(ptd::Params), hipMemcpyHostToDevice));
     PT_HIP(pt_launch_fields_eval(P, f->general ? 1 : 0, n, dp, dt, dB_, ddB, dV, dE, s));
     PT_HIP(hipStreamSynchronize(s));
     PT_HIP(hipMemcpy(B, dB_, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));

@@ -55,9 +55,9 @@ PTD bool tric_locate(const Field &f, double x, double y, double z, long &cell, d
         if (!(gs >= 0.0)) k = 0;
         else if (gs >= (double)(n[a] - 2)) k = n[a] - 2;
         else k = (int)gs;
-        double gk = g[k], gk1 = g[k + 1];
-        while (k > 0 && gk > p[a]) { --k; gk1 = gk; gk = g[k]; }
-        while (k < n[a] - 2 && gk1 <= p[a]) { ++k; gk = gk1; gk1 = g[k + 1]; }
+        double gk = gload(g + k), gk1 = gload(g + k + 1);
+        while (k > 0 && gk > p[a]) { --k; gk1 = gk; gk = gload(g + k); }
+        while (k < n[a] - 2 && gk1 <= p[a]) { ++k; gk = gk1; gk1 = gload(g + k + 1); }
         if (!(gk <= p[a] && p[a] < gk1)) return false;
         idx[a] = k;
         dist[a] = gk1 - gk;
@@ -84,7 +84,7 @@ PTD void tric_eval_grad64(const double *__restrict__ a, double x, double y, doub
         const double2 *c2 = reinterpret_cast<const double2 *>(a + 16 * i);
         double c[16];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) { double2 v = c2[q]; c[2 * q] = v.x; c[2 * q + 1] = v.y; }
+        for (int q = 0; q < 8; ++q) { double2 v = gload(c2 + q); c[2 * q] = v.x; c[2 * q + 1] = v.y; }
         double rY = 0., rYd = 0., rYz = 0.;
 #pragma unroll
         for (int j = 3; j >= 0; --j) {
@@ -779,7 +779,7 @@ PTD void dense_calc(const DenseRef &r, double t, double *y) {
     const size_t n = (size_t)r.P->st.n;
     const int i = r.i;
     constexpr double b1 = 35.0 / 384, b3 = 500.0 / 1113, b4 = 125.0 / 192, b5 = -2187.0 / 6784, b6 = 11.0 / 84;
-    const double st_t = ds[56 * n + i], st_told = ds[57 * n + i], polo = ds[58 * n + i];
+    const double st_t = gload(ds + 56 * n + i), st_told = gload(ds + 57 * n + i), polo = gload(ds + 58 * n + i);
     const double dt = (st_t - st_told);
     const double theta = (t - st_told) / dt;
     const double X1 = 5.0 * (2558722523.0 - 31403016.0 * theta) / 11282082432.0;
@@ -803,8 +803,9 @@ PTD void dense_calc(const DenseRef &r, double t, double *y) {
     double x[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-        x[k] = ds[k * n + i] + h1 * ds[(8 + k) * n + i] + h3 * ds[(16 + k) * n + i] + h4 * ds[(24 + k) * n + i] +
-               h5 * ds[(32 + k) * n + i] + h6 * ds[(40 + k) * n + i] + h7 * ds[(48 + k) * n + i];
+        x[k] = gload(ds + k * n + i) + h1 * gload(ds + (8 + k) * n + i) + h3 * gload(ds + (16 + k) * n + i) +
+               h4 * gload(ds + (24 + k) * n + i) + h5 * gload(ds + (32 + k) * n + i) + h6 * gload(ds + (40 + k) * n + i) +
+               h7 * gload(ds + (48 + k) * n + i);
     to9(x, polo + h1 * 0.0 + h3 * 0.0 + h4 * 0.0 + h5 * 0.0 + h6 * 0.0 + h7 * 0.0, y);
 }
 
@@ -1635,46 +1636,46 @@ struct Lane {
 PTD void load_lane(const Params &P, int i, Lane &L) {
     const State &S = P.st;
     const size_t n = (size_t)S.n;
-    L.sidx = S.species[i];
-    L.stop = S.stop[i]; L.flags = S.flags[i]; L.nsolid = S.nsolid[i];
-    L.Nstep = S.Nstep[i];
-    L.nacc = S.n_acc[i]; L.nrej = S.n_rej[i];
-    L.solids = S.solids[i];
-    L.x = S.t[i]; L.dt = S.dt[i]; L.tau = S.tau[i]; L.Hmax = S.Hmax[i];
-    L.pol = S.xc[7 * n + i];
+    L.sidx = gload(S.species + i);
+    L.stop = gload(S.stop + i); L.flags = gload(S.flags + i); L.nsolid = gload(S.nsolid + i);
+    L.Nstep = gload(S.Nstep + i);
+    L.nacc = gload(S.n_acc + i); L.nrej = gload(S.n_rej + i);
+    L.solids = gload(S.solids + i);
+    L.x = gload(S.t + i); L.dt = gload(S.dt + i); L.tau = gload(S.tau + i); L.Hmax = gload(S.Hmax + i);
+    L.pol = gload(S.xc + 7 * n + i);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) L.y[k] = S.y[k * n + i];
+    for (int k = 0; k < 9; ++k) L.y[k] = gload(S.y + k * n + i);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) L.dxc[k] = S.dxc[k * n + i];
-    L.dxc[6] = S.dxc[6 * n + i];
-    L.dxc[7] = S.dxc[8 * n + i];
-    L.Bc[0] = S.Bc[i]; L.Bc[1] = S.Bc[n + i]; L.Bc[2] = S.Bc[2 * n + i];
-    L.Vc = S.Vc[i];
+    for (int k = 0; k < 6; ++k) L.dxc[k] = gload(S.dxc + k * n + i);
+    L.dxc[6] = gload(S.dxc + 6 * n + i);
+    L.dxc[7] = gload(S.dxc + 8 * n + i);
+    L.Bc[0] = gload(S.Bc + i); L.Bc[1] = gload(S.Bc + n + i); L.Bc[2] = gload(S.Bc + 2 * n + i);
+    L.Vc = gload(S.Vc + i);
 }
 
 /* write the lane back (the stepper's xc is y with the stepper's polarisation) */
 PTD void store_lane(const Params &P, int i, const Lane &L) {
     const State &S = P.st;
     const size_t n = (size_t)S.n;
-    S.stop[i] = L.stop; S.flags[i] = L.flags; S.nsolid[i] = L.nsolid;
-    S.Nstep[i] = L.Nstep;
-    S.n_acc[i] = L.nacc; S.n_rej[i] = L.nrej;
-    S.solids[i] = L.solids;
-    S.t[i] = L.x; S.dt[i] = L.dt; S.tau[i] = L.tau; S.Hmax[i] = L.Hmax;
+    gstore(S.stop + i, L.stop); gstore(S.flags + i, L.flags); gstore(S.nsolid + i, L.nsolid);
+    gstore(S.Nstep + i, L.Nstep);
+    gstore(S.n_acc + i, L.nacc); gstore(S.n_rej + i, L.nrej);
+    gstore(S.solids + i, L.solids);
+    gstore(S.t + i, L.x); gstore(S.dt + i, L.dt); gstore(S.tau + i, L.tau); gstore(S.Hmax + i, L.Hmax);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) S.y[k * n + i] = L.y[k];
+    for (int k = 0; k < 9; ++k) gstore(S.y + k * n + i, L.y[k]);
     if (!(L.flags & FL_DETACHED)) {
-        S.ts[i] = L.x;
+        gstore(S.ts + i, L.x);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) S.xc[k * n + i] = (k == 7) ? L.pol : L.y[k];
+        for (int k = 0; k < 9; ++k) gstore(S.xc + k * n + i, (k == 7) ? L.pol : L.y[k]);
     }
 #pragma unroll
-    for (int k = 0; k < 6; ++k) S.dxc[k * n + i] = L.dxc[k];
-    S.dxc[6 * n + i] = L.dxc[6];
-    S.dxc[7 * n + i] = 0.0;
-    S.dxc[8 * n + i] = L.dxc[7];
-    S.Bc[i] = L.Bc[0]; S.Bc[n + i] = L.Bc[1]; S.Bc[2 * n + i] = L.Bc[2];
-    S.Vc[i] = L.Vc;
+    for (int k = 0; k < 6; ++k) gstore(S.dxc + k * n + i, L.dxc[k]);
+    gstore(S.dxc + 6 * n + i, L.dxc[6]);
+    gstore(S.dxc + 7 * n + i, 0.0);
+    gstore(S.dxc + 8 * n + i, L.dxc[7]);
+    gstore(S.Bc + i, L.Bc[0]); gstore(S.Bc + n + i, L.Bc[1]); gstore(S.Bc + 2 * n + i, L.Bc[2]);
+    gstore(S.Vc + i, L.Vc);
 }
 
 /* OnStep of the step path (src/neutron.cpp:267-282, src/proton.cpp:25-33,
@@ -1692,7 +1693,7 @@ PTD void on_step_A(const Params &P, const Species &sp, int i, int &stop, double 
             ki = ki * e;
             ki = ki / hb;
             double l = __builtin_sqrt((y2[0] - y1[0]) * (y2[0] - y1[0]) + (y2[1] - y1[1]) * (y2[1] - y1[1]) + (y2[2] - y1[2]) * (y2[2] - y1[2]));
-            uint64_t key = P.st.key[i], ctr = P.st.ctr[i];
+            uint64_t key = gload(P.st.key + i), ctr = gload(P.st.ctr + i);
             double abspath = -ptm_log(1.0 - ptm_canonical(ptm_draw_u64(key, &ctr))) / (2 * ki);
             P.st.ctr[i] = ctr;
             if (abspath < l) {
@@ -1724,7 +1725,7 @@ PTD bool do_step_hook_A(const Params &P, const Species &sp, Lane &L, int i, cons
     double H = kinetic_energy(sp, y2 + 3) +
                potential_energy<MODE>(P, sp, y2, x2, cached ? L.Bc : nullptr, cached ? &L.Vc : nullptr, c);
     L.Hmax = (H < L.Hmax) ? L.Hmax : H;
-    if (y2temp[7] != y2[7]) P.st.Nspinflip[i] += 1;
+    if (y2temp[7] != y2[7]) gstore(P.st.Nspinflip + i, gload(P.st.Nspinflip + i) + (1));
     L.Nstep++;
     return changed;
 }
@@ -1748,9 +1749,9 @@ PTD bool spin_step(const Params &P, int i, const Species &sp, double x1, double 
     double Babs1 = __builtin_sqrt(B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]);
     double Babs2 = __builtin_sqrt(B2[0] * B2[0] + B2[1] * B2[1] + B2[2] * B2[2]);
     double spin[5];
-    for (int k = 0; k < 5; ++k) spin[k] = S.spin[k * n + i];
+    for (int k = 0; k < 5; ++k) spin[k] = gload(S.spin + k * n + i);
     const double spin_in[3] = {spin[0], spin[1], spin[2]};
-    uint64_t key = S.key[i], ctr = S.ctr[i];
+    uint64_t key = gload(S.key + i), ctr = gload(S.ctr + i);
     double spinProjection;
     bool done = false;
     if (sp.gamma == 0 || Babs2 == 0) {
@@ -1789,18 +1790,18 @@ PTD bool spin_step(const Params &P, int i, const Species &sp, double x1, double 
     bool spinHasFlipped = polo != y[7];
     if (spinProjection != polo) {
         double flipprob = 0.5 * (1 - y[7] * spinProjection);
-        S.noflip[i] *= 1. - flipprob;
+        gstore(S.noflip + i, gload(S.noflip + i) * (1. - flipprob));
         if (sp.flipspin) {
             double prevpol = y[7];
             y[7] = (ptm_canonical(ptm_draw_u64(key, &ctr)) < 0.5 * (spinProjection + 1)) ? 1 : -1;
-            if (y[7] != prevpol) S.Nspinflip[i] += 1;
+            if (y[7] != prevpol) gstore(S.Nspinflip + i, gload(S.Nspinflip + i) + (1));
             if (spinHasFlipped) y[7] *= -1;
         }
     } else if (spinHasFlipped) {
         for (int k = 0; k < 5; ++k) spin[k] = 0;
     }
     S.ctr[i] = ctr;
-    for (int k = 0; k < 5; ++k) S.spin[k * n + i] = spin[k];
+    for (int k = 0; k < 5; ++k) gstore(S.spin + k * n + i, spin[k]);
     (void)spin_in;
     return true;
 }
@@ -1927,12 +1928,12 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
         if (need_dense) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                ds[k * n + i] = xc[k];
-                ds[(8 + k) * n + i] = L.dxc[k];
-                ds[(16 + k) * n + i] = K3(k);
-                ds[(24 + k) * n + i] = K4(k);
-                ds[(32 + k) * n + i] = K5(k);
-                ds[(40 + k) * n + i] = k6[k];
+                gstore(ds + k * n + i, xc[k]);
+                gstore(ds + (8 + k) * n + i, L.dxc[k]);
+                gstore(ds + (16 + k) * n + i, K3(k));
+                gstore(ds + (24 + k) * n + i, K4(k));
+                gstore(ds + (32 + k) * n + i, K5(k));
+                gstore(ds + (40 + k) * n + i, k6[k]);
             }
         }
         derivs8<MODE>(P, sp, xnew, ts + dt, polnew, k7, Bn, Vn, bn, c);
@@ -1986,10 +1987,10 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
     const double tnew = ts + dt_used;
     if (need_dense) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) ds[(48 + k) * n + i] = k7[k];
-        ds[56 * n + i] = tnew;
-        ds[57 * n + i] = told;
-        ds[58 * n + i] = polo;
+        for (int k = 0; k < 8; ++k) gstore(ds + (48 + k) * n + i, k7[k]);
+        gstore(ds + 56 * n + i, tnew);
+        gstore(ds + 57 * n + i, told);
+        gstore(ds + 58 * n + i, polo);
     }
     /* accepted: the FSAL derivative and field move to the new state */
     const bool b1v = (L.flags & FL_BVALID) != 0;
@@ -2019,8 +2020,8 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
     if (!at_end) {
         /* the tracker's end of step is interpolated; the stepper keeps its own */
 #pragma unroll
-        for (int k = 0; k < 9; ++k) P.st.xc[k * n + i] = (k == 7) ? polnew : xnew[k < 7 ? k : 7];
-        P.st.ts[i] = tnew;
+        for (int k = 0; k < 9; ++k) gstore(P.st.xc + k * n + i, (k == 7) ? polnew : xnew[k < 7 ? k : 7]);
+        gstore(P.st.ts + i, tnew);
         L.flags |= FL_DETACHED;
     } else {
         L.flags &= ~FL_DETACHED;
@@ -2056,10 +2057,10 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
             } else if (nh > 0) {
                 /* defer to the collision path: chord piece, step end and stepper state to HBM */
                 double *pd = P.st.pend;
-                pd[i] = xa;
-                pd[10 * n + i] = x2;
+                gstore(pd + i, xa);
+                gstore(pd + 10 * n + i, x2);
 #pragma unroll
-                for (int k = 0; k < 9; ++k) { pd[(1 + k) * n + i] = y1[k]; pd[(11 + k) * n + i] = y2[k]; }
+                for (int k = 0; k < 9; ++k) { gstore(pd + (1 + k) * n + i, y1[k]); gstore(pd + (11 + k) * n + i, y2[k]); }
                 L.x = x;
 #pragma unroll
                 for (int k = 0; k < 9; ++k) L.y[k] = y[k];
@@ -2091,10 +2092,10 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
         /* a spin-tracking window: the collision path finishes the step from its spin step
            (a chord piece of zero length at the step's end) */
         double *pd = P.st.pend;
-        pd[i] = x;
-        pd[10 * n + i] = x;
+        gstore(pd + i, x);
+        gstore(pd + 10 * n + i, x);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { pd[(1 + k) * n + i] = y[k]; pd[(11 + k) * n + i] = y[k]; }
+        for (int k = 0; k < 9; ++k) { gstore(pd + (1 + k) * n + i, y[k]); gstore(pd + (11 + k) * n + i, y[k]); }
         L.x = x;
 #pragma unroll
         for (int k = 0; k < 9; ++k) L.y[k] = y[k];
@@ -2233,7 +2234,8 @@ PTD void collision_path(const Params &P, int i, Ctr &c) {
  * step size, and the solid stack by +z ray parity.
  */
 template <int MODE>
-__global__ void __launch_bounds__(256) init_kernel(Params P) {
+__global__ void __launch_bounds__(256) init_kernel(const Params *__restrict__ Pp) {
+    const Params &P = *Pp;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const State &S = P.st;
     if (i >= S.n || (S.flags[i] & FL_STARTED)) return;
@@ -2286,7 +2288,8 @@ __global__ void __launch_bounds__(256) init_kernel(Params P) {
 #define PT_STEP_MIN_WAVES 2
 #endif
 template <int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STEP_MIN_WAVES))) step_kernel(Params P) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STEP_MIN_WAVES))) step_kernel(const Params *__restrict__ Pp) {
+    const Params &P = *Pp;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const State &S = P.st;
     if (i >= S.n) return;
@@ -2305,7 +2308,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
     S.budget[i] = budget;
     S.n_field[i] += c.field;
     if (pending) {
-        int slot = atomicAdd(S.qcount, 1);
+        int slot = gatomic_add(S.qcount, 1);
         S.queue[slot] = i;
     }
 }
@@ -2318,7 +2321,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
  * when the list is exhausted.
  */
 template <int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STEP_MIN_WAVES))) step_kernel_persistent(Params P) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STEP_MIN_WAVES))) step_kernel_persistent(const Params *__restrict__ Pp) {
+    const Params &P = *Pp;
     const State &S = P.st;
     const int count = S.wcount[0];
     const bool quit = *P.quit != 0;
@@ -2329,7 +2333,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
     for (;;) {
         if (!have) {
             if (quit) break;
-            int w = atomicAdd(S.wcount + 1, 1);
+            int w = gatomic_add(S.wcount + 1, 1);
             if (w >= count) break;
             i = S.work[w];
             budget = S.budget[i];
@@ -2343,9 +2347,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
         if (pending || L.stop != ID_UNKNOWN || budget <= 0) {
             store_lane(P, i, L);
             S.budget[i] = budget;
-            S.n_field[i] += c.field;
+            gstore(S.n_field + i, gload(S.n_field + i) + (c.field));
             if (pending) {
-                int slot = atomicAdd(S.qcount, 1);
+                int slot = gatomic_add(S.qcount, 1);
                 S.queue[slot] = i;
             }
             have = false;
@@ -2355,12 +2359,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
 
 /* work list of the persistent kernel: particles that are running and have steps left */
 #if PT_COMMON_TU
-__global__ void __launch_bounds__(256) work_kernel(Params P) {
+__global__ void __launch_bounds__(256) work_kernel(const Params *__restrict__ Pp) {
+    const Params &P = *Pp;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const State &S = P.st;
     if (i >= S.n) return;
     if (S.stop[i] == ID_UNKNOWN && (S.flags[i] & FL_STARTED) && S.budget[i] > 0) {
-        int w = atomicAdd(S.wcount, 1);
+        int w = gatomic_add(S.wcount, 1);
         S.work[w] = i;
     }
 }
@@ -2378,7 +2383,8 @@ __global__ void reset_count_kernel(int *count) { *count = 0; }
 /* TNeutron::Decay (src/neutron.cpp:285-362) for every neutron stopped with ID_DECAYED, from
    its final state and a copy of its stream position */
 #if PT_COMMON_TU
-__global__ void __launch_bounds__(256) decay_kernel(Params P, pt_beta_table T, pt_secondary *out, int *count) {
+__global__ void __launch_bounds__(256) decay_kernel(const Params *__restrict__ Pp, pt_beta_table T, pt_secondary *out, int *count) {
+    const Params &P = *Pp;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const State &S = P.st;
     if (i >= S.n) return;
@@ -2401,8 +2407,9 @@ __global__ void __launch_bounds__(256) decay_kernel(Params P, pt_beta_table T, p
 
 /* field sum at n points and times, FP64 tables; dB row-major [n][3][3] */
 template <int MODE>
-__global__ void __launch_bounds__(256) fields_eval_kernel(Params P, int n, const double *pts, const double *t, double *B,
+__global__ void __launch_bounds__(256) fields_eval_kernel(const Params *__restrict__ Pp, int n, const double *pts, const double *t, double *B,
                                                           double *dB, double *V, double *E) {
+    const Params &P = *Pp;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     long long nf = 0;
@@ -2466,7 +2473,8 @@ PTD unsigned spread3(unsigned v) {
 /* bucket of each particle with steps to do: its cell of a 32^3 grid over the mesh
    bounding box (the clearance grid's box), in Morton order */
 #if PT_COMMON_TU
-__global__ void __launch_bounds__(256) bucket_count_kernel(Params P) {
+__global__ void __launch_bounds__(256) bucket_count_kernel(const Params *__restrict__ Pp) {
+    const Params &P = *Pp;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const State &S = P.st;
     if (i >= S.n) return;
@@ -2482,8 +2490,8 @@ __global__ void __launch_bounds__(256) bucket_count_kernel(Params P) {
         }
         const int key = (int)(spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2));
         S.pkey[i] = key;
-        atomicAdd(S.bhist + key, 1);
-        atomicAdd(S.wcount, 1);
+        gatomic_add(S.bhist + key, 1);
+        gatomic_add(S.wcount, 1);
     }
 }
 #endif
@@ -2514,12 +2522,13 @@ __global__ void __launch_bounds__(1024) bucket_scan_kernel(int *bhist) {
 
 /* places each particle with steps to do at the next free slot of its bucket */
 #if PT_COMMON_TU
-__global__ void __launch_bounds__(256) bucket_scatter_kernel(Params P) {
+__global__ void __launch_bounds__(256) bucket_scatter_kernel(const Params *__restrict__ Pp) {
+    const Params &P = *Pp;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const State &S = P.st;
     if (i >= S.n) return;
     if (S.stop[i] == ID_UNKNOWN && (S.flags[i] & FL_STARTED) && S.budget[i] > 0) {
-        int slot = atomicAdd(S.bhist + S.pkey[i], 1);
+        int slot = gatomic_add(S.bhist + S.pkey[i], 1);
         S.work[slot] = i;
     }
 }
@@ -2538,7 +2547,8 @@ __global__ void __launch_bounds__(256) bucket_scatter_kernel(Params P) {
 #define PT_COLL_MIN_WAVES 3
 #endif
 template <int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_COLL_MIN_WAVES))) collision_kernel(Params P) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_COLL_MIN_WAVES))) collision_kernel(const Params *__restrict__ Pp) {
+    const Params &P = *Pp;
     const int count = *P.st.qcount;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
         Ctr c = {0, 0, 0};
@@ -2567,28 +2577,28 @@ __global__ void __launch_bounds__(256) budget_kernel(int *budget, int n, int ste
 /* the kernels of one mode (this translation unit is built once per mode) */
 template <int MODE>
 hipError_t pt_launch_init_mode(const ptd::Params &P, int grid, hipStream_t stream) {
-    hipLaunchKernelGGL(ptd::init_kernel<MODE>, dim3(grid), dim3(256), 0, stream, P);
+    hipLaunchKernelGGL(ptd::init_kernel<MODE>, dim3(grid), dim3(256), 0, stream, P.self);
     return hipGetLastError();
 }
 template <int MODE>
 hipError_t pt_launch_step_mode(const ptd::Params &P, int grid, int block, hipStream_t stream) {
-    hipLaunchKernelGGL(ptd::step_kernel<MODE>, dim3(grid), dim3(block), 0, stream, P);
+    hipLaunchKernelGGL(ptd::step_kernel<MODE>, dim3(grid), dim3(block), 0, stream, P.self);
     return hipGetLastError();
 }
 template <int MODE>
 hipError_t pt_launch_step_persistent_mode(const ptd::Params &P, int grid, hipStream_t stream) {
-    hipLaunchKernelGGL(ptd::step_kernel_persistent<MODE>, dim3(grid), dim3(256), 0, stream, P);
+    hipLaunchKernelGGL(ptd::step_kernel_persistent<MODE>, dim3(grid), dim3(256), 0, stream, P.self);
     return hipGetLastError();
 }
 template <int MODE>
 hipError_t pt_launch_collision_mode(const ptd::Params &P, int grid, hipStream_t stream) {
-    hipLaunchKernelGGL(ptd::collision_kernel<MODE>, dim3(grid), dim3(256), 0, stream, P);
+    hipLaunchKernelGGL(ptd::collision_kernel<MODE>, dim3(grid), dim3(256), 0, stream, P.self);
     return hipGetLastError();
 }
 template <int MODE>
 hipError_t pt_launch_fields_eval_mode(const ptd::Params &P, int n, const double *pts, const double *t, double *B,
                                       double *dB, double *V, double *E, hipStream_t stream) {
-    hipLaunchKernelGGL(ptd::fields_eval_kernel<MODE>, dim3((n + 255) / 256), dim3(256), 0, stream, P, n, pts, t, B, dB,
+    hipLaunchKernelGGL(ptd::fields_eval_kernel<MODE>, dim3((n + 255) / 256), dim3(256), 0, stream, P.self, n, pts, t, B, dB,
                        V, E);
     return hipGetLastError();
 }
@@ -2646,12 +2656,12 @@ hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, i
         static_assert(ptd::NBUCKET % 1024 == 0, "bucket scan layout");
         hipLaunchKernelGGL(ptd::clear_buckets_kernel, dim3(ptd::NBUCKET / 256 + 1), dim3(256), 0, stream, P.st.bhist,
                            P.st.wcount);
-        hipLaunchKernelGGL(ptd::bucket_count_kernel, dim3(wgrid), dim3(256), 0, stream, P);
+        hipLaunchKernelGGL(ptd::bucket_count_kernel, dim3(wgrid), dim3(256), 0, stream, P.self);
         hipLaunchKernelGGL(ptd::bucket_scan_kernel, dim3(1), dim3(1024), 0, stream, P.st.bhist);
-        hipLaunchKernelGGL(ptd::bucket_scatter_kernel, dim3(wgrid), dim3(256), 0, stream, P);
+        hipLaunchKernelGGL(ptd::bucket_scatter_kernel, dim3(wgrid), dim3(256), 0, stream, P.self);
     } else {
         hipLaunchKernelGGL(ptd::reset_work_kernel, dim3(1), dim3(1), 0, stream, P.st.wcount);
-        hipLaunchKernelGGL(ptd::work_kernel, dim3(wgrid), dim3(256), 0, stream, P);
+        hipLaunchKernelGGL(ptd::work_kernel, dim3(wgrid), dim3(256), 0, stream, P.self);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -2679,7 +2689,7 @@ hipError_t pt_launch_solid_at(const ptd::Geom &G, int n, const double *pts, cons
 hipError_t pt_launch_decay(const ptd::Params &P, pt_beta_table T, pt_secondary *out, int *count, hipStream_t stream) {
     hipLaunchKernelGGL(ptd::reset_count_kernel, dim3(1), dim3(1), 0, stream, count);
     if (P.st.n > 0)
-        hipLaunchKernelGGL(ptd::decay_kernel, dim3((P.st.n + 255) / 256), dim3(256), 0, stream, P, T, out, count);
+        hipLaunchKernelGGL(ptd::decay_kernel, dim3((P.st.n + 255) / 256), dim3(256), 0, stream, P.self, T, out, count);
     return hipGetLastError();
 }
 
