@@ -54,12 +54,18 @@ VARIANTS = {
     # work list handed out in Morton order of the particle positions
     "mixed_persistent_spatial": dict(mixed=True, persistent=1, graph=0, spatial=1),
     "fp64_persistent_spatial": dict(mixed=False, persistent=1, graph=0, spatial=1),
+    # Morton-ordered work list split into one chunk per XCD
+    "fp64_persistent_xcd": dict(mixed=False, persistent=1, graph=0, spatial=2),
+    "fp64_persistent_xcd_graph": dict(mixed=False, persistent=1, graph=1, spatial=2),
+    "mixed_persistent_xcd": dict(mixed=True, persistent=1, graph=0, spatial=2),
 }
 # graph replays carry no per-kernel events: the roofline uses the same kernels timed eagerly
-PROFILED_FOR = {"fp64_persistent_graph": "fp64_persistent", "mixed_persistent_graph": "mixed_persistent"}
+PROFILED_FOR = {"fp64_persistent_graph": "fp64_persistent", "mixed_persistent_graph": "mixed_persistent",
+                "fp64_persistent_xcd_graph": "fp64_persistent_xcd"}
 KERNEL_NAMES = {"fp64": "step_kernel_fp64", "mixed": "step_kernel_mixed",
                 "fp64_persistent": "step_kernel_persistent_fp64",
                 "fp64_persistent_spatial": "step_kernel_persistent_fp64_spatial",
+                "fp64_persistent_xcd": "step_kernel_persistent_fp64_xcd",
                 "mixed_persistent": "step_kernel_persistent_mixed",
                 "mixed_persistent_spatial": "step_kernel_persistent_mixed_spatial"}
 MIX_BLOCK, FP64_BLOCK = 288, 512

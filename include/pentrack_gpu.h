@@ -224,7 +224,10 @@ typedef struct {
                                     and replay it (needs a non-default stream; no per-kernel timing) */
     int spatial_order;           /* persistent kernel: 1 hands out particles in Morton order of their
                                     position (32^3 buckets over the mesh box) so concurrent waves
-                                    gather nearby field cells; 0 in index order.  Results are equal. */
+                                    gather nearby field cells; 2 also splits that list into one
+                                    contiguous chunk per XCD, taken first by the blocks on that XCD
+                                    (neighbouring particles share its L2); 0 in index order.
+                                    Results are equal. */
 } pt_launch_cfg;
 
 typedef struct pt_batch pt_batch;

@@ -190,13 +190,18 @@ struct State {
     int *queue;               /* indices of particles parked on a wall collision */
     int *qcount;              /* queue length */
     int *work;                /* persistent variant: indices of particles with steps to do */
-    int *wcount;              /* [0] work list length, [1] next work item */
+    int *wcount;              /* [0] work list length, [1] next work item; [WC_CHUNK0 + WC_STRIDE c]:
+                                 next item of chunk c of the list split by XCD */
     int *pkey;                /* spatial bucket of each particle (work-list ordering) */
     int *bhist;               /* [NBUCKET + 1] bucket counts, then running offsets */
 };
 
 constexpr int BUCKET_BITS = 5;                     /* 32 buckets per axis */
 constexpr int NBUCKET = 1 << (3 * BUCKET_BITS);     /* Morton-ordered spatial buckets */
+/* work list split into NXCD contiguous chunks, chunk c taken first by the blocks that run
+   on XCD c (workgroups are dispatched to the XCDs round robin, block b on XCD b % NXCD);
+   each chunk's counter sits in its own 64-byte line */
+constexpr int NXCD = 8, WC_CHUNK0 = 16, WC_STRIDE = 16, WCOUNT_LEN = WC_CHUNK0 + NXCD * WC_STRIDE;
 constexpr int DENSE = 59;     /* xo[8] dxo[8] k3..k6[32] dxc[8] t t_old pol */
 
 /* field containers and species held in the parameter block itself, so that the kernels
@@ -698,10 +703,15 @@ PTD int solid_ignored(const Geom &G, int sidx, double t) {
 }
 /* TTracker::GetCurrentsolid (std::max_element with the reference comparator) */
 PTD int current_solid(const Geom &G, unsigned long long s, int n) {
+    if (n <= 1) return sk_sidx(s, 0);
     int best = 0;
-    for (int i = 1; i < n; ++i) {
-        int ai = sk_ign(s, best), bi = sk_ign(s, i);
-        if (ai || (!bi && solid_ID(G, sk_sidx(s, best)) < solid_ID(G, sk_sidx(s, i)))) best = i;
+    unsigned best_id = solid_ID(G, sk_sidx(s, 0));
+#pragma unroll
+    for (int i = 1; i < MAX_SOLIDS; ++i) {
+        if (i < n) {
+            const unsigned id = solid_ID(G, sk_sidx(s, i));
+            if (sk_ign(s, best) || (!sk_ign(s, i) && best_id < id)) { best = i; best_id = id; }
+        }
     }
     return sk_sidx(s, best);
 }

@@ -1233,7 +1233,8 @@ extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_s
     if ((r = B->mem.alloc(&S.queue, N))) return r;
     if ((r = B->mem.upload(&S.qcount, izero.data(), 1))) return r;
     if ((r = B->mem.alloc(&S.work, N))) return r;
-    if ((r = B->mem.upload(&S.wcount, izero.data(), 2))) return r;
+    const std::vector<int> wzero(ptd::WCOUNT_LEN, 0);
+    if ((r = B->mem.upload(&S.wcount, wzero.data(), wzero.size()))) return r;
     if ((r = B->mem.alloc(&S.pkey, N))) return r;
     if ((r = B->mem.alloc(&S.bhist, (size_t)ptd::NBUCKET + 1))) return r;
     int zero = 0;
@@ -1263,6 +1264,7 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
         for (int k = 0; k < b->fields->n; ++k) any_map |= b->fields->host[k].type == PT_FIELD_TRICUBIC;
         if (any_map) return fail(PT_EINVAL, "pt_batch_run: mixed precision needs fields created with FP32 tables");
     }
+    if (cfg->spatial_order < 0 || cfg->spatial_order > 2) return fail(PT_EINVAL, "pt_batch_run: spatial_order must be 0, 1 or 2");
     int block = cfg->block > 0 ? cfg->block : 256;
     if (block % 64 != 0 || block > 256) return fail(PT_EINVAL, "pt_batch_run: block must be a multiple of 64 and <= 256");
     {

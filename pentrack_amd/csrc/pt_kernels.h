@@ -22,7 +22,7 @@ namespace ptd { struct Params; struct Geom; }
 /* the mode-templated launches, each instantiated in the translation unit of its mode */
 template <int MODE> hipError_t pt_launch_init_mode(const ptd::Params &P, int grid, hipStream_t stream);
 template <int MODE> hipError_t pt_launch_step_mode(const ptd::Params &P, int grid, int block, hipStream_t stream);
-template <int MODE> hipError_t pt_launch_step_persistent_mode(const ptd::Params &P, int grid, hipStream_t stream);
+template <int MODE> hipError_t pt_launch_step_persistent_mode(const ptd::Params &P, int grid, int split, hipStream_t stream);
 template <int MODE> hipError_t pt_launch_collision_mode(const ptd::Params &P, int grid, hipStream_t stream);
 template <int MODE>
 hipError_t pt_launch_fields_eval_mode(const ptd::Params &P, int n, const double *pts, const double *t, double *B,

@@ -42,7 +42,23 @@ namespace ptd {
 /* fields                                                                    */
 /* ------------------------------------------------------------------------ */
 
-/* TabField3::Interpolate cell search (std::upper_bound semantics, src/field_3d.cpp:490-500) */
+/* The cell k of grid g (n strictly increasing nodes) with g[k] <= p < g[k+1], or -1 when
+   p lies outside [g[0], g[n-1]) or is NaN: a binary search, for the rare points where the
+   uniform-spacing guess of tric_locate is off (rounding at a cell edge, a non-uniform grid) */
+PTD_NOINLINE int grid_cell_search(const double *g, int n, double p) {
+    if (!(g[0] <= p && p < g[n - 1])) return -1;
+    int lo = 0, hi = n - 1;   /* g[lo] <= p < g[hi] */
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (g[mid] <= p) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+/* TabField3::Interpolate cell search (std::upper_bound semantics, src/field_3d.cpp:490-500):
+   the unique cell whose nodes bracket the point, from the uniform-spacing guess or, when
+   the guess does not bracket it, from grid_cell_search */
 PTD bool tric_locate(const Field &f, double x, double y, double z, long &cell, double r[3], double dist[3]) {
     double p[3] = {x, y, z};
     int n[3] = {f.nx, f.ny, f.nz};
@@ -56,9 +72,12 @@ PTD bool tric_locate(const Field &f, double x, double y, double z, long &cell, d
         else if (gs >= (double)(n[a] - 2)) k = n[a] - 2;
         else k = (int)gs;
         double gk = gload(g + k), gk1 = gload(g + k + 1);
-        while (k > 0 && gk > p[a]) { --k; gk1 = gk; gk = gload(g + k); }
-        while (k < n[a] - 2 && gk1 <= p[a]) { ++k; gk = gk1; gk1 = gload(g + k + 1); }
-        if (!(gk <= p[a] && p[a] < gk1)) return false;
+        if (__builtin_expect(!(gk <= p[a] && p[a] < gk1), 0)) {
+            k = grid_cell_search(g, n[a], p[a]);
+            if (k < 0) return false;
+            gk = gload(g + k);
+            gk1 = gload(g + k + 1);
+        }
         idx[a] = k;
         dist[a] = gk1 - gk;
         r[a] = (p[a] - gk) / dist[a];
@@ -1626,6 +1645,7 @@ constexpr int FL_STARTED = 1, FL_DERIV = 2, FL_RESET = 4, FL_BVALID = 8, FL_DETA
  */
 struct Lane {
     int sidx, stop, flags, nsolid, Nstep;
+    int tries;                  /* trials of the current step so far; -1 before its first */
     long long nacc, nrej;
     unsigned long long solids;
     double x, dt, pol, tau, Hmax;
@@ -1651,6 +1671,7 @@ PTD void load_lane(const Params &P, int i, Lane &L) {
     L.dxc[7] = gload(S.dxc + 8 * n + i);
     L.Bc[0] = gload(S.Bc + i); L.Bc[1] = gload(S.Bc + n + i); L.Bc[2] = gload(S.Bc + 2 * n + i);
     L.Vc = gload(S.Vc + i);
+    L.tries = -1;
 }
 
 /* write the lane back (the stepper's xc is y with the stepper's polarisation) */
@@ -1815,27 +1836,42 @@ constexpr int STAGE_LDS = 4, STAGE_LANES = 256;
 __shared__ double stage_lds[STAGE_LDS][8][STAGE_LANES];
 static_assert(sizeof(stage_lds) == PT_STEP_LDS_BYTES, "the host checks PT_STEP_LDS_BYTES against the device");
 
+/* An accepted dopri5 trial step, handed from trialA to acceptA: the stepper's state at
+   the step start (xc, ts), the new state and its derivative and field, and whether the
+   rest of the step can need the dense output (then the stage vectors are in HBM). */
+struct Trial {
+    double xc[8], ts, xnew[8], k7[8], Bn[3], Vn, polnew, dt_used;
+    bool bn, need_dense;
+};
+
 /*
- * One step of TTracker::IntegrateParticle (src/tracking.cpp:71-148) on the step path.
- * The dopri5 trial keeps its stage vectors in registers; the dense output is written to
- * HBM only when the step can need it (truncation, chord splitting, bulk absorption, or
- * a chord that comes near a wall).  Returns pending = true, with the chord piece in HBM,
- * when a chord piece intersects the mesh; the collision path then finishes the step.
+ * One trial step of odeint's controlled dopri5 (dense_output_runge_kutta::do_step,
+ * src/tracking.cpp:71-93) on the step path: the stage vectors, the error norm and the
+ * step-size controller.  Returns 0 when the trial is rejected (L.dt shrunk), 1 when it is
+ * accepted (T filled, L.dt grown), 2 when the stepper fails (L.stop set).  The step
+ * kernels call it once per loop iteration for every lane and run acceptA only on the
+ * lanes whose trial was accepted, so a wave does not repeat trials until its last lane
+ * accepts: each lane's arithmetic is unchanged, only the lanes' schedule differs.
+ * L.tries < 0 marks the first trial of a step.
  */
 template <int MODE>
-PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
+PTD int trialA(const Params &P, int i, Lane &L, Trial &T, Ctr &c) {
     const Geom &G = P.geom;
     const Species &sp = P.species[L.sidx];
     const double tmax = P.tmax_global;
     const size_t n = (size_t)P.st.n;
     double *ds = P.st.dense;
-    if (L.flags & FL_RESET) {
-        /* stepper.initialize(y, x, current_time_step()) (src/tracking.cpp:72-74) */
-        L.pol = L.y[7];
-        L.flags &= ~(FL_DERIV | FL_DETACHED);
+    if (L.tries < 0) {
+        if (L.flags & FL_RESET) {
+            /* stepper.initialize(y, x, current_time_step()) (src/tracking.cpp:72-74) */
+            L.pol = L.y[7];
+            L.flags &= ~(FL_DERIV | FL_DETACHED);
+        }
+        L.tries = 0;
     }
     /* the stepper starts from its own state: the tracker's unless detached */
-    double xc[8], ts;
+    double *xc = T.xc;
+    double ts;
     if (L.flags & FL_DETACHED) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) xc[k] = P.st.xc[(k < 7 ? k : 8) * n + i];
@@ -1845,6 +1881,7 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
         for (int k = 0; k < 8; ++k) xc[k] = L.y[k < 7 ? k : 8];
         ts = L.x;
     }
+    T.ts = ts;
     if (!(L.flags & FL_DERIV)) {
         bool hb;
         derivs8<MODE>(P, sp, xc, ts, L.pol, L.dxc, L.Bc, L.Vc, hb, c);
@@ -1853,7 +1890,6 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
     }
     const int cursolid = current_solid(G, L.solids, L.nsolid);
     const bool absorb = sp.kind == 0 && solid_mat(G, cursolid)[1] > 0;
-    const double x1 = L.x;
 
     /* ---- dense_output_runge_kutta::do_step (controlled dopri5, abs = rel = 1e-9) ---- */
     constexpr double b21 = 1.0 / 5;
@@ -1864,16 +1900,17 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
     constexpr double c1 = 35.0 / 384, c3 = 500.0 / 1113, c4 = 125.0 / 192, c5 = -2187.0 / 6784, c6 = 11.0 / 84;
     constexpr double dc1 = c1 - 5179.0 / 57600, dc3 = c3 - 7571.0 / 16695, dc4 = c4 - 393.0 / 640,
                      dc5 = c5 - (-92097.0 / 339200), dc6 = c6 - 187.0 / 2100, dc7 = -1.0 / 40;
-    double xnew[8], k7[8], Bn[3], Vn, polnew, dt_used;
-    bool bn, need_dense;
-    int tries = 0;
+    double *xnew = T.xnew, *k7 = T.k7;
     /* this lane's slices of the stage vectors k2..k5 */
     const int lane = threadIdx.x;
 #define K2(k) stage_lds[0][k][lane]
 #define K3(k) stage_lds[1][k][lane]
 #define K4(k) stage_lds[2][k][lane]
 #define K5(k) stage_lds[3][k][lane]
-    for (;;) {
+    /* a compiler-only barrier after each stage's stores: later uses read LDS, so the
+       compiler cannot keep the stage vectors in registers by forwarding its own stores */
+#define STAGE_STORED() asm volatile("" ::: "memory")
+    {
         const double dt = L.dt;
         double tmp[8], kst[8], k6[8], Bd[3], Vd;
         bool bd;
@@ -1883,24 +1920,28 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
         derivs8<MODE>(P, sp, tmp, ts + dt * (1.0 / 5), L.pol, kst, Bd, Vd, bd, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) K2(k) = kst[k];
+        STAGE_STORED();
         const double h31 = dt * b31, h32 = dt * b32;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h31 * L.dxc[k] + h32 * K2(k);
         derivs8<MODE>(P, sp, tmp, ts + dt * (3.0 / 10), L.pol, kst, Bd, Vd, bd, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) K3(k) = kst[k];
+        STAGE_STORED();
         const double h41 = dt * b41, h42 = dt * b42, h43 = dt * b43;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h41 * L.dxc[k] + h42 * K2(k) + h43 * K3(k);
         derivs8<MODE>(P, sp, tmp, ts + dt * (4.0 / 5), L.pol, kst, Bd, Vd, bd, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) K4(k) = kst[k];
+        STAGE_STORED();
         const double h51 = dt * b51, h52 = dt * b52, h53 = dt * b53, h54 = dt * b54;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h51 * L.dxc[k] + h52 * K2(k) + h53 * K3(k) + h54 * K4(k);
         derivs8<MODE>(P, sp, tmp, ts + dt * (8.0 / 9), L.pol, kst, Bd, Vd, bd, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) K5(k) = kst[k];
+        STAGE_STORED();
         const double h61 = dt * b61, h62 = dt * b62, h63 = dt * b63, h64 = dt * b64, h65 = dt * b65;
 #pragma unroll
         for (int k = 0; k < 8; ++k)
@@ -1909,7 +1950,7 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
         const double g1 = dt * c1, g3 = dt * c3, g4 = dt * c4, g5 = dt * c5, g6 = dt * c6;
 #pragma unroll
         for (int k = 0; k < 8; ++k) xnew[k] = xc[k] + g1 * L.dxc[k] + g3 * K3(k) + g4 * K4(k) + g5 * K5(k) + g6 * k6[k];
-        polnew = L.pol + g1 * 0.0 + g3 * 0.0 + g4 * 0.0 + g5 * 0.0 + g6 * 0.0;
+        T.polnew = L.pol + g1 * 0.0 + g3 * 0.0 + g4 * 0.0 + g5 * 0.0 + g6 * 0.0;
         const double e1 = dt * dc1, e3 = dt * dc3, e4 = dt * dc4, e5 = dt * dc5, e6 = dt * dc6, e7 = dt * dc7;
         double errp[8];
 #pragma unroll
@@ -1921,11 +1962,11 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
             bool split = 0.25 * (l2 - d2) > MAX_TRACK_DEVIATION * MAX_TRACK_DEVIATION;
             bool trunc = (L.tau > 0 && xnew[6] > L.tau) || (ts + dt > tmax);
             /* detached: the chord starts at the tracker's point, not at xc */
-            need_dense = trunc || split || absorb || (L.flags & FL_DETACHED) || !clear_of_walls(G, xc, xnew) ||
+            T.need_dense = trunc || split || absorb || (L.flags & FL_DETACHED) || !clear_of_walls(G, xc, xnew) ||
                          (P.st.snap_n > 0 && snapshot_in(P.st, ts, ts + dt) >= 0) ||
                          (sp.n_spintimes > 0 && (spin_window(sp, ts) || spin_window(sp, ts + dt)));
         }
-        if (need_dense) {
+        if (T.need_dense) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 gstore(ds + k * n + i, xc[k]);
@@ -1936,7 +1977,7 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
                 gstore(ds + (40 + k) * n + i, k6[k]);
             }
         }
-        derivs8<MODE>(P, sp, xnew, ts + dt, polnew, k7, Bn, Vn, bn, c);
+        derivs8<MODE>(P, sp, xnew, ts + dt, T.polnew, k7, T.Bn, T.Vn, T.bn, c);
         double err = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -1951,7 +1992,7 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
             L.dt = dt * ((f < m) ? m : f);
             ++L.nrej;
         } else {
-            dt_used = dt;
+            T.dt_used = dt;
             if (err < 0.5) {
                 double emin = ptm_pow(5.0, -5.0);
                 err = (emin < err) ? err : emin;
@@ -1959,29 +2000,55 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
             }
             ++L.nacc;
         }
-        if (tries++ >= 500) {
+        if (L.tries++ >= 500) {
             /* the failed-step checker throws; IntegrateParticle keeps x, y and stops (src/tracking.cpp:78-85) */
             L.stop = ID_ODEINT_ERROR;
-            return;
+            return 2;
         }
-        if (ok) break;
+        if (!ok) return 0;
     }
 #undef K2
 #undef K3
 #undef K4
 #undef K5
+#undef STAGE_STORED
     {
         /* an accepted step with a non-finite state (a stage beyond the speed of light makes
            1/gamma a NaN, which the maximum of the error norm passes over) stops as a stepper
            failure, x and y kept at the step start (src/tracking.cpp:78-85) */
-        bool finite = polnew == polnew;
+        bool finite = T.polnew == T.polnew;
 #pragma unroll
         for (int k = 0; k < 8; ++k) finite = finite && (dabs(xnew[k]) <= DBL_MAX_V);
         if (!finite) {
             L.stop = ID_ODEINT_ERROR;
-            return;
+            return 2;
         }
     }
+    L.tries = -1;
+    return 1;
+}
+
+/*
+ * The rest of one step of TTracker::IntegrateParticle (src/tracking.cpp:94-148) on the
+ * step path, after trialA accepted a trial: decay/tmax truncation through the dense
+ * output (written to HBM only when the step can need it: truncation, chord splitting,
+ * bulk absorption, or a chord that comes near a wall), chord splitting and the
+ * collision test of each piece, spin step, logs and stop checks.  Returns pending =
+ * true, with the chord piece in HBM, when a chord piece intersects the mesh; the
+ * collision path then finishes the step.
+ */
+template <int MODE>
+PTD void acceptA(const Params &P, int i, Lane &L, bool &pending, Ctr &c, const Trial &T) {
+    const Geom &G = P.geom;
+    const Species &sp = P.species[L.sidx];
+    const double tmax = P.tmax_global;
+    const size_t n = (size_t)P.st.n;
+    double *ds = P.st.dense;
+    const int cursolid = current_solid(G, L.solids, L.nsolid);
+    const double x1 = L.x;
+    const double *xc = T.xc, *xnew = T.xnew, *k7 = T.k7;
+    const double ts = T.ts, polnew = T.polnew, dt_used = T.dt_used;
+    const bool need_dense = T.need_dense;
     const double polo = L.pol;
     const double told = ts;
     const double tnew = ts + dt_used;
@@ -1998,8 +2065,8 @@ PTD void stepA(const Params &P, int i, Lane &L, bool &pending, Ctr &c) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) L.dxc[k] = k7[k];
     L.pol = polnew;
-    L.Bc[0] = Bn[0]; L.Bc[1] = Bn[1]; L.Bc[2] = Bn[2]; L.Vc = Vn;
-    L.flags = bn ? (L.flags | FL_BVALID) : (L.flags & ~FL_BVALID);
+    L.Bc[0] = T.Bn[0]; L.Bc[1] = T.Bn[1]; L.Bc[2] = T.Bn[2]; L.Vc = T.Vn;
+    L.flags = T.bn ? (L.flags | FL_BVALID) : (L.flags & ~FL_BVALID);
 
     double x = tnew, y[9], y1[9];
     to9(xnew, polnew, y);
@@ -2301,8 +2368,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
     Ctr c = {0, 0, 0};
     bool pending = false;
     do {
-        stepA<MODE>(P, i, L, pending, c);
-        --budget;
+        Trial T;
+        const int r = trialA<MODE>(P, i, L, T, c);
+        if (r == 1) acceptA<MODE>(P, i, L, pending, c, T);
+        if (r != 0) --budget;
     } while (!pending && L.stop == ID_UNKNOWN && budget > 0);
     store_lane(P, i, L);
     S.budget[i] = budget;
@@ -2318,10 +2387,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
  * next particle from the work list (the particles with steps to do, compacted by
  * work_kernel) whenever its particle parks, stops or spends its step budget, so waves
  * stay full while particles of very different lifetimes drop out.  Every lane leaves
- * when the list is exhausted.
+ * when the list is exhausted.  With split set the list (in Morton order of the particle
+ * positions) is taken as NXCD chunks: a block takes from the chunk of its XCD first, so
+ * the particles an XCD steps at one time are neighbours and share field-map cells in its
+ * L2; when its chunk is done it helps with the next ones.
  */
 template <int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STEP_MIN_WAVES))) step_kernel_persistent(const Params *__restrict__ Pp) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STEP_MIN_WAVES))) step_kernel_persistent(const Params *__restrict__ Pp, int split) {
     const Params &P = *Pp;
     const State &S = P.st;
     const int count = S.wcount[0];
@@ -2333,8 +2405,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
     for (;;) {
         if (!have) {
             if (quit) break;
-            int w = gatomic_add(S.wcount + 1, 1);
-            if (w >= count) break;
+            int w = -1;
+            if (split) {
+                const int x0 = (int)(blockIdx.x % NXCD);
+                for (int d = 0; d < NXCD && w < 0; ++d) {
+                    const int ch = (x0 + d) % NXCD;
+                    const int lo = (int)((long long)count * ch / NXCD), hi = (int)((long long)count * (ch + 1) / NXCD);
+                    if (gload(S.wcount + WC_CHUNK0 + WC_STRIDE * ch) + lo >= hi) continue;
+                    const int k = gatomic_add(S.wcount + WC_CHUNK0 + WC_STRIDE * ch, 1);
+                    if (lo + k < hi) w = lo + k;
+                }
+                if (w < 0) break;
+            } else {
+                w = gatomic_add(S.wcount + 1, 1);
+                if (w >= count) break;
+            }
             i = S.work[w];
             budget = S.budget[i];
             load_lane(P, i, L);
@@ -2342,7 +2427,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_STE
             pending = false;
             have = true;
         }
-        stepA<MODE>(P, i, L, pending, c);
+        /* one trial per iteration on every lane; the accepted lanes finish their step */
+        Trial T;
+        const int r = trialA<MODE>(P, i, L, T, c);
+        if (r == 0) continue;
+        if (r == 1) acceptA<MODE>(P, i, L, pending, c, T);
         --budget;
         if (pending || L.stop != ID_UNKNOWN || budget <= 0) {
             store_lane(P, i, L);
@@ -2458,7 +2547,7 @@ __global__ void __launch_bounds__(256) solid_at_kernel(Geom G, int n, const doub
 __global__ void __launch_bounds__(256) clear_buckets_kernel(int *bhist, int *wcount) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b <= NBUCKET) bhist[b] = 0;
-    if (b == 0) { wcount[0] = 0; wcount[1] = 0; }
+    if (b < WCOUNT_LEN) wcount[b] = 0;
 }
 #endif
 
@@ -2586,8 +2675,8 @@ hipError_t pt_launch_step_mode(const ptd::Params &P, int grid, int block, hipStr
     return hipGetLastError();
 }
 template <int MODE>
-hipError_t pt_launch_step_persistent_mode(const ptd::Params &P, int grid, hipStream_t stream) {
-    hipLaunchKernelGGL(ptd::step_kernel_persistent<MODE>, dim3(grid), dim3(256), 0, stream, P.self);
+hipError_t pt_launch_step_persistent_mode(const ptd::Params &P, int grid, int split, hipStream_t stream) {
+    hipLaunchKernelGGL(ptd::step_kernel_persistent<MODE>, dim3(grid), dim3(256), 0, stream, P.self, split);
     return hipGetLastError();
 }
 template <int MODE>
@@ -2608,7 +2697,7 @@ hipError_t pt_launch_fields_eval_mode(const ptd::Params &P, int n, const double 
 template hipError_t pt_launch_init_mode<PT_MODE_TU>(const ptd::Params &, int, hipStream_t);
 template hipError_t pt_launch_step_mode<PT_MODE_TU>(const ptd::Params &, int, int, hipStream_t);
 #elif PT_KERNEL_SET == 2
-template hipError_t pt_launch_step_persistent_mode<PT_MODE_TU>(const ptd::Params &, int, hipStream_t);
+template hipError_t pt_launch_step_persistent_mode<PT_MODE_TU>(const ptd::Params &, int, int, hipStream_t);
 #else
 template hipError_t pt_launch_collision_mode<PT_MODE_TU>(const ptd::Params &, int, hipStream_t);
 #if (PT_MODE_TU & 1) == 0 /* fields_eval runs on FP64 tables only */
@@ -2665,11 +2754,12 @@ hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, i
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    const int split = spatial >= 2;
     switch (mode) {
-    case 0: return pt_launch_step_persistent_mode<0>(P, grid, stream);
-    case 1: return pt_launch_step_persistent_mode<1>(P, grid, stream);
-    case 2: return pt_launch_step_persistent_mode<2>(P, grid, stream);
-    default: return pt_launch_step_persistent_mode<3>(P, grid, stream);
+    case 0: return pt_launch_step_persistent_mode<0>(P, grid, split, stream);
+    case 1: return pt_launch_step_persistent_mode<1>(P, grid, split, stream);
+    case 2: return pt_launch_step_persistent_mode<2>(P, grid, split, stream);
+    default: return pt_launch_step_persistent_mode<3>(P, grid, split, stream);
     }
 }
 

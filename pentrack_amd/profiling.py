@@ -34,6 +34,8 @@ KERNELS = {
     "step_kernel_mixed": (1, 0, "step_kernel<1>", 0),
     "step_kernel_persistent_mixed": (1, 1, "step_kernel_persistent<1>", 0),
     "step_kernel_persistent_mixed_spatial": (1, 1, "step_kernel_persistent<1>", 1),
+    # Morton-ordered work list split into one chunk per XCD
+    "step_kernel_persistent_fp64_xcd": (0, 1, "step_kernel_persistent<0>", 2),
 }
 # the FP64 VALU instruction counters of a third pass (wave-level instruction counts)
 FLOP_COUNTERS = ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64")

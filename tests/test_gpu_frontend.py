@@ -29,6 +29,32 @@ def test_field_and_solid_queries_match_oracle(gpu, oracle_lib):
         assert sid[i] == (max(live) if live else 1), i
 
 
+def test_cell_search_on_non_uniform_grid_matches_oracle(gpu, oracle_lib):
+    """Points whose uniform-spacing cell guess misses (a strongly non-uniform grid), points
+    on interior nodes and on both grid edges, outside the grid and NaN: the GPU cell search
+    falls back to its binary search and must return the oracle's cells, so B, dB, V and E
+    are bit-exact."""
+    from pentrack_amd.fields import TFieldManager, tabfield3
+    xg = np.array([-0.2, -0.19, -0.15, 0.0, 0.02, 0.25, 0.3])
+    yg = np.array([0.0, 0.05, 0.13, 0.2, 0.31, 0.4])
+    zg = np.array([0.1, 0.11, 0.12, 0.4, 0.45, 0.5])
+    X, Y, Z = np.meshgrid(xg, yg, zg, indexing="ij")
+    B = np.stack([np.sin(3 * X) * Y, np.cos(2 * Y) + Z * X, X * Y * Z + 0.1])
+    V = X * X - Y * Z
+    fm = TFieldManager([tabfield3(xg, yg, zg, B=B, V=V, escale=1.0)])
+    rng = np.random.default_rng(11)
+    inner = np.stack([rng.uniform(-0.2, 0.3, 3000), rng.uniform(0.0, 0.4, 3000), rng.uniform(0.1, 0.5, 3000)], 1)
+    nodes = np.stack(np.meshgrid(xg, yg, zg, indexing="ij"), -1).reshape(-1, 3)
+    edges = np.array([[0.3, 0.2, 0.3], [-0.2, 0.0, 0.1], [-0.2000001, 0.2, 0.3], [0.0, 0.4, 0.3],
+                      [np.nan, 0.2, 0.3], [0.1, np.nan, 0.3], [5.0, 0.2, 0.3], [-5.0, -5.0, -5.0]])
+    pts = np.concatenate([inner, nodes, edges])
+    print(f"case: non-uniform grid field queries n={len(pts)}", flush=True)
+    got = queries.fields_eval(fm, pts, 0.0)
+    ref = oracle_lib.field_eval(fm, pts, 0.0, "fp64")
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b, equal_nan=True)
+
+
 def read_log(path):
     lines = open(path).read().splitlines()
     titles = lines[0].split()
