@@ -6,7 +6,10 @@ closed cylinder (R = 0.25 m, H = 0.5 m, 19,600 triangles), 2% Lambert walls with
 (|B| <= 1 T), low-field seekers, energies uniform in 0-250 neV, tmax 200 s.
 
 A bench step is one `pt_batch_run` call over the whole resident batch: it
-advances every live particle by up to `--rk` accepted dopri5 steps.  A variant's
+advances every live particle by up to `--rk` accepted dopri5 steps (32) in `--rounds`
+step-kernel/collision-kernel rounds (2): a particle whose third wall collision would
+fall in the call keeps its remaining steps for the next call, which skips the short,
+latency-bound rounds that resolve the last few collisions of a call.  A variant's
 rate is accepted RK steps per second of GPU time over the timed steps (inputs
 resident in HBM; HIP events on the library's stream).  Per-kernel times come from
 HIP events the library records around each step-kernel and collision-kernel launch.
@@ -77,10 +80,10 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--n", type=int, default=1_000_000)
-    p.add_argument("--rk", type=int, default=16, help="accepted RK steps per particle per bench step")
+    p.add_argument("--rk", type=int, default=32, help="accepted RK steps per particle per bench step")
     p.add_argument("--grid", type=int, default=64)
     p.add_argument("--block", type=int, default=256)
-    p.add_argument("--rounds", type=int, default=0, help="step/collision rounds per run call (0: library default)")
+    p.add_argument("--rounds", type=int, default=2, help="step/collision rounds per run call (0: library default, 4)")
     p.add_argument("--variants", default="fp64,fp64_persistent,fp64_persistent_graph,mixed,mixed_persistent,"
                                          "mixed_persistent_graph")
     p.add_argument("--no-mixed-error", action="store_true")
@@ -271,13 +274,13 @@ def uneven_leg(a, sptr):
     for mesh in [m for m in a.uneven_meshes.split(",") if m]:
         w = workloads.uneven_vessel(mesh, n=a.uneven_n, seed=4, n_grid=a.grid, n_tris=200_000)
         best, entry = autotune.tune(w, n_sample=min(200_000, w.n), calls=2, blocks=(256,), grids=(2, 4),
-                                    steps=(16, 64), leaf_sizes=(2, 4, 8), cache=None, stream=sptr)
+                                    steps=(16, 64), leaf_sizes=(2, 4), rounds=(2, 4), cache=None, stream=sptr)
         r = {"workload": f"{mesh} vessel ({w.meta['mesh_triangles']} mesh triangles, {w.geometry.n_triangles()} "
                          f"with the shell box), {w.n} particles: 90% neutrons tau=880 s and loss per bounce "
                          "U(1e-5, 1e-3), 8% protons 1-800 eV, 2% electrons 1-700 keV, uniform E 1e5 V/m + "
                          f"{a.grid}^3 B map",
              "tuned_config": {k: getattr(best, k) for k in ("block", "steps_per_launch", "persistent", "grid",
-                                                            "leaf_size")},
+                                                            "rounds", "leaf_size")},
              "autotune_trials": entry.get("trials")}
         for name, cfg in (("default", LaunchConfig(steps_per_launch=16)), ("autotuned", best)):
             b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
@@ -454,7 +457,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "cylinder R=0.25 H=0.5 (19600 tris), 1e6 neutrons, 64^3 tricubic B map of 8 loops, "
                                "2% Lambert, loss 1e-4, dopri5 rtol=atol=1e-9, tmax 200 s",
-                   "n_particles": w.n, "rk_steps_per_bench_step": a.rk, "variant": head, "block": a.block},
+                   "n_particles": w.n, "rk_steps_per_bench_step": a.rk, "rounds_per_bench_step": a.rounds, "variant": head, "block": a.block},
         "variants": {k: {"rk_steps_per_s": v["rate"], "gpu_ms": v["gpu_ms"], "accepted": v["steps"],
                          "rejected": v["rejected"], "particles_active": v["active"],
                          "step_kernel_ms_per_launch": v["kernels"]["step"][0] / max(v["kernels"]["step"][1], 1),

@@ -170,6 +170,10 @@ int pt_bicubic_precompute(int m, int n, const double *rg, const double *zg, int 
 /* the 64x64 integer matrix of libtricubic's tricubic_get_coeff (coefficients = A x),
    row-major into out[64*64]; for tests */
 int pt_tricubic_matrix(int32_t *out);
+/* the kernels' segment/triangle intersection (CGAL intersection(Triangle_3, Segment_3) as
+   TTriangleMesh::Collision uses it, src/trianglemesh.cpp:141-160), run on the host: 1 with
+   the point in pt[3], 0 none, -1 coplanar overlap; for tests */
+int pt_seg_tri(const double *a, const double *b, const double *c, const double *p, const double *q, double *pt);
 /* spline derivatives dy/dx at the n grid points of one line, as ALGLIB's
    spline1dgriddiffcubic with parabolic end conditions (src/field_3d.cpp:356); for tests */
 int pt_spline_grid_diff(int n, const double *x, const double *y, double *d);

@@ -2559,6 +2559,9 @@ int ptref_get_solids(const ptref_geom *g, double t, const double *p, uint32_t *i
 }
 
 int ptref_segment_bbox(const double *p, const double *q, const double *bb) { return seg_bbox(p, q, bb); }
+int ptref_seg_tri(const double *a, const double *b, const double *c, const double *p, const double *q, double *pt) {
+    return seg_tri(a, b, c, p, q, pt);
+}
 
 void ptref_rotate_vector(double *v, const double *n, const double *x) { rotate_vector(v, n, x); }
 

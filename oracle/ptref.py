@@ -94,6 +94,7 @@ def load(mode="fp64"):
                                    dp, dp]
     L.ptref_get_solids.argtypes = [C.POINTER(Geom), C.c_double, dp, C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.c_int]
     L.ptref_segment_bbox.argtypes = [dp, dp, dp]
+    L.ptref_seg_tri.argtypes = [dp, dp, dp, dp, dp, dp]
     L.ptref_rotate_vector.argtypes = [dp, dp, dp]
     L.ptref_rotate_vector.restype = None
     L.ptref_field_eval.argtypes = [C.c_int, C.POINTER(Field), C.c_int, dp, C.c_double, dp, dp, dp, dp]

@@ -146,6 +146,7 @@ EXPORTS = [
     "pt_tricubic_precompute_gpu", "pt_batch_set_snapshots", "pt_batch_snapshots",
     "pt_batch_set_track", "pt_batch_track", "pt_batch_set_spinlog", "pt_batch_spinlog",
     "pt_rk4_adjoint_noalloc", "pt_fields_mixed_info", "pt_gather_bandwidth", "pt_batch_set_loss",
+    "pt_seg_tri",
 ]
 
 _lib = None
@@ -182,6 +183,7 @@ def lib():
                                          C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), C.c_int]
     L.pt_tricubic_matrix.argtypes = [ip]
+    L.pt_seg_tri.argtypes = [C.POINTER(C.c_double)] * 6
     L.pt_tricubic_precompute_gpu.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                              C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
                                              C.POINTER(C.c_double)]

@@ -2,7 +2,10 @@
 
 Times the tracker on a sample of a workload over a grid of launch shapes -- kernel
 (per-particle grid or persistent work-queue slots), block size or persistent grid,
-accepted steps per run call, BVH leaf size -- and returns the fastest.  Results are
+accepted steps per run call, step/collision rounds per run call, BVH leaf size -- and
+returns the fastest.  Fewer rounds per call leave a particle's second or third wall
+collision of the call to the next call instead of running short, latency-bound rounds;
+the score is accepted steps per second, so that trade is measured, not assumed.  Results are
 cached per device and workload signature in a JSON file, so a production run tunes
 once.  Timing uses HIP events on the tracker's stream; every trial starts from the
 same state so trials are comparable.
@@ -33,7 +36,7 @@ def _device_name():
 
 
 def tune(w, n_sample=200_000, calls=3, blocks=(64, 128, 256), grids=(1, 2, 4), steps=(8, 16, 32, 64),
-         leaf_sizes=(2, 4, 8), variant=0, cache=DEFAULT_CACHE, stream=None, verbose=False):
+         leaf_sizes=(2, 4, 8), rounds=(2, 4), variant=0, cache=DEFAULT_CACHE, stream=None, verbose=False):
     """Best LaunchConfig for workload `w` (evaluated on its first n_sample particles).
 
     Each trial creates a fresh batch from the same initial states, runs `calls` untimed
@@ -41,7 +44,7 @@ def tune(w, n_sample=200_000, calls=3, blocks=(64, 128, 256), grids=(1, 2, 4), s
     """
     import torch
     n = min(n_sample, w.n)
-    key = f"{_device_name()}:{_signature(w, n)}:v{variant}"
+    key = f"{_device_name()}:{_signature(w, n)}:v{variant}:r{'-'.join(map(str, rounds))}"
     if cache and os.path.exists(cache):
         try:
             entry = json.load(open(cache)).get(key)
@@ -57,8 +60,8 @@ def tune(w, n_sample=200_000, calls=3, blocks=(64, 128, 256), grids=(1, 2, 4), s
     trials = []
     shapes = [dict(persistent=0, block=b) for b in blocks] + [dict(persistent=1, grid=g * cus) for g in grids]
     for leaf in leaf_sizes:
-        for shape, k in itertools.product(shapes, steps):
-            cfg = LaunchConfig(variant=variant, steps_per_launch=k, leaf_size=leaf, **shape)
+        for shape, k, nr in itertools.product(shapes, steps, rounds):
+            cfg = LaunchConfig(variant=variant, steps_per_launch=k, rounds=nr, leaf_size=leaf, **shape)
             b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx[sel], w.t0[sel], w.y0[sel],
                       w.spin0[sel], w.seed, w.pid[sel], with_mixed=(variant == 1), leaf_size=leaf)
             if getattr(w, "loss", None) is not None:
@@ -78,7 +81,7 @@ def tune(w, n_sample=200_000, calls=3, blocks=(64, 128, 256), grids=(1, 2, 4), s
             rate = (a1 - a0) / (ms / 1e3) if ms > 0 else 0.0
             trials.append((rate, cfg))
             if verbose:
-                print(f"[autotune] leaf {leaf} {shape} steps {k}: {rate:.3e} steps/s")
+                print(f"[autotune] leaf {leaf} {shape} steps {k} rounds {nr}: {rate:.3e} steps/s")
             del b
     best_rate, best = max(trials, key=lambda t: t[0])
     entry = {"cfg": {k: getattr(best, k) for k in ("variant", "block", "steps_per_launch", "persistent", "grid",

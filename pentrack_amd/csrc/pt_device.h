@@ -20,6 +20,7 @@
 #define PTD __device__ __forceinline__
 #define PTD_NOINLINE __device__ __noinline__
 #define PTHD __host__ __device__ __forceinline__
+#define PTHD_NOINLINE inline __host__ __device__ __noinline__
 
 /* Loads and stores through a pointer to the library's device allocations, issued as
    global (not flat) memory instructions: flat accesses also count against the wait
@@ -230,31 +231,41 @@ PTD double dabs(double x) { return __builtin_fabs(x); }
 
 /* ---- CGAL Simple_cartesian<double> predicates ---------------------------- */
 
-PTD int orient3(const double *p, const double *q, const double *r, const double *s) {
-    double a00 = q[0] - p[0], a01 = r[0] - p[0], a02 = s[0] - p[0];
-    double a10 = q[1] - p[1], a11 = r[1] - p[1], a12 = s[1] - p[1];
-    double a20 = q[2] - p[2], a21 = r[2] - p[2], a22 = s[2] - p[2];
+/* Points are passed by value (P3) through the predicates: a pointer to a per-lane array
+   that reached a non-inlined function, or a pointer chosen between two arrays at run
+   time, would force the arrays into scratch memory, stored and reloaded for every
+   triangle test. */
+struct P3 { double x, y, z; };
+PTHD P3 p3(const double *v) { return P3{v[0], v[1], v[2]}; }
+
+PTHD int orient3v(P3 p, P3 q, P3 r, P3 s) {
+    double a00 = q.x - p.x, a01 = r.x - p.x, a02 = s.x - p.x;
+    double a10 = q.y - p.y, a11 = r.y - p.y, a12 = s.y - p.y;
+    double a20 = q.z - p.z, a21 = r.z - p.z, a22 = s.z - p.z;
     double m01 = a00 * a11 - a10 * a01;
     double m02 = a00 * a21 - a20 * a01;
     double m12 = a10 * a21 - a20 * a11;
     double det = m01 * a22 - m02 * a12 + m12 * a02;
     return (det > 0) - (det < 0);
 }
+PTHD int orient3(const double *p, const double *q, const double *r, const double *s) {
+    return orient3v(p3(p), p3(q), p3(r), p3(s));
+}
 
-PTD int orient2(double px, double py, double qx, double qy, double rx, double ry) {
+PTHD int orient2(double px, double py, double qx, double qy, double rx, double ry) {
     double d = (qx - px) * (ry - py) - (rx - px) * (qy - py);
     return (d > 0) - (d < 0);
 }
-PTD int cop_orient(const double *p, const double *q, const double *r) {
-    int o = orient2(p[0], p[1], q[0], q[1], r[0], r[1]);
+PTHD int cop_orient(P3 p, P3 q, P3 r) {
+    int o = orient2(p.x, p.y, q.x, q.y, r.x, r.y);
     if (o != 0) return o;
-    o = orient2(p[1], p[2], q[1], q[2], r[1], r[2]);
+    o = orient2(p.y, p.z, q.y, q.z, r.y, r.z);
     if (o != 0) return o;
-    return orient2(p[0], p[2], q[0], q[2], r[0], r[2]);
+    return orient2(p.x, p.z, q.x, q.z, r.x, r.z);
 }
 /* CGAL do_intersect_coplanar(Triangle_3, Segment_3) */
-PTD_NOINLINE int coplanar_seg_tri(const double *A, const double *B, const double *C, const double *p, const double *q) {
-    const double *a = A, *b = B, *c = C;
+PTHD_NOINLINE int coplanar_seg_tri(P3 A, P3 B, P3 C, P3 p, P3 q) {
+    P3 a = A, b = B, c = C;
     if (cop_orient(A, B, C) <= 0) { b = C; c = B; }
     int pqa = cop_orient(p, q, a), pqb = cop_orient(p, q, b), pqc = cop_orient(p, q, c);
     if (pqa > 0) {
@@ -289,48 +300,36 @@ PTD_NOINLINE int coplanar_seg_tri(const double *A, const double *B, const double
     }
 }
 
-/* CGAL intersection(Triangle_3, Segment_3): 1 hit (point in pt), 0 none, -1 coplanar overlap */
-PTD int seg_tri(const double *a, const double *b, const double *c, const double *p, const double *q, double pt[3]) {
-    int abcp = orient3(a, b, c, p);
-    int abcq = orient3(a, b, c, q);
-    const double *u, *w;
-    if (abcp > 0) {
-        if (abcq > 0) return 0;
-        if (abcq == 0) {
-            if (orient3(p, q, a, b) <= 0 && orient3(p, q, b, c) <= 0 && orient3(p, q, c, a) <= 0) { pt[0] = q[0]; pt[1] = q[1]; pt[2] = q[2]; return 1; }
-            return 0;
-        }
-        u = p; w = q;
-    } else if (abcp < 0) {
-        if (abcq < 0) return 0;
-        if (abcq == 0) {
-            if (orient3(q, p, a, b) <= 0 && orient3(q, p, b, c) <= 0 && orient3(q, p, c, a) <= 0) { pt[0] = q[0]; pt[1] = q[1]; pt[2] = q[2]; return 1; }
-            return 0;
-        }
-        u = q; w = p;
-    } else {
-        if (abcq > 0) {
-            if (orient3(q, p, a, b) <= 0 && orient3(q, p, b, c) <= 0 && orient3(q, p, c, a) <= 0) { pt[0] = p[0]; pt[1] = p[1]; pt[2] = p[2]; return 1; }
-            return 0;
-        }
-        if (abcq < 0) {
-            if (orient3(p, q, a, b) <= 0 && orient3(p, q, b, c) <= 0 && orient3(p, q, c, a) <= 0) { pt[0] = p[0]; pt[1] = p[1]; pt[2] = p[2]; return 1; }
-            return 0;
-        }
-        return coplanar_seg_tri(a, b, c, p, q) ? -1 : 0;
-    }
-    if (orient3(u, w, a, b) > 0 || orient3(u, w, b, c) > 0 || orient3(u, w, c, a) > 0) return 0;
-    double rpx = a[0] - c[0], rpy = a[1] - c[1], rpz = a[2] - c[2];
-    double rqx = b[0] - c[0], rqy = b[1] - c[1], rqz = b[2] - c[2];
+/* CGAL intersection(Triangle_3, Segment_3): 1 hit (point in pt), 0 none, -1 coplanar overlap.
+   CGAL's case analysis on the sides of p and q, without its branches: in every case with
+   an end off the plane the segment meets the triangle iff the three edges are oriented
+   non-positively about the segment directed from p to q (p above the plane, or p on it
+   and q below) or from q to p (otherwise); the same orientation predicates are evaluated
+   on the same points, so each case's decision is CGAL's, and a wave whose lanes fall in
+   different cases runs one set of predicates instead of one per case. */
+PTHD int seg_tri(const double *a_, const double *b_, const double *c_, const double *p_, const double *q_, double pt[3]) {
+    const P3 a = p3(a_), b = p3(b_), c = p3(c_), p = p3(p_), q = p3(q_);
+    const int abcp = orient3v(a, b, c, p);
+    const int abcq = orient3v(a, b, c, q);
+    if (abcp == 0 && abcq == 0) return coplanar_seg_tri(a, b, c, p, q) ? -1 : 0;
+    const bool apart = (abcp > 0 && abcq > 0) || (abcp < 0 && abcq < 0);
+    const bool fwd = abcp > 0 || (abcp == 0 && abcq < 0);
+    const P3 u = fwd ? p : q, w = fwd ? q : p;
+    const int o1 = orient3v(u, w, a, b), o2 = orient3v(u, w, b, c), o3 = orient3v(u, w, c, a);
+    if (apart || o1 > 0 || o2 > 0 || o3 > 0) return 0;
+    if (abcq == 0) { pt[0] = q.x; pt[1] = q.y; pt[2] = q.z; return 1; }
+    if (abcp == 0) { pt[0] = p.x; pt[1] = p.y; pt[2] = p.z; return 1; }
+    double rpx = a.x - c.x, rpy = a.y - c.y, rpz = a.z - c.z;
+    double rqx = b.x - c.x, rqy = b.y - c.y, rqz = b.z - c.z;
     double pa = rpy * rqz - rqy * rpz;
     double pb = rpz * rqx - rqz * rpx;
     double pc = rpx * rqy - rqx * rpy;
-    double pd = -pa * c[0] - pb * c[1] - pc * c[2];
-    double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
-    double num = pa * p[0] + pb * p[1] + pc * p[2] + pd;
+    double pd = -pa * c.x - pb * c.y - pc * c.z;
+    double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+    double num = pa * p.x + pb * p.y + pc * p.z + pd;
     double den = pa * dx + pb * dy + pc * dz;
     if (den == 0) return 0;
-    double hx = den * p[0] - num * dx, hy = den * p[1] - num * dy, hz = den * p[2] - num * dz;
+    double hx = den * p.x - num * dx, hy = den * p.y - num * dy, hz = den * p.z - num * dz;
     if (den != 1.0) { pt[0] = hx / den; pt[1] = hy / den; pt[2] = hz / den; }
     else { pt[0] = hx; pt[1] = hy; pt[2] = hz; }
     return 1;
@@ -533,6 +532,14 @@ struct Cand {
     int idx[MAX_CAND];
 };
 
+/* The vertices of a lane's first CAND_LDS candidates, staged in LDS by cand_build for the
+   bisection's counting queries (cand_lds[k][vertex coordinate][lane], 36 B per candidate
+   and lane: 45 KB for a block of CAND_LANES threads, three blocks per CU): an LDS read
+   instead of an index and a triangle fetched from memory for every candidate test.  Only
+   kernels whose blocks have at most CAND_LANES threads may reach cand_build. */
+constexpr int CAND_LDS = 5, CAND_LANES = 256;
+__shared__ float cand_lds[CAND_LDS][9][CAND_LANES];
+
 PTD void cand_build(const Geom &G, const double *lo, const double *hi, Cand &C) {
     C.n = 0;
 #pragma unroll
@@ -568,6 +575,12 @@ PTD void cand_build(const Geom &G, const double *lo, const double *hi, Cand &C) 
                     }
                     if (!in) continue;
                     if (C.n >= MAX_CAND) { C.n = -1; return; }
+                    if (C.n < CAND_LDS) {
+                        const int l = threadIdx.x;
+                        cand_lds[C.n][0][l] = (float)a[0]; cand_lds[C.n][1][l] = (float)a[1]; cand_lds[C.n][2][l] = (float)a[2];
+                        cand_lds[C.n][3][l] = (float)b[0]; cand_lds[C.n][4][l] = (float)b[1]; cand_lds[C.n][5][l] = (float)b[2];
+                        cand_lds[C.n][6][l] = (float)c[0]; cand_lds[C.n][7][l] = (float)c[1]; cand_lds[C.n][8][l] = (float)c[2];
+                    }
                     C.idx[C.n++] = first + k;
                 }
             } else {
@@ -596,12 +609,21 @@ PTD int cand_query(const Geom &G, const Cand &C, const double *p, const double *
     double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
     double sl = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
     int count = 0;
+    /* the test counter is added once, not updated in memory per candidate */
+    *tests += C.n > 0 ? C.n : 0;
     for (int k = 0; k < C.n; ++k) {
-        const int t = C.idx[k];
         double a[3], b[3], c[3], pt[3];
-        int solid, orig;
-        load_tri(G.tri + t, a, b, c, solid, orig);
-        ++*tests;
+        int solid = 0, orig = 0, t = 0;
+        if (!list && k < CAND_LDS) {
+            /* a counting query: the vertices staged by cand_build (the STL's floats, exact) */
+            const int l = threadIdx.x;
+            a[0] = cand_lds[k][0][l]; a[1] = cand_lds[k][1][l]; a[2] = cand_lds[k][2][l];
+            b[0] = cand_lds[k][3][l]; b[1] = cand_lds[k][4][l]; b[2] = cand_lds[k][5][l];
+            c[0] = cand_lds[k][6][l]; c[1] = cand_lds[k][7][l]; c[2] = cand_lds[k][8][l];
+        } else {
+            t = C.idx[k];
+            load_tri(G.tri + t, a, b, c, solid, orig);
+        }
         int r = seg_tri(a, b, c, p, q, pt);
         if (r == 0) continue;
         if (r < 0) return -1;

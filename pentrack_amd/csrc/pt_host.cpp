@@ -1114,6 +1114,10 @@ extern "C" int pt_neutron_decay_host(uint64_t key, uint64_t ctr, const double *v
 }
 
 /* exported for tests: the 64x64 integer matrix */
+extern "C" int pt_seg_tri(const double *a, const double *b, const double *c, const double *p, const double *q, double *pt) {
+    return ptd::seg_tri(a, b, c, p, q, pt);
+}
+
 extern "C" int pt_tricubic_matrix(int32_t *out) {
     const TricubicMatrix &M = tricubic_matrix();
     for (int i = 0; i < 64; ++i)

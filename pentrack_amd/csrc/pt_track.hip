@@ -1200,10 +1200,12 @@ PTD_NOINLINE bool do_hit(const Params &P, Track &T, double x1, const double *y1,
 }
 
 /* number of intersections of segment p1->p2 (-1 coplanar, -2 overflow) */
-/* collision count of a segment on the collision path: from the piece's candidates when they cover it */
+/* collision count of a segment on the collision path: from the piece's candidates when they cover it
+   (an exact count; the clearance-grid shortcut, which needs a memory round trip and never
+   applies this close to a wall, only for segments the candidates do not cover) */
 PTD int count_hits(const Params &P, Track &T, const double *p1, const double *p2) {
-    if (clear_of_walls(P.geom, p1, p2)) return 0;
     if (cand_covers(T.cand, p1, p2)) return cand_query(P.geom, T.cand, p1, p2, nullptr, &T.c.tri);
+    if (clear_of_walls(P.geom, p1, p2)) return 0;
     int2 r = seg_query_v(P.geom, p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], nullptr);
     T.c.tri += r.y;
     return r.x;
@@ -1229,35 +1231,46 @@ PTD_NOINLINE int iterate_collision(const Params &P, Track &T, const Dopri &st, d
     int d = 0;
     double lo = X1, hi = X2;
     double plo[3] = {Y1[0], Y1[1], Y1[2]}, phi[3] = {Y2[0], Y2[1], Y2[2]};
-    double pmid[3], ymid[9];
-    double mid;
-    int branch; /* 0: evaluate frame, 1: try second half of frame d */
-    branch = 0;
+    double pmid[3] = {0., 0., 0.};
+    double mid = 0.;
+    /* one segment query per iteration: second = false tests the first half of frame d
+       (after the frame's stop checks and midpoint), second = true its second half; the
+       queries and the decisions come in the recursion's order */
+    bool second = false;
     for (;;) {
-        if (branch == 0) {
+        double a[3], b[3];
+        if (!second) {
             double dx0 = phi[0] - plo[0], dx1 = phi[1] - plo[1], dx2 = phi[2] - plo[2];
             if (dx0 * dx0 + dx1 * dx1 + dx2 * dx2 < REFLECT_TOLERANCE * REFLECT_TOLERANCE) break;
             if (hi - lo < 4 * (lo + hi) * DBL_EPS) break;
             if (d >= 100) break;
             mid = lo + (hi - lo) * 0.5;
+            double ymid[9];
             dopri_calc_state(st, mid, ymid);
             pmid[0] = ymid[0]; pmid[1] = ymid[1]; pmid[2] = ymid[2];
-            int n1 = count_hits(P, T, plo, pmid);
-            if (n1 < 0) return -1;
-            if (n1 > 0) {
-                ph[d >> 6] &= ~(1ull << (d & 63));
-                ++d;
-                hi = mid; phi[0] = pmid[0]; phi[1] = pmid[1]; phi[2] = pmid[2];
-                continue;
-            }
+            a[0] = plo[0]; a[1] = plo[1]; a[2] = plo[2];
+            b[0] = pmid[0]; b[1] = pmid[1]; b[2] = pmid[2];
+        } else {
+            a[0] = pmid[0]; a[1] = pmid[1]; a[2] = pmid[2];
+            b[0] = phi[0]; b[1] = phi[1]; b[2] = phi[2];
         }
-        branch = 0;
-        int n2 = count_hits(P, T, pmid, phi);
-        if (n2 < 0) return -1;
-        if (n2 > 0) {
-            ph[d >> 6] |= (1ull << (d & 63));
+        const int nh = count_hits(P, T, a, b);
+        if (nh < 0) return -1;
+        if (nh > 0) {
+            /* descend into the half that holds a collision */
+            if (!second) {
+                ph[d >> 6] &= ~(1ull << (d & 63));
+                hi = mid; phi[0] = pmid[0]; phi[1] = pmid[1]; phi[2] = pmid[2];
+            } else {
+                ph[d >> 6] |= (1ull << (d & 63));
+                lo = mid; plo[0] = pmid[0]; plo[1] = pmid[1]; plo[2] = pmid[2];
+            }
             ++d;
-            lo = mid; plo[0] = pmid[0]; plo[1] = pmid[1]; plo[2] = pmid[2];
+            second = false;
+            continue;
+        }
+        if (!second) {
+            second = true;
             continue;
         }
         /* backtrack to the nearest ancestor whose first half was tried */
@@ -1268,19 +1281,19 @@ PTD_NOINLINE int iterate_collision(const Params &P, Track &T, const Dopri &st, d
         }
         /* rebuild frame d: interval by replaying the halvings, states by dense output */
         lo = X1; hi = X2;
-        bool loroot = true, hiroot = true;
+        bool hiroot = true;
         for (int k = 0; k < d; ++k) {
             double m = lo + (hi - lo) * 0.5;
-            if ((ph[k >> 6] >> (k & 63)) & 1ull) { lo = m; loroot = false; }
+            if ((ph[k >> 6] >> (k & 63)) & 1ull) { lo = m; }
             else { hi = m; hiroot = false; }
         }
-        (void)loroot;
         mid = lo + (hi - lo) * 0.5;
+        double ymid[9];
         dopri_calc_state(st, mid, ymid);
         pmid[0] = ymid[0]; pmid[1] = ymid[1]; pmid[2] = ymid[2];
         if (hiroot) { phi[0] = Y2[0]; phi[1] = Y2[1]; phi[2] = Y2[2]; }
         else { double yh[9]; dopri_calc_state(st, hi, yh); phi[0] = yh[0]; phi[1] = yh[1]; phi[2] = yh[2]; }
-        branch = 1;
+        second = true;
     }
     /* success: the final bracket; its ends are root states or dense-output states */
     bool loroot = true, hiroot = true;
@@ -2635,6 +2648,7 @@ __global__ void __launch_bounds__(256) bucket_scatter_kernel(const Params *__res
 #ifndef PT_COLL_MIN_WAVES
 #define PT_COLL_MIN_WAVES 3
 #endif
+static_assert(CAND_LANES == 256, "the collision kernel's blocks fill the candidate staging area");
 template <int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_COLL_MIN_WAVES))) collision_kernel(const Params *__restrict__ Pp) {
     const Params &P = *Pp;
@@ -2681,7 +2695,8 @@ hipError_t pt_launch_step_persistent_mode(const ptd::Params &P, int grid, int sp
 }
 template <int MODE>
 hipError_t pt_launch_collision_mode(const ptd::Params &P, int grid, hipStream_t stream) {
-    hipLaunchKernelGGL(ptd::collision_kernel<MODE>, dim3(grid), dim3(256), 0, stream, P.self);
+    /* blocks of CAND_LANES threads: the size of the candidate staging area in LDS */
+    hipLaunchKernelGGL(ptd::collision_kernel<MODE>, dim3(grid), dim3(ptd::CAND_LANES), 0, stream, P.self);
     return hipGetLastError();
 }
 template <int MODE>

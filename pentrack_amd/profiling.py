@@ -3,7 +3,7 @@
 `python -m pentrack_amd.profiling run KERNEL` is the program profiled: a plain
 device copy of known size (the calibration: FETCH_SIZE and WRITE_SIZE of a kernel
 that reads and writes exactly COPY_BYTES), then the benchmark workload (1e6
-neutrons, cylinder, 64^3 map) advanced by a few run calls of 16 RK steps with the
+neutrons, cylinder, 64^3 map) advanced by a few run calls of 32 RK steps in 2 rounds with the
 step kernel KERNEL (step_kernel_fp64, step_kernel_persistent_fp64, step_kernel_mixed,
 step_kernel_persistent_mixed, or step_kernel_persistent_mixed_spatial: the persistent
 kernel with its work list in Morton order of the particle positions).  Profile it three
@@ -26,6 +26,8 @@ import sqlite3
 import sys
 
 COPY_BYTES = 1 << 30
+# the launch shape of the benchmark's run calls (bench.py --rk / --rounds defaults)
+RK_STEPS, ROUNDS = 32, 2
 KERNELS = {
     # name: (variant, persistent, dispatch-name pattern, spatial order)
     # (the kernels' template argument is the mode: 0 FP64 tables, 1 mixed-precision tables)
@@ -74,7 +76,8 @@ def run(kernel="step_kernel_fp64"):
     # for the components kept in FP64 (mixed)
     bpe = 3 * 512 if not variant else 3 * (288 + 512 * fp64 / max(blocks, 1))
     stream = torch.cuda.current_stream().cuda_stream
-    cfg = LaunchConfig(variant=variant, persistent=persistent, steps_per_launch=16, spatial_order=spatial)
+    cfg = LaunchConfig(variant=variant, persistent=persistent, steps_per_launch=RK_STEPS, rounds=ROUNDS,
+                       spatial_order=spatial)
     b.set_profiling(True)
     for _ in range(3):
         b.run(cfg, stream)
@@ -126,7 +129,8 @@ def summarize(kernel, fetch_db, write_db, algorithmic=None, flops_db=None):
     fs = sum(f[d][1] for d in step) / len(step)
     ws = sum(w[d][1] for d in step if d in w) / len(step)
     fetch, write = fs * fscale * 1024, ws * wscale * 1024
-    print(f"# HBM traffic per launch of {kernel}, cylinder 1e6 neutrons, 64^3 map, 16 RK steps per call")
+    print(f"# HBM traffic per launch of {kernel}, cylinder 1e6 neutrons, 64^3 map, {RK_STEPS} RK steps in "
+          f"{ROUNDS} rounds per call")
     print("commands:")
     print(f"  rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 -m pentrack_amd.profiling run {kernel}")
     print(f"  rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 -m pentrack_amd.profiling run {kernel}")
