@@ -159,73 +159,56 @@ def cpu_baseline(w, budget_s):
 
 def mixed_error_leg(a, sptr):
     """Mixed precision against FP64 on config 3's workload (1 m cube, 128^3 coil map,
-    |B| <= 1 T): every particle, full hit sequences.  The FP64 kernel is bit-exact with the
-    FP64 oracle (GPU tests), so it stands for the oracle here.  Deviations at longer
+    |B| <= 1 T): every particle, full hit sequences, trajectory deviation at snapshots every
+    10 ms up to the stated horizon (pentrack_amd.compare).  The FP64 kernel is bit-exact
+    with the FP64 oracle (GPU tests), so it stands for the oracle here.  Deviations at longer
     horizons are reported beside the stated one, with the deviation of the oracle's
     reference-semantics mode (the reference's own long double arithmetic) from its FP64 mode
     on a sample, the floor any change of arithmetic is measured against."""
     import torch
     from oracle import ptref
-    from pentrack_amd import _native, workloads
+    from pentrack_amd import _native, compare, workloads
     from pentrack_amd.tracker import Batch, LaunchConfig
     from pentrack_amd.workloads import Workload
     longer = [float(v) for v in a.err_longer.split(",") if v]
-    times = [a.err_tmax] + longer
-    tend = max(times) * 1.0001
-    w = workloads.cube(n=a.err_n, seed=3, tmax=tend, with_field=True, n_grid=128, Bmax=1.0)
+    times = compare.snapshot_times(a.err_tmax, longer)
+    horizons = [a.err_tmax] + longer
+    w = workloads.cube(n=a.err_n, seed=3, tmax=max(horizons), with_field=True, n_grid=128, Bmax=1.0)
     cap = 1024
-    out = {}
-    for name, vid in (("fp64", _native.PT_VARIANT_FP64), ("mixed", _native.PT_VARIANT_MIXED)):
-        b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
-                  hit_cap=cap, with_mixed=(name == "mixed"))
-        b.set_snapshots(times, len(times))
-        b.run_to_completion(LaunchConfig(variant=vid, steps_per_launch=256, rounds=8), max_launches=100000,
-                            stream=sptr)
-        o, h = b.download(sptr)
-        rec, cnt = b.snapshots(sptr)
-        out[name] = (o, h, rec, cnt)
-        del b
+    rows = []
+    for tj in horizons:
+        # both arithmetics tracked to tmax = the horizon: end states there, snapshots before it
+        tt = times[times <= tj]
+        out = {}
+        for name, vid in (("fp64", _native.PT_VARIANT_FP64), ("mixed", _native.PT_VARIANT_MIXED)):
+            b = Batch(w.geometry, w.fields, w.species, tj, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
+                      hit_cap=cap, with_mixed=(name == "mixed"))
+            b.set_snapshots(tt, len(tt))
+            b.run_to_completion(LaunchConfig(variant=vid, steps_per_launch=256, rounds=8), max_launches=100000,
+                                stream=sptr)
+            o, h = b.download(sptr)
+            rec, cnt = b.snapshots(sptr)
+            assert (o["n_hits"] <= cap).all()
+            out[name] = (o, h, rec, cnt)
+            del b
+        rows.append(compare.deviation(out["fp64"], out["mixed"], tt, tj))
     blocks, fp64_blocks = w.fields.mixed_info()
-    (g, gh, gr, gc), (m, mh, mr, mc) = out["fp64"], out["mixed"]
-    assert (g["n_hits"] <= cap).all() and (m["n_hits"] <= cap).all()
-
-    def at(j, tj):
-        """State at horizon tj: the snapshot there, or the end state of a particle that
-        stopped before it (both runs must agree on which)."""
-        snap_g, snap_m = gc > j, mc > j
-        stopped = ~snap_g & ~snap_m & (g["t"] < tj) & (m["t"] < tj) & (g["stopID"] == m["stopID"])
-        ok = (snap_g & snap_m) | stopped
-        yg = np.where(snap_g[:, None], gr[:, min(j, gr.shape[1] - 1)]["y"], g["y"])
-        ym = np.where(snap_m[:, None], mr[:, min(j, mr.shape[1] - 1)]["y"], m["y"])
-        dx = np.linalg.norm(yg[:, :3] - ym[:, :3], axis=1)[ok]
-        dv = np.linalg.norm(yg[:, 3:6] - ym[:, 3:6], axis=1)[ok]
-        same = np.zeros(w.n, bool)
-        for k in range(w.n):
-            x, y = gh[k, :g["n_hits"][k]], mh[k, :m["n_hits"][k]]
-            x, y = x[x["t"] < tj], y[y["t"] < tj]
-            same[k] = len(x) == len(y) and np.array_equal(x["tri"], y["tri"])
-        return {"horizon_s": tj, "particles": int(ok.sum()), "stopped_before": int(stopped.sum()),
-                "same_hit_sequences": float(same.mean()),
-                "end_position_max_m": float(dx.max()), "end_position_p99_m": float(np.percentile(dx, 99)),
-                "end_velocity_max_m_s": float(dv.max()), "end_velocity_p99_m_s": float(np.percentile(dv, 99))}
-    rows = [at(j, tj) for j, tj in enumerate(times)]
     # the reference's own arithmetic against the FP64 oracle, on a sample
     r = min(a.err_ref_n, w.n)
     sw = Workload(w.name, w.geometry, w.fields, w.species, w.tmax, w.species_idx[:r], w.t0[:r], w.y0[:r],
                   w.spin0[:r], w.pid[:r], w.seed)
     cores, _ = host_cpu()
-    ref = {}
-    for mode in ("fp64", "refsem"):
-        _, rr, rc = ptref.track_snapshots(sw, times, len(times), mode=mode, threads=cores)
-        ref[mode] = (rr, rc)
     refrows = []
-    for j, tj in enumerate(times):
-        ok = (ref["fp64"][1] > j) & (ref["refsem"][1] > j)
-        dx = np.linalg.norm(ref["fp64"][0][:, j]["y"][:, :3] - ref["refsem"][0][:, j]["y"][:, :3], axis=1)[ok]
-        refrows.append({"horizon_s": tj, "particles": int(ok.sum()), "end_position_max_m": float(dx.max()),
-                        "end_position_p99_m": float(np.percentile(dx, 99))})
-    s = rows[0]
-    holds = s["particles"] == w.n and s["same_hit_sequences"] == 1.0 and s["end_position_max_m"] <= 1e-6
+    for tj in horizons:
+        tt = times[times <= tj]
+        sw.tmax = tj
+        ref = {mode: ptref.track_snapshots(sw, tt, len(tt), mode=mode, threads=cores) for mode in ("fp64", "refsem")}
+        nh = np.zeros((r, 0), _native.HIT_DTYPE)
+        d = compare.deviation((ref["fp64"][0], nh) + tuple(ref["fp64"][1:]), (ref["refsem"][0], nh) + tuple(ref["refsem"][1:]),
+                              tt, tj)
+        refrows.append({"horizon_s": tj, "particles": r, "trajectory_position_max_m": d["trajectory_position_max_m"],
+                        "trajectory_position_p99_m": d["trajectory_position_p99_m"]})
+    holds = rows[0]["condition_holds"]
     # speed of the two arithmetics on this workload's map at the benchmark's scale
     speed = {}
     if a.err_speed_n > 0:

@@ -169,33 +169,39 @@ def test_edge_cases(gpu, oracle_lib):
 
 def test_mixed_precision_config3(gpu, oracle_lib):
     """The mixed-precision workload as specified: 1 m cube, 128^3 coil map (|B| <= 1 T),
-    neutrons from the config-1 source.  The FP64 kernel is bit-exact with the FP64 oracle on
-    it; the mixed-precision kernel (FP64 constant and linear terms, FP32 remainder, FP64
-    blocks where the remainder's bound exceeds the default tolerance) keeps every particle's
-    stop reason and full hit-triangle sequence and its end position within 1e-6 m of the
-    oracle over 0.25 s.  (The dynamics in this field are chaotic: deviations of any origin
-    grow about e-fold every 0.8 s, so the bound holds over a horizon, not over 50 s.)"""
-    w = workloads.cube(n=2000, seed=3, tmax=0.25, with_field=True, n_grid=128, Bmax=1.0)
-    print("case: mixed precision, cube, 128^3 map, n=2000, 0.25 s", flush=True)
+    10,000 neutrons from the config-1 source (the benchmark's error leg).  The FP64 kernel
+    is bit-exact with the FP64 oracle on it; the mixed-precision kernel (FP64 constant and
+    linear terms, FP32 remainder, FP64 blocks where the remainder's bound exceeds the default
+    tolerance) keeps every particle's stop reason and full hit-triangle sequence, and its
+    trajectory within 1e-6 m of FP64 (end state at 0.25 s and the snapshots both runs
+    recorded on a 10 ms grid), for every particle (those that leave the cube through the
+    floor on their path up to the exit).
+    The dynamics in this field are chaotic: deviations of any origin grow about e-fold every
+    0.8 s, so the bound holds over a horizon, not over 50 s."""
+    from pentrack_amd import compare
+    w = workloads.cube(n=10000, seed=3, tmax=0.25, with_field=True, n_grid=128, Bmax=1.0)
+    times = compare.snapshot_times(0.25)
+    print(f"case: mixed precision, cube, 128^3 map, n={w.n}, 0.25 s", flush=True)
     cap = 256
     o, oh = oracle_lib.track(w, "fp64", hit_cap=cap, threads=8)
     assert o["n_hits"].max() <= cap and o["Nhit"].sum() > w.n
-    g, gh = run_gpu(w, hit_cap=cap)
-    assert_bit_exact(g, gh, o, oh)
-    m, mh = run_gpu(w, hit_cap=cap, cfg=LaunchConfig(variant=_native.PT_VARIANT_MIXED), with_mixed=True)
+    runs = {}
+    for name, cfg in (("fp64", LaunchConfig()), ("mixed", LaunchConfig(variant=_native.PT_VARIANT_MIXED))):
+        b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
+                  hit_cap=cap, with_mixed=(name == "mixed"))
+        b.set_snapshots(times, len(times))
+        assert b.run_to_completion(cfg, max_launches=100000)
+        out, hits = b.download()
+        runs[name] = (out, hits) + b.snapshots()
+    assert_bit_exact(runs["fp64"][0], runs["fp64"][1], o, oh)
     blocks, fp64 = w.fields.mixed_info()
     assert 0 < fp64 < 0.5 * blocks
-    assert np.array_equal(m["stopID"], o["stopID"]) and np.array_equal(m["Nhit"], o["Nhit"])
-    assert np.array_equal(m["n_hits"], o["n_hits"])
-    for k in range(w.n):
-        nk = o["n_hits"][k]
-        assert np.array_equal(mh[k, :nk]["tri"], oh[k, :nk]["tri"]), k
-    dx = np.linalg.norm(m["y"][:, :3] - o["y"][:, :3], axis=1)
-    dv = np.linalg.norm(m["y"][:, 3:6] - o["y"][:, 3:6], axis=1)
-    print(f"mixed vs oracle: end position max {dx.max():.2e} p99 {np.percentile(dx, 99):.2e} m, "
-          f"end velocity max {dv.max():.2e} m/s, FP64 blocks {fp64}/{blocks}", flush=True)
-    assert dx.max() <= 1e-6
-    assert not np.array_equal(m["y"], o["y"])           # the mixed arithmetic is in use
+    r = compare.deviation(runs["fp64"], runs["mixed"], times, 0.25)
+    print(f"mixed vs FP64: {r}, FP64 blocks {fp64}/{blocks}", flush=True)
+    assert r["stopped_before"] > 0                    # the escapes through the floor are covered
+    assert r["same_stop_reasons"] == 1.0 and r["same_hit_sequences"] == 1.0
+    assert r["trajectory_position_max_m"] <= 1e-6 and r["condition_holds"]
+    assert not np.array_equal(runs["mixed"][0]["y"], o["y"])      # the mixed arithmetic is in use
 
 
 def test_persistent_work_queue_bit_exact(gpu, oracle_lib):
