@@ -440,9 +440,10 @@ int pt_rk4_adjoint_noalloc(void *stream, const pt_adj_params *params, int n, con
 /* ---- measurement aid ------------------------------------------------------ */
 /*
  * No counterpart in the reference: the bandwidth ceilings the tracking kernels are compared
- * with.  gather_gbs: `threads` lanes each read `gathers` blocks of block_bytes (contiguous per
- * lane, 16-byte loads) from uniformly random blocks of a table of table_bytes, the access
- * pattern of the tricubic field evaluation; copy_gbs: a streaming copy of the table (bytes
+ * with.  gather_gbs: `threads` lanes each read `gathers` blocks of block_bytes (512: one
+ * component's coefficients, 1536: a cell's three; contiguous per lane, 16-byte loads, a
+ * 128-byte line's eight issued together) from uniformly random blocks of a table of
+ * table_bytes, the access pattern of the tricubic field evaluation; copy_gbs: a streaming copy of the table (bytes
  * read + written).  Both averaged over `reps` launches after a warm-up; allocates and frees
  * 2 x table_bytes of device memory; synchronous on the default stream.
  */

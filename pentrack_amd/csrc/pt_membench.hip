@@ -2,11 +2,12 @@
  * pt_membench.hip -- the bandwidth ceilings the tracking kernels are measured against.
  *
  * No counterpart in the reference (a measurement aid).  The step kernel's dominant memory
- * traffic is one 64-coefficient block per field component gathered from a random cell by
- * each lane; this file measures what the device sustains for exactly that pattern (each
- * lane reads its own contiguous block of block_bytes from a uniformly random block of a
- * table of table_bytes), and for a plain streaming copy.  With tables that fit the 256 MB
- * Infinity Cache the gather is served from there; with larger ones from HBM.
+ * traffic is a cell's three 64-coefficient blocks (1536 bytes, twelve 128-byte lines) gathered
+ * from a random cell by each lane; this file measures what the device sustains for exactly
+ * that pattern (each lane reads its own contiguous block of block_bytes, 512 or 1536, from
+ * a uniformly random block of a table of table_bytes, a line's eight 16-byte loads issued
+ * together), and for a plain streaming copy.  Tables that fit an XCD's 4 MB L2 are served
+ * from there, tables that fit the 256 MB Infinity Cache from it, larger ones from HBM.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -25,18 +26,26 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-/* each thread gathers `gathers` random blocks of `d2` double2 words and keeps a sum */
-__global__ void __launch_bounds__(256) gather_kernel(const double2 *__restrict__ table, uint64_t nblocks, int d2,
+/* each thread gathers `gathers` random blocks of 16 * LINES double2 words (LINES 128-byte
+   lines, the step kernel's cell block of three 512-byte components is 12) and keeps a sum;
+   the eight 16-byte loads of one line are issued together, as the tricubic evaluation
+   issues them */
+template <int LINES>
+__global__ void __launch_bounds__(256) gather_kernel(const double2 *__restrict__ table, uint64_t nblocks,
                                                      int gathers, uint64_t seed, double *__restrict__ sink) {
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0.0;
     for (int g = 0; g < gathers; ++g) {
         /* uniform in [0, nblocks) by a multiply-high of 32 random bits (nblocks < 2^32) */
         const uint64_t b = ((mix64(seed ^ (tid * 0x100000001b3ull + (uint64_t)g)) >> 32) * nblocks) >> 32;
-        const double2 *p = table + b * (uint64_t)d2;
-        for (int q = 0; q < d2; ++q) {
-            const double2 v = p[q];
-            s += v.x + v.y;
+        const double2 *p = table + b * (uint64_t)(8 * LINES);
+#pragma unroll
+        for (int l = 0; l < LINES; ++l) {
+            double2 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = p[8 * l + q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s += v[q].x + v[q].y;
         }
     }
     sink[tid] = s;
@@ -62,7 +71,7 @@ __global__ void __launch_bounds__(256) fill_kernel(double2 *__restrict__ a, uint
 
 extern "C" int pt_gather_bandwidth(uint64_t table_bytes, int block_bytes, int threads, int gathers, int reps,
                                    double *gather_gbs, double *copy_gbs) {
-    if (block_bytes < 16 || block_bytes % 16 != 0 || table_bytes < (uint64_t)block_bytes ||
+    if ((block_bytes != 512 && block_bytes != 1536) || table_bytes < (uint64_t)block_bytes ||
         table_bytes / (uint64_t)block_bytes >= (1ull << 32) || threads < 256 ||
         threads % 256 != 0 || gathers < 1 || reps < 1)
         return pt_fail(PT_EINVAL, "pt_gather_bandwidth: bad arguments");
@@ -81,13 +90,17 @@ extern "C" int pt_gather_bandwidth(uint64_t table_bytes, int block_bytes, int th
     hipLaunchKernelGGL(ptm_bench::fill_kernel, dim3(4096), dim3(256), 0, 0, table, n2);
     PTB_HIP(hipGetLastError());
     /* warm-up, then the timed repetitions of the gather */
-    hipLaunchKernelGGL(ptm_bench::gather_kernel, dim3(threads / 256), dim3(256), 0, 0, table, nblocks, block_bytes / 16,
-                       gathers, 12345ull, sink);
-    PTB_HIP(hipGetLastError());
-    PTB_HIP(hipEventRecord(e0, 0));
-    for (int r = 0; r < reps; ++r)
-        hipLaunchKernelGGL(ptm_bench::gather_kernel, dim3(threads / 256), dim3(256), 0, 0, table, nblocks,
-                           block_bytes / 16, gathers, 777ull + (uint64_t)r, sink);
+    for (int r = -1; r < reps; ++r) {
+        if (r == 0) PTB_HIP(hipEventRecord(e0, 0));
+        const uint64_t seed = r < 0 ? 12345ull : 777ull + (uint64_t)r;
+        if (block_bytes == 512)
+            hipLaunchKernelGGL(ptm_bench::gather_kernel<4>, dim3(threads / 256), dim3(256), 0, 0, table, nblocks,
+                               gathers, seed, sink);
+        else
+            hipLaunchKernelGGL(ptm_bench::gather_kernel<12>, dim3(threads / 256), dim3(256), 0, 0, table, nblocks,
+                               gathers, seed, sink);
+        PTB_HIP(hipGetLastError());
+    }
     PTB_HIP(hipEventRecord(e1, 0));
     PTB_HIP(hipEventSynchronize(e1));
     PTB_HIP(hipEventElapsedTime(&ms, e0, e1));

@@ -43,10 +43,10 @@ KERNELS = {
 FLOP_COUNTERS = ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64")
 
 
-def gather_ceiling(table_bytes, block_bytes=512, threads=1 << 19, gathers=16, reps=5):
+def gather_ceiling(table_bytes, block_bytes=1536, threads=1 << 20, gathers=8, reps=5):
     """(random-gather GB/s, streaming-copy GB/s) of pt_gather_bandwidth for a table of
-    table_bytes: lanes reading their own block_bytes block from random blocks, as the
-    tricubic evaluation does."""
+    table_bytes: lanes reading their own block_bytes block (a cell's three components)
+    from random blocks, as the tricubic evaluation does."""
     from . import _native
     g, c = C.c_double(), C.c_double()
     _native.check(_native.lib().pt_gather_bandwidth(int(table_bytes), int(block_bytes), int(threads), int(gathers),
@@ -157,6 +157,69 @@ def summarize(kernel, fetch_db, write_db, algorithmic=None, flops_db=None):
             print(f"fp64_flops_per_launch: {sum(fl) / len(fl):.6e}")
 
 
+# counter passes of the step and collision kernels (each within one pass's per-block limits)
+COUNTER_PASSES = {
+    "sq": "SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR "
+          "SQ_INSTS_LDS",
+    "sq_wait": "SQ_ACTIVE_INST_ANY SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_IFETCH "
+               "SQ_INSTS_BRANCH SQ_INSTS_SMEM",
+    "mem": "TA_TA_BUSY_sum TA_ADDR_STALLED_BY_TC_CYCLES_sum TD_TD_BUSY_sum TD_TC_STALL_sum TCP_TCC_READ_REQ_sum "
+           "TCP_TCC_WRITE_REQ_sum TCP_PENDING_STALL_CYCLES_sum TCP_TOTAL_CACHE_ACCESSES_sum TCC_HIT_sum TCC_MISS_sum "
+           "TCC_EA0_RDREQ_sum GRBM_GUI_ACTIVE",
+}
+
+
+def summarize_counters(kernel, dbs):
+    """Per-dispatch averages of the counter passes (COUNTER_PASSES, one rocpd database each)
+    for the step kernel of `kernel` and the collision kernel of the same mode, with the
+    ratios the measurements quote."""
+    pattern = KERNELS[kernel][2]
+    mode = pattern[pattern.index("<"):]
+    kernels = {"step": pattern, "collision": "collision_kernel" + mode}
+    vals = {k: {} for k in kernels}
+    dur = {k: [] for k in kernels}
+    for db in dbs:
+        con = sqlite3.connect(db)
+        for k, pat in kernels.items():
+            rows = con.execute("select counter_name, avg(v), avg(d) from (select dispatch_id, counter_name, "
+                               "sum(value) as v, max(end - start) as d from counters_collection where kernel_name "
+                               "like ? group by dispatch_id, counter_name) group by counter_name",
+                               (f"%{pat}%",)).fetchall()
+            for c, v, d in rows:
+                vals[k][c] = float(v)
+                dur[k].append(float(d))
+    print(f"# rocprofv3 --pmc counter passes: {kernels['step']} and {kernels['collision']}, cylinder 1e6 "
+          f"neutrons, 64^3 map, {RK_STEPS} RK steps in {ROUNDS} rounds per call (averages per dispatch)")
+    print("commands:")
+    for name, counters in COUNTER_PASSES.items():
+        print(f"  rocprofv3 --pmc {counters} -d gpurun_out/pmc_{name} -o run -- python3 -m pentrack_amd.profiling "
+              f"run {kernel}")
+    print(f"  python3 -m pentrack_amd.profiling counters {kernel} " +
+          " ".join(f"gpurun_out/pmc_{n}/run_results.db" for n in COUNTER_PASSES))
+    for k in kernels:
+        v = vals[k]
+        print()
+        print(f"{kernels[k]}: {sum(dur[k]) / max(len(dur[k]), 1) / 1e6:.2f} ms per dispatch")
+        for c in sorted(v):
+            print(f"  {c}: {v[c]:.4e}")
+        g = v.get
+        if g("SQ_ACTIVE_INST_VALU") and g("SQ_THREAD_CYCLES_VALU"):
+            print(f"  active lanes per VALU instruction (THREAD_CYCLES_VALU / ACTIVE_INST_VALU, 64 = full): "
+                  f"{g('SQ_THREAD_CYCLES_VALU') / g('SQ_ACTIVE_INST_VALU'):.1f}")
+        if g("SQ_WAVE_CYCLES"):
+            for c in ("SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_ANY"):
+                if g(c):
+                    print(f"  {c} / SQ_WAVE_CYCLES: {g(c) / g('SQ_WAVE_CYCLES'):.3f}")
+        if g("TCC_HIT_sum") is not None and g("TCC_MISS_sum"):
+            print(f"  L2 hit rate: {g('TCC_HIT_sum') / (g('TCC_HIT_sum') + g('TCC_MISS_sum')):.3f}")
+        if g("GRBM_GUI_ACTIVE") and g("TD_TD_BUSY_sum"):
+            # GRBM_GUI_ACTIVE sums the 8 XCDs; the TA/TD/TCP sums run over the 256 CUs
+            per_cu = g("GRBM_GUI_ACTIVE") / 8
+            for c in ("TD_TD_BUSY_sum", "TD_TC_STALL_sum", "TA_TA_BUSY_sum", "TCP_PENDING_STALL_CYCLES_sum"):
+                if g(c):
+                    print(f"  {c} per CU / active cycles: {g(c) / 256 / per_cu:.3f}")
+
+
 def summarize_trace(csv_path, title, command):
     """The kernel table of a rocprofv3 --kernel-trace --output-format csv run: calls, average
     and total duration, registers, scratch and LDS of every kernel, slowest first."""
@@ -194,6 +257,8 @@ if __name__ == "__main__":
     elif len(sys.argv) in (5, 6, 7) and sys.argv[1] == "summarize":
         summarize(sys.argv[2], sys.argv[3], sys.argv[4], float(sys.argv[5]) if len(sys.argv) >= 6 else None,
                   sys.argv[6] if len(sys.argv) == 7 else None)
+    elif len(sys.argv) >= 4 and sys.argv[1] == "counters":
+        summarize_counters(sys.argv[2], sys.argv[3:])
     elif len(sys.argv) == 5 and sys.argv[1] == "trace":
         summarize_trace(sys.argv[2], sys.argv[3], sys.argv[4])
     else:
