@@ -101,6 +101,9 @@ def parse():
     p.add_argument("--no-ceiling", action="store_true")
     p.add_argument("--adj-n", type=int, default=100_000, help="differentiable leg: neutrons")
     p.add_argument("--adj-steps", type=int, default=2000, help="differentiable leg: RK4 steps (dt = 1e-4 s)")
+    p.add_argument("--adj-chunk", type=int, default=0,
+                   help="differentiable leg: steps per chunk, a multiple of 256 (0: one pass; config 5's 2e5 "
+                        "steps: 20224)")
     p.add_argument("--no-adjoint", action="store_true")
     return p.parse_args()
 
@@ -291,34 +294,49 @@ def uneven_leg(a, sptr):
 
 def adjoint_leg(a, sptr):
     """Differentiable RK4 path: cube, B = sum of 8 coil maps (64^3 each), dt = 1e-4 s;
-    forward with checkpoints every 256 steps, then the adjoint of J = <E_kin> + <z>."""
+    forward with checkpoints every 256 steps, then the adjoint of J = <E_kin> + <z>.
+    With --adj-chunk the horizon runs in chunks of that many steps (config 5's 2e5 steps;
+    pentrack_amd.adjoint.rk4_forward_chunks / rk4_adjoint_chunks), the GPU work timed as a
+    whole (the synchronisations between chunks only print progress)."""
     import torch
     from pentrack_amd import workloads
-    from pentrack_amd.adjoint import rk4_adjoint, rk4_forward
+    from pentrack_amd.adjoint import rk4_adjoint_chunks, rk4_forward, rk4_forward_chunks
     model, y0, s0 = workloads.adjoint_cube(n=a.adj_n, seed=5, n_grid=a.grid, nmaps=8, dt=1e-4)
     dev = torch.device("cuda:0")
     y0t = torch.tensor(y0, device=dev)
     rk4_forward(model, s0, y0t[:256], 16)          # warm-up (module load)
     torch.cuda.synchronize()
-    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    e[0].record()
-    yT, ck = rk4_forward(model, s0, y0t, a.adj_steps, 256)
-    e[1].record()
+    chunk = a.adj_chunk if 0 < a.adj_chunk < a.adj_steps else a.adj_steps
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+
+    def report(kind):
+        def f(j, nc, m):
+            if nc > 1:
+                torch.cuda.synchronize()
+                print(f"[bench] adjoint leg: {kind} chunk {j + 1}/{nc} ({m} steps) done", file=sys.stderr, flush=True)
+        return f
+    ev[0].record()
+    yT, cks, lens = rk4_forward_chunks(model, s0, y0t, a.adj_steps, chunk, 256, report("forward"))
+    ev[1].record()
     lamT = torch.zeros_like(yT)
     lamT[:, 3:] = model.mass * yT[:, 3:] / len(yT)
     lamT[:, 2] = 1.0 / len(yT)
-    e[2].record()
-    gy0, gs = rk4_adjoint(model, s0, ck, lamT, a.adj_steps, 256)
-    e[3].record()
+    ev[2].record()
+    lam, gs = rk4_adjoint_chunks(model, s0, cks, lens, lamT, 256, report("adjoint"))
+    ev[3].record()
     torch.cuda.synchronize()
-    fwd, adj = e[0].elapsed_time(e[1]), e[2].elapsed_time(e[3])
+    fwd, adj = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
     steps = a.adj_n * a.adj_steps
+    finite = bool(torch.isfinite(gs).all().item() and torch.isfinite(lam).all().item())
     print(f"[bench] adjoint leg: forward {fwd:.1f} ms, adjoint {adj:.1f} ms (ratio {adj / fwd:.2f}), "
           f"{steps / (fwd / 1e3):.3e} RK4 steps/s forward", file=sys.stderr, flush=True)
     return {"workload": f"cube, {a.adj_n} neutrons, B = sum of 8 coil maps ({a.grid}^3 each), RK4 dt=1e-4 s, "
-                        f"{a.adj_steps} steps, checkpoints every 256, J = <E_kin> + <z>",
+                        f"{a.adj_steps} steps ({a.adj_steps * 1e-4:g} s) in {len(lens)} chunk(s), checkpoints "
+                        "every 256, specular cube walls as reflection maps at the steps that crossed them, "
+                        "J = <E_kin> + <z>",
             "forward_ms": fwd, "adjoint_ms": adj, "adjoint_over_forward": adj / fwd,
-            "rk4_steps_per_s_forward": steps / (fwd / 1e3), "dJ_ds": gs.sum(0).cpu().tolist()}
+            "rk4_steps_per_s_forward": steps / (fwd / 1e3), "gradients_finite": finite,
+            "dJ_ds": gs.sum(0).cpu().tolist()}
 
 
 def read_profile(kernel, key):

@@ -142,6 +142,36 @@ def rk4_adjoint(model, s, ckpt, lamT, nsteps, K=256):
     return lam.t(), sbar[:model.nmaps].t()
 
 
+def rk4_forward_chunks(model, s, y0, nsteps, chunk, K=256, progress=None):
+    """The forward pass of a long horizon in chunks of `chunk` steps (a multiple of K, so
+    the checkpoints fall where one pass puts them): (yT, [checkpoints of each chunk],
+    [steps of each chunk]).  progress(j, nchunks, steps) is called after each chunk."""
+    if chunk < nsteps and chunk % K:
+        raise ValueError("the chunk length must be a multiple of the checkpoint interval K")
+    lens = [chunk] * (nsteps // chunk) + ([nsteps % chunk] if nsteps % chunk else [])
+    cks, y = [], y0
+    for j, m in enumerate(lens):
+        y, ck = rk4_forward(model, s, y, m, K)
+        cks.append(ck)
+        if progress:
+            progress(j, len(lens), m)
+    return y, cks, lens
+
+
+def rk4_adjoint_chunks(model, s, cks, lens, lamT, K=256, progress=None):
+    """The adjoint of rk4_forward_chunks' pass: the chunks backwards, dJ/dy at a chunk's end
+    the adjoint state at its start, dJ/ds summed over the chunks.  Returns (dJ/dy0 [n][6],
+    dJ/ds per particle [n][nmaps]); each chunk's checkpoints are released once used."""
+    lam, gs = lamT, None
+    for j in reversed(range(len(lens))):
+        lam, g = rk4_adjoint(model, s, cks[j], lam, lens[j], K)
+        cks[j] = None
+        gs = g if gs is None else gs + g
+        if progress:
+            progress(j, len(lens), lens[j])
+    return lam, gs
+
+
 def rk4_tracker(model, nsteps, K=256):
     """torch.autograd.Function applying the tracker: f(s, y0) -> yT."""
     import torch

@@ -95,15 +95,24 @@ PTD bool tric_locate(const Field &f, double x, double y, double z, long &cell, d
  * Device layout of a block: a[16 i + 4 j + k] holds the coefficient of x^i y^j z^k,
  * so the 16 coefficients of one power of x are one contiguous 128-byte chunk.
  */
+PTD void load_line64(const double *__restrict__ a, double c[16]) {
+    const double2 *c2 = reinterpret_cast<const double2 *>(a);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) { double2 v = gload(c2 + q); c[2 * q] = v.x; c[2 * q + 1] = v.y; }
+}
+
 PTD void tric_eval_grad64(const double *__restrict__ a, double x, double y, double z, bool grad,
                           double &F, double &Fx, double &Fy, double &Fz) {
     double ry[4], ryd[4], ryz[4];
+    /* two coefficient lines in flight: the next power of x is requested before the
+       current one is used (the lines are independent, the arithmetic unchanged) */
+    double c[16], cn[16];
+    load_line64(a + 48, cn);
 #pragma unroll
     for (int i = 3; i >= 0; --i) {
-        const double2 *c2 = reinterpret_cast<const double2 *>(a + 16 * i);
-        double c[16];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) { double2 v = gload(c2 + q); c[2 * q] = v.x; c[2 * q + 1] = v.y; }
+        for (int q = 0; q < 16; ++q) c[q] = cn[q];
+        if (i > 0) load_line64(a + 16 * (i - 1), cn);
         double rY = 0., rYd = 0., rYz = 0.;
 #pragma unroll
         for (int j = 3; j >= 0; --j) {

@@ -61,3 +61,27 @@ def test_autograd_gradient_against_finite_differences(case, gpu):
         fd = (J_of(sp) - J_of(sm)) / (2 * h)
         assert abs(gs[k] - fd) <= 1e-5 * abs(fd) + 1e-12, (k, gs[k], fd)
     assert y.grad is not None and torch.isfinite(y.grad).all()
+
+
+def test_chunked_horizon_equals_one_pass(case, gpu):
+    """A horizon run in chunks (config 5's 2e5 steps are run that way): the same final
+    states and dJ/dy0 bit for bit, dJ/ds to rounding (the per-chunk sums are added)."""
+    import torch
+    from pentrack_amd.adjoint import rk4_adjoint_chunks, rk4_forward_chunks
+    model, y0, s0 = case
+    nsteps, K, chunk = 1100, 64, 256
+    print(f"case: adjoint chunks n=500 steps={nsteps} chunk={chunk} K={K}", flush=True)
+    y0t = torch.tensor(y0, device=gpu)
+    yT, ck = rk4_forward(model, s0, y0t, nsteps, K)
+    yc, cks, lens = rk4_forward_chunks(model, s0, y0t, nsteps, chunk, K)
+    assert lens == [256, 256, 256, 256, 76]
+    assert torch.equal(yT, yc)
+    lamT = torch.zeros_like(yT)
+    lamT[:, 3:] = model.mass * yT[:, 3:] / len(yT)
+    lamT[:, 2] = 1.0 / len(yT)
+    gy0, gs = rk4_adjoint(model, s0, ck, lamT, nsteps, K)
+    cy0, cgs = rk4_adjoint_chunks(model, s0, cks, lens, lamT, K)
+    assert torch.equal(gy0, cy0)
+    assert np.allclose(cgs.cpu().numpy(), gs.cpu().numpy(), rtol=1e-12, atol=1e-14 * np.abs(gs.cpu().numpy()).max())
+    with pytest.raises(ValueError):
+        rk4_forward_chunks(model, s0, y0t, nsteps, 100, K)
