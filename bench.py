@@ -61,6 +61,9 @@ VARIANTS = {
     "fp64_persistent_xcd": dict(mixed=False, persistent=1, graph=0, spatial=2),
     "fp64_persistent_xcd_graph": dict(mixed=False, persistent=1, graph=1, spatial=2),
     "mixed_persistent_xcd": dict(mixed=True, persistent=1, graph=0, spatial=2),
+    # collision queue ordered by candidate count
+    "fp64_persistent_corder": dict(mixed=False, persistent=1, graph=0, corder=1),
+    "fp64_corder": dict(mixed=False, persistent=0, graph=0, corder=1),
 }
 # graph replays carry no per-kernel events: the roofline uses the same kernels timed eagerly
 PROFILED_FOR = {"fp64_persistent_graph": "fp64_persistent", "mixed_persistent_graph": "mixed_persistent",
@@ -386,7 +389,8 @@ def main():
         torch.cuda.synchronize()
         t_up = time.perf_counter() - t_up   # the particle state's allocation and upload from host arrays
         cfg = LaunchConfig(variant=vid, block=a.block, steps_per_launch=a.rk, rounds=a.rounds,
-                           persistent=kw["persistent"], graph=kw["graph"], spatial_order=kw.get("spatial", 0))
+                           persistent=kw["persistent"], graph=kw["graph"], spatial_order=kw.get("spatial", 0),
+                           collision_order=kw.get("corder", 0))
         for _ in range(a.warmup):
             b.run(cfg, sptr)
         torch.cuda.synchronize()

@@ -232,6 +232,10 @@ typedef struct {
                                     contiguous chunk per XCD, taken first by the blocks on that XCD
                                     (neighbouring particles share its L2); 0 in index order.
                                     Results are equal. */
+    int collision_order;         /* 1: the collision kernel takes the parked particles ordered by the
+                                    number of candidate triangles of their chord piece (counted and
+                                    counting-sorted first), so a wave's lanes loop over lists of
+                                    similar length; 0 in queue order.  Results are equal. */
 } pt_launch_cfg;
 
 typedef struct pt_batch pt_batch;

@@ -98,7 +98,7 @@ class LaunchCfg(C.Structure):
     _fields_ = [
         ("variant", C.c_int), ("block", C.c_int), ("steps_per_launch", C.c_int),
         ("persistent", C.c_int), ("grid", C.c_int), ("rounds", C.c_int), ("graph", C.c_int),
-        ("spatial_order", C.c_int),
+        ("spatial_order", C.c_int), ("collision_order", C.c_int),
     ]
 
 

@@ -195,6 +195,9 @@ struct State {
                                  next item of chunk c of the list split by XCD */
     int *pkey;                /* spatial bucket of each particle (work-list ordering) */
     int *bhist;               /* [NBUCKET + 1] bucket counts, then running offsets */
+    int *qkey;                /* candidate count of each queue entry's chord piece (collision order) */
+    int *qsorted;             /* the collision queue ordered by qkey */
+    int *qhist;               /* [QKEYS + 1] counts of the keys, then running offsets */
 };
 
 constexpr int BUCKET_BITS = 5;                     /* 32 buckets per axis */
@@ -204,6 +207,7 @@ constexpr int NBUCKET = 1 << (3 * BUCKET_BITS);     /* Morton-ordered spatial bu
    each chunk's counter sits in its own 64-byte line */
 constexpr int NXCD = 8, WC_CHUNK0 = 16, WC_STRIDE = 16, WCOUNT_LEN = WC_CHUNK0 + NXCD * WC_STRIDE;
 constexpr int DENSE = 59;     /* xo[8] dxo[8] k3..k6[32] dxc[8] t t_old pol */
+constexpr int QKEYS = 26;     /* collision-order keys: candidate counts 0..24, 25 = no list */
 
 /* field containers and species held in the parameter block itself, so that the kernels
    read them through the parameter pointer (uniform scalar loads) */
@@ -526,6 +530,7 @@ PTD_NOINLINE int2 seg_query_v(const Geom &G, double p0, double p1, double p2, do
  * full traversal's.  n < 0: no list (too many candidates).
  */
 constexpr int MAX_CAND = 24;
+static_assert(QKEYS == MAX_CAND + 2, "a key per candidate count and one for no list");
 struct Cand {
     int n;
     double lo[3], hi[3];
@@ -540,7 +545,9 @@ struct Cand {
 constexpr int CAND_LDS = 5, CAND_LANES = 256;
 __shared__ float cand_lds[CAND_LDS][9][CAND_LANES];
 
-PTD void cand_build(const Geom &G, const double *lo, const double *hi, Cand &C) {
+/* the candidates of a chord piece: triangles whose padded box meets [lo, hi], in BVH leaf
+   order; stage = false counts them without staging their vertices in LDS */
+PTD void cand_build(const Geom &G, const double *lo, const double *hi, Cand &C, bool stage = true) {
     C.n = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) { C.lo[k] = lo[k]; C.hi[k] = hi[k]; }
@@ -575,7 +582,7 @@ PTD void cand_build(const Geom &G, const double *lo, const double *hi, Cand &C) 
                     }
                     if (!in) continue;
                     if (C.n >= MAX_CAND) { C.n = -1; return; }
-                    if (C.n < CAND_LDS) {
+                    if (stage && C.n < CAND_LDS) {
                         const int l = threadIdx.x;
                         cand_lds[C.n][0][l] = (float)a[0]; cand_lds[C.n][1][l] = (float)a[1]; cand_lds[C.n][2][l] = (float)a[2];
                         cand_lds[C.n][3][l] = (float)b[0]; cand_lds[C.n][4][l] = (float)b[1]; cand_lds[C.n][5][l] = (float)b[2];
@@ -592,6 +599,21 @@ PTD void cand_build(const Geom &G, const double *lo, const double *hi, Cand &C) 
         }
         if (sp == 0) break;
         node = stack[--sp];
+    }
+}
+
+/* the box CheckHit's candidates are collected in for the chord piece y1 -> y2: every sub-segment
+   of the search has its ends on the dense-output path between y1 and y2; a path of length l with
+   chord d strays at most sqrt(l^2 - d^2)/2 from the chord, so the box of the chord widened by
+   twice that contains them (ends outside it are detected and fall back to the BVH) */
+PTD void piece_cand_box(const double *y1, const double *y2, double lo[3], double hi[3]) {
+    double l2 = (y2[8] - y1[8]) * (y2[8] - y1[8]);
+    double d2 = (y2[0] - y1[0]) * (y2[0] - y1[0]) + (y2[1] - y1[1]) * (y2[1] - y1[1]) + (y2[2] - y1[2]) * (y2[2] - y1[2]);
+    double dev = l2 > d2 ? __builtin_sqrt(l2 - d2) : 0.0;
+    double w = dev + 1e-6 + 1e-9 * __builtin_sqrt(d2);
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = (y1[k] < y2[k] ? y1[k] : y2[k]) - w;
+        hi[k] = (y1[k] < y2[k] ? y2[k] : y1[k]) + w;
     }
 }
 

@@ -1241,6 +1241,9 @@ extern "C" int pt_batch_create(const pt_geometry *g, const pt_fields *f, int n_s
     if ((r = B->mem.upload(&S.wcount, wzero.data(), wzero.size()))) return r;
     if ((r = B->mem.alloc(&S.pkey, N))) return r;
     if ((r = B->mem.alloc(&S.bhist, (size_t)ptd::NBUCKET + 1))) return r;
+    if ((r = B->mem.alloc(&S.qkey, N))) return r;
+    if ((r = B->mem.alloc(&S.qsorted, N))) return r;
+    if ((r = B->mem.alloc(&S.qhist, (size_t)ptd::QKEYS + 1))) return r;
     int zero = 0;
     if ((r = B->mem.upload(&B->d_quit, &zero, 1))) return r;
     if ((r = B->mem.upload(&B->d_count, &zero, 1))) return r;
@@ -1269,6 +1272,7 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
         if (any_map) return fail(PT_EINVAL, "pt_batch_run: mixed precision needs fields created with FP32 tables");
     }
     if (cfg->spatial_order < 0 || cfg->spatial_order > 2) return fail(PT_EINVAL, "pt_batch_run: spatial_order must be 0, 1 or 2");
+    if (cfg->collision_order < 0 || cfg->collision_order > 1) return fail(PT_EINVAL, "pt_batch_run: collision_order must be 0 or 1");
     int block = cfg->block > 0 ? cfg->block : 256;
     if (block % 64 != 0 || block > 256) return fail(PT_EINVAL, "pt_batch_run: block must be a multiple of 64 and <= 256");
     {
@@ -1307,7 +1311,7 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
                     PT_HIP(hipEventCreate(&e1));
                     PT_HIP(hipEventRecord(e0, s));
                 }
-                if (k == 1) PT_HIP(pt_launch_collision(b->P, mode, coll_grid, s));
+                if (k == 1) PT_HIP(pt_launch_collision(b->P, mode, coll_grid, cfg->collision_order, s));
                 else if (cfg->persistent) PT_HIP(pt_launch_step_persistent(b->P, mode, pgrid, cfg->spatial_order, s));
                 else PT_HIP(pt_launch_step(b->P, mode, block, s));
                 if (timed) {
@@ -1320,7 +1324,8 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
     };
     if (!cfg->graph) return launch_all(b->profiling);
     if (s == nullptr) return fail(PT_EINVAL, "pt_batch_run: graph mode needs a non-default stream");
-    std::vector<int> key = {cfg->variant, block, steps, rounds, cfg->persistent, pgrid, cfg->spatial_order};
+    std::vector<int> key = {cfg->variant, block, steps, rounds, cfg->persistent, pgrid, cfg->spatial_order,
+                            cfg->collision_order};
     pt_batch::GraphEntry *ge = nullptr;
     for (auto &g : b->graphs)
         if (g.key == key) ge = &g;

@@ -23,7 +23,7 @@ namespace ptd { struct Params; struct Geom; }
 template <int MODE> hipError_t pt_launch_init_mode(const ptd::Params &P, int grid, hipStream_t stream);
 template <int MODE> hipError_t pt_launch_step_mode(const ptd::Params &P, int grid, int block, hipStream_t stream);
 template <int MODE> hipError_t pt_launch_step_persistent_mode(const ptd::Params &P, int grid, int split, hipStream_t stream);
-template <int MODE> hipError_t pt_launch_collision_mode(const ptd::Params &P, int grid, hipStream_t stream);
+template <int MODE> hipError_t pt_launch_collision_mode(const ptd::Params &P, int grid, const int *queue, hipStream_t stream);
 template <int MODE>
 hipError_t pt_launch_fields_eval_mode(const ptd::Params &P, int n, const double *pts, const double *t, double *B,
                                       double *dB, double *V, double *E, hipStream_t stream);
@@ -35,7 +35,7 @@ hipError_t pt_launch_step(const ptd::Params &P, int mode, int block, hipStream_t
 /* persistent variant of the step kernel: the work list (in index order, or with spatial != 0
    in Morton order of the particle positions), then `grid` blocks of 256 slots */
 hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, int spatial, hipStream_t stream);
-hipError_t pt_launch_collision(const ptd::Params &P, int mode, int coll_grid, hipStream_t stream);
+hipError_t pt_launch_collision(const ptd::Params &P, int mode, int coll_grid, int order, hipStream_t stream);
 /* field sum (FP64 tables) at n points and times; general: containers with formulas or analytic fields */
 hipError_t pt_launch_fields_eval(const ptd::Params &P, int general, int n, const double *pts, const double *t, double *B,
                                  double *dB, double *V, double *E, hipStream_t stream);

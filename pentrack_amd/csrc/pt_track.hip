@@ -1321,19 +1321,8 @@ PTD_NOINLINE bool check_hit(const Params &P, Track &T, double x1, const double *
     double y1[9];
     for (int k = 0; k < 9; ++k) y1[k] = y1in[k];
     {
-        /* every sub-segment of the search has its ends on the dense-output path between
-           y1 and y2; a path of length l with chord d strays at most sqrt(l^2 - d^2)/2 from
-           the chord, so the box of the chord widened by twice that contains them (ends
-           outside it are detected and fall back to the BVH) */
-        double l2 = (y2[8] - y1[8]) * (y2[8] - y1[8]);
-        double d2 = (y2[0] - y1[0]) * (y2[0] - y1[0]) + (y2[1] - y1[1]) * (y2[1] - y1[1]) + (y2[2] - y1[2]) * (y2[2] - y1[2]);
-        double dev = l2 > d2 ? __builtin_sqrt(l2 - d2) : 0.0;
-        double w = dev + 1e-6 + 1e-9 * __builtin_sqrt(d2);
         double lo[3], hi[3];
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = (y1[k] < y2[k] ? y1[k] : y2[k]) - w;
-            hi[k] = (y1[k] < y2[k] ? y2[k] : y1[k]) + w;
-        }
+        piece_cand_box(y1, y2, lo, hi);
         cand_build(P.geom, lo, hi, T.cand);
     }
     for (;;) {
@@ -2659,14 +2648,66 @@ __global__ void __launch_bounds__(256) bucket_scatter_kernel(const Params *__res
 #endif
 static_assert(CAND_LANES == 256, "the collision kernel's blocks fill the candidate staging area");
 template <int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_COLL_MIN_WAVES))) collision_kernel(const Params *__restrict__ Pp) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_COLL_MIN_WAVES))) collision_kernel(const Params *__restrict__ Pp, const int *__restrict__ queue) {
     const Params &P = *Pp;
     const int count = *P.st.qcount;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
         Ctr c = {0, 0, 0};
-        collision_path<MODE>(P, P.st.queue[j], c);
+        collision_path<MODE>(P, queue[j], c);
     }
 }
+
+/* ---- collision order: the queue counting-sorted by the candidate count of each parked
+   chord piece (the particles are independent, so the order changes no result) ---- */
+#if PT_COMMON_TU
+__global__ void qclear_kernel(int *qhist) {
+    if (threadIdx.x <= QKEYS) qhist[threadIdx.x] = 0;
+}
+
+/* key of each queue entry: the number of candidates CheckHit will collect for its piece
+   (QKEYS - 1 when there are more than MAX_CAND, 0 for a spin-only piece of zero length) */
+__global__ void __launch_bounds__(256) qkey_kernel(const Params *__restrict__ Pp) {
+    const Params &P = *Pp;
+    const State &S = P.st;
+    const int count = *S.qcount;
+    const size_t n = (size_t)S.n;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
+        const int i = S.queue[j];
+        const double *pd = S.pend;
+        int key = 0;
+        if (pd[10 * n + i] > pd[i]) {
+            double y1[9], y2[9];
+            for (int k = 0; k < 9; ++k) { y1[k] = pd[(1 + k) * n + i]; y2[k] = pd[(11 + k) * n + i]; }
+            double lo[3], hi[3];
+            piece_cand_box(y1, y2, lo, hi);
+            Cand C;
+            cand_build(P.geom, lo, hi, C, false);
+            key = C.n < 0 ? QKEYS - 1 : C.n;
+        }
+        S.qkey[j] = key;
+        gatomic_add(S.qhist + key, 1);
+    }
+}
+
+/* exclusive prefix sum of the key counts (QKEYS of them: one thread) */
+__global__ void qscan_kernel(int *qhist) {
+    int run = 0;
+    for (int k = 0; k < QKEYS; ++k) {
+        const int c = qhist[k];
+        qhist[k] = run;
+        run += c;
+    }
+}
+
+__global__ void __launch_bounds__(256) qscatter_kernel(const Params *__restrict__ Pp) {
+    const State &S = Pp->st;
+    const int count = *S.qcount;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
+        const int slot = gatomic_add(S.qhist + S.qkey[j], 1);
+        S.qsorted[slot] = S.queue[j];
+    }
+}
+#endif
 
 /* empties the collision queue (a kernel, not a memset, so launch loops can be captured) */
 #if PT_COMMON_TU
@@ -2703,9 +2744,9 @@ hipError_t pt_launch_step_persistent_mode(const ptd::Params &P, int grid, int sp
     return hipGetLastError();
 }
 template <int MODE>
-hipError_t pt_launch_collision_mode(const ptd::Params &P, int grid, hipStream_t stream) {
+hipError_t pt_launch_collision_mode(const ptd::Params &P, int grid, const int *queue, hipStream_t stream) {
     /* blocks of CAND_LANES threads: the size of the candidate staging area in LDS */
-    hipLaunchKernelGGL(ptd::collision_kernel<MODE>, dim3(grid), dim3(ptd::CAND_LANES), 0, stream, P.self);
+    hipLaunchKernelGGL(ptd::collision_kernel<MODE>, dim3(grid), dim3(ptd::CAND_LANES), 0, stream, P.self, queue);
     return hipGetLastError();
 }
 template <int MODE>
@@ -2723,7 +2764,7 @@ template hipError_t pt_launch_step_mode<PT_MODE_TU>(const ptd::Params &, int, in
 #elif PT_KERNEL_SET == 2
 template hipError_t pt_launch_step_persistent_mode<PT_MODE_TU>(const ptd::Params &, int, int, hipStream_t);
 #else
-template hipError_t pt_launch_collision_mode<PT_MODE_TU>(const ptd::Params &, int, hipStream_t);
+template hipError_t pt_launch_collision_mode<PT_MODE_TU>(const ptd::Params &, int, const int *, hipStream_t);
 #if (PT_MODE_TU & 1) == 0 /* fields_eval runs on FP64 tables only */
 template hipError_t pt_launch_fields_eval_mode<PT_MODE_TU>(const ptd::Params &, int, const double *, const double *,
                                                            double *, double *, double *, double *, hipStream_t);
@@ -2807,14 +2848,23 @@ hipError_t pt_launch_decay(const ptd::Params &P, pt_beta_table T, pt_secondary *
     return hipGetLastError();
 }
 
-hipError_t pt_launch_collision(const ptd::Params &P, int mode, int coll_grid, hipStream_t stream) {
+hipError_t pt_launch_collision(const ptd::Params &P, int mode, int coll_grid, int order, hipStream_t stream) {
     if (P.st.n == 0) return hipSuccess;
     hipError_t e;
+    const int *queue = P.st.queue;
+    if (order) {
+        hipLaunchKernelGGL(ptd::qclear_kernel, dim3(1), dim3(64), 0, stream, P.st.qhist);
+        hipLaunchKernelGGL(ptd::qkey_kernel, dim3(coll_grid), dim3(256), 0, stream, P.self);
+        hipLaunchKernelGGL(ptd::qscan_kernel, dim3(1), dim3(1), 0, stream, P.st.qhist);
+        hipLaunchKernelGGL(ptd::qscatter_kernel, dim3(coll_grid), dim3(256), 0, stream, P.self);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        queue = P.st.qsorted;
+    }
     switch (mode) {
-    case 0: e = pt_launch_collision_mode<0>(P, coll_grid, stream); break;
-    case 1: e = pt_launch_collision_mode<1>(P, coll_grid, stream); break;
-    case 2: e = pt_launch_collision_mode<2>(P, coll_grid, stream); break;
-    default: e = pt_launch_collision_mode<3>(P, coll_grid, stream); break;
+    case 0: e = pt_launch_collision_mode<0>(P, coll_grid, queue, stream); break;
+    case 1: e = pt_launch_collision_mode<1>(P, coll_grid, queue, stream); break;
+    case 2: e = pt_launch_collision_mode<2>(P, coll_grid, queue, stream); break;
+    default: e = pt_launch_collision_mode<3>(P, coll_grid, queue, stream); break;
     }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(ptd::reset_queue_kernel, dim3(1), dim3(1), 0, stream, P.st.qcount);

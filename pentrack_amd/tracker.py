@@ -25,6 +25,7 @@ class LaunchConfig:
     rounds: int = 0
     graph: int = 0
     spatial_order: int = 0
+    collision_order: int = 0
     leaf_size: int = 4
 
     def native(self):
@@ -32,6 +33,7 @@ class LaunchConfig:
         c.variant, c.block, c.steps_per_launch, c.persistent, c.grid, c.rounds, c.graph, c.spatial_order = (
             self.variant, self.block, self.steps_per_launch, self.persistent, self.grid, self.rounds, self.graph,
             self.spatial_order)
+        c.collision_order = self.collision_order
         return c
 
 

@@ -218,6 +218,12 @@ def test_persistent_work_queue_bit_exact(gpu, oracle_lib):
     print("case: persistent cylinder n=1500 spatial order split by XCD", flush=True)
     x, xh = run_gpu(w, cfg=LaunchConfig(persistent=1, steps_per_launch=9, grid=64, spatial_order=2))
     assert_bit_exact(x, xh, o, oh)
+    print("case: persistent cylinder n=1500 collision queue in candidate-count order", flush=True)
+    q, qh = run_gpu(w, cfg=LaunchConfig(persistent=1, steps_per_launch=9, grid=64, collision_order=1))
+    assert_bit_exact(q, qh, o, oh)
+    print("case: cylinder n=1500 collision queue in candidate-count order", flush=True)
+    q, qh = run_gpu(w, cfg=LaunchConfig(steps_per_launch=9, collision_order=1))
+    assert_bit_exact(q, qh, o, oh)
 
 
 def test_graph_replays_match_eager_calls(gpu):
@@ -232,7 +238,7 @@ def test_graph_replays_match_eager_calls(gpu):
     s = torch.cuda.Stream()
     sp = s.cuda_stream
     for cfg_kw in ({}, {"persistent": 1, "grid": 32}, {"persistent": 1, "grid": 32, "spatial_order": 1},
-                   {"persistent": 1, "grid": 32, "spatial_order": 2}):
+                   {"persistent": 1, "grid": 32, "spatial_order": 2}, {"persistent": 1, "grid": 32, "collision_order": 1}):
         for call in range(6):
             a.run(LaunchConfig(steps_per_launch=13, **cfg_kw), sp)
             b.run(LaunchConfig(steps_per_launch=13, graph=1, **cfg_kw), sp)
