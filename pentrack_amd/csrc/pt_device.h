@@ -72,8 +72,13 @@ constexpr int BVH_STACK = 48;
 
 /* ---- device-side data ---------------------------------------------------- */
 
+/* BVH nodes in preorder (an internal node's left child is the next node).  On the device
+   an internal node holds its escape index, the first node after its subtree, so the
+   traversals need no stack: a box that is hit is entered at the next node, one that is
+   missed is left for its escape; leaves and their triangles are visited in the same
+   depth-first order as with a stack.  The host's build keeps the left child in a. */
 struct BvhNode {          /* 32 B: two 16-B loads */
-    float lo[3]; int a;   /* internal: a = left child; leaf: a = -(first + 1) */
+    float lo[3]; int a;   /* internal: a = escape index (host build: left child); leaf: a = -(first + 1) */
     float hi[3]; int b;   /* internal: b = right child; leaf: b = count */
 };
 
@@ -469,16 +474,18 @@ PTD_NOINLINE int2 seg_query_v(const Geom &G, double p0, double p1, double p2, do
     double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
     double inv[3] = {1.0 / d[0], 1.0 / d[1], 1.0 / d[2]};
     double sl = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    int stack[BVH_STACK];
-    int sp = 0;
     int node = 0;
     int count = 0;
-    if (G.n_node == 0) return make_int2(0, 0);
-    for (;;) {
+    while (node < G.n_node) {
         BvhNode nd;
         load_node(G.node + node, nd);
-        if (node_hit(nd, p, inv, d)) {
-            if (nd.a < 0) {
+        const bool hit = node_hit(nd, p, inv, d);
+        if (nd.a >= 0) {
+            node = hit ? node + 1 : nd.a;
+            continue;
+        }
+        if (hit) {
+            {
                 int first = -nd.a - 1;
                 for (int k = 0; k < nd.b; ++k) {
                     double a[3], b[3], c[3], pt[3];
@@ -507,15 +514,9 @@ PTD_NOINLINE int2 seg_query_v(const Geom &G, double p0, double p1, double p2, do
                         list->n++;
                     }
                 }
-            } else {
-                if (sp < BVH_STACK) stack[sp++] = nd.b;
-                else return make_int2(-2, ntests);
-                node = nd.a;
-                continue;
             }
         }
-        if (sp == 0) break;
-        node = stack[--sp];
+        ++node;
     }
     return make_int2(count, ntests);
 }
@@ -551,10 +552,8 @@ PTD void cand_build(const Geom &G, const double *lo, const double *hi, Cand &C, 
     C.n = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) { C.lo[k] = lo[k]; C.hi[k] = hi[k]; }
-    if (G.n_node == 0) return;
-    int stack[BVH_STACK];
-    int sp = 0, node = 0;
-    for (;;) {
+    int node = 0;
+    while (node < G.n_node) {
         BvhNode nd;
         load_node(G.node + node, nd);
         bool hit = true;
@@ -563,8 +562,12 @@ PTD void cand_build(const Geom &G, const double *lo, const double *hi, Cand &C, 
             double pad = 1e-9 + 1e-7 * (dabs((double)nd.lo[k]) + dabs((double)nd.hi[k]));
             hit = hit && ((double)nd.lo[k] - pad <= hi[k]) && ((double)nd.hi[k] + pad >= lo[k]);
         }
+        if (nd.a >= 0) {
+            node = hit ? node + 1 : nd.a;
+            continue;
+        }
         if (hit) {
-            if (nd.a < 0) {
+            {
                 int first = -nd.a - 1;
                 for (int k = 0; k < nd.b; ++k) {
                     double a[3], b[3], c[3];
@@ -590,15 +593,9 @@ PTD void cand_build(const Geom &G, const double *lo, const double *hi, Cand &C, 
                     }
                     C.idx[C.n++] = first + k;
                 }
-            } else {
-                if (sp >= BVH_STACK) { C.n = -1; return; }
-                stack[sp++] = nd.b;
-                node = nd.a;
-                continue;
             }
         }
-        if (sp == 0) break;
-        node = stack[--sp];
+        ++node;
     }
 }
 
@@ -679,17 +676,14 @@ PTD int seg_query(const Geom &G, const double *p, const double *q, HitList *list
 
 /* +z ray parity per solid (TTriangleMesh::GetSolids, include/trianglemesh.h:181-188):
    bit m of the result is set if the ray from p crosses solid m an odd number of times.
-   *overflow is set (and the parity is not to be used) if the traversal stack ran out;
-   pt_geometry_create rejects BVHs deeper than BVH_STACK - 2, so that does not happen for a
-   geometry built there, but a traversal never drops a subtree silently either. */
+   *overflow reports a traversal that could not visit every node; the stackless traversal
+   always can, so it is 0 (kept for the callers' checks). */
 PTD_NOINLINE unsigned long long ray_parity(const Geom &G, double p0, double p1, double p2, int *overflow) {
     const double p[3] = {p0, p1, p2};
     unsigned long long par = 0;
     *overflow = 0;
-    int stack[BVH_STACK];
-    int sp = 0, node = 0;
-    if (G.n_node == 0) return 0;
-    for (;;) {
+    int node = 0;
+    while (node < G.n_node) {
         BvhNode nd;
         load_node(G.node + node, nd);
         double pad0 = 1e-9 + 1e-7 * (dabs(nd.lo[0]) + dabs(nd.hi[0]));
@@ -698,24 +692,20 @@ PTD_NOINLINE unsigned long long ray_parity(const Geom &G, double p0, double p1, 
         bool hit = p[0] >= (double)nd.lo[0] - pad0 && p[0] <= (double)nd.hi[0] + pad0 &&
                    p[1] >= (double)nd.lo[1] - pad1 && p[1] <= (double)nd.hi[1] + pad1 &&
                    p[2] <= (double)nd.hi[2] + pad2;
+        if (nd.a >= 0) {
+            node = hit ? node + 1 : nd.a;
+            continue;
+        }
         if (hit) {
-            if (nd.a < 0) {
-                int first = -nd.a - 1;
-                for (int k = 0; k < nd.b; ++k) {
-                    double a[3], b[3], c[3];
-                    int solid, orig;
-                    load_tri(G.tri + first + k, a, b, c, solid, orig);
-                    if (ray_tri(a, b, c, p)) par ^= (1ull << solid);
-                }
-            } else {
-                if (sp >= BVH_STACK) { *overflow = 1; return 0; }
-                stack[sp++] = nd.b;
-                node = nd.a;
-                continue;
+            int first = -nd.a - 1;
+            for (int k = 0; k < nd.b; ++k) {
+                double a[3], b[3], c[3];
+                int solid, orig;
+                load_tri(G.tri + first + k, a, b, c, solid, orig);
+                if (ray_tri(a, b, c, p)) par ^= (1ull << solid);
             }
         }
-        if (sp == 0) break;
-        node = stack[--sp];
+        ++node;
     }
     return par;
 }

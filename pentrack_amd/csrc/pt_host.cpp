@@ -340,7 +340,20 @@ extern "C" int pt_geometry_create(int n_tri, const float *tri_vertices, const in
     std::vector<int> smat(solid_mat, solid_mat + n_solid);
     if ((r = G->mem.upload(&dtri, tris.data(), tris.size()))) return r;
     if ((r = G->mem.upload(&dnrm, nrm_ord.data(), nrm_ord.size()))) return r;
-    if ((r = G->mem.upload(&dnode, nodes.data(), nodes.size()))) return r;
+    {
+        /* device copy of the nodes with escape indices in place of left children: the end of
+           a node's subtree in preorder, a leaf's is the next node */
+        std::vector<ptd::BvhNode> dev(nodes);
+        std::vector<int> end(nodes.size());
+        for (int k = (int)nodes.size() - 1; k >= 0; --k) {
+            if (nodes[k].a < 0) { end[k] = k + 1; continue; }
+            if (nodes[k].a != k + 1 || nodes[k].b <= k || nodes[k].b >= (int)nodes.size())
+                return fail(PT_EINVAL, "pt_geometry_create: BVH nodes not in preorder");
+            end[k] = end[nodes[k].b];
+            dev[k].a = end[k];
+        }
+        if ((r = G->mem.upload(&dnode, dev.data(), dev.size()))) return r;
+    }
     if ((r = G->mem.upload(&dsid, solid_id, (size_t)n_solid))) return r;
     if ((r = G->mem.upload(&dsmat, smat.data(), smat.size()))) return r;
     if ((r = G->mem.upload(&dign_off, ioff.data(), ioff.size()))) return r;

@@ -16,8 +16,9 @@ namespace ptd { struct Params; struct Geom; }
    analytic fields or conductors present */
 
 /* static LDS of the step kernels: the dopri5 stage vectors k2..k5, 4 x 8 doubles for each of
-   at most 256 threads (must match stage_lds in pt_track.hip) */
-#define PT_STEP_LDS_BYTES (4 * 8 * 256 * 8)
+   at most 256 threads, and the staging of the wave-cooperative coefficient gathers, 4 KB for
+   each of at most 4 waves (must match stage_lds and coop_lds in pt_track.hip) */
+#define PT_STEP_LDS_BYTES (4 * 8 * 256 * 8 + 4 * 32 * 128)
 
 /* the mode-templated launches, each instantiated in the translation unit of its mode */
 template <int MODE> hipError_t pt_launch_init_mode(const ptd::Params &P, int grid, hipStream_t stream);

@@ -101,44 +101,41 @@ PTD void load_line64(const double *__restrict__ a, double c[16]) {
     for (int q = 0; q < 8; ++q) { double2 v = gload(c2 + q); c[2 * q] = v.x; c[2 * q + 1] = v.y; }
 }
 
-PTD void tric_eval_grad64(const double *__restrict__ a, double x, double y, double z, bool grad,
-                          double &F, double &Fx, double &Fy, double &Fz) {
-    double ry[4], ryd[4], ryz[4];
-    /* two coefficient lines in flight: the next power of x is requested before the
-       current one is used (the lines are independent, the arithmetic unchanged) */
-    double c[16], cn[16];
-    load_line64(a + 48, cn);
-#pragma unroll
-    for (int i = 3; i >= 0; --i) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) c[q] = cn[q];
-        if (i > 0) load_line64(a + 16 * (i - 1), cn);
-        double rY = 0., rYd = 0., rYz = 0.;
-#pragma unroll
-        for (int j = 3; j >= 0; --j) {
-            double a0 = c[4 * j], a1 = c[4 * j + 1], a2 = c[4 * j + 2], a3 = c[4 * j + 3];
-            double rz = 0.;
-            rz += a3; rz *= z;
-            rz += a2; rz *= z;
-            rz += a1; rz *= z;
-            rz += a0;
-            rY += rz;
-            if (grad) {
-                double rzd = 0.;
-                rzd += 3.0 * a3; rzd *= z;
-                rzd += 2.0 * a2; rzd *= z;
-                rzd += a1;
-                rYz += rzd;
-                if (j > 0) {
-                    rYd += (double)j * rz;
-                    if (j > 1) rYd *= y;
-                }
-                if (j > 0) rYz *= y;
-            }
-            if (j > 0) rY *= y;
+/* one row j (the coefficients of x^i y^j z^0..3) of the z- and y-Horner sums of an x-power line */
+PTD void tric_row(double a0, double a1, double a2, double a3, int j, double y, double z, bool grad, double &rY,
+                  double &rYd, double &rYz) {
+    double rz = 0.;
+    rz += a3; rz *= z;
+    rz += a2; rz *= z;
+    rz += a1; rz *= z;
+    rz += a0;
+    rY += rz;
+    if (grad) {
+        double rzd = 0.;
+        rzd += 3.0 * a3; rzd *= z;
+        rzd += 2.0 * a2; rzd *= z;
+        rzd += a1;
+        rYz += rzd;
+        if (j > 0) {
+            rYd += (double)j * rz;
+            if (j > 1) rYd *= y;
         }
-        ry[i] = rY; ryd[i] = rYd; ryz[i] = rYz;
+        if (j > 0) rYz *= y;
     }
+    if (j > 0) rY *= y;
+}
+
+/* the z- and y-Horner sums of one x-power line (the 16 coefficients of x^i), rows 3 to 0 */
+PTD void tric_line(const double c[16], double y, double z, bool grad, double &rYo, double &rYdo, double &rYzo) {
+    double rY = 0., rYd = 0., rYz = 0.;
+#pragma unroll
+    for (int j = 3; j >= 0; --j) tric_row(c[4 * j], c[4 * j + 1], c[4 * j + 2], c[4 * j + 3], j, y, z, grad, rY, rYd, rYz);
+    rYo = rY; rYdo = rYd; rYzo = rYz;
+}
+
+/* the x-Horner sums over the four lines' results: value and the three derivatives */
+PTD void tric_finish(const double ry[4], const double ryd[4], const double ryz[4], double x, bool grad, double &F,
+                     double &Fx, double &Fy, double &Fz) {
     double rx = 0.;
 #pragma unroll
     for (int i = 3; i >= 0; --i) { rx += ry[i]; if (i > 0) rx *= x; }
@@ -159,6 +156,141 @@ PTD void tric_eval_grad64(const double *__restrict__ a, double x, double y, doub
         Fz = dz;
     }
 }
+
+PTD void tric_eval_grad64(const double *__restrict__ a, double x, double y, double z, bool grad,
+                          double &F, double &Fx, double &Fy, double &Fz) {
+    double ry[4], ryd[4], ryz[4];
+    /* two coefficient lines in flight: the next power of x is requested before the
+       current one is used (the lines are independent, the arithmetic unchanged) */
+    double c[16], cn[16];
+    load_line64(a + 48, cn);
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) c[q] = cn[q];
+        if (i > 0) load_line64(a + 16 * (i - 1), cn);
+        tric_line(c, y, z, grad, ry[i], ryd[i], ryz[i]);
+    }
+    tric_finish(ry, ryd, ryz, x, grad, F, Fx, Fy, Fz);
+}
+
+/*
+ * Wave-cooperative gathers of the step kernels' trial steps.  A lane's 1536-byte cell
+ * block lies at a random place in the table, so one 16-byte load per lane touches as many
+ * 128-byte lines as the wave has lanes and the vector L1 serves it a lane at a time.  The
+ * active lanes of a wave therefore load each other's coefficients: a unit is one half
+ * (64 bytes, two rows of four coefficients) of one x-power line of every active lane's
+ * block, four lanes of consecutive rank read one half line together (the lane of rank r
+ * reads, for k = 0..3, chunk v & 3 of the half line of rank v >> 2, v = k n + r, n active
+ * lanes), so a load instruction of a full wave touches 16 lines instead of 64, and the
+ * chunks reach their owners through 4 KB of LDS per wave.  Each lane evaluates its own
+ * coefficients in the order tric_eval_grad64 does (rows 3 and 2 of a line, then rows 1
+ * and 0), so the results are bit-identical.
+ * LDS layout of a wave's slice: chunk q (0..3) of rank p at byte 1024 q + 16 (p ^ 2q), so
+ * the eight lanes of every 16-byte access of a full wave cover all banks and the reads
+ * are one XOR from a per-lane base plus a constant offset.  Before the units, the slice's
+ * first 256 bytes pass each rank's block index to the lanes that load for it.
+ */
+constexpr int COOP_WAVES = 4;   /* waves of a 256-thread block */
+__shared__ double2 coop_lds[COOP_WAVES][4][64];
+
+/* the map's table is addressable in 16-byte units with 32-bit indices */
+PTD bool coop_fits(const Field &f) {
+    return (uint64_t)(f.nx - 1) * (uint64_t)(f.ny - 1) * (uint64_t)(f.nz - 1) * (uint64_t)f.ncomp * 32ull < (1ull << 32);
+}
+
+struct Chunks4 { double2 a, b, c, d; };
+
+/* this lane's four chunk loads of one unit: src[k] is the 16-byte index of the chunk it
+   loads for the k-th of its four (rank, chunk) pairs */
+PTD Chunks4 coop_load(const double2 *__restrict__ t, const uint32_t *src, uint32_t off) {
+    Chunks4 u;
+    u.a = gload(t + (src[0] + off));
+    u.b = gload(t + (src[1] + off));
+    u.c = gload(t + (src[2] + off));
+    u.d = gload(t + (src[3] + off));
+    return u;
+}
+
+/* rank of this lane among the wave's active lanes (its lane index when all are active) */
+PTD uint32_t active_rank() {
+    uint32_t r;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, exec_lo, 0\n\tv_mbcnt_hi_u32_b32 %0, exec_hi, %0" : "=v"(r));
+    return r;
+}
+
+#define COOP_LDS_FENCE() asm volatile("" ::: "memory")
+#ifndef COOP_DEPTH
+#define COOP_DEPTH 2   /* units whose loads are in flight ahead of the one being staged */
+#endif
+
+/* Value and gradient of the three B components of every active lane's cell (FP64 table):
+   dist-scaled into Bt / dBt as container_B does */
+PTD void tric_eval_B_coop(const Field &f, long cell, const double r[3], const double dist[3], bool grad,
+                          double Bt[3], double dBt[3][3]) {
+    const uint32_t n = (uint32_t)__builtin_popcountll(__builtin_amdgcn_read_exec());
+    const uint32_t rank = active_rank();
+    char *S = reinterpret_cast<char *>(coop_lds[threadIdx.x >> 6]);
+    const double2 *t2 = reinterpret_cast<const double2 *>(f.coef);
+    const uint32_t cs = (uint32_t)f.ncomp * 32u;           /* 16-byte units per cell */
+    /* block indices by rank, then this lane's four (rank, chunk) pairs */
+    reinterpret_cast<uint32_t *>(S)[rank] = (uint32_t)cell * cs;
+    COOP_LDS_FENCE();
+    uint32_t src[4], wb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t v = (uint32_t)k * n + rank, o = v >> 2, c = v & 3;
+        src[k] = reinterpret_cast<const uint32_t *>(S)[o] + c;
+        wb[k] = (c << 10) | ((o ^ (2 * c)) << 4);
+    }
+    COOP_LDS_FENCE();
+    const double y = r[1], z = r[2];
+    /* unit u = 8 i + 2 (3 - L) + (1 - h) (component i, line L, half h) at 16-byte offset
+       32 i + 8 L + 4 h of a cell: the upper half (rows 3 and 2) of a line comes first.
+       COOP_DEPTH units are in flight ahead of the one being staged; the loop is unrolled,
+       so the unit buffers stay in registers and no loads in flight cross a back edge. */
+    constexpr int NU = 24;
+    auto unit_off = [](int u) -> uint32_t {
+        const int i = u / 8, L = 3 - (u % 8) / 2, h = 1 - (u & 1);
+        return (uint32_t)(32 * i + 8 * L + 4 * h);
+    };
+    Chunks4 buf[COOP_DEPTH + 1];
+#pragma unroll
+    for (int u = 0; u < COOP_DEPTH; ++u) buf[u] = coop_load(t2, src, unit_off(u));
+    double ry[4], ryd[4], ryz[4], rY = 0., rYd = 0., rYz = 0.;
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const int i = u / 8, L = 3 - (u % 8) / 2, h = 1 - (u & 1);
+        /* the loads COOP_DEPTH units ahead go out before this unit is staged */
+        if (u + COOP_DEPTH < NU) buf[(u + COOP_DEPTH) % (COOP_DEPTH + 1)] = coop_load(t2, src, unit_off(u + COOP_DEPTH));
+        const Chunks4 &vc = buf[u % (COOP_DEPTH + 1)];
+        *reinterpret_cast<double2 *>(S + wb[0]) = vc.a;
+        *reinterpret_cast<double2 *>(S + wb[1]) = vc.b;
+        *reinterpret_cast<double2 *>(S + wb[2]) = vc.c;
+        *reinterpret_cast<double2 *>(S + wb[3]) = vc.d;
+        COOP_LDS_FENCE();
+        /* the read base is rebuilt here from the rank, not kept live (or spilled) across
+           the evaluation */
+        const uint32_t rb = active_rank() << 4;
+        double2 c[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            c[q] = *reinterpret_cast<const double2 *>(S + ((rb ^ (uint32_t)(32 * q)) + 1024u * q));
+        COOP_LDS_FENCE();
+        if (h == 1) { rY = 0.; rYd = 0.; rYz = 0.; }
+        tric_row(c[2].x, c[2].y, c[3].x, c[3].y, 2 * h + 1, y, z, grad, rY, rYd, rYz);
+        tric_row(c[0].x, c[0].y, c[1].x, c[1].y, 2 * h, y, z, grad, rY, rYd, rYz);
+        if (h == 0) { ry[L] = rY; ryd[L] = rYd; ryz[L] = rYz; }
+        if (h == 0 && L == 0) {
+            double Fv, Fx = 0, Fy = 0, Fz = 0;
+            tric_finish(ry, ryd, ryz, r[0], grad, Fv, Fx, Fy, Fz);
+            double d0 = Fx / dist[0], d1 = Fy / dist[1], d2 = Fz / dist[2];
+            Bt[i] = Fv;
+            if (grad) { dBt[i][0] = d0; dBt[i][1] = d1; dBt[i][2] = d2; }
+        }
+    }
+}
+#undef COOP_LDS_FENCE
 
 /* FP32 Horner (with FMA) of a block held in registers, for the remainder of the
    mixed-precision tables */
@@ -535,7 +667,7 @@ PTD void tab2d_E(const Field &f, double x, double y, double z, double &V, double
 /* TFieldContainer::BField (src/field.cpp:139-153) with the scale factor bs already
    evaluated, accumulated into the manager sum (TFieldManager::BField, src/fields.cpp:139-164);
    tricubic maps and, through analytic_B, the analytic fields and conductors */
-template <int MODE>
+template <int MODE, bool COOP = false>
 PTD void container_B(const Field &f, double x, double y, double z, double bs, double B[3], double dB[3][3], bool grad,
                      long long &nfield) {
     double Bt[3] = {0, 0, 0};
@@ -555,6 +687,9 @@ PTD void container_B(const Field &f, double x, double y, double z, double bs, do
             long cell; double r[3], dist[3];
             if (tric_locate(f, x, y, z, cell, r, dist)) {
                 ++nfield;
+                if (COOP && !(MODE & MODE_MIXED) && f.ncomp >= 3 && coop_fits(f)) {
+                    tric_eval_B_coop(f, cell, r, dist, grad, Bt, dBt);
+                } else {
 #pragma unroll 1
                 for (int i = 0; i < 3; ++i) {
                     double Fv, Fx = 0, Fy = 0, Fz = 0;
@@ -568,6 +703,7 @@ PTD void container_B(const Field &f, double x, double y, double z, double bs, do
                     if (i == 0) { Bt[0] = Fv; if (grad) { dBt[0][0] = d0; dBt[0][1] = d1; dBt[0][2] = d2; } }
                     else if (i == 1) { Bt[1] = Fv; if (grad) { dBt[1][0] = d0; dBt[1][1] = d1; dBt[1][2] = d2; } }
                     else { Bt[2] = Fv; if (grad) { dBt[2][0] = d0; dBt[2][1] = d1; dBt[2][2] = d2; } }
+                }
                 }
             }
         }
@@ -595,7 +731,7 @@ PTD void container_B(const Field &f, double x, double y, double z, double bs, do
     }
 }
 
-template <int MODE>
+template <int MODE, bool COOP = false>
 PTD void field_B(const Params &P, double x, double y, double z, double t, double B[3], double dB[3][3], bool grad, long long &nfield) {
 #pragma unroll
     for (int i = 0; i < 3; i++) { B[i] = 0; if (grad) { dB[i][0] = 0; dB[i][1] = 0; dB[i][2] = 0; } }
@@ -603,7 +739,7 @@ PTD void field_B(const Params &P, double x, double y, double z, double t, double
         const Field &f = P.fields[k];
         /* TFieldScaler::scalingFactor (src/field.cpp:7-10), evaluated where the reference does */
         const double bs = (!(MODE & MODE_FIELDS) || f.blen == 0) ? f.bscale : (in_bounds(f, x, y, z) ? scale_B(f, t) : 0.);
-        container_B<MODE>(f, x, y, z, bs, B, dB, grad, nfield);
+        container_B<MODE, COOP>(f, x, y, z, bs, B, dB, grad, nfield);
     }
 }
 
@@ -780,7 +916,7 @@ PTD void to9(const double *s, double pol, double *y) {
    (x, y, z, vx, vy, vz, proper time, path length); the polarisation, whose
    derivative is identically +0.0, is passed separately.  B and V return the
    field the equation of motion used (hasB: B was evaluated). */
-template <int MODE>
+template <int MODE, bool COOP = false>
 PTD void derivs8(const Params &P, const Species &sp, const double *s, double t, double pol, double *d, double B[3],
                  double &V, bool &hasB, Ctr &c) {
     double y[9];
@@ -789,7 +925,7 @@ PTD void derivs8(const Params &P, const Species &sp, const double *s, double t, 
     B[0] = 0; B[1] = 0; B[2] = 0; V = 0;
     ++c.rhs;
     const bool needB = (sp.q != 0 || (sp.mu != 0 && pol != 0));
-    if (needB) field_B<MODE>(P, y[0], y[1], y[2], t, B, dB, true, c.field);
+    if (needB) field_B<MODE, COOP>(P, y[0], y[1], y[2], t, B, dB, true, c.field);
     /* B may stand in for later value-only evaluations at this point unless a 2D map
        computes those differently (TabField: spline2dcalc vs spline2ddiff) */
     hasB = needB && (!(MODE & MODE_FIELDS) || P.b_reuse);
@@ -1208,7 +1344,7 @@ PTD_NOINLINE bool do_hit(const Params &P, Track &T, double x1, const double *y1,
     return trajectoryaltered || T.stop != ID_UNKNOWN;
 }
 
-/* number of intersections of segment p1->p2 (-1 coplanar, -2 overflow) */
+/* number of intersections of segment p1->p2 (-1 coplanar overlap, -2 more than MAX_HITS) */
 /* collision count of a segment on the collision path: from the piece's candidates when they cover it
    (an exact count; the clearance-grid shortcut, which needs a memory round trip and never
    applies this close to a wall, only for segments the candidates do not cover) */
@@ -1845,7 +1981,9 @@ PTD bool spin_step(const Params &P, int i, const Species &sp, double x1, double 
    only its own slice, so no barrier is needed. */
 constexpr int STAGE_LDS = 4, STAGE_LANES = 256;
 __shared__ double stage_lds[STAGE_LDS][8][STAGE_LANES];
-static_assert(sizeof(stage_lds) == PT_STEP_LDS_BYTES, "the host checks PT_STEP_LDS_BYTES against the device");
+static_assert(sizeof(stage_lds) + sizeof(coop_lds) == PT_STEP_LDS_BYTES,
+              "the host checks PT_STEP_LDS_BYTES against the device");
+static_assert(COOP_WAVES * 64 == STAGE_LANES, "one LDS staging slice per wave of a step-kernel block");
 
 /* An accepted dopri5 trial step, handed from trialA to acceptA: the stepper's state at
    the step start (xc, ts), the new state and its derivative and field, and whether the
@@ -1928,28 +2066,28 @@ PTD int trialA(const Params &P, int i, Lane &L, Trial &T, Ctr &c) {
         const double h21 = dt * b21;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h21 * L.dxc[k];
-        derivs8<MODE>(P, sp, tmp, ts + dt * (1.0 / 5), L.pol, kst, Bd, Vd, bd, c);
+        derivs8<MODE, true>(P, sp, tmp, ts + dt * (1.0 / 5), L.pol, kst, Bd, Vd, bd, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) K2(k) = kst[k];
         STAGE_STORED();
         const double h31 = dt * b31, h32 = dt * b32;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h31 * L.dxc[k] + h32 * K2(k);
-        derivs8<MODE>(P, sp, tmp, ts + dt * (3.0 / 10), L.pol, kst, Bd, Vd, bd, c);
+        derivs8<MODE, true>(P, sp, tmp, ts + dt * (3.0 / 10), L.pol, kst, Bd, Vd, bd, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) K3(k) = kst[k];
         STAGE_STORED();
         const double h41 = dt * b41, h42 = dt * b42, h43 = dt * b43;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h41 * L.dxc[k] + h42 * K2(k) + h43 * K3(k);
-        derivs8<MODE>(P, sp, tmp, ts + dt * (4.0 / 5), L.pol, kst, Bd, Vd, bd, c);
+        derivs8<MODE, true>(P, sp, tmp, ts + dt * (4.0 / 5), L.pol, kst, Bd, Vd, bd, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) K4(k) = kst[k];
         STAGE_STORED();
         const double h51 = dt * b51, h52 = dt * b52, h53 = dt * b53, h54 = dt * b54;
 #pragma unroll
         for (int k = 0; k < 8; ++k) tmp[k] = xc[k] + h51 * L.dxc[k] + h52 * K2(k) + h53 * K3(k) + h54 * K4(k);
-        derivs8<MODE>(P, sp, tmp, ts + dt * (8.0 / 9), L.pol, kst, Bd, Vd, bd, c);
+        derivs8<MODE, true>(P, sp, tmp, ts + dt * (8.0 / 9), L.pol, kst, Bd, Vd, bd, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) K5(k) = kst[k];
         STAGE_STORED();
@@ -1957,7 +2095,7 @@ PTD int trialA(const Params &P, int i, Lane &L, Trial &T, Ctr &c) {
 #pragma unroll
         for (int k = 0; k < 8; ++k)
             tmp[k] = xc[k] + h61 * L.dxc[k] + h62 * K2(k) + h63 * K3(k) + h64 * K4(k) + h65 * K5(k);
-        derivs8<MODE>(P, sp, tmp, ts + dt, L.pol, k6, Bd, Vd, bd, c);
+        derivs8<MODE, true>(P, sp, tmp, ts + dt, L.pol, k6, Bd, Vd, bd, c);
         const double g1 = dt * c1, g3 = dt * c3, g4 = dt * c4, g5 = dt * c5, g6 = dt * c6;
 #pragma unroll
         for (int k = 0; k < 8; ++k) xnew[k] = xc[k] + g1 * L.dxc[k] + g3 * K3(k) + g4 * K4(k) + g5 * K5(k) + g6 * k6[k];
@@ -1988,7 +2126,7 @@ PTD int trialA(const Params &P, int i, Lane &L, Trial &T, Ctr &c) {
                 gstore(ds + (40 + k) * n + i, k6[k]);
             }
         }
-        derivs8<MODE>(P, sp, xnew, ts + dt, T.polnew, k7, T.Bn, T.Vn, T.bn, c);
+        derivs8<MODE, true>(P, sp, xnew, ts + dt, T.polnew, k7, T.Bn, T.Vn, T.bn, c);
         double err = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
