@@ -369,9 +369,18 @@ int pt_fields_eval(const pt_fields *f, void *stream, int n, const double *pts, c
                    double *dB, double *V, double *E);
 /* TGeometry::GetSolid (src/geometry.cpp:151-175) at n points and times: the ID of the
    highest-priority solid not ignored at t (1: default solid; -1: too deeply nested;
-   -2: BVH traversal stack exhausted, which geometries built by pt_geometry_create exclude) */
+   -2: reserved for an incomplete BVH traversal, which the stackless traversal excludes) */
 int pt_geometry_solid_at(const pt_geometry *g, void *stream, int n, const double *pts, const double *t,
                          int32_t *solid_id);
+/* The intersection count TTracker::CheckHit takes from TGeometry::GetCollisions
+   (src/tracking.cpp:193-195, src/trianglemesh.cpp:127-160) for n segments p[i] -> q[i]
+   (p, q [n][3]): the number of triangles each segment meets, -1 for a segment coplanar
+   with a triangle it overlaps (ID_CGAL_ERROR), -2 for more than 16 intersections.
+   mode 0 counts as the tracking kernels do (clearance grid, then the candidate list of the
+   segment's cell, then the BVH), mode 1 from the BVH alone (both give the same counts);
+   mode 2 returns which way mode 0 takes (0 cleared by the grid, 1 cell list, 2 BVH). */
+int pt_geometry_segment_hits(const pt_geometry *g, void *stream, int n, const double *p, const double *q, int mode,
+                             int32_t *out);
 /* for tests: the proton energies sampled for the variates p[0..n-1] in [0, 1) */
 int pt_beta_sample(int n, const double *p, double *out);
 /* for tests: the decay kinematics on the host for stream (key, ctr) and neutron velocity v[3]:

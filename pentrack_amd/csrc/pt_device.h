@@ -100,12 +100,23 @@ struct Geom {
     const double *ign;
     const double *mats;       /* [n_mat][9] */
     const double *mesh_bbox;  /* [n_solid][6] xmin ymin zmin xmax ymax zmax */
-    /* clearance grid: lower bound of the distance from any point of a cell to any
-       triangle; a segment shorter than the bound of its start cell hits nothing */
+    /* clearance grid: per cell a lower bound of the distance from its centre to the
+       nearest triangle less half its diagonal (clear_of_walls) */
     const float *clear;       /* [gn0][gn1][gn2] */
     int gn[3];
     double glo[3], ginv_h[3];
+    /* candidate lists of the clearance cells: the triangles (copies, BVH leaf order) whose
+       padded boxes meet the cell widened by cl_pad on every side; cl_rng[cell] = (first,
+       count), count < 0 where the list would be longer than CELL_LIST_MAX */
+    const int2 *cl_rng;
+    const Tri *cl_tri;
+    double cl_pad;
+    /* per cell the deepest BVH node whose subtree holds every triangle whose padded box
+       meets the cell widened by rt_pad (-2: no such triangle) */
+    const int *cl_root;
+    double rt_pad;
 };
+constexpr int CELL_LIST_MAX = 32;
 
 struct Field {
     int type;                 /* PT_FIELD_* of include/pentrack_gpu.h */
@@ -468,17 +479,20 @@ struct HitList {
  * list == nullptr: count only.  Returns the number of intersections, or
  * -1 (coplanar overlap: the reference throws) / -2 (more than MAX_HITS).
  */
-PTD_NOINLINE int2 seg_query_v(const Geom &G, double p0, double p1, double p2, double q0, double q1, double q2, HitList *list) {
+PTD_NOINLINE int2 seg_query_v(const Geom &G, double p0, double p1, double p2, double q0, double q1, double q2, HitList *list,
+                              int start = 0) {
     const double p[3] = {p0, p1, p2}, q[3] = {q0, q1, q2};
     int ntests = 0;
     double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
     double inv[3] = {1.0 / d[0], 1.0 / d[1], 1.0 / d[2]};
     double sl = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    int node = 0;
+    /* the subtree of start: the nodes from start up to its escape index */
+    int node = start, end = G.n_node;
     int count = 0;
-    while (node < G.n_node) {
+    while (node < end) {
         BvhNode nd;
         load_node(G.node + node, nd);
+        if (node == start && start > 0) end = nd.a >= 0 ? nd.a : start + 1;
         const bool hit = node_hit(nd, p, inv, d);
         if (nd.a >= 0) {
             node = hit ? node + 1 : nd.a;
@@ -750,21 +764,106 @@ PTD int current_solid(const Geom &G, unsigned long long s, int n) {
     return sk_sidx(s, best);
 }
 
-/* true if segment p->q provably misses every triangle (clearance grid) */
-PTD bool clear_of_walls(const Geom &G, const double *p, const double *q) {
+/* true if segment p->q provably misses every triangle (clearance grid).  A cell's entry is
+   a lower bound of d - r, d the distance from the cell's centre c to the nearest triangle
+   and r half the cell's diagonal (0 when that is not positive).  Every point of the segment
+   lies within |m - c| + L/2 of c (m its midpoint, L its length), so the segment misses the
+   walls when that is below d; the entry plus r stands for d, with a relative and an
+   absolute margin for the rounding of the entry, the centre and the distances.  cell[]
+   returns the midpoint's cell (cell[0] = -1 outside the grid). */
+PTD bool clear_of_walls(const Geom &G, const double *p, const double *q, int cell[3]) {
+    cell[0] = -1;
     if (G.clear == nullptr) return false;
     int c[3];
+    double m[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        double u = (p[a] - G.glo[a]) * G.ginv_h[a];
+        m[a] = 0.5 * (p[a] + q[a]);
+        double u = (m[a] - G.glo[a]) * G.ginv_h[a];
         if (!(u >= 0.0) || u >= (double)G.gn[a]) return false;
         c[a] = (int)u;
     }
-    double bound = (double)G.clear[((size_t)c[0] * G.gn[1] + c[1]) * G.gn[2] + c[2]];
-    double d0 = q[0] - p[0], d1 = q[1] - p[1], d2 = q[2] - p[2];
-    double L2 = d0 * d0 + d1 * d1 + d2 * d2;
-    double b = 0.999 * bound - 1e-9;
-    return b > 0.0 && L2 < b * b;
+    cell[0] = c[0]; cell[1] = c[1]; cell[2] = c[2];
+    const double bound = (double)G.clear[((size_t)c[0] * G.gn[1] + c[1]) * G.gn[2] + c[2]];
+    if (!(bound > 0.0)) return false;
+    double r2 = 0.0, o2 = 0.0, L2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double h = 1.0 / G.ginv_h[a];
+        const double o = m[a] - (G.glo[a] + ((double)c[a] + 0.5) * h);
+        const double d = q[a] - p[a];
+        r2 += h * h;
+        o2 += o * o;
+        L2 += d * d;
+    }
+    const double reach = __builtin_sqrt(o2) + 0.5 * __builtin_sqrt(L2);
+    return reach < 0.999 * (bound + 0.5 * __builtin_sqrt(r2)) - 1e-9;
+}
+
+PTD bool clear_of_walls(const Geom &G, const double *p, const double *q) {
+    int cell[3];
+    return clear_of_walls(G, p, q, cell);
+}
+
+/* p and q lie in cell c[] widened by pad on every side */
+PTD bool cell_covers(const Geom &G, const int c[3], double pad, const double *p, const double *q) {
+    bool in = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double h = 1.0 / G.ginv_h[a];
+        const double lo = G.glo[a] + (double)c[a] * h - pad, hi = G.glo[a] + (double)(c[a] + 1) * h + pad;
+        in = in && p[a] >= lo && p[a] <= hi && q[a] >= lo && q[a] <= hi;
+    }
+    return in;
+}
+
+/* the candidate list of cell c[] when it exists and its widened cell contains p and q
+   (then it contains the segment and every triangle the segment can meet is on the list):
+   returns (first, count), count < 0 when there is none */
+PTD int2 cell_list(const Geom &G, const int c[3], const double *p, const double *q) {
+    if (G.cl_rng == nullptr || c[0] < 0 || !cell_covers(G, c, G.cl_pad, p, q)) return make_int2(0, -1);
+    return G.cl_rng[((size_t)c[0] * G.gn[1] + c[1]) * G.gn[2] + c[2]];
+}
+
+/* the BVH node to start the segment's traversal from: the root of cell c[]'s triangles when
+   its widened cell contains p and q (a subtree holds a contiguous run of the leaf order, so
+   its traversal tests those triangles in the same order; -2: none to test), else 0 */
+PTD int cell_root(const Geom &G, const int c[3], const double *p, const double *q) {
+    if (G.cl_root == nullptr || c[0] < 0 || !cell_covers(G, c, G.rt_pad, p, q)) return 0;
+    return G.cl_root[((size_t)c[0] * G.gn[1] + c[1]) * G.gn[2] + c[2]];
+}
+
+/* seg_query_v's count (and its first error, the list being in BVH leaf order) from a cell's
+   candidate list; the triangles are fetched four at a time */
+PTD_NOINLINE int2 cell_list_count(const Geom &G, int first, int n, double p0, double p1, double p2, double q0,
+                                  double q1, double q2) {
+    const double p[3] = {p0, p1, p2}, q[3] = {q0, q1, q2};
+    int count = 0;
+    for (int k0 = 0; k0 < n; k0 += 4) {
+        float4 w[4][3];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (k0 + j < n) {
+                const float4 *t = reinterpret_cast<const float4 *>(G.cl_tri + first + k0 + j);
+                w[j][0] = t[0]; w[j][1] = t[1]; w[j][2] = t[2];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (k0 + j < n) {
+                const double a[3] = {w[j][0].x, w[j][0].y, w[j][0].z};
+                const double b[3] = {w[j][0].w, w[j][1].x, w[j][1].y};
+                const double c[3] = {w[j][1].z, w[j][1].w, w[j][2].x};
+                double pt[3];
+                const int r = seg_tri(a, b, c, p, q, pt);
+                if (r == 0) continue;
+                if (r < 0) return make_int2(-1, n);
+                if (count >= MAX_HITS) return make_int2(-2, n);
+                ++count;
+            }
+        }
+    }
+    return make_int2(count, n);
 }
 
 /* TGeometry::CheckSegment -> TTriangleMesh::InBoundingBox */

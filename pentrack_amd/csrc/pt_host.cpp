@@ -226,7 +226,8 @@ double nearest_tri_dist(const double *p, const std::vector<ptd::BvhNode> &nodes,
  * no triangle, so the tracker skips the BVH query for it.
  */
 void build_clearance(const std::vector<ptd::BvhNode> &nodes, const std::vector<double> &tv, const double lo[3],
-                     const double hi[3], int cells_long, std::vector<float> &clear, int gn[3], double ginv_h[3]) {
+                     const double hi[3], int cells_long, std::vector<float> &clear, int gn[3], double ginv_h[3],
+                     double *hs_out = nullptr) {
     double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
     double h = ext / cells_long;
     double hs[3];
@@ -236,6 +237,8 @@ void build_clearance(const std::vector<ptd::BvhNode> &nodes, const std::vector<d
         if (!(hs[a] > 0)) { hs[a] = h; }
         ginv_h[a] = 1.0 / hs[a];
     }
+    if (hs_out)
+        for (int a = 0; a < 3; ++a) hs_out[a] = hs[a];
     double half_diag = 0.5 * std::sqrt(hs[0] * hs[0] + hs[1] * hs[1] + hs[2] * hs[2]);
     size_t ncell = (size_t)gn[0] * gn[1] * gn[2];
     clear.assign(ncell, 0.f);
@@ -256,6 +259,89 @@ void build_clearance(const std::vector<ptd::BvhNode> &nodes, const std::vector<d
     std::vector<std::thread> th;
     for (int t = 0; t < threads; ++t) th.emplace_back(work, t);
     for (auto &t : th) t.join();
+}
+
+/* Candidate lists of the clearance cells (Geom::cl_rng, cl_tri): for every cell, the
+   triangles in leaf order whose boxes, padded as the BVH traversals pad them, meet the
+   cell widened by pad on every side (and by a small slack beyond, so that the device's
+   own rounding of the cell bounds stays inside).  A cell with more than CELL_LIST_MAX
+   triangles keeps no list (count -1). */
+template <class Fn>
+void visit_cells(const ptd::Tri &t, const double glo[3], const double hs[3], const int gn[3], double pad, Fn &&fn) {
+    double blo[3], bhi[3];
+    int ilo[3], ihi[3];
+    for (int a = 0; a < 3; ++a) {
+        const double v0 = t.v[a], v1 = t.v[3 + a], v2 = t.v[6 + a];
+        const double tlo = std::min(v0, std::min(v1, v2)), thi = std::max(v0, std::max(v1, v2));
+        const double tp = 1e-9 + 1e-7 * (std::fabs(tlo) + std::fabs(thi));
+        blo[a] = tlo - tp;
+        bhi[a] = thi + tp;
+        ilo[a] = std::max(0, (int)std::floor((blo[a] - pad - glo[a]) / hs[a]) - 1);
+        ihi[a] = std::min(gn[a] - 1, (int)std::floor((bhi[a] + pad - glo[a]) / hs[a]) + 1);
+    }
+    auto meets = [&](int a, int i) {
+        const double slack = 1e-9 + 1e-6 * hs[a];
+        const double clo = glo[a] + i * hs[a] - pad - slack, chi = glo[a] + (i + 1) * hs[a] + pad + slack;
+        return blo[a] <= chi && bhi[a] >= clo;
+    };
+    for (int i = ilo[0]; i <= ihi[0]; ++i) {
+        if (!meets(0, i)) continue;
+        for (int j = ilo[1]; j <= ihi[1]; ++j) {
+            if (!meets(1, j)) continue;
+            for (int l = ilo[2]; l <= ihi[2]; ++l)
+                if (meets(2, l)) fn(((size_t)i * gn[1] + j) * gn[2] + l);
+        }
+    }
+}
+
+void build_cell_lists(const std::vector<ptd::Tri> &tris, const double glo[3], const double hs[3], const int gn[3],
+                      double pad, std::vector<int2> &rng, std::vector<ptd::Tri> &out) {
+    const size_t ncell = (size_t)gn[0] * gn[1] * gn[2];
+    std::vector<int> cnt(ncell, 0);
+    auto visit = [&](const ptd::Tri &t, auto &&fn) { visit_cells(t, glo, hs, gn, pad, fn); };
+    for (const ptd::Tri &t : tris) visit(t, [&](size_t c) { ++cnt[c]; });
+    rng.assign(ncell, int2{0, -1});
+    size_t total = 0;
+    for (size_t c = 0; c < ncell; ++c)
+        if (cnt[c] <= ptd::CELL_LIST_MAX) { rng[c] = int2{(int)total, cnt[c]}; total += (size_t)cnt[c]; }
+    out.resize(total);
+    std::vector<int> pos(ncell, 0);
+    for (size_t c = 0; c < ncell; ++c) pos[c] = rng[c].x;
+    for (const ptd::Tri &t : tris)
+        visit(t, [&](size_t c) { if (rng[c].y >= 0) out[(size_t)pos[c]++] = t; });
+}
+
+/* Per cell the deepest BVH node whose subtree holds every triangle whose padded box meets
+   the cell widened by pad (Geom::cl_root; -2 where there is none): the triangles of a subtree
+   are a contiguous run of the leaf order, so the node is found by descending from the root
+   while one child's run contains the cell's lowest and highest leaf-order index. */
+void build_cell_roots(const std::vector<ptd::BvhNode> &nodes, const std::vector<ptd::Tri> &tris, const double glo[3],
+                      const double hs[3], const int gn[3], double pad, std::vector<int> &root) {
+    const size_t ncell = (size_t)gn[0] * gn[1] * gn[2];
+    std::vector<int> kmin(ncell, INT_MAX), kmax(ncell, -1);
+    for (int k = 0; k < (int)tris.size(); ++k)
+        visit_cells(tris[k], glo, hs, gn, pad, [&](size_t c) {
+            kmin[c] = std::min(kmin[c], k);
+            kmax[c] = std::max(kmax[c], k);
+        });
+    /* leaf-order runs of the subtrees (host nodes: a = left child, b = right child) */
+    std::vector<int> lo(nodes.size()), hi(nodes.size());
+    for (int n = (int)nodes.size() - 1; n >= 0; --n) {
+        if (nodes[n].a < 0) { lo[n] = -nodes[n].a - 1; hi[n] = lo[n] + nodes[n].b - 1; }
+        else { lo[n] = lo[nodes[n].a]; hi[n] = hi[nodes[n].b]; }
+    }
+    root.assign(ncell, -2);
+    for (size_t c = 0; c < ncell; ++c) {
+        if (kmax[c] < 0) continue;
+        int n = 0;
+        while (nodes[n].a >= 0) {
+            const int l = nodes[n].a, r = nodes[n].b;
+            if (lo[l] <= kmin[c] && kmax[c] <= hi[l]) n = l;
+            else if (lo[r] <= kmin[c] && kmax[c] <= hi[r]) n = r;
+            else break;
+        }
+        root[c] = n;
+    }
 }
 
 } // namespace
@@ -363,6 +449,8 @@ extern "C" int pt_geometry_create(int n_tri, const float *tri_vertices, const in
     g.tri = dtri; g.nrm = dnrm; g.node = dnode; g.solid_id = dsid; g.solid_mat = dsmat;
     g.ign_off = dign_off; g.ign = dign; g.mats = dmats; g.mesh_bbox = dbbox;
     g.clear = nullptr;
+    g.cl_rng = nullptr; g.cl_tri = nullptr; g.cl_pad = 0.0;
+    g.cl_root = nullptr; g.rt_pad = 0.0;
     if (n_tri > 0) {
         std::vector<double> tv((size_t)9 * n_tri);
         double glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -373,11 +461,30 @@ extern "C" int pt_geometry_create(int n_tri, const float *tri_vertices, const in
                 ghi[c % 3] = std::max(ghi[c % 3], (double)tris[k].v[c]);
             }
         std::vector<float> clear;
-        build_clearance(nodes, tv, glo, ghi, 96, clear, g.gn, g.ginv_h);
+        double hs[3];
+        build_clearance(nodes, tv, glo, ghi, 96, clear, g.gn, g.ginv_h, hs);
         for (int a = 0; a < 3; ++a) g.glo[a] = glo[a];
         float *dclear;
         if ((r = G->mem.upload(&dclear, clear.data(), clear.size()))) return r;
         g.clear = dclear;
+        /* candidate lists of the cells widened by two cells on every side */
+        std::vector<int2> rng;
+        std::vector<ptd::Tri> ltri;
+        const double pad = 2.0 * std::max(hs[0], std::max(hs[1], hs[2]));
+        build_cell_lists(tris, glo, hs, g.gn, pad, rng, ltri);
+        if (ltri.size() < (size_t)INT_MAX) {
+            int2 *drng; ptd::Tri *dltri;
+            if ((r = G->mem.upload(&drng, rng.data(), rng.size()))) return r;
+            if ((r = G->mem.upload(&dltri, ltri.data(), ltri.size()))) return r;
+            g.cl_rng = drng; g.cl_tri = dltri; g.cl_pad = pad;
+        }
+        /* BVH subtree roots of the cells widened by four cells on every side */
+        std::vector<int> roots;
+        const double rpad = 2.0 * pad;
+        build_cell_roots(nodes, tris, glo, hs, g.gn, rpad, roots);
+        int *droot;
+        if ((r = G->mem.upload(&droot, roots.data(), roots.size()))) return r;
+        g.cl_root = droot; g.rt_pad = rpad;
     }
     G->n_nodes = (int)nodes.size();
     G->depth = max_depth;
@@ -1463,6 +1570,25 @@ extern "C" int pt_geometry_solid_at(const pt_geometry *g, void *stream, int n, c
     PT_HIP(pt_launch_solid_at(g->g, n, dp, dt, dout, s));
     PT_HIP(hipStreamSynchronize(s));
     PT_HIP(hipMemcpy(solid_id, dout, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+/* intersection counts of n segments p[i] -> q[i] with the mesh (segment_hits_kernel's modes) */
+extern "C" int pt_geometry_segment_hits(const pt_geometry *g, void *stream, int n, const double *p, const double *q,
+                                        int mode, int32_t *out) {
+    if (!g || n < 0 || mode < 0 || mode > 2 || (n > 0 && (!p || !q || !out)))
+        return fail(PT_EINVAL, "pt_geometry_segment_hits: bad arguments");
+    if (n == 0) return PT_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevMem mem;
+    double *dp, *dq;
+    int *dout;
+    int r;
+    if ((r = mem.upload(&dp, p, 3 * (size_t)n)) || (r = mem.upload(&dq, q, 3 * (size_t)n)) || (r = mem.alloc(&dout, (size_t)n)))
+        return r;
+    PT_HIP(pt_launch_segment_hits(g->g, n, dp, dq, mode, dout, s));
+    PT_HIP(hipStreamSynchronize(s));
+    PT_HIP(hipMemcpy(out, dout, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
     return PT_OK;
 }
 

@@ -57,6 +57,8 @@ hipError_t pt_launch_mixed_table(const double *xg, const double *yg, const doubl
                                  hipStream_t stream);
 /* ID of the solid with the highest priority at each point and time */
 hipError_t pt_launch_solid_at(const ptd::Geom &G, int n, const double *pts, const double *t, int *out, hipStream_t stream);
+hipError_t pt_launch_segment_hits(const ptd::Geom &G, int n, const double *p, const double *q, int mode, int *out,
+                                  hipStream_t stream);
 /* decay products of the batch's decayed neutrons into out (count reset first) */
 hipError_t pt_launch_decay(const ptd::Params &P, pt_beta_table T, pt_secondary *out, int *count, hipStream_t stream);
 

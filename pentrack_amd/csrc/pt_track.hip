@@ -1344,23 +1344,36 @@ PTD_NOINLINE bool do_hit(const Params &P, Track &T, double x1, const double *y1,
     return trajectoryaltered || T.stop != ID_UNKNOWN;
 }
 
-/* number of intersections of segment p1->p2 (-1 coplanar overlap, -2 more than MAX_HITS) */
-/* collision count of a segment on the collision path: from the piece's candidates when they cover it
-   (an exact count; the clearance-grid shortcut, which needs a memory round trip and never
-   applies this close to a wall, only for segments the candidates do not cover) */
-PTD int count_hits(const Params &P, Track &T, const double *p1, const double *p2) {
-    if (cand_covers(T.cand, p1, p2)) return cand_query(P.geom, T.cand, p1, p2, nullptr, &T.c.tri);
-    if (clear_of_walls(P.geom, p1, p2)) return 0;
-    int2 r = seg_query_v(P.geom, p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], nullptr);
-    T.c.tri += r.y;
+/* Number of intersections of segment p1->p2 with the mesh (-1 coplanar overlap, -2 more
+   than MAX_HITS): nothing when the clearance grid clears the segment, else from the
+   candidate list of its midpoint's cell when that covers it, else from the BVH subtree of
+   the cell's triangles when the cell widened by rt_pad covers it, else from the whole BVH;
+   all give the BVH query's result. */
+PTD int count_hits_geom(const Geom &G, const double *p1, const double *p2, long long &tests) {
+    int cell[3];
+    if (clear_of_walls(G, p1, p2, cell)) return 0;
+    const int2 cl = cell_list(G, cell, p1, p2);
+    int2 r;
+    if (cl.y >= 0) {
+        r = cell_list_count(G, cl.x, cl.y, p1[0], p1[1], p1[2], p2[0], p2[1], p2[2]);
+    } else {
+        const int root = cell_root(G, cell, p1, p2);
+        if (root == -2) return 0;
+        r = seg_query_v(G, p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], nullptr, root);
+    }
+    tests += r.y;
     return r.x;
 }
 
+/* on the collision path: from the piece's candidates when they cover the segment (an exact
+   count; the clearance grid never applies this close to a wall), else as on the step path */
+PTD int count_hits(const Params &P, Track &T, const double *p1, const double *p2) {
+    if (cand_covers(T.cand, p1, p2)) return cand_query(P.geom, T.cand, p1, p2, nullptr, &T.c.tri);
+    return count_hits_geom(P.geom, p1, p2, T.c.tri);
+}
+
 PTD int count_hits(const Params &P, Ctr &c, const double *p1, const double *p2) {
-    if (clear_of_walls(P.geom, p1, p2)) return 0;
-    int2 r = seg_query_v(P.geom, p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], nullptr);
-    c.tri += r.y;
-    return r.x;
+    return count_hits_geom(P.geom, p1, p2, c.tri);
 }
 
 /*
@@ -2667,7 +2680,7 @@ __global__ void __launch_bounds__(256) fields_eval_kernel(const Params *__restri
 
 /* TGeometry::GetSolid (src/geometry.cpp:151-175): the solids containing the point (+z ray
    parity), the highest ID among those not ignored at t; -1 if more than MAX_SOLIDS nest,
-   -2 if the BVH traversal stack ran out */
+   -2 if the traversal reported an incomplete visit (the stackless traversal never does) */
 #if PT_COMMON_TU
 __global__ void __launch_bounds__(256) solid_at_kernel(Geom G, int n, const double *pts, const double *t, int *out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2685,6 +2698,30 @@ __global__ void __launch_bounds__(256) solid_at_kernel(Geom G, int n, const doub
         }
     }
     out[i] = (int)solid_ID(G, current_solid(G, sk, ns));
+}
+#endif
+
+/* TTriangleMesh::Collision's intersection count of the segments p[i] -> q[i] (-1 coplanar
+   overlap, -2 more than MAX_HITS): mode 0 as the tracking kernels count (clearance grid,
+   cell candidate list, cell subtree of the BVH, BVH), mode 1 from the BVH alone, mode 2 the
+   way mode 0 took (0 cleared by the grid, 1 cell list, 2 whole BVH, 3 cell subtree) */
+#if PT_COMMON_TU
+__global__ void __launch_bounds__(256) segment_hits_kernel(Geom G, int n, const double *p, const double *q, int mode,
+                                                           int *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a[3] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]}, b[3] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
+    long long tests = 0;
+    if (mode == 0) {
+        out[i] = count_hits_geom(G, a, b, tests);
+    } else if (mode == 1) {
+        out[i] = seg_query_v(G, a[0], a[1], a[2], b[0], b[1], b[2], nullptr).x;
+    } else {
+        int cell[3];
+        if (clear_of_walls(G, a, b, cell)) out[i] = 0;
+        else if (cell_list(G, cell, a, b).y >= 0) out[i] = 1;
+        else out[i] = cell_root(G, cell, a, b) != 0 ? 3 : 2;
+    }
 }
 #endif
 
@@ -2976,6 +3013,13 @@ hipError_t pt_launch_fields_eval(const ptd::Params &P, int general, int n, const
 hipError_t pt_launch_solid_at(const ptd::Geom &G, int n, const double *pts, const double *t, int *out, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(ptd::solid_at_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, G, n, pts, t, out);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_segment_hits(const ptd::Geom &G, int n, const double *p, const double *q, int mode, int *out,
+                                  hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ptd::segment_hits_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, G, n, p, q, mode, out);
     return hipGetLastError();
 }
 

@@ -1,4 +1,5 @@
-"""Field and solid queries on the GPU through the C ABI (pt_fields_eval, pt_geometry_solid_at)."""
+"""Field, solid and segment queries on the GPU through the C ABI (pt_fields_eval,
+pt_geometry_solid_at, pt_geometry_segment_hits)."""
 import ctypes as C
 
 import numpy as np
@@ -29,4 +30,21 @@ def solid_at(geometry, pts, t, stream=None, leaf_size=4):
     d = _native.dptr
     _native.check(_native.lib().pt_geometry_solid_at(geometry.device(leaf_size), stream, n, d(pts), d(t),
                                                      out.ctypes.data_as(C.POINTER(C.c_int32))), "pt_geometry_solid_at")
+    return out
+
+
+def segment_hits(geometry, p, q, mode=0, stream=None, leaf_size=4):
+    """Intersection counts of the segments p[i] -> q[i] ([n][3] each) with the mesh, as
+    TTracker::CheckHit takes them from TGeometry::GetCollisions (-1 coplanar overlap, -2 more
+    than 16 intersections).  mode 0 as the tracking kernels count, 1 from the BVH alone, 2
+    which way mode 0 took (0 clearance grid, 1 cell candidate list, 2 BVH)."""
+    p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    if p.shape != q.shape:
+        raise ValueError("p and q must have the same shape")
+    out = np.zeros(len(p), np.int32)
+    d = _native.dptr
+    _native.check(_native.lib().pt_geometry_segment_hits(geometry.device(leaf_size), stream, len(p), d(p), d(q), mode,
+                                                         out.ctypes.data_as(C.POINTER(C.c_int32))),
+                  "pt_geometry_segment_hits")
     return out
