@@ -23,7 +23,8 @@ Roofline of the dominant kernel (that variant's step kernel): algorithmic bytes 
 field-map cell evaluations made by the step kernel (device counters) x the bytes one
 evaluation gathers (3 x 512 B FP64; mixed: 3 x 288 B, plus 512 B for a component kept in
 FP64), divided by the step kernel's average launch duration, against HBM's 8 TB/s, and
-against the measured random-gather ceiling of the table (pt_gather_bandwidth).  `traffic`
+against the measured ceiling of the same random gather in the step kernels' cooperative
+form (pt_gather_bandwidth_coop; the per-lane form of pt_gather_bandwidth beside it).  `traffic`
 is the measured HBM bytes per step-kernel launch from the committed rocprofv3 counter
 summary of that kernel, when present.
 
@@ -430,14 +431,16 @@ def main():
     adj = None if a.no_adjoint else adjoint_leg(a, sptr)
     ceiling = None
     if not a.no_ceiling:
-        from pentrack_amd.profiling import gather_ceiling
+        from pentrack_amd.profiling import gather_ceiling, gather_ceiling_coop
         table = w.fields.containers[0].coef.nbytes
         gg, cc = gather_ceiling(table)
+        gcoop = gather_ceiling_coop(table)
         # the same gather from tables that an XCD's L2 (4 MB) and the Infinity Cache (256 MB) hold
         gl2, _ = gather_ceiling(2 << 20)
         gmall, _ = gather_ceiling(128 << 20)
-        ceiling = {"table_bytes": table, "random_1536B_gather_gbs": gg, "stream_copy_gbs": cc,
-                   "random_1536B_gather_gbs_2MB_table_l2": gl2, "random_1536B_gather_gbs_128MB_table_mall": gmall}
+        ceiling = {"table_bytes": table, "random_1536B_gather_gbs": gg, "cooperative_1536B_gather_gbs": gcoop,
+                   "stream_copy_gbs": cc, "random_1536B_gather_gbs_2MB_table_l2": gl2,
+                   "random_1536B_gather_gbs_128MB_table_mall": gmall}
         print(f"[bench] gather ceiling: {ceiling}", file=sys.stderr, flush=True)
     cpu = None if a.no_cpu else cpu_baseline(w, a.cpu_seconds)
 
@@ -489,8 +492,8 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "field_evals_per_accepted_step": rp["field_evals"] / max(rp["steps"], 1),
                      "bytes_per_field_eval": bpe,
-                     "gather_ceiling_gbs": ceiling["random_1536B_gather_gbs"] if ceiling else None,
-                     "frac_of_gather_ceiling": achieved / ceiling["random_1536B_gather_gbs"] if ceiling else None,
+                     "gather_ceiling_gbs": ceiling["cooperative_1536B_gather_gbs"] if ceiling else None,
+                     "frac_of_gather_ceiling": achieved / ceiling["cooperative_1536B_gather_gbs"] if ceiling else None,
                      "fp64_tflops": flops / (per_launch_ms * 1e9) if flops else None,
                      "fp64_frac": flops / (per_launch_ms * 1e9) / PEAK_FP64_TFLOPS if flops else None},
         "gather_ceiling": ceiling,

@@ -462,6 +462,13 @@ int pt_rk4_adjoint_noalloc(void *stream, const pt_adj_params *params, int n, con
  */
 int pt_gather_bandwidth(uint64_t table_bytes, int block_bytes, int threads, int gathers, int reps,
                         double *gather_gbs, double *copy_gbs);
+/* No counterpart in the reference: the same gather of 1536-byte blocks in the step kernels'
+ * cooperative form (the 64 lanes of a wave load each other's blocks half a 128-byte line
+ * per lane at a time, four lanes per 64 contiguous bytes, the chunks passed to their
+ * owners through LDS; two 256-thread blocks of 80 KB of LDS per CU, as the step kernels
+ * run).  Allocates and frees table_bytes of device memory; synchronous on the default
+ * stream. */
+int pt_gather_bandwidth_coop(uint64_t table_bytes, int threads, int gathers, int reps, double *gather_gbs);
 
 /* Cancellation flag mirroring the reference's global `quit` (include/globals.h:33):
    pt_batch_run copies it to the device when it is called, and its step kernels

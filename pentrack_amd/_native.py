@@ -145,7 +145,7 @@ EXPORTS = [
     "pt_distribution_table", "pt_fields_eval", "pt_geometry_solid_at", "pt_geometry_segment_hits", "pt_bicubic_precompute",
     "pt_tricubic_precompute_gpu", "pt_batch_set_snapshots", "pt_batch_snapshots",
     "pt_batch_set_track", "pt_batch_track", "pt_batch_set_spinlog", "pt_batch_spinlog",
-    "pt_rk4_adjoint_noalloc", "pt_fields_mixed_info", "pt_gather_bandwidth", "pt_batch_set_loss",
+    "pt_rk4_adjoint_noalloc", "pt_fields_mixed_info", "pt_gather_bandwidth", "pt_gather_bandwidth_coop", "pt_batch_set_loss",
     "pt_seg_tri",
 ]
 
@@ -179,6 +179,7 @@ def lib():
     L.pt_batch_set_loss.argtypes = [vp, C.POINTER(C.c_double)]
     L.pt_gather_bandwidth.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]
+    L.pt_gather_bandwidth_coop.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
     L.pt_tricubic_precompute.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), C.c_int]

@@ -54,6 +54,17 @@ def gather_ceiling(table_bytes, block_bytes=1536, threads=1 << 20, gathers=8, re
     return g.value, c.value
 
 
+def gather_ceiling_coop(table_bytes, threads=1 << 20, gathers=8, reps=5):
+    """GB/s of pt_gather_bandwidth_coop: the same random 1536-byte block gather in the step
+    kernels' cooperative form (half lines loaded by four lanes, passed through LDS) at
+    their occupancy."""
+    from . import _native
+    g = C.c_double()
+    _native.check(_native.lib().pt_gather_bandwidth_coop(int(table_bytes), int(threads), int(gathers), int(reps),
+                                                         C.byref(g)), "pt_gather_bandwidth_coop")
+    return g.value
+
+
 def run(kernel="step_kernel_fp64"):
     import torch
     from pentrack_amd import _native, workloads
