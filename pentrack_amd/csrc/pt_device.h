@@ -535,6 +535,98 @@ PTD_NOINLINE int2 seg_query_v(const Geom &G, double p0, double p1, double p2, do
     return make_int2(count, ntests);
 }
 
+/* Uniform (scalar-path) forms of the node and triangle loads: with a wave-uniform index
+   the read-only BVH is read through the constant address space, i.e. the scalar data cache,
+   which the field-map gathers of the vector path do not congest. */
+typedef const __attribute__((address_space(4))) pt_f4v pt_cf4v;
+PTD void load_node_u(const BvhNode *N, int idx, BvhNode &nd) {
+    pt_cf4v *q = (pt_cf4v *)(reinterpret_cast<const float4 *>(N) + 2 * (size_t)idx);
+    const pt_f4v w0 = q[0], w1 = q[1];
+    nd.lo[0] = w0.x; nd.lo[1] = w0.y; nd.lo[2] = w0.z; nd.a = __float_as_int(w0.w);
+    nd.hi[0] = w1.x; nd.hi[1] = w1.y; nd.hi[2] = w1.z; nd.b = __float_as_int(w1.w);
+}
+PTD void load_tri_u(const Tri *T, int idx, double a[3], double b[3], double c[3]) {
+    pt_cf4v *q = (pt_cf4v *)(reinterpret_cast<const float4 *>(T) + 3 * (size_t)idx);
+    const pt_f4v w0 = q[0], w1 = q[1], w2 = q[2];
+    a[0] = w0.x; a[1] = w0.y; a[2] = w0.z;
+    b[0] = w0.w; b[1] = w1.x; b[2] = w1.y;
+    c[0] = w1.z; c[1] = w1.w; c[2] = w2.x;
+}
+
+template <class T> PTD const T *uniform_ptr(const T *p) {
+    const unsigned long long u = (unsigned long long)p;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+    return reinterpret_cast<const T *>(((unsigned long long)hi << 32) | lo);
+}
+
+/* seg_query_v's count for a segment and start node that are the same on every lane of the
+   wave: the traversal's control flow is uniform and its loads take the scalar path */
+PTD int2 seg_count_uniform(const BvhNode *nodes, const Tri *tris, int n_node, const double p[3], const double q[3],
+                           int start) {
+    int ntests = 0;
+    const double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
+    const double inv[3] = {1.0 / d[0], 1.0 / d[1], 1.0 / d[2]};
+    /* every lane computes the same values; readfirstlane tells the compiler so, which keeps
+       the control flow and the load addresses scalar */
+    int node = __builtin_amdgcn_readfirstlane(start), end = n_node;
+    int count = 0;
+    while (node < end) {
+        BvhNode nd;
+        load_node_u(nodes, node, nd);
+        if (node == start && start > 0) end = nd.a >= 0 ? nd.a : start + 1;
+        const bool hit = __builtin_amdgcn_readfirstlane((int)node_hit(nd, p, inv, d)) != 0;
+        if (nd.a >= 0) {
+            node = __builtin_amdgcn_readfirstlane(hit ? node + 1 : nd.a);
+            continue;
+        }
+        if (hit) {
+            const int first = -nd.a - 1;
+            for (int k = 0; k < nd.b; ++k) {
+                double a[3], b[3], c[3], pt[3];
+                load_tri_u(tris, first + k, a, b, c);
+                ++ntests;
+                const int r = __builtin_amdgcn_readfirstlane(seg_tri(a, b, c, p, q, pt));
+                if (r == 0) continue;
+                if (r < 0) return make_int2(-1, ntests);
+                if (count >= MAX_HITS) return make_int2(-2, ntests);
+                ++count;
+            }
+        }
+        node = __builtin_amdgcn_readfirstlane(node + 1);
+    }
+    return make_int2(count, ntests);
+}
+
+PTD double readlane_d(double v, int l) {
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+/* The counts of the active lanes' segments p -> q from their start nodes: the lanes take
+   turns, each one's traversal run uniformly by the wave (seg_count_uniform) */
+PTD_NOINLINE int2 seg_count_lanes(const Geom &G, double p0, double p1, double p2, double q0, double q1, double q2,
+                                  int start) {
+    int2 mine = make_int2(0, 0);
+    unsigned long long pend = __builtin_amdgcn_read_exec();
+    const int me = (int)__lane_id();
+    const BvhNode *nodes = uniform_ptr(G.node);
+    const Tri *tris = uniform_ptr(G.tri);
+    const int n_node = __builtin_amdgcn_readfirstlane(G.n_node);
+    while (pend) {
+        const int l = __builtin_ctzll(pend);
+        pend &= pend - 1;
+        const double p[3] = {readlane_d(p0, l), readlane_d(p1, l), readlane_d(p2, l)};
+        const double q[3] = {readlane_d(q0, l), readlane_d(q1, l), readlane_d(q2, l)};
+        const int st = __builtin_amdgcn_readlane(start, l);
+        const int2 r = seg_count_uniform(nodes, tris, n_node, p, q, st);
+        if (me == l) mine = r;
+    }
+    return mine;
+}
+
 /*
  * Candidate list of one chord piece: the triangles whose bounding boxes meet a box
  * that contains every sub-segment the collision search can test on that piece,

@@ -1348,7 +1348,9 @@ PTD_NOINLINE bool do_hit(const Params &P, Track &T, double x1, const double *y1,
    than MAX_HITS): nothing when the clearance grid clears the segment, else from the
    candidate list of its midpoint's cell when that covers it, else from the BVH subtree of
    the cell's triangles when the cell widened by rt_pad covers it, else from the whole BVH;
-   all give the BVH query's result. */
+   all give the BVH query's result.  TURNS: the traversals of the lanes that need one run
+   one after the other on the scalar path (seg_count_lanes) instead of side by side. */
+template <bool TURNS>
 PTD int count_hits_geom(const Geom &G, const double *p1, const double *p2, long long &tests) {
     int cell[3];
     if (clear_of_walls(G, p1, p2, cell)) return 0;
@@ -1359,7 +1361,10 @@ PTD int count_hits_geom(const Geom &G, const double *p1, const double *p2, long 
     } else {
         const int root = cell_root(G, cell, p1, p2);
         if (root == -2) return 0;
-        r = seg_query_v(G, p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], nullptr, root);
+        /* the step kernels' lanes rarely need a traversal, and then it runs faster on the
+           scalar path, one lane's at a time; the collision kernel's lanes often need one */
+        r = TURNS ? seg_count_lanes(G, p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], root)
+                  : seg_query_v(G, p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], nullptr, root);
     }
     tests += r.y;
     return r.x;
@@ -1369,11 +1374,11 @@ PTD int count_hits_geom(const Geom &G, const double *p1, const double *p2, long 
    count; the clearance grid never applies this close to a wall), else as on the step path */
 PTD int count_hits(const Params &P, Track &T, const double *p1, const double *p2) {
     if (cand_covers(T.cand, p1, p2)) return cand_query(P.geom, T.cand, p1, p2, nullptr, &T.c.tri);
-    return count_hits_geom(P.geom, p1, p2, T.c.tri);
+    return count_hits_geom<false>(P.geom, p1, p2, T.c.tri);
 }
 
 PTD int count_hits(const Params &P, Ctr &c, const double *p1, const double *p2) {
-    return count_hits_geom(P.geom, p1, p2, c.tri);
+    return count_hits_geom<true>(P.geom, p1, p2, c.tri);
 }
 
 /*
@@ -2713,7 +2718,7 @@ __global__ void __launch_bounds__(256) segment_hits_kernel(Geom G, int n, const 
     const double a[3] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]}, b[3] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
     long long tests = 0;
     if (mode == 0) {
-        out[i] = count_hits_geom(G, a, b, tests);
+        out[i] = count_hits_geom<true>(G, a, b, tests);
     } else if (mode == 1) {
         out[i] = seg_query_v(G, a[0], a[1], a[2], b[0], b[1], b[2], nullptr).x;
     } else {
