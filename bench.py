@@ -6,7 +6,7 @@ closed cylinder (R = 0.25 m, H = 0.5 m, 19,600 triangles), 2% Lambert walls with
 (|B| <= 1 T), low-field seekers, energies uniform in 0-250 neV, tmax 200 s.
 
 A bench step is one `pt_batch_run` call over the whole resident batch: it
-advances every live particle by up to `--rk` accepted dopri5 steps (32) in `--rounds`
+advances every live particle by up to `--rk` accepted dopri5 steps (64) in `--rounds`
 step-kernel/collision-kernel rounds (2): a particle whose third wall collision would
 fall in the call keeps its remaining steps for the next call, which skips the short,
 latency-bound rounds that resolve the last few collisions of a call.  A variant's
@@ -84,7 +84,7 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--n", type=int, default=1_000_000)
-    p.add_argument("--rk", type=int, default=32, help="accepted RK steps per particle per bench step")
+    p.add_argument("--rk", type=int, default=64, help="accepted RK steps per particle per bench step")
     p.add_argument("--grid", type=int, default=64)
     p.add_argument("--block", type=int, default=256)
     p.add_argument("--rounds", type=int, default=2, help="step/collision rounds per run call (0: library default, 4)")

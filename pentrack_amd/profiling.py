@@ -3,7 +3,7 @@
 `python -m pentrack_amd.profiling run KERNEL` is the program profiled: a plain
 device copy of known size (the calibration: FETCH_SIZE and WRITE_SIZE of a kernel
 that reads and writes exactly COPY_BYTES), then the benchmark workload (1e6
-neutrons, cylinder, 64^3 map) advanced by a few run calls of 32 RK steps in 2 rounds with the
+neutrons, cylinder, 64^3 map) advanced by a few run calls of 64 RK steps in 2 rounds with the
 step kernel KERNEL (step_kernel_fp64, step_kernel_persistent_fp64, step_kernel_mixed,
 step_kernel_persistent_mixed, or step_kernel_persistent_mixed_spatial: the persistent
 kernel with its work list in Morton order of the particle positions).  Profile it three
@@ -27,7 +27,7 @@ import sys
 
 COPY_BYTES = 1 << 30
 # the launch shape of the benchmark's run calls (bench.py --rk / --rounds defaults)
-RK_STEPS, ROUNDS = 32, 2
+RK_STEPS, ROUNDS = 64, 2
 KERNELS = {
     # name: (variant, persistent, dispatch-name pattern, spatial order)
     # (the kernels' template argument is the mode: 0 FP64 tables, 1 mixed-precision tables)
