@@ -462,7 +462,9 @@ extern "C" int pt_geometry_create(int n_tri, const float *tri_vertices, const in
             }
         std::vector<float> clear;
         double hs[3];
-        build_clearance(nodes, tv, glo, ghi, 96, clear, g.gn, g.ginv_h, hs);
+        /* 128 cells on the longest axis: 2.36e8 RK steps/s on the benchmark against 2.30e8
+           with 96 and no gain from 160 */
+        build_clearance(nodes, tv, glo, ghi, 128, clear, g.gn, g.ginv_h, hs);
         for (int a = 0; a < 3; ++a) g.glo[a] = glo[a];
         float *dclear;
         if ((r = G->mem.upload(&dclear, clear.data(), clear.size()))) return r;
