@@ -31,30 +31,26 @@ namespace pta {
 /* block layout a[16 i + 4 j + k] = coefficient of x^i y^j z^k (see pt_host.cpp) */
 struct D2 { double f, fx, fy, fz, fxx, fxy, fxz, fyy, fyz, fzz; };
 
-PTA void tric_d2(const double *__restrict__ a, double x, double y, double z, bool hess, D2 &o) {
-    /* q*: y-Horner results per power of x */
-    double q[4], qy[4], qz[4], qyy[4], qyz[4], qzz[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        double r = 0, ry = 0, rz = 0, ryy = 0, ryz = 0, rzz = 0;
-#pragma unroll
-        for (int j = 3; j >= 0; --j) {
-            const double *c = a + 16 * i + 4 * j;
-            /* z-Horner: p, p', p'' */
-            double p = ((c[3] * z + c[2]) * z + c[1]) * z + c[0];
-            double pz = (3.0 * c[3] * z + 2.0 * c[2]) * z + c[1];
-            double pzz = 6.0 * c[3] * z + 2.0 * c[2];
-            /* y-Horner with y-derivatives (second derivative via the product rule) */
-            ryy = ryy * y + 2.0 * ry;
-            ry = ry * y + r;
-            r = r * y + p;
-            ryz = ryz * y + rz;
-            rz = rz * y + pz;
-            rzz = rzz * y + pzz;
-        }
-        q[i] = r; qy[i] = ry; qz[i] = rz; qyy[i] = ryy; qyz[i] = ryz; qzz[i] = rzz;
-    }
-    /* x-Horner of every quantity, with x-derivatives */
+/* one row j (coefficients a0..a3 of x^i y^j z^0..3) of the z- and y-Horner sums of an
+   x-power line, with their y- and z-derivatives */
+PTA void d2_row(double a0, double a1, double a2, double a3, double y, double z, double &r, double &ry, double &rz,
+                double &ryy, double &ryz, double &rzz) {
+    /* z-Horner: p, p', p'' */
+    const double p = ((a3 * z + a2) * z + a1) * z + a0;
+    const double pz = (3.0 * a3 * z + 2.0 * a2) * z + a1;
+    const double pzz = 6.0 * a3 * z + 2.0 * a2;
+    /* y-Horner with y-derivatives (second derivative via the product rule) */
+    ryy = ryy * y + 2.0 * ry;
+    ry = ry * y + r;
+    r = r * y + p;
+    ryz = ryz * y + rz;
+    rz = rz * y + pz;
+    rzz = rzz * y + pzz;
+}
+
+/* x-Horner of every quantity over the four lines' sums, with x-derivatives */
+PTA void d2_finish(const double q[4], const double qy[4], const double qz[4], const double qyy[4], const double qyz[4],
+                   const double qzz[4], double x, bool hess, D2 &o) {
     double F = 0, Fx = 0, Fxx = 0, Fy = 0, Fxy = 0, Fz = 0, Fxz = 0, Fyy = 0, Fyz = 0, Fzz = 0;
 #pragma unroll
     for (int i = 3; i >= 0; --i) {
@@ -73,6 +69,112 @@ PTA void tric_d2(const double *__restrict__ a, double x, double y, double z, boo
     }
     o.f = F; o.fx = Fx; o.fy = Fy; o.fz = Fz;
     o.fxx = Fxx; o.fxy = Fxy; o.fxz = Fxz; o.fyy = Fyy; o.fyz = Fyz; o.fzz = Fzz;
+}
+
+PTA void tric_d2(const double *__restrict__ a, double x, double y, double z, bool hess, D2 &o) {
+    /* q*: y-Horner results per power of x */
+    double q[4], qy[4], qz[4], qyy[4], qyz[4], qzz[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double r = 0, ry = 0, rz = 0, ryy = 0, ryz = 0, rzz = 0;
+#pragma unroll
+        for (int j = 3; j >= 0; --j) {
+            const double *c = a + 16 * i + 4 * j;
+            d2_row(c[0], c[1], c[2], c[3], y, z, r, ry, rz, ryy, ryz, rzz);
+        }
+        q[i] = r; qy[i] = ry; qz[i] = rz; qyy[i] = ryy; qyz[i] = ryz; qzz[i] = rzz;
+    }
+    d2_finish(q, qy, qz, qyy, qyz, qzz, x, hess, o);
+}
+
+/*
+ * Cooperative gathers, as the tracking kernels' trial steps do them: the active lanes of a
+ * wave (n, ranks r) load each other's coefficients in units of one half x-power line (64 B,
+ * two rows of four coefficients) of every lane's block, the lane of rank r loading, for
+ * k = 0..3, chunk v & 3 of the half line of rank v >> 2 (v = k n + r), and the chunks reach
+ * their owners through a 4 KB LDS slice per wave (chunk q of rank p at 1024 q + 16 (p ^ 2q)).
+ * Every lane then runs the rows of its own block in tric_d2's order, so the results are
+ * bit-identical.  All maps share the grid, so a lane's block offset is the same in each.
+ */
+constexpr int ACOOP_WAVES = 4;   /* waves of a 256-thread block */
+__shared__ double2 acoop_lds[ACOOP_WAVES][4][64];
+
+struct CoopIdx { uint32_t src[4], wb[4]; };
+
+PTA uint32_t active_rank() {
+    uint32_t r;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, exec_lo, 0\n\tv_mbcnt_hi_u32_b32 %0, exec_hi, %0" : "=v"(r));
+    return r;
+}
+
+#define ACOOP_FENCE() asm volatile("" ::: "memory")
+
+/* this lane's four (rank, chunk) load indices (16-byte units within a map) and LDS write offsets */
+PTA void coop_setup(long cell, CoopIdx &ci) {
+    const uint32_t n = (uint32_t)__builtin_popcountll(__builtin_amdgcn_read_exec());
+    const uint32_t rank = active_rank();
+    char *S = reinterpret_cast<char *>(acoop_lds[threadIdx.x >> 6]);
+    reinterpret_cast<uint32_t *>(S)[rank] = (uint32_t)cell * 96u;
+    ACOOP_FENCE();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t v = (uint32_t)k * n + rank, o = v >> 2, c = v & 3;
+        ci.src[k] = reinterpret_cast<const uint32_t *>(S)[o] + c;
+        ci.wb[k] = (c << 10) | ((o ^ (2 * c)) << 4);
+    }
+    ACOOP_FENCE();
+}
+
+struct AChunks { double2 a, b, c, d; };
+PTA AChunks acoop_load(const double2 *__restrict__ t, const uint32_t *src, uint32_t off) {
+    AChunks u;
+    u.a = t[src[0] + off]; u.b = t[src[1] + off]; u.c = t[src[2] + off]; u.d = t[src[3] + off];
+    return u;
+}
+
+/* tric_d2 of the three components of every active lane's cell in one map, gathered
+   cooperatively: unit u = 8 c + 2 i + (1 - h) (component c, x-power i, half h) at 16-byte
+   offset 32 c + 8 i + 4 h of the lane's cell; two units' loads in flight */
+PTA void tric_d2_map_coop(const double *__restrict__ coef, const CoopIdx &ci, double x, double y, double z, bool hess,
+                          D2 out[3]) {
+    const double2 *t2 = reinterpret_cast<const double2 *>(coef);
+    char *S = reinterpret_cast<char *>(acoop_lds[threadIdx.x >> 6]);
+    auto off = [](int u) -> uint32_t { return (uint32_t)(32 * (u / 8) + 8 * ((u % 8) / 2) + 4 * (1 - (u & 1))); };
+    AChunks buf[3];
+    buf[0] = acoop_load(t2, ci.src, off(0));
+    buf[1] = acoop_load(t2, ci.src, off(1));
+    double q[4], qy[4], qz[4], qyy[4], qyz[4], qzz[4];
+    double r = 0, ry = 0, rz = 0, ryy = 0, ryz = 0, rzz = 0;
+#pragma unroll
+    for (int u = 0; u < 24; ++u) {
+        const int c = u / 8, i = (u % 8) / 2, h = 1 - (u & 1);
+        if (u + 2 < 24) buf[(u + 2) % 3] = acoop_load(t2, ci.src, off(u + 2));
+        const AChunks &vc = buf[u % 3];
+        *reinterpret_cast<double2 *>(S + ci.wb[0]) = vc.a;
+        *reinterpret_cast<double2 *>(S + ci.wb[1]) = vc.b;
+        *reinterpret_cast<double2 *>(S + ci.wb[2]) = vc.c;
+        *reinterpret_cast<double2 *>(S + ci.wb[3]) = vc.d;
+        ACOOP_FENCE();
+        const uint32_t rb = active_rank() << 4;
+        double2 cc[4];
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq)
+            cc[qq] = *reinterpret_cast<const double2 *>(S + ((rb ^ (uint32_t)(32 * qq)) + 1024u * qq));
+        ACOOP_FENCE();
+        if (h == 1) { r = 0; ry = 0; rz = 0; ryy = 0; ryz = 0; rzz = 0; }
+        /* this half's rows: 2h + 1 (chunks 2, 3) then 2h (chunks 0, 1) */
+        d2_row(cc[2].x, cc[2].y, cc[3].x, cc[3].y, y, z, r, ry, rz, ryy, ryz, rzz);
+        d2_row(cc[0].x, cc[0].y, cc[1].x, cc[1].y, y, z, r, ry, rz, ryy, ryz, rzz);
+        if (h == 0) {
+            q[i] = r; qy[i] = ry; qz[i] = rz; qyy[i] = ryy; qyz[i] = ryz; qzz[i] = rzz;
+            if (i == 3) d2_finish(q, qy, qz, qyy, qyz, qzz, x, hess, out[c]);
+        }
+    }
+}
+
+/* the map set's table is addressable in 16-byte units with 32-bit indices */
+PTA bool coop_fits(const MapSet &M) {
+    return (uint64_t)(M.nx - 1) * (uint64_t)(M.ny - 1) * (uint64_t)(M.nz - 1) * 96ull < (1ull << 32);
 }
 
 /* cell search (std::upper_bound semantics of TabField3::Interpolate) */
@@ -115,13 +217,19 @@ PTA void field_total(const MapSet &M, const double *p, bool hess, Field2 &F) {
     F.in = locate(M, p, F.cell, F.r, F.inv);
     if (!F.in) return;
     const double ix = F.inv[0], iy = F.inv[1], iz = F.inv[2];
+    const bool coop = coop_fits(M);
+    CoopIdx ci;
+    if (coop) coop_setup(F.cell, ci);
 #pragma unroll 1
     for (int k = 0; k < M.nmaps; ++k) {
         const double sk = M.s[k];
+        D2 dm[3];
+        if (coop) tric_d2_map_coop(M.coef[k], ci, F.r[0], F.r[1], F.r[2], hess, dm);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             D2 d;
-            tric_d2(M.coef[k] + ((size_t)F.cell * 3 + c) * 64, F.r[0], F.r[1], F.r[2], hess, d);
+            if (coop) d = dm[c];
+            else tric_d2(M.coef[k] + ((size_t)F.cell * 3 + c) * 64, F.r[0], F.r[1], F.r[2], hess, d);
             F.B[c] += sk * d.f;
             F.G[c][0] += sk * (d.fx * ix);
             F.G[c][1] += sk * (d.fy * iy);
@@ -232,13 +340,19 @@ PTA void vjp(const AdjParams &A, const double *u, const double *w, double *ubar,
 #pragma unroll
         for (int j = 0; j < 3; ++j) beta[c][j] = wp[j] * b[c];
     }
+    const bool coop = coop_fits(A.maps);
+    CoopIdx ci;
+    if (coop) coop_setup(F.cell, ci);
 #pragma unroll 1
     for (int k = 0; k < A.maps.nmaps; ++k) {
         double acc = 0;
+        D2 dm[3];
+        if (coop) tric_d2_map_coop(A.maps.coef[k], ci, F.r[0], F.r[1], F.r[2], false, dm);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             D2 d;
-            tric_d2(A.maps.coef[k] + ((size_t)F.cell * 3 + c) * 64, F.r[0], F.r[1], F.r[2], false, d);
+            if (coop) d = dm[c];
+            else tric_d2(A.maps.coef[k] + ((size_t)F.cell * 3 + c) * 64, F.r[0], F.r[1], F.r[2], false, d);
             acc += alpha[c] * d.f + beta[c][0] * (d.fx * F.inv[0]) + beta[c][1] * (d.fy * F.inv[1]) +
                    beta[c][2] * (d.fz * F.inv[2]);
         }
