@@ -926,22 +926,23 @@ PTD int cell_root(const Geom &G, const int c[3], const double *p, const double *
 }
 
 /* seg_query_v's count (and its first error, the list being in BVH leaf order) from a cell's
-   candidate list; the triangles are fetched four at a time */
+   candidate list; the triangles are fetched two at a time (four or eight at a time keep more
+   registers live across the call and cost the step kernel 1.5% and 20%) */
 PTD_NOINLINE int2 cell_list_count(const Geom &G, int first, int n, double p0, double p1, double p2, double q0,
                                   double q1, double q2) {
     const double p[3] = {p0, p1, p2}, q[3] = {q0, q1, q2};
     int count = 0;
-    for (int k0 = 0; k0 < n; k0 += 4) {
-        float4 w[4][3];
+    for (int k0 = 0; k0 < n; k0 += 2) {
+        float4 w[2][3];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 2; ++j) {
             if (k0 + j < n) {
                 const float4 *t = reinterpret_cast<const float4 *>(G.cl_tri + first + k0 + j);
                 w[j][0] = t[0]; w[j][1] = t[1]; w[j][2] = t[2];
             }
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 2; ++j) {
             if (k0 + j < n) {
                 const double a[3] = {w[j][0].x, w[j][0].y, w[j][0].z};
                 const double b[3] = {w[j][0].w, w[j][1].x, w[j][1].y};
