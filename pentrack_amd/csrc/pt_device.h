@@ -646,10 +646,10 @@ struct Cand {
 
 /* The vertices of a lane's first CAND_LDS candidates, staged in LDS by cand_build for the
    bisection's counting queries (cand_lds[k][vertex coordinate][lane], 36 B per candidate
-   and lane: 45 KB for a block of CAND_LANES threads, three blocks per CU): an LDS read
+   and lane: 36 KB for a block of CAND_LANES threads, four blocks per CU): an LDS read
    instead of an index and a triangle fetched from memory for every candidate test.  Only
    kernels whose blocks have at most CAND_LANES threads may reach cand_build. */
-constexpr int CAND_LDS = 5, CAND_LANES = 256;
+constexpr int CAND_LDS = 4, CAND_LANES = 256;
 __shared__ float cand_lds[CAND_LDS][9][CAND_LANES];
 
 /* the candidates of a chord piece: triangles whose padded box meets [lo, hi], in BVH leaf

@@ -2819,12 +2819,13 @@ __global__ void __launch_bounds__(256) bucket_scatter_kernel(const Params *__res
  * waves, run the collision path (CheckHit's bisection, DoHit/OnHit) on their
  * pending chord piece and finish the rest of that step.
  */
-/* three waves per SIMD: the collision path's registers (256 VGPRs + 192 AGPRs at one wave)
-   fit in 168 VGPRs with 138 spilled values, and the extra waves hide the latency of its
-   BVH walks and dense-output reads (cylinder, 1e6 neutrons: 18.6 -> 12.6 ms per launch;
-   two waves 13.1 ms, four 13.6 ms) */
+/* four waves per SIMD (128 VGPRs, the rest spilled; four 36 KB blocks of staged candidates
+   per CU): the extra waves hide the latency of the collision path's BVH walks, scratch and
+   dense-output reads (cylinder, 1e6 neutrons, 64 steps in 2 rounds: 25.1 -> 23.6 ms per
+   launch against three waves with five staged candidates; earlier, one wave 18.6 ms against
+   three 12.6 ms) */
 #ifndef PT_COLL_MIN_WAVES
-#define PT_COLL_MIN_WAVES 3
+#define PT_COLL_MIN_WAVES 4
 #endif
 static_assert(CAND_LANES == 256, "the collision kernel's blocks fill the candidate staging area");
 template <int MODE>
