@@ -63,6 +63,11 @@ typedef struct pt_geometry pt_geometry;
  * mats           [n_mat][9] FermiReal FermiImag DiffProb SpinflipProb RMSRoughness CorrelLength
  *                InternalBField ModifiedLambertProb LossPerBounce (include/geometry.h:20-31)
  * leaf_size      BVH leaf size (triangles per leaf), 1..32
+ * Besides the BVH it builds, for the tracking kernels' wall test, a clearance grid (128
+ * cells on the bounding box's longest axis) and per cell a candidate list (copies of the
+ * triangles whose boxes meet the cell widened by two cells, at most 32) and a BVH subtree
+ * root (of the triangles meeting the cell widened by four): for the 19,600-triangle
+ * benchmark cylinder 1.8 s of host time (16 threads) and about 310 MB of device memory.
  */
 int pt_geometry_create(int n_tri, const float *tri_vertices, const int32_t *tri_solid,
                        int n_solid, const uint32_t *solid_id, const int32_t *solid_mat,
