@@ -21,8 +21,9 @@ of FP64 and identical full hit sequences for every particle -- over the stated h
 
 Roofline of the dominant kernel (that variant's step kernel): algorithmic bytes =
 field-map cell evaluations made by the step kernel (device counters) x the bytes one
-evaluation gathers (3 x 512 B FP64; mixed: 3 x 288 B, plus 512 B for a component kept in
-FP64), divided by the step kernel's average launch duration, against HBM's 8 TB/s, and
+evaluation gathers (3 x 512 B FP64; mixed: the 896 B cell record of a 3-component map,
+128 B of FP64 constant and linear terms and 3 x 256 B of FP32 lines, plus 512 B for a
+component kept in FP64), divided by the step kernel's average launch duration, against HBM's 8 TB/s, and
 against the measured ceiling of the same random gather in the step kernels' cooperative
 form (pt_gather_bandwidth_coop; the per-lane form of pt_gather_bandwidth beside it).  `traffic`
 is the measured HBM bytes per step-kernel launch from the committed rocprofv3 counter
@@ -75,7 +76,7 @@ KERNEL_NAMES = {"fp64": "step_kernel_fp64", "mixed": "step_kernel_mixed",
                 "fp64_persistent_xcd": "step_kernel_persistent_fp64_xcd",
                 "mixed_persistent": "step_kernel_persistent_mixed",
                 "mixed_persistent_spatial": "step_kernel_persistent_mixed_spatial"}
-MIX_BLOCK, FP64_BLOCK = 288, 512
+MIX_CELL, FP64_BLOCK = 896, 512
 
 
 def parse():
@@ -452,7 +453,7 @@ def main():
     r, rp = results[head], results[prof]
     step_ms, step_launches = rp["kernels"]["step"]
     per_launch_ms = step_ms / max(step_launches, 1)
-    bpe = 3 * (MIX_BLOCK + f64frac * FP64_BLOCK) if rp["mixed"] else 3 * FP64_BLOCK
+    bpe = MIX_CELL + 3 * f64frac * FP64_BLOCK if rp["mixed"] else 3 * FP64_BLOCK
     bytes_per_launch = rp["field_evals"] * bpe / max(step_launches, 1)
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
     kernel_name = KERNEL_NAMES.get(prof, prof)

@@ -372,6 +372,16 @@ int pt_distribution_table(const char *formula, double lo, double hi, int cap, do
    maps differs from the one with derivatives in the last bits, as in the reference). */
 int pt_fields_eval(const pt_fields *f, void *stream, int n, const double *pts, const double *t, double *B,
                    double *dB, double *V, double *E);
+/* The same field sum evaluated as the tracking kernels evaluate it: variant PT_VARIANT_FP64
+   or PT_VARIANT_MIXED (FP64 linear part plus FP32 remainder; the fields must have been
+   created with their mixed-precision tables), with per-lane coefficient loads
+   (cooperative 0) or the step kernels' wave-cooperative gathers (cooperative 1; the mixed
+   tables are read per lane either way, as the step kernels read them).  The two gathers
+   give bit-identical results; variant FP64 with cooperative 0 is pt_fields_eval.
+   No counterpart in the reference: it exposes the kernels' variants of
+   TFieldManager::BField / EField (src/fields.cpp:139-182) for parity checks. */
+int pt_fields_eval_tables(const pt_fields *f, void *stream, int n, const double *pts, const double *t, int variant,
+                          int cooperative, double *B, double *dB, double *V, double *E);
 /* TGeometry::GetSolid (src/geometry.cpp:151-175) at n points and times: the ID of the
    highest-priority solid not ignored at t (1: default solid; -1: too deeply nested;
    -2: reserved for an incomplete BVH traversal, which the stackless traversal excludes) */

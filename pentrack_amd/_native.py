@@ -142,7 +142,7 @@ EXPORTS = [
     "pt_request_quit", "pt_last_error", "pt_clearance_grid", "pt_tricubic_matrix",
     "pt_spline_grid_diff", "pt_rk4_forward_ckpt", "pt_rk4_adjoint", "pt_formula_eval",
     "pt_batch_secondaries", "pt_proton_beta_table", "pt_neutron_decay_host", "pt_beta_sample",
-    "pt_distribution_table", "pt_fields_eval", "pt_geometry_solid_at", "pt_geometry_segment_hits", "pt_bicubic_precompute",
+    "pt_distribution_table", "pt_fields_eval", "pt_fields_eval_tables", "pt_geometry_solid_at", "pt_geometry_segment_hits", "pt_bicubic_precompute",
     "pt_tricubic_precompute_gpu", "pt_batch_set_snapshots", "pt_batch_snapshots",
     "pt_batch_set_track", "pt_batch_track", "pt_batch_set_spinlog", "pt_batch_spinlog",
     "pt_rk4_adjoint_noalloc", "pt_fields_mixed_info", "pt_gather_bandwidth", "pt_gather_bandwidth_coop", "pt_batch_set_loss",
@@ -221,6 +221,7 @@ def lib():
     L.pt_distribution_table.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp, dp,
                                          C.POINTER(C.c_int)]
     L.pt_fields_eval.argtypes = [vp, vp, C.c_int, dp, dp, dp, dp, dp, dp]
+    L.pt_fields_eval_tables.argtypes = [vp, vp, C.c_int, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp]
     L.pt_geometry_solid_at.argtypes = [vp, vp, C.c_int, dp, dp, C.POINTER(C.c_int32)]
     L.pt_geometry_segment_hits.argtypes = [vp, vp, C.c_int, dp, dp, C.c_int, C.POINTER(C.c_int32)]
     L.pt_neutron_decay_host.argtypes = [C.c_uint64, C.c_uint64, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]

@@ -132,14 +132,19 @@ struct Field {
     double x0[3], inv_h[3];   /* first grid coordinate and 1/spacing guess for the cell search */
     const double *g[3];       /* grid coordinates */
     const double *coef;       /* FP64 table */
-    const unsigned char *mix; /* mixed-precision table: MIX_BLOCK_BYTES per cell and component */
+    const unsigned char *mix; /* mixed-precision table: mix_cell_bytes(ncomp) per cell */
 };
 
-/* A block of the mixed-precision table: the constant and linear coefficients in FP64
-   (device order: slots 0 = 1, 1 = z, 4 = y, 16 = x) and the 64 coefficients with those four
-   slots zeroed in FP32; a component whose FP32 remainder may err by more than the map's
-   tolerance keeps its FP64 block instead, marked by a NaN constant term. */
-constexpr int MIX_BLOCK_BYTES = 4 * 8 + 64 * 4;
+/* A cell record of the mixed-precision table: first every component's constant and linear
+   coefficients in FP64 (32 bytes each, device order: slots 0 = 1, 1 = z, 4 = y, 16 = x),
+   padded to a multiple of 128 bytes, then every component's 64 coefficients in FP32 with
+   those four slots zeroed (256 bytes each).  Records are 128-byte aligned, so the FP64 part
+   of a 3-component cell is one 128-byte line and each FP32 x-power line (64 bytes) lies
+   within one.  A component whose FP32 remainder may err by more than the map's tolerance
+   keeps its FP64 block instead, marked by a NaN constant term. */
+constexpr int MIX_LIN_BYTES = 4 * 8, MIX_REM_BYTES = 64 * 4;
+PTHD size_t mix_lin_bytes(int ncomp) { return ((size_t)ncomp * MIX_LIN_BYTES + 127) & ~(size_t)127; }
+PTHD size_t mix_cell_bytes(int ncomp) { return mix_lin_bytes(ncomp) + (size_t)ncomp * MIX_REM_BYTES; }
 
 struct Species {
     int kind;

@@ -667,7 +667,8 @@ extern "C" int pt_fields_create(int n, const pt_field_desc *descs, int with_fp32
                 unsigned char *dm;
                 int *dcount;
                 const int zero = 0;
-                if ((r = F->mem.alloc(&dm, nblk * ptd::MIX_BLOCK_BYTES)) || (r = F->mem.upload(&dcount, &zero, 1)))
+                if ((r = F->mem.alloc(&dm, nblk / f.ncomp * ptd::mix_cell_bytes(f.ncomp))) ||
+                    (r = F->mem.upload(&dcount, &zero, 1)))
                     return r;
                 PT_HIP(pt_launch_mixed_table(f.g[0], f.g[1], f.g[2], d.nx, d.ny, d.nz, f.ncomp, dc, tol, dm, dcount, 0));
                 int nfp64 = 0;
@@ -1528,10 +1529,9 @@ extern "C" int pt_batch_secondaries(pt_batch *b, void *stream, int cap, pt_secon
 }
 
 /* TFieldManager::BField / EField at n points and times (host arrays), FP64 tables */
-extern "C" int pt_fields_eval(const pt_fields *f, void *stream, int n, const double *pts, const double *t, double *B,
-                              double *dB, double *V, double *E) {
-    if (!f || n < 0 || (n > 0 && (!pts || !t || !B || !V || !E))) return fail(PT_EINVAL, "pt_fields_eval: bad arguments");
-    if (n == 0) return PT_OK;
+namespace {
+int fields_eval(const pt_fields *f, void *stream, int n, const double *pts, const double *t, int mixed, int coop,
+                double *B, double *dB, double *V, double *E) {
     hipStream_t s = static_cast<hipStream_t>(stream);
     DevMem mem;
     double *dp, *dt, *dB_, *ddB, *dV, *dE;
@@ -1547,13 +1547,32 @@ extern "C" int pt_fields_eval(const pt_fields *f, void *stream, int n, const dou
     if ((r = mem.alloc(&dpar, 1))) return r;
     P.self = dpar;
     PT_HIP(hipMemcpy(dpar, &P, sizeof(ptd::Params), hipMemcpyHostToDevice));
-    PT_HIP(pt_launch_fields_eval(P, f->general ? 1 : 0, n, dp, dt, dB_, ddB, dV, dE, s));
+    PT_HIP(pt_launch_fields_eval(P, (mixed ? 1 : 0) | (f->general ? 2 : 0), coop, n, dp, dt, dB_, ddB, dV, dE, s));
     PT_HIP(hipStreamSynchronize(s));
     PT_HIP(hipMemcpy(B, dB_, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     if (dB) PT_HIP(hipMemcpy(dB, ddB, 9 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     PT_HIP(hipMemcpy(V, dV, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     PT_HIP(hipMemcpy(E, dE, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     return PT_OK;
+}
+}  // namespace
+
+extern "C" int pt_fields_eval(const pt_fields *f, void *stream, int n, const double *pts, const double *t, double *B,
+                              double *dB, double *V, double *E) {
+    if (!f || n < 0 || (n > 0 && (!pts || !t || !B || !V || !E))) return fail(PT_EINVAL, "pt_fields_eval: bad arguments");
+    if (n == 0) return PT_OK;
+    return fields_eval(f, stream, n, pts, t, 0, 0, B, dB, V, E);
+}
+
+extern "C" int pt_fields_eval_tables(const pt_fields *f, void *stream, int n, const double *pts, const double *t,
+                                     int variant, int cooperative, double *B, double *dB, double *V, double *E) {
+    if (!f || n < 0 || (n > 0 && (!pts || !t || !B || !V || !E)) ||
+        (variant != PT_VARIANT_FP64 && variant != PT_VARIANT_MIXED))
+        return fail(PT_EINVAL, "pt_fields_eval_tables: bad arguments");
+    if (variant == PT_VARIANT_MIXED && !f->has_fp32)
+        return fail(PT_EINVAL, "pt_fields_eval_tables: the fields were created without mixed-precision tables");
+    if (n == 0) return PT_OK;
+    return fields_eval(f, stream, n, pts, t, variant == PT_VARIANT_MIXED, cooperative ? 1 : 0, B, dB, V, E);
 }
 
 /* TGeometry::GetSolid (src/geometry.cpp:151-175) at n points and times: solid IDs (1 = default);

@@ -26,7 +26,7 @@ template <int MODE> hipError_t pt_launch_step_mode(const ptd::Params &P, int gri
 template <int MODE> hipError_t pt_launch_step_persistent_mode(const ptd::Params &P, int grid, int split, hipStream_t stream);
 template <int MODE> hipError_t pt_launch_collision_mode(const ptd::Params &P, int grid, const int *queue, hipStream_t stream);
 template <int MODE>
-hipError_t pt_launch_fields_eval_mode(const ptd::Params &P, int n, const double *pts, const double *t, double *B,
+hipError_t pt_launch_fields_eval_mode(const ptd::Params &P, int coop, int n, const double *pts, const double *t, double *B,
                                       double *dB, double *V, double *E, hipStream_t stream);
 
 /* particle set-up (first call only) and the step budget of every particle */
@@ -37,9 +37,10 @@ hipError_t pt_launch_step(const ptd::Params &P, int mode, int block, hipStream_t
    in Morton order of the particle positions), then `grid` blocks of 256 slots */
 hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, int spatial, hipStream_t stream);
 hipError_t pt_launch_collision(const ptd::Params &P, int mode, int coll_grid, int order, hipStream_t stream);
-/* field sum (FP64 tables) at n points and times; general: containers with formulas or analytic fields */
-hipError_t pt_launch_fields_eval(const ptd::Params &P, int general, int n, const double *pts, const double *t, double *B,
-                                 double *dB, double *V, double *E, hipStream_t stream);
+/* field sum at n points and times; mode: 1 mixed-precision tables | 2 containers with formulas or
+   analytic fields; coop: the step kernels' cooperative coefficient gathers instead of per-lane loads */
+hipError_t pt_launch_fields_eval(const ptd::Params &P, int mode, int coop, int n, const double *pts, const double *t,
+                                 double *B, double *dB, double *V, double *E, hipStream_t stream);
 /* TabField3::PreInterpol on the device: the seven spline-derivative tables der[1..7] of the
    samples der[0] (each [nx][ny][nz]; scratch: 2 nx ny nz doubles), then the 64 coefficients
    per cell of component comp (der_dev: the 8 table pointers in device memory, A: the 64 x 64
@@ -49,7 +50,7 @@ hipError_t pt_launch_tricubic_derivs(const double *const g[3], int nx, int ny, i
 hipError_t pt_launch_tricubic_coef(const double *xg, const double *yg, const double *zg, int nx, int ny, int nz,
                                    const double *const *der_dev, const double *A, int ncomp, int comp, int device_order,
                                    double *coef, hipStream_t stream);
-/* the mixed-precision table (ptd::MIX_BLOCK_BYTES per cell and component) of a map from its
+/* the mixed-precision table (ptd::mix_cell_bytes(ncomp) per cell) of a map from its
    FP64 table in the tracker's order; *n_fp64 counts the blocks kept in FP64 (tol: bound on
    the FP32 remainder's gradient error, field unit per metre) */
 hipError_t pt_launch_mixed_table(const double *xg, const double *yg, const double *zg, int nx, int ny, int nz,

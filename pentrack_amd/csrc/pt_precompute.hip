@@ -18,6 +18,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "pt_device.h"
+
 namespace ptp {
 
 /* derivatives along `axis` of tab [nx][ny][nz] for every grid line; scr: [2][len][nlines] */
@@ -128,8 +130,9 @@ __global__ void __launch_bounds__(256) tricubic_coef_kernel(const double *__rest
 
 /*
  * The mixed-precision table of a map from its FP64 table in the tracker's order: one thread
- * per cell and component writes a block of 4 doubles (the constant and linear coefficients,
- * slots 0, 1, 4, 16: 1, z, y, x) and 64 floats (all coefficients, those four zeroed).  The
+ * per cell and component writes its 4 doubles (the constant and linear coefficients, slots
+ * 0, 1, 4, 16: 1, z, y, x) and 64 floats (all coefficients, those four zeroed) into the
+ * cell's record (ptd::mix_cell_bytes: the doubles of all components first, then the floats).  The
  * FP32 remainder's error is bounded by 2^-20 sum_{i+j+k>=2} |a_ijk| for the value and by
  * 2^-20 sum |a_ijk| (i/hx + j/hy + k/hz) for the gradient (coefficients rounded to FP32 and
  * an FMA Horner over [0,1]^3, with room for the rounding of the cell coordinates); where the
@@ -160,8 +163,10 @@ __global__ void __launch_bounds__(256) mixed_table_kernel(const double *__restri
     const double u = 1.0 / 1048576.0;   /* 2^-20 */
     const double hmin = fmin(hx, fmin(hy, hz));
     const bool keep64 = !(gb * u <= tol) || !(vb * u <= tol * hmin);
-    double *lin = reinterpret_cast<double *>(mix + (size_t)t * (4 * 8 + 64 * 4));
-    float *rem = reinterpret_cast<float *>(lin + 4);
+    const int comp = (int)(t - cell * ncomp);
+    unsigned char *rec = mix + (size_t)cell * ptd::mix_cell_bytes(ncomp);
+    double *lin = reinterpret_cast<double *>(rec + (size_t)comp * ptd::MIX_LIN_BYTES);
+    float *rem = reinterpret_cast<float *>(rec + ptd::mix_lin_bytes(ncomp) + (size_t)comp * ptd::MIX_REM_BYTES);
     lin[0] = keep64 ? __builtin_nan("") : a[0];
     lin[1] = a[1];
     lin[2] = a[4];

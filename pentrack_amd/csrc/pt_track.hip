@@ -290,30 +290,23 @@ PTD void tric_eval_B_coop(const Field &f, long cell, const double r[3], const do
         }
     }
 }
-#undef COOP_LDS_FENCE
 
 /* FP32 Horner (with FMA) of a block held in registers, for the remainder of the
-   mixed-precision tables */
-PTD void tric_eval_grad32(const float a[64], float x, float y, float z, bool grad,
-                          float &F, float &Fx, float &Fy, float &Fz) {
-    float ry[4], ryd[4], ryz[4];
-#pragma unroll
-    for (int i = 3; i >= 0; --i) {
-        const float *c = a + 16 * i;
-        float rY = 0.f, rYd = 0.f, rYz = 0.f;
-#pragma unroll
-        for (int j = 3; j >= 0; --j) {
-            float a0 = c[4 * j], a1 = c[4 * j + 1], a2 = c[4 * j + 2], a3 = c[4 * j + 3];
-            float rz = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(a3, z, a2), z, a1), z, a0);
-            rY = __builtin_fmaf(rY, y, rz);
-            if (grad) {
-                float rzd = __builtin_fmaf(__builtin_fmaf(3.0f * a3, z, 2.0f * a2), z, a1);
-                rYz = __builtin_fmaf(rYz, y, rzd);
-                if (j > 0) rYd = __builtin_fmaf(rYd, y, (float)j * rz);
-            }
-        }
-        ry[i] = rY; ryd[i] = rYd; ryz[i] = rYz;
+   mixed-precision tables: one row j (a0..a3 the coefficients of x^i y^j z^0..3), the
+   x-Horner over the four lines' sums, and the whole block */
+PTD void tric_row32(float a0, float a1, float a2, float a3, int j, float y, float z, bool grad, float &rY, float &rYd,
+                    float &rYz) {
+    const float rz = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(a3, z, a2), z, a1), z, a0);
+    rY = __builtin_fmaf(rY, y, rz);
+    if (grad) {
+        const float rzd = __builtin_fmaf(__builtin_fmaf(3.0f * a3, z, 2.0f * a2), z, a1);
+        rYz = __builtin_fmaf(rYz, y, rzd);
+        if (j > 0) rYd = __builtin_fmaf(rYd, y, (float)j * rz);
     }
+}
+
+PTD void tric_finish32(const float ry[4], const float ryd[4], const float ryz[4], float x, bool grad, float &F,
+                       float &Fx, float &Fy, float &Fz) {
     F = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(ry[3], x, ry[2]), x, ry[1]), x, ry[0]);
     if (grad) {
         Fx = __builtin_fmaf(__builtin_fmaf(3.0f * ry[3], x, 2.0f * ry[2]), x, ry[1]);
@@ -322,19 +315,34 @@ PTD void tric_eval_grad32(const float a[64], float x, float y, float z, bool gra
     }
 }
 
+PTD void tric_eval_grad32(const float a[64], float x, float y, float z, bool grad,
+                          float &F, float &Fx, float &Fy, float &Fz) {
+    float ry[4], ryd[4], ryz[4];
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+        const float *c = a + 16 * i;
+        float rY = 0.f, rYd = 0.f, rYz = 0.f;
+#pragma unroll
+        for (int j = 3; j >= 0; --j) tric_row32(c[4 * j], c[4 * j + 1], c[4 * j + 2], c[4 * j + 3], j, y, z, grad, rY, rYd, rYz);
+        ry[i] = rY; ryd[i] = rYd; ryz[i] = rYz;
+    }
+    tric_finish32(ry, ryd, ryz, x, grad, F, Fx, Fy, Fz);
+}
+
 /* Mixed precision: value and gradient of component comp of a cell from the mixed table,
    the FP64 linear part (in FP64 arithmetic on the FP64 cell coordinates) plus the FP32
    Horner of the remainder; or the FP64 block where the component kept it. */
 PTD void tric_eval_mixed(const Field &f, size_t cell, int comp, const double r[3], bool grad, double &F, double &Fx,
                          double &Fy, double &Fz) {
     const size_t blk = cell * (size_t)f.ncomp + (size_t)comp;
-    const double *lin = reinterpret_cast<const double *>(f.mix + blk * MIX_BLOCK_BYTES);
+    const unsigned char *rec = f.mix + cell * mix_cell_bytes(f.ncomp);
+    const double *lin = reinterpret_cast<const double *>(rec + (size_t)comp * MIX_LIN_BYTES);
     /* the whole block is loaded before the marker is tested, so the loads of the common
        (mixed) case do not wait for the marker's round trip to memory */
     const double2 l01 = reinterpret_cast<const double2 *>(lin)[0];
     const double2 l23 = reinterpret_cast<const double2 *>(lin)[1];
     float c[64];
-    const float4 *c4 = reinterpret_cast<const float4 *>(lin + 4);
+    const float4 *c4 = reinterpret_cast<const float4 *>(rec + mix_lin_bytes(f.ncomp) + (size_t)comp * MIX_REM_BYTES);
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const float4 v = c4[q];
@@ -353,6 +361,8 @@ PTD void tric_eval_mixed(const Field &f, size_t cell, int comp, const double r[3
         Fz = l01.y + (double)fz;
     }
 }
+
+#undef COOP_LDS_FENCE
 
 PTD bool in_bounds(const Field &f, double x, double y, double z) {
     const double *b = f.box;
@@ -687,6 +697,10 @@ PTD void container_B(const Field &f, double x, double y, double z, double bs, do
             long cell; double r[3], dist[3];
             if (tric_locate(f, x, y, z, cell, r, dist)) {
                 ++nfield;
+                /* the mixed tables keep per-lane loads: gathered cooperatively (the FP32 lines
+                   as units, the components kept in FP64 dealt to the wave's lanes in one
+                   FP64 pass) they ran at 0.79x the per-lane form on a map with 26% FP64
+                   components */
                 if (COOP && !(MODE & MODE_MIXED) && f.ncomp >= 3 && coop_fits(f)) {
                     tric_eval_B_coop(f, cell, r, dist, grad, Bt, dBt);
                 } else {
@@ -2662,7 +2676,7 @@ __global__ void __launch_bounds__(256) decay_kernel(const Params *__restrict__ P
 /* ---- field and solid queries (TFieldManager::BField/EField, TGeometry::GetSolid) ---- */
 
 /* field sum at n points and times, FP64 tables; dB row-major [n][3][3] */
-template <int MODE>
+template <int MODE, bool COOP>
 __global__ void __launch_bounds__(256) fields_eval_kernel(const Params *__restrict__ Pp, int n, const double *pts, const double *t, double *B,
                                                           double *dB, double *V, double *E) {
     const Params &P = *Pp;
@@ -2672,7 +2686,7 @@ __global__ void __launch_bounds__(256) fields_eval_kernel(const Params *__restri
     double b[3], db[3][3], v, e[3];
     const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
     const bool grad = dB != nullptr;       /* without dB: BField(x, y, z, t, B), value-only */
-    field_B<MODE>(P, x, y, z, t[i], b, db, grad, nf);
+    field_B<MODE, COOP>(P, x, y, z, t[i], b, db, grad, nf);
     field_E<MODE>(P, x, y, z, t[i], &v, e, nf);
     for (int k = 0; k < 3; ++k) {
         B[3 * i + k] = b[k];
@@ -2931,10 +2945,14 @@ hipError_t pt_launch_collision_mode(const ptd::Params &P, int grid, const int *q
     return hipGetLastError();
 }
 template <int MODE>
-hipError_t pt_launch_fields_eval_mode(const ptd::Params &P, int n, const double *pts, const double *t, double *B,
+hipError_t pt_launch_fields_eval_mode(const ptd::Params &P, int coop, int n, const double *pts, const double *t, double *B,
                                       double *dB, double *V, double *E, hipStream_t stream) {
-    hipLaunchKernelGGL(ptd::fields_eval_kernel<MODE>, dim3((n + 255) / 256), dim3(256), 0, stream, P.self, n, pts, t, B, dB,
-                       V, E);
+    if (coop)
+        hipLaunchKernelGGL((ptd::fields_eval_kernel<MODE, true>), dim3((n + 255) / 256), dim3(256), 0, stream, P.self, n, pts,
+                           t, B, dB, V, E);
+    else
+        hipLaunchKernelGGL((ptd::fields_eval_kernel<MODE, false>), dim3((n + 255) / 256), dim3(256), 0, stream, P.self, n,
+                           pts, t, B, dB, V, E);
     return hipGetLastError();
 }
 /* PT_KERNEL_SET 0: the set-up and step kernels of the mode, 2: its persistent step kernel,
@@ -2946,10 +2964,8 @@ template hipError_t pt_launch_step_mode<PT_MODE_TU>(const ptd::Params &, int, in
 template hipError_t pt_launch_step_persistent_mode<PT_MODE_TU>(const ptd::Params &, int, int, hipStream_t);
 #else
 template hipError_t pt_launch_collision_mode<PT_MODE_TU>(const ptd::Params &, int, const int *, hipStream_t);
-#if (PT_MODE_TU & 1) == 0 /* fields_eval runs on FP64 tables only */
-template hipError_t pt_launch_fields_eval_mode<PT_MODE_TU>(const ptd::Params &, int, const double *, const double *,
+template hipError_t pt_launch_fields_eval_mode<PT_MODE_TU>(const ptd::Params &, int, int, const double *, const double *,
                                                            double *, double *, double *, double *, hipStream_t);
-#endif
 #endif
 #endif /* PT_MODE_TU >= 0 */
 
@@ -3009,11 +3025,15 @@ hipError_t pt_launch_step_persistent(const ptd::Params &P, int mode, int grid, i
     }
 }
 
-hipError_t pt_launch_fields_eval(const ptd::Params &P, int general, int n, const double *pts, const double *t, double *B,
-                                 double *dB, double *V, double *E, hipStream_t stream) {
+hipError_t pt_launch_fields_eval(const ptd::Params &P, int mode, int coop, int n, const double *pts, const double *t,
+                                 double *B, double *dB, double *V, double *E, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    if (general) return pt_launch_fields_eval_mode<2>(P, n, pts, t, B, dB, V, E, stream);
-    return pt_launch_fields_eval_mode<0>(P, n, pts, t, B, dB, V, E, stream);
+    switch (mode) {
+    case 0: return pt_launch_fields_eval_mode<0>(P, coop, n, pts, t, B, dB, V, E, stream);
+    case 1: return pt_launch_fields_eval_mode<1>(P, coop, n, pts, t, B, dB, V, E, stream);
+    case 2: return pt_launch_fields_eval_mode<2>(P, coop, n, pts, t, B, dB, V, E, stream);
+    default: return pt_launch_fields_eval_mode<3>(P, coop, n, pts, t, B, dB, V, E, stream);
+    }
 }
 
 hipError_t pt_launch_solid_at(const ptd::Geom &G, int n, const double *pts, const double *t, int *out, hipStream_t stream) {

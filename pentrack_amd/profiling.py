@@ -83,9 +83,9 @@ def run(kernel="step_kernel_fp64"):
     b = Batch(w.geometry, w.fields, w.species, w.tmax, w.species_idx, w.t0, w.y0, w.spin0, w.seed, w.pid,
               with_mixed=bool(variant))
     blocks, fp64 = w.fields.mixed_info()
-    # bytes one field evaluation gathers: 3 blocks of 512 B (FP64), or of 288 B plus 512 B
-    # for the components kept in FP64 (mixed)
-    bpe = 3 * 512 if not variant else 3 * (288 + 512 * fp64 / max(blocks, 1))
+    # bytes one field evaluation gathers: 3 blocks of 512 B (FP64), or the 896 B cell record
+    # plus 512 B for each component kept in FP64 (mixed)
+    bpe = 3 * 512 if not variant else 896 + 3 * 512 * fp64 / max(blocks, 1)
     stream = torch.cuda.current_stream().cuda_stream
     cfg = LaunchConfig(variant=variant, persistent=persistent, steps_per_launch=RK_STEPS, rounds=ROUNDS,
                        spatial_order=spatial)

@@ -7,17 +7,25 @@ import numpy as np
 from . import _native
 
 
-def fields_eval(fields, pts, t, stream=None, grad=True):
+def fields_eval(fields, pts, t, stream=None, grad=True, variant=None, cooperative=False):
     """TFieldManager::BField / EField at pts [n][3], times t (scalar or [n]): B [n][3],
     dB [n][3][3], V [n], E [n][3].  grad False: the value-only BField(x, y, z, t, B) the
-    reference uses for energies and logs, dB returned as None."""
+    reference uses for energies and logs, dB returned as None.  variant (PT_VARIANT_FP64 or
+    PT_VARIANT_MIXED) and cooperative select the tables and the coefficient gathers the
+    tracking kernels use (pt_fields_eval_tables)."""
     pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
     n = len(pts)
     t = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (n,)))
     B, dB, V, E = np.zeros((n, 3)), np.zeros((n, 3, 3)), np.zeros(n), np.zeros((n, 3))
     d = _native.dptr
-    _native.check(_native.lib().pt_fields_eval(fields.device(), stream, n, d(pts), d(t), d(B), d(dB) if grad else None,
-                                               d(V), d(E)), "pt_fields_eval")
+    if variant is None and not cooperative:
+        _native.check(_native.lib().pt_fields_eval(fields.device(), stream, n, d(pts), d(t), d(B),
+                                                   d(dB) if grad else None, d(V), d(E)), "pt_fields_eval")
+    else:
+        v = _native.PT_VARIANT_FP64 if variant is None else variant
+        dev = fields.device(with_mixed=True) if v == _native.PT_VARIANT_MIXED else fields.device()
+        _native.check(_native.lib().pt_fields_eval_tables(dev, stream, n, d(pts), d(t), v, 1 if cooperative else 0, d(B),
+                                                          d(dB) if grad else None, d(V), d(E)), "pt_fields_eval_tables")
     return B, dB if grad else None, V, E
 
 
