@@ -223,10 +223,11 @@ def mixed_error_leg(a, sptr):
         ws = workloads.cube(n=a.err_speed_n, seed=31, tmax=50.0, with_field=False)
         from pentrack_amd.fields import TFieldManager
         ws.fields = TFieldManager(w.fields.containers)
-        for name, vid in (("fp64_persistent", _native.PT_VARIANT_FP64), ("mixed_persistent", _native.PT_VARIANT_MIXED)):
+        for name, vid, pers in (("fp64", _native.PT_VARIANT_FP64, 0), ("fp64_persistent", _native.PT_VARIANT_FP64, 1),
+                                ("mixed_persistent", _native.PT_VARIANT_MIXED, 1)):
             b = Batch(ws.geometry, ws.fields, ws.species, ws.tmax, ws.species_idx, ws.t0, ws.y0, ws.spin0, ws.seed,
                       ws.pid, with_mixed=(vid == _native.PT_VARIANT_MIXED))
-            cfg = LaunchConfig(variant=vid, steps_per_launch=a.rk, persistent=1)
+            cfg = LaunchConfig(variant=vid, steps_per_launch=a.rk, persistent=pers)
             for _ in range(2):
                 b.run(cfg, sptr)
             torch.cuda.synchronize()
@@ -240,6 +241,9 @@ def mixed_error_leg(a, sptr):
             a1, _ = b.step_counts(sptr)
             speed[name] = (a1 - a0) / (e0.elapsed_time(e1) / 1e3)
             del b
+        # the metric's comparison (mixed + persistent against the plain FP64 kernel), and the
+        # same persistent kernel in both arithmetics
+        speed["mixed_persistent_over_plain_fp64"] = speed["mixed_persistent"] / speed["fp64"]
         speed["mixed_over_fp64"] = speed["mixed_persistent"] / speed["fp64_persistent"]
         speed["particles"] = ws.n
     res = {"workload": f"cube 1 m, {w.n} neutrons, 128^3 coil map (|B| <= 1 T), mixed tables with "
@@ -486,6 +490,8 @@ def main():
                          "counts_for_value": (not v["mixed"]) or mixed_ok}
                      for k, v in results.items()},
         "mixed_over_best_fp64": {k: v / max(fp64_rates) for k, v in mixed_rates.items()} if fp64_rates else None,
+        "mixed_persistent_over_plain_fp64": (results["mixed_persistent"]["rate"] / results["fp64"]["rate"]
+                                             if "mixed_persistent" in results and "fp64" in results else None),
         "mixed_tables_fp64_block_fraction": f64frac,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
