@@ -50,3 +50,32 @@ def test_mixed_tables(gpu, cube128):
     V0 = fields_eval(f, pts, t, grad=False, variant=_native.PT_VARIANT_MIXED, cooperative=False)[0]
     V1 = fields_eval(f, pts, t, grad=False, variant=_native.PT_VARIANT_MIXED, cooperative=True)[0]
     assert np.array_equal(V0, V1)
+
+
+@pytest.mark.parametrize("with_b", [True, False])
+def test_mixed_tables_with_potential(gpu, with_b):
+    """Cell records of 4 components (B and V: 128 bytes of FP64 terms, four FP32 blocks) and
+    of 1 (V alone: the FP64 terms padded to 128 bytes): the mixed tables' B, V and E agree
+    with the FP64 tables' within the remainder's bound, and differ from them (in use)."""
+    from pentrack_amd.fields import TFieldManager, tabfield3
+    rng = np.random.default_rng(5)
+    ax = [np.linspace(-0.1, 1.1, 25) for _ in range(3)]
+    X, Y, Z = np.meshgrid(*ax, indexing="ij")
+    # amplitudes small enough that the FP32 remainders stay within the default tolerance
+    V = 1e-6 * np.sin(2.0 * X) * np.cos(1.5 * Y) * (1.0 + 0.3 * Z * Z)
+    B = np.stack([2e-4 * np.cos(X + Y), 1e-4 * np.sin(Y * Z), 3e-4 * np.cos(Z - X)]) if with_b else None
+    fm = TFieldManager([tabfield3(*ax, B=B, V=V, escale=1.0)])
+    print(f"case: mixed tables with a potential, B {with_b}", flush=True)
+    pts = rng.uniform(-0.05, 1.05, (20011, 3))
+    t = np.zeros(len(pts))
+    Bm, dBm, Vm, Em = fields_eval(fm, pts, t, variant=_native.PT_VARIANT_MIXED)
+    Bd, dBd, Vd, Ed = fields_eval(fm, pts, t, variant=_native.PT_VARIANT_FP64)
+    blocks, fp64 = fm.mixed_info()
+    assert blocks == 24 ** 3 * (4 if with_b else 1) and fp64 < 0.5 * blocks
+    assert np.abs(Vm - Vd).max() <= 1e-6 * np.abs(Vd).max() and not np.array_equal(Vm, Vd)
+    assert np.abs(Em - Ed).max() <= 1e-5 * np.abs(Ed).max()
+    if with_b:
+        assert np.abs(Bm - Bd).max() <= 1e-6 * np.abs(Bd).max() and not np.array_equal(Bm, Bd)
+        assert np.abs(dBm - dBd).max() <= 1e-5 * np.abs(dBd).max()
+    else:
+        assert not Bm.any() and not Bd.any()
