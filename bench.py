@@ -457,7 +457,8 @@ def main():
     r, rp = results[head], results[prof]
     step_ms, step_launches = rp["kernels"]["step"]
     per_launch_ms = step_ms / max(step_launches, 1)
-    bpe = MIX_CELL + 3 * f64frac * FP64_BLOCK if rp["mixed"] else 3 * FP64_BLOCK
+    # tables that keep every block in FP64 run the FP64 kernels
+    bpe = MIX_CELL + 3 * f64frac * FP64_BLOCK if rp["mixed"] and f64frac < 1 else 3 * FP64_BLOCK
     bytes_per_launch = rp["field_evals"] * bpe / max(step_launches, 1)
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
     kernel_name = KERNEL_NAMES.get(prof, prof)

@@ -1407,7 +1407,11 @@ extern "C" int pt_batch_run(pt_batch *b, void *stream, const pt_launch_cfg *cfg)
             return fail(PT_EDEVICE, "pt_batch_run: the step kernels need " + std::to_string(PT_STEP_LDS_BYTES) +
                                         " bytes of LDS per block, the device allows " + std::to_string(lds));
     }
-    const int mode = (cfg->variant == PT_VARIANT_MIXED ? 1 : 0) | (b->fields->general ? 2 : 0);
+    /* mixed-precision tables that keep every block in FP64 give exactly the FP64 kernels'
+       arithmetic (a kept block is evaluated as the FP64 kernels evaluate it): such a batch
+       runs the FP64 kernels, with their cooperative gathers */
+    const bool mixed = cfg->variant == PT_VARIANT_MIXED && b->fields->mix_fp64 < b->fields->mix_blocks;
+    const int mode = (mixed ? 1 : 0) | (b->fields->general ? 2 : 0);
     hipStream_t s = static_cast<hipStream_t>(stream);
     int q = g_quit.load();
     PT_HIP(hipMemcpyAsync(b->d_quit, &q, sizeof(int), hipMemcpyHostToDevice, s));

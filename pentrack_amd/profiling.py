@@ -85,7 +85,7 @@ def run(kernel="step_kernel_fp64"):
     blocks, fp64 = w.fields.mixed_info()
     # bytes one field evaluation gathers: 3 blocks of 512 B (FP64), or the 896 B cell record
     # plus 512 B for each component kept in FP64 (mixed)
-    bpe = 3 * 512 if not variant else 896 + 3 * 512 * fp64 / max(blocks, 1)
+    bpe = 3 * 512 if not variant or fp64 == blocks else 896 + 3 * 512 * fp64 / max(blocks, 1)
     stream = torch.cuda.current_stream().cuda_stream
     cfg = LaunchConfig(variant=variant, persistent=persistent, steps_per_launch=RK_STEPS, rounds=ROUNDS,
                        spatial_order=spatial)

@@ -204,6 +204,21 @@ def test_mixed_precision_config3(gpu, oracle_lib):
     assert not np.array_equal(runs["mixed"][0]["y"], o["y"])      # the mixed arithmetic is in use
 
 
+def test_mixed_variant_on_tables_kept_in_fp64(gpu, oracle_lib):
+    """A map whose mixed-precision tables keep every block in FP64 (the cylinder's coil map at
+    1 T): the mixed variant then computes the FP64 arithmetic and is bit-exact with the FP64
+    oracle, in the per-particle and the persistent kernel."""
+    w = workloads.cylinder(n=1500, seed=8, tmax=2.0, n_grid=24)
+    print("case: mixed variant, cylinder n=1500, every block in FP64", flush=True)
+    o, oh = oracle_lib.track(w, "fp64", hit_cap=HITS, threads=8)
+    for persistent in (0, 1):
+        g, gh = run_gpu(w, cfg=LaunchConfig(variant=_native.PT_VARIANT_MIXED, persistent=persistent, steps_per_launch=9,
+                                            grid=64), with_mixed=True)
+        blocks, fp64 = w.fields.mixed_info()
+        assert blocks > 0 and fp64 == blocks
+        assert_bit_exact(g, gh, o, oh)
+
+
 def test_persistent_work_queue_bit_exact(gpu, oracle_lib):
     """The persistent kernel (slots refilled from the device work list) gives the same
     particles the same arithmetic: bit-exact with the oracle, in sliced calls."""
