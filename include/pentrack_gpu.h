@@ -219,7 +219,9 @@ typedef struct {
 
 #define PT_VARIANT_FP64 0        /* plain FP64, one thread per particle */
 #define PT_VARIANT_MIXED 1       /* mixed-precision field tables (FP64 linear part, FP32 remainder
-                                    within mixed_tol), FP64 state, control, bisection, triangle test */
+                                    within mixed_tol), FP64 state, control, bisection, triangle test;
+                                    tables that keep every block in FP64 run the FP64 kernels
+                                    (the same arithmetic) */
 
 typedef struct {
     int variant;
